@@ -1,0 +1,101 @@
+/* parcomm_amd: MI355X-native QAFold prover (F65521 / Ext8 backend), C ABI.
+ *
+ * The entry points replace the Rust library API of the reference crate
+ * `parcomm-qafold` (qafold/f65521/src/protocol.rs), which has no FFI of its
+ * own; each declaration cites the item it stands in for. Buffers are
+ * caller-owned host memory unless the name says `device`; device memory is
+ * owned by the handles. Functions return 0 on success or one of the QF_ERR_*
+ * codes (the reference engine's error kinds, vendor/qafold/src/lib.rs:20-35,
+ * plus device and argument errors); qf_last_error() gives the message of the
+ * last failure on the calling thread. No exception crosses this boundary.
+ * Handles are independent: distinct handles may be used from distinct threads.
+ */
+#ifndef PARCOMM_AMD_H
+#define PARCOMM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QF_OK 0
+#define QF_ERR_ENCODING 1
+#define QF_ERR_HASHING 2
+#define QF_ERR_INVALID_COMMITMENT 3
+#define QF_ERR_INVALID_VARIABLES 4
+#define QF_ERR_INVALID_PARAMETERS 5
+#define QF_ERR_TRANSCRIPT 6
+#define QF_ERR_DEVICE 7
+#define QF_ERR_ARGUMENT 8
+
+#define QF_FIELD_F65521 0
+#define QF_CACHE_CACHED 0
+#define QF_CACHE_RECOMPUTE 1
+
+typedef struct qf_params qf_params;
+typedef struct qf_committed qf_committed;
+
+/* Message of the last failure on this thread (empty after success). */
+const char* qf_last_error(void);
+
+/* Parameters::new(shapes, Config{chunk_log, packed_log, query_rf_bits})
+ * (protocol.rs:109-130): layout, QA sequence, setup commitments. `shapes` holds
+ * n (rows, cols) pairs. */
+int qf_params_new(int field, const uint64_t* shapes, size_t n, uint32_t chunk_log, uint32_t packed_log,
+                  uint32_t query_rf_bits, qf_params** out);
+void qf_params_free(qf_params* p);
+
+/* PackedPcs::parameter_digest (packed.rs:159-161). */
+int qf_params_digest(const qf_params* p, uint8_t out[32]);
+
+/* Layout fields (layout.rs:25-31): active and logical BVA length, effective
+ * block length and number of PCS blocks (protocol.rs:116-122). */
+int qf_params_info(const qf_params* p, uint64_t* active, uint64_t* logical, uint64_t* block_len, uint64_t* blocks,
+                   uint64_t* n_buckets);
+
+/* Bucket i as 8 u64: matrix, row_offset, col_offset, rows, cols,
+ * physical_rows, physical_cols, offset (layout.rs:9-19). */
+int qf_params_bucket(const qf_params* p, uint64_t i, uint64_t out[8]);
+
+/* Parameters::commit / commit_recomputed (protocol.rs:131-185) over a BVA of
+ * `len` == active signed bytes in host memory. Non-physical padding must be
+ * zero (layout.rs:136-155). cache_mode QF_CACHE_CACHED keeps every codeword
+ * and tree resident in HBM; QF_CACHE_RECOMPUTE keeps roots only. */
+int qf_commit(const qf_params* p, const int8_t* values, size_t len, int cache_mode, qf_committed** out);
+
+/* As qf_commit, with the BVA already resident in device memory (borrowed: it
+ * must stay valid while the returned handle lives). */
+int qf_commit_device(const qf_params* p, const int8_t* device_values, size_t len, int cache_mode,
+                     qf_committed** out);
+void qf_committed_free(qf_committed* c);
+
+/* Commitment.roots (protocol.rs:33-40): blocks x 32 bytes. */
+int qf_committed_roots(const qf_committed* c, uint8_t* out, size_t cap_blocks);
+
+/* Copy block `block`'s column-major codeword (32 x R Ext8 elements, 16 bytes
+ * each) to host memory; only for cached commitments. `cap` is in bytes. */
+int qf_committed_codeword(const qf_committed* c, uint64_t block, void* out, size_t cap);
+
+/* Device time in milliseconds of the kernels of the last qf_commit*, and the
+ * mean duration of its leaf-hash kernel (HIP events on the commit stream). */
+int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms, double* encode_ms);
+
+/* Device memory for a resident BVA (no reference counterpart: the reference
+ * keeps the BVA in host RAM). qf_device_alloc returns zeroed memory. */
+int qf_device_alloc(size_t bytes, void** device_out);
+int qf_device_upload(void* device_dst, const void* host_src, size_t bytes);
+int qf_device_download(void* host_dst, const void* device_src, size_t bytes);
+int qf_device_free(void* device_ptr);
+
+/* Synthetic workload generator (no reference counterpart; the reference CLI
+ * streams real checkpoints): fills the active BVA in device memory with
+ * clip(round(N(0, sigma)), -128, 127) on every physical entry and zero on
+ * boundary padding, from a counter-based generator keyed by `seed`. */
+int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* device_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+// C entry points of the CPU oracle, loaded by tests/ and bench.py through
+// ctypes (test infrastructure only; see qf_core.hpp).
+#include "qf_commit.hpp"
+#include <memory>
+
+using namespace orc;
+
+namespace {
+thread_local std::string g_err;
+template <class F>
+int guarded(F&& f) {
+  try { f(); return 0; } catch (const std::exception& e) { g_err = e.what(); return 1; }
+}
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void orc_blake2s(const uint8_t* data, size_t len, uint8_t* out) {
+  Digest d = blake2s(data, len);
+  std::memcpy(out, d.data(), 32);
+}
+
+void orc_chacha_stream(const uint8_t* key, int rounds, uint8_t* out, size_t n) {
+  ChaChaRng r(key, rounds);
+  r.fill_bytes(out, n);
+}
+
+void orc_chacha_block(const uint8_t* key, uint64_t counter, uint64_t stream, int rounds, uint32_t* out16) {
+  chacha_block(key, counter, stream, rounds, out16);
+}
+
+// Ext8 arithmetic for property tests: op 0 add, 1 sub, 2 mul, 3 inv(a), 4 frob(a, k=b.c[0]).
+int orc_e8_op(int op, const uint16_t* a, const uint16_t* b, uint16_t* out) {
+  return guarded([&] {
+    E8 x, y;
+    std::memcpy(x.c, a, 16);
+    std::memcpy(y.c, b, 16);
+    E8 r;
+    switch (op) {
+      case 0: r = x + y; break;
+      case 1: r = x - y; break;
+      case 2: r = x * y; break;
+      case 3: r = e8_inv(x); break;
+      case 4: r = e8_frob(x, y.c[0]); break;
+      default: throw Error("bad op");
+    }
+    std::memcpy(out, r.c, 16);
+  });
+}
+
+void orc_wht(uint16_t* v, size_t n) { wht(reinterpret_cast<E8*>(v), n); }
+void orc_mobius(uint16_t* v, size_t n) { mobius(reinterpret_cast<E8*>(v), n); }
+
+// Layout: shapes[2*i] = rows, shapes[2*i+1] = cols. Writes up to `cap` buckets of
+// 8 u64 (matrix,row_offset,col_offset,rows,cols,physical_rows,physical_cols,offset).
+int orc_layout(const uint64_t* shapes, size_t n, uint64_t* buckets, size_t cap, size_t* n_buckets,
+               uint64_t* active, uint64_t* logical) {
+  return guarded([&] {
+    std::vector<Shape> s(n);
+    for (size_t i = 0; i < n; i++) s[i] = {shapes[2 * i], shapes[2 * i + 1]};
+    Layout l = layout_new(s);
+    *n_buckets = l.buckets.size();
+    *active = l.active;
+    *logical = l.logical;
+    for (size_t i = 0; i < l.buckets.size() && i < cap; i++) {
+      const Bucket& b = l.buckets[i];
+      uint64_t v[8] = {b.matrix, b.row_offset, b.col_offset, b.rows, b.cols, b.physical_rows, b.physical_cols, b.offset};
+      std::memcpy(buckets + 8 * i, v, sizeof v);
+    }
+  });
+}
+
+// PCS parameters of PackedPcs::new(chunk_log, 32, bits, seed) (packed.rs:134-158).
+void* orc_pcs_new(int chunk_log, int security_bits, const uint8_t* seed32) {
+  try {
+    return new PcsParams(pcs_setup(size_t(1) << (chunk_log - 3), 32, 2, size_t(security_bits), seed32));
+  } catch (const std::exception& e) { g_err = e.what(); return nullptr; }
+}
+void orc_pcs_free(void* h) { delete static_cast<PcsParams*>(h); }
+void orc_pcs_parameter_digest(void* h, uint8_t* out) { std::memcpy(out, static_cast<PcsParams*>(h)->parameter_digest.data(), 32); }
+size_t orc_pcs_levels(void* h, size_t* lens) {
+  auto* pp = static_cast<PcsParams*>(h);
+  for (size_t i = 0; i < pp->seq.codes.size(); i++) lens[i] = pp->seq.codes[i].message_len;
+  return pp->seq.codes.size();
+}
+// alpha_1 of the top code (length message_len), 8 u16 per element.
+void orc_pcs_top_alpha(void* h, uint16_t* out) {
+  auto* pp = static_cast<PcsParams*>(h);
+  const auto& a = pp->seq.codes.back().alphas[1];
+  std::memcpy(out, a.data(), a.size() * 16);
+}
+
+// Commit one block of 2^chunk_log signed bytes (entries past `available` are 0).
+// Optionally returns the Mobius-transformed witness and the codeword.
+int orc_commit_block(void* h, const int8_t* values, uint64_t available, uint8_t* root,
+                     uint16_t* coeffs_out, uint16_t* codeword_out) {
+  return guarded([&] {
+    auto* pp = static_cast<PcsParams*>(h);
+    u64 block_len = u64(pp->n_witness) * 8;
+    Committed c = packed_commit(*pp, pack_block_i8(values, available, block_len));
+    Digest r = c.root();
+    std::memcpy(root, r.data(), 32);
+    if (coeffs_out) std::memcpy(coeffs_out, c.coeffs.data(), c.coeffs.size() * 16);
+    if (codeword_out) std::memcpy(codeword_out, c.encoded.data(), c.encoded.size() * 16);
+  });
+}
+
+// Merkle root over column-major Ext8 entries with the base-row leaf domain.
+void orc_merkle_root_rows(const uint16_t* entries, size_t rows, size_t cols, uint8_t* root) {
+  Merkle m = merkle_from_leaves(hash_rows(reinterpret_cast<const E8*>(entries), rows, cols, LEAF_BASE_DOMAIN));
+  Digest r = m.root();
+  std::memcpy(root, r.data(), 32);
+}
+
+}  // extern "C"

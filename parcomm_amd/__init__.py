@@ -1,0 +1,242 @@
+"""MI355X-native QAFold prover (F65521 / Ext8 backend).
+
+Python mirror of the reference crate's `Parameters` API
+(qafold/f65521/src/protocol.rs:18-263): `Config`, `Shape`, `Parameters.new`,
+`Parameters.commit` / `commit_recomputed` and the `Commitment` statement. All
+work runs in the HIP library `libparcomm_amd.so` built from `csrc/`; loading
+fails loudly when that library is missing.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import List, Sequence
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libparcomm_amd.so")
+
+QF_CACHE_CACHED = 0
+QF_CACHE_RECOMPUTE = 1
+QF_FIELD_F65521 = 0
+
+_ERRORS = {
+    1: "EncodingError",
+    2: "HashingError",
+    3: "InvalidCommitment",
+    4: "InvalidNumberOfVariables",
+    5: "InvalidParameters",
+    6: "TranscriptError",
+    7: "DeviceError",
+    8: "ArgumentError",
+}
+
+
+class QfError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{_ERRORS.get(status, status)}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m parcomm_amd.build`"
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, sz, i32, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32
+        P = ctypes.POINTER
+        L.qf_last_error.restype = ctypes.c_char_p
+        L.qf_params_new.argtypes = [i32, P(u64), sz, u32, u32, u32, P(vp)]
+        L.qf_params_free.argtypes = [vp]
+        L.qf_params_free.restype = None
+        L.qf_params_digest.argtypes = [vp, ctypes.c_char_p]
+        L.qf_params_info.argtypes = [vp, P(u64), P(u64), P(u64), P(u64), P(u64)]
+        L.qf_params_bucket.argtypes = [vp, u64, P(u64)]
+        L.qf_commit.argtypes = [vp, vp, sz, i32, P(vp)]
+        L.qf_commit_device.argtypes = [vp, vp, sz, i32, P(vp)]
+        L.qf_committed_free.argtypes = [vp]
+        L.qf_committed_free.restype = None
+        L.qf_committed_roots.argtypes = [vp, ctypes.c_char_p, sz]
+        L.qf_committed_codeword.argtypes = [vp, u64, vp, sz]
+        L.qf_committed_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)]
+        L.qf_device_alloc.argtypes = [sz, P(vp)]
+        L.qf_device_upload.argtypes = [vp, vp, sz]
+        L.qf_device_download.argtypes = [vp, vp, sz]
+        L.qf_device_free.argtypes = [vp]
+        L.qf_synthetic_bva.argtypes = [vp, u64, ctypes.c_float, vp]
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise QfError(status, lib().qf_last_error().decode())
+
+
+class DeviceBuffer:
+    """Device memory owned by the library (resident BVA)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().qf_device_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, array):
+        import numpy as np
+        arr = np.ascontiguousarray(array)
+        buf = cls(arr.nbytes)
+        _check(lib().qf_device_upload(buf.ptr, arr.ctypes.data, arr.nbytes))
+        return buf
+
+    def to_host(self, dtype="int8"):
+        import numpy as np
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(lib().qf_device_download(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().qf_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass(frozen=True)
+class Shape:
+    rows: int
+    cols: int
+
+
+@dataclass
+class Config:
+    """protocol.rs:18-32"""
+    chunk_log: int = 30
+    packed_log: int = 4
+    query_rf_bits: int = 100
+
+
+@dataclass
+class Commitment:
+    """Public statement (protocol.rs:33-40)."""
+    version: int
+    config: Config
+    shapes: List[Shape]
+    parameter_digest: bytes
+    roots: List[bytes] = field(default_factory=list)
+
+
+class Committed:
+    """Device-resident commitment state (protocol.rs:46-50)."""
+
+    def __init__(self, params, handle, statement):
+        self._params = params
+        self._h = ctypes.c_void_p(handle)
+        self.statement = statement
+
+    def codeword(self, block):
+        import numpy as np
+        n_rows = 2 * (self._params.block_len // 256)
+        out = np.empty((32, n_rows, 8), dtype=np.uint16)
+        _check(lib().qf_committed_codeword(self._h, block, out.ctypes.data, out.nbytes))
+        return out
+
+    def timing(self):
+        t, l, e = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().qf_committed_timing(self._h, ctypes.byref(t), ctypes.byref(l), ctypes.byref(e)))
+        return {"total_ms": t.value, "leaf_ms": l.value, "encode_ms": e.value}
+
+    def close(self):
+        if self._h:
+            lib().qf_committed_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Parameters:
+    """protocol.rs:41-45, 109-130"""
+
+    def __init__(self, shapes: Sequence[Shape], config: Config = None):
+        config = config or Config()
+        flat = (ctypes.c_uint64 * (2 * len(shapes)))(*[v for s in shapes for v in (s.rows, s.cols)])
+        h = ctypes.c_void_p()
+        _check(lib().qf_params_new(QF_FIELD_F65521, flat, len(shapes), config.chunk_log, config.packed_log,
+                                   config.query_rf_bits, ctypes.byref(h)))
+        self._h = h
+        self.config = config
+        self.shapes = list(shapes)
+        a, l, b, n, nb = (ctypes.c_uint64() for _ in range(5))
+        _check(lib().qf_params_info(h, ctypes.byref(a), ctypes.byref(l), ctypes.byref(b), ctypes.byref(n), ctypes.byref(nb)))
+        self.active, self.logical, self.block_len, self.blocks, self.n_buckets = a.value, l.value, b.value, n.value, nb.value
+
+    new = classmethod(lambda cls, shapes, config=None: cls(shapes, config))
+
+    def parameter_digest(self) -> bytes:
+        out = ctypes.create_string_buffer(32)
+        _check(lib().qf_params_digest(self._h, out))
+        return out.raw
+
+    def buckets(self):
+        out = []
+        row = (ctypes.c_uint64 * 8)()
+        for i in range(self.n_buckets):
+            _check(lib().qf_params_bucket(self._h, i, row))
+            out.append(tuple(row))
+        return out
+
+    def _statement(self, handle):
+        buf = ctypes.create_string_buffer(32 * self.blocks)
+        _check(lib().qf_committed_roots(handle, buf, self.blocks))
+        roots = [buf.raw[32 * i:32 * i + 32] for i in range(self.blocks)]
+        return Commitment(2, self.config, self.shapes, self.parameter_digest(), roots)
+
+    def _commit(self, values, mode):
+        h = ctypes.c_void_p()
+        if isinstance(values, DeviceBuffer):  # BVA resident in HBM
+            _check(lib().qf_commit_device(self._h, values.ptr, values.nbytes, mode, ctypes.byref(h)))
+        else:
+            import numpy as np
+            arr = np.ascontiguousarray(values, dtype=np.int8)
+            _check(lib().qf_commit(self._h, arr.ctypes.data, arr.size, mode, ctypes.byref(h)))
+        return Committed(self, h.value, self._statement(h))
+
+    def commit(self, values):
+        """Parameters::commit (protocol.rs:131-134): codewords stay resident."""
+        return self._commit(values, QF_CACHE_CACHED)
+
+    def commit_recomputed(self, values):
+        """Parameters::commit_recomputed (protocol.rs:161-185): roots only."""
+        return self._commit(values, QF_CACHE_RECOMPUTE)
+
+    def synthetic_bva(self, seed, sigma=20.0):
+        """Device-resident synthetic BVA: clip(round(N(0, sigma))) on physical entries."""
+        buf = DeviceBuffer(self.active)
+        _check(lib().qf_synthetic_bva(self._h, seed, sigma, buf.ptr))
+        return buf
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qf_params_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

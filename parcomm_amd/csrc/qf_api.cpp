@@ -1,0 +1,281 @@
+// C ABI of the product library (declared in include/parcomm_amd.h).
+#include "../../include/parcomm_amd.h"
+#include "qf_kernels.hpp"
+#include <memory>
+
+using namespace qf;
+
+struct qf_params {
+  Layout layout;
+  uint32_t cfg_chunk_log = 30, packed_log = 4, query_bits = 100;
+  int chunk_log = 0;
+  uint64_t block_len = 0, blocks = 0;
+  PcsParams pp;
+  CommitGeom geom{};
+  mutable E8* d_alpha = nullptr;  // top-code alpha_1 on the device, uploaded on first commit
+  mutable std::vector<uint64_t> zero_ranges;  // padding intervals (start, len) chunked for the device check
+  ~qf_params() { if (d_alpha) (void)hipFree(d_alpha); }
+};
+
+struct qf_committed {
+  const qf_params* p = nullptr;
+  int cache_mode = 0;
+  std::vector<Digest> roots;
+  std::vector<E8*> d_cw;         // per block (cached) or one workspace
+  std::vector<uint32_t*> d_tree;
+  double total_ms = 0, leaf_ms = 0, encode_ms = 0;
+  ~qf_committed() {
+    std::vector<void*> freed;
+    for (auto* q : d_cw) if (q && std::find(freed.begin(), freed.end(), (void*)q) == freed.end()) { freed.push_back(q); (void)hipFree(q); }
+    for (auto* q : d_tree) if (q && std::find(freed.begin(), freed.end(), (void*)q) == freed.end()) { freed.push_back(q); (void)hipFree(q); }
+  }
+};
+
+namespace {
+thread_local std::string g_error;
+}
+namespace qf {
+const Layout& params_layout(const qf_params* p) { return p->layout; }
+void set_last_error(const char* msg) { g_error = msg; }
+}
+
+namespace {
+template <class F>
+int guarded(F&& f) {
+  g_error.clear();
+  try {
+    f();
+    return QF_OK;
+  } catch (const QfError& e) {
+    g_error = e.what();
+    return e.status;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return QF_ERR_ARGUMENT;
+  }
+}
+
+// Interval list of entries that must be zero (layout.rs:136-155), split into
+// chunks of at most 64 KiB so the device check spreads over many workgroups.
+std::vector<uint64_t> padding_ranges(const Layout& l) {
+  std::vector<uint64_t> out;
+  auto add = [&](uint64_t start, uint64_t len) {
+    for (uint64_t o = 0; o < len; o += 65536) { out.push_back(start + o); out.push_back(std::min<uint64_t>(65536, len - o)); }
+  };
+  for (auto& b : l.buckets) {
+    if (b.physical_cols < b.cols)
+      for (uint64_t r = 0; r < b.physical_rows; r++) add(b.offset + r * b.cols + b.physical_cols, b.cols - b.physical_cols);
+    if (b.physical_rows < b.rows) add(b.offset + b.physical_rows * b.cols, (b.rows - b.physical_rows) * b.cols);
+  }
+  return out;
+}
+
+__global__ void k_check_zero(const int8_t* bva, const uint64_t* ranges, uint32_t* flag) {
+  uint64_t start = ranges[2 * blockIdx.x], len = ranges[2 * blockIdx.x + 1];
+  uint32_t bad = 0;
+  for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) bad |= bva[start + i] != 0;
+  if (bad) atomicOr(flag, 1u);
+}
+
+void validate_device(const qf_params* p, const int8_t* d_bva, hipStream_t st) {
+  if (p->zero_ranges.empty()) p->zero_ranges = padding_ranges(p->layout);
+  if (p->zero_ranges.empty()) return;
+  size_t count = p->zero_ranges.size() / 2;
+  uint64_t* d_ranges = nullptr;
+  uint32_t* d_flag = nullptr;
+  QF_HIP(hipMalloc(&d_ranges, p->zero_ranges.size() * 8));
+  QF_HIP(hipMalloc(&d_flag, 4));
+  QF_HIP(hipMemcpyAsync(d_ranges, p->zero_ranges.data(), p->zero_ranges.size() * 8, hipMemcpyHostToDevice, st));
+  QF_HIP(hipMemsetAsync(d_flag, 0, 4, st));
+  for (size_t first = 0; first < count; first += 65535) {
+    size_t n = std::min<size_t>(65535, count - first);
+    hipLaunchKernelGGL(k_check_zero, dim3(uint32_t(n)), dim3(256), 0, st, d_bva, d_ranges + 2 * first, d_flag);
+  }
+  uint32_t flag = 0;
+  QF_HIP(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, st));
+  QF_HIP(hipStreamSynchronize(st));
+  (void)hipFree(d_ranges);
+  (void)hipFree(d_flag);
+  require(flag == 0, QF_ERR_INVALID_PARAMETERS, "nonzero boundary padding");
+}
+
+void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_mode, qf_committed** out) {
+  require(len == p->layout.active, QF_ERR_INVALID_PARAMETERS, "active BVA length mismatch");
+  require(cache_mode == QF_CACHE_CACHED || cache_mode == QF_CACHE_RECOMPUTE, QF_ERR_ARGUMENT, "cache mode");
+  hipStream_t st;
+  QF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  std::unique_ptr<qf_committed> c(new qf_committed);
+  try {
+    c->p = p;
+    c->cache_mode = cache_mode;
+    validate_device(p, d_bva, st);
+    const uint64_t n = uint64_t(1) << p->geom.log_n, R = 2 * n;
+    const size_t cw_bytes = size_t(32 * R) * 16, tree_bytes = size_t(2 * R - 1) * 32;
+    if (!p->d_alpha) {
+      const auto& a = p->pp.codes.back().alpha1;
+      QF_HIP(hipMalloc(&p->d_alpha, a.size() * 16));
+      QF_HIP(hipMemcpy(p->d_alpha, a.data(), a.size() * 16, hipMemcpyHostToDevice));
+    }
+    hipEvent_t e0, e1, ev[4];
+    QF_HIP(hipEventCreate(&e0)); QF_HIP(hipEventCreate(&e1));
+    for (auto& e : ev) QF_HIP(hipEventCreate(&e));
+    QF_HIP(hipEventRecord(e0, st));
+    int zero_block = -1;
+    for (uint64_t b = 0; b < p->blocks; b++) {
+      uint64_t first = b * p->block_len;
+      if (first >= len && zero_block >= 0) {  // identical all-zero blocks share one commitment
+        c->roots.push_back(c->roots[zero_block]);
+        if (cache_mode == QF_CACHE_CACHED) { c->d_cw.push_back(c->d_cw[zero_block]); c->d_tree.push_back(c->d_tree[zero_block]); }
+        continue;
+      }
+      E8* cw = nullptr;
+      uint32_t* tree = nullptr;
+      if (cache_mode == QF_CACHE_CACHED || c->d_cw.empty()) {
+        QF_HIP(hipMalloc(&cw, cw_bytes));
+        QF_HIP(hipMalloc(&tree, tree_bytes));
+        c->d_cw.push_back(cw);
+        c->d_tree.push_back(tree);
+      } else {
+        cw = c->d_cw[0];
+        tree = c->d_tree[0];
+      }
+      launch_commit_block(p->geom, d_bva, first, len, cw, p->d_alpha, tree, st, ev);
+      QF_HIP(hipGetLastError());
+      Digest root;
+      QF_HIP(hipMemcpyAsync(root.data(), tree + 8 * (2 * R - 2), 32, hipMemcpyDeviceToHost, st));
+      QF_HIP(hipStreamSynchronize(st));
+      float a = 0, b2 = 0;
+      QF_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+      QF_HIP(hipEventElapsedTime(&b2, ev[1], ev[2]));
+      c->encode_ms += a;
+      c->leaf_ms += b2;
+      c->roots.push_back(root);
+      if (first >= len) zero_block = int(b);
+    }
+    QF_HIP(hipEventRecord(e1, st));
+    QF_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    QF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    c->total_ms = ms;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  } catch (...) {
+    (void)hipStreamDestroy(st);
+    throw;
+  }
+  QF_HIP(hipStreamDestroy(st));
+  *out = c.release();
+}
+}  // namespace
+
+extern "C" {
+
+const char* qf_last_error(void) { return g_error.c_str(); }
+
+int qf_params_new(int field, const uint64_t* shapes, size_t n, uint32_t chunk_log, uint32_t packed_log,
+                  uint32_t query_rf_bits, qf_params** out) {
+  return guarded([&] {
+    require(out && shapes, QF_ERR_ARGUMENT, "null argument");
+    require(field == QF_FIELD_F65521, QF_ERR_INVALID_PARAMETERS, "unsupported field backend");
+    require(chunk_log >= 10 && chunk_log <= 30 && packed_log >= 1 && packed_log <= 6 && query_rf_bits >= 32 &&
+                query_rf_bits <= 120,
+            QF_ERR_INVALID_PARAMETERS, "invalid ParComm config");
+    std::unique_ptr<qf_params> p(new qf_params);
+    std::vector<Shape> s(n);
+    for (size_t i = 0; i < n; i++) s[i] = {shapes[2 * i], shapes[2 * i + 1]};
+    p->layout = make_layout(s);
+    p->cfg_chunk_log = chunk_log;
+    p->packed_log = packed_log;
+    p->query_bits = query_rf_bits;
+    p->chunk_log = std::min<int>(int(chunk_log), ilog2u(p->layout.logical));
+    p->block_len = uint64_t(1) << p->chunk_log;
+    p->blocks = p->layout.logical / p->block_len;
+    require(p->blocks <= 128, QF_ERR_INVALID_PARAMETERS, "at most 128 PCS blocks are supported");
+    uint8_t seed[32];
+    std::memset(seed, 0x51, 32);  // protocol.rs:123
+    p->pp = make_pcs_params(p->chunk_log, query_rf_bits, seed);
+    p->geom.log_n = p->chunk_log - 8;
+    p->geom.leaf = leaf_prefix(32);
+    p->geom.inner = inner_prefix();
+    *out = p.release();
+  });
+}
+
+void qf_params_free(qf_params* p) { delete p; }
+
+int qf_params_digest(const qf_params* p, uint8_t out[32]) {
+  return guarded([&] { require(p && out, QF_ERR_ARGUMENT, "null argument"); std::memcpy(out, p->pp.parameter_digest.data(), 32); });
+}
+
+int qf_params_info(const qf_params* p, uint64_t* active, uint64_t* logical, uint64_t* block_len, uint64_t* blocks,
+                   uint64_t* n_buckets) {
+  return guarded([&] {
+    require(p, QF_ERR_ARGUMENT, "null argument");
+    if (active) *active = p->layout.active;
+    if (logical) *logical = p->layout.logical;
+    if (block_len) *block_len = p->block_len;
+    if (blocks) *blocks = p->blocks;
+    if (n_buckets) *n_buckets = p->layout.buckets.size();
+  });
+}
+
+int qf_params_bucket(const qf_params* p, uint64_t i, uint64_t out[8]) {
+  return guarded([&] {
+    require(p && out && i < p->layout.buckets.size(), QF_ERR_ARGUMENT, "bucket index");
+    const Bucket& b = p->layout.buckets[i];
+    uint64_t v[8] = {b.matrix, b.row_offset, b.col_offset, b.rows, b.cols, b.physical_rows, b.physical_cols, b.offset};
+    std::memcpy(out, v, sizeof v);
+  });
+}
+
+int qf_commit(const qf_params* p, const int8_t* values, size_t len, int cache_mode, qf_committed** out) {
+  return guarded([&] {
+    require(p && out && (values || len == 0), QF_ERR_ARGUMENT, "null argument");
+    int8_t* d = nullptr;
+    QF_HIP(hipMalloc(&d, std::max<size_t>(len, 8)));
+    try {
+      QF_HIP(hipMemcpy(d, values, len, hipMemcpyHostToDevice));
+      commit_impl(p, d, len, cache_mode, out);
+    } catch (...) {
+      (void)hipFree(d);
+      throw;
+    }
+    (void)hipFree(d);
+  });
+}
+
+int qf_commit_device(const qf_params* p, const int8_t* device_values, size_t len, int cache_mode, qf_committed** out) {
+  return guarded([&] {
+    require(p && out && device_values, QF_ERR_ARGUMENT, "null argument");
+    commit_impl(p, device_values, len, cache_mode, out);
+  });
+}
+
+void qf_committed_free(qf_committed* c) { delete c; }
+
+int qf_committed_roots(const qf_committed* c, uint8_t* out, size_t cap_blocks) {
+  return guarded([&] {
+    require(c && out && cap_blocks >= c->roots.size(), QF_ERR_ARGUMENT, "root buffer too small");
+    for (size_t i = 0; i < c->roots.size(); i++) std::memcpy(out + 32 * i, c->roots[i].data(), 32);
+  });
+}
+
+int qf_committed_codeword(const qf_committed* c, uint64_t block, void* out, size_t cap) {
+  return guarded([&] {
+    require(c && out && c->cache_mode == QF_CACHE_CACHED && block < c->d_cw.size(), QF_ERR_ARGUMENT, "codeword request");
+    size_t bytes = size_t(32) * 2 * (size_t(1) << c->p->geom.log_n) * 16;
+    require(cap >= bytes, QF_ERR_ARGUMENT, "codeword buffer too small");
+    QF_HIP(hipMemcpy(out, c->d_cw[block], bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms, double* encode_ms) {
+  return guarded([&] {
+    require(c, QF_ERR_ARGUMENT, "null argument");
+    if (total_ms) *total_ms = c->total_ms;
+    if (leaf_ms) *leaf_ms = c->leaf_ms;
+    if (encode_ms) *encode_ms = c->encode_ms;
+  });
+}
+
+}  // extern "C"

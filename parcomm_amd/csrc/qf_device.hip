@@ -1,0 +1,94 @@
+// Device buffers and the synthetic BVA generator (plumbing for resident inputs).
+#include "../../include/parcomm_amd.h"
+#include "qf_kernels.hpp"
+
+using namespace qf;
+
+struct qf_params;
+namespace qf {
+const Layout& params_layout(const qf_params* p);
+}
+
+namespace qf {
+void set_last_error(const char* msg);
+}
+
+namespace {
+
+__device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Entry i of the BVA: Box-Muller normal from two 32-bit uniforms of mix64(seed ^ i),
+// scaled, rounded half away from zero, clipped to int8.
+__global__ void k_synthetic(int8_t* out, uint64_t n, uint64_t seed, float sigma) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) {
+    uint64_t r = mix64(seed * 0x9e3779b97f4a7c15ull + i);
+    float u1 = (float(uint32_t(r >> 40)) + 0.5f) * (1.0f / 16777216.0f);
+    float u2 = float(uint32_t(r) & 0xffffffu) * (1.0f / 16777216.0f);
+    float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+    float v = rintf(z * sigma);
+    v = fminf(fmaxf(v, -128.0f), 127.0f);
+    out[i] = int8_t(int(v));
+  }
+}
+
+// Zero a rows x width rectangle of a row-major bucket (padding region).
+__global__ void k_zero_rect(int8_t* base, uint64_t rows, uint64_t stride, uint64_t width) {
+  uint64_t total = rows * width;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += uint64_t(gridDim.x) * blockDim.x)
+    base[(i / width) * stride + (i % width)] = 0;
+}
+
+template <class F>
+int guard(F&& f) {
+  set_last_error("");
+  try { f(); return QF_OK; } catch (const QfError& e) { set_last_error(e.what()); return e.status; }
+  catch (const std::exception& e) { set_last_error(e.what()); return QF_ERR_ARGUMENT; }
+}
+}  // namespace
+
+extern "C" {
+
+int qf_device_alloc(size_t bytes, void** out) {
+  return guard([&] {
+    require(out != nullptr, QF_ERR_ARGUMENT, "null argument");
+    QF_HIP(hipMalloc(out, bytes ? bytes : 1));
+    QF_HIP(hipMemset(*out, 0, bytes ? bytes : 1));
+  });
+}
+int qf_device_upload(void* dst, const void* src, size_t bytes) {
+  return guard([&] { QF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int qf_device_download(void* dst, const void* src, size_t bytes) {
+  return guard([&] { QF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int qf_device_free(void* p) {
+  return guard([&] { if (p) QF_HIP(hipFree(p)); });
+}
+
+int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* device_out) {
+  return guard([&] {
+    require(p && device_out, QF_ERR_ARGUMENT, "null argument");
+    const Layout& l = params_layout(p);
+    int8_t* out = static_cast<int8_t*>(device_out);
+    hipLaunchKernelGGL(k_synthetic, dim3(65536), dim3(256), 0, 0, out, l.active, seed, sigma);
+    for (const Bucket& b : l.buckets) {
+      int8_t* base = out + b.offset;
+      if (b.physical_cols < b.cols)
+        hipLaunchKernelGGL(k_zero_rect, dim3(1024), dim3(256), 0, 0, base + b.physical_cols, b.physical_rows, b.cols,
+                           b.cols - b.physical_cols);
+      if (b.physical_rows < b.rows)
+        hipLaunchKernelGGL(k_zero_rect, dim3(1024), dim3(256), 0, 0, base + b.physical_rows * b.cols,
+                           b.rows - b.physical_rows, b.cols, b.cols);
+    }
+    QF_HIP(hipGetLastError());
+    QF_HIP(hipDeviceSynchronize());
+  });
+}
+
+}  // extern "C"

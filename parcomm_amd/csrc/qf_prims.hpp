@@ -1,0 +1,134 @@
+// Field and hash primitives shared by the HIP kernels and the host protocol code.
+//
+// F65521 = Z/(2^16 - 15) (qafold_short.rs:325-477); its octic extension
+// F65521[X]/(X^8 - 17) (qafold_short.rs:482-660) is stored as eight canonical
+// u16 coefficients (16 bytes, coefficient i = X^i), which is also the
+// canonical wire encoding (full_extension.rs:157-164).
+//
+// Device code keeps coefficients unpacked in 32-bit lanes and reduces with the
+// unsigned-min trick: for s < 2P, min(s, s - P) (wrapping) is s mod P.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstring>
+
+namespace qf {
+
+constexpr uint32_t P = 65521;
+
+struct alignas(16) E8 {
+  uint16_t c[8];
+};
+
+__host__ __device__ inline uint32_t madd(uint32_t a, uint32_t b) {
+  uint32_t s = a + b;
+  uint32_t t = s - P;
+  return s < t ? s : t;
+}
+__host__ __device__ inline uint32_t msub(uint32_t a, uint32_t b) {
+  uint32_t d = a - b;
+  uint32_t e = d + P;
+  return d < e ? d : e;
+}
+// x < 2^32 -> x mod P (2^16 = 15 mod P)
+__host__ __device__ inline uint32_t mred32(uint32_t x) {
+  uint32_t y = (x & 0xffffu) + 15u * (x >> 16);  // < 2^20
+  y = (y & 0xffffu) + 15u * (y >> 16);           // < 65536 + 240
+  uint32_t t = y - P;
+  return y < t ? y : t;
+}
+__host__ __device__ inline uint32_t mred64(uint64_t x) {
+  uint64_t y = (x & 0xffffu) + 15u * (x >> 16);   // < 2^44 for x < 2^64? x < 2^40 here
+  y = (y & 0xffffu) + 15u * (y >> 16);
+  return mred32(uint32_t(y));
+}
+
+// Unpacked register form of one Ext8 element.
+struct V8 {
+  uint32_t c[8];
+};
+__host__ __device__ inline V8 v8_load(const E8& e) {
+  V8 r;
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(&e);
+#pragma unroll
+  for (int i = 0; i < 4; i++) { r.c[2 * i] = w[i] & 0xffffu; r.c[2 * i + 1] = w[i] >> 16; }
+  return r;
+}
+__host__ __device__ inline E8 v8_store(const V8& v) {
+  E8 e;
+  uint32_t* w = reinterpret_cast<uint32_t*>(&e);
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] = v.c[2 * i] | (v.c[2 * i + 1] << 16);
+  return e;
+}
+__host__ __device__ inline V8 v8_add(const V8& a, const V8& b) { V8 r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.c[i] = madd(a.c[i], b.c[i]); return r; }
+__host__ __device__ inline V8 v8_sub(const V8& a, const V8& b) { V8 r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.c[i] = msub(a.c[i], b.c[i]); return r; }
+__host__ __device__ inline V8 v8_zero() { V8 r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.c[i] = 0; return r; }
+// Product in F65521[X]/(X^8 - 17); 64-bit accumulation, one reduction per coefficient.
+__host__ __device__ inline V8 v8_mul(const V8& a, const V8& b) {
+  uint64_t t[15];
+#pragma unroll
+  for (int k = 0; k < 15; k++) t[k] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) t[i + j] += uint64_t(a.c[i] * b.c[j]);
+  V8 r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.c[i] = mred64(t[i] + (i < 7 ? 17ull * t[i + 8] : 0ull));
+  return r;
+}
+
+// ---------------------------------------------------------------- BLAKE2s-256 (RFC 7693)
+__host__ __device__ inline uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+#define QF_B2S_G(a, b, c, d, x, y)        \
+  v[a] = v[a] + v[b] + (x);               \
+  v[d] = rotr32(v[d] ^ v[a], 16);         \
+  v[c] = v[c] + v[d];                     \
+  v[b] = rotr32(v[b] ^ v[c], 12);         \
+  v[a] = v[a] + v[b] + (y);               \
+  v[d] = rotr32(v[d] ^ v[a], 8);          \
+  v[c] = v[c] + v[d];                     \
+  v[b] = rotr32(v[b] ^ v[c], 7);
+
+// Compression of one 64-byte block given as 16 little-endian words.
+// `t` is the byte count including this block; `last` sets the final flag.
+__host__ __device__ inline void b2s_compress(uint32_t h[8], const uint32_t m[16], uint32_t t, bool last) {
+  constexpr uint32_t IV0 = 0x6A09E667, IV1 = 0xBB67AE85, IV2 = 0x3C6EF372, IV3 = 0xA54FF53A,
+                     IV4 = 0x510E527F, IV5 = 0x9B05688C, IV6 = 0x1F83D9AB, IV7 = 0x5BE0CD19;
+  uint32_t v[16] = {h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7],
+                    IV0,  IV1,  IV2,  IV3,  IV4 ^ t, IV5, last ? ~IV6 : IV6, IV7};
+  // Message schedule, RFC 7693 table SIGMA (rounds 0..9).
+#define QF_B2S_ROUND(s0, s1, s2, s3, s4, s5, s6, s7, s8, s9, s10, s11, s12, s13, s14, s15) \
+  QF_B2S_G(0, 4, 8, 12, m[s0], m[s1]) QF_B2S_G(1, 5, 9, 13, m[s2], m[s3])                  \
+  QF_B2S_G(2, 6, 10, 14, m[s4], m[s5]) QF_B2S_G(3, 7, 11, 15, m[s6], m[s7])               \
+  QF_B2S_G(0, 5, 10, 15, m[s8], m[s9]) QF_B2S_G(1, 6, 11, 12, m[s10], m[s11])             \
+  QF_B2S_G(2, 7, 8, 13, m[s12], m[s13]) QF_B2S_G(3, 4, 9, 14, m[s14], m[s15])
+  QF_B2S_ROUND(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)
+  QF_B2S_ROUND(14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3)
+  QF_B2S_ROUND(11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4)
+  QF_B2S_ROUND(7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8)
+  QF_B2S_ROUND(9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13)
+  QF_B2S_ROUND(2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9)
+  QF_B2S_ROUND(12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11)
+  QF_B2S_ROUND(13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10)
+  QF_B2S_ROUND(6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5)
+  QF_B2S_ROUND(10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0)
+#undef QF_B2S_ROUND
+#pragma unroll
+  for (int i = 0; i < 8; i++) h[i] ^= v[i] ^ v[i + 8];
+}
+
+__host__ __device__ inline void b2s_init(uint32_t h[8]) {
+  h[0] = 0x6A09E667u ^ 0x01010020u; h[1] = 0xBB67AE85; h[2] = 0x3C6EF372; h[3] = 0xA54FF53A;
+  h[4] = 0x510E527F; h[5] = 0x9B05688C; h[6] = 0x1F83D9AB; h[7] = 0x5BE0CD19;
+}
+
+}  // namespace qf

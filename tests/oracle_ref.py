@@ -1,0 +1,60 @@
+"""Helpers that run the CPU oracle (the checker) on the same inputs as the
+product, for the parity tests."""
+import ctypes
+
+import numpy as np
+
+DEMO_SHAPES = [(31, 37), (65, 17), (1, 7), (9, 1)]
+
+
+def demo_bva(layout_encode):
+    """Demo matrices of the reference CLI: entry e of matrix i = (e*73 + i*19) as u8 as i8
+    (qafold/f65521/src/main.rs:133-144)."""
+    mats = []
+    for i, (r, c) in enumerate(DEMO_SHAPES):
+        e = np.arange(r * c, dtype=np.int64)
+        mats.append(((e * 73 + i * 19) % 256).astype(np.uint8).view(np.int8))
+    return layout_encode(mats)
+
+
+def layout_encode(params, mats):
+    """layout.rs:115-135 on the host from the product's bucket table."""
+    out = np.zeros(params.active, dtype=np.int8)
+    for (m, ro, co, rows, cols, pr, pc, off) in params.buckets():
+        src = mats[m].reshape(params.shapes[m].rows, params.shapes[m].cols)
+        block = src[ro:ro + pr, co:co + pc]
+        view = out[off:off + rows * cols].reshape(rows, cols)
+        view[:pr, :pc] = block
+    return out
+
+
+def oracle_commit(oracle, chunk_log, values, blocks, want_codeword=False):
+    h = oracle.orc_pcs_new(chunk_log, 100, bytes([0x51] * 32))
+    assert h, oracle.orc_last_error()
+    try:
+        block_len = 1 << chunk_log
+        roots, cws = [], []
+        for b in range(blocks):
+            first = b * block_len
+            avail = max(0, min(block_len, len(values) - first))
+            chunk = np.zeros(block_len, dtype=np.int8)
+            chunk[:avail] = values[first:first + avail]
+            root = (ctypes.c_uint8 * 32)()
+            cw = None
+            if want_codeword:
+                cw = np.empty((32, 2 * (block_len // 256), 8), dtype=np.uint16)
+            rc = oracle.orc_commit_block(ctypes.c_void_p(h), chunk.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_uint64(block_len), root, None,
+                                         cw.ctypes.data_as(ctypes.c_void_p) if cw is not None else None)
+            assert rc == 0, oracle.orc_last_error()
+            roots.append(bytes(root))
+            cws.append(cw)
+        return roots, cws
+    finally:
+        oracle.orc_pcs_free(h)
+
+
+def synthetic_int8(shape_count, seed, sigma=20.0):
+    """round(N(0, sigma)) clipped to [-128, 127], seeded."""
+    rng = np.random.default_rng(seed)
+    return np.clip(np.rint(rng.normal(0.0, sigma, size=shape_count)), -128, 127).astype(np.int8)
