@@ -115,3 +115,64 @@ void orc_merkle_root_rows(const uint16_t* entries, size_t rows, size_t cols, uin
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- full prover (main.rs:175-220 flow)
+#include "qf_open.hpp"
+extern "C" {
+// Commits the BVA, derives the benchmark points from the statement
+// (main.rs:190-204), evaluates the claims and proves. Writes the bincode proof
+// into `proof` (capacity `cap`), its length into `proof_len`, and the compact
+// JSON of the commitment and claims into the optional buffers.
+int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg, uint32_t packed_log,
+                        const int8_t* values, uint64_t n_values, uint8_t* proof, size_t cap, size_t* proof_len,
+                        char* statement_json, size_t statement_cap, char* claims_json, size_t claims_cap) {
+  return guarded([&] {
+    std::vector<Shape> s(n);
+    for (size_t i = 0; i < n; i++) s[i] = {shapes[2 * i], shapes[2 * i + 1]};
+    Layout l = layout_new(s);
+    layout_validate(l, values, n_values);
+    int chunk_log = std::min<int>(int(chunk_log_cfg), ilog2(l.logical));
+    u64 block_len = u64(1) << chunk_log;
+    u64 blocks = l.logical / block_len;
+    check(blocks <= 128, "at most 128 PCS blocks are supported");
+    u8 seed[32];
+    std::memset(seed, 0x51, 32);
+    PcsParams pp = pcs_setup(size_t(1) << (chunk_log - 3), 32, 2, 100, seed);
+    Statement st;
+    st.config = {chunk_log_cfg, packed_log, 100};
+    st.shapes = s;
+    st.parameter_digest = pp.parameter_digest;
+    for (u64 b = 0; b < blocks; b++) {
+      u64 first = b * block_len;
+      u64 avail = first < n_values ? std::min<u64>(block_len, n_values - first) : 0;
+      Committed c = packed_commit(pp, pack_block_i8(values + (avail ? first : 0), avail, block_len));
+      st.roots.push_back(c.root());
+    }
+    OuterTranscript pts;
+    pts.object("benchmark-points", json_statement(st));
+    std::vector<std::pair<std::vector<E8>, std::vector<E8>>> points;
+    for (auto& sh : s) {
+      u64 rp = 1, cp = 1;
+      while (rp < sh.rows) rp <<= 1;
+      while (cp < sh.cols) cp <<= 1;
+      std::vector<E8> row, col;
+      for (int i = 0; i < ilog2(rp); i++) row.push_back(pts.challenge("row"));
+      for (int i = 0; i < ilog2(cp); i++) col.push_back(pts.challenge("col"));
+      points.push_back({row, col});
+    }
+    auto claims = evaluate_claims(l, values, n_values, points);
+    Config cfg{chunk_log_cfg, packed_log, 100};
+    Bytes pf = parcomm_prove(l, cfg, pp, chunk_log, st, values, n_values, claims);
+    *proof_len = pf.size();
+    check(pf.size() <= cap, "proof buffer too small");
+    std::memcpy(proof, pf.data(), pf.size());
+    auto put = [](const std::string& js, char* out, size_t c) {
+      if (!out) return;
+      check(js.size() + 1 <= c, "json buffer too small");
+      std::memcpy(out, js.c_str(), js.size() + 1);
+    };
+    put(json_statement(st), statement_json, statement_cap);
+    put(json_claims(claims), claims_json, claims_cap);
+  });
+}
+}
