@@ -176,3 +176,45 @@ int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg
   });
 }
 }
+
+extern "C" {
+// qafold_short_pc.rs:3605-3647 (native_opening_and_decide_wire_fixture):
+// ShortQaFoldPC<F>::new(n_witness, ext_cols, 2, bits) with StdRng::seed_from_u64(rng_seed);
+// writes BLAKE2s(root || opening || decide) and the opening/decide lengths.
+int orc_native_fixture(uint64_t n_witness, uint64_t ext_cols, uint64_t bits, uint64_t rng_seed, uint8_t* digest,
+                       uint64_t* opening_len, uint64_t* decide_len) {
+  return guarded([&] {
+    u8 seed[32];
+    seed_from_u64(rng_seed, seed);
+    ChaChaRng rng(seed, 12);
+    PcsParams pp = pcs_setup_rng(n_witness, ext_cols, 2, bits, rng);
+    auto sample = [&]() {
+      for (;;) {
+        u8 b[16];
+        u64 lo = rng.next_u64(), hi = rng.next_u64();
+        for (int i = 0; i < 8; i++) { b[i] = u8(lo >> (8 * i)); b[8 + i] = u8(hi >> (8 * i)); }
+        E8 v;
+        if (e8_read(b, v)) return v;
+      }
+    };
+    std::vector<E8> witness(n_witness);
+    for (auto& w : witness) w = sample();
+    std::vector<E8> point(ilog2(n_witness));
+    for (auto& p : point) p = sample();
+    Committed c = pcs_commit_owned(pp, witness);
+    Bytes opening;
+    OpeningReduction red;
+    pcs_open_with_reduction(pp, c.root(), point, c, opening, red);
+    Bytes decide = pcs_prove_decide(pp, red);
+    Blake2s h;
+    Digest r = c.root();
+    h.update(r.data(), 32);
+    h.update(opening);
+    h.update(decide);
+    Digest d = h.finalize();
+    std::memcpy(digest, d.data(), 32);
+    *opening_len = opening.size();
+    *decide_len = decide.size();
+  });
+}
+}

@@ -315,12 +315,16 @@ static Digest parameter_digest(const PcsParams& pp) {
 
 // qafold_short_pc.rs:401-443 with qafold_short_setup.rs:203-221, 427-537, 655-741
 PcsParams pcs_setup(size_t n_witness, size_t ext_cols, size_t recursive_index, size_t security_bits, const u8 rng_seed[32]) {
+  ChaChaRng rng(rng_seed, 12);
+  return pcs_setup_rng(n_witness, ext_cols, recursive_index, security_bits, rng);
+}
+
+PcsParams pcs_setup_rng(size_t n_witness, size_t ext_cols, size_t recursive_index, size_t security_bits, ChaChaRng& rng) {
   PcsParams pp;
   pp.n_witness = n_witness;
   pp.ext_cols = ext_cols;
   pp.recursive_index = recursive_index;
   pp.security_bits = security_bits;
-  ChaChaRng rng(rng_seed, 12);
   Digest seed;
   rng.fill_bytes(seed.data(), 32);
   pp.seq = qa_sequence_new(n_witness, ext_cols, recursive_index, seed);

@@ -66,6 +66,8 @@ struct PcsParams {
   std::vector<SetupLevel> setup;  // setup[level], level 0 unused
 };
 PcsParams pcs_setup(size_t n_witness, size_t ext_cols, size_t recursive_index, size_t security_bits, const u8 rng_seed[32]);
+// Setup drawing the sequence seed from a caller-owned StdRng (ShortQaFoldPC::setup).
+PcsParams pcs_setup_rng(size_t n_witness, size_t ext_cols, size_t recursive_index, size_t security_bits, ChaChaRng& rng);
 Digest setup_row_hash(const SetupShape& s, size_t row, const E8* row_vals);
 
 // ---------------------------------------------------------------- commit of one packed block

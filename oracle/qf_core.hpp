@@ -207,6 +207,18 @@ struct ChaChaRng {
   }
 };
 
+// rand_core 0.6 SeedableRng::seed_from_u64: PCG32 output words fill the 32-byte seed.
+inline void seed_from_u64(u64 state, u8 seed[32]) {
+  const u64 MUL = 6364136223846793005ull, INC = 11634580027462260723ull;
+  for (int w = 0; w < 8; w++) {
+    state = state * MUL + INC;
+    u32 xs = u32(((state >> 18) ^ state) >> 27);
+    u32 rot = u32(state >> 59);
+    u32 x = (xs >> rot) | (xs << ((32 - rot) & 31));
+    for (int j = 0; j < 4; j++) seed[4 * w + j] = u8(x >> (8 * j));
+  }
+}
+
 inline void put_u64(Bytes& out, u64 v) { for (int i = 0; i < 8; i++) out.push_back(u8(v >> (8 * i))); }
 inline void put_e8(Bytes& out, const E8& v) { u8 b[16]; e8_write(v, b); out.insert(out.end(), b, b + 16); }
 inline int ilog2(u64 x) { int r = 0; while ((u64(1) << (r + 1)) <= x && r < 63) r++; return r; }

@@ -792,8 +792,116 @@ E8 pcs_open_with_reduction(const PcsParams& pp, const Digest& root, const std::v
   return claimed;
 }
 
-// qafold_short_decide_driver.rs:51-80 and qafold_short_decide_protocol.rs:29-52: the
-// opening statement is absorbed; with a single QA level there is nothing to decide.
+// ---------------------------------------------------------------- Decide (direct-interleaved setup)
+// qafold_short.rs:1575-1625: G_j claim -> H_j (alpha matrix) claim.
+struct MatrixClaim { E8 value; std::vector<E8> left, right; };
+static MatrixClaim reduce_generator_claim(u64 message_len, u64 index, const MatrixClaim& c) {
+  size_t bv = ilog2(index);
+  check(c.left.size() == size_t(ilog2(message_len)) + bv && c.right.size() == size_t(ilog2(message_len)), "generator claim dims");
+  E8 norm = e8_one();
+  std::vector<E8> merged;
+  for (size_t i = 0; i < c.right.size(); i++) {
+    E8 l = c.left[bv + i], r = c.right[i];
+    E8 dl = e8_one() + l, dr = e8_one() + r;
+    norm = norm * dl * dr;
+    merged.push_back(((e8_one() - l) * e8_inv(dl)) * ((e8_one() - r) * e8_inv(dr)));
+  }
+  MatrixClaim out;
+  out.value = c.value * e8_inv(norm);
+  out.left = merged;  // row point of the alpha matrix
+  out.right.assign(c.left.begin(), c.left.begin() + bv);  // block (column) point
+  return out;
+}
+static E8 tensor_inner_product(const std::vector<E8>& a, const std::vector<E8>& b) {
+  check(a.size() == b.size(), "tensor inner product");
+  E8 r = e8_one();
+  for (size_t i = 0; i < a.size(); i++) r = r * (e8_one() + a[i] * b[i]);
+  return r;
+}
+struct Selection { bool all; u64 count; std::vector<u64> rows; };
+static void absorb_selection(PcsTranscript& tr, const Selection& s) {  // qafold_short_decide_carry.rs:721-739
+  tr.absorb_str("selection-kind", s.all ? "all" : "partial");
+  tr.absorb_u64("selection-count", s.all ? s.count : s.rows.size());
+  if (!s.all) for (u64 r : s.rows) tr.absorb_u64("selection-row", r);
+}
+static std::vector<u64> selection_rows(const Selection& s) {
+  if (!s.all) return s.rows;
+  std::vector<u64> v(s.count);
+  for (u64 i = 0; i < s.count; i++) v[i] = i;
+  return v;
+}
+// Multi-evaluation reduction on the mixed column-major matrix (qafold_short_decide.rs:486-551),
+// restated as one LiteBullet over the flattened matrix and weight vectors.
+struct EvalClaim { E8 value; std::vector<E8> row_point, col_weight; };
+static std::pair<E8, std::vector<E8>> prove_multi_eval(const std::vector<EvalClaim>& claims,
+                                                      const std::vector<const E8*>& columns, size_t log_rows,
+                                                      size_t log_cols, PcsTranscript& tr, std::vector<E8>& folded_vector) {
+  size_t rows = size_t(1) << log_rows, cols = size_t(1) << log_cols;
+  tr.absorb_str("multi-eval-domain", "qafold-short-decide/multi-eval/v2");
+  tr.absorb_u64("multi-eval-log-rows", log_rows);
+  tr.absorb_u64("multi-eval-log-cols", log_cols);
+  tr.absorb_u64("multi-eval-claim-count", claims.size());
+  for (auto& c : claims) {
+    tr.absorb_ext("multi-eval-claim-value", c.value);
+    tr.absorb_exts("multi-eval-row-point", c.row_point);
+    tr.absorb_exts("multi-eval-column-weight", c.col_weight);
+  }
+  E8 rho = tr.challenge("multi-eval-batch");
+  E8 rhs = e8_zero(), pw = e8_one();
+  std::vector<E8> left(rows * cols, e8_zero()), right(rows * cols, e8_zero());
+  for (size_t k = 0; k < columns.size(); k++) std::memcpy(&left[k * rows], columns[k], rows * sizeof(E8));
+  for (auto& c : claims) {
+    rhs += pw * c.value;
+    auto rw = tensor_weights(c.row_point);
+    for (size_t k = 0; k < c.col_weight.size(); k++) {
+      E8 s = pw * c.col_weight[k];
+      for (size_t r = 0; r < rows; r++) right[k * rows + r] += s * rw[r];
+    }
+    pw = pw * rho;
+  }
+  std::vector<E8> a_point, b_point;
+  size_t n = rows * cols;
+  while (n > 1) {
+    size_t h = n / 2;
+    E8 lc = e8_zero(), rc = e8_zero();
+    for (size_t i = 0; i < h; i++) { lc += left[i] * right[h + i]; rc += left[h + i] * right[i]; }
+    tr.push_response(lc);
+    tr.push_response(rc);
+    E8 ch = tr.challenge("multi-eval-fold", true), ci = e8_inv(ch);
+    rhs += lc * ch + rc * ci;
+    for (size_t i = 0; i < h; i++) { left[i] += left[h + i] * ci; right[i] += right[h + i] * ch; }
+    a_point.push_back(ci);
+    b_point.push_back(ch);
+    n = h;
+    if (n == rows) folded_vector.assign(left.begin(), left.begin() + rows);
+  }
+  if (cols == 1) folded_vector.assign(columns[0], columns[0] + rows);
+  tr.push_response(left[0]);
+  tr.push_response(right[0]);
+  check(left[0] * right[0] == rhs, "multi-eval terminal");
+  return {left[0], a_point};
+}
+static void write_ext_rows(Bytes& out, const Committed& c, const Selection& s) {  // carry rows wire
+  if (s.all) {
+    out.push_back(1);
+    put_u64(out, c.rows);
+    put_u64(out, c.cols);
+    put_u64(out, c.encoded.size());
+    for (auto& v : c.encoded) put_e8(out, v);
+  } else {
+    out.push_back(0);
+    put_u64(out, s.rows.size());
+    for (u64 r : s.rows) {
+      put_u64(out, c.cols);
+      for (size_t k = 0; k < c.cols; k++) put_e8(out, c.encoded[k * c.rows + r]);
+      auto path = c.tree.path(r);
+      put_u64(out, path.size());
+      for (auto& d : path) out.insert(out.end(), d.begin(), d.end());
+    }
+  }
+}
+
+// qafold_short_decide_driver.rs:51-318 (DirectInterleavedV1 branches)
 Bytes pcs_prove_decide(const PcsParams& pp, const OpeningReduction& red) {
   size_t levels = pp.seq.codes.size();
   PcsTranscript tr;
@@ -810,14 +918,201 @@ Bytes pcs_prove_decide(const PcsParams& pp, const OpeningReduction& red) {
     tr.absorb_exts("pending-g-left", p.left);
     tr.absorb_exts("pending-g-right", p.right);
   }
-  check(levels <= 1, "multi-level Decide is not restated in the oracle yet");
+  std::vector<std::vector<MatrixClaim>> h_claims(levels);
+  for (auto& p : red.pending)
+    h_claims[p.level].push_back(reduce_generator_claim(pp.seq.codes[p.level].message_len, pp.seq.codes[p.level].index,
+                                                       {p.value, p.left, p.right}));
+  Bytes rounds_wire;
+  size_t n_rounds = 0;
+  std::vector<E8> base_vector;
+  const size_t rf = pp.ext_cols;
+  bool have_w = false;
+  EvalClaim w_claim;
+  std::unique_ptr<Committed> carry;         // encoded carry of the current level
+  std::vector<E8> carry_message;            // its logical message column
+  for (size_t level = levels - 1; level >= 1; level--) {
+    const SetupLevel& sl = pp.setup[level];
+    const QaCode& code = pp.seq.codes[level];
+    size_t rows = code.message_len, log_rows = ilog2(rows);
+    std::vector<EvalClaim> claims;
+    for (auto& h : h_claims[level]) claims.push_back({h.value, h.left, tensor_weights(h.right)});
+    h_claims[level].clear();
+    size_t setup_cols = code.alphas.size(), carry_cols = 0;
+    if (have_w) {
+      EvalClaim c = w_claim;
+      c.col_weight.assign(setup_cols, e8_zero());
+      c.col_weight.push_back(e8_one());
+      claims.push_back(c);
+      carry_cols = 1;
+    }
+    check(!claims.empty(), "no Decide claims");
+    u64 real = setup_cols + carry_cols, padded = 1;
+    while (padded < real) padded <<= 1;
+    size_t log_cols = ilog2(padded);
+    // round context (decide_driver.rs:716-749)
+    tr.absorb_str("decide-round-domain", "qafold-short/decide-round/v1");
+    tr.absorb_u64("decide-round-level", level);
+    const SetupShape& sh = sl.shape;
+    tr.absorb_u64("decide-round-shape-level", sh.level);
+    tr.absorb_u64("decide-round-shape-encoded-rows", sh.encoded_rows);
+    tr.absorb_u64("decide-round-shape-message-rows", sh.message_rows);
+    tr.absorb_u64("decide-round-shape-t-cols", sh.t_cols);
+    tr.absorb_u64("decide-round-shape-t-elements", sh.t_elements);
+    tr.absorb_u64("decide-round-shape-t-last-len", sh.t_last_len);
+    tr.absorb_u64("decide-round-shape-h-cols", sh.h_cols);
+    Digest setup_root = sl.tree.root();
+    tr.absorb_digest("decide-round-setup-root", setup_root);
+    u8 present = carry ? 1 : 0;
+    tr.absorb_bytes("decide-round-carry-present", &present, 1);
+    if (carry) tr.absorb_digest("decide-round-carry-root", carry->root());
+    std::vector<const E8*> columns;
+    for (auto& a : code.alphas) columns.push_back(a.data());
+    if (carry) columns.push_back(carry_message.data());
+    std::vector<E8> folded_vector;
+    auto [red_value, red_point] = prove_multi_eval(claims, columns, log_rows, log_cols, tr, folded_vector);
+    std::vector<E8> col_point(red_point.begin(), red_point.begin() + log_cols);
+    std::vector<E8> merged_left(red_point.begin() + log_cols, red_point.end());
+    auto col_weights = tensor_weights(col_point);
+    const QaCode& target = pp.seq.codes[level - 1];
+    u64 n_enc = target.codeword_len();
+    check(sh.encoded_rows == n_enc, "setup encoded rows");
+    u64 count = num_queries(target, pp.security_bits, n_enc);
+    Digest seed = tr.challenge_digest("decide-row-selection");
+    Selection sel{count == n_enc, count, {}};
+    if (!sel.all) sel.rows = sample_without_replacement(count, n_enc, seed);
+    auto sel_rows = selection_rows(sel);
+    // iota weights over physical columns (decide_driver.rs:617-653)
+    size_t block_log = ilog2(rf);
+    auto block_w = tensor_weights(std::vector<E8>(merged_left.begin(), merged_left.begin() + block_log));
+    u64 phys = real * rf, padded_phys = 1;
+    while (padded_phys < phys) padded_phys <<= 1;
+    std::vector<E8> iw(padded_phys, e8_zero());
+    for (u64 lc = 0; lc < real; lc++)
+      for (size_t b = 0; b < rf; b++) iw[lc * rf + b] = col_weights[lc] * block_w[b];
+    size_t phys_carry = carry_cols * rf;
+    // iota statement and values (decide_carry.rs:336-380, 471-615)
+    tr.absorb_str("iota-domain", "qafold-short/decide-iota/v1");
+    tr.absorb_u64("iota-level", level);
+    tr.absorb_u64("iota-shape-level", sh.level);
+    tr.absorb_u64("iota-shape-encoded-rows", sh.encoded_rows);
+    tr.absorb_u64("iota-shape-message-rows", sh.message_rows);
+    tr.absorb_u64("iota-shape-t-cols", sh.t_cols);
+    tr.absorb_u64("iota-shape-t-elements", sh.t_elements);
+    tr.absorb_u64("iota-shape-t-last-len", sh.t_last_len);
+    tr.absorb_u64("iota-shape-h-cols", sh.h_cols);
+    tr.absorb_digest("iota-setup-root", setup_root);
+    tr.absorb_bytes("iota-carry-present", &present, 1);
+    if (carry) tr.absorb_digest("iota-carry-root", carry->root());
+    tr.absorb_u64("iota-carry-cols", phys_carry);
+    absorb_selection(tr, sel);
+    tr.absorb_exts("iota-column-weights", iw);
+    size_t scols = sh.t_cols + sh.h_cols;
+    std::vector<E8> iota;
+    for (u64 r : sel_rows) {
+      E8 v = e8_zero();
+      for (size_t c = 0; c < scols; c++) v += iw[c] * sl.encoded[c * n_enc + r];
+      if (carry)
+        for (size_t c = 0; c < phys_carry; c++) v += iw[scols + c] * carry->encoded[c * n_enc + r];
+      iota.push_back(v);
+    }
+    for (auto& v : iota) tr.push_response(v);
+    // round wire: level, next carry root, setup rows, carry rows
+    Bytes round;
+    put_u64(round, level);
+    // fold the block coordinate (decide_carry.rs:182-222)
+    size_t trows = target.message_len;
+    std::vector<E8> rec(trows, e8_zero());
+    for (size_t r = 0; r < trows; r++)
+      for (size_t b = 0; b < rf; b++) rec[r] += folded_vector[b * trows + r] * block_w[b];
+    std::unique_ptr<Committed> next_carry;
+    if (level > 1) {
+      const QaCode& nt = pp.seq.codes[level - 2];
+      check(trows == nt.message_len * rf, "carry row factor");
+      next_carry.reset(new Committed(commit_extension(nt, rec, rf)));
+    }
+    tr.absorb_str("decide-next-carry-domain", "qafold-short/decide-next-carry/v1");
+    u8 np = next_carry ? 1 : 0;
+    tr.absorb_bytes("decide-next-carry-present", &np, 1);
+    if (next_carry) tr.absorb_digest("decide-next-carry-root", next_carry->root());
+    round.push_back(np);
+    if (next_carry) { Digest r = next_carry->root(); round.insert(round.end(), r.begin(), r.end()); }
+    if (sel.all) {
+      round.push_back(1);
+      for (u64 v : {sh.level, sh.encoded_rows, sh.message_rows, sh.t_cols, sh.t_elements, sh.t_last_len, sh.h_cols}) put_u64(round, v);
+      put_u64(round, sl.encoded.size());
+      for (auto& v : sl.encoded) put_e8(round, v);
+    } else {
+      round.push_back(0);
+      put_u64(round, sel.rows.size());
+      for (u64 r : sel.rows) {
+        put_u64(round, r);
+        put_u64(round, scols);
+        for (size_t c = 0; c < scols; c++) put_e8(round, sl.encoded[c * n_enc + r]);
+        auto path = sl.tree.path(r);
+        put_u64(round, path.size());
+        for (auto& d : path) round.insert(round.end(), d.begin(), d.end());
+      }
+    }
+    if (carry) { round.push_back(1); write_ext_rows(round, *carry, sel); } else round.push_back(0);
+    // direct code switch (decide_carry.rs:940-1027)
+    std::vector<E8> base_point(merged_left.begin() + block_log, merged_left.end());
+    E8 mv;
+    check(mle_eval(rec, base_point, mv) && mv == red_value, "merged carry value");
+    tr.absorb_str("direct-code-switch-domain", "qafold-short/decide-direct-code-switch/v2");
+    tr.absorb_u64("direct-code-switch-target-level", level - 1);
+    tr.absorb_u64("direct-code-switch-row-factor", rf);
+    tr.absorb_u64("direct-code-switch-message-len", target.message_len);
+    tr.absorb_u64("direct-code-switch-encoded-len", n_enc);
+    tr.absorb_u64("direct-code-switch-target-index", target.index);
+    tr.absorb_digest("direct-code-switch-target-generator", target.generator_digest);
+    absorb_selection(tr, sel);
+    tr.absorb_exts("direct-code-switch-iota", iota);
+    tr.absorb_ext("direct-code-switch-merged-value", red_value);
+    tr.absorb_exts("direct-code-switch-merged-left-point", merged_left);
+    u64 sp = 1;
+    while (sp < sel_rows.size()) sp <<= 1;
+    std::vector<E8> slp;
+    for (int i = 0; i < ilog2(sp); i++) slp.push_back(tr.challenge("direct-code-switch-selection-left"));
+    std::vector<E8> sw(sel_rows.size());
+    for (size_t i = 0; i < sel_rows.size(); i++) sw[i] = tensor_weight_at(slp, i);
+    E8 selected_value = inner(sw, iota);
+    std::vector<E8> sparse(n_enc, e8_zero());
+    for (size_t i = 0; i < sel_rows.size(); i++) sparse[sel_rows[i]] += sw[i];
+    auto gl = generator_left(target, sparse);
+    E8 rho = tr.challenge("direct-code-switch-opening-batch");
+    auto ml = tensor_weights(base_point);
+    for (size_t i = 0; i < gl.size(); i++) gl[i] += rho * ml[i];
+    LiteBullet msg = prove_litebullet(selected_value + rho * red_value, gl, rec, tr, "decide-direct-carry-message-litebullet");
+    E8 sle = msg.a_value - rho * tensor_inner_product(base_point, msg.a_point);
+    LiteBullet gen = prove_litebullet(sle, sparse, generator_right(target, tensor_weights(msg.a_point)), tr,
+                                      "decide-direct-carry-generator-litebullet");
+    MatrixClaim gclaim{gen.b_value, gen.b_point, msg.a_point};
+    if (level > 1) {
+      h_claims[level - 1].push_back(reduce_generator_claim(target.message_len, target.index, gclaim));
+      w_claim = {msg.b_value, msg.b_point, {}};
+      have_w = true;
+      carry = std::move(next_carry);
+      carry_message = rec;
+    } else {
+      E8 tv;
+      check(generator_mle(target, gclaim.left, gclaim.right) == gclaim.value, "base generator claim");
+      check(mle_eval(rec, msg.b_point, tv) && tv == msg.b_value, "terminal carry");
+      for (auto& v : rec) tr.push_response(v);
+      base_vector = rec;
+      carry.reset();
+    }
+    rounds_wire.insert(rounds_wire.end(), round.begin(), round.end());
+    n_rounds++;
+  }
   Bytes w;
   const char* magic = "QFSDCID1";
   w.insert(w.end(), magic, magic + 8);
   put_u64(w, tr.responses.size());
   for (auto& v : tr.responses) put_e8(w, v);
-  put_u64(w, 0);  // rounds
-  put_u64(w, 0);  // base vector
+  put_u64(w, n_rounds);
+  w.insert(w.end(), rounds_wire.begin(), rounds_wire.end());
+  put_u64(w, base_vector.size());
+  for (auto& v : base_vector) put_e8(w, v);
   return w;
 }
 

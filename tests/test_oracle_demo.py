@@ -68,3 +68,29 @@ def test_tampered_witness_changes_the_proof(oracle):
     values[5] ^= 1
     proof, _, _ = oracle_prove(oracle, flat, len(DEMO_SHAPES), 10, 4, values)
     assert hashlib.blake2s(proof).hexdigest() != DEMO_PROOF_BLAKE2S
+
+
+NATIVE_FIXTURE_BLAKE2S = "bb4993475432e1fb1adb6e0181f80a48d625069a88182a6c721715b687f6d177"
+
+
+def test_native_opening_and_decide_fixture(oracle):
+    # qafold_short_pc.rs:3605-3647: ShortQaFoldPC<F>::new(2^16, 16, 2, 80),
+    # StdRng::seed_from_u64(0x7061_7261_6c6c_656c); BLAKE2s(root || opening || decide).
+    d = (ctypes.c_uint8 * 32)()
+    ol, dl = ctypes.c_uint64(), ctypes.c_uint64()
+    rc = oracle.orc_native_fixture(ctypes.c_uint64(1 << 16), ctypes.c_uint64(16), ctypes.c_uint64(80),
+                                   ctypes.c_uint64(0x7061_7261_6c6c_656c), d, ctypes.byref(ol), ctypes.byref(dl))
+    assert rc == 0, oracle.orc_last_error()
+    assert bytes(d).hex() == NATIVE_FIXTURE_BLAKE2S
+
+
+def test_multilevel_proof_runs_with_prover_self_checks(oracle):
+    # 2^14-entry blocks give three QA levels and two Decide rounds; every
+    # algebraic terminal check inside the prover must pass.
+    shapes = [(300, 200), (129, 65)]
+    flat, buckets, active = oracle_layout(oracle, shapes)
+    rng = np.random.default_rng(3)
+    mats = [np.clip(np.rint(rng.normal(0, 20, r * c)), -128, 127).astype(np.int8) for r, c in shapes]
+    values = oracle_encode(shapes, buckets, active, mats)
+    proof, _, _ = oracle_prove(oracle, flat, len(shapes), 14, 4, values)
+    assert len(proof) > 10000
