@@ -65,8 +65,8 @@ int qf_params_bucket(const qf_params* p, uint64_t i, uint64_t out[8]);
  * and tree resident in HBM; QF_CACHE_RECOMPUTE keeps roots only. */
 int qf_commit(const qf_params* p, const int8_t* values, size_t len, int cache_mode, qf_committed** out);
 
-/* As qf_commit, with the BVA already resident in device memory (borrowed: it
- * must stay valid while the returned handle lives). */
+/* As qf_commit, with the BVA already resident in device memory (16-byte
+ * aligned; borrowed: it must stay valid while the returned handle lives). */
 int qf_commit_device(const qf_params* p, const int8_t* device_values, size_t len, int cache_mode,
                      qf_committed** out);
 void qf_committed_free(qf_committed* c);
@@ -81,6 +81,39 @@ int qf_committed_codeword(const qf_committed* c, uint64_t block, void* out, size
 /* Device time in milliseconds of the kernels of the last qf_commit*, and the
  * mean duration of its leaf-hash kernel (HIP events on the commit stream). */
 int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms, double* encode_ms);
+
+typedef struct qf_proof qf_proof;
+
+/* Size in bytes of a packed claim list for these parameters: for every matrix
+ * in order, its row point (log2(next_pow2(rows)) Ext8 elements), its column
+ * point (log2(next_pow2(cols)) elements) and its value, each element 16 bytes
+ * of little-endian u16 coefficients (tbv.rs:6-11, Claim; layout of the
+ * reference's serde bytes for Scalar). */
+int qf_claims_size(const qf_params* p, size_t* bytes);
+
+/* The benchmark statement of the reference CLI (main.rs:188-209): points drawn
+ * from the transcript over the commitment statement, values evaluated on the
+ * committed BVA (tbv.rs:143-185, evaluate_claims). Writes qf_claims_size bytes. */
+int qf_benchmark_claims(const qf_params* p, const qf_committed* c, uint8_t* claims, size_t cap);
+
+/* Parameters::prove (protocol.rs:201-263): TBV compile, packed even/odd
+ * reduction, PCS opening (batched over blocks, or single-block) and Decide,
+ * returning the bincode Proof (protocol.rs:57-95, Proof::to_bytes). The
+ * commitment is not consumed. Invalid claims (wrong dimensions, non-canonical
+ * elements, false values) fail with the reference's error kinds. */
+int qf_prove(const qf_params* p, const qf_committed* c, const uint8_t* claims, size_t claims_len, qf_proof** out);
+/* Proof bytes, valid while the handle lives. */
+int qf_proof_bytes(const qf_proof* pr, const uint8_t** data, size_t* len);
+/* Host wall-clock phases of the proof in milliseconds (ProverProfile,
+ * protocol.rs:68-80): conversion, packed_eo, pcs_projection, folding,
+ * sumcheck, binding, native_opening, decide, total. */
+int qf_proof_timing(const qf_proof* pr, double out[9]);
+void qf_proof_free(qf_proof* pr);
+
+/* The public statement as the reference serialises it (serde_json of
+ * Commitment, protocol.rs:33-40; commitment.json of main.rs). NUL-terminated;
+ * *len receives the length without the NUL. */
+int qf_statement_json(const qf_params* p, const qf_committed* c, char* out, size_t cap, size_t* len);
 
 /* Device memory for a resident BVA (no reference counterpart: the reference
  * keeps the BVA in host RAM). qf_device_alloc returns zeroed memory. */

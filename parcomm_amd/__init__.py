@@ -69,6 +69,14 @@ def lib():
         L.qf_device_download.argtypes = [vp, vp, sz]
         L.qf_device_free.argtypes = [vp]
         L.qf_synthetic_bva.argtypes = [vp, u64, ctypes.c_float, vp]
+        L.qf_claims_size.argtypes = [vp, P(sz)]
+        L.qf_benchmark_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
+        L.qf_prove.argtypes = [vp, vp, ctypes.c_char_p, sz, P(vp)]
+        L.qf_proof_bytes.argtypes = [vp, P(P(ctypes.c_uint8)), P(sz)]
+        L.qf_proof_timing.argtypes = [vp, P(ctypes.c_double)]
+        L.qf_proof_free.argtypes = [vp]
+        L.qf_proof_free.restype = None
+        L.qf_statement_json.argtypes = [vp, vp, ctypes.c_char_p, sz, P(sz)]
         _lib = L
     return _lib
 
@@ -137,6 +145,19 @@ class Commitment:
     roots: List[bytes] = field(default_factory=list)
 
 
+@dataclass
+class Claim:
+    """tbv.rs:6-11: evaluation claim of one matrix. Scalars are 16-byte
+    canonical Ext8 elements (8 little-endian u16 coefficients)."""
+    row: List[bytes]
+    col: List[bytes]
+    value: bytes
+
+
+PROFILE_KEYS = ("conversion", "packed_eo", "pcs_projection", "terminal_batch_folding", "pcs_sumcheck",
+                "terminal_batch_binding", "pcs_native_opening", "pcs_decide", "total")
+
+
 class Committed:
     """Device-resident commitment state (protocol.rs:46-50)."""
 
@@ -151,6 +172,14 @@ class Committed:
         out = np.empty((32, n_rows, 8), dtype=np.uint16)
         _check(lib().qf_committed_codeword(self._h, block, out.ctypes.data, out.nbytes))
         return out
+
+    def statement_json(self) -> str:
+        """serde_json of the Commitment statement (commitment.json of the CLI)."""
+        n = ctypes.c_size_t()
+        lib().qf_statement_json(self._params._h, self._h, None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _check(lib().qf_statement_json(self._params._h, self._h, buf, n.value + 1, ctypes.byref(n)))
+        return buf.value.decode()
 
     def timing(self):
         t, l, e = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
@@ -223,6 +252,54 @@ class Parameters:
     def commit_recomputed(self, values):
         """Parameters::commit_recomputed (protocol.rs:161-185): roots only."""
         return self._commit(values, QF_CACHE_RECOMPUTE)
+
+    def _claim_dims(self):
+        return [((s.rows - 1).bit_length(), (s.cols - 1).bit_length()) for s in self.shapes]
+
+    def claims_to_bytes(self, claims: Sequence[Claim]) -> bytes:
+        out = bytearray()
+        for c in claims:
+            for v in list(c.row) + list(c.col) + [c.value]:
+                if len(v) != 16:
+                    raise ValueError("scalars are 16 bytes")
+                out += v
+        return bytes(out)
+
+    def claims_from_bytes(self, raw: bytes) -> List[Claim]:
+        out, o = [], 0
+        for r, c in self._claim_dims():
+            row = [raw[o + 16 * i:o + 16 * i + 16] for i in range(r)]
+            o += 16 * r
+            col = [raw[o + 16 * i:o + 16 * i + 16] for i in range(c)]
+            o += 16 * c
+            out.append(Claim(row, col, raw[o:o + 16]))
+            o += 16
+        return out
+
+    def benchmark_claims(self, committed: "Committed") -> List[Claim]:
+        """The CLI's benchmark statement (main.rs:188-209)."""
+        n = ctypes.c_size_t()
+        _check(lib().qf_claims_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().qf_benchmark_claims(self._h, committed._h, buf, n.value))
+        return self.claims_from_bytes(buf.raw)
+
+    def prove(self, committed: "Committed", claims: Sequence[Claim]):
+        """Parameters::prove (protocol.rs:201-263): (proof bytes, profile in seconds)."""
+        raw = self.claims_to_bytes(claims)
+        h = ctypes.c_void_p()
+        _check(lib().qf_prove(self._h, committed._h, raw, len(raw), ctypes.byref(h)))
+        try:
+            data = ctypes.POINTER(ctypes.c_uint8)()
+            n = ctypes.c_size_t()
+            _check(lib().qf_proof_bytes(h, ctypes.byref(data), ctypes.byref(n)))
+            proof = ctypes.string_at(data, n.value)
+            t = (ctypes.c_double * 9)()
+            _check(lib().qf_proof_timing(h, t))
+            profile = {k + "_seconds": t[i] / 1000.0 for i, k in enumerate(PROFILE_KEYS)}
+        finally:
+            lib().qf_proof_free(h)
+        return proof, profile
 
     def synthetic_bva(self, seed, sigma=20.0):
         """Device-resident synthetic BVA: clip(round(N(0, sigma))) on physical entries."""

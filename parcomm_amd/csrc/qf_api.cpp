@@ -1,34 +1,15 @@
 // C ABI of the product library (declared in include/parcomm_amd.h).
 #include "../../include/parcomm_amd.h"
-#include "qf_kernels.hpp"
+#include "qf_state.hpp"
 #include <memory>
 
 using namespace qf;
 
-struct qf_params {
-  Layout layout;
-  uint32_t cfg_chunk_log = 30, packed_log = 4, query_bits = 100;
-  int chunk_log = 0;
-  uint64_t block_len = 0, blocks = 0;
-  PcsParams pp;
-  CommitGeom geom{};
-  mutable E8* d_alpha = nullptr;  // top-code alpha_1 on the device, uploaded on first commit
-  mutable std::vector<uint64_t> zero_ranges;  // padding intervals (start, len) chunked for the device check
-  ~qf_params() { if (d_alpha) (void)hipFree(d_alpha); }
-};
-
-struct qf_committed {
-  const qf_params* p = nullptr;
-  int cache_mode = 0;
-  std::vector<Digest> roots;
-  std::vector<E8*> d_cw;         // per block (cached) or one workspace
-  std::vector<uint32_t*> d_tree;
-  double total_ms = 0, leaf_ms = 0, encode_ms = 0;
-  ~qf_committed() {
-    std::vector<void*> freed;
-    for (auto* q : d_cw) if (q && std::find(freed.begin(), freed.end(), (void*)q) == freed.end()) { freed.push_back(q); (void)hipFree(q); }
-    for (auto* q : d_tree) if (q && std::find(freed.begin(), freed.end(), (void*)q) == freed.end()) { freed.push_back(q); (void)hipFree(q); }
-  }
+struct qf_params : qf::ParamsState {};
+struct qf_committed : qf::CommittedState {};
+struct qf_proof {
+  qf::Bytes bytes;
+  qf::ProveTimings t;
 };
 
 namespace {
@@ -77,7 +58,7 @@ __global__ void k_check_zero(const int8_t* bva, const uint64_t* ranges, uint32_t
   if (bad) atomicOr(flag, 1u);
 }
 
-void validate_device(const qf_params* p, const int8_t* d_bva, hipStream_t st) {
+void validate_device(const ParamsState* p, const int8_t* d_bva, hipStream_t st) {
   if (p->zero_ranges.empty()) p->zero_ranges = padding_ranges(p->layout);
   if (p->zero_ranges.empty()) return;
   size_t count = p->zero_ranges.size() / 2;
@@ -108,14 +89,11 @@ void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_
   try {
     c->p = p;
     c->cache_mode = cache_mode;
+    c->d_bva = d_bva;
     validate_device(p, d_bva, st);
     const uint64_t n = uint64_t(1) << p->geom.log_n, R = 2 * n;
     const size_t cw_bytes = size_t(32 * R) * 16, tree_bytes = size_t(2 * R - 1) * 32;
-    if (!p->d_alpha) {
-      const auto& a = p->pp.codes.back().alpha1;
-      QF_HIP(hipMalloc(&p->d_alpha, a.size() * 16));
-      QF_HIP(hipMemcpy(p->d_alpha, a.data(), a.size() * 16, hipMemcpyHostToDevice));
-    }
+    const E8* alpha = p->alpha_device(p->pp.codes.size() - 1);
     hipEvent_t e0, e1, ev[4];
     QF_HIP(hipEventCreate(&e0)); QF_HIP(hipEventCreate(&e1));
     for (auto& e : ev) QF_HIP(hipEventCreate(&e));
@@ -139,7 +117,7 @@ void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_
         cw = c->d_cw[0];
         tree = c->d_tree[0];
       }
-      launch_commit_block(p->geom, d_bva, first, len, cw, p->d_alpha, tree, st, ev);
+      launch_commit_block(p->geom, d_bva, first, len, cw, alpha, tree, st, ev);
       QF_HIP(hipGetLastError());
       Digest root;
       QF_HIP(hipMemcpyAsync(root.data(), tree + 8 * (2 * R - 2), 32, hipMemcpyDeviceToHost, st));
@@ -165,6 +143,27 @@ void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_
   }
   QF_HIP(hipStreamDestroy(st));
   *out = c.release();
+}
+
+std::vector<std::pair<size_t, size_t>> claim_dims(const ParamsState& p) {
+  std::vector<std::pair<size_t, size_t>> d;
+  for (auto& s : p.layout.shapes) {
+    uint64_t rp = 1, cp = 1;
+    while (rp < s.rows) rp <<= 1;
+    while (cp < s.cols) cp <<= 1;
+    d.push_back({size_t(ilog2u(rp)), size_t(ilog2u(cp))});
+  }
+  return d;
+}
+size_t claims_bytes(const ParamsState& p) {
+  size_t n = 0;
+  for (auto [r, c] : claim_dims(p)) n += 16 * (r + c + 1);
+  return n;
+}
+E8 read_scalar(const uint8_t* b) {
+  E8 v;
+  require(e8_from_bytes(b, v), QF_ERR_ENCODING, "non-canonical field element in claims");
+  return v;
 }
 }  // namespace
 
@@ -197,6 +196,7 @@ int qf_params_new(int field, const uint64_t* shapes, size_t n, uint32_t chunk_lo
     p->geom.log_n = p->chunk_log - 8;
     p->geom.leaf = leaf_prefix(32);
     p->geom.inner = inner_prefix();
+    p->leaf_ext = leaf_prefix(32, "qafold-short/extension-row/v1");
     *out = p.release();
   });
 }
@@ -240,13 +240,14 @@ int qf_commit(const qf_params* p, const int8_t* values, size_t len, int cache_mo
       (void)hipFree(d);
       throw;
     }
-    (void)hipFree(d);
+    (*out)->owns_bva = true;  // the opening reads the BVA again
   });
 }
 
 int qf_commit_device(const qf_params* p, const int8_t* device_values, size_t len, int cache_mode, qf_committed** out) {
   return guarded([&] {
     require(p && out && device_values, QF_ERR_ARGUMENT, "null argument");
+    require(reinterpret_cast<uintptr_t>(device_values) % 16 == 0, QF_ERR_ARGUMENT, "device BVA must be 16-byte aligned");
     commit_impl(p, device_values, len, cache_mode, out);
   });
 }
@@ -275,6 +276,74 @@ int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms
     if (total_ms) *total_ms = c->total_ms;
     if (leaf_ms) *leaf_ms = c->leaf_ms;
     if (encode_ms) *encode_ms = c->encode_ms;
+  });
+}
+
+int qf_claims_size(const qf_params* p, size_t* bytes) {
+  return guarded([&] { require(p && bytes, QF_ERR_ARGUMENT, "null argument"); *bytes = claims_bytes(*p); });
+}
+
+int qf_benchmark_claims(const qf_params* p, const qf_committed* c, uint8_t* claims, size_t cap) {
+  return guarded([&] {
+    require(p && c && claims && c->p == p, QF_ERR_ARGUMENT, "null or mismatched argument");
+    require(cap >= claims_bytes(*p), QF_ERR_ARGUMENT, "claims buffer too small");
+    auto cl = benchmark_claims(*p, *c);
+    uint8_t* o = claims;
+    for (auto& x : cl) {
+      for (auto& v : x.row) { std::memcpy(o, &v, 16); o += 16; }
+      for (auto& v : x.col) { std::memcpy(o, &v, 16); o += 16; }
+      std::memcpy(o, &x.value, 16);
+      o += 16;
+    }
+  });
+}
+
+int qf_prove(const qf_params* p, const qf_committed* c, const uint8_t* claims, size_t claims_len, qf_proof** out) {
+  return guarded([&] {
+    require(p && c && claims && out && c->p == p, QF_ERR_ARGUMENT, "null or mismatched argument");
+    require(claims_len == claims_bytes(*p), QF_ERR_INVALID_PARAMETERS, "claims length mismatch");
+    std::vector<ClaimIn> cl;
+    const uint8_t* q = claims;
+    for (auto [r, cc] : claim_dims(*p)) {
+      ClaimIn x;
+      for (size_t i = 0; i < r; i++, q += 16) x.row.push_back(read_scalar(q));
+      for (size_t i = 0; i < cc; i++, q += 16) x.col.push_back(read_scalar(q));
+      x.value = read_scalar(q);
+      q += 16;
+      cl.push_back(std::move(x));
+    }
+    std::unique_ptr<qf_proof> pr(new qf_proof);
+    pr->bytes = prove(*p, *c, cl, pr->t);
+    *out = pr.release();
+  });
+}
+
+int qf_proof_bytes(const qf_proof* pr, const uint8_t** data, size_t* len) {
+  return guarded([&] {
+    require(pr && data && len, QF_ERR_ARGUMENT, "null argument");
+    *data = pr->bytes.data();
+    *len = pr->bytes.size();
+  });
+}
+
+int qf_proof_timing(const qf_proof* pr, double out[9]) {
+  return guarded([&] {
+    require(pr && out, QF_ERR_ARGUMENT, "null argument");
+    const ProveTimings& t = pr->t;
+    double v[9] = {t.conversion_ms, t.eo_ms, t.projection_ms, t.folding_ms, t.sumcheck_ms, t.binding_ms, t.native_ms, t.decide_ms, t.total_ms};
+    std::memcpy(out, v, sizeof v);
+  });
+}
+
+void qf_proof_free(qf_proof* pr) { delete pr; }
+
+int qf_statement_json(const qf_params* p, const qf_committed* c, char* out, size_t cap, size_t* len) {
+  return guarded([&] {
+    require(p && c && len && c->p == p, QF_ERR_ARGUMENT, "null or mismatched argument");
+    std::string s = statement_json(*p, *c);
+    *len = s.size();
+    require(out && cap > s.size(), QF_ERR_ARGUMENT, "statement buffer too small");
+    std::memcpy(out, s.c_str(), s.size() + 1);
   });
 }
 

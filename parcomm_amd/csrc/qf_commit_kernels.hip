@@ -152,10 +152,14 @@ __global__ __launch_bounds__(WG) void k_mobius_columns(E8* __restrict__ cw, uint
   for (int k = 0; k < 32; k++) base[k * cstride] = lo[k] | (hi[k] << 16);
 }
 
-// Leaf hash of one codeword row: BLAKE2s(prefix75 || row bytes), 587 bytes,
-// 10 compressions of which the first (all prefix) is precomputed on the host.
+// Leaf hash of one 32-column codeword row: BLAKE2s(prefix || row bytes), with
+// the framed domain + width prefix of PLEN bytes (75 for base rows, 80 for
+// extension rows). The first 64 prefix bytes are compressed on the host.
+template <int PLEN>
 __global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uint64_t rows, LeafPrefix lp,
                                                   uint32_t* __restrict__ leaves) {
+  constexpr int TOTAL = PLEN + 512;
+  constexpr int BLOCKS = (TOTAL + 63) / 64;
   uint64_t r = uint64_t(blockIdx.x) * WG + threadIdx.x;
   if (r >= rows) return;
   uint32_t R[129];
@@ -169,20 +173,26 @@ __global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uin
 #pragma unroll
   for (int i = 0; i < 8; i++) h[i] = lp.h[i];
 #pragma unroll
-  for (int b = 1; b < 10; b++) {
+  for (int b = 1; b < BLOCKS; b++) {
     uint32_t m[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const int w = 16 * b + i;
       uint32_t x;
-      if (w == 16) x = lp.tail[0];
-      else if (w == 17) x = lp.tail[1];
-      else if (w == 18) x = lp.tail[2] | (R[0] << 24);
-      else if (w <= 146) x = (R[w - 19] >> 8) | (R[w - 18] << 24);
-      else x = 0;
+      if (4 * w + 4 <= PLEN) {
+        x = lp.tail[w - 16];
+      } else if (4 * w < PLEN) {
+        x = lp.tail[w - 16] | (R[0] << (8 * (PLEN - 4 * w)));
+      } else if (4 * w >= TOTAL) {
+        x = 0;
+      } else {
+        const int o = 4 * w - PLEN, q = o >> 2, sh = o & 3;
+        const uint32_t lo = q < 129 ? R[q] : 0u, hi = q + 1 < 129 ? R[q + 1] : 0u;
+        x = sh == 0 ? lo : (lo >> (8 * sh)) | (hi << (32 - 8 * sh));
+      }
       m[i] = x;
     }
-    b2s_compress(h, m, b == 9 ? 587u : uint32_t(64 * (b + 1)), b == 9);
+    b2s_compress(h, m, b == BLOCKS - 1 ? uint32_t(TOTAL) : uint32_t(64 * (b + 1)), b == BLOCKS - 1);
   }
   uint4* o = reinterpret_cast<uint4*>(leaves + 8 * r);
   o[0] = make_uint4(h[0], h[1], h[2], h[3]);
@@ -250,6 +260,90 @@ static std::vector<std::pair<int, int>> bit_groups(int L) {
   return gs;
 }
 
+// Copies a flat packed vector (column k = entries [k n, (k+1) n)) into the
+// message halves of a 32-column codeword.
+__global__ __launch_bounds__(WG) void k_scatter_message(const E8* __restrict__ values, uint64_t n, uint64_t rows,
+                                                        E8* __restrict__ cw) {
+  uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (i >= 32 * n) return;
+  cw[(i / n) * rows + (i % n)] = values[i];
+}
+
+// One-row messages: the WHT of length one is the identity, so parity = alpha_1[0] * m.
+__global__ void k_encode_one_row(E8* cw, uint64_t cols, const E8* alpha) {
+  uint64_t c = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (c < cols) cw[2 * c + 1] = v8_store(v8_mul(v8_load(cw[2 * c]), v8_load(alpha[0])));
+}
+
+void launch_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream_t st) {
+  if (L == 0) return;
+  for (auto& gp : bit_groups(L)) launch_pass_geom(v, v, vectors, stride, stride, L, gp.first, gp.second, OP_WHT, nullptr, st);
+}
+
+void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStream_t st) {
+  const uint64_t n = uint64_t(1) << L, R = 2 * n;
+  if (L == 0) {
+    hipLaunchKernelGGL(k_encode_one_row, dim3(uint32_t((cols + WG - 1) / WG)), dim3(WG), 0, st, cw, cols, alpha);
+    return;
+  }
+  // parity = WHT(alpha . WHT(m)): low bits copy m into the parity half, the
+  // high bits run forward, multiply, forward again, then the low bits again.
+  auto groups = bit_groups(L);
+  if (groups.size() == 1) {
+    launch_pass_geom(cw, cw + n, cols, R, R, L, 0, groups[0].second, OP_WHT, alpha, st);
+    return;
+  }
+  launch_pass_geom(cw, cw + n, cols, R, R, L, 0, groups[0].second, OP_WHT, nullptr, st);
+  for (size_t i = 1; i + 1 < groups.size(); i++)
+    launch_pass_geom(cw + n, cw + n, cols, R, R, L, groups[i].first, groups[i].second, OP_WHT, nullptr, st);
+  launch_pass_geom(cw + n, cw + n, cols, R, R, L, groups.back().first, groups.back().second, OP_WHT, alpha, st);
+  for (size_t i = groups.size() - 1; i-- > 1;)
+    launch_pass_geom(cw + n, cw + n, cols, R, R, L, groups[i].first, groups[i].second, OP_WHT, nullptr, st);
+  launch_pass_geom(cw + n, cw + n, cols, R, R, L, 0, groups[0].second, OP_WHT, nullptr, st);
+}
+
+void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
+                 hipStream_t st) {
+  if (leaf.prefix_len == 75)
+    hipLaunchKernelGGL(k_leaf_hash<75>, dim3(uint32_t((R + WG - 1) / WG)), dim3(WG), 0, st, cw, R, leaf, tree);
+  else if (leaf.prefix_len == 80)
+    hipLaunchKernelGGL(k_leaf_hash<80>, dim3(uint32_t((R + WG - 1) / WG)), dim3(WG), 0, st, cw, R, leaf, tree);
+  else
+    throw QfError(QF_ERR_ARGUMENT, "unsupported leaf prefix");
+  uint32_t* level = tree;
+  for (uint64_t cnt = R; cnt > 1; cnt /= 2) {
+    uint32_t* next = level + 8 * cnt;
+    hipLaunchKernelGGL(k_inner_level, dim3(uint32_t((cnt / 2 + WG - 1) / WG)), dim3(WG), 0, st, level, cnt / 2, inner,
+                       next);
+    level = next;
+  }
+}
+
+void launch_mobius_message(E8* cw, int L, hipStream_t st) {
+  const uint64_t n = uint64_t(1) << L, R = 2 * n;
+  if (L > 0)
+    for (auto& gp : bit_groups(L)) launch_pass_geom(cw, cw, 32, R, R, L, gp.first, gp.second, OP_MOBIUS, nullptr, st);
+  hipLaunchKernelGGL(k_mobius_columns, dim3(uint32_t((n * 4 + WG - 1) / WG)), dim3(WG), 0, st, cw, n, R);
+}
+
+void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E8* alpha, uint32_t* tree,
+                          hipStream_t st) {
+  const int L = g.log_n;
+  const uint64_t n = uint64_t(1) << L, R = 2 * n;
+  hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
+  launch_mobius_message(cw, L, st);
+  launch_encode_parity(cw, 32, L, alpha, st);
+  launch_tree(cw, R, g.leaf, g.inner, tree, st);
+}
+
+void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, const LeafPrefix& leaf,
+                             const InnerPrefix& inner, uint32_t* tree, hipStream_t st) {
+  const uint64_t n = uint64_t(1) << L, R = 2 * n;
+  hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
+  launch_encode_parity(cw, 32, L, alpha, st);
+  launch_tree(cw, R, leaf, inner, tree, st);
+}
+
 void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_first, uint64_t available, E8* cw,
                          const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev) {
   const int L = g.log_n;
@@ -265,31 +359,12 @@ void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_
     if (gp.first > 0) launch_pass_geom(cw, cw, 32, R, R, L, gp.first, gp.second, OP_MOBIUS, nullptr, st);
   // 3. Mobius over the column bits
   hipLaunchKernelGGL(k_mobius_columns, dim3(uint32_t((n * 4 + WG - 1) / WG)), dim3(WG), 0, st, cw, n, R);
-  // 4. parity = WHT(alpha . WHT(m)): low bits copy m into the parity half, the
-  //    high bits run forward, multiply, forward again, then the low bits again.
-  auto groups = bit_groups(L);
-  if (groups.size() == 1) {
-    launch_pass_geom(cw, cw + n, 32, R, R, L, 0, groups[0].second, OP_WHT, alpha, st);
-  } else {
-    launch_pass_geom(cw, cw + n, 32, R, R, L, 0, groups[0].second, OP_WHT, nullptr, st);
-    for (size_t i = 1; i + 1 < groups.size(); i++)
-      launch_pass_geom(cw + n, cw + n, 32, R, R, L, groups[i].first, groups[i].second, OP_WHT, nullptr, st);
-    launch_pass_geom(cw + n, cw + n, 32, R, R, L, groups.back().first, groups.back().second, OP_WHT, alpha, st);
-    for (size_t i = groups.size() - 1; i-- > 1;)
-      launch_pass_geom(cw + n, cw + n, 32, R, R, L, groups[i].first, groups[i].second, OP_WHT, nullptr, st);
-    launch_pass_geom(cw + n, cw + n, 32, R, R, L, 0, groups[0].second, OP_WHT, nullptr, st);
-  }
+  // 4. parity
+  launch_encode_parity(cw, 32, L, alpha, st);
   // 5. leaves and inner levels; tree level l occupies (R >> l) digests after the previous levels.
   if (ev) QF_HIP(hipEventRecord(ev[1], st));
-  hipLaunchKernelGGL(k_leaf_hash, dim3(uint32_t((R + WG - 1) / WG)), dim3(WG), 0, st, cw, R, g.leaf, tree);
+  launch_tree(cw, R, g.leaf, g.inner, tree, st);
   if (ev) QF_HIP(hipEventRecord(ev[2], st));
-  uint32_t* level = tree;
-  for (uint64_t cnt = R; cnt > 1; cnt /= 2) {
-    uint32_t* next = level + 8 * cnt;
-    hipLaunchKernelGGL(k_inner_level, dim3(uint32_t((cnt / 2 + WG - 1) / WG)), dim3(WG), 0, st, level, cnt / 2, g.inner,
-                       next);
-    level = next;
-  }
   if (ev) QF_HIP(hipEventRecord(ev[3], st));
 }
 

@@ -84,6 +84,14 @@ inline E8 e8_inv(const E8& a) {
   E8 norm = e8_mul(a, prod);
   return e8_scale(prod, fp_inv(norm.c[0]));
 }
+inline E8 operator+(const E8& a, const E8& b) { return e8_add(a, b); }
+inline E8 operator-(const E8& a, const E8& b) { return e8_sub(a, b); }
+inline E8 operator-(const E8& a) { return e8_neg(a); }
+inline E8 operator*(const E8& a, const E8& b) { return e8_mul(a, b); }
+inline E8& operator+=(E8& a, const E8& b) { a = e8_add(a, b); return a; }
+inline bool operator==(const E8& a, const E8& b) { return e8_eq(a, b); }
+inline bool operator!=(const E8& a, const E8& b) { return !e8_eq(a, b); }
+inline E8 e8_basis(int j) { E8 b{}; b.c[j] = 1; return b; }
 inline bool e8_from_bytes(const uint8_t* b, E8& out) {
   for (int i = 0; i < 8; i++) { uint32_t v = b[2 * i] | uint32_t(b[2 * i + 1]) << 8; if (v >= P) return false; out.c[i] = uint16_t(v); }
   return true;

@@ -23,4 +23,21 @@ void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_
                          const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev = nullptr);
 // ev (optional, 4 events): recorded at start, before the leaf kernel, after it, and at the end.
 
+// Building blocks shared with the opening (all asynchronous on `st`):
+// WHT over the L low index bits of `vectors` vectors spaced `stride` apart.
+void launch_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream_t st);
+// Parity halves of `cols` columns (R = 2^(L+1) rows each) from their message halves.
+void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStream_t st);
+// Leaves (prefix chosen by `leaf`) and all inner levels of a 32-column codeword.
+void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
+                 hipStream_t st);
+// Mobius transform of the 32 x 2^L message halves (all 5 + L index bits).
+void launch_mobius_message(E8* cw, int L, hipStream_t st);
+// Commit of a flat packed vector of 32 * 2^L elements (the folded block of a batched opening).
+// Commit of an extension matrix: flat 32 x 2^L message, no Mobius, extension leaves.
+void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, const LeafPrefix& leaf,
+                             const InnerPrefix& inner, uint32_t* tree, hipStream_t st);
+void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E8* alpha, uint32_t* tree,
+                          hipStream_t st);
+
 }  // namespace qf

@@ -297,27 +297,26 @@ PcsParams make_pcs_params(int log_n, uint64_t security_bits, const uint8_t seed3
   return pp;
 }
 
-LeafPrefix leaf_prefix(uint64_t width) {
+LeafPrefix leaf_prefix(uint64_t width, const char* dom) {
   Bytes pre;
   auto frame = [&](const char* label, const void* b, size_t n) {
     size_t l = std::strlen(label);
     put_u64(pre, l); pre.insert(pre.end(), label, label + l);
     put_u64(pre, n); pre.insert(pre.end(), (const uint8_t*)b, (const uint8_t*)b + n);
   };
-  const char* dom = "qafold-short/base-row/v1";
   frame("domain", dom, std::strlen(dom));
   uint8_t w[8]; for (int i = 0; i < 8; i++) w[i] = uint8_t(width >> (8 * i));
   frame("width", w, 8);
-  require(pre.size() == 75, QF_ERR_ENCODING, "unexpected leaf prefix length");
+  require(pre.size() == 75 || pre.size() == 80, QF_ERR_ENCODING, "unexpected leaf prefix length");
   LeafPrefix lp{};
+  lp.prefix_len = uint32_t(pre.size());
   b2s_init(lp.h);
   uint32_t m[16];
   for (int i = 0; i < 16; i++) m[i] = pre[4 * i] | uint32_t(pre[4 * i + 1]) << 8 | uint32_t(pre[4 * i + 2]) << 16 | uint32_t(pre[4 * i + 3]) << 24;
   b2s_compress(lp.h, m, 64, false);
-  lp.tail[0] = pre[64] | uint32_t(pre[65]) << 8 | uint32_t(pre[66]) << 16 | uint32_t(pre[67]) << 24;
-  lp.tail[1] = pre[68] | uint32_t(pre[69]) << 8 | uint32_t(pre[70]) << 16 | uint32_t(pre[71]) << 24;
-  lp.tail[2] = pre[72] | uint32_t(pre[73]) << 8 | uint32_t(pre[74]) << 16;
-  lp.prefix_len = 75;
+  pre.resize(80, 0);
+  for (int i = 0; i < 4; i++)
+    lp.tail[i] = pre[64 + 4 * i] | uint32_t(pre[65 + 4 * i]) << 8 | uint32_t(pre[66 + 4 * i]) << 16 | uint32_t(pre[67 + 4 * i]) << 24;
   return lp;
 }
 

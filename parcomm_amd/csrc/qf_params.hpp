@@ -54,10 +54,11 @@ void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)>& bod
 // Prefix state of the base-row leaf hash: framed domain + width (qafold_short_pc.rs:2437-2475).
 struct LeafPrefix {
   uint32_t h[8];        // chaining value after the first 64 prefix bytes
-  uint32_t tail[3];     // prefix bytes 64..74 as words 16,17 and the 3 bytes of word 18
-  uint32_t prefix_len;  // 75
+  uint32_t tail[4];     // prefix bytes 64.. as little-endian words (last one partial)
+  uint32_t prefix_len;  // 75 (base rows) or 80 (extension rows)
 };
-LeafPrefix leaf_prefix(uint64_t width);
+// domain: "qafold-short/base-row/v1" or "qafold-short/extension-row/v1"
+LeafPrefix leaf_prefix(uint64_t width, const char* domain = "qafold-short/base-row/v1");
 struct InnerPrefix { uint32_t w[13]; };  // bytes 0..49 of the inner-node message, word 12 partial
 InnerPrefix inner_prefix();
 

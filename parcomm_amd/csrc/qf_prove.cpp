@@ -1,0 +1,1486 @@
+// The opening prover: Parameters::prove (protocol.rs:201-263) over a
+// device-resident commitment. Fiat-Shamir transcripts are replayed on the
+// host; every vector larger than a handful of elements lives in HBM and is
+// processed by the kernels of qf_vec.hip and qf_commit_kernels.hip.
+#include "qf_state.hpp"
+#include "qf_vec.hpp"
+#include <chrono>
+#include <cmath>
+#include <map>
+
+namespace qf {
+
+const E8* ParamsState::alpha_device(size_t level) const {
+  if (d_alpha_levels.size() != pp.codes.size()) d_alpha_levels.assign(pp.codes.size(), nullptr);
+  if (!d_alpha_levels[level]) {
+    const auto& a = pp.codes[level].alpha1;
+    E8* d = nullptr;
+    QF_HIP(hipMalloc(&d, a.size() * 16));
+    QF_HIP(hipMemcpy(d, a.data(), a.size() * 16, hipMemcpyHostToDevice));
+    d_alpha_levels[level] = d;
+    if (level + 1 == pp.codes.size()) d_alpha = d;
+  }
+  return d_alpha_levels[level];
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+
+// ================================================================ statement JSON (serde_json compact)
+void js_bytes(std::string& s, const uint8_t* b, size_t n) {
+  s += '[';
+  for (size_t i = 0; i < n; i++) { if (i) s += ','; s += std::to_string(unsigned(b[i])); }
+  s += ']';
+}
+void js_e8(std::string& s, const E8& v) { js_bytes(s, reinterpret_cast<const uint8_t*>(&v), 16); }
+std::string js_scalars(const std::vector<E8>& v) {
+  std::string s = "[";
+  for (size_t i = 0; i < v.size(); i++) { if (i) s += ','; js_e8(s, v[i]); }
+  return s + "]";
+}
+
+// ================================================================ transcripts
+struct Outer {  // transcript.rs:8-70
+  Digest state;
+  uint64_t counter = 0;
+  Outer() { state = blake2s("parcomm-qafold/public/v1", 24); }
+  void bytes(const char* label, const void* d, size_t n) {
+    Hasher h;
+    h.update(state.data(), 32);
+    h.u64le(std::strlen(label));
+    h.update(label, std::strlen(label));
+    h.u64le(n);
+    h.update(d, n);
+    state = h.finish();
+  }
+  void object(const char* label, const std::string& js) { bytes(label, js.data(), js.size()); }
+  void scalar(const char* label, const E8& v) { bytes(label, &v, 16); }
+  E8 challenge(const char* label) {
+    for (;;) {
+      uint64_t ct = counter++;
+      uint8_t b[8];
+      for (int i = 0; i < 8; i++) b[i] = uint8_t(ct >> (8 * i));
+      bytes(label, b, 8);
+      E8 v;
+      if (e8_from_bytes(state.data(), v)) return v;
+    }
+  }
+  E8 nonzero(const char* label) { for (;;) { E8 v = challenge(label); if (!e8_is_zero(v)) return v; } }
+  E8 fold_challenge(uint64_t packed_log) {
+    E8 m1 = e8_neg(e8_one());
+    for (;;) {
+      E8 v = nonzero("eo-fold"), p = v;
+      bool ok = true;
+      for (uint64_t i = 0; i < std::max<uint64_t>(packed_log, 1); i++) { ok = ok && p != m1; p = p * p; }
+      if (ok) return v;
+    }
+  }
+};
+
+struct Pcs {  // qafold_short_pc.rs:2011-2229
+  Hasher h;
+  uint64_t counter = 0;
+  std::vector<E8> responses;
+  Pcs() { bytes("domain", "brakefold/qafold-short-pcs/v2"); }
+  void bytes(const char* label, const void* b, size_t n) { h.frame(label, b, n); }
+  void bytes(const char* label, const char* s) { h.frame(label, s); }
+  void u64(const char* label, uint64_t v) { h.frame_u64(label, v); }
+  void digest(const char* label, const Digest& d) { bytes(label, d.data(), 32); }
+  void ext(const char* label, const E8& v) { bytes(label, &v, 16); }
+  void exts(const char* label, const std::vector<E8>& v) {
+    bytes("vector-label", label);
+    u64("vector-length", v.size());
+    for (auto& x : v) ext("vector-element", x);
+  }
+  void respond(const E8& v) { ext("response", v); responses.push_back(v); }
+  Digest derive(const char* label, uint64_t ctr, uint64_t attempt) const {
+    Hasher c = h;
+    c.frame("challenge-label", label);
+    c.u64le(ctr);
+    c.u64le(0);
+    c.u64le(attempt);
+    return c.finish();
+  }
+  Digest challenge_digest(const char* label) {
+    Digest d = derive(label, counter, 0);
+    counter++;
+    bytes("digest-challenge-label", label);
+    digest("digest-challenge", d);
+    return d;
+  }
+  E8 challenge(const char* label, bool nonzero = false) {
+    uint64_t ctr = counter;
+    for (uint64_t outer = 0;; outer++) {
+      E8 v;
+      for (uint64_t a = outer;; a++) if (e8_from_bytes(derive(label, ctr, a).data(), v)) break;
+      if (!nonzero || !e8_is_zero(v)) {
+        counter++;
+        bytes("field-challenge-label", label);
+        ext("field-challenge", v);
+        return v;
+      }
+    }
+  }
+};
+
+struct Packed {  // packed.rs:754-779
+  Hasher h;
+  void bytes(const void* b, size_t n) { h.u64le(n); h.update(b, n); }
+  void field(const E8& v) { bytes(&v, 16); }
+  E8 challenge() {
+    for (uint64_t ctr = 0;; ctr++) {
+      Hasher c = h;
+      c.update("challenge", 9);
+      c.u64le(ctr);
+      Digest d = c.finish();
+      E8 v;
+      if (e8_from_bytes(d.data(), v)) { bytes(d.data(), 32); return v; }
+    }
+  }
+};
+
+// ================================================================ small host helpers
+// MSB-first monomial weights: w[i] = prod over set bits b of p[len-1-b] (qafold_short_pc.rs:2846-2873)
+std::vector<E8> tw_host(const E8* p, size_t m) {
+  std::vector<E8> w(size_t(1) << m, E8{});
+  w[0] = e8_one();
+  size_t filled = 1;
+  for (size_t k = m; k-- > 0;) {
+    for (size_t i = 0; i < filled; i++) w[filled + i] = w[i] * p[k];
+    filled *= 2;
+  }
+  return w;
+}
+std::vector<E8> tw_host(const std::vector<E8>& p) { return tw_host(p.data(), p.size()); }
+// MSB-first equality table (packed.rs:525-545)
+std::vector<E8> eq_host(const E8* p, size_t m) {
+  std::vector<E8> t(1, e8_one());
+  for (size_t k = 0; k < m; k++) {
+    std::vector<E8> nx(t.size() * 2);
+    for (size_t i = 0; i < t.size(); i++) { nx[2 * i + 1] = t[i] * p[k]; nx[2 * i] = t[i] - nx[2 * i + 1]; }
+    t.swap(nx);
+  }
+  return t;
+}
+E8 inner_host(const std::vector<E8>& a, const std::vector<E8>& b) {
+  require(a.size() == b.size(), QF_ERR_INVALID_VARIABLES, "inner product length");
+  E8 s{};
+  for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i];
+  return s;
+}
+E8 tensor_inner(const std::vector<E8>& a, const std::vector<E8>& b) {
+  E8 r = e8_one();
+  for (size_t i = 0; i < a.size(); i++) r = r * (e8_one() + a[i] * b[i]);
+  return r;
+}
+E8 weight_at(const std::vector<E8>& p, uint64_t pos) {
+  E8 r = e8_one();
+  for (size_t b = 0; b < p.size(); b++) if ((pos >> b) & 1) r = r * p[p.size() - b - 1];
+  return r;
+}
+// Device tensor weights / equality table of an MSB-first point (tensor of two halves).
+void tw_device(DevCtx& c, E8* out, const std::vector<E8>& p) {
+  size_t m = p.size(), a = m / 2;
+  dv::tensor(c, out, tw_host(p.data(), m - a), tw_host(p.data() + (m - a), a));
+}
+void eq_tables(const std::vector<E8>& p, std::vector<E8>& hi, std::vector<E8>& lo) {
+  size_t m = p.size(), a = m / 2;
+  hi = eq_host(p.data(), m - a);
+  lo = eq_host(p.data() + (m - a), a);
+}
+void put_part(Bytes& w, const Bytes& part) { put_u64(w, part.size()); w.insert(w.end(), part.begin(), part.end()); }
+void put_tag(Bytes& w, const char* tag) { w.insert(w.end(), tag, tag + std::strlen(tag)); }
+void put_digest(Bytes& w, const Digest& d) { w.insert(w.end(), d.begin(), d.end()); }
+Digest download_root(DevCtx& c, const uint32_t* tree, uint64_t R) {
+  Digest d;
+  QF_HIP(hipMemcpyAsync(d.data(), tree + 8 * (2 * R - 2), 32, hipMemcpyDeviceToHost, c.st));
+  c.sync();
+  return d;
+}
+
+// ================================================================ query counts (qafold.rs:75-178, brakefold.rs:4-53)
+bool log1m_exp(double x, double& out) {
+  if (x >= 0.0) return false;
+  out = x < -M_LN2 ? std::log1p(-std::exp(x)) : std::log(-std::expm1(x));
+  return true;
+}
+double log_add_exp(double a, double b) { double m = std::max(a, b); return m + std::log(std::exp(a - m) + std::exp(b - m)); }
+bool epsilon_at(uint64_t n, uint64_t index, double d, double ln_p, double& eps) {
+  if (n == 0 || index <= 1 || !(d > 0.0 && d < 1.0) || !(ln_p > 0.0)) return false;
+  double lpm1 = ln_p + std::log1p(-std::exp(-ln_p));
+  double g = 1.0 - d * lpm1 / ln_p + d * std::log(d) / ln_p + (1.0 - d) * std::log(1.0 - d) / ln_p;
+  eps = g - (1.0 + std::log(double(n)) / ln_p) / double(index);
+  return eps > 0.0;
+}
+bool bound_at(uint64_t n, uint64_t index, double d, double ln_p, double& logf) {
+  double eps;
+  if (!epsilon_at(n, index, d, ln_p, eps)) return false;
+  double lpm1 = ln_p + std::log1p(-std::exp(-ln_p)), lome;
+  if (!log1m_exp(-double(index) * eps * ln_p, lome)) return false;
+  double den = lome + lpm1;
+  double fcl = std::log(double(index) * double(index - 1) * double(n) / 2.0) - 2.0 * ln_p;
+  double ftl = -std::ceil(double(index - 1) / (double(index) * d)) * double(index) * eps * ln_p - den;
+  double scl = std::log(double(index) * double(n)) - ln_p;
+  double stl = -std::ceil(1.0 / d) * double(index) * eps * ln_p - den;
+  logf = std::min(log_add_exp(fcl, ftl), log_add_exp(scl, stl));
+  return true;
+}
+uint64_t num_queries(const QaCode& code, uint64_t bits, uint64_t rows) {
+  unsigned __int128 card = 1;
+  for (int i = 0; i < 8; i++) card *= 65521;
+  double ln_p = std::log2(double(card)) * M_LN2;
+  double upper = 1.0 - 1.0 / double(code.index), target = -double(bits) * M_LN2;
+  auto valid = [&](double d) { double lf; return bound_at(code.message_len, code.index, d, ln_p, lf) && lf <= target; };
+  double lower = 0.0, ub = upper;
+  for (uint64_t step = 1; step <= 4096; step++) {
+    double cand = upper * double(step) / 4096.0;
+    if (valid(cand)) lower = cand;
+    else if (cand > lower) { ub = cand; break; }
+  }
+  if (lower == 0.0) return rows;  // no distance bound: every row is queried
+  for (int i = 0; i < 96; i++) { double mid = (lower + ub) * 0.5; if (valid(mid)) lower = mid; else ub = mid; }
+  double rel = lower * (1.0 - 1e-12);
+  double q = -double(bits) / std::log2(1.0 - rel / 3.0);
+  require(std::isfinite(q) && q >= 0.0, QF_ERR_ENCODING, "query count");
+  return std::min<uint64_t>(uint64_t(std::ceil(q)), rows);
+}
+std::vector<uint64_t> sample_rows(uint64_t count, uint64_t population, const Digest& seed) {  // qafold_short_pc.rs:3000-3047
+  std::vector<uint64_t> out;
+  if (count == population) { for (uint64_t i = 0; i < population; i++) out.push_back(i); return out; }
+  std::map<uint64_t, uint64_t> swaps;
+  uint64_t counter = 0;
+  for (uint64_t off = 0; off < count; off++) {
+    uint64_t remaining = population - off;
+    unsigned __int128 range = (unsigned __int128)1 << 64, limit = range - range % remaining;
+    uint64_t s;
+    for (;;) {
+      Hasher h;
+      h.update("qafold-short-index/v1", 21);
+      h.update(seed.data(), 32);
+      h.u64le(counter++);
+      Digest d = h.finish();
+      uint64_t cand = 0;
+      for (int i = 0; i < 8; i++) cand |= uint64_t(d[i]) << (8 * i);
+      if ((unsigned __int128)cand < limit) { s = cand % remaining; break; }
+    }
+    uint64_t last = remaining - 1;
+    uint64_t sv = swaps.count(s) ? swaps[s] : s, lv = swaps.count(last) ? swaps[last] : last;
+    swaps[s] = lv;
+    out.push_back(sv);
+  }
+  return out;
+}
+
+// ================================================================ device codewords
+// A 32-column codeword (column-major, R rows) and its Merkle tree in HBM.
+struct DCode {
+  DVec cw;
+  uint32_t* tree = nullptr;
+  bool own_tree = false;
+  const E8* cwp = nullptr;  // borrowed codeword when cw is empty
+  uint64_t R = 0, cols = 32;
+  Digest root{};
+  const E8* data() const { return cw.p ? cw.p : cwp; }
+  DCode() = default;
+  DCode(DCode&& o) noexcept
+      : cw(std::move(o.cw)), tree(o.tree), own_tree(o.own_tree), cwp(o.cwp), R(o.R), cols(o.cols), root(o.root) {
+    o.tree = nullptr;
+    o.own_tree = false;
+  }
+  DCode& operator=(DCode&& o) noexcept {
+    if (this != &o) {
+      if (own_tree && tree) (void)hipFree(tree);
+      cw = std::move(o.cw); tree = o.tree; own_tree = o.own_tree; cwp = o.cwp; R = o.R; cols = o.cols; root = o.root;
+      o.tree = nullptr; o.own_tree = false;
+    }
+    return *this;
+  }
+  ~DCode() { if (own_tree && tree) (void)hipFree(tree); }
+};
+DCode alloc_code(uint64_t R) {
+  DCode d;
+  d.R = R;
+  d.cw = DVec(32 * R);
+  QF_HIP(hipMalloc(&d.tree, (2 * R - 1) * 32));
+  d.own_tree = true;
+  return d;
+}
+// Commit of an extension matrix under `code` (qafold_short_pc.rs:2327-2362).
+DCode commit_extension(DevCtx& c, const ParamsState& p, size_t level, const E8* values) {
+  const QaCode& code = p.pp.codes[level];
+  DCode d = alloc_code(code.codeword_len());
+  launch_commit_extension(values, d.cw.p, ilog2u(code.message_len), p.alpha_device(level), p.leaf_ext, p.geom.inner,
+                          d.tree, c.st);
+  QF_HIP(hipGetLastError());
+  d.root = download_root(c, d.tree, d.R);
+  return d;
+}
+// G^T applied to a codeword-length vector: left[:n] + WHT(alpha_1 . WHT(left[n:])) (qafold_short.rs:1106-1163)
+DVec generator_left(DevCtx& c, const ParamsState& p, size_t level, const DVec& left) {
+  const QaCode& code = p.pp.codes[level];
+  uint64_t n = code.message_len;
+  DVec out(n);
+  dv::copy(c, out.p, left.p + n, n);
+  launch_wht(out.p, 1, n, ilog2u(n), c.st);
+  dv::mul(c, out.p, p.alpha_device(level), n);
+  launch_wht(out.p, 1, n, ilog2u(n), c.st);
+  dv::axpy(c, out.p, left.p, n, e8_one());
+  return out;
+}
+// G applied to a message-length vector: the codeword of `right` (qafold_short.rs:1091-1104)
+DVec generator_right(DevCtx& c, const ParamsState& p, size_t level, const DVec& right) {
+  const QaCode& code = p.pp.codes[level];
+  uint64_t n = code.message_len;
+  DVec out(2 * n);
+  dv::copy(c, out.p, right.p, n);
+  launch_encode_parity(out.p, 1, ilog2u(n), p.alpha_device(level), c.st);
+  QF_HIP(hipGetLastError());
+  return out;
+}
+DVec tw_vec(DevCtx& c, const std::vector<E8>& point) {
+  DVec v(uint64_t(1) << point.size());
+  tw_device(c, v.p, point);
+  return v;
+}
+
+struct LB { E8 a_value, b_value; std::vector<E8> a_point, b_point; };
+// qafold_short_pc.rs:1903-1948 (label null: the multi-evaluation form without a domain frame)
+LB litebullet(DevCtx& c, E8 claimed, DVec& left, DVec& right, Pcs& tr, const char* label, const char* fold_label,
+              bool check_claim = true) {
+  uint64_t n = left.n;
+  require(pow2(n) && right.n == n, QF_ERR_INVALID_COMMITMENT, "litebullet shape");
+  if (check_claim) require(dv::inner(c, left.p, right.p, n) == claimed, QF_ERR_INVALID_COMMITMENT, "litebullet claim");
+  if (label) tr.bytes("litebullet-domain", label);
+  LB lb;
+  E8 rhs = claimed;
+  while (n > 1) {
+    uint64_t h = n / 2;
+    auto [lc, rc] = dv::cross_halves(c, left.p, right.p, h);
+    tr.respond(lc);
+    tr.respond(rc);
+    E8 ch = tr.challenge(fold_label, true), ci = e8_inv(ch);
+    rhs += lc * ch + rc * ci;
+    dv::lb_fold(c, left.p, right.p, h, ci, ch);
+    lb.a_point.push_back(ci);
+    lb.b_point.push_back(ch);
+    n = h;
+  }
+  auto a = dv::download(c, left.p, 1), b = dv::download(c, right.p, 1);
+  lb.a_value = a[0];
+  lb.b_value = b[0];
+  tr.respond(lb.a_value);
+  tr.respond(lb.b_value);
+  require(lb.a_value * lb.b_value == rhs, QF_ERR_INVALID_COMMITMENT, "litebullet terminal");
+  return lb;
+}
+
+Digest hash_terminal(const std::vector<E8>& v) {
+  Hasher h;
+  h.frame("domain", "qafold-short/terminal-vector/v1");
+  h.frame_u64("length", v.size());
+  for (auto& x : v) h.frame("element", &x, 16);
+  return h.finish();
+}
+
+// Row payload of a codeword: all entries, or sampled rows with Merkle paths (qafold_short_pc.rs:3105-3212).
+void write_rows(DevCtx& c, Bytes& out, const DCode& d, bool all, const std::vector<uint64_t>& rows) {
+  if (all) {
+    out.push_back(1);
+    put_u64(out, d.R);
+    put_u64(out, d.cols);
+    put_u64(out, d.R * d.cols);
+    auto e = dv::download(c, d.data(), d.R * d.cols);
+    for (auto& v : e) put_e8(out, v);
+    return;
+  }
+  out.push_back(0);
+  put_u64(out, rows.size());
+  auto vals = dv::gather_rows(c, d.data(), d.R, d.cols, rows);
+  auto paths = dv::gather_paths(c, d.tree, d.R, rows);
+  for (size_t k = 0; k < rows.size(); k++) {
+    put_u64(out, d.cols);
+    for (uint64_t col = 0; col < d.cols; col++) put_e8(out, vals[k * d.cols + col]);
+    put_u64(out, paths[k].size());
+    for (auto& s : paths[k]) put_digest(out, s);
+  }
+}
+
+// ================================================================ native opening
+struct RoundOut { E8 next_value, gen_value; std::vector<E8> next_point, gen_left, gen_right; };
+struct Pending { uint64_t level; E8 value; std::vector<E8> left, right; };
+
+// qafold_short_pc.rs:1454-1525
+RoundOut round_reductions(DevCtx& c, const ParamsState& p, size_t level, const std::vector<E8>& point,
+                          const E8& fv, DVec& folded, const DVec& row_w, const std::vector<uint64_t>& sel,
+                          const std::vector<E8>& sampled, Pcs& tr) {
+  const QaCode& code = p.pp.codes[level];
+  uint64_t n = code.message_len, n_ext = 2 * n;
+  int col_log = 5;
+  uint64_t sp = 1;
+  while (sp < sel.size()) sp <<= 1;
+  std::vector<E8> sel_point;
+  for (int i = 0; i < ilog2u(sp); i++) sel_point.push_back(tr.challenge("selection-left"));
+  std::vector<E8> sw(sel.size());
+  for (size_t i = 0; i < sel.size(); i++) sw[i] = weight_at(sel_point, i);
+  E8 selected_value = inner_host(sw, sampled);
+  DVec pi(n_ext);
+  dv::fill(c, pi.p, n_ext, E8{});
+  dv::scatter_add(c, pi.p, sel, sw);
+  DVec comb = generator_left(c, p, level, pi);
+  E8 rho = tr.challenge("opening-batch");
+  dv::axpy(c, comb.p, row_w.p, n, rho);
+  LB msg = litebullet(c, selected_value + rho * fv, comb, folded, tr, "message-litebullet", "litebullet-fold");
+  E8 ole = e8_one();
+  for (size_t i = 0; i < msg.a_point.size(); i++) ole = ole * (e8_one() + point[col_log + i] * msg.a_point[i]);
+  E8 sle = msg.a_value - rho * ole;
+  DVec tw = tw_vec(c, msg.a_point);
+  DVec gr = generator_right(c, p, level, tw);
+  LB gen = litebullet(c, sle, pi, gr, tr, "generator-litebullet", "litebullet-fold");
+  return {msg.b_value, gen.b_value, msg.b_point, gen.b_point, msg.a_point};
+}
+
+// One opening round of the native PCS (qafold_short_pc.rs:1120-1451).
+RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode& cache, const std::vector<E8>& point,
+                     const E8& value, bool top, Pcs& tr, Bytes& rows_wire, DCode& next, std::vector<E8>& terminal) {
+  const QaCode& code = p.pp.codes[level];
+  uint64_t n = code.message_len, R = cache.R;
+  require(R == code.codeword_len() && point.size() == size_t(5 + ilog2u(n)), QF_ERR_INVALID_VARIABLES, "round dims");
+  std::vector<E8> rp(point.begin() + 5, point.end());
+  DVec row_w = tw_vec(c, rp);
+  auto opening = dv::col_projection(c, cache.data(), R, n, 32, row_w.p);
+  if (!top) require(inner_host(opening, tw_host(point.data(), 5)) == value, QF_ERR_INVALID_COMMITMENT, "round opening");
+  tr.digest("round-root", cache.root);
+  for (auto& o : opening) tr.respond(o);
+  std::vector<E8> fr;
+  for (int k = 0; k < 32; k++) fr.push_back(tr.challenge("freivalds"));
+  E8 fv = inner_host(opening, fr);
+  DVec folded(n);
+  dv::col_fold(c, cache.data(), R, n, 32, fr, folded.p);
+  if (level > 0) {
+    next = commit_extension(c, p, level - 1, folded.p);
+    tr.bytes("next-layer-kind", "extension-root");
+    tr.digest("next-layer-binding", next.root);
+  } else {
+    terminal = dv::download(c, folded);
+    tr.bytes("next-layer-kind", "terminal-vector");
+    tr.digest("next-layer-binding", hash_terminal(terminal));
+  }
+  uint64_t count = num_queries(code, p.pp.security_bits, R);
+  Digest seed = tr.challenge_digest("row-selection");
+  bool all = count == R;
+  std::vector<uint64_t> sel = sample_rows(count, R, seed);
+  write_rows(c, rows_wire, cache, all, sel);
+  std::vector<E8> sampled;
+  if (all) {
+    DVec rv(R);
+    dv::col_fold(c, cache.data(), R, R, 32, fr, rv.p);
+    sampled = dv::download(c, rv);
+  } else {
+    auto rows = dv::gather_rows(c, cache.data(), R, 32, sel);
+    for (size_t k = 0; k < sel.size(); k++) {
+      E8 s{};
+      for (int col = 0; col < 32; col++) s += fr[col] * rows[k * 32 + col];
+      sampled.push_back(s);
+    }
+  }
+  tr.exts("sampled-values", sampled);
+  return round_reductions(c, p, level, point, fv, folded, row_w, sel, sampled, tr);
+}
+
+// qafold_short_pc.rs:578-725
+E8 open_with_reduction(DevCtx& c, const ParamsState& p, const DCode& top_code, const std::vector<E8>& point, Bytes& wire,
+                       std::vector<Pending>& pending) {
+  size_t levels = p.pp.codes.size(), top = levels - 1;
+  uint64_t n = p.pp.codes[top].message_len;
+  std::vector<E8> rp(point.begin() + 5, point.end());
+  DVec row_w = tw_vec(c, rp);
+  auto opening = dv::col_projection(c, top_code.data(), top_code.R, n, 32, row_w.p);
+  E8 claimed = inner_host(opening, tw_host(point.data(), 5));
+  Pcs tr;
+  tr.digest("parameters", p.pp.parameter_digest);
+  tr.digest("commitment", top_code.root);
+  tr.exts("point", point);
+  tr.ext("claimed-value", claimed);
+  Bytes top_rows;
+  std::vector<std::pair<Digest, Bytes>> ext_rounds;
+  DCode next;
+  std::vector<E8> terminal;
+  RoundOut ro = prove_round(c, p, top, top_code, point, claimed, true, tr, top_rows, next, terminal);
+  pending.clear();
+  auto settle = [&](size_t level, const RoundOut& o) {
+    if (level > 0) pending.push_back({level, o.gen_value, o.gen_left, o.gen_right});
+  };
+  settle(top, ro);
+  for (size_t level = top; level-- > 0;) {
+    DCode cur = std::move(next);
+    Bytes rows;
+    ro = prove_round(c, p, level, cur, ro.next_point, ro.next_value, false, tr, rows, next, terminal);
+    ext_rounds.push_back({cur.root, std::move(rows)});
+    settle(level, ro);
+  }
+  require(!terminal.empty() && inner_host(terminal, tw_host(ro.next_point)) == ro.next_value, QF_ERR_INVALID_COMMITMENT,
+          "terminal vector");
+  wire.clear();
+  put_tag(wire, "QFSHORT2");
+  put_u64(wire, tr.responses.size());
+  for (auto& v : tr.responses) put_e8(wire, v);
+  wire.insert(wire.end(), top_rows.begin(), top_rows.end());
+  put_u64(wire, ext_rounds.size());
+  for (auto& [r, rows] : ext_rounds) { put_digest(wire, r); wire.insert(wire.end(), rows.begin(), rows.end()); }
+  put_u64(wire, terminal.size());
+  for (auto& v : terminal) put_e8(wire, v);
+  return claimed;
+}
+
+// ================================================================ Decide (direct-interleaved setup)
+struct MClaim { E8 value; std::vector<E8> left, right; };
+MClaim reduce_generator(uint64_t message_len, const MClaim& g) {  // qafold_short.rs:1575-1625, index two
+  E8 norm = e8_one();
+  std::vector<E8> merged;
+  for (size_t i = 0; i < g.right.size(); i++) {
+    E8 l = g.left[1 + i], r = g.right[i];
+    E8 dl = e8_one() + l, dr = e8_one() + r;
+    norm = norm * dl * dr;
+    merged.push_back(((e8_one() - l) * e8_inv(dl)) * ((e8_one() - r) * e8_inv(dr)));
+  }
+  (void)message_len;
+  return {g.value * e8_inv(norm), merged, {g.left[0]}};
+}
+struct Sel { bool all; uint64_t count; std::vector<uint64_t> rows; };
+void absorb_sel(Pcs& tr, const Sel& s) {
+  tr.bytes("selection-kind", s.all ? "all" : "partial");
+  tr.u64("selection-count", s.all ? s.count : s.rows.size());
+  if (!s.all) for (auto r : s.rows) tr.u64("selection-row", r);
+}
+
+// qafold_short_decide_driver.rs:51-318 and the direct-interleaved helpers of
+// qafold_short_decide*.rs
+Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& pending) {
+  const auto& codes = p.pp.codes;
+  size_t levels = codes.size();
+  Pcs tr;
+  tr.bytes("decide-domain", "qafold-short/decide/v1");
+  tr.digest("decide-parameters", p.pp.parameter_digest);
+  tr.digest("decide-setup", p.pp.setup_digest);
+  tr.u64("decide-levels", levels);
+  uint8_t one = 1, zero = 0;
+  tr.bytes("opening-reduction-ok", &one, 1);
+  tr.u64("pending-h-count", pending.size());
+  for (auto& q : pending) {
+    tr.u64("pending-h-level", q.level);
+    tr.ext("pending-g-value", q.value);
+    tr.exts("pending-g-left", q.left);
+    tr.exts("pending-g-right", q.right);
+  }
+  std::vector<std::vector<MClaim>> h(levels);
+  for (auto& q : pending) h[q.level].push_back(reduce_generator(codes[q.level].message_len, {q.value, q.left, q.right}));
+  Bytes rounds;
+  uint64_t n_rounds = 0;
+  std::vector<E8> base_vector;
+  const uint64_t rf = 32;
+  bool have_w = false;
+  E8 w_value{};
+  std::vector<E8> w_point;
+  DCode carry;
+  DVec carry_msg;
+  bool have_carry = false;
+  for (size_t level = levels - 1; level >= 1; level--) {
+    const SetupLevel& sl = p.pp.setup[level];
+    const QaCode& code = codes[level];
+    uint64_t rows = code.message_len, log_rows = ilog2u(rows);
+    struct EC { E8 value; std::vector<E8> row_point, col_weight; };
+    std::vector<EC> claims;
+    for (auto& hc : h[level]) claims.push_back({hc.value, hc.left, tw_host(hc.right)});
+    h[level].clear();
+    uint64_t setup_cols = 2, carry_cols = have_w ? 1 : 0;
+    if (have_w) {
+      std::vector<E8> cwt(setup_cols, E8{});
+      cwt.push_back(e8_one());
+      claims.push_back({w_value, w_point, cwt});
+    }
+    require(!claims.empty(), QF_ERR_INVALID_VARIABLES, "no Decide claims");
+    uint64_t real = setup_cols + carry_cols, padded = 1;
+    while (padded < real) padded <<= 1;
+    uint64_t log_cols = ilog2u(padded);
+    const SetupShape& sh = sl.shape;
+    tr.bytes("decide-round-domain", "qafold-short/decide-round/v1");
+    tr.u64("decide-round-level", level);
+    tr.u64("decide-round-shape-level", sh.level);
+    tr.u64("decide-round-shape-encoded-rows", sh.encoded_rows);
+    tr.u64("decide-round-shape-message-rows", sh.message_rows);
+    tr.u64("decide-round-shape-t-cols", sh.t_cols);
+    tr.u64("decide-round-shape-t-elements", sh.t_elements);
+    tr.u64("decide-round-shape-t-last-len", sh.t_last_len);
+    tr.u64("decide-round-shape-h-cols", sh.h_cols);
+    const Digest& setup_root = sl.tree.back()[0];
+    tr.digest("decide-round-setup-root", setup_root);
+    tr.bytes("decide-round-carry-present", have_carry ? &one : &zero, 1);
+    if (have_carry) tr.digest("decide-round-carry-root", carry.root);
+    // multi-evaluation over the flattened mixed matrix (qafold_short_decide.rs:486-551)
+    tr.bytes("multi-eval-domain", "qafold-short-decide/multi-eval/v2");
+    tr.u64("multi-eval-log-rows", log_rows);
+    tr.u64("multi-eval-log-cols", log_cols);
+    tr.u64("multi-eval-claim-count", claims.size());
+    for (auto& cl : claims) {
+      tr.ext("multi-eval-claim-value", cl.value);
+      tr.exts("multi-eval-row-point", cl.row_point);
+      tr.exts("multi-eval-column-weight", cl.col_weight);
+    }
+    E8 rho = tr.challenge("multi-eval-batch");
+    DVec left(padded * rows), right(padded * rows);
+    dv::fill(c, left.p, padded * rows, E8{});
+    dv::fill(c, right.p, padded * rows, E8{});
+    dv::fill(c, left.p, rows, e8_inv(e8(uint32_t(rows % P))));  // alpha_0 = 1/n
+    dv::copy(c, left.p + rows, p.alpha_device(level), rows);
+    if (have_carry) dv::copy(c, left.p + 2 * rows, carry_msg.p, rows);
+    E8 rhs{}, pw = e8_one();
+    for (auto& cl : claims) {
+      rhs += pw * cl.value;
+      DVec rw = tw_vec(c, cl.row_point);
+      for (size_t k = 0; k < cl.col_weight.size(); k++)
+        if (!e8_is_zero(cl.col_weight[k])) dv::axpy(c, right.p + k * rows, rw.p, rows, pw * cl.col_weight[k]);
+      pw = pw * rho;
+    }
+    std::vector<E8> red_point;
+    DVec folded_vector(rows);
+    uint64_t n = padded * rows;
+    while (n > 1) {
+      uint64_t hh = n / 2;
+      auto [lc, rc] = dv::cross_halves(c, left.p, right.p, hh);
+      tr.respond(lc);
+      tr.respond(rc);
+      E8 ch = tr.challenge("multi-eval-fold", true), ci = e8_inv(ch);
+      rhs += lc * ch + rc * ci;
+      dv::lb_fold(c, left.p, right.p, hh, ci, ch);
+      red_point.push_back(ci);
+      n = hh;
+      if (n == rows) dv::copy(c, folded_vector.p, left.p, rows);
+    }
+    E8 red_value = dv::download(c, left.p, 1)[0], bval = dv::download(c, right.p, 1)[0];
+    tr.respond(red_value);
+    tr.respond(bval);
+    require(red_value * bval == rhs, QF_ERR_INVALID_COMMITMENT, "multi-eval terminal");
+    std::vector<E8> col_point(red_point.begin(), red_point.begin() + log_cols);
+    std::vector<E8> merged(red_point.begin() + log_cols, red_point.end());
+    auto col_w = tw_host(col_point);
+    const QaCode& target = codes[level - 1];
+    uint64_t n_enc = target.codeword_len();
+    uint64_t count = num_queries(target, p.pp.security_bits, n_enc);
+    Digest seed = tr.challenge_digest("decide-row-selection");
+    Sel sel{count == n_enc, count, {}};
+    std::vector<uint64_t> srows = sample_rows(count, n_enc, seed);
+    if (!sel.all) sel.rows = srows;
+    auto bw = tw_host(merged.data(), 5);
+    uint64_t phys = real * rf, pphys = 1;
+    while (pphys < phys) pphys <<= 1;
+    std::vector<E8> iw(pphys, E8{});
+    for (uint64_t lc = 0; lc < real; lc++)
+      for (uint64_t b = 0; b < rf; b++) iw[lc * rf + b] = col_w[lc] * bw[b];
+    uint64_t phys_carry = carry_cols * rf, scols = sh.t_cols + sh.h_cols;
+    tr.bytes("iota-domain", "qafold-short/decide-iota/v1");
+    tr.u64("iota-level", level);
+    tr.u64("iota-shape-level", sh.level);
+    tr.u64("iota-shape-encoded-rows", sh.encoded_rows);
+    tr.u64("iota-shape-message-rows", sh.message_rows);
+    tr.u64("iota-shape-t-cols", sh.t_cols);
+    tr.u64("iota-shape-t-elements", sh.t_elements);
+    tr.u64("iota-shape-t-last-len", sh.t_last_len);
+    tr.u64("iota-shape-h-cols", sh.h_cols);
+    tr.digest("iota-setup-root", setup_root);
+    tr.bytes("iota-carry-present", have_carry ? &one : &zero, 1);
+    if (have_carry) tr.digest("iota-carry-root", carry.root);
+    tr.u64("iota-carry-cols", phys_carry);
+    absorb_sel(tr, sel);
+    tr.exts("iota-column-weights", iw);
+    std::vector<E8> carry_rows;
+    if (have_carry) carry_rows = dv::gather_rows(c, carry.data(), carry.R, 32, srows);
+    std::vector<E8> iota;
+    for (size_t k = 0; k < srows.size(); k++) {
+      E8 v{};
+      for (uint64_t col = 0; col < scols; col++) v += iw[col] * sl.encoded[col * n_enc + srows[k]];
+      if (have_carry)
+        for (uint64_t col = 0; col < phys_carry; col++) v += iw[scols + col] * carry_rows[k * 32 + col];
+      iota.push_back(v);
+    }
+    for (auto& v : iota) tr.respond(v);
+    Bytes round;
+    put_u64(round, level);
+    uint64_t trows = target.message_len;
+    DVec rec(trows);
+    dv::col_fold(c, folded_vector.p, trows, trows, rf, bw, rec.p);  // fold the block coordinate
+    DCode next_carry;
+    bool have_next = level > 1;
+    if (have_next) next_carry = commit_extension(c, p, level - 2, rec.p);
+    tr.bytes("decide-next-carry-domain", "qafold-short/decide-next-carry/v1");
+    tr.bytes("decide-next-carry-present", have_next ? &one : &zero, 1);
+    if (have_next) tr.digest("decide-next-carry-root", next_carry.root);
+    round.push_back(have_next ? 1 : 0);
+    if (have_next) put_digest(round, next_carry.root);
+    if (sel.all) {
+      round.push_back(1);
+      for (uint64_t v : {sh.level, sh.encoded_rows, sh.message_rows, sh.t_cols, sh.t_elements, sh.t_last_len, sh.h_cols})
+        put_u64(round, v);
+      put_u64(round, sl.encoded.size());
+      for (auto& v : sl.encoded) put_e8(round, v);
+    } else {
+      round.push_back(0);
+      put_u64(round, sel.rows.size());
+      for (uint64_t r : sel.rows) {
+        put_u64(round, r);
+        put_u64(round, scols);
+        for (uint64_t col = 0; col < scols; col++) put_e8(round, sl.encoded[col * n_enc + r]);
+        uint64_t idx = r;
+        put_u64(round, sl.tree.size() - 1);
+        for (size_t l = 0; l + 1 < sl.tree.size(); l++) { put_digest(round, sl.tree[l][idx ^ 1]); idx >>= 1; }
+      }
+    }
+    if (have_carry) { round.push_back(1); write_rows(c, round, carry, sel.all, srows); } else round.push_back(0);
+    // direct code switch (qafold_short_decide_carry.rs:940-1027)
+    std::vector<E8> base_point(merged.begin() + 5, merged.end());
+    DVec tbp = tw_vec(c, base_point);
+    require(dv::inner(c, rec.p, tbp.p, trows) == red_value, QF_ERR_INVALID_COMMITMENT, "merged carry value");
+    tr.bytes("direct-code-switch-domain", "qafold-short/decide-direct-code-switch/v2");
+    tr.u64("direct-code-switch-target-level", level - 1);
+    tr.u64("direct-code-switch-row-factor", rf);
+    tr.u64("direct-code-switch-message-len", trows);
+    tr.u64("direct-code-switch-encoded-len", n_enc);
+    tr.u64("direct-code-switch-target-index", target.index);
+    tr.digest("direct-code-switch-target-generator", target.generator_digest);
+    absorb_sel(tr, sel);
+    tr.exts("direct-code-switch-iota", iota);
+    tr.ext("direct-code-switch-merged-value", red_value);
+    tr.exts("direct-code-switch-merged-left-point", merged);
+    uint64_t sp = 1;
+    while (sp < srows.size()) sp <<= 1;
+    std::vector<E8> slp;
+    for (int i = 0; i < ilog2u(sp); i++) slp.push_back(tr.challenge("direct-code-switch-selection-left"));
+    std::vector<E8> sw(srows.size());
+    for (size_t i = 0; i < srows.size(); i++) sw[i] = weight_at(slp, i);
+    E8 selected_value = inner_host(sw, iota);
+    DVec sparse(n_enc);
+    dv::fill(c, sparse.p, n_enc, E8{});
+    dv::scatter_add(c, sparse.p, srows, sw);
+    DVec gl = generator_left(c, p, level - 1, sparse);
+    E8 rho2 = tr.challenge("direct-code-switch-opening-batch");
+    dv::axpy(c, gl.p, tbp.p, trows, rho2);
+    DVec rec_copy(trows);
+    dv::copy(c, rec_copy.p, rec.p, trows);
+    LB msg = litebullet(c, selected_value + rho2 * red_value, gl, rec_copy, tr, "decide-direct-carry-message-litebullet",
+                        "litebullet-fold");
+    E8 sle = msg.a_value - rho2 * tensor_inner(base_point, msg.a_point);
+    DVec tw = tw_vec(c, msg.a_point);
+    DVec gr = generator_right(c, p, level - 1, tw);
+    LB gen = litebullet(c, sle, sparse, gr, tr, "decide-direct-carry-generator-litebullet", "litebullet-fold");
+    if (level > 1) {
+      h[level - 1].push_back(reduce_generator(trows, {gen.b_value, gen.b_point, msg.a_point}));
+      w_value = msg.b_value;
+      w_point = msg.b_point;
+      have_w = true;
+      carry = std::move(next_carry);
+      carry_msg = std::move(rec);
+      have_carry = true;
+    } else {
+      auto term = dv::download(c, rec);
+      require(inner_host(term, tw_host(msg.b_point)) == msg.b_value, QF_ERR_INVALID_COMMITMENT, "terminal carry");
+      for (auto& v : term) tr.respond(v);
+      base_vector = term;
+      have_carry = false;
+    }
+    rounds.insert(rounds.end(), round.begin(), round.end());
+    n_rounds++;
+  }
+  Bytes w;
+  put_tag(w, "QFSDCID1");
+  put_u64(w, tr.responses.size());
+  for (auto& v : tr.responses) put_e8(w, v);
+  put_u64(w, n_rounds);
+  w.insert(w.end(), rounds.begin(), rounds.end());
+  put_u64(w, base_vector.size());
+  for (auto& v : base_vector) put_e8(w, v);
+  return w;
+}
+
+// ================================================================ TBV (tbv.rs)
+struct Term {
+  uint64_t offset = 0;
+  E8 scale{};
+  std::vector<std::array<E8, 2>> f;
+  uint64_t len() const { return uint64_t(1) << f.size(); }
+  E8 weight(uint64_t i) const { E8 v = scale; for (size_t b = 0; b < f.size(); b++) v = v * f[b][(i >> b) & 1]; return v; }
+};
+E8 monomial(uint64_t index, const E8* pl, size_t n) {
+  E8 w = e8_one();
+  for (size_t b = 0; b < n; b++) if ((index >> b) & 1) w = w * pl[b];
+  return w;
+}
+std::vector<std::pair<uint64_t, uint64_t>> aligned(uint64_t start, uint64_t end) {
+  std::vector<std::pair<uint64_t, uint64_t>> out;
+  while (start < end) {
+    int bits = ilog2u(end - start);
+    if (start) bits = std::min(bits, __builtin_ctzll(start));
+    out.push_back({start, uint64_t(1) << bits});
+    start += uint64_t(1) << bits;
+  }
+  return out;
+}
+Term compact(Term t) {
+  while (!t.f.empty()) {
+    auto pr = t.f.back();
+    uint64_t fixed;
+    if (e8_is_zero(pr[0])) fixed = 1; else if (e8_is_zero(pr[1])) fixed = 0; else break;
+    t.scale = t.scale * pr[fixed];
+    t.offset += fixed << (t.f.size() - 1);
+    t.f.pop_back();
+  }
+  return t;
+}
+struct Query { std::vector<Term> terms; size_t n_bucket_terms = 0; E8 claim{}; };
+E8 query_eval(const Query& q, const std::vector<E8>& pl) {
+  E8 s{};
+  for (auto& t : q.terms) {
+    E8 local = t.scale;
+    size_t k = t.f.size();
+    for (size_t i = 0; i < k; i++) local = local * (t.f[i][0] + t.f[i][1] * pl[i]);
+    s += local * monomial(t.offset >> k, pl.data() + k, pl.size() - k);
+  }
+  return s;
+}
+E8 eq_weight(uint64_t index, const std::vector<E8>& p) {
+  E8 w = e8_one();
+  for (size_t c = 0; c < p.size(); c++) w = w * (((index >> (p.size() - 1 - c)) & 1) ? p[c] : e8_one() - p[c]);
+  return w;
+}
+Query compile(const Layout& l, const std::vector<ClaimIn>& claims, Outer& tr) {  // tbv.rs:187-284
+  uint64_t np = 1;
+  while (np < claims.size()) np <<= 1;
+  std::vector<E8> batch;
+  for (int i = 0; i < ilog2u(np); i++) batch.push_back(tr.challenge("batch"));
+  std::vector<E8> coef(claims.size());
+  Query q;
+  for (size_t i = 0; i < claims.size(); i++) { coef[i] = eq_weight(i, batch); q.claim += coef[i] * claims[i].value; }
+  for (auto& b : l.buckets) {
+    const ClaimIn& cl = claims[b.matrix];
+    Term t;
+    t.offset = b.offset;
+    t.scale = coef[b.matrix] * monomial(b.row_offset, cl.row.data(), cl.row.size()) * monomial(b.col_offset, cl.col.data(), cl.col.size());
+    for (int i = 0; i < ilog2u(b.cols); i++) t.f.push_back({e8_one(), cl.col[i]});
+    for (int i = 0; i < ilog2u(b.rows); i++) t.f.push_back({e8_one(), cl.row[i]});
+    q.terms.push_back(std::move(t));
+  }
+  q.n_bucket_terms = q.terms.size();
+  std::vector<E8> pp;
+  for (int i = 0; i < ilog2u(l.logical); i++) pp.push_back(tr.challenge("padding-point"));
+  E8 mix = tr.nonzero("padding-mix");
+  for (auto& b : l.buckets) {
+    int cb = ilog2u(b.cols), bits = ilog2u(b.len()), rb = ilog2u(b.rows);
+    std::vector<std::array<uint64_t, 4>> rects;
+    for (auto [s, n] : aligned(b.physical_rows, b.rows)) rects.push_back({s, n, 0, b.cols});
+    for (auto [rs, rl] : aligned(0, b.physical_rows))
+      for (auto [cs, cl] : aligned(b.physical_cols, b.cols)) rects.push_back({rs, rl, cs, cl});
+    for (auto& r : rects) {
+      Term t;
+      for (int i = 0; i < bits; i++) t.f.push_back({e8_one(), pp[i]});
+      for (int bit = ilog2u(r[3]); bit < cb; bit++) t.f[bit][1 - ((r[2] >> bit) & 1)] = E8{};
+      for (int bit = ilog2u(r[1]); bit < rb; bit++) t.f[cb + bit][1 - ((r[0] >> bit) & 1)] = E8{};
+      t.offset = b.offset;
+      t.scale = mix * monomial(b.offset >> bits, pp.data() + bits, pp.size() - bits);
+      q.terms.push_back(compact(std::move(t)));
+    }
+  }
+  for (auto [off, len] : aligned(l.active, l.logical)) {
+    int bits = ilog2u(len);
+    Term t;
+    t.offset = off;
+    t.scale = mix * monomial(off >> bits, pp.data() + bits, pp.size() - bits);
+    for (int i = 0; i < bits; i++) t.f.push_back({e8_one(), pp[i]});
+    q.terms.push_back(std::move(t));
+  }
+  return q;
+}
+std::vector<E8> tbv_w(const std::array<E8, 2>* f, size_t n) {
+  std::vector<E8> r(1, e8_one());
+  for (size_t k = 0; k < n; k++) {
+    size_t len = r.size();
+    r.resize(2 * len);
+    for (size_t i = 0; i < len; i++) { r[len + i] = r[i] * f[k][1]; r[i] = r[i] * f[k][0]; }
+  }
+  return r;
+}
+E8 small_byte(int v) { return e8(uint32_t(v < 0 ? v + int(P) : v)); }
+std::vector<int8_t> read_bytes(DevCtx& c, const int8_t* bva, uint64_t active, uint64_t start, uint64_t n) {
+  std::vector<int8_t> out(n, 0);
+  if (start < active) {
+    uint64_t m = std::min(n, active - start);
+    QF_HIP(hipMemcpyAsync(out.data(), bva + start, m, hipMemcpyDeviceToHost, c.st));
+    c.sync();
+  }
+  return out;
+}
+// Lane projections of the terms with len >= s on the device (tbv.rs:288-352).
+std::vector<std::vector<E8>> project_terms(DevCtx& c, const Query& q, const int8_t* bva, uint64_t active, uint64_t s,
+                                           std::vector<size_t>& which) {
+  std::vector<ProjTask> tasks;
+  std::vector<std::array<E8, 2>> pairs;
+  int ls = ilog2u(s);
+  for (size_t i = 0; i < q.terms.size(); i++) {
+    const Term& t = q.terms[i];
+    if (t.len() < s || t.offset >= active || e8_is_zero(t.scale)) continue;
+    ProjTask pt;
+    pt.base = t.offset;
+    pt.rows = t.len() / s;
+    pt.nbits = uint32_t(t.f.size() - ls);
+    pt.pair_off = uint32_t(pairs.size());
+    pt.scale = t.scale;
+    for (size_t b = ls; b < t.f.size(); b++) pairs.push_back(t.f[b]);
+    tasks.push_back(pt);
+    which.push_back(i);
+  }
+  auto flat = tensor_project(c, bva, active, int(s), tasks, pairs);
+  std::vector<std::vector<E8>> out(tasks.size());
+  for (size_t k = 0; k < tasks.size(); k++) out[k].assign(flat.begin() + k * s, flat.begin() + (k + 1) * s);
+  return out;
+}
+
+// ================================================================ EO (eo.rs:97-181)
+struct EoOut { std::vector<E8> pos, neg; std::vector<std::array<E8, 2>> rounds; E8 value, query; std::vector<E8> point; E8 scale; };
+EoOut eo_prove(DevCtx& c, const Layout& l, const int8_t* bva, uint64_t s, const Query& q, Outer& tr) {
+  EoOut o;
+  o.pos.assign(s - 1, E8{});
+  o.neg.assign(s - 1, E8{});
+  int ls = ilog2u(s);
+  std::vector<size_t> which;
+  auto proj = project_terms(c, q, bva, l.active, s, which);
+  for (size_t k = 0; k < which.size(); k++) {
+    const Term& t = q.terms[which[k]];
+    auto low = tbv_w(t.f.data(), ls);
+    for (uint64_t d = 1; d < s; d++)
+      for (uint64_t lane = 0; lane + d < s; lane++) {
+        o.pos[d - 1] += proj[k][lane] * low[lane + d];
+        o.neg[d - 1] += proj[k][lane + d] * low[lane];
+      }
+  }
+  for (auto& t : q.terms) {
+    if (t.len() >= s) continue;
+    uint64_t start = t.offset / s * s;
+    auto vals = read_bytes(c, bva, l.active, start, s);
+    std::vector<E8> query(s, E8{});
+    for (uint64_t i = 0; i < t.len(); i++) query[t.offset - start + i] = t.weight(i);
+    for (uint64_t d = 1; d < s; d++)
+      for (uint64_t lane = 0; lane + d < s; lane++) {
+        o.pos[d - 1] += query[lane + d] * small_byte(vals[lane]);
+        o.neg[d - 1] += query[lane] * small_byte(vals[lane + d]);
+      }
+  }
+  tr.object("packed-positive", js_scalars(o.pos));
+  tr.object("packed-negative", js_scalars(o.neg));
+  E8 ch = tr.fold_challenge(ls);
+  std::vector<E8> challenges;
+  E8 pw = ch;
+  for (int i = 0; i < ls; i++) { challenges.push_back(pw); pw = pw * pw; }
+  E8 inv = e8_inv(ch);
+  std::vector<E8> pinv(s), pows(s);
+  pinv[0] = pows[0] = e8_one();
+  for (uint64_t i = 1; i < s; i++) { pinv[i] = pinv[i - 1] * inv; pows[i] = pows[i - 1] * ch; }
+  uint64_t rows = l.logical / s;
+  DVec w(rows), qv(rows);
+  eo_witness(c, bva, l.active, rows, int(s), pinv, w.p);
+  dv::fill(c, qv.p, rows, E8{});
+  // query fold (tbv.rs:393-428): bucket terms, then padding/suffix terms; each phase has disjoint supports.
+  std::map<uint64_t, E8> small;
+  for (int phase = 0; phase < 2; phase++) {
+    std::vector<FillTask> tasks;
+    std::vector<std::array<E8, 2>> pairs;
+    size_t a = phase ? q.n_bucket_terms : 0, b = phase ? q.terms.size() : q.n_bucket_terms;
+    for (size_t i = a; i < b; i++) {
+      const Term& t = q.terms[i];
+      if (t.len() >= s) {
+        auto low = tbv_w(t.f.data(), ls);
+        E8 acc{};
+        for (uint64_t j = 0; j < s; j++) acc += low[j] * pows[j];
+        FillTask ft{t.offset / s, uint32_t(t.f.size() - ls), uint32_t(pairs.size()), t.scale * acc};
+        for (size_t bb = ls; bb < t.f.size(); bb++) pairs.push_back(t.f[bb]);
+        tasks.push_back(ft);
+      } else {
+        for (uint64_t j = 0; j < t.len(); j++) small[t.offset / s] += t.weight(j) * pows[(t.offset + j) % s];
+      }
+    }
+    tensor_fill_add(c, qv.p, tasks, pairs);
+  }
+  {
+    std::vector<uint64_t> idx;
+    std::vector<E8> vals;
+    for (auto& [k, v] : small) { idx.push_back(k); vals.push_back(v); }
+    dv::scatter_add(c, qv.p, idx, vals);
+  }
+  E8 claim = q.claim, fwd = ch, bwd = inv;
+  for (uint64_t d = 0; d + 1 < s; d++) { claim += o.pos[d] * fwd + o.neg[d] * bwd; fwd = fwd * ch; bwd = bwd * inv; }
+  require(dv::inner(c, w.p, qv.p, rows) == claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
+  DVec w2(rows / 2 ? rows / 2 : 1), q2(rows / 2 ? rows / 2 : 1);
+  DVec *cw = &w, *cq = &qv, *nw = &w2, *nq = &q2;
+  uint64_t n = rows;
+  while (n > 1) {
+    uint64_t h = n / 2;
+    auto [c0, c1] = dv::eo_cross(c, cw->p, cq->p, h);
+    tr.object("binary-cross", js_scalars({c0, c1}));
+    E8 r = tr.fold_challenge(1), ri = e8_inv(r);
+    claim += c0 * r + c1 * ri;
+    challenges.push_back(r);
+    dv::eo_fold(c, cw->p, cq->p, nw->p, nq->p, h, ri, r);
+    std::swap(cw, nw);
+    std::swap(cq, nq);
+    o.rounds.push_back({c0, c1});
+    n = h;
+  }
+  o.value = dv::download(c, cw->p, 1)[0];
+  o.query = dv::download(c, cq->p, 1)[0];
+  require(claim == o.value * o.query && query_eval(q, challenges) == o.query, QF_ERR_INVALID_COMMITMENT, "EO terminal mismatch");
+  tr.scalar("terminal-value", o.value);
+  tr.scalar("terminal-query", o.query);
+  o.scale = e8_one();
+  for (auto& x : challenges) {
+    E8 m = e8_inv(x), f = e8_one() + m;
+    o.point.push_back(m * e8_inv(f));
+    o.scale = o.scale * f;
+  }
+  std::reverse(o.point.begin(), o.point.end());
+  return o;
+}
+
+// ================================================================ packed PCS openings (packed.rs, packed_batch.rs)
+E8 projected_claim(const std::array<E8, 8>& ev, const std::vector<E8>& low) {  // packed.rs:502-523
+  E8 x1 = e8_inv(e8_basis(1));
+  std::array<E8, 8> ib;
+  ib[0] = e8_one();
+  for (int i = 1; i < 8; i++) ib[i] = ib[i - 1] * x1;
+  uint32_t omega_inv = fp_inv(57852), inv8 = fp_inv(8), step = 1;
+  E8 claim{};
+  for (int j = 0; j < 8; j++) {
+    uint32_t phase = 1;
+    E8 sum{};
+    for (int i = 0; i < 8; i++) { sum += e8_scale(e8_frob(ev[i], i), phase); phase = fp_mul(phase, step); }
+    claim += low[j] * e8_scale(sum * ib[j], inv8);
+    step = fp_mul(step, omega_inv);
+  }
+  return claim;
+}
+Packed packed_transcript(const ParamsState& p, const Digest& root, const std::vector<E8>& point, const E8& value,
+                         const Bytes& context) {
+  Packed t;
+  const char* dom = "mikros/qafold-f65521-ext8/packed-boolean-mle/v1";
+  t.bytes(dom, std::strlen(dom));
+  t.bytes(p.pp.parameter_digest.data(), 32);
+  uint8_t ln[8];
+  for (int i = 0; i < 8; i++) ln[i] = uint8_t(uint64_t(p.chunk_log) >> (8 * i));
+  t.bytes(ln, 8);
+  t.bytes(root.data(), 32);
+  t.bytes(context.data(), context.size());
+  for (auto& x : point) t.field(x);
+  t.field(value);
+  return t;
+}
+std::array<E8, 8> powers8(const E8& v) { std::array<E8, 8> o; o[0] = e8_one(); for (int i = 1; i < 8; i++) o[i] = o[i - 1] * v; return o; }
+E8 batch_weight(const E8* point, size_t n, const std::vector<E8>& red, const std::array<E8, 8>& powers) {
+  E8 sum{};
+  for (int i = 0; i < 8; i++) {
+    E8 w = e8_one();
+    for (size_t k = 0; k < n; k++) {
+      E8 s = e8_frob(point[k], (8 - i) % 8);
+      w = w * ((e8_one() - s) * (e8_one() - red[k]) + s * red[k]);
+    }
+    sum += powers[i] * w;
+  }
+  return sum;
+}
+void sumcheck(DevCtx& c, DVec& v, DVec& w, E8& claim, Packed& t, std::vector<std::array<E8, 2>>& rounds,
+              std::vector<E8>& reduced, E8& fval, E8& fwt) {
+  uint64_t n = v.n;
+  while (n > 1) {
+    uint64_t h = n / 2;
+    auto [m0, m1] = dv::sc_msg(c, v.p, w.p, h);
+    t.field(m0);
+    t.field(m1);
+    E8 r = t.challenge();
+    claim = m0 + r * ((claim - m0 - m0 - m1) + r * m1);
+    dv::sc_fold(c, v.p, w.p, h, r);
+    rounds.push_back({m0, m1});
+    reduced.push_back(r);
+    n = h;
+  }
+  fval = dv::download(c, v.p, 1)[0];
+  fwt = dv::download(c, w.p, 1)[0];
+}
+// Lanes of every block against eq(outer): lanes[b][j] = sum_i byte(b, 8i + j) * eq_i
+std::vector<std::array<E8, 8>> block_lanes(DevCtx& c, const ParamsState& p, const int8_t* bva, uint64_t active,
+                                           uint64_t blocks, const std::vector<E8>& outer) {
+  std::vector<std::array<E8, 2>> pairs;
+  size_t m = outer.size();
+  for (size_t b = 0; b < m; b++) { const E8& pc = outer[m - 1 - b]; pairs.push_back({e8_one() - pc, pc}); }
+  std::vector<ProjTask> tasks;
+  for (uint64_t b = 0; b < blocks; b++) tasks.push_back({b * p.block_len, uint64_t(1) << m, uint32_t(m), 0, e8_one()});
+  auto flat = tensor_project(c, bva, active, 8, tasks, pairs);
+  std::vector<std::array<E8, 8>> out(blocks);
+  for (uint64_t b = 0; b < blocks; b++)
+    for (int j = 0; j < 8; j++) out[b][j] = flat[b * 8 + j];
+  return out;
+}
+std::array<E8, 8> conjugates(const std::array<E8, 8>& lanes) {
+  std::array<E8, 8> e;
+  for (int img = 0; img < 8; img++) {
+    E8 s{};
+    for (int j = 0; j < 8; j++) s += e8_basis(j) * e8_frob(lanes[j], (8 - img) % 8);
+    e[img] = s;
+  }
+  return e;
+}
+
+struct Timing { ProveTimings* t; };
+
+// Codeword of block b: the resident cache, or a fresh commit in recompute mode.
+DCode block_code(DevCtx& c, const ParamsState& p, const CommittedState& cs, uint64_t b, DCode& workspace) {
+  uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
+  DCode d;
+  d.R = R;
+  d.root = cs.roots[b];
+  if (cs.cache_mode == QF_CACHE_CACHED && b < cs.d_cw.size()) {
+    d.cwp = cs.d_cw[b];
+    d.tree = cs.d_tree[b];
+    return d;
+  }
+  if (!workspace.cw.p) workspace = alloc_code(R);
+  launch_commit_block(p.geom, cs.d_bva, b * p.block_len, p.layout.active, workspace.cw.p,
+                      p.alpha_device(p.pp.codes.size() - 1), workspace.tree, c.st);
+  QF_HIP(hipGetLastError());
+  Digest r = download_root(c, workspace.tree, R);
+  require(r == cs.roots[b], QF_ERR_INVALID_COMMITMENT, "recomputed block root mismatch");
+  d.cwp = workspace.cw.p;
+  d.tree = workspace.tree;
+  return d;
+}
+
+Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const std::vector<E8>& point,
+                 const Bytes& context, E8& claimed, ProveTimings& tm) {
+  auto t0 = Clock::now();
+  uint64_t blocks = cs.roots.size();
+  size_t split = ilog2u(blocks), n_outer = point.size() - 3 - split;
+  std::vector<E8> outer(point.begin() + split, point.begin() + split + n_outer);
+  auto low = eq_host(point.data() + point.size() - 3, 3);
+  auto high = eq_host(point.data(), split);
+  uint64_t plen = uint64_t(1) << n_outer;
+  auto lanes = block_lanes(c, p, cs.d_bva, p.layout.active, blocks, outer);
+  std::vector<std::array<E8, 8>> ebb(blocks);
+  for (uint64_t b = 0; b < blocks; b++) ebb[b] = conjugates(lanes[b]);
+  std::array<E8, 8> ev;
+  for (int img = 0; img < 8; img++) {
+    E8 s{};
+    for (uint64_t b = 0; b < blocks; b++) s += e8_frob(high[b], (8 - img) % 8) * ebb[b][img];
+    ev[img] = s;
+  }
+  claimed = projected_claim(ev, low);
+  Packed t = packed_transcript(p, cs.roots[0], point, claimed, context);
+  const char* bd = "one-opening-block-packed-eo/v1";
+  t.bytes(bd, std::strlen(bd));
+  uint8_t nb[8];
+  for (int i = 0; i < 8; i++) nb[i] = uint8_t(blocks >> (8 * i));
+  t.bytes(nb, 8);
+  for (auto& r : cs.roots) t.bytes(r.data(), 32);
+  for (auto& v : ev) t.field(v);
+  auto powers = powers8(t.challenge());
+  E8 claim{};
+  for (int i = 0; i < 8; i++) claim += powers[i] * ev[i];
+  auto cross = [&](uint64_t w, uint64_t q) {
+    E8 s{};
+    for (int i = 0; i < 8; i++) s += powers[i] * e8_frob(high[q], (8 - i) % 8) * ebb[w][i];
+    return s;
+  };
+  std::vector<E8> pos, neg;
+  for (uint64_t d = 1; d < blocks; d++) {
+    E8 a{}, b2{};
+    for (uint64_t b = 0; b + d < blocks; b++) { a += cross(b, b + d); b2 += cross(b + d, b); }
+    pos.push_back(a);
+    neg.push_back(b2);
+  }
+  const char* cd = "block-packed-eo-cross";
+  t.bytes(cd, std::strlen(cd));
+  for (auto& v : pos) t.field(v);
+  for (auto& v : neg) t.field(v);
+  E8 omega;
+  do { omega = t.challenge(); } while (e8_is_zero(omega));
+  {
+    E8 inv = e8_inv(omega), f = omega, bk = inv;
+    for (size_t d = 0; d < pos.size(); d++) { claim += pos[d] * f + neg[d] * bk; f = f * omega; bk = bk * inv; }
+  }
+  std::vector<E8> fw(blocks);
+  fw[0] = e8_one();
+  E8 oinv = e8_inv(omega);
+  for (uint64_t b = 1; b < blocks; b++) fw[b] = fw[b - 1] * oinv;
+  std::array<E8, 8> coefs;
+  for (int img = 0; img < 8; img++) {
+    E8 s{}, f = e8_one();
+    for (uint64_t b = 0; b < blocks; b++) { s += f * e8_frob(high[b], (8 - img) % 8); f = f * omega; }
+    coefs[img] = powers[img] * s;
+  }
+  std::array<E8, 8> images;
+  for (int lane = 0; lane < 8; lane++) {
+    E8 s{};
+    for (int img = 0; img < 8; img++) s += coefs[img] * e8_frob(e8_basis(lane), (8 - img) % 8);
+    images[lane] = s;
+  }
+  std::vector<E8> ehi, elo;
+  eq_tables(outer, ehi, elo);
+  DVec weights(plen), values(plen);
+  weight_images(c, weights.p, ehi, elo, images);
+  tm.projection_ms = ms_since(t0);
+  t0 = Clock::now();
+  block_fold(c, cs.d_bva, p.layout.active, p.block_len, blocks, fw, values.p, plen);
+  uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
+  DCode folded = alloc_code(R);
+  launch_commit_values(p.geom, values.p, folded.cw.p, p.alpha_device(p.pp.codes.size() - 1), folded.tree, c.st);
+  QF_HIP(hipGetLastError());
+  folded.root = download_root(c, folded.tree, R);
+  t.bytes(folded.root.data(), 32);
+  tm.folding_ms = ms_since(t0);
+  t0 = Clock::now();
+  std::vector<std::array<E8, 2>> rounds;
+  std::vector<E8> reduced;
+  E8 fval, fwt;
+  sumcheck(c, values, weights, claim, t, rounds, reduced, fval, fwt);
+  require(fwt == batch_weight(outer.data(), n_outer, reduced, coefs) && claim == fval * fwt, QF_ERR_INVALID_COMMITMENT,
+          "batch sumcheck terminal");
+  tm.sumcheck_ms = ms_since(t0);
+  t0 = Clock::now();
+  const char* bsd = "authenticated-codeword-fold-queries/v1";
+  t.bytes(bsd, std::strlen(bsd));
+  t.field(fval);
+  Digest seed = t.h.finish();
+  const QaCode& top = p.pp.codes.back();
+  uint64_t count = num_queries(top, p.pp.security_bits, R);
+  bool all = count == R;
+  std::vector<uint64_t> sel = sample_rows(count, R, seed);
+  Bytes binding;
+  put_tag(binding, "QAFOLDB1");
+  put_u64(binding, blocks);
+  DCode workspace;
+  Bytes payload;
+  for (uint64_t b = 0; b < blocks; b++) {
+    if (b == 0 || cs.roots[b] != cs.roots[b - 1]) {
+      DCode bc = block_code(c, p, cs, b, workspace);
+      payload.clear();
+      write_rows(c, payload, bc, all, sel);
+    }
+    binding.insert(binding.end(), payload.begin(), payload.end());
+  }
+  write_rows(c, binding, folded, all, sel);
+  tm.binding_ms = ms_since(t0);
+  t0 = Clock::now();
+  Bytes opening;
+  std::vector<Pending> pending;
+  E8 actual = open_with_reduction(c, p, folded, reduced, opening, pending);
+  require(actual == fval, QF_ERR_INVALID_COMMITMENT, "native opening value");
+  tm.native_ms = ms_since(t0);
+  t0 = Clock::now();
+  Bytes decide = prove_decide(c, p, pending);
+  tm.decide_ms = ms_since(t0);
+  Bytes w;
+  put_tag(w, "QAPKBAT1");
+  put_e8(w, claimed);
+  for (auto& v : ev) put_e8(w, v);
+  for (auto& v : pos) put_e8(w, v);
+  for (auto& v : neg) put_e8(w, v);
+  put_digest(w, folded.root);
+  for (auto& r : rounds) { put_e8(w, r[0]); put_e8(w, r[1]); }
+  put_e8(w, fval);
+  put_part(w, binding);
+  put_part(w, opening);
+  put_part(w, decide);
+  return w;
+}
+
+// packed.rs:294-393 (one block)
+Bytes open_single(DevCtx& c, const ParamsState& p, const CommittedState& cs, const std::vector<E8>& point,
+                  const Bytes& context, E8& claimed, ProveTimings& tm) {
+  auto t0 = Clock::now();
+  size_t n_outer = point.size() - 3;
+  std::vector<E8> outer(point.begin(), point.begin() + n_outer);
+  uint64_t plen = uint64_t(1) << n_outer;
+  auto lanes = block_lanes(c, p, cs.d_bva, p.layout.active, 1, outer)[0];
+  auto low = eq_host(point.data() + n_outer, 3);
+  claimed = E8{};
+  for (int j = 0; j < 8; j++) claimed += lanes[j] * low[j];
+  auto ev = conjugates(lanes);
+  require(projected_claim(ev, low) == claimed, QF_ERR_INVALID_COMMITMENT, "projected claim");
+  Packed t = packed_transcript(p, cs.roots[0], point, claimed, context);
+  for (auto& v : ev) t.field(v);
+  auto powers = powers8(t.challenge());
+  E8 claim{};
+  for (int i = 0; i < 8; i++) claim += powers[i] * ev[i];
+  std::array<E8, 8> images;
+  for (int j = 0; j < 8; j++) {
+    E8 s{};
+    for (int i = 0; i < 8; i++) s += powers[i] * e8_frob(e8_basis(j), (8 - i) % 8);
+    images[j] = s;
+  }
+  std::vector<E8> ehi, elo;
+  eq_tables(outer, ehi, elo);
+  DVec weights(plen), values(plen);
+  weight_images(c, weights.p, ehi, elo, images);
+  block_fold(c, cs.d_bva, p.layout.active, p.block_len, 1, {e8_one()}, values.p, plen);
+  tm.projection_ms = ms_since(t0);
+  t0 = Clock::now();
+  std::vector<std::array<E8, 2>> rounds;
+  std::vector<E8> reduced;
+  E8 fval, fwt;
+  sumcheck(c, values, weights, claim, t, rounds, reduced, fval, fwt);
+  require(fwt == batch_weight(point.data(), n_outer, reduced, powers) && claim == fval * fwt, QF_ERR_INVALID_COMMITMENT,
+          "sumcheck terminal");
+  tm.sumcheck_ms = ms_since(t0);
+  t0 = Clock::now();
+  DCode workspace;
+  DCode bc = block_code(c, p, cs, 0, workspace);
+  Bytes opening;
+  std::vector<Pending> pending;
+  E8 actual = open_with_reduction(c, p, bc, reduced, opening, pending);
+  require(actual == fval, QF_ERR_INVALID_COMMITMENT, "native opening value");
+  tm.native_ms = ms_since(t0);
+  t0 = Clock::now();
+  Bytes decide = prove_decide(c, p, pending);
+  tm.decide_ms = ms_since(t0);
+  Bytes w;
+  put_tag(w, "QA655PK1");
+  put_e8(w, claimed);
+  for (auto& v : ev) put_e8(w, v);
+  put_u64(w, rounds.size());
+  for (auto& r : rounds) { put_e8(w, r[0]); put_e8(w, r[1]); }
+  put_e8(w, fval);
+  put_part(w, opening);
+  put_part(w, decide);
+  return w;
+}
+
+}  // namespace
+
+std::string statement_json(const ParamsState& p, const CommittedState& c) {  // protocol.rs:33-40
+  std::string s = "{\"version\":2,\"config\":{\"chunk_log\":" + std::to_string(p.cfg_chunk_log) +
+                  ",\"packed_log\":" + std::to_string(p.packed_log) + ",\"query_rf_bits\":" +
+                  std::to_string(p.query_bits) + "},\"shapes\":[";
+  for (size_t i = 0; i < p.layout.shapes.size(); i++) {
+    if (i) s += ',';
+    s += "{\"rows\":" + std::to_string(p.layout.shapes[i].rows) + ",\"cols\":" + std::to_string(p.layout.shapes[i].cols) + "}";
+  }
+  s += "],\"parameter_digest\":";
+  js_bytes(s, p.pp.parameter_digest.data(), 32);
+  s += ",\"roots\":[";
+  for (size_t i = 0; i < c.roots.size(); i++) { if (i) s += ','; js_bytes(s, c.roots[i].data(), 32); }
+  return s + "]}";
+}
+
+std::string claims_json(const std::vector<ClaimIn>& claims) {  // tbv.rs:6-11
+  std::string s = "[";
+  for (size_t i = 0; i < claims.size(); i++) {
+    if (i) s += ',';
+    s += "{\"row\":" + js_scalars(claims[i].row) + ",\"col\":" + js_scalars(claims[i].col) + ",\"value\":";
+    js_e8(s, claims[i].value);
+    s += "}";
+  }
+  return s + "]";
+}
+
+// main.rs:188-209 with tbv.rs:143-185 evaluated on the device
+std::vector<ClaimIn> benchmark_claims(const ParamsState& p, const CommittedState& cs) {
+  DevCtx c;
+  Outer t;
+  t.object("benchmark-points", statement_json(p, cs));
+  std::vector<ClaimIn> claims;
+  for (auto& sh : p.layout.shapes) {
+    uint64_t rp = 1, cp = 1;
+    while (rp < sh.rows) rp <<= 1;
+    while (cp < sh.cols) cp <<= 1;
+    ClaimIn cl;
+    for (int i = 0; i < ilog2u(rp); i++) cl.row.push_back(t.challenge("row"));
+    for (int i = 0; i < ilog2u(cp); i++) cl.col.push_back(t.challenge("col"));
+    cl.value = E8{};
+    claims.push_back(cl);
+  }
+  Query q;
+  for (auto& b : p.layout.buckets) {
+    const ClaimIn& cl = claims[b.matrix];
+    Term tm;
+    for (int i = 0; i < ilog2u(b.cols); i++) tm.f.push_back({e8_one(), cl.col[i]});
+    for (int i = 0; i < ilog2u(b.rows); i++) tm.f.push_back({e8_one(), cl.row[i]});
+    tm.scale = monomial(b.row_offset, cl.row.data(), cl.row.size()) * monomial(b.col_offset, cl.col.data(), cl.col.size());
+    tm.offset = b.offset;
+    q.terms.push_back(std::move(tm));
+  }
+  std::vector<size_t> which;
+  auto proj = project_terms(c, q, cs.d_bva, p.layout.active, 16, which);
+  std::vector<bool> done(q.terms.size(), false);
+  for (size_t k = 0; k < which.size(); k++) {
+    const Term& tm = q.terms[which[k]];
+    auto low = tbv_w(tm.f.data(), 4);
+    E8 v{};
+    for (int i = 0; i < 16; i++) v += proj[k][i] * low[i];
+    claims[p.layout.buckets[which[k]].matrix].value += v;
+    done[which[k]] = true;
+  }
+  for (size_t i = 0; i < q.terms.size(); i++) {
+    const Term& tm = q.terms[i];
+    if (done[i] || tm.len() >= 16) continue;  // big terms past the active data project to zero
+    auto vals = read_bytes(c, cs.d_bva, p.layout.active, tm.offset, tm.len());
+    E8 v{};
+    for (uint64_t j = 0; j < tm.len(); j++) v += tm.weight(j) * small_byte(vals[j]);
+    claims[p.layout.buckets[i].matrix].value += v;
+  }
+  return claims;
+}
+
+Bytes prove(const ParamsState& p, const CommittedState& cs, const std::vector<ClaimIn>& claims, ProveTimings& tm) {
+  auto start = Clock::now();
+  require(claims.size() == p.layout.shapes.size(), QF_ERR_INVALID_PARAMETERS, "claim count mismatch");
+  for (size_t i = 0; i < claims.size(); i++) {
+    uint64_t rp = 1, cp = 1;
+    while (rp < p.layout.shapes[i].rows) rp <<= 1;
+    while (cp < p.layout.shapes[i].cols) cp <<= 1;
+    require(claims[i].row.size() == size_t(ilog2u(rp)) && claims[i].col.size() == size_t(ilog2u(cp)),
+            QF_ERR_INVALID_PARAMETERS, "point dimension mismatch");
+  }
+  require(cs.d_bva != nullptr, QF_ERR_ARGUMENT, "commitment has no resident BVA");
+  uint64_t s = uint64_t(1) << p.packed_log;
+  DevCtx c;
+  auto t0 = Clock::now();
+  Outer tr;
+  tr.object("collection", statement_json(p, cs));
+  tr.object("claims", claims_json(claims));
+  Query q = compile(p.layout, claims, tr);
+  tm.conversion_ms = ms_since(t0);
+  t0 = Clock::now();
+  EoOut eo = eo_prove(c, p.layout, cs.d_bva, s, q, tr);
+  tm.eo_ms = ms_since(t0);
+  Bytes context;
+  const char* cx = "parcomm/qafold/terminal/v2";
+  context.insert(context.end(), cx, cx + std::strlen(cx));
+  context.insert(context.end(), tr.state.begin(), tr.state.end());
+  put_u64(context, 0);
+  E8 actual;
+  Bytes wire = cs.roots.size() > 1 ? open_batch(c, p, cs, eo.point, context, actual, tm)
+                                   : open_single(c, p, cs, eo.point, context, actual, tm);
+  require(actual * eo.scale == eo.value, QF_ERR_INVALID_COMMITMENT, "QAFold/EO terminal mismatch");
+  Bytes out;  // bincode fixint (protocol.rs:57-66, 89-95)
+  put_u64(out, eo.pos.size());
+  for (auto& v : eo.pos) put_e8(out, v);
+  put_u64(out, eo.neg.size());
+  for (auto& v : eo.neg) put_e8(out, v);
+  put_u64(out, eo.rounds.size());
+  for (auto& r : eo.rounds) { put_e8(out, r[0]); put_e8(out, r[1]); }
+  put_e8(out, eo.value);
+  put_e8(out, eo.query);
+  put_u64(out, 1);
+  put_e8(out, actual);
+  put_u64(out, wire.size());
+  out.insert(out.end(), wire.begin(), wire.end());
+  require(out.size() <= (256ull << 20), QF_ERR_ENCODING, "oversize ParComm proof");
+  tm.total_ms = ms_since(start);
+  return out;
+}
+
+}  // namespace qf

@@ -1,0 +1,64 @@
+// Resident state behind the C ABI handles: public parameters (host setup plus
+// the device copies of the QA generators) and one commitment (per-block
+// codewords and Merkle trees kept in HBM, or roots only).
+#pragma once
+#include "qf_kernels.hpp"
+#include <algorithm>
+
+namespace qf {
+
+struct ParamsState {
+  Layout layout;
+  uint32_t cfg_chunk_log = 30, packed_log = 4, query_bits = 100;
+  int chunk_log = 0;
+  uint64_t block_len = 0, blocks = 0;
+  PcsParams pp;
+  CommitGeom geom{};
+  LeafPrefix leaf_ext{};
+  mutable E8* d_alpha = nullptr;              // top-code alpha_1, uploaded on first use
+  mutable std::vector<E8*> d_alpha_levels;    // alpha_1 of every level (top included), on first prove
+  mutable std::vector<uint64_t> zero_ranges;  // padding intervals (start, len) chunked for the device check
+  ~ParamsState() {
+    for (auto* a : d_alpha_levels) if (a && a != d_alpha) (void)hipFree(a);
+    if (d_alpha) (void)hipFree(d_alpha);
+  }
+  const E8* alpha_device(size_t level) const;
+};
+
+struct CommittedState {
+  const ParamsState* p = nullptr;
+  int cache_mode = 0;
+  std::vector<Digest> roots;
+  std::vector<E8*> d_cw;        // per block (cached) or one workspace
+  std::vector<uint32_t*> d_tree;
+  const int8_t* d_bva = nullptr;  // the committed BVA (active bytes) in HBM
+  bool owns_bva = false;
+  double total_ms = 0, leaf_ms = 0, encode_ms = 0;
+  ~CommittedState() {
+    std::vector<void*> freed;
+    auto release = [&](void* q) {
+      if (q && std::find(freed.begin(), freed.end(), q) == freed.end()) { freed.push_back(q); (void)hipFree(q); }
+    };
+    for (auto* q : d_cw) release(q);
+    for (auto* q : d_tree) release(q);
+    if (owns_bva) release(const_cast<int8_t*>(d_bva));
+  }
+};
+
+struct ProveTimings {
+  double conversion_ms = 0, eo_ms = 0, projection_ms = 0, folding_ms = 0, sumcheck_ms = 0, binding_ms = 0,
+         native_ms = 0, decide_ms = 0, total_ms = 0;
+};
+
+// Parameters::prove (protocol.rs:201-263) on the device-resident commitment.
+// `claims` holds n_matrices claims in the reference's order (row point,
+// column point, value). Returns the bincode proof bytes.
+struct ClaimIn { std::vector<E8> row, col; E8 value; };
+Bytes prove(const ParamsState& p, const CommittedState& c, const std::vector<ClaimIn>& claims, ProveTimings& t);
+// The benchmark statement of the reference CLI (main.rs:188-209): points from
+// the commitment transcript, values evaluated on the device BVA.
+std::vector<ClaimIn> benchmark_claims(const ParamsState& p, const CommittedState& c);
+std::string statement_json(const ParamsState& p, const CommittedState& c);
+std::string claims_json(const std::vector<ClaimIn>& claims);
+
+}  // namespace qf

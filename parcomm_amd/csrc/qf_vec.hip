@@ -1,0 +1,672 @@
+// Device vector operations and BVA-scale kernels of the opening (see qf_vec.hpp).
+#include "qf_vec.hpp"
+
+namespace qf {
+
+namespace {
+constexpr int WG = 256;
+constexpr uint64_t TAB_CAP = 1u << 18;  // E8 entries of the small-table scratch
+
+__device__ inline V8 v8_madd(const V8& acc, const V8& x) { return v8_add(acc, x); }
+
+// ---------------------------------------------------------------- reductions
+enum { R_INNER = 0, R_CROSS = 1, R_EO = 2, R_SC = 3 };
+
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_reduce(const E8* __restrict__ a, const E8* __restrict__ b, uint64_t n,
+                                               uint32_t* __restrict__ partials) {
+  __shared__ uint32_t red[WG][16];
+  V8 s0 = v8_zero(), s1 = v8_zero();
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG) {
+    if (MODE == R_INNER) {
+      s0 = v8_add(s0, v8_mul(v8_load(a[i]), v8_load(b[i])));
+    } else if (MODE == R_CROSS) {
+      s0 = v8_add(s0, v8_mul(v8_load(a[i]), v8_load(b[n + i])));
+      s1 = v8_add(s1, v8_mul(v8_load(a[n + i]), v8_load(b[i])));
+    } else if (MODE == R_EO) {
+      s0 = v8_add(s0, v8_mul(v8_load(a[2 * i]), v8_load(b[2 * i + 1])));
+      s1 = v8_add(s1, v8_mul(v8_load(a[2 * i + 1]), v8_load(b[2 * i])));
+    } else {
+      V8 vl = v8_load(a[i]), wl = v8_load(b[i]);
+      s0 = v8_add(s0, v8_mul(vl, wl));
+      s1 = v8_add(s1, v8_mul(v8_sub(v8_load(a[n + i]), vl), v8_sub(v8_load(b[n + i]), wl)));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++) { red[threadIdx.x][k] = s0.c[k]; red[threadIdx.x][8 + k] = s1.c[k]; }
+  __syncthreads();
+  for (int w = WG / 2; w > 0; w /= 2) {
+    if (int(threadIdx.x) < w)
+      for (int k = 0; k < 16; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + w][k]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 16) partials[blockIdx.x * 16 + threadIdx.x] = red[0][threadIdx.x];
+}
+
+uint32_t groups_for(uint64_t n) {
+  uint64_t g = (n + WG - 1) / WG;
+  if (g < 1) g = 1;
+  return uint32_t(g < DevCtx::MAX_GROUPS ? g : DevCtx::MAX_GROUPS);
+}
+
+std::pair<E8, E8> finish_partials(DevCtx& c, uint32_t groups) {
+  QF_HIP(hipMemcpyAsync(c.host_partials, c.partials, size_t(groups) * 64, hipMemcpyDeviceToHost, c.st));
+  c.sync();
+  V8 s0 = v8_zero(), s1 = v8_zero();
+  for (uint32_t g = 0; g < groups; g++)
+    for (int k = 0; k < 8; k++) {
+      s0.c[k] = madd(s0.c[k], c.host_partials[g * 16 + k]);
+      s1.c[k] = madd(s1.c[k], c.host_partials[g * 16 + 8 + k]);
+    }
+  return {v8_store(s0), v8_store(s1)};
+}
+
+template <int MODE>
+std::pair<E8, E8> reduce(DevCtx& c, const E8* a, const E8* b, uint64_t n) {
+  if (n == 0) return {E8{}, E8{}};
+  uint32_t g = groups_for(n);
+  hipLaunchKernelGGL(k_reduce<MODE>, dim3(g), dim3(WG), 0, c.st, a, b, n, c.partials);
+  QF_HIP(hipGetLastError());
+  return finish_partials(c, g);
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void k_fill(E8* p, uint64_t n, E8 v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG) p[i] = v;
+}
+__global__ void k_tensor(E8* out, uint64_t n, const E8* hi, const E8* lo, int a) {
+  uint64_t mask = (uint64_t(1) << a) - 1;
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
+    out[i] = v8_store(v8_mul(v8_load(hi[i >> a]), v8_load(lo[i & mask])));
+}
+__global__ void k_lb_fold(E8* a, E8* b, uint64_t h, E8 x, E8 y) {
+  V8 vx = v8_load(x), vy = v8_load(y);
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
+    a[i] = v8_store(v8_add(v8_load(a[i]), v8_mul(v8_load(a[h + i]), vx)));
+    b[i] = v8_store(v8_add(v8_load(b[i]), v8_mul(v8_load(b[h + i]), vy)));
+  }
+}
+__global__ void k_eo_fold(const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, E8 ri, E8 r) {
+  V8 vri = v8_load(ri), vr = v8_load(r);
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
+    V8 w0 = v8_load(w[2 * i]), w1 = v8_load(w[2 * i + 1]), q0 = v8_load(q[2 * i]), q1 = v8_load(q[2 * i + 1]);
+    ow[i] = v8_store(v8_add(w0, v8_mul(w1, vri)));
+    oq[i] = v8_store(v8_add(q0, v8_mul(q1, vr)));
+  }
+}
+__global__ void k_sc_fold(E8* v, E8* w, uint64_t h, E8 r) {
+  V8 vr = v8_load(r);
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
+    V8 vl = v8_load(v[i]), wl = v8_load(w[i]);
+    v[i] = v8_store(v8_add(vl, v8_mul(vr, v8_sub(v8_load(v[h + i]), vl))));
+    w[i] = v8_store(v8_add(wl, v8_mul(vr, v8_sub(v8_load(w[h + i]), wl))));
+  }
+}
+__global__ void k_axpy(E8* y, const E8* x, uint64_t n, E8 s) {
+  V8 vs = v8_load(s);
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
+    y[i] = v8_store(v8_add(v8_load(y[i]), v8_mul(vs, v8_load(x[i]))));
+}
+__global__ void k_mul(E8* y, const E8* x, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
+    y[i] = v8_store(v8_mul(v8_load(y[i]), v8_load(x[i])));
+}
+__global__ void k_scale(E8* y, uint64_t n, E8 s) {
+  V8 vs = v8_load(s);
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
+    y[i] = v8_store(v8_mul(v8_load(y[i]), vs));
+}
+// Column projection: grid (groups, cols); partials[(col * groups + g)][0..8)
+__global__ __launch_bounds__(WG) void k_col_projection(const E8* __restrict__ cw, uint64_t R, uint64_t n,
+                                                       const E8* __restrict__ w, uint32_t* __restrict__ partials) {
+  __shared__ uint32_t red[WG][8];
+  const E8* col = cw + uint64_t(blockIdx.y) * R;
+  V8 s = v8_zero();
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
+    s = v8_add(s, v8_mul(v8_load(col[i]), v8_load(w[i])));
+  for (int k = 0; k < 8; k++) red[threadIdx.x][k] = s.c[k];
+  __syncthreads();
+  for (int h = WG / 2; h > 0; h /= 2) {
+    if (int(threadIdx.x) < h)
+      for (int k = 0; k < 8; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + h][k]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) partials[(uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * 8 + threadIdx.x] = red[0][threadIdx.x];
+}
+__global__ __launch_bounds__(WG) void k_col_fold(const E8* __restrict__ cw, uint64_t R, uint64_t n, uint64_t cols,
+                                                 const E8* __restrict__ f, E8* __restrict__ out) {
+  for (uint64_t r = uint64_t(blockIdx.x) * WG + threadIdx.x; r < n; r += uint64_t(gridDim.x) * WG) {
+    V8 s = v8_zero();
+    for (uint64_t c = 0; c < cols; c++) s = v8_add(s, v8_mul(v8_load(f[c]), v8_load(cw[c * R + r])));
+    out[r] = v8_store(s);
+  }
+}
+__global__ void k_gather_rows(const E8* cw, uint64_t R, uint64_t cols, const uint64_t* rows, uint64_t count, E8* out) {
+  uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (i >= count * cols) return;
+  uint64_t k = i / cols, c = i % cols;
+  out[i] = cw[c * R + rows[k]];
+}
+__global__ void k_gather_paths(const uint32_t* tree, uint64_t R, int depth, const uint64_t* rows, uint64_t count,
+                               uint32_t* out) {
+  uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (i >= count * uint64_t(depth)) return;
+  uint64_t k = i / depth;
+  int l = int(i % depth);
+  uint64_t off = 0, cnt = R;
+  for (int j = 0; j < l; j++) { off += cnt; cnt >>= 1; }
+  uint64_t sib = ((rows[k] >> l) ^ 1) + off;
+  for (int w = 0; w < 8; w++) out[i * 8 + w] = tree[sib * 8 + w];
+}
+__global__ void k_scatter_add(E8* dst, const uint64_t* idx, const E8* vals, uint64_t n) {
+  uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (i >= n) return;
+  dst[idx[i]] = v8_store(v8_add(v8_load(dst[idx[i]]), v8_load(vals[i])));
+}
+
+// ---------------------------------------------------------------- BVA-scale kernels
+struct ProjTaskDev {
+  uint64_t base, rows, chunk_first;  // chunk_first: first row of this chunk
+  uint32_t nbits, pair_off, k, chunk_rows_log;
+  E8 scale;
+};
+
+__device__ inline uint32_t mod_signed(int64_t x) {
+  int64_t r = x % int64_t(P);
+  return uint32_t(r < 0 ? r + P : r);
+}
+
+// Builds T[i] = prod_{b<k} pairs[b][(i >> b) & 1] in LDS (k <= 12).
+__device__ inline void build_table(E8* T, const E8 (*pairs)[2], int k) {
+  if (threadIdx.x == 0) T[0] = v8_store(v8_zero()), T[0].c[0] = 1;
+  __syncthreads();
+  for (int b = 0; b < k; b++) {
+    const int half = 1 << b;
+    V8 p0 = v8_load(pairs[b][0]), p1 = v8_load(pairs[b][1]);
+    for (int i = threadIdx.x; i < half; i += WG) {
+      V8 t = v8_load(T[i]);
+      T[i + half] = v8_store(v8_mul(t, p1));
+      T[i] = v8_store(v8_mul(t, p0));
+    }
+    __syncthreads();
+  }
+}
+
+// Q (2 or 4) consecutive BVA bytes at `byte` as a little-endian word; bytes at
+// or past `available` read as zero. `byte` is a multiple of Q.
+template <int Q>
+__device__ inline uint32_t load_lanes(const int8_t* __restrict__ bva, uint64_t byte, uint64_t available) {
+  if (byte + Q <= available) {
+    if constexpr (Q == 4) return *reinterpret_cast<const uint32_t*>(bva + byte);
+    else return *reinterpret_cast<const uint16_t*>(bva + byte);
+  }
+  uint32_t w = 0;
+  for (int j = 0; j < Q; j++) if (byte + j < available) w |= uint32_t(uint8_t(bva[byte + j])) << (8 * j);
+  return w;
+}
+
+// One workgroup = one chunk of one projection task (2^chunk_rows_log rows).
+template <int L>
+__global__ __launch_bounds__(WG) void k_tensor_project(const int8_t* __restrict__ bva, uint64_t available,
+                                                       const ProjTaskDev* __restrict__ tasks,
+                                                       const E8 (*__restrict__ pairs)[2], E8* __restrict__ out) {
+  constexpr int Q = L < 4 ? L : 4;       // lanes per thread
+  constexpr int TPR = L / Q;             // threads per row
+  constexpr int RPP = WG / TPR;          // rows per pass
+  __shared__ E8 T[4096];
+  __shared__ E8 C;
+  const ProjTaskDev t = tasks[blockIdx.x];
+  const E8(*tp)[2] = pairs + t.pair_off;
+  build_table(T, tp, int(t.k));
+  const int q = threadIdx.x % TPR;
+  const int r0 = threadIdx.x / TPR;
+  const uint64_t lo_rows = uint64_t(1) << t.k;
+  const uint64_t chunk_rows = uint64_t(1) << t.chunk_rows_log;
+  V8 acc[Q];
+  for (int l = 0; l < Q; l++) acc[l] = v8_zero();
+  for (uint64_t hs = 0; hs < chunk_rows; hs += lo_rows) {
+    const uint64_t row_hi = t.chunk_first + hs;  // first row of this hi group
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      V8 c = v8_load(t.scale);
+      for (uint32_t b = t.k; b < t.nbits; b++) c = v8_mul(c, v8_load(tp[b][(row_hi >> b) & 1]));
+      C = v8_store(c);
+    }
+    __syncthreads();
+    int32_t ai[Q][8];
+    for (int l = 0; l < Q; l++)
+      for (int k = 0; k < 8; k++) ai[l][k] = 0;
+    for (uint64_t lo = r0; lo < lo_rows; lo += RPP) {
+      const uint32_t w = load_lanes<Q>(bva, t.base + (row_hi + lo) * L + Q * q, available);
+      const E8 wt = T[lo];
+#pragma unroll
+      for (int l = 0; l < Q; l++) {
+        const int32_t v = int8_t(w >> (8 * l));
+#pragma unroll
+        for (int k = 0; k < 8; k++) ai[l][k] += v * int32_t(wt.c[k]);
+      }
+    }
+    V8 c = v8_load(C);
+#pragma unroll
+    for (int l = 0; l < Q; l++) {
+      V8 x;
+#pragma unroll
+      for (int k = 0; k < 8; k++) x.c[k] = mod_signed(ai[l][k]);
+      acc[l] = v8_add(acc[l], v8_mul(x, c));
+    }
+  }
+  __syncthreads();
+  // reduce acc over threads sharing the same lane quad; reuse the table as scratch
+  uint32_t* red = reinterpret_cast<uint32_t*>(T);  // [WG][8 Q]
+  constexpr int RW = 8 * Q;
+  for (int l = 0; l < Q; l++)
+    for (int k = 0; k < 8; k++) red[threadIdx.x * RW + l * 8 + k] = acc[l].c[k];
+  __syncthreads();
+  for (int h = WG / 2; h >= TPR; h /= 2) {
+    if (int(threadIdx.x) < h)
+      for (int j = 0; j < RW; j++) red[threadIdx.x * RW + j] = madd(red[threadIdx.x * RW + j], red[(threadIdx.x + h) * RW + j]);
+    __syncthreads();
+  }
+  if (int(threadIdx.x) < TPR) {
+    for (int l = 0; l < Q; l++) {
+      V8 x;
+      for (int k = 0; k < 8; k++) x.c[k] = red[threadIdx.x * RW + l * 8 + k];
+      out[uint64_t(blockIdx.x) * L + Q * threadIdx.x + l] = v8_store(x);
+    }
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(WG) void k_eo_witness(const int8_t* __restrict__ bva, uint64_t available, uint64_t n_rows,
+                                                   const E8* __restrict__ pinv_g, E8* __restrict__ w) {
+  __shared__ E8 pinv[S];
+  for (int i = threadIdx.x; i < S; i += WG) pinv[i] = pinv_g[i];
+  __syncthreads();
+  for (uint64_t row = uint64_t(blockIdx.x) * WG + threadIdx.x; row < n_rows; row += uint64_t(gridDim.x) * WG) {
+    int32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int Q = S < 4 ? S : 4;
+    const uint64_t byte = row * S;
+    for (int g = 0; g < S / Q; g++) {
+      const uint32_t x = load_lanes<Q>(bva, byte + Q * g, available);
+#pragma unroll
+      for (int j = 0; j < Q; j++) {
+        const int32_t v = int8_t(x >> (8 * j));
+        const E8 p = pinv[Q * g + j];
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc[k] += v * int32_t(p.c[k]);
+      }
+    }
+    V8 o;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o.c[k] = mod_signed(acc[k]);
+    w[row] = v8_store(o);
+  }
+}
+
+struct FillTaskDev { uint64_t start, first; uint32_t nbits, pair_off, k, count_log; E8 scale; };
+// out[start + j] += scale * W(j), j in [first, first + 2^count_log)
+__global__ __launch_bounds__(WG) void k_tensor_fill_add(E8* __restrict__ out, const FillTaskDev* __restrict__ tasks,
+                                                        const E8 (*__restrict__ pairs)[2]) {
+  __shared__ E8 T[4096];
+  __shared__ E8 C;
+  const FillTaskDev t = tasks[blockIdx.x];
+  const E8(*tp)[2] = pairs + t.pair_off;
+  build_table(T, tp, int(t.k));
+  const uint64_t lo_n = uint64_t(1) << t.k, count = uint64_t(1) << t.count_log;
+  for (uint64_t hs = 0; hs < count; hs += lo_n) {
+    const uint64_t j_hi = t.first + hs;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      V8 c = v8_load(t.scale);
+      for (uint32_t b = t.k; b < t.nbits; b++) c = v8_mul(c, v8_load(tp[b][(j_hi >> b) & 1]));
+      C = v8_store(c);
+    }
+    __syncthreads();
+    const V8 c = v8_load(C);
+    for (uint64_t lo = threadIdx.x; lo < lo_n; lo += WG) {
+      E8* o = out + t.start + j_hi + lo;
+      *o = v8_store(v8_add(v8_load(*o), v8_mul(c, v8_load(T[lo]))));
+    }
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_block_fold(const int8_t* __restrict__ bva, uint64_t available,
+                                                   uint64_t block_len, uint32_t blocks, const E8* __restrict__ fw_g,
+                                                   E8* __restrict__ values, uint64_t n) {
+  __shared__ E8 fw[128];
+  for (uint32_t i = threadIdx.x; i < blocks; i += WG) fw[i] = fw_g[i];
+  __syncthreads();
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG) {
+    int32_t S[8][8];
+    for (int j = 0; j < 8; j++)
+      for (int k = 0; k < 8; k++) S[j][k] = 0;
+    for (uint32_t b = 0; b < blocks; b++) {
+      const uint64_t byte = uint64_t(b) * block_len + 8 * i;
+      uint32_t x[2] = {0, 0};
+      if (byte + 8 <= available) {
+        uint2 u = *reinterpret_cast<const uint2*>(bva + byte);
+        x[0] = u.x;
+        x[1] = u.y;
+      } else {
+        for (int j = 0; j < 8; j++) if (byte + j < available) x[j >> 2] |= uint32_t(uint8_t(bva[byte + j])) << (8 * (j & 3));
+      }
+      const E8 f = fw[b];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int32_t v = int8_t(x[j >> 2] >> (8 * (j & 3)));
+#pragma unroll
+        for (int k = 0; k < 8; k++) S[j][k] += v * int32_t(f.c[k]);
+      }
+    }
+    // value = sum_j X^j * S_j with X^8 = 17
+    uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint32_t s = mod_signed(S[j][k]);
+        const int d = j + k;
+        if (d < 8) acc[d] += s; else acc[d - 8] += 17ull * s;
+      }
+    V8 o;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o.c[k] = mred64(acc[k]);
+    values[i] = v8_store(o);
+  }
+}
+
+struct Images { E8 m[8]; };
+__global__ __launch_bounds__(WG) void k_weight_images(E8* __restrict__ out, uint64_t n, const E8* __restrict__ hi,
+                                                      const E8* __restrict__ lo, int a, Images im) {
+  const uint64_t mask = (uint64_t(1) << a) - 1;
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG) {
+    V8 e = v8_mul(v8_load(hi[i >> a]), v8_load(lo[i & mask]));
+    uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+#pragma unroll
+      for (int k = 0; k < 8; k++) acc[k] += uint64_t(im.m[j].c[k]) * e.c[j];
+    V8 o;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o.c[k] = mred64(acc[k]);
+    out[i] = v8_store(o);
+  }
+}
+
+dim3 grid_for(uint64_t n) {
+  uint64_t g = (n + WG - 1) / WG;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return dim3(uint32_t(g));
+}
+
+// Small host tables go to scratch in stream order; the synchronous copy keeps
+// the host buffer valid only for the duration of the call.
+E8* put_table(DevCtx& c, E8* scratch, const void* host, size_t bytes) {
+  QF_HIP(hipMemcpyAsync(scratch, host, bytes, hipMemcpyHostToDevice, c.st));
+  c.sync();
+  return scratch;
+}
+
+struct Scratch {
+  E8* tab = nullptr;
+  ~Scratch() { if (tab) (void)hipFree(tab); }
+};
+thread_local Scratch g_scratch;
+E8* scratch_tab() {
+  if (!g_scratch.tab) QF_HIP(hipMalloc(&g_scratch.tab, TAB_CAP * 16));
+  return g_scratch.tab;
+}
+}  // namespace
+
+// ---------------------------------------------------------------- DVec / DevCtx
+DVec::DVec(uint64_t len) : n(len) { QF_HIP(hipMalloc(&p, std::max<uint64_t>(len, 1) * 16)); }
+DVec& DVec::operator=(DVec&& o) noexcept {
+  if (this != &o) { if (p) (void)hipFree(p); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+  return *this;
+}
+DVec::~DVec() { if (p) (void)hipFree(p); }
+DevCtx::DevCtx() {
+  QF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  QF_HIP(hipMalloc(&partials, size_t(MAX_GROUPS) * 64 * 32));
+  QF_HIP(hipHostMalloc(&host_partials, size_t(MAX_GROUPS) * 64 * 32, 0));
+}
+DevCtx::~DevCtx() {
+  if (partials) (void)hipFree(partials);
+  if (host_partials) (void)hipHostFree(host_partials);
+  if (st) (void)hipStreamDestroy(st);
+}
+void DevCtx::sync() { QF_HIP(hipStreamSynchronize(st)); }
+
+namespace dv {
+DVec upload(DevCtx& c, const std::vector<E8>& v) {
+  DVec d(v.size());
+  if (!v.empty()) QF_HIP(hipMemcpyAsync(d.p, v.data(), v.size() * 16, hipMemcpyHostToDevice, c.st));
+  c.sync();
+  return d;
+}
+std::vector<E8> download(DevCtx& c, const E8* p, uint64_t n) {
+  std::vector<E8> v(n);
+  if (n) QF_HIP(hipMemcpyAsync(v.data(), p, n * 16, hipMemcpyDeviceToHost, c.st));
+  c.sync();
+  return v;
+}
+void fill(DevCtx& c, E8* p, uint64_t n, const E8& v) { hipLaunchKernelGGL(k_fill, grid_for(n), dim3(WG), 0, c.st, p, n, v); }
+void copy(DevCtx& c, E8* dst, const E8* src, uint64_t n) {
+  if (n) QF_HIP(hipMemcpyAsync(dst, src, n * 16, hipMemcpyDeviceToDevice, c.st));
+}
+void tensor(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo) {
+  require(pow2(hi.size()) && pow2(lo.size()) && hi.size() + lo.size() <= TAB_CAP, QF_ERR_ARGUMENT, "tensor tables");
+  E8* t = scratch_tab();
+  put_table(c, t, hi.data(), hi.size() * 16);
+  put_table(c, t + hi.size(), lo.data(), lo.size() * 16);
+  uint64_t n = hi.size() * lo.size();
+  hipLaunchKernelGGL(k_tensor, grid_for(n), dim3(WG), 0, c.st, out, n, t, t + hi.size(), ilog2u(lo.size()));
+  c.sync();  // the scratch tables are reused by the next call
+}
+E8 inner(DevCtx& c, const E8* a, const E8* b, uint64_t n) { return reduce<R_INNER>(c, a, b, n).first; }
+std::pair<E8, E8> cross_halves(DevCtx& c, const E8* a, const E8* b, uint64_t h) { return reduce<R_CROSS>(c, a, b, h); }
+void lb_fold(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y) {
+  hipLaunchKernelGGL(k_lb_fold, grid_for(h), dim3(WG), 0, c.st, a, b, h, x, y);
+}
+std::pair<E8, E8> eo_cross(DevCtx& c, const E8* w, const E8* q, uint64_t h) { return reduce<R_EO>(c, w, q, h); }
+void eo_fold(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri, const E8& r) {
+  hipLaunchKernelGGL(k_eo_fold, grid_for(h), dim3(WG), 0, c.st, w, q, ow, oq, h, ri, r);
+}
+std::pair<E8, E8> sc_msg(DevCtx& c, const E8* v, const E8* w, uint64_t h) { return reduce<R_SC>(c, v, w, h); }
+void sc_fold(DevCtx& c, E8* v, E8* w, uint64_t h, const E8& r) {
+  hipLaunchKernelGGL(k_sc_fold, grid_for(h), dim3(WG), 0, c.st, v, w, h, r);
+}
+void axpy(DevCtx& c, E8* y, const E8* x, uint64_t n, const E8& s) {
+  hipLaunchKernelGGL(k_axpy, grid_for(n), dim3(WG), 0, c.st, y, x, n, s);
+}
+void mul(DevCtx& c, E8* y, const E8* x, uint64_t n) { hipLaunchKernelGGL(k_mul, grid_for(n), dim3(WG), 0, c.st, y, x, n); }
+void scale(DevCtx& c, E8* y, uint64_t n, const E8& s) { hipLaunchKernelGGL(k_scale, grid_for(n), dim3(WG), 0, c.st, y, n, s); }
+
+std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const E8* w) {
+  require(cols <= 64, QF_ERR_ARGUMENT, "column projection width");
+  uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(DevCtx::MAX_GROUPS / uint32_t(cols), groups_for(n)));
+  hipLaunchKernelGGL(k_col_projection, dim3(g, uint32_t(cols)), dim3(WG), 0, c.st, cw, R, n, w, c.partials);
+  QF_HIP(hipGetLastError());
+  QF_HIP(hipMemcpyAsync(c.host_partials, c.partials, size_t(g) * cols * 32, hipMemcpyDeviceToHost, c.st));
+  c.sync();
+  std::vector<E8> out(cols);
+  for (uint64_t col = 0; col < cols; col++) {
+    V8 s = v8_zero();
+    for (uint32_t i = 0; i < g; i++)
+      for (int k = 0; k < 8; k++) s.c[k] = madd(s.c[k], c.host_partials[(col * g + i) * 8 + k]);
+    out[col] = v8_store(s);
+  }
+  return out;
+}
+void col_fold(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& f, E8* out) {
+  E8* t = put_table(c, scratch_tab(), f.data(), f.size() * 16);
+  hipLaunchKernelGGL(k_col_fold, grid_for(n), dim3(WG), 0, c.st, cw, R, n, cols, t, out);
+  c.sync();
+}
+std::vector<E8> gather_rows(DevCtx& c, const E8* cw, uint64_t R, uint64_t cols, const std::vector<uint64_t>& rows) {
+  if (rows.empty()) return {};
+  uint64_t* d_rows = nullptr;
+  QF_HIP(hipMalloc(&d_rows, rows.size() * 8));
+  DVec tmp(rows.size() * cols);
+  QF_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, c.st));
+  uint64_t total = rows.size() * cols;
+  hipLaunchKernelGGL(k_gather_rows, dim3(uint32_t((total + WG - 1) / WG)), dim3(WG), 0, c.st, cw, R, cols, d_rows,
+                     uint64_t(rows.size()), tmp.p);
+  auto out = download(c, tmp.p, total);
+  (void)hipFree(d_rows);
+  return out;
+}
+std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, uint64_t R,
+                                              const std::vector<uint64_t>& rows) {
+  int depth = ilog2u(R);
+  std::vector<std::vector<Digest>> out(rows.size());
+  if (rows.empty() || depth == 0) return out;
+  uint64_t* d_rows = nullptr;
+  uint32_t* d_out = nullptr;
+  uint64_t total = rows.size() * depth;
+  QF_HIP(hipMalloc(&d_rows, rows.size() * 8));
+  QF_HIP(hipMalloc(&d_out, total * 32));
+  QF_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, c.st));
+  hipLaunchKernelGGL(k_gather_paths, dim3(uint32_t((total + WG - 1) / WG)), dim3(WG), 0, c.st, tree, R, depth, d_rows,
+                     uint64_t(rows.size()), d_out);
+  std::vector<uint8_t> host(total * 32);
+  QF_HIP(hipMemcpyAsync(host.data(), d_out, total * 32, hipMemcpyDeviceToHost, c.st));
+  c.sync();
+  (void)hipFree(d_rows);
+  (void)hipFree(d_out);
+  for (size_t k = 0; k < rows.size(); k++) {
+    out[k].resize(depth);
+    for (int l = 0; l < depth; l++) std::memcpy(out[k][l].data(), host.data() + (k * depth + l) * 32, 32);
+  }
+  return out;
+}
+void scatter_add(DevCtx& c, E8* dst, const std::vector<uint64_t>& idx, const std::vector<E8>& vals) {
+  if (idx.empty()) return;
+  uint64_t* d_idx = nullptr;
+  QF_HIP(hipMalloc(&d_idx, idx.size() * 8));
+  DVec v = upload(c, vals);
+  QF_HIP(hipMemcpyAsync(d_idx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, c.st));
+  hipLaunchKernelGGL(k_scatter_add, dim3(uint32_t((idx.size() + WG - 1) / WG)), dim3(WG), 0, c.st, dst, d_idx, v.p,
+                     uint64_t(idx.size()));
+  c.sync();
+  (void)hipFree(d_idx);
+}
+}  // namespace dv
+
+// ---------------------------------------------------------------- BVA-scale wrappers
+static void upload_pairs(DevCtx& c, const std::vector<std::array<E8, 2>>& pairs, E8 (**out)[2]) {
+  QF_HIP(hipMalloc(reinterpret_cast<void**>(out), std::max<size_t>(pairs.size(), 1) * 32));
+  if (!pairs.empty()) QF_HIP(hipMemcpyAsync(*out, pairs.data(), pairs.size() * 32, hipMemcpyHostToDevice, c.st));
+}
+
+std::vector<E8> tensor_project(DevCtx& c, const int8_t* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
+                               const std::vector<std::array<E8, 2>>& pairs) {
+  require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
+  std::vector<ProjTaskDev> chunks;
+  std::vector<size_t> owner;
+  for (size_t i = 0; i < tasks.size(); i++) {
+    const ProjTask& t = tasks[i];
+    require(pow2(t.rows) && uint64_t(1) << t.nbits == t.rows, QF_ERR_ARGUMENT, "projection task rows");
+    uint32_t k = std::min<uint32_t>(t.nbits, 12);
+    uint32_t clog = std::min<uint32_t>(t.nbits, 16);
+    for (uint64_t first = 0; first < t.rows; first += uint64_t(1) << clog) {
+      if (t.base + first * L >= available) break;  // rows past the active data contribute zero
+      chunks.push_back({t.base, t.rows, first, t.nbits, t.pair_off, k, clog, t.scale});
+      owner.push_back(i);
+    }
+  }
+  std::vector<E8> result(tasks.size() * L, E8{});
+  if (chunks.empty()) return result;
+  E8(*d_pairs)[2] = nullptr;
+  ProjTaskDev* d_tasks = nullptr;
+  upload_pairs(c, pairs, &d_pairs);
+  QF_HIP(hipMalloc(&d_tasks, chunks.size() * sizeof(ProjTaskDev)));
+  QF_HIP(hipMemcpyAsync(d_tasks, chunks.data(), chunks.size() * sizeof(ProjTaskDev), hipMemcpyHostToDevice, c.st));
+  DVec out(chunks.size() * L);
+  for (size_t first = 0; first < chunks.size(); first += 65535) {
+    uint32_t n = uint32_t(std::min<size_t>(65535, chunks.size() - first));
+    auto* tk = d_tasks + first;
+    auto* pr = (const E8(*)[2])d_pairs;
+    E8* o = out.p + first * L;
+    switch (L) {
+      case 2: hipLaunchKernelGGL(k_tensor_project<2>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+      case 4: hipLaunchKernelGGL(k_tensor_project<4>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+      case 8: hipLaunchKernelGGL(k_tensor_project<8>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+      case 16: hipLaunchKernelGGL(k_tensor_project<16>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+      case 32: hipLaunchKernelGGL(k_tensor_project<32>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+      default: hipLaunchKernelGGL(k_tensor_project<64>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+    }
+  }
+  QF_HIP(hipGetLastError());
+  auto parts = dv::download(c, out);
+  (void)hipFree(d_pairs);
+  (void)hipFree(d_tasks);
+  for (size_t ch = 0; ch < chunks.size(); ch++)
+    for (int l = 0; l < L; l++) result[owner[ch] * L + l] = e8_add(result[owner[ch] * L + l], parts[ch * L + l]);
+  return result;
+}
+
+void eo_witness(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
+                E8* w) {
+  require(s >= 2 && s <= 64 && pow2(s) && int(pinv.size()) == s, QF_ERR_ARGUMENT, "witness stride");
+  E8* t = put_table(c, scratch_tab(), pinv.data(), pinv.size() * 16);
+  switch (s) {
+    case 2: hipLaunchKernelGGL(k_eo_witness<2>, grid_for(n_rows), dim3(WG), 0, c.st, bva, available, n_rows, t, w); break;
+    case 4: hipLaunchKernelGGL(k_eo_witness<4>, grid_for(n_rows), dim3(WG), 0, c.st, bva, available, n_rows, t, w); break;
+    case 8: hipLaunchKernelGGL(k_eo_witness<8>, grid_for(n_rows), dim3(WG), 0, c.st, bva, available, n_rows, t, w); break;
+    case 16: hipLaunchKernelGGL(k_eo_witness<16>, grid_for(n_rows), dim3(WG), 0, c.st, bva, available, n_rows, t, w); break;
+    case 32: hipLaunchKernelGGL(k_eo_witness<32>, grid_for(n_rows), dim3(WG), 0, c.st, bva, available, n_rows, t, w); break;
+    default: hipLaunchKernelGGL(k_eo_witness<64>, grid_for(n_rows), dim3(WG), 0, c.st, bva, available, n_rows, t, w); break;
+  }
+  QF_HIP(hipGetLastError());
+  c.sync();
+}
+
+void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, const std::vector<std::array<E8, 2>>& pairs) {
+  std::vector<FillTaskDev> chunks;
+  for (const FillTask& t : tasks) {
+    uint32_t k = std::min<uint32_t>(t.nbits, 12);
+    uint32_t clog = std::min<uint32_t>(t.nbits, 16);
+    for (uint64_t first = 0; first < (uint64_t(1) << t.nbits); first += uint64_t(1) << clog)
+      chunks.push_back({t.start, first, t.nbits, t.pair_off, k, clog, t.scale});
+  }
+  if (chunks.empty()) return;
+  E8(*d_pairs)[2] = nullptr;
+  FillTaskDev* d_tasks = nullptr;
+  upload_pairs(c, pairs, &d_pairs);
+  QF_HIP(hipMalloc(&d_tasks, chunks.size() * sizeof(FillTaskDev)));
+  QF_HIP(hipMemcpyAsync(d_tasks, chunks.data(), chunks.size() * sizeof(FillTaskDev), hipMemcpyHostToDevice, c.st));
+  for (size_t first = 0; first < chunks.size(); first += 65535) {
+    uint32_t n = uint32_t(std::min<size_t>(65535, chunks.size() - first));
+    hipLaunchKernelGGL(k_tensor_fill_add, dim3(n), dim3(WG), 0, c.st, out, d_tasks + first, (const E8(*)[2])d_pairs);
+  }
+  QF_HIP(hipGetLastError());
+  c.sync();
+  (void)hipFree(d_pairs);
+  (void)hipFree(d_tasks);
+}
+
+void block_fold(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
+                const std::vector<E8>& fw, E8* values, uint64_t n) {
+  require(blocks <= 128 && fw.size() == blocks, QF_ERR_ARGUMENT, "block fold weights");
+  E8* t = put_table(c, scratch_tab(), fw.data(), fw.size() * 16);
+  hipLaunchKernelGGL(k_block_fold, grid_for(n), dim3(WG), 0, c.st, bva, available, block_len, uint32_t(blocks), t,
+                     values, n);
+  QF_HIP(hipGetLastError());
+  c.sync();
+}
+
+void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo, const std::array<E8, 8>& images) {
+  E8* t = scratch_tab();
+  put_table(c, t, hi.data(), hi.size() * 16);
+  put_table(c, t + hi.size(), lo.data(), lo.size() * 16);
+  Images im;
+  for (int j = 0; j < 8; j++) im.m[j] = images[j];
+  uint64_t n = hi.size() * lo.size();
+  hipLaunchKernelGGL(k_weight_images, grid_for(n), dim3(WG), 0, c.st, out, n, t, t + hi.size(), ilog2u(lo.size()), im);
+  QF_HIP(hipGetLastError());
+  c.sync();
+}
+
+}  // namespace qf

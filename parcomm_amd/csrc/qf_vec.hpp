@@ -1,0 +1,99 @@
+// Device vectors of Ext8 elements and the vector operations of the opening
+// (LiteBullet, even/odd and sumcheck rounds, tensor tables, codeword column
+// projections). Reductions return their result to the host: each workgroup
+// writes a partial sum and the host adds at most a few thousand partials.
+#pragma once
+#include "qf_kernels.hpp"
+#include <memory>
+#include <utility>
+
+namespace qf {
+
+// Owning device buffer of `n` Ext8 elements.
+struct DVec {
+  E8* p = nullptr;
+  uint64_t n = 0;
+  DVec() = default;
+  explicit DVec(uint64_t len);
+  DVec(const DVec&) = delete;
+  DVec& operator=(const DVec&) = delete;
+  DVec(DVec&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DVec& operator=(DVec&& o) noexcept;
+  ~DVec();
+  void resize_view(uint64_t len) { n = len; }  // shrink the logical length (storage kept)
+};
+
+// Scratch for reduction partials, owned by the prover of one proof.
+struct DevCtx {
+  hipStream_t st = nullptr;
+  uint32_t* partials = nullptr;  // [max_groups][2][8] u32
+  uint32_t* host_partials = nullptr;
+  static constexpr uint32_t MAX_GROUPS = 1024;
+  DevCtx();
+  ~DevCtx();
+  void sync();
+};
+
+namespace dv {
+DVec upload(DevCtx& c, const std::vector<E8>& v);
+std::vector<E8> download(DevCtx& c, const E8* p, uint64_t n);
+inline std::vector<E8> download(DevCtx& c, const DVec& v) { return download(c, v.p, v.n); }
+void fill(DevCtx& c, E8* p, uint64_t n, const E8& value);
+void copy(DevCtx& c, E8* dst, const E8* src, uint64_t n);
+// out[i] = hi[i >> a] * lo[i & (2^a - 1)]  (tensor product of two host tables)
+void tensor(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo);
+E8 inner(DevCtx& c, const E8* a, const E8* b, uint64_t n);
+// (sum_i a[i] b[h+i], sum_i a[h+i] b[i]), i < h
+std::pair<E8, E8> cross_halves(DevCtx& c, const E8* a, const E8* b, uint64_t h);
+// a[i] += a[h+i] x; b[i] += b[h+i] y
+void lb_fold(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y);
+// (sum w[2i] q[2i+1], sum w[2i+1] q[2i]), i < h
+std::pair<E8, E8> eo_cross(DevCtx& c, const E8* w, const E8* q, uint64_t h);
+// ow[i] = w[2i] + w[2i+1] ri; oq[i] = q[2i] + q[2i+1] r
+void eo_fold(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri, const E8& r);
+// (sum v_lo w_lo, sum (v_hi - v_lo)(w_hi - w_lo))
+std::pair<E8, E8> sc_msg(DevCtx& c, const E8* v, const E8* w, uint64_t h);
+// v_lo += r (v_hi - v_lo), same for w
+void sc_fold(DevCtx& c, E8* v, E8* w, uint64_t h, const E8& r);
+void axpy(DevCtx& c, E8* y, const E8* x, uint64_t n, const E8& s);  // y += s x
+void mul(DevCtx& c, E8* y, const E8* x, uint64_t n);                // y *= x
+void scale(DevCtx& c, E8* y, uint64_t n, const E8& s);              // y *= s
+// out[col] = sum_{r < n} cw[col*R + r] * w[r]
+std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const E8* w);
+// out[r] = sum_col f[col] * cw[col*R + r], r < n
+void col_fold(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& f, E8* out);
+// Gather full rows (cols entries each) of a column-major matrix: out[k*cols + col].
+std::vector<E8> gather_rows(DevCtx& c, const E8* cw, uint64_t R, uint64_t cols, const std::vector<uint64_t>& rows);
+// Merkle paths (siblings leaf to root) of `rows` in a device tree of R leaves.
+std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, uint64_t R,
+                                              const std::vector<uint64_t>& rows);
+void scatter_add(DevCtx& c, E8* dst, const std::vector<uint64_t>& idx, const std::vector<E8>& vals);
+}  // namespace dv
+
+// ---------------------------------------------------------------- BVA-scale kernels of the opening
+// One projection task: lanes[l] = sum_row W(row) * v[base + row*L + l] over `rows`
+// rows, W(row) = scale * prod_b pair_b[(row >> b) & 1] for the task's bit pairs
+// (LSB first). Bytes at or past `available` read as zero.
+struct ProjTask {
+  uint64_t base;      // byte offset of row 0
+  uint64_t rows;      // power of two
+  uint32_t nbits;     // log2(rows)
+  uint32_t pair_off;  // index of this task's first pair in the pair table
+  E8 scale;
+};
+// Returns tasks.size() x L lane sums.
+std::vector<E8> tensor_project(DevCtx& c, const int8_t* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
+                               const std::vector<std::array<E8, 2>>& pairs);
+// Packed EO witness: w[row] = sum_lane v[row*s + lane] * pinv[lane] for rows < n_rows.
+void eo_witness(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
+                E8* w);
+// out[start + j] += scale * prod_b pair_b[(j >> b) & 1], j < 2^nbits, per task (tasks disjoint).
+struct FillTask { uint64_t start; uint32_t nbits, pair_off; E8 scale; };
+void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, const std::vector<std::array<E8, 2>>& pairs);
+// values[i] = sum_b fw[b] * packed(b, i), packed(b,i) lanes = bytes (b*block_len + 8i + j)
+void block_fold(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
+                const std::vector<E8>& fw, E8* values, uint64_t n);
+// weights[i] = sum_j eq(i).c[j] * images[j], eq = hi (x) lo tables
+void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo, const std::array<E8, 8>& images);
+
+}  // namespace qf

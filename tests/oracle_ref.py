@@ -58,3 +58,23 @@ def synthetic_int8(shape_count, seed, sigma=20.0):
     """round(N(0, sigma)) clipped to [-128, 127], seeded."""
     rng = np.random.default_rng(seed)
     return np.clip(np.rint(rng.normal(0.0, sigma, size=shape_count)), -128, 127).astype(np.int8)
+
+
+def shapes_flat(shapes):
+    return (ctypes.c_uint64 * (2 * len(shapes)))(*[v for s in shapes for v in s])
+
+
+def oracle_prove(oracle, flat, n_shapes, chunk_log, packed_log, values):
+    """The reference CLI flow (main.rs:188-209) in the oracle: benchmark claims,
+    Parameters::prove, Proof::to_bytes. Returns (proof, statement JSON, claims JSON)."""
+    cap = 1 << 26
+    proof = (ctypes.c_uint8 * cap)()
+    plen = ctypes.c_size_t()
+    sj = ctypes.create_string_buffer(1 << 22)
+    cj = ctypes.create_string_buffer(1 << 22)
+    rc = oracle.orc_prove_benchmark(flat, ctypes.c_size_t(n_shapes), chunk_log, packed_log,
+                                    values.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(len(values)), proof,
+                                    ctypes.c_size_t(cap), ctypes.byref(plen), sj, ctypes.c_size_t(1 << 22), cj,
+                                    ctypes.c_size_t(1 << 22))
+    assert rc == 0, oracle.orc_last_error()
+    return bytes(proof[:plen.value]), sj.value, cj.value

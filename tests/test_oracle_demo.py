@@ -7,7 +7,7 @@ import hashlib
 
 import numpy as np
 
-from oracle_ref import DEMO_SHAPES
+from oracle_ref import DEMO_SHAPES, oracle_prove
 
 DEMO_PROOF_BLAKE2S = "821aecd9a71dea52b8a29dc9ef8a810431a0605c05b3ec8f1783b47e15d1fb46"
 
@@ -37,20 +37,6 @@ def demo_values(oracle):
         e = np.arange(r * c, dtype=np.int64)
         mats.append(((e * 73 + i * 19) % 256).astype(np.uint8).view(np.int8))
     return flat, oracle_encode(DEMO_SHAPES, buckets, active, mats)
-
-
-def oracle_prove(oracle, flat, n_shapes, chunk_log, packed_log, values):
-    cap = 1 << 26
-    proof = (ctypes.c_uint8 * cap)()
-    plen = ctypes.c_size_t()
-    sj = ctypes.create_string_buffer(1 << 22)
-    cj = ctypes.create_string_buffer(1 << 22)
-    rc = oracle.orc_prove_benchmark(flat, ctypes.c_size_t(n_shapes), chunk_log, packed_log,
-                                    values.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(len(values)), proof,
-                                    ctypes.c_size_t(cap), ctypes.byref(plen), sj, ctypes.c_size_t(1 << 22), cj,
-                                    ctypes.c_size_t(1 << 22))
-    assert rc == 0, oracle.orc_last_error()
-    return bytes(proof[:plen.value]), sj.value, cj.value
 
 
 def test_demo_proof_matches_reference_digest(oracle):
