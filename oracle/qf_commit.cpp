@@ -2,7 +2,7 @@
 //
 // Pinning: the layout is pinned by the reference's llama8b layout fixture
 // (qafold/f65521/tests/fixtures/llama8b-layout.json, checked by
-// tests/test_oracle_layout.py). WHT and Mobius are pinned by closed forms
+// tests/test_layout.py). WHT and Mobius are pinned by closed forms
 // (WHT o WHT = n I, Mobius inverse = zeta transform, brute force on tiny
 // inputs in tests/test_oracle_core.py). Merkle leaves/inner nodes are pinned by
 // hashlib.blake2s over the framed bytes. The QA generator sampling, setup

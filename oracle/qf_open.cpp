@@ -1,11 +1,21 @@
 // CPU oracle (test infrastructure only; see qf_core.hpp): the opening path.
 //
-// Pinning: the whole path is pinned end to end by the reference's recorded
-// demo proof (validation/qafold8-smoke-final/run.log: 26 974 bytes, BLAKE2s
-// 821aecd9...; tests/test_oracle_demo.py). That instance has one QA level, so
-// recursive extension rounds and multi-level Decide are not covered by it; the
-// prover's own algebraic checks (EO terminal, sumcheck terminal, LiteBullet
-// products, terminal MLE) run in every proof and throw on a mismatch.
+// Pinning:
+//   * ParComm composition (statement/claims JSON, TBV, EO, batched packed
+//     opening with block EO, fold and binding, wire formats): the reference's
+//     recorded demo proof (validation/qafold8-smoke-final/run.log: 26 974 bytes,
+//     BLAKE2s 821aecd9...; tests/test_oracle_demo.py). That instance has one QA
+//     level.
+//   * Recursive native-opening rounds, partial-row Merkle payloads, f64 query
+//     counts below the row count and the multi-level direct-interleaved Decide:
+//     the reference engine's wire fixture (qafold_short_pc.rs:3605-3647,
+//     BLAKE2s bb4993...; tests/test_oracle_demo.py) at n = 2^16.
+//   * Every proof also runs the prover's own algebraic checks (EO terminal,
+//     sumcheck terminal, LiteBullet products, terminal MLE) and throws on a
+//     mismatch.
+//   Parity unpinned: the exact bytes of proofs whose code sequences are deeper
+//   than both goldens (chunk_log >= 22), and the f64 query counts of those
+//   levels, rest on the composition of pinned pieces only.
 #include "qf_open.hpp"
 #include <cmath>
 #include <map>

@@ -243,7 +243,10 @@ class Parameters:
             import numpy as np
             arr = np.ascontiguousarray(values, dtype=np.int8)
             _check(lib().qf_commit(self._h, arr.ctypes.data, arr.size, mode, ctypes.byref(h)))
-        return Committed(self, h.value, self._statement(h))
+        committed = Committed(self, h.value, self._statement(h))
+        if isinstance(values, DeviceBuffer):
+            committed._bva = values  # borrowed by the handle: the opening reads it again
+        return committed
 
     def commit(self, values):
         """Parameters::commit (protocol.rs:131-134): codewords stay resident."""
