@@ -127,35 +127,13 @@ __global__ __launch_bounds__(WG) void k_pack_mobius(const int8_t* __restrict__ b
   for (int e = threadIdx.x; e < n; e += WG) cw[col * rows + row0 + e] = t[e];
 }
 
-// Mobius over the 5 column bits: for each message row, the 32 column entries.
-// Thread = (row, 32-bit word of the 16-byte element).
-__global__ __launch_bounds__(WG) void k_mobius_columns(E8* __restrict__ cw, uint64_t n, uint64_t rows) {
-  uint64_t gid = uint64_t(blockIdx.x) * WG + threadIdx.x;
-  uint64_t row = gid >> 2;
-  int word = int(gid & 3);
-  if (row >= n) return;
-  uint32_t* base = reinterpret_cast<uint32_t*>(cw) + row * 4 + word;
-  const uint64_t cstride = rows * 4;
-  uint32_t lo[32], hi[32];
-#pragma unroll
-  for (int k = 0; k < 32; k++) {
-    uint32_t w = base[k * cstride];
-    lo[k] = w & 0xffffu;
-    hi[k] = w >> 16;
-  }
-#pragma unroll
-  for (int h = 1; h < 32; h *= 2)
-#pragma unroll
-    for (int k = 0; k < 32; k++)
-      if (k & h) { lo[k] = msub(lo[k], lo[k ^ h]); hi[k] = msub(hi[k], hi[k ^ h]); }
-#pragma unroll
-  for (int k = 0; k < 32; k++) base[k * cstride] = lo[k] | (hi[k] << 16);
-}
-
 // Leaf hash of one 32-column codeword row: BLAKE2s(prefix || row bytes), with
 // the framed domain + width prefix of PLEN bytes (75 for base rows, 80 for
 // extension rows). The first 64 prefix bytes are compressed on the host.
-template <int PLEN>
+// With PRECOL the stored rows are the codeword before the Mobius transform over
+// the 5 column bits (that map acts within a row and commutes with encoding);
+// it is applied here, in registers, before hashing.
+template <int PLEN, bool PRECOL>
 __global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uint64_t rows, LeafPrefix lp,
                                                   uint32_t* __restrict__ leaves) {
   constexpr int TOTAL = PLEN + 512;
@@ -167,6 +145,21 @@ __global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uin
   for (int k = 0; k < 32; k++) {
     uint4 w = *reinterpret_cast<const uint4*>(cw + k * rows + r);
     R[4 * k] = w.x; R[4 * k + 1] = w.y; R[4 * k + 2] = w.z; R[4 * k + 3] = w.w;
+  }
+  if (PRECOL) {
+#pragma unroll
+    for (int word = 0; word < 4; word++) {
+      uint32_t lo[32], hi[32];
+#pragma unroll
+      for (int k = 0; k < 32; k++) { lo[k] = R[4 * k + word] & 0xffffu; hi[k] = R[4 * k + word] >> 16; }
+#pragma unroll
+      for (int h = 1; h < 32; h *= 2)
+#pragma unroll
+        for (int k = 0; k < 32; k++)
+          if (k & h) { lo[k] = msub(lo[k], lo[k ^ h]); hi[k] = msub(hi[k], hi[k ^ h]); }
+#pragma unroll
+      for (int k = 0; k < 32; k++) R[4 * k + word] = lo[k] | (hi[k] << 16);
+    }
   }
   R[128] = 0;
   uint32_t h[8];
@@ -275,8 +268,64 @@ __global__ void k_encode_one_row(E8* cw, uint64_t cols, const E8* alpha) {
   if (c < cols) cw[2 * c + 1] = v8_store(v8_mul(v8_load(cw[2 * c]), v8_load(alpha[0])));
 }
 
+// High index bits [12, L) in groups of at most 10 (tile passes with C = 4096 >> t contiguous positions).
+static std::vector<std::pair<int, int>> hi_groups(int L) {
+  std::vector<std::pair<int, int>> gs;
+  for (int s = 12; s < L;) { int t = L - s < 10 ? L - s : 10; gs.push_back({s, t}); s += t; }
+  return gs;
+}
+
+// Tile-pass schedules (qf_tile.hip); each returns false when the geometry needs
+// the generic LDS-stage passes instead (vectors shorter than a tile).
+static bool tiled_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream_t st) {
+  if (!tile_pass(TP_W, false, v, v, nullptr, nullptr, vectors, stride, stride, L, 0, std::min(L, 12), 0, 0, st))
+    return false;
+  for (auto& g : hi_groups(L)) tile_pass(TP_W, false, v, v, nullptr, nullptr, vectors, stride, stride, L, g.first, g.second, 0, 0, st);
+  return true;
+}
+// Second half of the parity, in place: WHT on the low bits, alpha, WHT on the
+// low bits (one pass), then WHT on the high groups. Precondition: the parity
+// region holds the message with the WHT already applied on every high group.
+static void tiled_parity_tail(E8* par, uint64_t cols, uint64_t R, int L, const E8* alpha, hipStream_t st) {
+  tile_pass(TP_WAW, false, par, par, nullptr, alpha, cols, R, R, L, 0, std::min(L, 12), 0, 0, st);
+  for (auto& g : hi_groups(L)) tile_pass(TP_W, false, par, par, nullptr, nullptr, cols, R, R, L, g.first, g.second, 0, 0, st);
+}
+// Parity halves WHT(alpha . WHT(m)) from the message halves of `cols` columns.
+static bool tiled_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStream_t st) {
+  const uint64_t n = uint64_t(1) << L, R = 2 * n;
+  auto gs = hi_groups(L);
+  if (gs.empty())  // one pass: message -> WHT . alpha . WHT -> parity
+    return tile_pass(TP_WAW, false, cw, cw + n, nullptr, alpha, cols, R, R, L, 0, std::min(L, 12), 0, 0, st);
+  if (!tile_pass(TP_W, false, cw, cw + n, nullptr, nullptr, cols, R, R, L, gs[0].first, gs[0].second, 0, 0, st))
+    return false;
+  for (size_t i = 1; i < gs.size(); i++)
+    tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, cols, R, R, L, gs[i].first, gs[i].second, 0, 0, st);
+  tiled_parity_tail(cw + n, cols, R, L, alpha, st);
+  return true;
+}
+// Message halves = Mobius over the L row bits of the 32 packed columns (from the
+// BVA when pack, else from a flat E8 vector with column stride n), then the parity.
+static bool tiled_commit_rows(bool pack, const void* src, uint64_t block_first, uint64_t available, E8* cw, int L,
+                              const E8* alpha, hipStream_t st) {
+  if (L < 12) return false;
+  const uint64_t n = uint64_t(1) << L, R = 2 * n;
+  tile_pass(TP_M, pack, src, cw, nullptr, nullptr, 32, n, R, L, 0, 12, block_first, available, st);
+  auto gs = hi_groups(L);
+  if (gs.empty())  // L = 12: message complete; one pass gives the parity
+    return tile_pass(TP_WAW, false, cw, cw + n, nullptr, alpha, 32, R, R, L, 0, 12, 0, 0, st);
+  for (size_t i = 0; i + 1 < gs.size(); i++)
+    tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st);
+  // last Mobius group completes the message; the same pass starts the WHT in the parity half
+  tile_pass(TP_M_W, false, cw, cw, cw + n, nullptr, 32, R, R, L, gs.back().first, gs.back().second, 0, 0, st);
+  for (size_t i = 0; i + 1 < gs.size(); i++)
+    tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st);
+  tiled_parity_tail(cw + n, 32, R, L, alpha, st);
+  return true;
+}
+
 void launch_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream_t st) {
   if (L == 0) return;
+  if (tiled_wht(v, vectors, stride, L, st)) return;
   for (auto& gp : bit_groups(L)) launch_pass_geom(v, v, vectors, stride, stride, L, gp.first, gp.second, OP_WHT, nullptr, st);
 }
 
@@ -286,6 +335,7 @@ void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStre
     hipLaunchKernelGGL(k_encode_one_row, dim3(uint32_t((cols + WG - 1) / WG)), dim3(WG), 0, st, cw, cols, alpha);
     return;
   }
+  if (tiled_encode_parity(cw, cols, L, alpha, st)) return;
   // parity = WHT(alpha . WHT(m)): low bits copy m into the parity half, the
   // high bits run forward, multiply, forward again, then the low bits again.
   auto groups = bit_groups(L);
@@ -303,11 +353,14 @@ void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStre
 }
 
 void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
-                 hipStream_t st) {
-  if (leaf.prefix_len == 75)
-    hipLaunchKernelGGL(k_leaf_hash<75>, dim3(uint32_t((R + WG - 1) / WG)), dim3(WG), 0, st, cw, R, leaf, tree);
-  else if (leaf.prefix_len == 80)
-    hipLaunchKernelGGL(k_leaf_hash<80>, dim3(uint32_t((R + WG - 1) / WG)), dim3(WG), 0, st, cw, R, leaf, tree);
+                 hipStream_t st, bool precol) {
+  const dim3 grid{uint32_t((R + WG - 1) / WG), 1, 1}, block{uint32_t(WG), 1, 1};
+  if (leaf.prefix_len == 75 && precol)
+    hipLaunchKernelGGL((k_leaf_hash<75, true>), grid, block, 0, st, cw, R, leaf, tree);
+  else if (leaf.prefix_len == 75)
+    hipLaunchKernelGGL((k_leaf_hash<75, false>), grid, block, 0, st, cw, R, leaf, tree);
+  else if (leaf.prefix_len == 80 && !precol)
+    hipLaunchKernelGGL((k_leaf_hash<80, false>), grid, block, 0, st, cw, R, leaf, tree);
   else
     throw QfError(QF_ERR_ARGUMENT, "unsupported leaf prefix");
   uint32_t* level = tree;
@@ -319,29 +372,46 @@ void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPr
   }
 }
 
-void launch_mobius_message(E8* cw, int L, hipStream_t st) {
+// Mobius over the L row bits of each message half (generic passes, in place).
+static void launch_mobius_rows(E8* cw, int L, hipStream_t st) {
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
+  (void)n;
   if (L > 0)
     for (auto& gp : bit_groups(L)) launch_pass_geom(cw, cw, 32, R, R, L, gp.first, gp.second, OP_MOBIUS, nullptr, st);
-  hipLaunchKernelGGL(k_mobius_columns, dim3(uint32_t((n * 4 + WG - 1) / WG)), dim3(WG), 0, st, cw, n, R);
 }
 
 void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E8* alpha, uint32_t* tree,
                           hipStream_t st) {
   const int L = g.log_n;
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
-  launch_mobius_message(cw, L, st);
-  launch_encode_parity(cw, 32, L, alpha, st);
-  launch_tree(cw, R, g.leaf, g.inner, tree, st);
+  if (!tiled_commit_rows(false, values, 0, 0, cw, L, alpha, st)) {
+    hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
+    launch_mobius_rows(cw, L, st);
+    launch_encode_parity(cw, 32, L, alpha, st);
+  }
+  launch_tree(cw, R, g.leaf, g.inner, tree, st, true);
 }
 
 void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, const LeafPrefix& leaf,
                              const InnerPrefix& inner, uint32_t* tree, hipStream_t st) {
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
-  launch_encode_parity(cw, 32, L, alpha, st);
-  launch_tree(cw, R, leaf, inner, tree, st);
+  auto gs = hi_groups(L);
+  bool tiled;
+  if (gs.empty()) {  // copy the message and encode the parity in one pass
+    tiled = tile_pass(TP_COPY_WAW, false, values, cw, cw + n, alpha, 32, n, R, L, 0, std::min(L, 12), 0, 0, st);
+  } else {  // copy the message while starting the WHT on the first high group
+    tiled = tile_pass(TP_COPY_W, false, values, cw, cw + n, nullptr, 32, n, R, L, gs[0].first, gs[0].second, 0, 0, st);
+    if (tiled) {
+      for (size_t i = 1; i < gs.size(); i++)
+        tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st);
+      tiled_parity_tail(cw + n, 32, R, L, alpha, st);
+    }
+  }
+  if (!tiled) {
+    hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
+    launch_encode_parity(cw, 32, L, alpha, st);
+  }
+  launch_tree(cw, R, leaf, inner, tree, st, false);
 }
 
 void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_first, uint64_t available, E8* cw,
@@ -349,21 +419,21 @@ void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_
   const int L = g.log_n;
   if (ev) QF_HIP(hipEventRecord(ev[0], st));
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  const int tp = L < 12 ? L : 12;
-  // 1. pack + Mobius on the low bits within each column
-  uint64_t packed = n * 32;
-  hipLaunchKernelGGL(k_pack_mobius, dim3(uint32_t(packed >> tp)), dim3(WG), 0, st, bva, block_first, available, cw, L,
-                     R, tp);
-  // 2. remaining Mobius bits within each column
-  for (auto& gp : bit_groups(L))
-    if (gp.first > 0) launch_pass_geom(cw, cw, 32, R, R, L, gp.first, gp.second, OP_MOBIUS, nullptr, st);
-  // 3. Mobius over the column bits
-  hipLaunchKernelGGL(k_mobius_columns, dim3(uint32_t((n * 4 + WG - 1) / WG)), dim3(WG), 0, st, cw, n, R);
-  // 4. parity
-  launch_encode_parity(cw, 32, L, alpha, st);
-  // 5. leaves and inner levels; tree level l occupies (R >> l) digests after the previous levels.
+  if (!tiled_commit_rows(true, bva, block_first, available, cw, L, alpha, st)) {
+    const int tp = L < 12 ? L : 12;
+    // 1. pack + Mobius on the low bits within each column
+    uint64_t packed = n * 32;
+    hipLaunchKernelGGL(k_pack_mobius, dim3(uint32_t(packed >> tp)), dim3(WG), 0, st, bva, block_first, available, cw,
+                       L, R, tp);
+    // 2. remaining Mobius bits within each column
+    for (auto& gp : bit_groups(L))
+      if (gp.first > 0) launch_pass_geom(cw, cw, 32, R, R, L, gp.first, gp.second, OP_MOBIUS, nullptr, st);
+    // 3. parity
+    launch_encode_parity(cw, 32, L, alpha, st);
+  }
+  // 4. leaves (column Mobius applied in registers) and inner levels
   if (ev) QF_HIP(hipEventRecord(ev[1], st));
-  launch_tree(cw, R, g.leaf, g.inner, tree, st);
+  launch_tree(cw, R, g.leaf, g.inner, tree, st, true);
   if (ev) QF_HIP(hipEventRecord(ev[2], st));
   if (ev) QF_HIP(hipEventRecord(ev[3], st));
 }

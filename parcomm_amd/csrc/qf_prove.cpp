@@ -275,27 +275,60 @@ std::vector<uint64_t> sample_rows(uint64_t count, uint64_t population, const Dig
 
 // ================================================================ device codewords
 // A 32-column codeword (column-major, R rows) and its Merkle tree in HBM.
+// Mobius over the 5 column bits of one stored 32-entry row: the codeword row.
+void col_mobius(E8* row) {
+  for (int h = 1; h < 32; h *= 2)
+    for (int k = 0; k < 32; k++)
+      if (k & h) row[k] = row[k] - row[k ^ h];
+}
+// Its transpose: weights on codeword columns -> weights on stored columns.
+std::vector<E8> col_mobius_t(std::vector<E8> f) {
+  for (int h = 1; h < 32; h *= 2)
+    for (int k = 0; k < 32; k++)
+      if (!(k & h)) f[k] = f[k] - f[k | h];
+  return f;
+}
+
 struct DCode {
   DVec cw;
   uint32_t* tree = nullptr;
   bool own_tree = false;
   const E8* cwp = nullptr;  // borrowed codeword when cw is empty
   uint64_t R = 0, cols = 32;
+  bool precol = false;      // stored rows precede the column Mobius (base and folded codewords)
   Digest root{};
   const E8* data() const { return cw.p ? cw.p : cwp; }
   DCode() = default;
   DCode(DCode&& o) noexcept
-      : cw(std::move(o.cw)), tree(o.tree), own_tree(o.own_tree), cwp(o.cwp), R(o.R), cols(o.cols), root(o.root) {
+      : cw(std::move(o.cw)), tree(o.tree), own_tree(o.own_tree), cwp(o.cwp), R(o.R), cols(o.cols), precol(o.precol),
+        root(o.root) {
     o.tree = nullptr;
     o.own_tree = false;
   }
   DCode& operator=(DCode&& o) noexcept {
     if (this != &o) {
       if (own_tree && tree) (void)hipFree(tree);
-      cw = std::move(o.cw); tree = o.tree; own_tree = o.own_tree; cwp = o.cwp; R = o.R; cols = o.cols; root = o.root;
+      cw = std::move(o.cw); tree = o.tree; own_tree = o.own_tree; cwp = o.cwp; R = o.R; cols = o.cols;
+      precol = o.precol; root = o.root;
       o.tree = nullptr; o.own_tree = false;
     }
     return *this;
+  }
+  // 32 column projections of the codeword (qafold_short_pc.rs:2641-2671)
+  std::vector<E8> projection(DevCtx& c, uint64_t n, const E8* w) const {
+    auto o = dv::col_projection(c, data(), R, n, cols, w);
+    if (precol) col_mobius(o.data());
+    return o;
+  }
+  // out[r] = sum_col f[col] * codeword[col][r], r < n
+  void fold(DevCtx& c, uint64_t n, const std::vector<E8>& f, E8* out) const {
+    dv::col_fold(c, data(), R, n, cols, precol ? col_mobius_t(f) : f, out);
+  }
+  // rows[k * cols + col] of the codeword
+  std::vector<E8> rows(DevCtx& c, const std::vector<uint64_t>& sel) const {
+    auto v = dv::gather_rows(c, data(), R, cols, sel);
+    if (precol) for (size_t k = 0; k < sel.size(); k++) col_mobius(v.data() + k * cols);
+    return v;
   }
   ~DCode() { if (own_tree && tree) (void)hipFree(tree); }
 };
@@ -391,13 +424,21 @@ void write_rows(DevCtx& c, Bytes& out, const DCode& d, bool all, const std::vect
     put_u64(out, d.R);
     put_u64(out, d.cols);
     put_u64(out, d.R * d.cols);
-    auto e = dv::download(c, d.data(), d.R * d.cols);
+    auto e = dv::download(c, d.data(), d.R * d.cols);  // column-major
+    if (d.precol) {
+      std::vector<E8> row(d.cols);
+      for (uint64_t r = 0; r < d.R; r++) {
+        for (uint64_t col = 0; col < d.cols; col++) row[col] = e[col * d.R + r];
+        col_mobius(row.data());
+        for (uint64_t col = 0; col < d.cols; col++) e[col * d.R + r] = row[col];
+      }
+    }
     for (auto& v : e) put_e8(out, v);
     return;
   }
   out.push_back(0);
   put_u64(out, rows.size());
-  auto vals = dv::gather_rows(c, d.data(), d.R, d.cols, rows);
+  auto vals = d.rows(c, rows);
   auto paths = dv::gather_paths(c, d.tree, d.R, rows);
   for (size_t k = 0; k < rows.size(); k++) {
     put_u64(out, d.cols);
@@ -449,7 +490,7 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
   require(R == code.codeword_len() && point.size() == size_t(5 + ilog2u(n)), QF_ERR_INVALID_VARIABLES, "round dims");
   std::vector<E8> rp(point.begin() + 5, point.end());
   DVec row_w = tw_vec(c, rp);
-  auto opening = dv::col_projection(c, cache.data(), R, n, 32, row_w.p);
+  auto opening = cache.projection(c, n, row_w.p);
   if (!top) require(inner_host(opening, tw_host(point.data(), 5)) == value, QF_ERR_INVALID_COMMITMENT, "round opening");
   tr.digest("round-root", cache.root);
   for (auto& o : opening) tr.respond(o);
@@ -457,7 +498,7 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
   for (int k = 0; k < 32; k++) fr.push_back(tr.challenge("freivalds"));
   E8 fv = inner_host(opening, fr);
   DVec folded(n);
-  dv::col_fold(c, cache.data(), R, n, 32, fr, folded.p);
+  cache.fold(c, n, fr, folded.p);
   if (level > 0) {
     next = commit_extension(c, p, level - 1, folded.p);
     tr.bytes("next-layer-kind", "extension-root");
@@ -475,10 +516,10 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
   std::vector<E8> sampled;
   if (all) {
     DVec rv(R);
-    dv::col_fold(c, cache.data(), R, R, 32, fr, rv.p);
+    cache.fold(c, R, fr, rv.p);
     sampled = dv::download(c, rv);
   } else {
-    auto rows = dv::gather_rows(c, cache.data(), R, 32, sel);
+    auto rows = cache.rows(c, sel);
     for (size_t k = 0; k < sel.size(); k++) {
       E8 s{};
       for (int col = 0; col < 32; col++) s += fr[col] * rows[k * 32 + col];
@@ -496,7 +537,7 @@ E8 open_with_reduction(DevCtx& c, const ParamsState& p, const DCode& top_code, c
   uint64_t n = p.pp.codes[top].message_len;
   std::vector<E8> rp(point.begin() + 5, point.end());
   DVec row_w = tw_vec(c, rp);
-  auto opening = dv::col_projection(c, top_code.data(), top_code.R, n, 32, row_w.p);
+  auto opening = top_code.projection(c, n, row_w.p);
   E8 claimed = inner_host(opening, tw_host(point.data(), 5));
   Pcs tr;
   tr.digest("parameters", p.pp.parameter_digest);
@@ -695,7 +736,7 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     absorb_sel(tr, sel);
     tr.exts("iota-column-weights", iw);
     std::vector<E8> carry_rows;
-    if (have_carry) carry_rows = dv::gather_rows(c, carry.data(), carry.R, 32, srows);
+    if (have_carry) carry_rows = carry.rows(c, srows);
     std::vector<E8> iota;
     for (size_t k = 0; k < srows.size(); k++) {
       E8 v{};
@@ -1142,6 +1183,7 @@ DCode block_code(DevCtx& c, const ParamsState& p, const CommittedState& cs, uint
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode d;
   d.R = R;
+  d.precol = true;
   d.root = cs.roots[b];
   if (cs.cache_mode == QF_CACHE_CACHED && b < cs.d_cw.size()) {
     d.cwp = cs.d_cw[b];
@@ -1236,6 +1278,7 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
   block_fold(c, cs.d_bva, p.layout.active, p.block_len, blocks, fw, values.p, plen);
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode folded = alloc_code(R);
+  folded.precol = true;
   launch_commit_values(p.geom, values.p, folded.cw.p, p.alpha_device(p.pp.codes.size() - 1), folded.tree, c.st);
   QF_HIP(hipGetLastError());
   folded.root = download_root(c, folded.tree, R);

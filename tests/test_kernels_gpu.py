@@ -1,0 +1,94 @@
+"""Kernel-level parity: the register-radix transform passes (qf_tile.hip)
+against numpy restatements of the same 2x2 tensor maps over F65521/Ext8."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+P = 65521
+
+
+def e8_mul(a, b):
+    """Rows of (..., 8) canonical coefficients: a * b in F_p[X]/(X^8 - 17)."""
+    a = a.astype(np.int64)
+    b = b.astype(np.int64)
+    t = np.zeros(a.shape[:-1] + (15,), dtype=np.int64)
+    for i in range(8):
+        for j in range(8):
+            t[..., i + j] = (t[..., i + j] + a[..., i] * b[..., j]) % P
+    out = t[..., :8].copy()
+    out[..., :7] = (out[..., :7] + 17 * t[..., 8:]) % P
+    return out % P
+
+
+def transform(x, bits, wht):
+    """x: (vectors, n, 8); 2x2 maps on the given index bits."""
+    y = x.astype(np.int64).copy()
+    n = y.shape[1]
+    idx = np.arange(n)
+    for b in bits:
+        lo = idx[((idx >> b) & 1) == 0]
+        hi = lo | (1 << b)
+        u, w = y[:, lo].copy(), y[:, hi].copy()
+        if wht:
+            y[:, lo] = (u + w) % P
+            y[:, hi] = (u - w) % P
+        else:
+            y[:, hi] = (w - u) % P
+    return y % P
+
+
+def run(mode, x, alpha, L, s, t, vstride_in=None, vstride_out=None):
+    import parcomm_amd as pa
+    vectors = x.shape[0]
+    n = 1 << L
+    vin = vstride_in or n
+    vout = vstride_out or n
+    host_in = np.zeros((vectors * vin, 8), dtype=np.uint16)
+    for v in range(vectors):
+        host_in[v * vin:v * vin + n] = x[v]
+    d_in = pa.DeviceBuffer.from_host(host_in)
+    d_o1 = pa.DeviceBuffer(vectors * vout * 16)
+    d_o2 = pa.DeviceBuffer(vectors * vout * 16)
+    d_a = pa.DeviceBuffer.from_host(alpha.astype(np.uint16)) if alpha is not None else None
+    rc = pa.lib().qf_test_tile_pass(mode, d_in.ptr, d_o1.ptr, d_o2.ptr, d_a.ptr if d_a else None, vectors, vin, vout,
+                                    L, s, t)
+    assert rc == 0, pa.lib().qf_last_error()
+    o1 = d_o1.to_host("uint16").reshape(vectors * vout, 8)
+    o2 = d_o2.to_host("uint16").reshape(vectors * vout, 8)
+    take = lambda o: np.stack([o[v * vout:v * vout + n] for v in range(vectors)])
+    return take(o1), take(o2)
+
+
+GEOMS = [(32, 12, 0, 12), (2, 14, 12, 2), (1, 16, 12, 4), (16, 8, 0, 8), (1, 22, 12, 10), (1, 20, 12, 8),
+         (32, 17, 12, 5), (2, 13, 0, 12)]
+
+
+@pytest.mark.parametrize("vectors,L,s,t", GEOMS)
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
+def test_tile_pass_matches_numpy(vectors, L, s, t, mode):
+    rng = np.random.default_rng(1000 * L + 10 * mode + s)
+    x = rng.integers(0, P, size=(vectors, 1 << L, 8), dtype=np.int64)
+    alpha = rng.integers(0, P, size=(1 << L, 8), dtype=np.int64)
+    print("tile pass mode", mode, "vectors", vectors, "L", L, "bits", s, s + t, flush=True)
+    o1, o2 = run(mode, x, alpha if mode in (3, 5) else None, L, s, t, vstride_in=(1 << L) + 64 if L < 20 else None)
+    bits = list(range(s, s + t))
+    if mode == 0:
+        assert np.array_equal(o1, transform(x, bits, False))
+    elif mode == 1:
+        m = transform(x, bits, False)
+        assert np.array_equal(o1, m)
+        assert np.array_equal(o2, transform(m, bits, True))
+    elif mode == 2:
+        assert np.array_equal(o1, transform(x, bits, True))
+    elif mode == 3:
+        w = transform(x, bits, True)
+        assert np.array_equal(o1, transform(e8_mul(w, alpha[None]), bits, True))
+    elif mode == 4:
+        assert np.array_equal(o1, x)
+        assert np.array_equal(o2, transform(x, bits, True))
+    else:
+        w = transform(x, bits, True)
+        assert np.array_equal(o1, x)
+        assert np.array_equal(o2, transform(e8_mul(w, alpha[None]), bits, True))
