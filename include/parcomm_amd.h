@@ -132,11 +132,13 @@ int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* devic
  * one register-radix transform pass (mode 0 Mobius, 1 Mobius then WHT into
  * out2, 2 WHT, 3 WHT . alpha . WHT, 4 copy into out1 and WHT into out2, 5 copy
  * into out1 and WHT . alpha . WHT into out2) over
- * `vectors` device vectors of 2^L Ext8 elements, on index bits [s, s+t).
- * Returns QF_ERR_ARGUMENT when the geometry does not fit a tile. Synchronous. */
+ * `vectors` device vectors of 2^L Ext8 elements, on index bits [s, s+t) and,
+ * when xform_vectors is set (s = 0), also on the index bits of the 2^(12-t)
+ * vectors that share a tile. Returns QF_ERR_ARGUMENT when the geometry does
+ * not fit a tile. Synchronous. */
 int qf_test_tile_pass(int mode, const void* device_in, void* device_out1, void* device_out2,
                       const void* device_alpha, uint64_t vectors, uint64_t in_vstride, uint64_t out_vstride, int L,
-                      int s, int t);
+                      int s, int t, int xform_vectors);
 
 #ifdef __cplusplus
 }

@@ -77,7 +77,7 @@ def lib():
         L.qf_proof_free.argtypes = [vp]
         L.qf_proof_free.restype = None
         L.qf_statement_json.argtypes = [vp, vp, ctypes.c_char_p, sz, P(sz)]
-        L.qf_test_tile_pass.argtypes = [i32, vp, vp, vp, vp, u64, u64, u64, i32, i32, i32]
+        L.qf_test_tile_pass.argtypes = [i32, vp, vp, vp, vp, u64, u64, u64, i32, i32, i32, i32]
         _lib = L
     return _lib
 

@@ -267,12 +267,6 @@ int qf_committed_codeword(const qf_committed* c, uint64_t block, void* out, size
     const size_t R = 2 * (size_t(1) << c->p->geom.log_n), bytes = 32 * R * 16;
     require(cap >= bytes, QF_ERR_ARGUMENT, "codeword buffer too small");
     QF_HIP(hipMemcpy(out, c->d_cw[block], bytes, hipMemcpyDeviceToHost));
-    // stored rows precede the Mobius over the 5 column bits: apply it per row
-    E8* cw = static_cast<E8*>(out);
-    for (size_t r = 0; r < R; r++)
-      for (size_t h = 1; h < 32; h *= 2)
-        for (size_t k = 0; k < 32; k++)
-          if (k & h) cw[k * R + r] = cw[k * R + r] - cw[(k ^ h) * R + r];
   });
 }
 
@@ -355,11 +349,12 @@ int qf_statement_json(const qf_params* p, const qf_committed* c, char* out, size
 
 int qf_test_tile_pass(int mode, const void* device_in, void* device_out1, void* device_out2,
                       const void* device_alpha, uint64_t vectors, uint64_t in_vstride, uint64_t out_vstride, int L,
-                      int s, int t) {
+                      int s, int t, int xform_vectors) {
   return guarded([&] {
     require(device_in && device_out1, QF_ERR_ARGUMENT, "null argument");
     bool ok = tile_pass(mode, false, device_in, static_cast<E8*>(device_out1), static_cast<E8*>(device_out2),
-                        static_cast<const E8*>(device_alpha), vectors, in_vstride, out_vstride, L, s, t, 0, 0, nullptr);
+                        static_cast<const E8*>(device_alpha), vectors, in_vstride, out_vstride, L, s, t, 0, 0, nullptr,
+                        xform_vectors != 0);
     require(ok, QF_ERR_ARGUMENT, "geometry does not fit a tile pass");
     QF_HIP(hipDeviceSynchronize());
   });

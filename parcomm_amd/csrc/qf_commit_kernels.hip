@@ -127,13 +127,35 @@ __global__ __launch_bounds__(WG) void k_pack_mobius(const int8_t* __restrict__ b
   for (int e = threadIdx.x; e < n; e += WG) cw[col * rows + row0 + e] = t[e];
 }
 
+// Mobius over the 5 column bits: for each message row, the 32 column entries.
+// Thread = (row, 32-bit word of the 16-byte element).
+__global__ __launch_bounds__(WG) void k_mobius_columns(E8* __restrict__ cw, uint64_t n, uint64_t rows) {
+  uint64_t gid = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  uint64_t row = gid >> 2;
+  int word = int(gid & 3);
+  if (row >= n) return;
+  uint32_t* base = reinterpret_cast<uint32_t*>(cw) + row * 4 + word;
+  const uint64_t cstride = rows * 4;
+  uint32_t lo[32], hi[32];
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    uint32_t w = base[k * cstride];
+    lo[k] = w & 0xffffu;
+    hi[k] = w >> 16;
+  }
+#pragma unroll
+  for (int h = 1; h < 32; h *= 2)
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+      if (k & h) { lo[k] = msub(lo[k], lo[k ^ h]); hi[k] = msub(hi[k], hi[k ^ h]); }
+#pragma unroll
+  for (int k = 0; k < 32; k++) base[k * cstride] = lo[k] | (hi[k] << 16);
+}
+
 // Leaf hash of one 32-column codeword row: BLAKE2s(prefix || row bytes), with
 // the framed domain + width prefix of PLEN bytes (75 for base rows, 80 for
 // extension rows). The first 64 prefix bytes are compressed on the host.
-// With PRECOL the stored rows are the codeword before the Mobius transform over
-// the 5 column bits (that map acts within a row and commutes with encoding);
-// it is applied here, in registers, before hashing.
-template <int PLEN, bool PRECOL>
+template <int PLEN>
 __global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uint64_t rows, LeafPrefix lp,
                                                   uint32_t* __restrict__ leaves) {
   constexpr int TOTAL = PLEN + 512;
@@ -145,21 +167,6 @@ __global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uin
   for (int k = 0; k < 32; k++) {
     uint4 w = *reinterpret_cast<const uint4*>(cw + k * rows + r);
     R[4 * k] = w.x; R[4 * k + 1] = w.y; R[4 * k + 2] = w.z; R[4 * k + 3] = w.w;
-  }
-  if (PRECOL) {
-#pragma unroll
-    for (int word = 0; word < 4; word++) {
-      uint32_t lo[32], hi[32];
-#pragma unroll
-      for (int k = 0; k < 32; k++) { lo[k] = R[4 * k + word] & 0xffffu; hi[k] = R[4 * k + word] >> 16; }
-#pragma unroll
-      for (int h = 1; h < 32; h *= 2)
-#pragma unroll
-        for (int k = 0; k < 32; k++)
-          if (k & h) { lo[k] = msub(lo[k], lo[k ^ h]); hi[k] = msub(hi[k], hi[k ^ h]); }
-#pragma unroll
-      for (int k = 0; k < 32; k++) R[4 * k + word] = lo[k] | (hi[k] << 16);
-    }
   }
   R[128] = 0;
   uint32_t h[8];
@@ -303,13 +310,17 @@ static bool tiled_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, h
   tiled_parity_tail(cw + n, cols, R, L, alpha, st);
   return true;
 }
-// Message halves = Mobius over the L row bits of the 32 packed columns (from the
-// BVA when pack, else from a flat E8 vector with column stride n), then the parity.
+// Message halves = Mobius over all 5 + L index bits of the packed block (from
+// the BVA when pack, else from a flat E8 vector with column stride n), then the
+// parity. Pass 1 holds 32 columns x 128 rows per tile (column bits and row bits
+// 0..6), pass 2 row bits 7..11, then the high groups, the last of which also
+// starts the parity WHT.
 static bool tiled_commit_rows(bool pack, const void* src, uint64_t block_first, uint64_t available, E8* cw, int L,
                               const E8* alpha, hipStream_t st) {
   if (L < 12) return false;
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  tile_pass(TP_M, pack, src, cw, nullptr, nullptr, 32, n, R, L, 0, 12, block_first, available, st);
+  tile_pass(TP_M, pack, src, cw, nullptr, nullptr, 32, n, R, L, 0, 7, block_first, available, st, true);
+  tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, 7, 5, 0, 0, st);
   auto gs = hi_groups(L);
   if (gs.empty())  // L = 12: message complete; one pass gives the parity
     return tile_pass(TP_WAW, false, cw, cw + n, nullptr, alpha, 32, R, R, L, 0, 12, 0, 0, st);
@@ -353,14 +364,12 @@ void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStre
 }
 
 void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
-                 hipStream_t st, bool precol) {
+                 hipStream_t st) {
   const dim3 grid{uint32_t((R + WG - 1) / WG), 1, 1}, block{uint32_t(WG), 1, 1};
-  if (leaf.prefix_len == 75 && precol)
-    hipLaunchKernelGGL((k_leaf_hash<75, true>), grid, block, 0, st, cw, R, leaf, tree);
-  else if (leaf.prefix_len == 75)
-    hipLaunchKernelGGL((k_leaf_hash<75, false>), grid, block, 0, st, cw, R, leaf, tree);
-  else if (leaf.prefix_len == 80 && !precol)
-    hipLaunchKernelGGL((k_leaf_hash<80, false>), grid, block, 0, st, cw, R, leaf, tree);
+  if (leaf.prefix_len == 75)
+    hipLaunchKernelGGL(k_leaf_hash<75>, grid, block, 0, st, cw, R, leaf, tree);
+  else if (leaf.prefix_len == 80)
+    hipLaunchKernelGGL(k_leaf_hash<80>, grid, block, 0, st, cw, R, leaf, tree);
   else
     throw QfError(QF_ERR_ARGUMENT, "unsupported leaf prefix");
   uint32_t* level = tree;
@@ -372,12 +381,12 @@ void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPr
   }
 }
 
-// Mobius over the L row bits of each message half (generic passes, in place).
-static void launch_mobius_rows(E8* cw, int L, hipStream_t st) {
+// Mobius over the L row bits and the 5 column bits of the message halves (generic passes, in place).
+static void launch_mobius_message(E8* cw, int L, hipStream_t st) {
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  (void)n;
   if (L > 0)
     for (auto& gp : bit_groups(L)) launch_pass_geom(cw, cw, 32, R, R, L, gp.first, gp.second, OP_MOBIUS, nullptr, st);
+  hipLaunchKernelGGL(k_mobius_columns, dim3(uint32_t((n * 4 + WG - 1) / WG)), dim3(WG), 0, st, cw, n, R);
 }
 
 void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E8* alpha, uint32_t* tree,
@@ -386,10 +395,10 @@ void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
   if (!tiled_commit_rows(false, values, 0, 0, cw, L, alpha, st)) {
     hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
-    launch_mobius_rows(cw, L, st);
+    launch_mobius_message(cw, L, st);
     launch_encode_parity(cw, 32, L, alpha, st);
   }
-  launch_tree(cw, R, g.leaf, g.inner, tree, st, true);
+  launch_tree(cw, R, g.leaf, g.inner, tree, st);
 }
 
 void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, const LeafPrefix& leaf,
@@ -411,7 +420,7 @@ void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, c
     hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
     launch_encode_parity(cw, 32, L, alpha, st);
   }
-  launch_tree(cw, R, leaf, inner, tree, st, false);
+  launch_tree(cw, R, leaf, inner, tree, st);
 }
 
 void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_first, uint64_t available, E8* cw,
@@ -425,15 +434,16 @@ void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_
     uint64_t packed = n * 32;
     hipLaunchKernelGGL(k_pack_mobius, dim3(uint32_t(packed >> tp)), dim3(WG), 0, st, bva, block_first, available, cw,
                        L, R, tp);
-    // 2. remaining Mobius bits within each column
+    // 2. remaining Mobius bits within each column, then over the column bits
     for (auto& gp : bit_groups(L))
       if (gp.first > 0) launch_pass_geom(cw, cw, 32, R, R, L, gp.first, gp.second, OP_MOBIUS, nullptr, st);
+    hipLaunchKernelGGL(k_mobius_columns, dim3(uint32_t((n * 4 + WG - 1) / WG)), dim3(WG), 0, st, cw, n, R);
     // 3. parity
     launch_encode_parity(cw, 32, L, alpha, st);
   }
-  // 4. leaves (column Mobius applied in registers) and inner levels
+  // 4. leaves and inner levels
   if (ev) QF_HIP(hipEventRecord(ev[1], st));
-  launch_tree(cw, R, g.leaf, g.inner, tree, st, true);
+  launch_tree(cw, R, g.leaf, g.inner, tree, st);
   if (ev) QF_HIP(hipEventRecord(ev[2], st));
   if (ev) QF_HIP(hipEventRecord(ev[3], st));
 }

@@ -18,8 +18,7 @@ struct CommitGeom {
 
 // Commits one PCS block: pack + Mobius into the message half of `cw`,
 // encode the parity half, hash the rows and build all Merkle levels in `tree`
-// (2R - 1 digests, leaves first). Everything is enqueued on `st`. The codeword
-// is stored in pre-column form (see below).
+// (2R - 1 digests, leaves first). Everything is enqueued on `st`.
 void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_first, uint64_t available, E8* cw,
                          const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev = nullptr);
 // ev (optional, 4 events): recorded at start, before the leaf kernel, after it, and at the end.
@@ -30,10 +29,8 @@ void launch_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream_t st)
 // Parity halves of `cols` columns (R = 2^(L+1) rows each) from their message halves.
 void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStream_t st);
 // Leaves (prefix chosen by `leaf`) and all inner levels of a 32-column codeword.
-// precol: the stored rows precede the Mobius transform over the 5 column bits,
-// which the leaf kernel applies before hashing.
 void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
-                 hipStream_t st, bool precol);
+                 hipStream_t st);
 // Commit of an extension matrix: flat 32 x 2^L message, no Mobius, extension leaves.
 void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, const LeafPrefix& leaf,
                              const InnerPrefix& inner, uint32_t* tree, hipStream_t st);
@@ -41,15 +38,13 @@ void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, c
 void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E8* alpha, uint32_t* tree,
                           hipStream_t st);
 
-// Base-block and folded codewords are stored in "pre-column" form: row r holds
-// the 32 entries before the Mobius transform over the column index, so that
-// codeword row r = col_mobius(stored row r). Extension codewords are stored as is.
-
 // One register-radix tile pass (qf_tile.hip); false when the geometry does not
 // fit a 4096-element tile.
 enum : int { TP_M = 0, TP_M_W = 1, TP_W = 2, TP_WAW = 3, TP_COPY_W = 4, TP_COPY_WAW = 5 };
+// With xform_vectors the pass also transforms the index bits of the vectors a
+// tile holds (s = 0, t < 12: 2^(12-t) vectors per tile).
 bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8* alpha, uint64_t vectors,
                uint64_t in_vstride, uint64_t out_vstride, int L, int s, int t, uint64_t block_first,
-               uint64_t available, hipStream_t st);
+               uint64_t available, hipStream_t st, bool xform_vectors = false);
 
 }  // namespace qf

@@ -275,33 +275,17 @@ std::vector<uint64_t> sample_rows(uint64_t count, uint64_t population, const Dig
 
 // ================================================================ device codewords
 // A 32-column codeword (column-major, R rows) and its Merkle tree in HBM.
-// Mobius over the 5 column bits of one stored 32-entry row: the codeword row.
-void col_mobius(E8* row) {
-  for (int h = 1; h < 32; h *= 2)
-    for (int k = 0; k < 32; k++)
-      if (k & h) row[k] = row[k] - row[k ^ h];
-}
-// Its transpose: weights on codeword columns -> weights on stored columns.
-std::vector<E8> col_mobius_t(std::vector<E8> f) {
-  for (int h = 1; h < 32; h *= 2)
-    for (int k = 0; k < 32; k++)
-      if (!(k & h)) f[k] = f[k] - f[k | h];
-  return f;
-}
-
 struct DCode {
   DVec cw;
   uint32_t* tree = nullptr;
   bool own_tree = false;
   const E8* cwp = nullptr;  // borrowed codeword when cw is empty
   uint64_t R = 0, cols = 32;
-  bool precol = false;      // stored rows precede the column Mobius (base and folded codewords)
   Digest root{};
   const E8* data() const { return cw.p ? cw.p : cwp; }
   DCode() = default;
   DCode(DCode&& o) noexcept
-      : cw(std::move(o.cw)), tree(o.tree), own_tree(o.own_tree), cwp(o.cwp), R(o.R), cols(o.cols), precol(o.precol),
-        root(o.root) {
+      : cw(std::move(o.cw)), tree(o.tree), own_tree(o.own_tree), cwp(o.cwp), R(o.R), cols(o.cols), root(o.root) {
     o.tree = nullptr;
     o.own_tree = false;
   }
@@ -309,26 +293,22 @@ struct DCode {
     if (this != &o) {
       if (own_tree && tree) (void)hipFree(tree);
       cw = std::move(o.cw); tree = o.tree; own_tree = o.own_tree; cwp = o.cwp; R = o.R; cols = o.cols;
-      precol = o.precol; root = o.root;
+      root = o.root;
       o.tree = nullptr; o.own_tree = false;
     }
     return *this;
   }
   // 32 column projections of the codeword (qafold_short_pc.rs:2641-2671)
   std::vector<E8> projection(DevCtx& c, uint64_t n, const E8* w) const {
-    auto o = dv::col_projection(c, data(), R, n, cols, w);
-    if (precol) col_mobius(o.data());
-    return o;
+    return dv::col_projection(c, data(), R, n, cols, w);
   }
   // out[r] = sum_col f[col] * codeword[col][r], r < n
   void fold(DevCtx& c, uint64_t n, const std::vector<E8>& f, E8* out) const {
-    dv::col_fold(c, data(), R, n, cols, precol ? col_mobius_t(f) : f, out);
+    dv::col_fold(c, data(), R, n, cols, f, out);
   }
   // rows[k * cols + col] of the codeword
   std::vector<E8> rows(DevCtx& c, const std::vector<uint64_t>& sel) const {
-    auto v = dv::gather_rows(c, data(), R, cols, sel);
-    if (precol) for (size_t k = 0; k < sel.size(); k++) col_mobius(v.data() + k * cols);
-    return v;
+    return dv::gather_rows(c, data(), R, cols, sel);
   }
   ~DCode() { if (own_tree && tree) (void)hipFree(tree); }
 };
@@ -425,14 +405,6 @@ void write_rows(DevCtx& c, Bytes& out, const DCode& d, bool all, const std::vect
     put_u64(out, d.cols);
     put_u64(out, d.R * d.cols);
     auto e = dv::download(c, d.data(), d.R * d.cols);  // column-major
-    if (d.precol) {
-      std::vector<E8> row(d.cols);
-      for (uint64_t r = 0; r < d.R; r++) {
-        for (uint64_t col = 0; col < d.cols; col++) row[col] = e[col * d.R + r];
-        col_mobius(row.data());
-        for (uint64_t col = 0; col < d.cols; col++) e[col * d.R + r] = row[col];
-      }
-    }
     for (auto& v : e) put_e8(out, v);
     return;
   }
@@ -1183,7 +1155,6 @@ DCode block_code(DevCtx& c, const ParamsState& p, const CommittedState& cs, uint
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode d;
   d.R = R;
-  d.precol = true;
   d.root = cs.roots[b];
   if (cs.cache_mode == QF_CACHE_CACHED && b < cs.d_cw.size()) {
     d.cwp = cs.d_cw[b];
@@ -1278,7 +1249,6 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
   block_fold(c, cs.d_bva, p.layout.active, p.block_len, blocks, fw, values.p, plen);
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode folded = alloc_code(R);
-  folded.precol = true;
   launch_commit_values(p.geom, values.p, folded.cw.p, p.alpha_device(p.pp.codes.size() - 1), folded.tree, c.st);
   QF_HIP(hipGetLastError());
   folded.root = download_root(c, folded.tree, R);

@@ -4,13 +4,13 @@
 // Walsh-Hadamard [[1,1],[1,-1]]) on up to 12 index bits of many vectors.
 // Modes: M; M then WHT (second output); WHT; WHT . alpha . WHT; copy then WHT;
 // copy then WHT . alpha . WHT. One
-// workgroup (256 threads) owns a tile of 4096 elements: C contiguous positions
+// workgroup (512 threads) owns a tile of 4096 elements: C contiguous positions
 // x 2^t positions spaced 2^s apart (x 2^logV whole vectors when a vector is
-// shorter than a tile). Each thread keeps 16 elements as eight signed 32-bit
+// shorter than a tile). Each thread keeps 8 elements as eight signed 32-bit
 // lanes without modular reduction (12 stages grow a canonical value by at
-// most 2^12, so |x| < 2^28), runs four butterfly stages in registers, and
-// trades elements through LDS to bring the next four tile bits into its
-// registers (mappings A, B, C below). Values are reduced to canonical u16 only
+// most 2^12, so |x| < 2^28), runs three butterfly stages in registers, and
+// trades elements through LDS to bring the next three tile bits into its
+// registers (mappings with register bits 9..11, 6..8, 3..5, 0..2). Values are reduced to canonical u16 only
 // before they are stored or multiplied. Workgroups are persistent: each walks
 // a contiguous range of tiles (neighbouring tiles share 128-byte lines, so
 // they stay on one XCD's L2) and prefetches the next tile's input while the
@@ -20,14 +20,16 @@
 namespace qf {
 namespace {
 
-constexpr int TWG = 256;               // threads per workgroup
-constexpr int TQ = 16;                 // elements per thread
+constexpr int TWG = 512;               // threads per workgroup
+constexpr int TQB = 3;                 // register bits per phase
+constexpr int TQ = 1 << TQB;           // elements per thread
 constexpr int TN = TWG * TQ;           // elements per tile
-constexpr int TPAD = TN + TN / 16;     // LDS row with one pad slot per 16 elements
+constexpr int TPAD = TN + TN / 8;      // LDS row with one pad slot per 8 elements
 constexpr int32_t BIAS = 32768 * int32_t(P);  // multiple of P above every lazy magnitude
 
 enum : int { TM_M = 0, TM_M_W = 1, TM_W = 2, TM_WAW = 3, TM_COPY_W = 4, TM_COPY_WAW = 5 };
-enum : int { MAP_A = 0, MAP_B = 1, MAP_C = 2 };
+// Mappings are named by the first tile bit held in registers.
+enum : int { MAP_A = 9, MAP_B = 6, MAP_C = 3, MAP_D = 0 };
 
 struct TileArgs {
   const void* in;           // E8 vectors, or the BVA bytes when packing
@@ -38,16 +40,16 @@ struct TileArgs {
   uint64_t block_first, available, pack_n;  // packing: byte offset, active bytes, positions per packed vector
   uint32_t tiles, per_wg;
   int logC, t, s, logV, log_lo_chunks, log_hi_count;
+  int xlo, xhi;             // transformed tile bits [xlo, xhi)
 };
 
-// Tile element e held by thread `tid` in register slot q.
+// Tile element e held by thread `tid` in register slot q: q supplies tile bits
+// [MAP, MAP+3), tid the other nine bits in order.
 template <int MAP>
 __device__ inline int tile_e(int tid, int q) {
-  if (MAP == MAP_A) return tid + TWG * q;                         // bits 8..11 in registers
-  if (MAP == MAP_B) return (tid & 15) + 16 * q + ((tid >> 4) << 8);  // bits 4..7
-  return q + 16 * tid;                                            // bits 0..3
+  return (tid & ((1 << MAP) - 1)) | (q << MAP) | ((tid >> MAP) << (MAP + TQB));
 }
-__device__ inline int lds_slot(int e) { return e + (e >> 4); }
+__device__ inline int lds_slot(int e) { return e + (e >> 3); }
 
 // Position of tile T's element 0.
 __device__ inline void tile_base(const TileArgs& a, uint64_t T, uint64_t& vec0, uint64_t& pos0) {
@@ -67,13 +69,22 @@ __device__ inline uint32_t tile_off(const TileArgs& a, int e, uint64_t vs) {
   return lowc + (h << a.s) + v * uint32_t(vs);
 }
 
-// Butterfly stages on the register bits whose tile bit (bit0 + b) lies in [lo, hi).
-template <bool WHT>
-__device__ inline void reg_stages(int32_t (&x)[TQ][8], int bit0, int lo, int hi) {
+// Butterfly stages on the register bits whose tile bit (MAP + b) lies in the
+// transformed range: [LO, 12) when LO >= 0 (known at compile time, so the
+// stages carry no branches), else the runtime range [lo, hi). The scheduler
+// barriers keep the compiler from interleaving whole phases, which would
+// lengthen live ranges.
+template <bool WHT, int MAP, int LO>
+__device__ inline void phase(int32_t (&x)[TQ][8], int lo, int hi) {
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const int tb = bit0 + b;
-    if (tb < lo || tb >= hi) continue;
+  for (int b = 0; b < TQB; b++) {
+    const int tb = MAP + b;
+    if (LO >= 0) {
+      if (tb < LO) continue;
+    } else if (tb < lo || tb >= hi) {
+      continue;
+    }
 #pragma unroll
     for (int q = 0; q < TQ; q++) {
       if (q & (1 << b)) continue;
@@ -86,11 +97,7 @@ __device__ inline void reg_stages(int32_t (&x)[TQ][8], int bit0, int lo, int hi)
       }
     }
   }
-}
-
-template <bool WHT, int MAP>
-__device__ inline void phase(int32_t (&x)[TQ][8], int lo, int hi) {
-  reg_stages<WHT>(x, MAP == MAP_A ? 8 : (MAP == MAP_B ? 4 : 0), lo, hi);
+  __builtin_amdgcn_sched_barrier(0);
 }
 
 template <int FROM, int TO>
@@ -125,10 +132,12 @@ __device__ inline uint32_t reduce_lazy(int32_t v) {
   return u < w ? u : w;
 }
 __device__ inline void reduce_all(int32_t (&x)[TQ][8]) {
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int q = 0; q < TQ; q++)
 #pragma unroll
     for (int k = 0; k < 8; k++) x[q][k] = int32_t(reduce_lazy(x[q][k]));
+  __builtin_amdgcn_sched_barrier(0);
 }
 
 __device__ inline void unpack_e8(const uint4& w, int32_t (&c)[8]) {
@@ -145,49 +154,62 @@ __device__ inline void unpack_i8(const uint2& w, int32_t (&c)[8]) {
   for (int k = 0; k < 8; k++) c[k] = int32_t(int8_t(((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xffu));
 }
 
-// a * b in F_p[X]/(X^8 - 17) for canonical a, b: coefficient i sums exactly
-// eight products a_j * b'_(i-j mod 8) with b' = b for j <= i and 17 b above.
-// a is split into 8-bit limbs so every product and every sum of eight stays
-// below 2^27 in a 24-bit multiply-add.
+// a * b in F_p[X]/(X^8 - 17) for canonical a, b. Coefficient i sums eight
+// products a_j * E[7 + i - j] over the sequence E = (17 b_1, ..., 17 b_7,
+// b_0, ..., b_7) (the X^8 = 17 wrap folded into E). a is split into 8-bit
+// limbs; each v_dot2_u32_u16 adds two limb x E products, so every partial
+// sum of eight stays below 2^27.
+typedef unsigned short qf_u16x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t dot2(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(qf_u16x2, a), __builtin_bit_cast(qf_u16x2, b), c, false);
+}
+__device__ inline uint32_t fold_p(uint32_t u) {  // u < 2^31 -> (u & 0xffff) + 15 (u >> 16)
+  return u - uint32_t(__mul24(int32_t(u >> 16), int32_t(P)));
+}
 __device__ inline void mul_alpha(int32_t (&a)[8], const uint4& bw) {
-  uint32_t b[8], b17[8];
+  uint32_t E[15];
   {
     const uint32_t v[4] = {bw.x, bw.y, bw.z, bw.w};
 #pragma unroll
-    for (int i = 0; i < 4; i++) { b[2 * i] = v[i] & 0xffffu; b[2 * i + 1] = v[i] >> 16; }
+    for (int i = 0; i < 4; i++) { E[7 + 2 * i] = v[i] & 0xffffu; E[8 + 2 * i] = v[i] >> 16; }
   }
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    uint32_t u = __umul24(b[j], 17u);  // < 2^21
-    u = u - uint32_t(__mul24(int32_t(u >> 16), int32_t(P)));
+  for (int j = 1; j < 8; j++) {
+    const uint32_t u = fold_p(__umul24(E[7 + j], 17u));  // 17 b_j < 2^21 -> < 2^16 + 255
     const uint32_t w = u - P;
-    b17[j] = u < w ? u : w;
+    E[j - 1] = u < w ? u : w;
   }
-  uint32_t lo[8], hi[8];
+  uint32_t D[15];  // D[m] = (E[m], E[m-1]) as a u16 pair
 #pragma unroll
-  for (int j = 0; j < 8; j++) { lo[j] = uint32_t(a[j]) & 0xffu; hi[j] = uint32_t(a[j]) >> 8; }
+  for (int m = 1; m < 15; m++) D[m] = E[m] | (E[m - 1] << 16);
+  uint32_t lo[4], hi[4];
+#pragma unroll
+  for (int jp = 0; jp < 4; jp++) {
+    const uint32_t pr = uint32_t(a[2 * jp]) | (uint32_t(a[2 * jp + 1]) << 16);
+    lo[jp] = pr & 0x00ff00ffu;
+    hi[jp] = (pr >> 8) & 0x00ff00ffu;
+  }
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     uint32_t sl = 0, sh = 0;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const uint32_t bb = j <= i ? b[i - j] : b17[i + 8 - j];
-      sl = __umul24(lo[j], bb) + sl;
-      sh = __umul24(hi[j], bb) + sh;
+    for (int jp = 0; jp < 4; jp++) {
+      sl = dot2(lo[jp], D[7 + i - 2 * jp], sl);
+      sh = dot2(hi[jp], D[7 + i - 2 * jp], sh);
     }
-    // sh < 2^27: reduce, shift by 8, add sl (< 2^27), reduce
-    uint32_t r = sh - uint32_t(__mul24(int32_t(sh >> 16), int32_t(P)));   // < 2^16 + 15 * 2^11
-    r = (r << 8) + sl;                                                     // < 2^25 + 2^27
-    r = r - uint32_t(__mul24(int32_t(r >> 16), int32_t(P)));               // < 2^16 + 15 * 2^12
-    r = r - uint32_t(__mul24(int32_t(r >> 16), int32_t(P)));               // < 2^16 + 45
+    uint32_t r = fold_p(sh);        // < 2^16 + 15 * 2^11
+    r = (r << 8) + sl;              // < 2^25 + 2^27
+    r = fold_p(fold_p(r));          // < 2^16 + 15
     const uint32_t w = r - P;
     a[i] = int32_t(r < w ? r : w);
   }
 }
 
-// Per-thread element offsets of the mapping-A slots: slot q = thread part + q part.
+// Per-thread byte offsets of the mapping-A slots from the tile's first
+// element: slot q = thread part + q part (the q part is uniform). 32-bit
+// offsets against a uniform base keep addresses out of 64-bit registers.
 struct SlotOffs {
-  uint32_t in_t, out_t;      // thread part (input stride / output stride)
+  uint32_t in_t, out_t;
 };
 __device__ inline uint32_t q_part(const TileArgs& a, int q, uint64_t vs) { return tile_off(a, TWG * q, vs); }
 
@@ -196,15 +218,17 @@ __device__ inline void load_tile(const TileArgs& a, uint64_t T, const SlotOffs& 
   uint64_t vec0, pos0;
   tile_base(a, T, vec0, pos0);
   if (PACK) {
-    const int8_t* bva = static_cast<const int8_t*>(a.in);
     const uint64_t b0 = a.block_first + 8 * (vec0 * a.pack_n + pos0);
+    const char* base = static_cast<const char*>(a.in) + b0;
 #pragma unroll
     for (int q = 0; q < TQ; q++) {
-      const uint64_t byte = b0 + 8 * uint64_t(so.in_t + q_part(a, q, a.pack_n));
+      const uint32_t off = so.in_t + 8u * q_part(a, q, a.pack_n);
+      const uint64_t byte = b0 + off;
       uint2 w;
       if (byte + 8 <= a.available) {
-        w = *reinterpret_cast<const uint2*>(bva + byte);
+        w = *reinterpret_cast<const uint2*>(base + off);
       } else {
+        const int8_t* bva = static_cast<const int8_t*>(a.in);
         w = make_uint2(0, 0);
         for (int j = 0; j < 8; j++)
           if (byte + j < a.available) {
@@ -215,30 +239,32 @@ __device__ inline void load_tile(const TileArgs& a, uint64_t T, const SlotOffs& 
       raw[q] = make_uint4(w.x, w.y, 0, 0);
     }
   } else {
-    const uint4* base = reinterpret_cast<const uint4*>(static_cast<const E8*>(a.in) + vec0 * a.in_vstride + pos0);
+    const char* base = reinterpret_cast<const char*>(static_cast<const E8*>(a.in) + vec0 * a.in_vstride + pos0);
 #pragma unroll
-    for (int q = 0; q < TQ; q++) raw[q] = base[so.in_t + q_part(a, q, a.in_vstride)];
+    for (int q = 0; q < TQ; q++)
+      raw[q] = *reinterpret_cast<const uint4*>(base + so.in_t + 16u * q_part(a, q, a.in_vstride));
   }
 }
 
 __device__ inline void store_tile(const TileArgs& a, uint64_t T, const SlotOffs& so, E8* out, const int32_t (&x)[TQ][8]) {
   uint64_t vec0, pos0;
   tile_base(a, T, vec0, pos0);
-  uint4* base = reinterpret_cast<uint4*>(out + vec0 * a.out_vstride + pos0);
+  char* base = reinterpret_cast<char*>(out + vec0 * a.out_vstride + pos0);
 #pragma unroll
-  for (int q = 0; q < TQ; q++) base[so.out_t + q_part(a, q, a.out_vstride)] = pack_e8(x[q]);
+  for (int q = 0; q < TQ; q++)
+    *reinterpret_cast<uint4*>(base + so.out_t + 16u * q_part(a, q, a.out_vstride)) = pack_e8(x[q]);
 }
 
-template <int MODE, bool PACK>
+template <int MODE, bool PACK, int LO>
 __global__ __launch_bounds__(TWG) void k_tile(TileArgs a) {
   __shared__ uint2 lds[4][TPAD];
   const uint64_t first = uint64_t(blockIdx.x) * a.per_wg;
   const uint64_t last = first + a.per_wg < a.tiles ? first + a.per_wg : a.tiles;
   if (first >= last) return;
-  const int lo = a.logC, hi = a.logC + a.t;  // transformed tile bits
+  const int lo = a.xlo, hi = a.xhi;  // transformed tile bits (runtime form)
   SlotOffs so;
-  so.in_t = tile_off(a, tile_e<MAP_A>(threadIdx.x, 0), PACK ? a.pack_n : a.in_vstride);
-  so.out_t = tile_off(a, tile_e<MAP_A>(threadIdx.x, 0), a.out_vstride);
+  so.in_t = (PACK ? 8u : 16u) * tile_off(a, tile_e<MAP_A>(threadIdx.x, 0), PACK ? a.pack_n : a.in_vstride);
+  so.out_t = 16u * tile_off(a, tile_e<MAP_A>(threadIdx.x, 0), a.out_vstride);
   uint4 raw[TQ];
   load_tile<PACK>(a, first, so, raw);
   for (uint64_t T = first; T < last; T++) {
@@ -252,48 +278,59 @@ __global__ __launch_bounds__(TWG) void k_tile(TileArgs a) {
     constexpr bool WAW = MODE == TM_WAW || MODE == TM_COPY_WAW;
     if (!WAW && T + 1 < last) load_tile<PACK>(a, T + 1, so, raw);
     if (MODE == TM_M || MODE == TM_M_W) {
-      phase<false, MAP_A>(x, lo, hi);
+      phase<false, MAP_A, LO>(x, lo, hi);
       exchange<MAP_A, MAP_B>(x, lds);
-      phase<false, MAP_B>(x, lo, hi);
+      phase<false, MAP_B, LO>(x, lo, hi);
       exchange<MAP_B, MAP_C>(x, lds);
-      phase<false, MAP_C>(x, lo, hi);
-      exchange<MAP_C, MAP_A>(x, lds);
+      phase<false, MAP_C, LO>(x, lo, hi);
+      exchange<MAP_C, MAP_D>(x, lds);
+      phase<false, MAP_D, LO>(x, lo, hi);
+      exchange<MAP_D, MAP_A>(x, lds);
       reduce_all(x);
       store_tile(a, T, so, a.out1, x);
     }
     if (MODE == TM_M_W || MODE == TM_W || MODE == TM_COPY_W) {
-      phase<true, MAP_A>(x, lo, hi);
+      phase<true, MAP_A, LO>(x, lo, hi);
       exchange<MAP_A, MAP_B>(x, lds);
-      phase<true, MAP_B>(x, lo, hi);
+      phase<true, MAP_B, LO>(x, lo, hi);
       exchange<MAP_B, MAP_C>(x, lds);
-      phase<true, MAP_C>(x, lo, hi);
-      exchange<MAP_C, MAP_A>(x, lds);
+      phase<true, MAP_C, LO>(x, lo, hi);
+      exchange<MAP_C, MAP_D>(x, lds);
+      phase<true, MAP_D, LO>(x, lo, hi);
+      exchange<MAP_D, MAP_A>(x, lds);
       reduce_all(x);
       store_tile(a, T, so, MODE == TM_W ? a.out1 : a.out2, x);
     }
     if (WAW) {
-      phase<true, MAP_A>(x, lo, hi);
+      phase<true, MAP_A, LO>(x, lo, hi);
       exchange<MAP_A, MAP_B>(x, lds);
-      uint4 al[TQ];  // alpha_1 of the elements held in mapping C
+      phase<true, MAP_B, LO>(x, lo, hi);
+      exchange<MAP_B, MAP_C>(x, lds);
+      uint4 al[TQ];  // alpha_1 of the elements held in mapping D
       {
         uint64_t vec0, pos0;
         tile_base(a, T, vec0, pos0);
-        const uint4* ab = reinterpret_cast<const uint4*>(a.alpha) + pos0;
-        const uint32_t ct = tile_off(a, tile_e<MAP_C>(threadIdx.x, 0), 0);
+        const char* ab = reinterpret_cast<const char*>(a.alpha + pos0);
+        const uint32_t dt = 16u * tile_off(a, tile_e<MAP_D>(threadIdx.x, 0), 0);
 #pragma unroll
-        for (int q = 0; q < TQ; q++) al[q] = ab[ct + tile_off(a, q, 0)];
+        for (int q = 0; q < TQ; q++) al[q] = *reinterpret_cast<const uint4*>(ab + dt + 16u * tile_off(a, q, 0));
       }
-      phase<true, MAP_B>(x, lo, hi);
-      exchange<MAP_B, MAP_C>(x, lds);
-      phase<true, MAP_C>(x, lo, hi);
+      phase<true, MAP_C, LO>(x, lo, hi);
+      exchange<MAP_C, MAP_D>(x, lds);
+      phase<true, MAP_D, LO>(x, lo, hi);
       reduce_all(x);
 #pragma unroll
-      for (int q = 0; q < TQ; q++) mul_alpha(x[q], al[q]);
-      phase<true, MAP_C>(x, lo, hi);
+      for (int q = 0; q < TQ; q++) {
+        mul_alpha(x[q], al[q]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      phase<true, MAP_D, LO>(x, lo, hi);
+      exchange<MAP_D, MAP_C>(x, lds);
+      phase<true, MAP_C, LO>(x, lo, hi);
       exchange<MAP_C, MAP_B>(x, lds);
-      phase<true, MAP_B>(x, lo, hi);
+      phase<true, MAP_B, LO>(x, lo, hi);
       exchange<MAP_B, MAP_A>(x, lds);
-      phase<true, MAP_A>(x, lo, hi);
+      phase<true, MAP_A, LO>(x, lo, hi);
       reduce_all(x);
       store_tile(a, T, so, MODE == TM_WAW ? a.out1 : a.out2, x);
       if (T + 1 < last) load_tile<PACK>(a, T + 1, so, raw);
@@ -315,12 +352,13 @@ int device_cus() {
 
 }  // namespace
 
-// One tile pass over `vectors` vectors of length 2^L, transforming bits [s, s+t).
-// Requirements: a tile holds 4096 elements, so t + logC + logV = 12 with C = 2^logC
-// contiguous positions (s >= logC) and 2^logV whole vectors (only when s = 0, t = L).
+// One tile pass over `vectors` vectors of length 2^L, transforming bits [s, s+t)
+// (and, with xform_vectors, also the index bits of the 2^logV vectors a tile
+// holds). A tile holds 4096 elements: t + logC + logV = 12 with C = 2^logC
+// contiguous positions (s >= logC) and 2^logV vectors (only when s = 0).
 bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8* alpha, uint64_t vectors,
                uint64_t in_vstride, uint64_t out_vstride, int L, int s, int t, uint64_t block_first,
-               uint64_t available, hipStream_t st) {
+               uint64_t available, hipStream_t st, bool xform_vectors) {
   TileArgs a{};
   a.in = in;
   a.out1 = out1;
@@ -333,20 +371,27 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
   a.pack_n = uint64_t(1) << L;
   a.s = s;
   a.t = t;
+  if (t < 1 || t > 12 || s + t > L) return false;
   if (s == 0) {
-    if (t != std::min(L, 12) || t < 1) return false;
     a.logC = 0;
     a.logV = 12 - t;
+    if (!xform_vectors && t != std::min(L, 12)) return false;
     if (vectors % (uint64_t(1) << a.logV)) return false;
     a.log_lo_chunks = 0;
   } else {
-    if (t < 1 || t > 10 || s + t > L) return false;
+    if (t > 10 || xform_vectors) return false;
     a.logC = 12 - t;
     a.logV = 0;
     if (s < a.logC) return false;
     a.log_lo_chunks = s - a.logC;
   }
   a.log_hi_count = L - s - t;
+  // byte offsets inside a tile must fit 32 bits
+  const uint64_t span_pos = (uint64_t(1) << L) * 16;
+  const uint64_t vmax = (uint64_t(1) << a.logV) - 1;
+  if (vmax * std::max(in_vstride * (pack ? 8 : 16), out_vstride * 16) + span_pos >= (1ull << 32)) return false;
+  a.xlo = a.logC;
+  a.xhi = xform_vectors ? 12 : a.logC + a.t;
   const uint64_t tiles = (vectors << L) / TN;
   if (tiles == 0 || tiles >= (1ull << 31)) return false;
   a.tiles = uint32_t(tiles);
@@ -354,7 +399,18 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
   a.per_wg = uint32_t((tiles + wgs - 1) / wgs);
   const uint32_t grid = uint32_t((tiles + a.per_wg - 1) / a.per_wg);
   dim3 g{grid, 1, 1}, b{uint32_t(TWG), 1, 1};
-#define QF_TILE_LAUNCH(M, PK) hipLaunchKernelGGL((k_tile<M, PK>), g, b, 0, st, a)
+  // compile-time stage ranges for the full-width tiles ([logC, 12)), runtime otherwise
+  const int lo_ct = a.xhi == 12 && (a.xlo == 0 || a.xlo == 2 || a.xlo == 7) ? a.xlo : -1;
+#define QF_TILE_LAUNCH_LO(M, PK, LOV) hipLaunchKernelGGL((k_tile<M, PK, LOV>), g, b, 0, st, a)
+#define QF_TILE_LAUNCH(M, PK)                        \
+  do {                                               \
+    switch (lo_ct) {                                 \
+      case 0: QF_TILE_LAUNCH_LO(M, PK, 0); break;    \
+      case 2: QF_TILE_LAUNCH_LO(M, PK, 2); break;    \
+      case 7: QF_TILE_LAUNCH_LO(M, PK, 7); break;    \
+      default: QF_TILE_LAUNCH_LO(M, PK, -1); break;  \
+    }                                                \
+  } while (0)
   if (pack) {
     if (mode != TM_M) return false;
     QF_TILE_LAUNCH(TM_M, true);
@@ -370,6 +426,7 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
     }
   }
 #undef QF_TILE_LAUNCH
+#undef QF_TILE_LAUNCH_LO
   QF_HIP(hipGetLastError());
   return true;
 }

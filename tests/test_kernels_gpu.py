@@ -39,7 +39,7 @@ def transform(x, bits, wht):
     return y % P
 
 
-def run(mode, x, alpha, L, s, t, vstride_in=None, vstride_out=None):
+def run(mode, x, alpha, L, s, t, vstride_in=None, vstride_out=None, xvec=False):
     import parcomm_amd as pa
     vectors = x.shape[0]
     n = 1 << L
@@ -53,7 +53,7 @@ def run(mode, x, alpha, L, s, t, vstride_in=None, vstride_out=None):
     d_o2 = pa.DeviceBuffer(vectors * vout * 16)
     d_a = pa.DeviceBuffer.from_host(alpha.astype(np.uint16)) if alpha is not None else None
     rc = pa.lib().qf_test_tile_pass(mode, d_in.ptr, d_o1.ptr, d_o2.ptr, d_a.ptr if d_a else None, vectors, vin, vout,
-                                    L, s, t)
+                                    L, s, t, int(xvec))
     assert rc == 0, pa.lib().qf_last_error()
     o1 = d_o1.to_host("uint16").reshape(vectors * vout, 8)
     o2 = d_o2.to_host("uint16").reshape(vectors * vout, 8)
@@ -92,3 +92,32 @@ def test_tile_pass_matches_numpy(vectors, L, s, t, mode):
         w = transform(x, bits, True)
         assert np.array_equal(o1, x)
         assert np.array_equal(o2, transform(e8_mul(w, alpha[None]), bits, True))
+
+
+def transform_vectors(x, nbits, wht):
+    """2x2 maps on the low nbits of the vector index (axis 0)."""
+    y = x.astype(np.int64).copy()
+    idx = np.arange(y.shape[0])
+    for b in range(nbits):
+        lo = idx[((idx >> b) & 1) == 0]
+        hi = lo | (1 << b)
+        u, w = y[lo].copy(), y[hi].copy()
+        if wht:
+            y[lo] = (u + w) % P
+            y[hi] = (u - w) % P
+        else:
+            y[hi] = (w - u) % P
+    return y % P
+
+
+@pytest.mark.parametrize("L", [12, 15])
+@pytest.mark.parametrize("mode", [0, 2])
+def test_tile_pass_over_column_bits(L, mode):
+    # 32 columns x 128 rows per tile: the first commit pass (column bits + row bits 0..6)
+    rng = np.random.default_rng(77 + L + mode)
+    x = rng.integers(0, P, size=(32, 1 << L, 8), dtype=np.int64)
+    print("column-bit tile pass mode", mode, "L", L, flush=True)
+    o1, _ = run(mode, x, None, L, 0, 7, vstride_in=(1 << L) + 16, vstride_out=2 << L, xvec=True)
+    wht = mode == 2
+    assert np.array_equal(o1, transform_vectors(transform(x, range(7), wht), 5, wht))
+
