@@ -102,6 +102,13 @@ int qf_benchmark_claims(const qf_params* p, const qf_committed* c, uint8_t* clai
  * commitment is not consumed. Invalid claims (wrong dimensions, non-canonical
  * elements, false values) fail with the reference's error kinds. */
 int qf_prove(const qf_params* p, const qf_committed* c, const uint8_t* claims, size_t claims_len, qf_proof** out);
+/* Parameters::verify (protocol.rs:266-319): the witness-free verifier over
+ * the public roots (n_roots x 32 bytes), the packed claims and the proof
+ * bytes. Returns QF_OK when the proof is accepted, else an error code with the
+ * rejected check in qf_last_error(). Host only; needs no device. */
+int qf_verify(const qf_params* p, const uint8_t* roots, size_t n_roots, const uint8_t* claims, size_t claims_len,
+              const uint8_t* proof, size_t proof_len);
+
 /* Proof bytes, valid while the handle lives. */
 int qf_proof_bytes(const qf_proof* pr, const uint8_t** data, size_t* len);
 /* Host wall-clock phases of the proof in milliseconds (ProverProfile,

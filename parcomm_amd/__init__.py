@@ -72,6 +72,7 @@ def lib():
         L.qf_claims_size.argtypes = [vp, P(sz)]
         L.qf_benchmark_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
         L.qf_prove.argtypes = [vp, vp, ctypes.c_char_p, sz, P(vp)]
+        L.qf_verify.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, sz, ctypes.c_char_p, sz]
         L.qf_proof_bytes.argtypes = [vp, P(P(ctypes.c_uint8)), P(sz)]
         L.qf_proof_timing.argtypes = [vp, P(ctypes.c_double)]
         L.qf_proof_free.argtypes = [vp]
@@ -304,6 +305,13 @@ class Parameters:
         finally:
             lib().qf_proof_free(h)
         return proof, profile
+
+    def verify(self, statement: "Commitment", claims: Sequence[Claim], proof: bytes):
+        """Parameters::verify (protocol.rs:266-319); raises QfError on rejection."""
+        raw = self.claims_to_bytes(claims)
+        roots = b"".join(statement.roots)
+        _check(lib().qf_verify(self._h, roots, len(statement.roots), raw, len(raw), proof, len(proof)))
+        return True
 
     def synthetic_bva(self, seed, sigma=20.0):
         """Device-resident synthetic BVA: clip(round(N(0, sigma))) on physical entries."""

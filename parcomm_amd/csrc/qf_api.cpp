@@ -165,6 +165,18 @@ E8 read_scalar(const uint8_t* b) {
   require(e8_from_bytes(b, v), QF_ERR_ENCODING, "non-canonical field element in claims");
   return v;
 }
+std::vector<ClaimIn> parse_claims(const ParamsState& p, const uint8_t* q) {
+  std::vector<ClaimIn> cl;
+  for (auto [r, cc] : claim_dims(p)) {
+    ClaimIn x;
+    for (size_t i = 0; i < r; i++, q += 16) x.row.push_back(read_scalar(q));
+    for (size_t i = 0; i < cc; i++, q += 16) x.col.push_back(read_scalar(q));
+    x.value = read_scalar(q);
+    q += 16;
+    cl.push_back(std::move(x));
+  }
+  return cl;
+}
 }  // namespace
 
 extern "C" {
@@ -302,19 +314,20 @@ int qf_prove(const qf_params* p, const qf_committed* c, const uint8_t* claims, s
   return guarded([&] {
     require(p && c && claims && out && c->p == p, QF_ERR_ARGUMENT, "null or mismatched argument");
     require(claims_len == claims_bytes(*p), QF_ERR_INVALID_PARAMETERS, "claims length mismatch");
-    std::vector<ClaimIn> cl;
-    const uint8_t* q = claims;
-    for (auto [r, cc] : claim_dims(*p)) {
-      ClaimIn x;
-      for (size_t i = 0; i < r; i++, q += 16) x.row.push_back(read_scalar(q));
-      for (size_t i = 0; i < cc; i++, q += 16) x.col.push_back(read_scalar(q));
-      x.value = read_scalar(q);
-      q += 16;
-      cl.push_back(std::move(x));
-    }
     std::unique_ptr<qf_proof> pr(new qf_proof);
-    pr->bytes = prove(*p, *c, cl, pr->t);
+    pr->bytes = prove(*p, *c, parse_claims(*p, claims), pr->t);
     *out = pr.release();
+  });
+}
+
+int qf_verify(const qf_params* p, const uint8_t* roots, size_t n_roots, const uint8_t* claims, size_t claims_len,
+              const uint8_t* proof, size_t proof_len) {
+  return guarded([&] {
+    require(p && roots && claims && proof, QF_ERR_ARGUMENT, "null argument");
+    require(claims_len == claims_bytes(*p), QF_ERR_INVALID_PARAMETERS, "claims length mismatch");
+    std::vector<Digest> rs(n_roots);
+    for (size_t i = 0; i < n_roots; i++) std::memcpy(rs[i].data(), roots + 32 * i, 32);
+    verify(*p, rs, parse_claims(*p, claims), proof, proof_len);
   });
 }
 

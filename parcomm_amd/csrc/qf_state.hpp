@@ -59,6 +59,9 @@ Bytes prove(const ParamsState& p, const CommittedState& c, const std::vector<Cla
 // the commitment transcript, values evaluated on the device BVA.
 std::vector<ClaimIn> benchmark_claims(const ParamsState& p, const CommittedState& c);
 std::string statement_json(const ParamsState& p, const CommittedState& c);
+// Parameters::verify (protocol.rs:266-319): throws QfError on rejection.
+void verify(const ParamsState& p, const std::vector<Digest>& roots, const std::vector<ClaimIn>& claims,
+            const uint8_t* proof, size_t len);
 std::string claims_json(const std::vector<ClaimIn>& claims);
 
 }  // namespace qf

@@ -39,6 +39,7 @@ def check_against_oracle(oracle, shapes, chunk_log, values, packed_log=4, recomp
     assert len(proof) == len(want)
     assert proof == want
     assert profile["total_seconds"] > 0
+    assert p.verify(committed.statement, claims, proof)
     return p, committed, claims, proof
 
 
