@@ -62,10 +62,8 @@ void validate_device(const ParamsState* p, const int8_t* d_bva, hipStream_t st) 
   if (p->zero_ranges.empty()) p->zero_ranges = padding_ranges(p->layout);
   if (p->zero_ranges.empty()) return;
   size_t count = p->zero_ranges.size() / 2;
-  uint64_t* d_ranges = nullptr;
-  uint32_t* d_flag = nullptr;
-  QF_HIP(hipMalloc(&d_ranges, p->zero_ranges.size() * 8));
-  QF_HIP(hipMalloc(&d_flag, 4));
+  uint64_t* d_ranges = static_cast<uint64_t*>(dmalloc(p->zero_ranges.size() * 8, st));
+  uint32_t* d_flag = static_cast<uint32_t*>(dmalloc(4, st));
   QF_HIP(hipMemcpyAsync(d_ranges, p->zero_ranges.data(), p->zero_ranges.size() * 8, hipMemcpyHostToDevice, st));
   QF_HIP(hipMemsetAsync(d_flag, 0, 4, st));
   for (size_t first = 0; first < count; first += 65535) {
@@ -75,16 +73,15 @@ void validate_device(const ParamsState* p, const int8_t* d_bva, hipStream_t st) 
   uint32_t flag = 0;
   QF_HIP(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, st));
   QF_HIP(hipStreamSynchronize(st));
-  (void)hipFree(d_ranges);
-  (void)hipFree(d_flag);
+  dfree(d_ranges, st);
+  dfree(d_flag, st);
   require(flag == 0, QF_ERR_INVALID_PARAMETERS, "nonzero boundary padding");
 }
 
 void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_mode, qf_committed** out) {
   require(len == p->layout.active, QF_ERR_INVALID_PARAMETERS, "active BVA length mismatch");
   require(cache_mode == QF_CACHE_CACHED || cache_mode == QF_CACHE_RECOMPUTE, QF_ERR_ARGUMENT, "cache mode");
-  hipStream_t st;
-  QF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipStream_t st = thread_stream();
   std::unique_ptr<qf_committed> c(new qf_committed);
   try {
     c->p = p;
@@ -109,8 +106,8 @@ void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_
       E8* cw = nullptr;
       uint32_t* tree = nullptr;
       if (cache_mode == QF_CACHE_CACHED || c->d_cw.empty()) {
-        QF_HIP(hipMalloc(&cw, cw_bytes));
-        QF_HIP(hipMalloc(&tree, tree_bytes));
+        cw = static_cast<E8*>(dmalloc(cw_bytes, st));
+        tree = static_cast<uint32_t*>(dmalloc(tree_bytes, st));
         c->d_cw.push_back(cw);
         c->d_tree.push_back(tree);
       } else {
@@ -138,10 +135,9 @@ void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
     for (auto& e : ev) (void)hipEventDestroy(e);
   } catch (...) {
-    (void)hipStreamDestroy(st);
+    (void)hipStreamSynchronize(st);
     throw;
   }
-  QF_HIP(hipStreamDestroy(st));
   *out = c.release();
 }
 

@@ -10,6 +10,18 @@ namespace qf {
     if (e_ != hipSuccess) throw QfError(QF_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Stream-ordered device allocation from the device's default memory pool,
+// which keeps freed blocks cached for reuse: no device-wide synchronisation
+// and no page-table setup on the hot path.
+void* dmalloc(size_t bytes, hipStream_t st);
+void dfree(void* p, hipStream_t st);
+// The stream of the innermost device context on this thread (null stream if none).
+hipStream_t current_stream();
+// This thread's persistent work stream (non-blocking, created on first use and
+// kept for the life of the process): commits and proofs issued from one
+// thread run on it, so pooled blocks are reused in stream order.
+hipStream_t thread_stream();
+
 struct CommitGeom {
   int log_n;          // log2 of message rows per column (chunk_log - 8)
   LeafPrefix leaf;

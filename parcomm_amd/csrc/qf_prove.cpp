@@ -47,19 +47,21 @@ struct DCode {
   bool own_tree = false;
   const E8* cwp = nullptr;  // borrowed codeword when cw is empty
   uint64_t R = 0, cols = 32;
+  hipStream_t st = nullptr;  // stream the owned tree was allocated on
   Digest root{};
   const E8* data() const { return cw.p ? cw.p : cwp; }
   DCode() = default;
   DCode(DCode&& o) noexcept
-      : cw(std::move(o.cw)), tree(o.tree), own_tree(o.own_tree), cwp(o.cwp), R(o.R), cols(o.cols), root(o.root) {
+      : cw(std::move(o.cw)), tree(o.tree), own_tree(o.own_tree), cwp(o.cwp), R(o.R), cols(o.cols), st(o.st),
+        root(o.root) {
     o.tree = nullptr;
     o.own_tree = false;
   }
   DCode& operator=(DCode&& o) noexcept {
     if (this != &o) {
-      if (own_tree && tree) (void)hipFree(tree);
+      if (own_tree) dfree(tree, st);
       cw = std::move(o.cw); tree = o.tree; own_tree = o.own_tree; cwp = o.cwp; R = o.R; cols = o.cols;
-      root = o.root;
+      st = o.st; root = o.root;
       o.tree = nullptr; o.own_tree = false;
     }
     return *this;
@@ -76,13 +78,14 @@ struct DCode {
   std::vector<E8> rows(DevCtx& c, const std::vector<uint64_t>& sel) const {
     return dv::gather_rows(c, data(), R, cols, sel);
   }
-  ~DCode() { if (own_tree && tree) (void)hipFree(tree); }
+  ~DCode() { if (own_tree) dfree(tree, st); }
 };
 DCode alloc_code(uint64_t R) {
   DCode d;
   d.R = R;
   d.cw = DVec(32 * R);
-  QF_HIP(hipMalloc(&d.tree, (2 * R - 1) * 32));
+  d.st = current_stream();
+  d.tree = static_cast<uint32_t*>(dmalloc((2 * R - 1) * 32, d.st));
   d.own_tree = true;
   return d;
 }

@@ -36,12 +36,12 @@ struct CommittedState {
   double total_ms = 0, leaf_ms = 0, encode_ms = 0;
   ~CommittedState() {
     std::vector<void*> freed;
-    auto release = [&](void* q) {
-      if (q && std::find(freed.begin(), freed.end(), q) == freed.end()) { freed.push_back(q); (void)hipFree(q); }
+    auto release = [&](void* q) {  // codewords and trees come from the stream-ordered pool
+      if (q && std::find(freed.begin(), freed.end(), q) == freed.end()) { freed.push_back(q); dfree(q, thread_stream()); }
     };
     for (auto* q : d_cw) release(q);
     for (auto* q : d_tree) release(q);
-    if (owns_bva) release(const_cast<int8_t*>(d_bva));
+    if (owns_bva && d_bva) (void)hipFree(const_cast<int8_t*>(d_bva));
   }
 };
 

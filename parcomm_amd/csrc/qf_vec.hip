@@ -1,5 +1,8 @@
 // Device vector operations and BVA-scale kernels of the opening (see qf_vec.hpp).
 #include "qf_vec.hpp"
+#include <map>
+#include <mutex>
+#include <unordered_map>
 
 namespace qf {
 
@@ -419,22 +422,118 @@ E8* scratch_tab() {
 }
 }  // namespace
 
+// ---------------------------------------------------------------- allocation
+// A caching allocator: released blocks are kept by exact (rounded) size and
+// handed out again, so repeated commits and proofs of one shape never call
+// hipMalloc/hipFree (both synchronise the device) after the first. A block
+// released on stream S carries an event recorded on S; a later user on
+// another stream waits for that event before touching it.
+namespace {
+thread_local hipStream_t g_current_stream = nullptr;
+struct CachedBlock { void* p; hipEvent_t ready; };
+struct BlockCache {
+  std::mutex mu;
+  std::unordered_map<void*, size_t> live;               // pointer -> rounded size
+  std::multimap<size_t, CachedBlock> idle;              // rounded size -> block
+  size_t idle_bytes = 0;
+  static constexpr size_t LIMIT = size_t(64) << 30;     // idle bytes kept before trimming
+  static size_t round(size_t b) {
+    const size_t g = b >= (size_t(1) << 20) ? (size_t(2) << 20) : 256;
+    return (std::max<size_t>(b, 16) + g - 1) / g * g;
+  }
+  void trim_locked(size_t keep) {
+    while (idle_bytes > keep && !idle.empty()) {
+      auto it = idle.begin();
+      (void)hipEventSynchronize(it->second.ready);
+      (void)hipEventDestroy(it->second.ready);
+      (void)hipFree(it->second.p);
+      idle_bytes -= it->first;
+      idle.erase(it);
+    }
+  }
+};
+BlockCache& cache() {
+  static BlockCache* c = new BlockCache;  // lives until process exit
+  return *c;
+}
+}  // namespace
+hipStream_t current_stream() { return g_current_stream; }
+hipStream_t thread_stream() {
+  thread_local hipStream_t st = nullptr;
+  if (!st) QF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  return st;
+}
+void* dmalloc(size_t bytes, hipStream_t st) {
+  BlockCache& bc = cache();
+  const size_t size = BlockCache::round(bytes);
+  {
+    std::lock_guard<std::mutex> g(bc.mu);
+    auto it = bc.idle.find(size);
+    if (it != bc.idle.end()) {
+      CachedBlock b = it->second;
+      bc.idle.erase(it);
+      bc.idle_bytes -= size;
+      bc.live[b.p] = size;
+      QF_HIP(hipStreamWaitEvent(st, b.ready, 0));
+      (void)hipEventDestroy(b.ready);
+      return b.p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, size);
+  if (e == hipErrorOutOfMemory) {  // give the idle blocks back and retry once
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> g(bc.mu);
+    bc.trim_locked(0);
+    e = hipMalloc(&p, size);
+  }
+  if (e != hipSuccess) throw QfError(QF_ERR_DEVICE, std::string("device allocation: ") + hipGetErrorString(e));
+  std::lock_guard<std::mutex> g(bc.mu);
+  bc.live[p] = size;
+  return p;
+}
+void dfree(void* p, hipStream_t st) {
+  if (!p) return;
+  BlockCache& bc = cache();
+  hipEvent_t ev;
+  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, st) != hipSuccess) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(p);
+    std::lock_guard<std::mutex> g(bc.mu);
+    bc.live.erase(p);
+    return;
+  }
+  std::lock_guard<std::mutex> g(bc.mu);
+  auto it = bc.live.find(p);
+  if (it == bc.live.end()) { (void)hipEventDestroy(ev); return; }
+  const size_t size = it->second;
+  bc.live.erase(it);
+  bc.idle.insert({size, {p, ev}});
+  bc.idle_bytes += size;
+  bc.trim_locked(BlockCache::LIMIT);
+}
+
 // ---------------------------------------------------------------- DVec / DevCtx
-DVec::DVec(uint64_t len) : n(len) { QF_HIP(hipMalloc(&p, std::max<uint64_t>(len, 1) * 16)); }
+DVec::DVec(uint64_t len) : n(len), st(current_stream()) {
+  p = static_cast<E8*>(dmalloc(std::max<uint64_t>(len, 1) * 16, st));
+}
 DVec& DVec::operator=(DVec&& o) noexcept {
-  if (this != &o) { if (p) (void)hipFree(p); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+  if (this != &o) { dfree(p, st); p = o.p; n = o.n; st = o.st; o.p = nullptr; o.n = 0; }
   return *this;
 }
-DVec::~DVec() { if (p) (void)hipFree(p); }
+DVec::~DVec() { dfree(p, st); }
 DevCtx::DevCtx() {
-  QF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  QF_HIP(hipMalloc(&partials, size_t(MAX_GROUPS) * 64 * 32));
+  st = thread_stream();
+  prev = g_current_stream;
+  g_current_stream = st;
+  partials = static_cast<uint32_t*>(dmalloc(size_t(MAX_GROUPS) * 64 * 32, st));
   QF_HIP(hipHostMalloc(&host_partials, size_t(MAX_GROUPS) * 64 * 32, 0));
 }
 DevCtx::~DevCtx() {
-  if (partials) (void)hipFree(partials);
+  dfree(partials, st);
+  if (st) (void)hipStreamSynchronize(st);
   if (host_partials) (void)hipHostFree(host_partials);
-  if (st) (void)hipStreamDestroy(st);
+  g_current_stream = prev;
 }
 void DevCtx::sync() { QF_HIP(hipStreamSynchronize(st)); }
 
@@ -506,15 +605,14 @@ void col_fold(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, co
 }
 std::vector<E8> gather_rows(DevCtx& c, const E8* cw, uint64_t R, uint64_t cols, const std::vector<uint64_t>& rows) {
   if (rows.empty()) return {};
-  uint64_t* d_rows = nullptr;
-  QF_HIP(hipMalloc(&d_rows, rows.size() * 8));
+  uint64_t* d_rows = static_cast<uint64_t*>(dmalloc(rows.size() * 8, c.st));
   DVec tmp(rows.size() * cols);
   QF_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, c.st));
   uint64_t total = rows.size() * cols;
   hipLaunchKernelGGL(k_gather_rows, dim3(uint32_t((total + WG - 1) / WG)), dim3(WG), 0, c.st, cw, R, cols, d_rows,
                      uint64_t(rows.size()), tmp.p);
   auto out = download(c, tmp.p, total);
-  (void)hipFree(d_rows);
+  dfree(d_rows, c.st);
   return out;
 }
 std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, uint64_t R,
@@ -522,19 +620,17 @@ std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, u
   int depth = ilog2u(R);
   std::vector<std::vector<Digest>> out(rows.size());
   if (rows.empty() || depth == 0) return out;
-  uint64_t* d_rows = nullptr;
-  uint32_t* d_out = nullptr;
   uint64_t total = rows.size() * depth;
-  QF_HIP(hipMalloc(&d_rows, rows.size() * 8));
-  QF_HIP(hipMalloc(&d_out, total * 32));
+  uint64_t* d_rows = static_cast<uint64_t*>(dmalloc(rows.size() * 8, c.st));
+  uint32_t* d_out = static_cast<uint32_t*>(dmalloc(total * 32, c.st));
   QF_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, c.st));
   hipLaunchKernelGGL(k_gather_paths, dim3(uint32_t((total + WG - 1) / WG)), dim3(WG), 0, c.st, tree, R, depth, d_rows,
                      uint64_t(rows.size()), d_out);
   std::vector<uint8_t> host(total * 32);
   QF_HIP(hipMemcpyAsync(host.data(), d_out, total * 32, hipMemcpyDeviceToHost, c.st));
   c.sync();
-  (void)hipFree(d_rows);
-  (void)hipFree(d_out);
+  dfree(d_rows, c.st);
+  dfree(d_out, c.st);
   for (size_t k = 0; k < rows.size(); k++) {
     out[k].resize(depth);
     for (int l = 0; l < depth; l++) std::memcpy(out[k][l].data(), host.data() + (k * depth + l) * 32, 32);
@@ -543,20 +639,19 @@ std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, u
 }
 void scatter_add(DevCtx& c, E8* dst, const std::vector<uint64_t>& idx, const std::vector<E8>& vals) {
   if (idx.empty()) return;
-  uint64_t* d_idx = nullptr;
-  QF_HIP(hipMalloc(&d_idx, idx.size() * 8));
+  uint64_t* d_idx = static_cast<uint64_t*>(dmalloc(idx.size() * 8, c.st));
   DVec v = upload(c, vals);
   QF_HIP(hipMemcpyAsync(d_idx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, c.st));
   hipLaunchKernelGGL(k_scatter_add, dim3(uint32_t((idx.size() + WG - 1) / WG)), dim3(WG), 0, c.st, dst, d_idx, v.p,
                      uint64_t(idx.size()));
   c.sync();
-  (void)hipFree(d_idx);
+  dfree(d_idx, c.st);
 }
 }  // namespace dv
 
 // ---------------------------------------------------------------- BVA-scale wrappers
 static void upload_pairs(DevCtx& c, const std::vector<std::array<E8, 2>>& pairs, E8 (**out)[2]) {
-  QF_HIP(hipMalloc(reinterpret_cast<void**>(out), std::max<size_t>(pairs.size(), 1) * 32));
+  *out = reinterpret_cast<E8(*)[2]>(dmalloc(std::max<size_t>(pairs.size(), 1) * 32, c.st));
   if (!pairs.empty()) QF_HIP(hipMemcpyAsync(*out, pairs.data(), pairs.size() * 32, hipMemcpyHostToDevice, c.st));
 }
 
@@ -581,7 +676,7 @@ std::vector<E8> tensor_project(DevCtx& c, const int8_t* bva, uint64_t available,
   E8(*d_pairs)[2] = nullptr;
   ProjTaskDev* d_tasks = nullptr;
   upload_pairs(c, pairs, &d_pairs);
-  QF_HIP(hipMalloc(&d_tasks, chunks.size() * sizeof(ProjTaskDev)));
+  d_tasks = static_cast<ProjTaskDev*>(dmalloc(chunks.size() * sizeof(ProjTaskDev), c.st));
   QF_HIP(hipMemcpyAsync(d_tasks, chunks.data(), chunks.size() * sizeof(ProjTaskDev), hipMemcpyHostToDevice, c.st));
   DVec out(chunks.size() * L);
   for (size_t first = 0; first < chunks.size(); first += 65535) {
@@ -600,8 +695,8 @@ std::vector<E8> tensor_project(DevCtx& c, const int8_t* bva, uint64_t available,
   }
   QF_HIP(hipGetLastError());
   auto parts = dv::download(c, out);
-  (void)hipFree(d_pairs);
-  (void)hipFree(d_tasks);
+  dfree(d_pairs, c.st);
+  dfree(d_tasks, c.st);
   for (size_t ch = 0; ch < chunks.size(); ch++)
     for (int l = 0; l < L; l++) result[owner[ch] * L + l] = e8_add(result[owner[ch] * L + l], parts[ch * L + l]);
   return result;
@@ -635,7 +730,7 @@ void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, con
   E8(*d_pairs)[2] = nullptr;
   FillTaskDev* d_tasks = nullptr;
   upload_pairs(c, pairs, &d_pairs);
-  QF_HIP(hipMalloc(&d_tasks, chunks.size() * sizeof(FillTaskDev)));
+  d_tasks = static_cast<FillTaskDev*>(dmalloc(chunks.size() * sizeof(FillTaskDev), c.st));
   QF_HIP(hipMemcpyAsync(d_tasks, chunks.data(), chunks.size() * sizeof(FillTaskDev), hipMemcpyHostToDevice, c.st));
   for (size_t first = 0; first < chunks.size(); first += 65535) {
     uint32_t n = uint32_t(std::min<size_t>(65535, chunks.size() - first));
@@ -643,8 +738,8 @@ void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, con
   }
   QF_HIP(hipGetLastError());
   c.sync();
-  (void)hipFree(d_pairs);
-  (void)hipFree(d_tasks);
+  dfree(d_pairs, c.st);
+  dfree(d_tasks, c.st);
 }
 
 void block_fold(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t block_len, uint64_t blocks,

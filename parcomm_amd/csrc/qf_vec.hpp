@@ -10,22 +10,27 @@
 namespace qf {
 
 // Owning device buffer of `n` Ext8 elements.
+// Allocated and released in order on the thread's current stream.
 struct DVec {
   E8* p = nullptr;
   uint64_t n = 0;
+  hipStream_t st = nullptr;
   DVec() = default;
   explicit DVec(uint64_t len);
   DVec(const DVec&) = delete;
   DVec& operator=(const DVec&) = delete;
-  DVec(DVec&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DVec(DVec&& o) noexcept : p(o.p), n(o.n), st(o.st) { o.p = nullptr; o.n = 0; }
   DVec& operator=(DVec&& o) noexcept;
   ~DVec();
   void resize_view(uint64_t len) { n = len; }  // shrink the logical length (storage kept)
 };
 
-// Scratch for reduction partials, owned by the prover of one proof.
+// Stream and scratch for reduction partials, owned by the prover of one
+// proof; while it lives it is the thread's current stream. Its destructor
+// drains the stream.
 struct DevCtx {
   hipStream_t st = nullptr;
+  hipStream_t prev = nullptr;
   uint32_t* partials = nullptr;  // [max_groups][2][8] u32
   uint32_t* host_partials = nullptr;
   static constexpr uint32_t MAX_GROUPS = 1024;
