@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--chunk-log", type=int, default=30)
     ap.add_argument("--seed", type=int, default=20260)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is designed, not built"}))
@@ -132,6 +133,22 @@ def main():
     verify_s = time.perf_counter() - tv
     c.close()
     print(f"# proof verified in {verify_s:.3f} s", file=sys.stderr, flush=True)
+
+    # the plain per-matrix GPU form: one launch and host round trip per bucket term
+    pa.set_launch_mode(pa.LAUNCH_PER_MATRIX)
+    try:
+        pm = step()
+    finally:
+        pa.set_launch_mode(pa.LAUNCH_BATCHED)
+    per_matrix_s = pm["commit_s"] + pm["open_s"]
+    print(f"# per-matrix form: commit {pm['commit_s']:.3f} s open {pm['open_s']:.3f} s", file=sys.stderr, flush=True)
+
+    # serving: resident commitment, Poisson opening requests (mean 50/s), concurrent workers
+    from parcomm_amd.serve import serve
+    c = params.commit(bva)
+    serving, _ = serve(params, c, claims, n_requests=args.serve_requests, rate=50.0, seed=args.seed, workers=4)
+    c.close()
+    print(f"# serving {serving}", file=sys.stderr, flush=True)
     mean = lambda k: sum(t[k] for t in timings) / len(timings)
     phases = {k: round(sum(t["profile"][k] for t in timings) / len(timings), 4) for k in timings[0]["profile"]}
     out = {
@@ -151,7 +168,10 @@ def main():
                    "active_entries": params.active, "blocks": blocks, "phases_timed": ["commit", "open"],
                    "cache_mode": "cached", "setup_s": round(setup_s, 2), "commit_s": round(mean("commit_s"), 4),
                    "open_s": round(mean("open_s"), 4), "open_phases_s": phases,
-                   "proof_bytes": timings[0]["proof_bytes"], "verified": True, "verify_s": round(verify_s, 4)},
+                   "proof_bytes": timings[0]["proof_bytes"], "verified": True, "verify_s": round(verify_s, 4),
+                   "per_matrix_s": round(per_matrix_s, 4),
+                   "batched_speedup": round(per_matrix_s / (mean("commit_s") + mean("open_s")), 3),
+                   "serving": serving},
         "roofline": roofline,
         "cpu_baseline": None,
     }

@@ -96,6 +96,11 @@ int qf_claims_size(const qf_params* p, size_t* bytes);
  * committed BVA (tbv.rs:143-185, evaluate_claims). Writes qf_claims_size bytes. */
 int qf_benchmark_claims(const qf_params* p, const qf_committed* c, uint8_t* claims, size_t cap);
 
+/* tbv.rs:143-185 (evaluate_claims): fills the value of every claim in the
+ * packed list from its row and column points, on the committed BVA. The
+ * values already in the buffer are ignored. */
+int qf_evaluate_claims(const qf_params* p, const qf_committed* c, uint8_t* claims, size_t claims_len);
+
 /* Parameters::prove (protocol.rs:201-263): TBV compile, packed even/odd
  * reduction, PCS opening (batched over blocks, or single-block) and Decide,
  * returning the bincode Proof (protocol.rs:57-95, Proof::to_bytes). The
@@ -129,11 +134,24 @@ int qf_device_upload(void* device_dst, const void* host_src, size_t bytes);
 int qf_device_download(void* host_dst, const void* device_src, size_t bytes);
 int qf_device_free(void* device_ptr);
 
+/* Free and total bytes of the current device (admission control of the
+ * serving loop). */
+int qf_device_memory(size_t* free_bytes, size_t* total_bytes);
+
 /* Synthetic workload generator (no reference counterpart; the reference CLI
  * streams real checkpoints): fills the active BVA in device memory with
  * clip(round(N(0, sigma)), -128, 127) on every physical entry and zero on
  * boundary padding, from a counter-based generator keyed by `seed`. */
 int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* device_out);
+
+/* Launch form of the opening's per-term kernels (no reference counterpart; the
+ * reference runs on CPU threads): QF_LAUNCH_BATCHED (default) issues one launch
+ * per phase over every bucket term of every matrix; QF_LAUNCH_PER_MATRIX issues
+ * one launch and one host round trip per term, the plain per-matrix GPU form
+ * the benchmark compares against. Proofs are identical in both. Process-wide. */
+#define QF_LAUNCH_BATCHED 0
+#define QF_LAUNCH_PER_MATRIX 1
+int qf_set_launch_mode(int mode);
 
 /* Kernel-level entry point for the parity tests (no reference counterpart):
  * one register-radix transform pass (mode 0 Mobius, 1 Mobius then WHT into

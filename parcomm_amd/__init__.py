@@ -72,12 +72,15 @@ def lib():
         L.qf_claims_size.argtypes = [vp, P(sz)]
         L.qf_benchmark_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
         L.qf_prove.argtypes = [vp, vp, ctypes.c_char_p, sz, P(vp)]
+        L.qf_evaluate_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
         L.qf_verify.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, sz, ctypes.c_char_p, sz]
         L.qf_proof_bytes.argtypes = [vp, P(P(ctypes.c_uint8)), P(sz)]
         L.qf_proof_timing.argtypes = [vp, P(ctypes.c_double)]
         L.qf_proof_free.argtypes = [vp]
         L.qf_proof_free.restype = None
         L.qf_statement_json.argtypes = [vp, vp, ctypes.c_char_p, sz, P(sz)]
+        L.qf_set_launch_mode.argtypes = [i32]
+        L.qf_device_memory.argtypes = [P(sz), P(sz)]
         L.qf_test_tile_pass.argtypes = [i32, vp, vp, vp, vp, u64, u64, u64, i32, i32, i32, i32]
         _lib = L
     return _lib
@@ -86,6 +89,15 @@ def lib():
 def _check(status):
     if status != 0:
         raise QfError(status, lib().qf_last_error().decode())
+
+
+LAUNCH_BATCHED = 0
+LAUNCH_PER_MATRIX = 1
+
+
+def set_launch_mode(mode):
+    """Batched (one launch per phase over all matrices) or plain per-matrix launches."""
+    _check(lib().qf_set_launch_mode(mode))
 
 
 class DeviceBuffer:
@@ -287,6 +299,16 @@ class Parameters:
         _check(lib().qf_claims_size(self._h, ctypes.byref(n)))
         buf = ctypes.create_string_buffer(n.value)
         _check(lib().qf_benchmark_claims(self._h, committed._h, buf, n.value))
+        return self.claims_from_bytes(buf.raw)
+
+    def evaluate_claims(self, committed: "Committed", points) -> List[Claim]:
+        """tbv.rs:143-185: claims at the given (row point, col point) pairs, one per matrix."""
+        raw = bytearray()
+        for row, col in points:
+            for v in list(row) + list(col) + [bytes(16)]:
+                raw += v
+        buf = ctypes.create_string_buffer(bytes(raw), len(raw))
+        _check(lib().qf_evaluate_claims(self._h, committed._h, buf, len(raw)))
         return self.claims_from_bytes(buf.raw)
 
     def prove(self, committed: "Committed", claims: Sequence[Claim]):

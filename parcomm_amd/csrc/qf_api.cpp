@@ -1,6 +1,7 @@
 // C ABI of the product library (declared in include/parcomm_amd.h).
 #include "../../include/parcomm_amd.h"
 #include "qf_state.hpp"
+#include "qf_vec.hpp"
 #include <memory>
 
 using namespace qf;
@@ -59,7 +60,10 @@ __global__ void k_check_zero(const int8_t* bva, const uint64_t* ranges, uint32_t
 }
 
 void validate_device(const ParamsState* p, const int8_t* d_bva, hipStream_t st) {
-  if (p->zero_ranges.empty()) p->zero_ranges = padding_ranges(p->layout);
+  {
+    std::lock_guard<std::mutex> g(p->mu);
+    if (p->zero_ranges.empty()) p->zero_ranges = padding_ranges(p->layout);
+  }
   if (p->zero_ranges.empty()) return;
   size_t count = p->zero_ranges.size() / 2;
   uint64_t* d_ranges = static_cast<uint64_t*>(dmalloc(p->zero_ranges.size() * 8, st));
@@ -306,6 +310,21 @@ int qf_benchmark_claims(const qf_params* p, const qf_committed* c, uint8_t* clai
   });
 }
 
+int qf_evaluate_claims(const qf_params* p, const qf_committed* c, uint8_t* claims, size_t claims_len) {
+  return guarded([&] {
+    require(p && c && claims && c->p == p, QF_ERR_ARGUMENT, "null or mismatched argument");
+    require(claims_len == claims_bytes(*p), QF_ERR_INVALID_PARAMETERS, "claims length mismatch");
+    auto cl = parse_claims(*p, claims);
+    evaluate_claims(*p, *c, cl);
+    uint8_t* o = claims;
+    for (auto& x : cl) {
+      o += 16 * (x.row.size() + x.col.size());
+      std::memcpy(o, &x.value, 16);
+      o += 16;
+    }
+  });
+}
+
 int qf_prove(const qf_params* p, const qf_committed* c, const uint8_t* claims, size_t claims_len, qf_proof** out) {
   return guarded([&] {
     require(p && c && claims && out && c->p == p, QF_ERR_ARGUMENT, "null or mismatched argument");
@@ -366,6 +385,20 @@ int qf_test_tile_pass(int mode, const void* device_in, void* device_out1, void* 
                         xform_vectors != 0);
     require(ok, QF_ERR_ARGUMENT, "geometry does not fit a tile pass");
     QF_HIP(hipDeviceSynchronize());
+  });
+}
+
+int qf_set_launch_mode(int mode) {
+  return guarded([&] {
+    require(mode == LAUNCH_BATCHED || mode == LAUNCH_PER_MATRIX, QF_ERR_ARGUMENT, "launch mode");
+    set_launch_mode(mode);
+  });
+}
+
+int qf_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  return guarded([&] {
+    require(free_bytes && total_bytes, QF_ERR_ARGUMENT, "null argument");
+    QF_HIP(hipMemGetInfo(free_bytes, total_bytes));
   });
 }
 

@@ -11,6 +11,7 @@
 namespace qf {
 
 const E8* ParamsState::alpha_device(size_t level) const {
+  std::lock_guard<std::mutex> g(mu);
   if (d_alpha_levels.size() != pp.codes.size()) d_alpha_levels.assign(pp.codes.size(), nullptr);
   if (!d_alpha_levels[level]) {
     const auto& a = pp.codes[level].alpha1;
@@ -992,7 +993,6 @@ std::string claims_json(const std::vector<ClaimIn>& claims) {  // tbv.rs:6-11
 
 // main.rs:188-209 with tbv.rs:143-185 evaluated on the device
 std::vector<ClaimIn> benchmark_claims(const ParamsState& p, const CommittedState& cs) {
-  DevCtx c;
   Outer t;
   t.object("benchmark-points", statement_json(p, cs));
   std::vector<ClaimIn> claims;
@@ -1006,6 +1006,22 @@ std::vector<ClaimIn> benchmark_claims(const ParamsState& p, const CommittedState
     cl.value = E8{};
     claims.push_back(cl);
   }
+  evaluate_claims(p, cs, claims);
+  return claims;
+}
+
+// tbv.rs:143-185: values of the matrices at the claims' points, on the device BVA.
+void evaluate_claims(const ParamsState& p, const CommittedState& cs, std::vector<ClaimIn>& claims) {
+  require(claims.size() == p.layout.shapes.size(), QF_ERR_INVALID_PARAMETERS, "claim count mismatch");
+  for (size_t i = 0; i < claims.size(); i++) {
+    uint64_t rp = 1, cp = 1;
+    while (rp < p.layout.shapes[i].rows) rp <<= 1;
+    while (cp < p.layout.shapes[i].cols) cp <<= 1;
+    require(claims[i].row.size() == size_t(ilog2u(rp)) && claims[i].col.size() == size_t(ilog2u(cp)),
+            QF_ERR_INVALID_PARAMETERS, "point dimension mismatch");
+    claims[i].value = E8{};
+  }
+  DevCtx c;
   Query q;
   for (auto& b : p.layout.buckets) {
     const ClaimIn& cl = claims[b.matrix];
@@ -1035,7 +1051,6 @@ std::vector<ClaimIn> benchmark_claims(const ParamsState& p, const CommittedState
     for (uint64_t j = 0; j < tm.len(); j++) v += tm.weight(j) * small_byte(vals[j]);
     claims[p.layout.buckets[i].matrix].value += v;
   }
-  return claims;
 }
 
 Bytes prove(const ParamsState& p, const CommittedState& cs, const std::vector<ClaimIn>& claims, ProveTimings& tm) {

@@ -4,6 +4,7 @@
 #pragma once
 #include "qf_kernels.hpp"
 #include <algorithm>
+#include <mutex>
 
 namespace qf {
 
@@ -18,6 +19,7 @@ struct ParamsState {
   mutable E8* d_alpha = nullptr;              // top-code alpha_1, uploaded on first use
   mutable std::vector<E8*> d_alpha_levels;    // alpha_1 of every level (top included), on first prove
   mutable std::vector<uint64_t> zero_ranges;  // padding intervals (start, len) chunked for the device check
+  mutable std::mutex mu;                      // guards the lazily filled members above
   ~ParamsState() {
     for (auto* a : d_alpha_levels) if (a && a != d_alpha) (void)hipFree(a);
     if (d_alpha) (void)hipFree(d_alpha);
@@ -58,6 +60,8 @@ Bytes prove(const ParamsState& p, const CommittedState& c, const std::vector<Cla
 // The benchmark statement of the reference CLI (main.rs:188-209): points from
 // the commitment transcript, values evaluated on the device BVA.
 std::vector<ClaimIn> benchmark_claims(const ParamsState& p, const CommittedState& c);
+// tbv.rs:143-185: fills claims[i].value with matrix i's MLE at its (row, col) point.
+void evaluate_claims(const ParamsState& p, const CommittedState& c, std::vector<ClaimIn>& claims);
 std::string statement_json(const ParamsState& p, const CommittedState& c);
 // Parameters::verify (protocol.rs:266-319): throws QfError on rejection.
 void verify(const ParamsState& p, const std::vector<Digest>& roots, const std::vector<ClaimIn>& claims,

@@ -1,5 +1,6 @@
 // Device vector operations and BVA-scale kernels of the opening (see qf_vec.hpp).
 #include "qf_vec.hpp"
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -436,7 +437,7 @@ struct BlockCache {
   std::unordered_map<void*, size_t> live;               // pointer -> rounded size
   std::multimap<size_t, CachedBlock> idle;              // rounded size -> block
   size_t idle_bytes = 0;
-  static constexpr size_t LIMIT = size_t(64) << 30;     // idle bytes kept before trimming
+  size_t limit = 0;                                     // idle bytes kept before trimming: 3/4 of the device
   static size_t round(size_t b) {
     const size_t g = b >= (size_t(1) << 20) ? (size_t(2) << 20) : 256;
     return (std::max<size_t>(b, 16) + g - 1) / g * g;
@@ -510,7 +511,11 @@ void dfree(void* p, hipStream_t st) {
   bc.live.erase(it);
   bc.idle.insert({size, {p, ev}});
   bc.idle_bytes += size;
-  bc.trim_locked(BlockCache::LIMIT);
+  if (!bc.limit) {
+    size_t fr = 0, total = 0;
+    bc.limit = hipMemGetInfo(&fr, &total) == hipSuccess ? total / 4 * 3 : (size_t(64) << 30);
+  }
+  bc.trim_locked(bc.limit);
 }
 
 // ---------------------------------------------------------------- DVec / DevCtx
@@ -655,9 +660,29 @@ static void upload_pairs(DevCtx& c, const std::vector<std::array<E8, 2>>& pairs,
   if (!pairs.empty()) QF_HIP(hipMemcpyAsync(*out, pairs.data(), pairs.size() * 32, hipMemcpyHostToDevice, c.st));
 }
 
+namespace {
+std::atomic<int> g_launch_mode{LAUNCH_BATCHED};
+}
+void set_launch_mode(int mode) { g_launch_mode = mode; }
+int launch_mode() { return g_launch_mode; }
+
 std::vector<E8> tensor_project(DevCtx& c, const int8_t* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
                                const std::vector<std::array<E8, 2>>& pairs) {
   require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
+  if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
+    // the plain per-matrix form: one launch (and one round trip) per bucket term
+    std::vector<E8> result;
+    for (const ProjTask& t : tasks) {
+      ProjTask one = t;
+      one.pair_off = 0;
+      std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
+      set_launch_mode(LAUNCH_BATCHED);
+      auto r = tensor_project(c, bva, available, L, {one}, own);
+      set_launch_mode(LAUNCH_PER_MATRIX);
+      result.insert(result.end(), r.begin(), r.end());
+    }
+    return result;
+  }
   std::vector<ProjTaskDev> chunks;
   std::vector<size_t> owner;
   for (size_t i = 0; i < tasks.size(); i++) {
@@ -719,6 +744,17 @@ void eo_witness(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t n_row
 }
 
 void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, const std::vector<std::array<E8, 2>>& pairs) {
+  if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
+    for (const FillTask& t : tasks) {
+      FillTask one = t;
+      one.pair_off = 0;
+      std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
+      set_launch_mode(LAUNCH_BATCHED);
+      tensor_fill_add(c, out, {one}, own);
+      set_launch_mode(LAUNCH_PER_MATRIX);
+    }
+    return;
+  }
   std::vector<FillTaskDev> chunks;
   for (const FillTask& t : tasks) {
     uint32_t k = std::min<uint32_t>(t.nbits, 12);

@@ -75,6 +75,13 @@ std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, u
 void scatter_add(DevCtx& c, E8* dst, const std::vector<uint64_t>& idx, const std::vector<E8>& vals);
 }  // namespace dv
 
+// Launch form of the per-term opening kernels: one batched launch over all
+// bucket terms (default), or the plain per-matrix form with one launch and one
+// host round trip per term (kept for comparison). Process-wide.
+enum : int { LAUNCH_BATCHED = 0, LAUNCH_PER_MATRIX = 1 };
+void set_launch_mode(int mode);
+int launch_mode();
+
 // ---------------------------------------------------------------- BVA-scale kernels of the opening
 // One projection task: lanes[l] = sum_row W(row) * v[base + row*L + l] over `rows`
 // rows, W(row) = scale * prod_b pair_b[(row >> b) & 1] for the task's bit pairs

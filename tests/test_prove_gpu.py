@@ -103,3 +103,39 @@ def test_malformed_claims_are_rejected():
     with pytest.raises(pa.QfError) as e:
         p.prove(committed, claims[:2] + [pa.Claim(claims[2].row, claims[2].col, bad)] + claims[3:])
     assert e.value.status == 1  # EncodingError (tbv.rs:131-138)
+
+
+def test_per_matrix_launch_form_gives_identical_proofs(oracle):
+    import parcomm_amd as pa
+    shapes = [(300, 200), (7, 1), (64, 33)]
+    p = params(shapes, 14, 4)
+    values = layout_encode(p, [synthetic_int8(r * c, 30 + i) for i, (r, c) in enumerate(shapes)])
+    committed = p.commit(values)
+    claims = p.benchmark_claims(committed)
+    batched, _ = p.prove(committed, claims)
+    pa.set_launch_mode(pa.LAUNCH_PER_MATRIX)
+    try:
+        per_matrix, _ = p.prove(committed, claims)
+    finally:
+        pa.set_launch_mode(pa.LAUNCH_BATCHED)
+    assert per_matrix == batched
+
+
+def test_served_requests_match_their_proofs_alone():
+    # the serving loop runs requests concurrently on separate streams; each
+    # result must equal the proof of the same request computed alone
+    import numpy as np
+    from parcomm_amd.serve import request_claims, serve
+    shapes = [(513, 129), (33, 65), (100, 7)]
+    p = params(shapes, 16, 4)
+    values = layout_encode(p, [synthetic_int8(r * c, 50 + i) for i, (r, c) in enumerate(shapes)])
+    committed = p.commit(values)
+    base = p.benchmark_claims(committed)
+    print("serving 6 requests over 3 workers", flush=True)
+    stats, reqs = serve(p, committed, base, n_requests=6, rate=500.0, seed=7, workers=3, verify=True)
+    assert stats["concurrency"] >= 1 and stats["p99_ms"] >= stats["p50_ms"]
+    for r in reqs:
+        claims = request_claims(p, committed, base, r.subset, np.random.default_rng(7 * 1_000_003 + r.rid))
+        alone, _ = p.prove(committed, claims)
+        assert [c.value for c in claims] == [c.value for c in r.claims]
+        assert alone == r.proof, r.rid
