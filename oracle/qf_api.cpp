@@ -16,6 +16,8 @@ int guarded(F&& f) {
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+// Extension degree of this build: 8 (F65521 / Ext8) or 4 (KoalaBear / Ext4).
+int orc_lanes() { return LANES; }
 
 void orc_blake2s(const uint8_t* data, size_t len, uint8_t* out) {
   Digest d = blake2s(data, len);
@@ -31,8 +33,9 @@ void orc_chacha_block(const uint8_t* key, uint64_t counter, uint64_t stream, int
   chacha_block(key, counter, stream, rounds, out16);
 }
 
-// Ext8 arithmetic for property tests: op 0 add, 1 sub, 2 mul, 3 inv(a), 4 frob(a, k=b.c[0]).
-int orc_e8_op(int op, const uint16_t* a, const uint16_t* b, uint16_t* out) {
+// Extension arithmetic for property tests on 16-byte canonical elements:
+// op 0 add, 1 sub, 2 mul, 3 inv(a), 4 frob(a, k=b.c[0]).
+int orc_e8_op(int op, const void* a, const void* b, void* out) {
   return guarded([&] {
     E8 x, y;
     std::memcpy(x.c, a, 16);
@@ -50,8 +53,8 @@ int orc_e8_op(int op, const uint16_t* a, const uint16_t* b, uint16_t* out) {
   });
 }
 
-void orc_wht(uint16_t* v, size_t n) { wht(reinterpret_cast<E8*>(v), n); }
-void orc_mobius(uint16_t* v, size_t n) { mobius(reinterpret_cast<E8*>(v), n); }
+void orc_wht(void* v, size_t n) { wht(reinterpret_cast<E8*>(v), n); }
+void orc_mobius(void* v, size_t n) { mobius(reinterpret_cast<E8*>(v), n); }
 
 // Layout: shapes[2*i] = rows, shapes[2*i+1] = cols. Writes up to `cap` buckets of
 // 8 u64 (matrix,row_offset,col_offset,rows,cols,physical_rows,physical_cols,offset).
@@ -75,7 +78,7 @@ int orc_layout(const uint64_t* shapes, size_t n, uint64_t* buckets, size_t cap, 
 // PCS parameters of PackedPcs::new(chunk_log, 32, bits, seed) (packed.rs:134-158).
 void* orc_pcs_new(int chunk_log, int security_bits, const uint8_t* seed32) {
   try {
-    return new PcsParams(pcs_setup(size_t(1) << (chunk_log - 3), 32, 2, size_t(security_bits), seed32));
+    return new PcsParams(pcs_setup(size_t(1) << (chunk_log - LOG_LANES), 32, 2, size_t(security_bits), seed32));
   } catch (const std::exception& e) { g_err = e.what(); return nullptr; }
 }
 void orc_pcs_free(void* h) { delete static_cast<PcsParams*>(h); }
@@ -85,21 +88,22 @@ size_t orc_pcs_levels(void* h, size_t* lens) {
   for (size_t i = 0; i < pp->seq.codes.size(); i++) lens[i] = pp->seq.codes[i].message_len;
   return pp->seq.codes.size();
 }
-// alpha_1 of the top code (length message_len), 8 u16 per element.
-void orc_pcs_top_alpha(void* h, uint16_t* out) {
+// alpha_1 of the top code (length message_len), 16 canonical bytes per element.
+void orc_pcs_top_alpha(void* h, void* out) {
   auto* pp = static_cast<PcsParams*>(h);
   const auto& a = pp->seq.codes.back().alphas[1];
   std::memcpy(out, a.data(), a.size() * 16);
 }
 
-// Commit one block of 2^chunk_log signed bytes (entries past `available` are 0).
+// Commit one block of 2^chunk_log signed entries (i8 for F65521, i16 for
+// KoalaBear; entries past `available` are 0).
 // Optionally returns the Mobius-transformed witness and the codeword.
-int orc_commit_block(void* h, const int8_t* values, uint64_t available, uint8_t* root,
-                     uint16_t* coeffs_out, uint16_t* codeword_out) {
+int orc_commit_block(void* h, const void* values, uint64_t available, uint8_t* root, void* coeffs_out,
+                     void* codeword_out) {
   return guarded([&] {
     auto* pp = static_cast<PcsParams*>(h);
-    u64 block_len = u64(pp->n_witness) * 8;
-    Committed c = packed_commit(*pp, pack_block_i8(values, available, block_len));
+    u64 block_len = u64(pp->n_witness) * LANES;
+    Committed c = packed_commit(*pp, pack_block(static_cast<const Val*>(values), available, block_len));
     Digest r = c.root();
     std::memcpy(root, r.data(), 32);
     if (coeffs_out) std::memcpy(coeffs_out, c.coeffs.data(), c.coeffs.size() * 16);
@@ -107,8 +111,8 @@ int orc_commit_block(void* h, const int8_t* values, uint64_t available, uint8_t*
   });
 }
 
-// Merkle root over column-major Ext8 entries with the base-row leaf domain.
-void orc_merkle_root_rows(const uint16_t* entries, size_t rows, size_t cols, uint8_t* root) {
+// Merkle root over column-major extension entries with the base-row leaf domain.
+void orc_merkle_root_rows(const void* entries, size_t rows, size_t cols, uint8_t* root) {
   Merkle m = merkle_from_leaves(hash_rows(reinterpret_cast<const E8*>(entries), rows, cols, LEAF_BASE_DOMAIN));
   Digest r = m.root();
   std::memcpy(root, r.data(), 32);
@@ -124,9 +128,10 @@ extern "C" {
 // into `proof` (capacity `cap`), its length into `proof_len`, and the compact
 // JSON of the commitment and claims into the optional buffers.
 int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg, uint32_t packed_log,
-                        const int8_t* values, uint64_t n_values, uint8_t* proof, size_t cap, size_t* proof_len,
+                        const void* values_in, uint64_t n_values, uint8_t* proof, size_t cap, size_t* proof_len,
                         char* statement_json, size_t statement_cap, char* claims_json, size_t claims_cap) {
   return guarded([&] {
+    const Val* values = static_cast<const Val*>(values_in);
     std::vector<Shape> s(n);
     for (size_t i = 0; i < n; i++) s[i] = {shapes[2 * i], shapes[2 * i + 1]};
     Layout l = layout_new(s);
@@ -137,7 +142,7 @@ int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg
     check(blocks <= 128, "at most 128 PCS blocks are supported");
     u8 seed[32];
     std::memset(seed, 0x51, 32);
-    PcsParams pp = pcs_setup(size_t(1) << (chunk_log - 3), 32, 2, 100, seed);
+    PcsParams pp = pcs_setup(size_t(1) << (chunk_log - LOG_LANES), 32, 2, 100, seed);
     Statement st;
     st.config = {chunk_log_cfg, packed_log, 100};
     st.shapes = s;
@@ -145,7 +150,7 @@ int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg
     for (u64 b = 0; b < blocks; b++) {
       u64 first = b * block_len;
       u64 avail = first < n_values ? std::min<u64>(block_len, n_values - first) : 0;
-      Committed c = packed_commit(pp, pack_block_i8(values + (avail ? first : 0), avail, block_len));
+      Committed c = packed_commit(pp, pack_block(values + (avail ? first : 0), avail, block_len));
       st.roots.push_back(c.root());
     }
     OuterTranscript pts;
@@ -194,7 +199,7 @@ int orc_native_fixture(uint64_t n_witness, uint64_t ext_cols, uint64_t bits, uin
         u64 lo = rng.next_u64(), hi = rng.next_u64();
         for (int i = 0; i < 8; i++) { b[i] = u8(lo >> (8 * i)); b[8 + i] = u8(hi >> (8 * i)); }
         E8 v;
-        if (e8_read(b, v)) return v;
+        if (e8_from_uniform(b, v)) return v;
       }
     };
     std::vector<E8> witness(n_witness);

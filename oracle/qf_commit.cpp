@@ -17,7 +17,6 @@ namespace orc {
 const char* LEAF_BASE_DOMAIN = "qafold-short/base-row/v1";
 const char* LEAF_EXTENSION_DOMAIN = "qafold-short/extension-row/v1";
 static const char* INNER_DOMAIN = "qafold-short/merkle-inner/v1";
-static const char* FIELD_NAME = "FullPrime65521Ext8/X^8-17/v1";  // full_extension.rs:130
 
 static void modulus_frame(Blake2s& h) { h.frame_u64("modulus", P); }
 
@@ -72,10 +71,10 @@ Layout layout_new(const std::vector<Shape>& shapes) {
 }
 
 // layout.rs:115-135
-std::vector<int8_t> layout_encode(const Layout& l, const std::vector<std::vector<int8_t>>& m) {
+std::vector<Val> layout_encode(const Layout& l, const std::vector<std::vector<Val>>& m) {
   check(m.size() == l.shapes.size(), "matrix count mismatch");
   for (size_t i = 0; i < m.size(); i++) check(m[i].size() == l.shapes[i].rows * l.shapes[i].cols, "matrix length mismatch");
-  std::vector<int8_t> v(l.active, 0);
+  std::vector<Val> v(l.active, 0);
   for (auto& b : l.buckets)
     for (u64 r = 0; r < b.physical_rows; r++) {
       u64 src = (b.row_offset + r) * l.shapes[b.matrix].cols + b.col_offset;
@@ -86,7 +85,7 @@ std::vector<int8_t> layout_encode(const Layout& l, const std::vector<std::vector
 }
 
 // layout.rs:136-155
-void layout_validate(const Layout& l, const int8_t* values, u64 n) {
+void layout_validate(const Layout& l, const Val* values, u64 n) {
   check(n == l.active, "active BVA length mismatch");
   for (auto& b : l.buckets)
     for (u64 r = 0; r < b.rows; r++) {
@@ -136,7 +135,7 @@ static E8 sample_generator_element(const Digest& seed, size_t message_len, size_
     Digest d = h.finalize();
     for (int lane = 0; lane < 2; lane++) {
       E8 v;
-      if (e8_read(d.data() + 16 * lane, v)) return v;
+      if (e8_from_uniform(d.data() + 16 * lane, v)) return v;
     }
   }
 }
@@ -399,15 +398,14 @@ Committed pcs_commit_owned(const PcsParams& pp, std::vector<E8> witness) {
   return c;
 }
 
-// field.rs:24-26 and packed.rs:220-243: lane j of element i = entry 8i + j (mod p),
-// entries past `available` are zero.
-std::vector<E8> pack_block_i8(const int8_t* values, u64 available, u64 block_len) {
-  std::vector<E8> out(block_len / 8);
-  for (u64 i = 0; i < block_len / 8; i++)
-    for (int j = 0; j < 8; j++) {
-      u64 idx = 8 * i + j;
-      int v = idx < available ? values[idx] : 0;
-      out[i].c[j] = u16(v < 0 ? v + int(P) : v);
+// field.rs:24-26 and packed.rs:220-243: lane j of element i = entry LANES*i + j
+// (mod p), entries past `available` are zero.
+std::vector<E8> pack_block(const Val* values, u64 available, u64 block_len) {
+  std::vector<E8> out(block_len / LANES);
+  for (u64 i = 0; i < block_len / LANES; i++)
+    for (int j = 0; j < LANES; j++) {
+      u64 idx = LANES * i + j;
+      out[i].c[j] = fsmall(idx < available ? values[idx] : 0);
     }
   return out;
 }

@@ -16,8 +16,8 @@ struct Layout {
 };
 std::vector<std::pair<u64, u64>> intervals(u64 length, size_t budget);
 Layout layout_new(const std::vector<Shape>& shapes);
-std::vector<int8_t> layout_encode(const Layout& l, const std::vector<std::vector<int8_t>>& m);
-void layout_validate(const Layout& l, const int8_t* values, u64 n);
+std::vector<Val> layout_encode(const Layout& l, const std::vector<std::vector<Val>>& m);
+void layout_validate(const Layout& l, const Val* values, u64 n);
 
 // ---------------------------------------------------------------- QA codes over Ext8
 void wht(E8* v, size_t n);          // unnormalised Walsh-Hadamard, in place
@@ -81,7 +81,7 @@ struct Committed {
 // `witness` is the column-major message (n_witness entries). Encodes and hashes.
 Committed pcs_commit_owned(const PcsParams& pp, std::vector<E8> witness);
 // PackedPcs: 8 consecutive base values per Ext8 element, then Mobius (packed.rs:220-258).
-std::vector<E8> pack_block_i8(const int8_t* values, u64 available, u64 block_len);
+std::vector<E8> pack_block(const Val* values, u64 available, u64 block_len);
 Committed packed_commit(const PcsParams& pp, const std::vector<E8>& packed);
 
 }  // namespace orc

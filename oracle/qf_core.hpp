@@ -12,6 +12,11 @@
 //                          end to end through the reference's golden digests.
 //   * F65521 / Ext8      : field axioms, Frobenius = x^p, inverse, X^8 = 17
 //                          (tests/test_oracle_core.py, brute force in Python).
+//   * KoalaBear / Ext4   : the same checks with X^4 = 3 plus the reference's
+//                          Frobenius phase table (tests/test_oracle_kb.py).
+// The field is chosen at compile time: the default build is F65521 / Ext8 with
+// i8 entries (liboracle.so); -DORC_KB gives KoalaBear / Ext4 with i16 entries
+// (liboracle_kb.so). Both builds reproduce the reference's demo proof digests.
 // Anything not listed is pinned in the file that defines it.
 #pragma once
 #include <cstdint>
@@ -35,70 +40,124 @@ struct Error : std::runtime_error {
 };
 inline void check(bool ok, const char* m) { if (!ok) throw Error(m); }
 
+#ifdef ORC_KB
+// ---------------------------------------------------------------- KoalaBear
+// koalabear.rs:1-2, :85-120: p = 2^31 - 2^24 + 1, canonical u32; BVA entries are i16.
+constexpr u32 P = 2130706433;
+using Coef = u32;
+using Val = int16_t;
+constexpr int LANES = 4;                 // extension degree = packing lanes per element
+constexpr int LOG_LANES = 2;
+constexpr Coef FROB_ROOT = 2113994754;   // x -> x^p scales coefficient i by FROB_ROOT^i (koalabear.rs:155)
+static const char* const FIELD_NAME = "FullKoalaBearExt4/X^4-3/scalar/v1";  // koalabear.rs:195
+static const char* const OUTER_DOMAIN = "parcomm-koalabear-i16/public/v1";   // transcript.rs:15
+static const char* const TERMINAL_DOMAIN = "parcomm/koalabear-i16/terminal/v2";  // protocol.rs:330
+static const char* const PACKED_DOMAIN = "mikros/qafold-koalabear-ext4/packed-boolean-mle/v1";  // packed_koalabear.rs:478
+static const char* const PACKED_MAGIC = "QAKOAPK1";                            // packed_koalabear.rs:24
+#else
 // ---------------------------------------------------------------- F65521
-// qafold_short.rs:325-477 (Prime65521): p = 2^16 - 15, canonical u16.
+// qafold_short.rs:325-477 (Prime65521): p = 2^16 - 15, canonical u16; BVA entries are i8.
 constexpr u32 P = 65521;
-inline u16 fadd(u16 a, u16 b) { u32 s = u32(a) + b; return u16(s >= P ? s - P : s); }
-inline u16 fsub(u16 a, u16 b) { return u16(a >= b ? a - b : u32(a) + P - b); }
-inline u16 fneg(u16 a) { return u16(a == 0 ? 0 : P - a); }
-inline u16 fmul(u16 a, u16 b) { return u16((u64(a) * b) % P); }
-inline u16 fpow(u16 a, u64 e) {
-  u16 r = 1;
+using Coef = u16;
+using Val = int8_t;
+constexpr int LANES = 8;
+constexpr int LOG_LANES = 3;
+constexpr Coef FROB_ROOT = 57852;        // 17^((p-1)/8) (qafold_short.rs:486-525)
+static const char* const FIELD_NAME = "FullPrime65521Ext8/X^8-17/v1";  // full_extension.rs:130
+static const char* const OUTER_DOMAIN = "parcomm-qafold/public/v1";
+static const char* const TERMINAL_DOMAIN = "parcomm/qafold/terminal/v2";
+static const char* const PACKED_DOMAIN = "mikros/qafold-f65521-ext8/packed-boolean-mle/v1";
+static const char* const PACKED_MAGIC = "QA655PK1";
+#endif
+inline Coef fadd(Coef a, Coef b) { u32 s = u32(a) + b; return Coef(s >= P ? s - P : s); }
+inline Coef fsub(Coef a, Coef b) { return Coef(a >= b ? u32(a) - b : u32(a) + P - b); }
+inline Coef fneg(Coef a) { return Coef(a == 0 ? 0 : P - a); }
+inline Coef fmul(Coef a, Coef b) { return Coef((u64(a) * b) % P); }
+inline Coef fpow(Coef a, u64 e) {
+  Coef r = 1;
   while (e) { if (e & 1) r = fmul(r, a); a = fmul(a, a); e >>= 1; }
   return r;
 }
-inline u16 finv(u16 a) { check(a != 0, "zero inverse"); return fpow(a, P - 2); }
+inline Coef finv(Coef a) { check(a != 0, "zero inverse"); return fpow(a, P - 2); }
+inline Coef fsmall(long v) { long m = v % long(P); return Coef(m < 0 ? m + long(P) : m); }
 
-// ---------------------------------------------------------------- Ext8
-// F65521[X]/(X^8 - 17), coefficient i = X^i (qafold_short.rs:482-660).
+// ---------------------------------------------------------------- extension field
+// F65521[X]/(X^8 - 17) (qafold_short.rs:482-660) or KoalaBear[X]/(X^4 - 3)
+// (koalabear.rs:1-2, 131-145), coefficient i = X^i, 16 canonical bytes.
+// The type keeps the name E8 in both builds.
 struct E8 {
-  u16 c[8];
-  bool operator==(const E8& o) const { return std::memcmp(c, o.c, 16) == 0; }
+  Coef c[LANES];
+  bool operator==(const E8& o) const { return std::memcmp(c, o.c, sizeof c) == 0; }
   bool operator!=(const E8& o) const { return !(*this == o); }
 };
-inline E8 e8_zero() { E8 r; std::memset(r.c, 0, 16); return r; }
+static_assert(sizeof(E8) == 16, "16-byte field elements");
+inline E8 e8_zero() { E8 r; std::memset(r.c, 0, sizeof r.c); return r; }
 inline E8 e8_one() { E8 r = e8_zero(); r.c[0] = 1; return r; }
-inline E8 e8_base(u16 v) { E8 r = e8_zero(); r.c[0] = v; return r; }
-inline E8 e8_from_u64(u64 v) { return e8_base(u16(v % P)); }
-inline bool e8_is_zero(const E8& a) { for (int i = 0; i < 8; i++) if (a.c[i]) return false; return true; }
-inline E8 operator+(const E8& a, const E8& b) { E8 r; for (int i = 0; i < 8; i++) r.c[i] = fadd(a.c[i], b.c[i]); return r; }
-inline E8 operator-(const E8& a, const E8& b) { E8 r; for (int i = 0; i < 8; i++) r.c[i] = fsub(a.c[i], b.c[i]); return r; }
-inline E8 operator-(const E8& a) { E8 r; for (int i = 0; i < 8; i++) r.c[i] = fneg(a.c[i]); return r; }
+inline E8 e8_base(Coef v) { E8 r = e8_zero(); r.c[0] = v; return r; }
+inline E8 e8_from_u64(u64 v) { return e8_base(Coef(v % P)); }
+inline bool e8_is_zero(const E8& a) { for (int i = 0; i < LANES; i++) if (a.c[i]) return false; return true; }
+inline E8 operator+(const E8& a, const E8& b) { E8 r; for (int i = 0; i < LANES; i++) r.c[i] = fadd(a.c[i], b.c[i]); return r; }
+inline E8 operator-(const E8& a, const E8& b) { E8 r; for (int i = 0; i < LANES; i++) r.c[i] = fsub(a.c[i], b.c[i]); return r; }
+inline E8 operator-(const E8& a) { E8 r; for (int i = 0; i < LANES; i++) r.c[i] = fneg(a.c[i]); return r; }
 inline E8& operator+=(E8& a, const E8& b) { a = a + b; return a; }
 inline E8& operator-=(E8& a, const E8& b) { a = a - b; return a; }
-// Schoolbook product with the X^8 = 17 fold; any exact product gives the same
-// canonical coefficients as the reference's Karatsuba (qafold_short.rs:534-540).
+// Schoolbook product with the X^LANES = W fold (W = 17 or 3); any exact product
+// gives the same canonical coefficients as the reference's Karatsuba / tower.
 inline E8 operator*(const E8& a, const E8& b) {
-  u64 t[15] = {0};
-  for (int i = 0; i < 8; i++)
-    for (int j = 0; j < 8; j++) t[i + j] += u64(a.c[i]) * b.c[j];
+#ifdef ORC_KB
+  constexpr u64 W = 3;
+#else
+  constexpr u64 W = 17;
+#endif
+  u64 t[2 * LANES - 1] = {0};
+  for (int i = 0; i < LANES; i++)
+    for (int j = 0; j < LANES; j++) t[i + j] = (t[i + j] + u64(a.c[i]) * b.c[j]) % P;
   E8 r;
-  for (int i = 0; i < 8; i++) r.c[i] = u16((t[i] + 17 * (i < 7 ? t[i + 8] : 0)) % P);
+  for (int i = 0; i < LANES; i++) r.c[i] = Coef((t[i] + W * (i < LANES - 1 ? t[i + LANES] : 0)) % P);
   return r;
 }
 inline E8& operator*=(E8& a, const E8& b) { a = a * b; return a; }
-inline E8 e8_mul_base(const E8& a, u16 s) { E8 r; for (int i = 0; i < 8; i++) r.c[i] = fmul(a.c[i], s); return r; }
-// Frobenius x -> x^(p^k): coefficient i scales by zeta^(i*k), zeta = 17^((p-1)/8) = 57852
-// (qafold_short.rs:486-525).
-inline u16 frob_root_pow(int k) { return fpow(57852, u64(k % 8)); }
+inline E8 e8_mul_base(const E8& a, Coef s) { E8 r; for (int i = 0; i < LANES; i++) r.c[i] = fmul(a.c[i], s); return r; }
+// Frobenius x -> x^(p^k): coefficient i scales by FROB_ROOT^(i k).
+inline Coef frob_root_pow(int k) { return fpow(FROB_ROOT, u64(k % LANES)); }
 inline E8 e8_frob(const E8& a, int power) {
-  power %= 8;
+  power = ((power % LANES) + LANES) % LANES;
   if (power == 0) return a;
   E8 r;
-  for (int i = 0; i < 8; i++) r.c[i] = fmul(a.c[i], frob_root_pow((i * power) % 8));
+  for (int i = 0; i < LANES; i++) r.c[i] = fmul(a.c[i], frob_root_pow((i * power) % LANES));
   return r;
 }
-inline E8 e8_inv(const E8& a) {
+inline E8 e8_inv(const E8& a) {  // a^-1 = (product of the other conjugates) / norm
   check(!e8_is_zero(a), "zero inverse");
   E8 prod = e8_frob(a, 1);
-  for (int k = 2; k < 8; k++) prod = e8_frob(prod * a, 1);
+  for (int k = 2; k < LANES; k++) prod = e8_frob(prod * a, 1);
   E8 norm = a * prod;
   return e8_mul_base(prod, finv(norm.c[0]));
 }
-inline void e8_write(const E8& a, u8* out) { for (int i = 0; i < 8; i++) { out[2 * i] = u8(a.c[i]); out[2 * i + 1] = u8(a.c[i] >> 8); } }
-inline bool e8_read(const u8* in, E8& a) {
-  for (int i = 0; i < 8; i++) { u32 v = in[2 * i] | (u32(in[2 * i + 1]) << 8); if (v >= P) return false; a.c[i] = u16(v); }
+inline void e8_write(const E8& a, u8* out) {
+  for (int i = 0; i < LANES; i++)
+    for (size_t b = 0; b < sizeof(Coef); b++) out[sizeof(Coef) * i + b] = u8(u32(a.c[i]) >> (8 * b));
+}
+inline bool e8_read(const u8* in, E8& a) {  // canonical bytes; rejects coefficients >= p
+  for (int i = 0; i < LANES; i++) {
+    u32 v = 0;
+    for (size_t b = 0; b < sizeof(Coef); b++) v |= u32(in[sizeof(Coef) * i + b]) << (8 * b);
+    if (v >= P) return false;
+    a.c[i] = Coef(v);
+  }
   return true;
+}
+// from_uniform_bytes: F65521 reads canonical u16s; KoalaBear first clears bit 7
+// of bytes 3, 7, 11, 15 (koalabear.rs:248-255). Both reject values >= p.
+inline bool e8_from_uniform(const u8* in, E8& a) {
+#ifdef ORC_KB
+  u8 b[16];
+  std::memcpy(b, in, 16);
+  for (int i : {3, 7, 11, 15}) b[i] &= 0x7f;
+  return e8_read(b, a);
+#else
+  return e8_read(in, a);
+#endif
 }
 inline std::array<u8, 16> e8_bytes(const E8& a) { std::array<u8, 16> b; e8_write(a, b.data()); return b; }
 

@@ -61,7 +61,7 @@ std::string json_claims(const std::vector<Claim>& c) {  // tbv.rs:6-11
 }
 
 // ================================================================ outer transcript (transcript.rs:8-70)
-OuterTranscript::OuterTranscript() { const char* d = "parcomm-qafold/public/v1"; state = blake2s(d, std::strlen(d)); }
+OuterTranscript::OuterTranscript() { state = blake2s(OUTER_DOMAIN, std::strlen(OUTER_DOMAIN)); }
 void OuterTranscript::bytes(const char* label, const void* data, size_t n) {
   Blake2s h;
   h.update(state.data(), 32);
@@ -79,7 +79,7 @@ E8 OuterTranscript::challenge(const char* label) {
     for (int i = 0; i < 8; i++) b[i] = u8(count >> (8 * i));
     bytes(label, b, 8);
     E8 v;
-    if (e8_read(state.data(), v)) return v;
+    if (e8_from_uniform(state.data(), v)) return v;
   }
 }
 E8 OuterTranscript::nonzero(const char* label) {
@@ -161,34 +161,35 @@ static Term compact_term(Term t) {
   }
   return t;
 }
-static E8 small_i8(int v) { return e8_base(u16(v < 0 ? v + int(P) : v)); }
+static E8 small_val(long v) { return e8_base(fsmall(v)); }
 // tbv.rs:288-352: lane sums of value * weight over the term's rows (exact integers mod p)
-std::vector<E8> tbv_project(const Term& t, const int8_t* values, u64 n_values, u64 stride) {
+std::vector<E8> tbv_project(const Term& t, const Val* values, u64 n_values, u64 stride) {
   size_t low_bits = std::min<size_t>(ilog2(stride), t.factors.size());
   u64 lanes = u64(1) << low_bits;
   std::vector<E8> out(lanes, e8_zero());
   if (t.offset >= n_values || e8_is_zero(t.scale)) return out;
   std::vector<E8> hw = tbv_weights(t.factors.data() + low_bits, t.factors.size() - low_bits);
   u64 row_count = std::min<u64>((n_values - t.offset + lanes - 1) / lanes, t.len() / lanes);
-  std::vector<std::array<int64_t, 8>> acc(lanes);
+  // i16 x u31 products summed over up to 2^c rows need more than 64 bits.
+  std::vector<std::array<__int128, LANES>> acc(lanes);
   for (auto& a : acc) a.fill(0);
   for (u64 row = 0; row < row_count; row++) {
     E8 w = t.scale * hw[row];
     u64 idx = t.offset + row * lanes;
     for (u64 lane = 0; lane < lanes && idx + lane < n_values; lane++) {
-      int64_t v = values[idx + lane];
-      for (int k = 0; k < 8; k++) acc[lane][k] += v * int64_t(w.c[k]);
+      __int128 v = values[idx + lane];
+      for (int k = 0; k < LANES; k++) acc[lane][k] += v * __int128(w.c[k]);
     }
   }
   for (u64 lane = 0; lane < lanes; lane++)
-    for (int k = 0; k < 8; k++) out[lane].c[k] = u16(((acc[lane][k] % int64_t(P)) + P) % P);
+    for (int k = 0; k < LANES; k++) out[lane].c[k] = Coef(((acc[lane][k] % __int128(P)) + P) % P);
   return out;
 }
 // tbv.rs:361-392
-void cross_terms(const Query& q, const int8_t* values, u64 n_values, u64 s, std::vector<E8>& pos, std::vector<E8>& neg) {
+void cross_terms(const Query& q, const Val* values, u64 n_values, u64 s, std::vector<E8>& pos, std::vector<E8>& neg) {
   pos.assign(s - 1, e8_zero());
   neg.assign(s - 1, e8_zero());
-  auto val = [&](u64 i) { return i < n_values ? small_i8(values[i]) : e8_zero(); };
+  auto val = [&](u64 i) { return i < n_values ? small_val(values[i]) : e8_zero(); };
   for (auto& t : q.terms) {
     if (t.len() >= s) {
       auto proj = tbv_project(t, values, n_values, s);
@@ -293,7 +294,7 @@ Query tbv_compile(const Layout& l, const std::vector<Claim>& claims, OuterTransc
   return q;
 }
 // tbv.rs:143-185 (statement generation for benchmarks; never used by verification)
-std::vector<Claim> evaluate_claims(const Layout& l, const int8_t* values, u64 n_values,
+std::vector<Claim> evaluate_claims(const Layout& l, const Val* values, u64 n_values,
                                    const std::vector<std::pair<std::vector<E8>, std::vector<E8>>>& points) {
   check(n_values == l.active && points.size() == l.shapes.size(), "claim input dimensions");
   std::vector<Claim> claims(points.size());
@@ -316,7 +317,7 @@ std::vector<Claim> evaluate_claims(const Layout& l, const int8_t* values, u64 n_
 }
 
 // ================================================================ EO (eo.rs:97-181)
-EoProof eo_prove(const int8_t* values, u64 n_values, u64 logical, u64 s, const Query& q, OuterTranscript& tr,
+EoProof eo_prove(const Val* values, u64 n_values, u64 logical, u64 s, const Query& q, OuterTranscript& tr,
                  Terminal& terminal) {
   EoProof pf;
   cross_terms(q, values, n_values, s, pf.positive, pf.negative);
@@ -332,13 +333,13 @@ EoProof eo_prove(const int8_t* values, u64 n_values, u64 logical, u64 s, const Q
   for (u64 i = 1; i < s; i++) pinv[i] = pinv[i - 1] * inv;
   std::vector<E8> w(logical / s);
   for (u64 row = 0; row < logical / s; row++) {
-    int64_t acc[8] = {0};
+    int64_t acc[LANES] = {0};
     for (u64 lane = 0; lane < s; lane++) {
       u64 i = row * s + lane;
       int64_t v = i < n_values ? values[i] : 0;
-      for (int k = 0; k < 8; k++) acc[k] += v * int64_t(pinv[lane].c[k]);
+      for (int k = 0; k < LANES; k++) acc[k] += v * int64_t(pinv[lane].c[k]);
     }
-    for (int k = 0; k < 8; k++) w[row].c[k] = u16(((acc[k] % int64_t(P)) + P) % P);
+    for (int k = 0; k < LANES; k++) w[row].c[k] = Coef(((acc[k] % int64_t(P)) + P) % P);
   }
   std::vector<E8> qv = query_fold(q, logical, s, c);
   // update_claim (eo.rs:27-38)
@@ -422,7 +423,7 @@ struct PcsTranscript {
       E8 v;
       for (u64 attempt = outer;; attempt++) {
         Digest d = derive(label, ctr, 0, attempt);
-        if (e8_read(d.data(), v)) break;
+        if (e8_from_uniform(d.data(), v)) break;
       }
       if (!nonzero || !e8_is_zero(v)) {
         counter++;
@@ -493,11 +494,11 @@ static bool qa_distance_bound(u64 n, u64 index, u64 bits, double modulus_log2, d
   check(qa_bound_at(n, index, rel, ln_p, e, lf), "distance bound");
   return true;
 }
-// ShortQaCode::num_queries for the full octic field (FINITE_LENGTH_PROXIMITY, no RF layers)
+// ShortQaCode::num_queries for the full extension field (FINITE_LENGTH_PROXIMITY, no RF layers)
 u64 num_queries(const QaCode& code, u64 bits, u64 rows) {
   check(rows > 0, "no rows");
   unsigned __int128 card = 1;
-  for (int i = 0; i < 8; i++) card *= 65521;
+  for (int i = 0; i < LANES; i++) card *= P;
   double log2card = std::log2(double(card));
   double rel;
   if (!qa_distance_bound(code.message_len, code.index, bits, log2card, rel)) return rows;
@@ -1138,15 +1139,14 @@ struct PackedTranscript {  // packed.rs:754-779
       c.update_u64(ctr);
       Digest d = c.finalize();
       E8 v;
-      if (e8_read(d.data(), v)) { bytes(d.data(), 32); return v; }
+      if (e8_from_uniform(d.data(), v)) { bytes(d.data(), 32); return v; }
     }
   }
 };
 static PackedTranscript packed_transcript(const PcsParams& pp, int log_n, const Digest& root, const std::vector<E8>& point,
                                           const E8& value, const Bytes& context) {
   PackedTranscript t;
-  const char* dom = "mikros/qafold-f65521-ext8/packed-boolean-mle/v1";
-  t.bytes(dom, std::strlen(dom));
+  t.bytes(PACKED_DOMAIN, std::strlen(PACKED_DOMAIN));
   t.bytes(pp.parameter_digest.data(), 32);
   u8 ln[8];
   for (int i = 0; i < 8; i++) ln[i] = u8(u64(log_n) >> (8 * i));
@@ -1167,54 +1167,55 @@ static std::vector<E8> equality_table(const E8* p, size_t n) {  // packed.rs:525
   return t;
 }
 static E8 basis(int j) { E8 b = e8_zero(); b.c[j] = 1; return b; }
-static std::array<E8, 8> powers8(const E8& v) {
-  std::array<E8, 8> o;
+using Lanes = std::array<E8, LANES>;
+static Lanes lane_powers(const E8& v) {
+  Lanes o;
   o[0] = e8_one();
-  for (int i = 1; i < 8; i++) o[i] = o[i - 1] * v;
+  for (int i = 1; i < LANES; i++) o[i] = o[i - 1] * v;
   return o;
 }
 // packed.rs:502-523
-static E8 projected_claim(const std::array<E8, 8>& ev, const std::vector<E8>& low) {
-  auto inv_basis = powers8(e8_inv(basis(1)));
-  u16 omega_inv = finv(57852), inv8 = finv(8);
-  std::array<E8, 8> conj;
-  for (int i = 0; i < 8; i++) conj[i] = e8_frob(ev[i], i);
-  u16 step = 1;
+static E8 projected_claim(const Lanes& ev, const std::vector<E8>& low) {
+  auto inv_basis = lane_powers(e8_inv(basis(1)));
+  Coef omega_inv = finv(FROB_ROOT), inv_lanes = finv(LANES);
+  Lanes conj;
+  for (int i = 0; i < LANES; i++) conj[i] = e8_frob(ev[i], i);
+  Coef step = 1;
   E8 claim = e8_zero();
-  for (int j = 0; j < 8; j++) {
-    u16 phase = 1;
+  for (int j = 0; j < LANES; j++) {
+    Coef phase = 1;
     E8 sum = e8_zero();
-    for (int i = 0; i < 8; i++) { sum += e8_mul_base(conj[i], phase); phase = fmul(phase, step); }
-    E8 lane = e8_mul_base(sum * inv_basis[j], inv8);
+    for (int i = 0; i < LANES; i++) { sum += e8_mul_base(conj[i], phase); phase = fmul(phase, step); }
+    E8 lane = e8_mul_base(sum * inv_basis[j], inv_lanes);
     claim += low[j] * lane;
     step = fmul(step, omega_inv);
   }
   return claim;
 }
 // lanes_j = sum_i coeff_j(values_i) * weights_i (packed.rs:569-592)
-static std::array<E8, 8> project_lanes(const std::vector<E8>& values, const std::vector<E8>& w) {
-  std::array<E8, 8> out;
-  for (int j = 0; j < 8; j++) {
-    u64 raw[8] = {0};
+static Lanes project_lanes(const std::vector<E8>& values, const std::vector<E8>& w) {
+  Lanes out;
+  for (int j = 0; j < LANES; j++) {
+    unsigned __int128 raw[LANES] = {0};
     for (size_t i = 0; i < values.size(); i++)
-      for (int k = 0; k < 8; k++) raw[k] += u64(values[i].c[j]) * w[i].c[k];
-    for (int k = 0; k < 8; k++) out[j].c[k] = u16(raw[k] % P);
+      for (int k = 0; k < LANES; k++) raw[k] += (unsigned __int128)(values[i].c[j]) * w[i].c[k];
+    for (int k = 0; k < LANES; k++) out[j].c[k] = Coef(raw[k] % P);
   }
   return out;
 }
-static void apply_weight_images(std::vector<E8>& w, const std::array<E8, 8>& images) {  // packed.rs:634-650
+static void apply_weight_images(std::vector<E8>& w, const Lanes& images) {  // packed.rs:634-650
   for (auto& x : w) {
     E8 r = e8_zero();
-    for (int j = 0; j < 8; j++) r += e8_mul_base(images[j], x.c[j]);
+    for (int j = 0; j < LANES; j++) r += e8_mul_base(images[j], x.c[j]);
     x = r;
   }
 }
-static E8 batch_weight(const E8* point, size_t n, const std::vector<E8>& reduced, const std::array<E8, 8>& powers) {
+static E8 batch_weight(const E8* point, size_t n, const std::vector<E8>& reduced, const Lanes& powers) {
   E8 sum = e8_zero();
-  for (int i = 0; i < 8; i++) {
+  for (int i = 0; i < LANES; i++) {
     E8 w = e8_one();
     for (size_t k = 0; k < n; k++) {
-      E8 s = e8_frob(point[k], (8 - i) % 8);
+      E8 s = e8_frob(point[k], (LANES - i) % LANES);
       w = w * ((e8_one() - s) * (e8_one() - reduced[k]) + s * reduced[k]);
     }
     sum += powers[i] * w;
@@ -1256,31 +1257,31 @@ Bytes packed_open_batch(const PcsParams& pp, int log_n, const std::vector<Digest
   size_t blocks = roots.size();
   check(blocks >= 2 && blocks <= 128 && is_pow2(blocks) && point.size() == size_t(log_n + ilog2(blocks)), "batch dims");
   size_t split = ilog2(blocks);
-  size_t n_outer = point.size() - 3 - split;
+  size_t n_outer = point.size() - LOG_LANES - split;
   const E8* outer = point.data() + split;
-  auto low = equality_table(point.data() + point.size() - 3, 3);
+  auto low = equality_table(point.data() + point.size() - LOG_LANES, LOG_LANES);
   auto high = equality_table(point.data(), split);
   auto weights = equality_table(outer, n_outer);
-  size_t plen = size_t(1) << (log_n - 3);
+  size_t plen = size_t(1) << (log_n - LOG_LANES);
   auto packed_block = [&](size_t b) {
     std::vector<E8> v(plen);
     for (size_t i = 0; i < plen; i++)
-      for (int j = 0; j < 8; j++) { int x = src(b, i * 8 + j); v[i].c[j] = u16(x < 0 ? x + int(P) : x); }
+      for (int j = 0; j < LANES; j++) v[i].c[j] = fsmall(src(b, i * LANES + j));
     return v;
   };
-  std::vector<std::array<E8, 8>> ebb(blocks);
+  std::vector<Lanes> ebb(blocks);
   for (size_t b = 0; b < blocks; b++) {
     auto lanes = project_lanes(packed_block(b), weights);
-    for (int img = 0; img < 8; img++) {
+    for (int img = 0; img < LANES; img++) {
       E8 s = e8_zero();
-      for (int lane = 0; lane < 8; lane++) s += basis(lane) * e8_frob(lanes[lane], (8 - img) % 8);
+      for (int lane = 0; lane < LANES; lane++) s += basis(lane) * e8_frob(lanes[lane], (LANES - img) % LANES);
       ebb[b][img] = s;
     }
   }
-  std::array<E8, 8> ev;
-  for (int img = 0; img < 8; img++) {
+  Lanes ev;
+  for (int img = 0; img < LANES; img++) {
     E8 s = e8_zero();
-    for (size_t b = 0; b < blocks; b++) s += e8_frob(high[b], (8 - img) % 8) * ebb[b][img];
+    for (size_t b = 0; b < blocks; b++) s += e8_frob(high[b], (LANES - img) % LANES) * ebb[b][img];
     ev[img] = s;
   }
   claimed_value = projected_claim(ev, low);
@@ -1292,12 +1293,12 @@ Bytes packed_open_batch(const PcsParams& pp, int log_n, const std::vector<Digest
   t.bytes(nb, 8);
   for (auto& r : roots) t.bytes(r.data(), 32);
   for (auto& v : ev) t.field(v);
-  auto powers = powers8(t.challenge());
+  auto powers = lane_powers(t.challenge());
   E8 claim = e8_zero();
-  for (int i = 0; i < 8; i++) claim += powers[i] * ev[i];
+  for (int i = 0; i < LANES; i++) claim += powers[i] * ev[i];
   auto cross = [&](size_t w, size_t q) {
     E8 s = e8_zero();
-    for (int i = 0; i < 8; i++) s += powers[i] * e8_frob(high[q], (8 - i) % 8) * ebb[w][i];
+    for (int i = 0; i < LANES; i++) s += powers[i] * e8_frob(high[q], (LANES - i) % LANES) * ebb[w][i];
     return s;
   };
   std::vector<E8> pos, neg;
@@ -1321,16 +1322,16 @@ Bytes packed_open_batch(const PcsParams& pp, int log_n, const std::vector<Digest
   fold_w[0] = e8_one();
   E8 oinv = e8_inv(omega);
   for (size_t b = 1; b < blocks; b++) fold_w[b] = fold_w[b - 1] * oinv;
-  std::array<E8, 8> coefs;
-  for (int img = 0; img < 8; img++) {
+  Lanes coefs;
+  for (int img = 0; img < LANES; img++) {
     E8 s = e8_zero(), fw = e8_one();
-    for (size_t b = 0; b < blocks; b++) { s += fw * e8_frob(high[b], (8 - img) % 8); fw = fw * omega; }
+    for (size_t b = 0; b < blocks; b++) { s += fw * e8_frob(high[b], (LANES - img) % LANES); fw = fw * omega; }
     coefs[img] = powers[img] * s;
   }
-  std::array<E8, 8> images;
-  for (int lane = 0; lane < 8; lane++) {
+  Lanes images;
+  for (int lane = 0; lane < LANES; lane++) {
     E8 s = e8_zero();
-    for (int img = 0; img < 8; img++) s += coefs[img] * e8_frob(basis(lane), (8 - img) % 8);
+    for (int img = 0; img < LANES; img++) s += coefs[img] * e8_frob(basis(lane), (LANES - img) % LANES);
     images[lane] = s;
   }
   apply_weight_images(weights, images);
@@ -1402,7 +1403,7 @@ Bytes packed_open_batch(const PcsParams& pp, int log_n, const std::vector<Digest
 Bytes packed_open_single(const PcsParams& pp, int log_n, const Committed& data, const std::vector<E8>& point,
                          const Bytes& context, E8& claimed_value) {
   check(point.size() == size_t(log_n), "packed point length");
-  size_t n_outer = log_n - 3;
+  size_t n_outer = log_n - LOG_LANES;
   // The packed witness is recovered from the Mobius coefficients by the zeta transform.
   std::vector<E8> values = data.coeffs;
   for (size_t h = 1; h < values.size(); h *= 2)
@@ -1410,26 +1411,26 @@ Bytes packed_open_single(const PcsParams& pp, int log_n, const Committed& data, 
       for (size_t i = b; i < b + h; i++) values[i + h] += values[i];
   auto weights = equality_table(point.data(), n_outer);
   auto lanes = project_lanes(values, weights);
-  auto low = equality_table(point.data() + n_outer, 3);
+  auto low = equality_table(point.data() + n_outer, LOG_LANES);
   claimed_value = e8_zero();
-  for (int j = 0; j < 8; j++) claimed_value += lanes[j] * low[j];
-  std::array<E8, 8> ev;
-  for (int i = 0; i < 8; i++) {
+  for (int j = 0; j < LANES; j++) claimed_value += lanes[j] * low[j];
+  Lanes ev;
+  for (int i = 0; i < LANES; i++) {
     E8 s = e8_zero();
-    for (int j = 0; j < 8; j++) s += basis(j) * e8_frob(lanes[j], (8 - i) % 8);
+    for (int j = 0; j < LANES; j++) s += basis(j) * e8_frob(lanes[j], (LANES - i) % LANES);
     ev[i] = s;
   }
   check(projected_claim(ev, low) == claimed_value, "projected claim");
   Digest root = data.root();
   PackedTranscript t = packed_transcript(pp, log_n, root, point, claimed_value, context);
   for (auto& v : ev) t.field(v);
-  auto powers = powers8(t.challenge());
+  auto powers = lane_powers(t.challenge());
   E8 claim = e8_zero();
-  for (int i = 0; i < 8; i++) claim += powers[i] * ev[i];
-  std::array<E8, 8> images;
-  for (int j = 0; j < 8; j++) {
+  for (int i = 0; i < LANES; i++) claim += powers[i] * ev[i];
+  Lanes images;
+  for (int j = 0; j < LANES; j++) {
     E8 s = e8_zero();
-    for (int i = 0; i < 8; i++) s += powers[i] * e8_frob(basis(j), (8 - i) % 8);
+    for (int i = 0; i < LANES; i++) s += powers[i] * e8_frob(basis(j), (LANES - i) % LANES);
     images[j] = s;
   }
   apply_weight_images(weights, images);
@@ -1444,8 +1445,7 @@ Bytes packed_open_single(const PcsParams& pp, int log_n, const Committed& data, 
   check(actual == fval, "native opening value");
   Bytes decide = pcs_prove_decide(pp, red);
   Bytes w;
-  const char* mg = "QA655PK1";
-  w.insert(w.end(), mg, mg + 8);
+  w.insert(w.end(), PACKED_MAGIC, PACKED_MAGIC + 8);
   put_e8(w, claimed_value);
   for (auto& v : ev) put_e8(w, v);
   put_u64(w, rounds.size());
@@ -1458,7 +1458,7 @@ Bytes packed_open_single(const PcsParams& pp, int log_n, const Committed& data, 
 
 // ================================================================ ParComm prove (protocol.rs:186-263)
 Bytes parcomm_prove(const Layout& l, const Config& cfg, const PcsParams& pp, int chunk_log, const Statement& st,
-                    const int8_t* values, u64 n_values, const std::vector<Claim>& claims) {
+                    const Val* values, u64 n_values, const std::vector<Claim>& claims) {
   OuterTranscript tr;
   tr.object("collection", json_statement(st));
   tr.object("claims", json_claims(claims));
@@ -1467,20 +1467,19 @@ Bytes parcomm_prove(const Layout& l, const Config& cfg, const PcsParams& pp, int
   u64 stride = u64(1) << cfg.packed_log;
   EoProof eo = eo_prove(values, n_values, l.logical, stride, q, tr, term);
   Bytes context;
-  const char* cx = "parcomm/qafold/terminal/v2";
-  context.insert(context.end(), cx, cx + std::strlen(cx));
+  context.insert(context.end(), TERMINAL_DOMAIN, TERMINAL_DOMAIN + std::strlen(TERMINAL_DOMAIN));
   context.insert(context.end(), tr.state.begin(), tr.state.end());
   put_u64(context, 0);
   u64 block_len = u64(1) << chunk_log;
   E8 actual;
   Bytes wire;
   if (st.roots.size() > 1) {
-    BlockSource src = [&](u64 b, u64 i) -> int8_t { u64 k = b * block_len + i; return k < n_values ? values[k] : 0; };
+    BlockSource src = [&](u64 b, u64 i) -> Val { u64 k = b * block_len + i; return k < n_values ? values[k] : 0; };
     wire = packed_open_batch(pp, chunk_log, st.roots, term.point_msb, context, src, actual);
   } else {
-    std::vector<int8_t> blk(block_len, 0);
-    std::memcpy(blk.data(), values, std::min<u64>(n_values, block_len));
-    Committed c = packed_commit(pp, pack_block_i8(blk.data(), block_len, block_len));
+    std::vector<Val> blk(block_len, 0);
+    std::memcpy(blk.data(), values, std::min<u64>(n_values, block_len) * sizeof(Val));
+    Committed c = packed_commit(pp, pack_block(blk.data(), block_len, block_len));
     wire = packed_open_single(pp, chunk_log, c, term.point_msb, context, actual);
   }
   check(actual * term.scale == term.value, "QAFold/EO terminal mismatch");

@@ -52,17 +52,17 @@ struct Query {
 E8 monomial_weight(u64 index, const E8* point_low, size_t n);
 E8 equality_weight(u64 index, const std::vector<E8>& point_msb);
 std::vector<E8> tbv_weights(const std::array<E8, 2>* factors, size_t n);
-std::vector<E8> tbv_project(const Term& t, const int8_t* values, u64 n_values, u64 stride);
-void cross_terms(const Query& q, const int8_t* values, u64 n_values, u64 stride, std::vector<E8>& pos, std::vector<E8>& neg);
+std::vector<E8> tbv_project(const Term& t, const Val* values, u64 n_values, u64 stride);
+void cross_terms(const Query& q, const Val* values, u64 n_values, u64 stride, std::vector<E8>& pos, std::vector<E8>& neg);
 std::vector<E8> query_fold(const Query& q, u64 logical, u64 stride, const E8& challenge);
 Query tbv_compile(const Layout& l, const std::vector<Claim>& claims, OuterTranscript& t);
-std::vector<Claim> evaluate_claims(const Layout& l, const int8_t* values, u64 n_values,
+std::vector<Claim> evaluate_claims(const Layout& l, const Val* values, u64 n_values,
                                    const std::vector<std::pair<std::vector<E8>, std::vector<E8>>>& points);
 
 // ---------------------------------------------------------------- EO (eo.rs)
 struct EoProof { std::vector<E8> positive, negative; std::vector<std::array<E8, 2>> rounds; E8 value, query; };
 struct Terminal { E8 value; std::vector<E8> point_msb; E8 scale; };
-EoProof eo_prove(const int8_t* values, u64 n_values, u64 logical, u64 stride, const Query& q, OuterTranscript& t,
+EoProof eo_prove(const Val* values, u64 n_values, u64 logical, u64 stride, const Query& q, OuterTranscript& t,
                  Terminal& terminal);
 
 // ---------------------------------------------------------------- native PCS opening
@@ -80,7 +80,7 @@ void write_base_rows_all(Bytes& out, const Committed& c);
 void write_base_rows_partial(Bytes& out, const Committed& c, const std::vector<u64>& rows);
 
 // ---------------------------------------------------------------- packed PCS (packed.rs, packed_batch.rs)
-using BlockSource = std::function<int8_t(u64 block, u64 index)>;
+using BlockSource = std::function<Val(u64 block, u64 index)>;
 // Batched opening over 2..128 blocks (recompute mode: blocks are re-committed from source).
 Bytes packed_open_batch(const PcsParams& pp, int log_n, const std::vector<Digest>& roots, const std::vector<E8>& point,
                         const Bytes& context, const BlockSource& src, E8& claimed_value);
@@ -90,6 +90,6 @@ Bytes packed_open_single(const PcsParams& pp, int log_n, const Committed& block,
 // ---------------------------------------------------------------- ParComm prover (protocol.rs)
 struct ProveResult { Bytes proof; E8 opening_value; };
 Bytes parcomm_prove(const Layout& l, const Config& cfg, const PcsParams& pp, int chunk_log, const Statement& st,
-                    const int8_t* values, u64 n_values, const std::vector<Claim>& claims);
+                    const Val* values, u64 n_values, const std::vector<Claim>& claims);
 
 }  // namespace orc
