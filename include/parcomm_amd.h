@@ -1,8 +1,16 @@
-/* parcomm_amd: MI355X-native QAFold prover (F65521 / Ext8 backend), C ABI.
+/* parcomm_amd: MI355X-native QAFold prover, C ABI.
  *
- * The entry points replace the Rust library API of the reference crate
- * `parcomm-qafold` (qafold/f65521/src/protocol.rs), which has no FFI of its
- * own; each declaration cites the item it stands in for. Buffers are
+ * The entry points replace the Rust library API of the reference crates
+ * `parcomm-qafold` (qafold/f65521/src/protocol.rs, F65521 / Ext8, i8 entries)
+ * and its KoalaBear twin (qafold/koalabear/src/protocol.rs, KoalaBear / Ext4,
+ * i16 entries), which have no FFI of their own; each declaration cites the
+ * item it stands in for (file:line in the F65521 crate; the KoalaBear crate has
+ * the same items). Like the reference, which ships one crate per field, the
+ * library is built once per field: libparcomm_amd.so (F65521) and
+ * libparcomm_amd_kb.so (KoalaBear) export the same symbols; qf_field() names
+ * the backend of the loaded library. "Entries" below are int8_t in the
+ * F65521 build and int16_t in the KoalaBear build; extension elements are 16
+ * bytes in both (8 little-endian u16 or 4 little-endian u32 coefficients). Buffers are
  * caller-owned host memory unless the name says `device`; device memory is
  * owned by the handles. Functions return 0 on success or one of the QF_ERR_*
  * codes (the reference engine's error kinds, vendor/qafold/src/lib.rs:20-35,
@@ -31,6 +39,7 @@ extern "C" {
 #define QF_ERR_ARGUMENT 8
 
 #define QF_FIELD_F65521 0
+#define QF_FIELD_KOALABEAR 1
 #define QF_CACHE_CACHED 0
 #define QF_CACHE_RECOMPUTE 1
 
@@ -40,9 +49,13 @@ typedef struct qf_committed qf_committed;
 /* Message of the last failure on this thread (empty after success). */
 const char* qf_last_error(void);
 
+/* Field backend of this library build: QF_FIELD_F65521 or QF_FIELD_KOALABEAR
+ * (the reference selects it by crate: qafold/f65521 or qafold/koalabear). */
+int qf_field(void);
+
 /* Parameters::new(shapes, Config{chunk_log, packed_log, query_rf_bits})
  * (protocol.rs:109-130): layout, QA sequence, setup commitments. `shapes` holds
- * n (rows, cols) pairs. */
+ * n (rows, cols) pairs; `field` must be this build's qf_field(). */
 int qf_params_new(int field, const uint64_t* shapes, size_t n, uint32_t chunk_log, uint32_t packed_log,
                   uint32_t query_rf_bits, qf_params** out);
 void qf_params_free(qf_params* p);
@@ -60,22 +73,22 @@ int qf_params_info(const qf_params* p, uint64_t* active, uint64_t* logical, uint
 int qf_params_bucket(const qf_params* p, uint64_t i, uint64_t out[8]);
 
 /* Parameters::commit / commit_recomputed (protocol.rs:131-185) over a BVA of
- * `len` == active signed bytes in host memory. Non-physical padding must be
+ * `len` == active entries in host memory. Non-physical padding must be
  * zero (layout.rs:136-155). cache_mode QF_CACHE_CACHED keeps every codeword
  * and tree resident in HBM; QF_CACHE_RECOMPUTE keeps roots only. */
-int qf_commit(const qf_params* p, const int8_t* values, size_t len, int cache_mode, qf_committed** out);
+int qf_commit(const qf_params* p, const void* values, size_t len, int cache_mode, qf_committed** out);
 
 /* As qf_commit, with the BVA already resident in device memory (16-byte
  * aligned; borrowed: it must stay valid while the returned handle lives). */
-int qf_commit_device(const qf_params* p, const int8_t* device_values, size_t len, int cache_mode,
+int qf_commit_device(const qf_params* p, const void* device_values, size_t len, int cache_mode,
                      qf_committed** out);
 void qf_committed_free(qf_committed* c);
 
 /* Commitment.roots (protocol.rs:33-40): blocks x 32 bytes. */
 int qf_committed_roots(const qf_committed* c, uint8_t* out, size_t cap_blocks);
 
-/* Copy block `block`'s column-major codeword (32 x R Ext8 elements, 16 bytes
- * each) to host memory; only for cached commitments. `cap` is in bytes. */
+/* Copy block `block`'s column-major codeword (32 x R extension elements, 16
+ * bytes each) to host memory; only for cached commitments. `cap` is in bytes. */
 int qf_committed_codeword(const qf_committed* c, uint64_t block, void* out, size_t cap);
 
 /* Device time in milliseconds of the kernels of the last qf_commit*, and the
@@ -85,10 +98,10 @@ int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms
 typedef struct qf_proof qf_proof;
 
 /* Size in bytes of a packed claim list for these parameters: for every matrix
- * in order, its row point (log2(next_pow2(rows)) Ext8 elements), its column
- * point (log2(next_pow2(cols)) elements) and its value, each element 16 bytes
- * of little-endian u16 coefficients (tbv.rs:6-11, Claim; layout of the
- * reference's serde bytes for Scalar). */
+ * in order, its row point (log2(next_pow2(rows)) extension elements), its column
+ * point (log2(next_pow2(cols)) elements) and its value, each element its 16
+ * canonical bytes (tbv.rs:6-11, Claim; layout of the reference's serde bytes
+ * for Scalar). */
 int qf_claims_size(const qf_params* p, size_t* bytes);
 
 /* The benchmark statement of the reference CLI (main.rs:188-209): points drawn
@@ -139,9 +152,11 @@ int qf_device_free(void* device_ptr);
 int qf_device_memory(size_t* free_bytes, size_t* total_bytes);
 
 /* Synthetic workload generator (no reference counterpart; the reference CLI
- * streams real checkpoints): fills the active BVA in device memory with
- * clip(round(N(0, sigma)), -128, 127) on every physical entry and zero on
- * boundary padding, from a counter-based generator keyed by `seed`. */
+ * streams real checkpoints): fills the active BVA in device memory, from a
+ * counter-based generator keyed by `seed`, with clip(round(N(0, sigma)),
+ * -128, 127) (F65521 build) or the raw bfloat16 bit pattern of N(0, sigma),
+ * rounded to nearest even (KoalaBear build), on every physical entry and zero
+ * on boundary padding. */
 int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* device_out);
 
 /* Launch form of the opening's per-term kernels (no reference counterpart; the
@@ -157,7 +172,7 @@ int qf_set_launch_mode(int mode);
  * one register-radix transform pass (mode 0 Mobius, 1 Mobius then WHT into
  * out2, 2 WHT, 3 WHT . alpha . WHT, 4 copy into out1 and WHT into out2, 5 copy
  * into out1 and WHT . alpha . WHT into out2) over
- * `vectors` device vectors of 2^L Ext8 elements, on index bits [s, s+t) and,
+ * `vectors` device vectors of 2^L extension elements, on index bits [s, s+t) and,
  * when xform_vectors is set (s = 0), also on the index bits of the 2^(12-t)
  * vectors that share a tile. Returns QF_ERR_ARGUMENT when the geometry does
  * not fit a tile. Synchronous. */

@@ -1,10 +1,13 @@
-"""MI355X-native QAFold prover (F65521 / Ext8 backend).
+"""MI355X-native QAFold prover.
 
-Python mirror of the reference crate's `Parameters` API
-(qafold/f65521/src/protocol.rs:18-263): `Config`, `Shape`, `Parameters.new`,
-`Parameters.commit` / `commit_recomputed` and the `Commitment` statement. All
-work runs in the HIP library `libparcomm_amd.so` built from `csrc/`; loading
-fails loudly when that library is missing.
+Python mirror of the reference crates' `Parameters` API
+(qafold/f65521/src/protocol.rs:18-263 for F65521 / Ext8 with i8 entries,
+qafold/koalabear/src/protocol.rs for KoalaBear / Ext4 with i16 entries):
+`Config`, `Shape`, `Parameters.new`, `Parameters.commit` /
+`commit_recomputed`, `prove`, `verify` and the `Commitment` statement. All work
+runs in the HIP libraries built from `csrc/`, one per field
+(`libparcomm_amd.so`, `libparcomm_amd_kb.so`); loading fails loudly when the
+library of the requested field is missing.
 """
 import ctypes
 import os
@@ -13,10 +16,19 @@ from typing import List, Sequence
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libparcomm_amd.so")
+LIB_PATH_KB = os.path.join(_HERE, "libparcomm_amd_kb.so")
 
 QF_CACHE_CACHED = 0
 QF_CACHE_RECOMPUTE = 1
 QF_FIELD_F65521 = 0
+QF_FIELD_KOALABEAR = 1
+F65521 = QF_FIELD_F65521
+KOALABEAR = QF_FIELD_KOALABEAR
+# entry type and coefficient type of each backend
+_FIELD_INFO = {
+    QF_FIELD_F65521: {"path": LIB_PATH, "entry": "int8", "coef": "<u2", "lanes": 8, "p": 65521},
+    QF_FIELD_KOALABEAR: {"path": LIB_PATH_KB, "entry": "int16", "coef": "<u4", "lanes": 4, "p": 2130706433},
+}
 
 _ERRORS = {
     1: "EncodingError",
@@ -36,21 +48,29 @@ class QfError(RuntimeError):
         self.status = status
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    """The loaded HIP library; raises if it has not been built."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def field_info(field=QF_FIELD_F65521):
+    """Library path, entry dtype, coefficient dtype, lanes and modulus of a backend."""
+    if field not in _FIELD_INFO:
+        raise ValueError(f"unknown field backend {field}")
+    return _FIELD_INFO[field]
+
+
+def lib(field=QF_FIELD_F65521):
+    """The loaded HIP library of a field backend; raises if it has not been built."""
+    if field not in _libs:
+        path = field_info(field)["path"]
+        if not os.path.exists(path):
             raise RuntimeError(
-                f"{LIB_PATH} is missing: build it with `python -m parcomm_amd.build`"
+                f"{path} is missing: build it with `python -m parcomm_amd.build`"
             )
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         vp, u64, sz, i32, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32
         P = ctypes.POINTER
         L.qf_last_error.restype = ctypes.c_char_p
+        L.qf_field.restype = i32
         L.qf_params_new.argtypes = [i32, P(u64), sz, u32, u32, u32, P(vp)]
         L.qf_params_free.argtypes = [vp]
         L.qf_params_free.restype = None
@@ -82,50 +102,54 @@ def lib():
         L.qf_set_launch_mode.argtypes = [i32]
         L.qf_device_memory.argtypes = [P(sz), P(sz)]
         L.qf_test_tile_pass.argtypes = [i32, vp, vp, vp, vp, u64, u64, u64, i32, i32, i32, i32]
-        _lib = L
-    return _lib
+        if L.qf_field() != field:
+            raise RuntimeError(f"{path} implements field {L.qf_field()}, not {field}")
+        _libs[field] = L
+    return _libs[field]
 
 
-def _check(status):
+def _check(status, field=QF_FIELD_F65521):
     if status != 0:
-        raise QfError(status, lib().qf_last_error().decode())
+        raise QfError(status, lib(field).qf_last_error().decode())
 
 
 LAUNCH_BATCHED = 0
 LAUNCH_PER_MATRIX = 1
 
 
-def set_launch_mode(mode):
+def set_launch_mode(mode, field=QF_FIELD_F65521):
     """Batched (one launch per phase over all matrices) or plain per-matrix launches."""
-    _check(lib().qf_set_launch_mode(mode))
+    _check(lib(field).qf_set_launch_mode(mode), field)
 
 
 class DeviceBuffer:
     """Device memory owned by the library (resident BVA)."""
 
-    def __init__(self, nbytes):
+    def __init__(self, nbytes, field=QF_FIELD_F65521):
         self.nbytes = int(nbytes)
+        self.field = field
+        self._L = lib(field)
         p = ctypes.c_void_p()
-        _check(lib().qf_device_alloc(self.nbytes, ctypes.byref(p)))
+        _check(self._L.qf_device_alloc(self.nbytes, ctypes.byref(p)), field)
         self.ptr = p
 
     @classmethod
-    def from_host(cls, array):
+    def from_host(cls, array, field=QF_FIELD_F65521):
         import numpy as np
         arr = np.ascontiguousarray(array)
-        buf = cls(arr.nbytes)
-        _check(lib().qf_device_upload(buf.ptr, arr.ctypes.data, arr.nbytes))
+        buf = cls(arr.nbytes, field)
+        _check(buf._L.qf_device_upload(buf.ptr, arr.ctypes.data, arr.nbytes), field)
         return buf
 
     def to_host(self, dtype="int8"):
         import numpy as np
         out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
-        _check(lib().qf_device_download(out.ctypes.data, self.ptr, out.nbytes))
+        _check(self._L.qf_device_download(out.ctypes.data, self.ptr, out.nbytes), self.field)
         return out
 
     def close(self):
         if self.ptr:
-            lib().qf_device_free(self.ptr)
+            self._L.qf_device_free(self.ptr)
             self.ptr = ctypes.c_void_p()
 
     def __del__(self):
@@ -161,8 +185,9 @@ class Commitment:
 
 @dataclass
 class Claim:
-    """tbv.rs:6-11: evaluation claim of one matrix. Scalars are 16-byte
-    canonical Ext8 elements (8 little-endian u16 coefficients)."""
+    """tbv.rs:6-11: evaluation claim of one matrix. Scalars are the 16-byte
+    canonical form of extension elements (8 little-endian u16 coefficients
+    for F65521, 4 little-endian u32 for KoalaBear)."""
     row: List[bytes]
     col: List[bytes]
     value: bytes
@@ -177,32 +202,37 @@ class Committed:
 
     def __init__(self, params, handle, statement):
         self._params = params
+        self._L = params._L
         self._h = ctypes.c_void_p(handle)
         self.statement = statement
 
     def codeword(self, block):
+        """Column-major codeword of a block: (32 columns, R rows, coefficients)."""
         import numpy as np
-        n_rows = 2 * (self._params.block_len // 256)
-        out = np.empty((32, n_rows, 8), dtype=np.uint16)
-        _check(lib().qf_committed_codeword(self._h, block, out.ctypes.data, out.nbytes))
+        info = field_info(self._params.field)
+        n_rows = 2 * (self._params.block_len // (32 * info["lanes"]))
+        out = np.empty((32, n_rows, info["lanes"]), dtype=info["coef"])
+        _check(self._L.qf_committed_codeword(self._h, block, out.ctypes.data, out.nbytes), self._params.field)
         return out
 
     def statement_json(self) -> str:
         """serde_json of the Commitment statement (commitment.json of the CLI)."""
         n = ctypes.c_size_t()
-        lib().qf_statement_json(self._params._h, self._h, None, 0, ctypes.byref(n))
+        self._L.qf_statement_json(self._params._h, self._h, None, 0, ctypes.byref(n))
         buf = ctypes.create_string_buffer(n.value + 1)
-        _check(lib().qf_statement_json(self._params._h, self._h, buf, n.value + 1, ctypes.byref(n)))
+        _check(self._L.qf_statement_json(self._params._h, self._h, buf, n.value + 1, ctypes.byref(n)),
+               self._params.field)
         return buf.value.decode()
 
     def timing(self):
         t, l, e = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        _check(lib().qf_committed_timing(self._h, ctypes.byref(t), ctypes.byref(l), ctypes.byref(e)))
+        _check(self._L.qf_committed_timing(self._h, ctypes.byref(t), ctypes.byref(l), ctypes.byref(e)),
+               self._params.field)
         return {"total_ms": t.value, "leaf_ms": l.value, "encode_ms": e.value}
 
     def close(self):
         if self._h:
-            lib().qf_committed_free(self._h)
+            self._L.qf_committed_free(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):
@@ -213,50 +243,59 @@ class Committed:
 
 
 class Parameters:
-    """protocol.rs:41-45, 109-130"""
+    """protocol.rs:41-45, 109-130. `field` selects the backend crate:
+    F65521 (i8 entries) or KOALABEAR (i16 entries)."""
 
-    def __init__(self, shapes: Sequence[Shape], config: Config = None):
+    def __init__(self, shapes: Sequence[Shape], config: Config = None, field: int = QF_FIELD_F65521):
         config = config or Config()
+        self.field = field
+        self._L = lib(field)
+        self.entry_dtype = field_info(field)["entry"]
         flat = (ctypes.c_uint64 * (2 * len(shapes)))(*[v for s in shapes for v in (s.rows, s.cols)])
         h = ctypes.c_void_p()
-        _check(lib().qf_params_new(QF_FIELD_F65521, flat, len(shapes), config.chunk_log, config.packed_log,
-                                   config.query_rf_bits, ctypes.byref(h)))
+        self._check(self._L.qf_params_new(field, flat, len(shapes), config.chunk_log, config.packed_log,
+                                          config.query_rf_bits, ctypes.byref(h)))
         self._h = h
         self.config = config
         self.shapes = list(shapes)
         a, l, b, n, nb = (ctypes.c_uint64() for _ in range(5))
-        _check(lib().qf_params_info(h, ctypes.byref(a), ctypes.byref(l), ctypes.byref(b), ctypes.byref(n), ctypes.byref(nb)))
+        self._check(self._L.qf_params_info(h, ctypes.byref(a), ctypes.byref(l), ctypes.byref(b), ctypes.byref(n),
+                                           ctypes.byref(nb)))
         self.active, self.logical, self.block_len, self.blocks, self.n_buckets = a.value, l.value, b.value, n.value, nb.value
 
-    new = classmethod(lambda cls, shapes, config=None: cls(shapes, config))
+    new = classmethod(lambda cls, shapes, config=None, field=QF_FIELD_F65521: cls(shapes, config, field))
+
+    def _check(self, status):
+        _check(status, self.field)
 
     def parameter_digest(self) -> bytes:
         out = ctypes.create_string_buffer(32)
-        _check(lib().qf_params_digest(self._h, out))
+        self._check(self._L.qf_params_digest(self._h, out))
         return out.raw
 
     def buckets(self):
         out = []
         row = (ctypes.c_uint64 * 8)()
         for i in range(self.n_buckets):
-            _check(lib().qf_params_bucket(self._h, i, row))
+            self._check(self._L.qf_params_bucket(self._h, i, row))
             out.append(tuple(row))
         return out
 
     def _statement(self, handle):
         buf = ctypes.create_string_buffer(32 * self.blocks)
-        _check(lib().qf_committed_roots(handle, buf, self.blocks))
+        self._check(self._L.qf_committed_roots(handle, buf, self.blocks))
         roots = [buf.raw[32 * i:32 * i + 32] for i in range(self.blocks)]
         return Commitment(2, self.config, self.shapes, self.parameter_digest(), roots)
 
     def _commit(self, values, mode):
+        import numpy as np
         h = ctypes.c_void_p()
         if isinstance(values, DeviceBuffer):  # BVA resident in HBM
-            _check(lib().qf_commit_device(self._h, values.ptr, values.nbytes, mode, ctypes.byref(h)))
+            entries = values.nbytes // np.dtype(self.entry_dtype).itemsize
+            self._check(self._L.qf_commit_device(self._h, values.ptr, entries, mode, ctypes.byref(h)))
         else:
-            import numpy as np
-            arr = np.ascontiguousarray(values, dtype=np.int8)
-            _check(lib().qf_commit(self._h, arr.ctypes.data, arr.size, mode, ctypes.byref(h)))
+            arr = np.ascontiguousarray(values, dtype=self.entry_dtype)
+            self._check(self._L.qf_commit(self._h, arr.ctypes.data, arr.size, mode, ctypes.byref(h)))
         committed = Committed(self, h.value, self._statement(h))
         if isinstance(values, DeviceBuffer):
             committed._bva = values  # borrowed by the handle: the opening reads it again
@@ -296,9 +335,9 @@ class Parameters:
     def benchmark_claims(self, committed: "Committed") -> List[Claim]:
         """The CLI's benchmark statement (main.rs:188-209)."""
         n = ctypes.c_size_t()
-        _check(lib().qf_claims_size(self._h, ctypes.byref(n)))
+        self._check(self._L.qf_claims_size(self._h, ctypes.byref(n)))
         buf = ctypes.create_string_buffer(n.value)
-        _check(lib().qf_benchmark_claims(self._h, committed._h, buf, n.value))
+        self._check(self._L.qf_benchmark_claims(self._h, committed._h, buf, n.value))
         return self.claims_from_bytes(buf.raw)
 
     def evaluate_claims(self, committed: "Committed", points) -> List[Claim]:
@@ -308,42 +347,47 @@ class Parameters:
             for v in list(row) + list(col) + [bytes(16)]:
                 raw += v
         buf = ctypes.create_string_buffer(bytes(raw), len(raw))
-        _check(lib().qf_evaluate_claims(self._h, committed._h, buf, len(raw)))
+        self._check(self._L.qf_evaluate_claims(self._h, committed._h, buf, len(raw)))
         return self.claims_from_bytes(buf.raw)
 
     def prove(self, committed: "Committed", claims: Sequence[Claim]):
         """Parameters::prove (protocol.rs:201-263): (proof bytes, profile in seconds)."""
         raw = self.claims_to_bytes(claims)
         h = ctypes.c_void_p()
-        _check(lib().qf_prove(self._h, committed._h, raw, len(raw), ctypes.byref(h)))
+        self._check(self._L.qf_prove(self._h, committed._h, raw, len(raw), ctypes.byref(h)))
         try:
             data = ctypes.POINTER(ctypes.c_uint8)()
             n = ctypes.c_size_t()
-            _check(lib().qf_proof_bytes(h, ctypes.byref(data), ctypes.byref(n)))
+            self._check(self._L.qf_proof_bytes(h, ctypes.byref(data), ctypes.byref(n)))
             proof = ctypes.string_at(data, n.value)
             t = (ctypes.c_double * 9)()
-            _check(lib().qf_proof_timing(h, t))
+            self._check(self._L.qf_proof_timing(h, t))
             profile = {k + "_seconds": t[i] / 1000.0 for i, k in enumerate(PROFILE_KEYS)}
         finally:
-            lib().qf_proof_free(h)
+            self._L.qf_proof_free(h)
         return proof, profile
 
     def verify(self, statement: "Commitment", claims: Sequence[Claim], proof: bytes):
         """Parameters::verify (protocol.rs:266-319); raises QfError on rejection."""
         raw = self.claims_to_bytes(claims)
         roots = b"".join(statement.roots)
-        _check(lib().qf_verify(self._h, roots, len(statement.roots), raw, len(raw), proof, len(proof)))
+        self._check(self._L.qf_verify(self._h, roots, len(statement.roots), raw, len(raw), proof, len(proof)))
         return True
 
-    def synthetic_bva(self, seed, sigma=20.0):
-        """Device-resident synthetic BVA: clip(round(N(0, sigma))) on physical entries."""
-        buf = DeviceBuffer(self.active)
-        _check(lib().qf_synthetic_bva(self._h, seed, sigma, buf.ptr))
+    def synthetic_bva(self, seed, sigma=None):
+        """Device-resident synthetic BVA on physical entries: clip(round(N(0, sigma)))
+        as int8 (F65521, sigma 20 by default) or the BF16 bit patterns of
+        N(0, sigma) (KoalaBear, sigma 0.02 by default)."""
+        import numpy as np
+        if sigma is None:
+            sigma = 0.02 if self.field == QF_FIELD_KOALABEAR else 20.0
+        buf = DeviceBuffer(self.active * np.dtype(self.entry_dtype).itemsize, self.field)
+        self._check(self._L.qf_synthetic_bva(self._h, seed, sigma, buf.ptr))
         return buf
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().qf_params_free(self._h)
+            self._L.qf_params_free(self._h)
             self._h = None
 
     def __del__(self):

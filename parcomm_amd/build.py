@@ -1,4 +1,7 @@
-"""Builds the product library parcomm_amd/libparcomm_amd.so with hipcc for gfx950.
+"""Builds the product libraries with hipcc for gfx950: one per field backend,
+from the same sources (the reference ships one crate per field):
+parcomm_amd/libparcomm_amd.so (F65521 / Ext8, i8 entries) and
+parcomm_amd/libparcomm_amd_kb.so (KoalaBear / Ext4, i16 entries, -DQF_KB).
 
 Run as `python -m parcomm_amd.build` or through __graft_entry__.build().
 """
@@ -9,6 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libparcomm_amd.so")
+OUT_KB = os.path.join(HERE, "libparcomm_amd_kb.so")
+TARGETS = ((OUT, []), (OUT_KB, ["-DQF_KB"]))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -26,17 +31,25 @@ def sources():
 
 
 def build(verbose=False):
+    """Compiles both libraries concurrently; returns their paths."""
     srcs = sources()
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libparcomm_amd.so")
-    os.replace(tmp, OUT)
-    return OUT
+    procs = []
+    for out, extra in TARGETS:
+        cmd = [HIPCC] + FLAGS + extra + ["-o", out + ".tmp"] + srcs
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((out, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for out, proc in procs:
+        log, _ = proc.communicate()
+        if proc.returncode != 0:
+            sys.stderr.write(log)
+            failed.append(os.path.basename(out))
+    if failed:
+        raise RuntimeError("hipcc failed building " + ", ".join(failed))
+    for out, _ in procs:
+        os.replace(out + ".tmp", out)
+    return [out for out, _ in TARGETS]
 
 
 if __name__ == "__main__":

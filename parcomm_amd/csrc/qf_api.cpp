@@ -52,14 +52,14 @@ std::vector<uint64_t> padding_ranges(const Layout& l) {
   return out;
 }
 
-__global__ void k_check_zero(const int8_t* bva, const uint64_t* ranges, uint32_t* flag) {
+__global__ void k_check_zero(const Val* bva, const uint64_t* ranges, uint32_t* flag) {
   uint64_t start = ranges[2 * blockIdx.x], len = ranges[2 * blockIdx.x + 1];
   uint32_t bad = 0;
   for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) bad |= bva[start + i] != 0;
   if (bad) atomicOr(flag, 1u);
 }
 
-void validate_device(const ParamsState* p, const int8_t* d_bva, hipStream_t st) {
+void validate_device(const ParamsState* p, const Val* d_bva, hipStream_t st) {
   {
     std::lock_guard<std::mutex> g(p->mu);
     if (p->zero_ranges.empty()) p->zero_ranges = padding_ranges(p->layout);
@@ -82,7 +82,7 @@ void validate_device(const ParamsState* p, const int8_t* d_bva, hipStream_t st) 
   require(flag == 0, QF_ERR_INVALID_PARAMETERS, "nonzero boundary padding");
 }
 
-void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_mode, qf_committed** out) {
+void commit_impl(const qf_params* p, const Val* d_bva, size_t len, int cache_mode, qf_committed** out) {
   require(len == p->layout.active, QF_ERR_INVALID_PARAMETERS, "active BVA length mismatch");
   require(cache_mode == QF_CACHE_CACHED || cache_mode == QF_CACHE_RECOMPUTE, QF_ERR_ARGUMENT, "cache mode");
   hipStream_t st = thread_stream();
@@ -94,7 +94,7 @@ void commit_impl(const qf_params* p, const int8_t* d_bva, size_t len, int cache_
     validate_device(p, d_bva, st);
     const uint64_t n = uint64_t(1) << p->geom.log_n, R = 2 * n;
     const size_t cw_bytes = size_t(32 * R) * 16, tree_bytes = size_t(2 * R - 1) * 32;
-    const E8* alpha = p->alpha_device(p->pp.codes.size() - 1);
+    const E8* alpha = p->alpha_encode(p->pp.codes.size() - 1);
     hipEvent_t e0, e1, ev[4];
     QF_HIP(hipEventCreate(&e0)); QF_HIP(hipEventCreate(&e1));
     for (auto& e : ev) QF_HIP(hipEventCreate(&e));
@@ -182,12 +182,13 @@ std::vector<ClaimIn> parse_claims(const ParamsState& p, const uint8_t* q) {
 extern "C" {
 
 const char* qf_last_error(void) { return g_error.c_str(); }
+int qf_field(void) { return QF_FIELD; }
 
 int qf_params_new(int field, const uint64_t* shapes, size_t n, uint32_t chunk_log, uint32_t packed_log,
                   uint32_t query_rf_bits, qf_params** out) {
   return guarded([&] {
     require(out && shapes, QF_ERR_ARGUMENT, "null argument");
-    require(field == QF_FIELD_F65521, QF_ERR_INVALID_PARAMETERS, "unsupported field backend");
+    require(field == QF_FIELD, QF_ERR_INVALID_PARAMETERS, "this library is built for the other field backend");
     require(chunk_log >= 10 && chunk_log <= 30 && packed_log >= 1 && packed_log <= 6 && query_rf_bits >= 32 &&
                 query_rf_bits <= 120,
             QF_ERR_INVALID_PARAMETERS, "invalid ParComm config");
@@ -205,7 +206,7 @@ int qf_params_new(int field, const uint64_t* shapes, size_t n, uint32_t chunk_lo
     uint8_t seed[32];
     std::memset(seed, 0x51, 32);  // protocol.rs:123
     p->pp = make_pcs_params(p->chunk_log, query_rf_bits, seed);
-    p->geom.log_n = p->chunk_log - 8;
+    p->geom.log_n = p->chunk_log - LOG_LANES - 5;  // 32 columns of packed elements
     p->geom.leaf = leaf_prefix(32);
     p->geom.inner = inner_prefix();
     p->leaf_ext = leaf_prefix(32, "qafold-short/extension-row/v1");
@@ -240,13 +241,13 @@ int qf_params_bucket(const qf_params* p, uint64_t i, uint64_t out[8]) {
   });
 }
 
-int qf_commit(const qf_params* p, const int8_t* values, size_t len, int cache_mode, qf_committed** out) {
+int qf_commit(const qf_params* p, const void* values, size_t len, int cache_mode, qf_committed** out) {
   return guarded([&] {
     require(p && out && (values || len == 0), QF_ERR_ARGUMENT, "null argument");
-    int8_t* d = nullptr;
-    QF_HIP(hipMalloc(&d, std::max<size_t>(len, 8)));
+    Val* d = nullptr;
+    QF_HIP(hipMalloc(&d, std::max<size_t>(len * sizeof(Val), 16)));
     try {
-      QF_HIP(hipMemcpy(d, values, len, hipMemcpyHostToDevice));
+      QF_HIP(hipMemcpy(d, values, len * sizeof(Val), hipMemcpyHostToDevice));
       commit_impl(p, d, len, cache_mode, out);
     } catch (...) {
       (void)hipFree(d);
@@ -256,11 +257,11 @@ int qf_commit(const qf_params* p, const int8_t* values, size_t len, int cache_mo
   });
 }
 
-int qf_commit_device(const qf_params* p, const int8_t* device_values, size_t len, int cache_mode, qf_committed** out) {
+int qf_commit_device(const qf_params* p, const void* device_values, size_t len, int cache_mode, qf_committed** out) {
   return guarded([&] {
     require(p && out && device_values, QF_ERR_ARGUMENT, "null argument");
     require(reinterpret_cast<uintptr_t>(device_values) % 16 == 0, QF_ERR_ARGUMENT, "device BVA must be 16-byte aligned");
-    commit_impl(p, device_values, len, cache_mode, out);
+    commit_impl(p, static_cast<const Val*>(device_values), len, cache_mode, out);
   });
 }
 
@@ -379,12 +380,20 @@ int qf_test_tile_pass(int mode, const void* device_in, void* device_out1, void* 
                       const void* device_alpha, uint64_t vectors, uint64_t in_vstride, uint64_t out_vstride, int L,
                       int s, int t, int xform_vectors) {
   return guarded([&] {
-    require(device_in && device_out1, QF_ERR_ARGUMENT, "null argument");
-    bool ok = tile_pass(mode, false, device_in, static_cast<E8*>(device_out1), static_cast<E8*>(device_out2),
-                        static_cast<const E8*>(device_alpha), vectors, in_vstride, out_vstride, L, s, t, 0, 0, nullptr,
-                        xform_vectors != 0);
+    require(device_in && device_out1 && L >= 0 && L < 40, QF_ERR_ARGUMENT, "null argument");
+    const E8* alpha = static_cast<const E8*>(device_alpha);
+    E8* enc = nullptr;  // alpha_1 in the encoding kernels' form
+    if (alpha && LANES != 8) {
+      QF_HIP(hipMalloc(&enc, (uint64_t(1) << L) * 16));
+      launch_encode_form(alpha, enc, uint64_t(1) << L, nullptr);
+      alpha = enc;
+    }
+    bool ok = tile_pass(mode, false, device_in, static_cast<E8*>(device_out1), static_cast<E8*>(device_out2), alpha,
+                        vectors, in_vstride, out_vstride, L, s, t, 0, 0, nullptr, xform_vectors != 0);
+    hipError_t e = hipDeviceSynchronize();
+    if (enc) (void)hipFree(enc);
+    QF_HIP(e);
     require(ok, QF_ERR_ARGUMENT, "geometry does not fit a tile pass");
-    QF_HIP(hipDeviceSynchronize());
   });
 }
 

@@ -1,8 +1,8 @@
 // HIP kernels of the commitment phase for gfx950 (CDNA4).
 //
-// One PCS block of 2^c signed bytes becomes a 32-column codeword of Ext8
-// elements, column-major in HBM: column k occupies entries [k*R, (k+1)*R),
-// R = 2n rows, n = 2^(c-8) message rows. The message half of each column is
+// One PCS block of 2^c signed entries (i8 for F65521, i16 for KoalaBear)
+// becomes a 32-column codeword of extension elements, column-major in HBM: column k occupies entries [k*R, (k+1)*R),
+// R = 2n rows, n = 2^c / (32 LANES) message rows. The message half of each column is
 // the Mobius-transformed packed witness (packed.rs:220-258, :708-752); the
 // parity half is WHT(alpha_1 . WHT(m)) (qafold_short.rs:969-1000). Rows are
 // hashed into BLAKE2s leaves and a binary Merkle tree
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(WG) void k_pass(const E8* __restrict__ in, E8* __re
   if (ALPHA) {
     for (int e = threadIdx.x; e < n; e += WG) {
       uint64_t j = base + (e % g.C) + (uint64_t(e / g.C) << g.s);
-      t[e] = v8_store(v8_mul(v8_load(t[e]), v8_load(alpha[j])));
+      t[e] = v8_store(v8_mul_enc(v8_load(t[e]), v8_load(alpha[j])));
     }
     __syncthreads();
     lds_stages<OP>(t, g.C, g.t);
@@ -90,36 +90,40 @@ __global__ __launch_bounds__(WG) void k_pass(const E8* __restrict__ in, E8* __re
   }
 }
 
-// Pack 8 signed bytes per element (lane j = byte j mod p; bytes at or past
-// `available` read as zero), run the Mobius stages on the low `t` bits of the
-// element index inside a contiguous tile, and write the message half.
-__global__ __launch_bounds__(WG) void k_pack_mobius(const int8_t* __restrict__ bva, uint64_t block_first,
+// The packed element whose LANES entries start at BVA byte `byte`: lane j =
+// entry j mod p (field.rs:24-26, packed.rs:230-243); bytes at or past
+// `available` read as zero. Both fields pack 8 bytes per element.
+__device__ inline E8 packed_element(const uint8_t* __restrict__ bva, uint64_t byte, uint64_t available) {
+  uint32_t w[2] = {0, 0};
+  if (byte + 8 <= available) {
+    const uint2 u = *reinterpret_cast<const uint2*>(bva + byte);
+    w[0] = u.x;
+    w[1] = u.y;
+  } else {
+    for (int j = 0; j < 8; j++)
+      if (byte + j < available) w[j >> 2] |= uint32_t(bva[byte + j]) << (8 * (j & 3));
+  }
+  V8 v;
+#ifdef QF_KB
+#pragma unroll
+  for (int k = 0; k < 4; k++) v.c[k] = from_val(int16_t(w[k >> 1] >> (16 * (k & 1))));
+#else
+#pragma unroll
+  for (int k = 0; k < 8; k++) v.c[k] = from_val(int8_t(w[k >> 2] >> (8 * (k & 3))));
+#endif
+  return v8_store(v);
+}
+
+// Pack one element per 8 BVA bytes, run the Mobius stages on the low `t`
+// bits of the element index inside a contiguous tile, and write the message
+// half. `block_first` and `available` are byte offsets.
+__global__ __launch_bounds__(WG) void k_pack_mobius(const uint8_t* __restrict__ bva, uint64_t block_first,
                                                     uint64_t available, E8* __restrict__ cw, int log_n,
                                                     uint64_t rows, int tbits) {
   __shared__ E8 t[TILE];
   const int n = 1 << tbits;
   uint64_t first = uint64_t(blockIdx.x) << tbits;  // packed element index within the block
-  for (int e = threadIdx.x; e < n; e += WG) {
-    uint64_t i = first + e;
-    uint64_t byte = block_first + 8 * i;
-    E8 v;
-    if (byte + 8 <= available) {
-      uint2 w = *reinterpret_cast<const uint2*>(bva + byte);
-      uint32_t b[2] = {w.x, w.y};
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        int x = int8_t(b[k >> 2] >> (8 * (k & 3)));
-        v.c[k] = uint16_t(x < 0 ? x + int(P) : x);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        int x = byte + k < available ? int(bva[byte + k]) : 0;
-        v.c[k] = uint16_t(x < 0 ? x + int(P) : x);
-      }
-    }
-    t[e] = v;
-  }
+  for (int e = threadIdx.x; e < n; e += WG) t[e] = packed_element(bva, block_first + 8 * (first + e), available);
   __syncthreads();
   lds_stages<OP_MOBIUS>(t, 1, tbits);
   uint64_t col = first >> log_n;
@@ -136,6 +140,18 @@ __global__ __launch_bounds__(WG) void k_mobius_columns(E8* __restrict__ cw, uint
   if (row >= n) return;
   uint32_t* base = reinterpret_cast<uint32_t*>(cw) + row * 4 + word;
   const uint64_t cstride = rows * 4;
+#ifdef QF_KB  // one coefficient per word
+  uint32_t c[32];
+#pragma unroll
+  for (int k = 0; k < 32; k++) c[k] = base[k * cstride];
+#pragma unroll
+  for (int h = 1; h < 32; h *= 2)
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+      if (k & h) c[k] = msub(c[k], c[k ^ h]);
+#pragma unroll
+  for (int k = 0; k < 32; k++) base[k * cstride] = c[k];
+#else  // two u16 coefficients per word
   uint32_t lo[32], hi[32];
 #pragma unroll
   for (int k = 0; k < 32; k++) {
@@ -150,6 +166,7 @@ __global__ __launch_bounds__(WG) void k_mobius_columns(E8* __restrict__ cw, uint
       if (k & h) { lo[k] = msub(lo[k], lo[k ^ h]); hi[k] = msub(hi[k], hi[k ^ h]); }
 #pragma unroll
   for (int k = 0; k < 32; k++) base[k * cstride] = lo[k] | (hi[k] << 16);
+#endif
 }
 
 // Leaf hash of one 32-column codeword row: BLAKE2s(prefix || row bytes), with
@@ -272,7 +289,12 @@ __global__ __launch_bounds__(WG) void k_scatter_message(const E8* __restrict__ v
 // One-row messages: the WHT of length one is the identity, so parity = alpha_1[0] * m.
 __global__ void k_encode_one_row(E8* cw, uint64_t cols, const E8* alpha) {
   uint64_t c = uint64_t(blockIdx.x) * WG + threadIdx.x;
-  if (c < cols) cw[2 * c + 1] = v8_store(v8_mul(v8_load(cw[2 * c]), v8_load(alpha[0])));
+  if (c < cols) cw[2 * c + 1] = v8_store(v8_mul_enc(v8_load(cw[2 * c]), v8_load(alpha[0])));
+}
+
+__global__ void k_encode_form(const E8* in, E8* out, uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
+    out[i] = v8_store(v8_encode_form(v8_load(in[i])));
 }
 
 // High index bits [12, L) in groups of at most 10 (tile passes with C = 4096 >> t contiguous positions).
@@ -423,9 +445,17 @@ void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, c
   launch_tree(cw, R, leaf, inner, tree, st);
 }
 
-void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_first, uint64_t available, E8* cw,
+void launch_encode_form(const E8* in, E8* out, uint64_t n, hipStream_t st) {
+  const uint64_t g = std::min<uint64_t>((n + WG - 1) / WG, 65536);
+  hipLaunchKernelGGL(k_encode_form, dim3(uint32_t(std::max<uint64_t>(g, 1))), dim3(WG), 0, st, in, out, n);
+}
+
+void launch_commit_block(const CommitGeom& g, const Val* values, uint64_t first_entry, uint64_t active, E8* cw,
                          const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev) {
   const int L = g.log_n;
+  // the packing kernels address the BVA in bytes
+  const uint8_t* bva = reinterpret_cast<const uint8_t*>(values);
+  const uint64_t block_first = first_entry * sizeof(Val), available = active * sizeof(Val);
   if (ev) QF_HIP(hipEventRecord(ev[0], st));
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
   if (!tiled_commit_rows(true, bva, block_first, available, cw, L, alpha, st)) {

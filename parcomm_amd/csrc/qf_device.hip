@@ -21,9 +21,11 @@ __device__ inline uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// Entry i of the BVA: Box-Muller normal from two 32-bit uniforms of mix64(seed ^ i),
-// scaled, rounded half away from zero, clipped to int8.
-__global__ void k_synthetic(int8_t* out, uint64_t n, uint64_t seed, float sigma) {
+// Entry i of the BVA: a Box-Muller normal z from two uniforms of
+// mix64(seed, i), scaled by sigma. F65521 entries are round(z sigma) clipped
+// to int8; KoalaBear entries are the bfloat16 bit pattern of z sigma (round to
+// nearest even), the raw 16-bit payload of a BF16 weight.
+__global__ void k_synthetic(Val* out, uint64_t n, uint64_t seed, float sigma) {
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
   for (; i < n; i += stride) {
@@ -31,14 +33,20 @@ __global__ void k_synthetic(int8_t* out, uint64_t n, uint64_t seed, float sigma)
     float u1 = (float(uint32_t(r >> 40)) + 0.5f) * (1.0f / 16777216.0f);
     float u2 = float(uint32_t(r) & 0xffffffu) * (1.0f / 16777216.0f);
     float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+#ifdef QF_KB
+    const uint32_t b = __float_as_uint(z * sigma);
+    const uint32_t rb = b + 0x7fffu + ((b >> 16) & 1u);  // finite inputs only
+    out[i] = Val(uint16_t(rb >> 16));
+#else
     float v = rintf(z * sigma);
     v = fminf(fmaxf(v, -128.0f), 127.0f);
-    out[i] = int8_t(int(v));
+    out[i] = Val(int(v));
+#endif
   }
 }
 
 // Zero a rows x width rectangle of a row-major bucket (padding region).
-__global__ void k_zero_rect(int8_t* base, uint64_t rows, uint64_t stride, uint64_t width) {
+__global__ void k_zero_rect(Val* base, uint64_t rows, uint64_t stride, uint64_t width) {
   uint64_t total = rows * width;
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += uint64_t(gridDim.x) * blockDim.x)
     base[(i / width) * stride + (i % width)] = 0;
@@ -75,10 +83,10 @@ int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* devic
   return guard([&] {
     require(p && device_out, QF_ERR_ARGUMENT, "null argument");
     const Layout& l = params_layout(p);
-    int8_t* out = static_cast<int8_t*>(device_out);
+    Val* out = static_cast<Val*>(device_out);
     hipLaunchKernelGGL(k_synthetic, dim3(65536), dim3(256), 0, 0, out, l.active, seed, sigma);
     for (const Bucket& b : l.buckets) {
-      int8_t* base = out + b.offset;
+      Val* base = out + b.offset;
       if (b.physical_cols < b.cols)
         hipLaunchKernelGGL(k_zero_rect, dim3(1024), dim3(256), 0, 0, base + b.physical_cols, b.physical_rows, b.cols,
                            b.cols - b.physical_cols);

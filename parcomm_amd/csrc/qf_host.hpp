@@ -1,5 +1,6 @@
 // Host-side helpers of the product library: streaming BLAKE2s with the
-// reference's frame encoding, Ext8 host arithmetic and little-endian writers.
+// reference's frame encoding, extension-field host arithmetic and
+// little-endian writers.
 #pragma once
 #include "qf_prims.hpp"
 #include "../../include/parcomm_amd.h"
@@ -12,6 +13,23 @@ namespace qf {
 
 using Bytes = std::vector<uint8_t>;
 using Digest = std::array<uint8_t, 32>;
+
+// Field-specific names and domain separators of the reference crates.
+#ifdef QF_KB
+constexpr int QF_FIELD = QF_FIELD_KOALABEAR;
+constexpr const char* FIELD_NAME = "FullKoalaBearExt4/X^4-3/scalar/v1";                // koalabear.rs:195
+constexpr const char* OUTER_DOMAIN = "parcomm-koalabear-i16/public/v1";                // transcript.rs:15
+constexpr const char* TERMINAL_DOMAIN = "parcomm/koalabear-i16/terminal/v2";           // protocol.rs:330
+constexpr const char* PACKED_DOMAIN = "mikros/qafold-koalabear-ext4/packed-boolean-mle/v1";  // packed_koalabear.rs:478
+constexpr const char* PACKED_MAGIC = "QAKOAPK1";                                       // packed_koalabear.rs:25
+#else
+constexpr int QF_FIELD = QF_FIELD_F65521;
+constexpr const char* FIELD_NAME = "FullPrime65521Ext8/X^8-17/v1";                     // full_extension.rs:130
+constexpr const char* OUTER_DOMAIN = "parcomm-qafold/public/v1";                       // transcript.rs:15
+constexpr const char* TERMINAL_DOMAIN = "parcomm/qafold/terminal/v2";                  // protocol.rs:323
+constexpr const char* PACKED_DOMAIN = "mikros/qafold-f65521-ext8/packed-boolean-mle/v1";  // packed.rs:473
+constexpr const char* PACKED_MAGIC = "QA655PK1";                                       // packed.rs:19
+#endif
 
 // Status codes: the reference engine's error kinds (vendor/qafold/src/lib.rs:20-35)
 // plus device and argument failures, as declared in the public header.
@@ -56,8 +74,8 @@ struct Hasher {
 };
 inline Digest blake2s(const void* p, size_t n) { Hasher h; h.update(p, n); return h.finish(); }
 
-// Host Ext8 arithmetic on the canonical 16-byte form.
-inline E8 e8(uint32_t c0) { E8 e{}; e.c[0] = uint16_t(c0 % P); return e; }
+// Host extension arithmetic on the canonical 16-byte form.
+inline E8 e8(uint32_t c0) { E8 e{}; e.c[0] = Coef(c0 % P); return e; }
 inline E8 e8_zero() { return E8{}; }
 inline E8 e8_one() { return e8(1); }
 inline bool e8_eq(const E8& a, const E8& b) { return std::memcmp(&a, &b, 16) == 0; }
@@ -69,18 +87,19 @@ inline E8 e8_neg(const E8& a) { return e8_sub(E8{}, a); }
 inline uint32_t fp_mul(uint32_t a, uint32_t b) { return uint32_t((uint64_t(a) * b) % P); }
 inline uint32_t fp_pow(uint32_t a, uint64_t e) { uint32_t r = 1; while (e) { if (e & 1) r = fp_mul(r, a); a = fp_mul(a, a); e >>= 1; } return r; }
 inline uint32_t fp_inv(uint32_t a) { require(a != 0, QF_ERR_ENCODING, "zero inverse"); return fp_pow(a, P - 2); }
-inline E8 e8_scale(const E8& a, uint32_t s) { E8 r; for (int i = 0; i < 8; i++) r.c[i] = uint16_t(fp_mul(a.c[i], s)); return r; }
-// x -> x^(p^k): coefficient i times 57852^(i k mod 8) (qafold_short.rs:486-525)
+inline E8 e8_scale(const E8& a, uint32_t s) { E8 r; for (int i = 0; i < LANES; i++) r.c[i] = Coef(fp_mul(a.c[i], s)); return r; }
+// x -> x^(p^k): coefficient i times FROB_ROOT^(i k mod LANES) (qafold_short.rs:486-525,
+// koalabear.rs:153-163)
 inline E8 e8_frob(const E8& a, int k) {
-  k = ((k % 8) + 8) % 8;
+  k = ((k % LANES) + LANES) % LANES;
   E8 r;
-  for (int i = 0; i < 8; i++) r.c[i] = uint16_t(fp_mul(a.c[i], fp_pow(57852, uint64_t((i * k) % 8))));
+  for (int i = 0; i < LANES; i++) r.c[i] = Coef(fp_mul(a.c[i], fp_pow(FROB_ROOT, uint64_t((i * k) % LANES))));
   return r;
 }
-inline E8 e8_inv(const E8& a) {
+inline E8 e8_inv(const E8& a) {  // (product of the other conjugates) / norm
   require(!e8_is_zero(a), QF_ERR_ENCODING, "zero inverse");
   E8 prod = e8_frob(a, 1);
-  for (int k = 2; k < 8; k++) prod = e8_frob(e8_mul(prod, a), 1);
+  for (int k = 2; k < LANES; k++) prod = e8_frob(e8_mul(prod, a), 1);
   E8 norm = e8_mul(a, prod);
   return e8_scale(prod, fp_inv(norm.c[0]));
 }
@@ -92,9 +111,27 @@ inline E8& operator+=(E8& a, const E8& b) { a = e8_add(a, b); return a; }
 inline bool operator==(const E8& a, const E8& b) { return e8_eq(a, b); }
 inline bool operator!=(const E8& a, const E8& b) { return !e8_eq(a, b); }
 inline E8 e8_basis(int j) { E8 b{}; b.c[j] = 1; return b; }
+// Canonical 16-byte encoding; rejects coefficients >= p.
 inline bool e8_from_bytes(const uint8_t* b, E8& out) {
-  for (int i = 0; i < 8; i++) { uint32_t v = b[2 * i] | uint32_t(b[2 * i + 1]) << 8; if (v >= P) return false; out.c[i] = uint16_t(v); }
+  for (int i = 0; i < LANES; i++) {
+    uint32_t v = 0;
+    for (size_t k = 0; k < sizeof(Coef); k++) v |= uint32_t(b[sizeof(Coef) * i + k]) << (8 * k);
+    if (v >= P) return false;
+    out.c[i] = Coef(v);
+  }
   return true;
+}
+// from_uniform_bytes of the extension: F65521 reads the canonical form;
+// KoalaBear first clears bit 7 of bytes 3, 7, 11 and 15 (koalabear.rs:248-255).
+inline bool e8_from_uniform(const uint8_t* b, E8& out) {
+#ifdef QF_KB
+  uint8_t t[16];
+  std::memcpy(t, b, 16);
+  for (int i : {3, 7, 11, 15}) t[i] &= 0x7f;
+  return e8_from_bytes(t, out);
+#else
+  return e8_from_bytes(b, out);
+#endif
 }
 inline void put_u64(Bytes& o, uint64_t v) { for (int i = 0; i < 8; i++) o.push_back(uint8_t(v >> (8 * i))); }
 inline void put_e8(Bytes& o, const E8& v) { const uint8_t* p = reinterpret_cast<const uint8_t*>(&v); o.insert(o.end(), p, p + 16); }

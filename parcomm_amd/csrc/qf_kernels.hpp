@@ -28,11 +28,15 @@ struct CommitGeom {
   InnerPrefix inner;
 };
 
-// Commits one PCS block: pack + Mobius into the message half of `cw`,
-// encode the parity half, hash the rows and build all Merkle levels in `tree`
-// (2R - 1 digests, leaves first). Everything is enqueued on `st`.
-void launch_commit_block(const CommitGeom& g, const int8_t* bva, uint64_t block_first, uint64_t available, E8* cw,
+// Commits one PCS block (entries [first_entry, first_entry + 2^c) of the BVA;
+// entries at or past `active` are zero): pack + Mobius into the message half
+// of `cw`, encode the parity half, hash the rows and build all Merkle levels
+// in `tree` (2R - 1 digests, leaves first). Everything is enqueued on `st`.
+// Every `alpha` below is alpha_1 in its encode form (v8_encode_form).
+void launch_commit_block(const CommitGeom& g, const Val* bva, uint64_t first_entry, uint64_t active, E8* cw,
                          const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev = nullptr);
+// out[i] = encode form of in[i] (Montgomery form for KoalaBear, a copy for F65521).
+void launch_encode_form(const E8* in, E8* out, uint64_t n, hipStream_t st);
 // ev (optional, 4 events): recorded at start, before the leaf kernel, after it, and at the end.
 
 // Building blocks shared with the opening (all asynchronous on `st`):

@@ -6,7 +6,6 @@
 
 namespace qf {
 
-static const char* FIELD_NAME = "FullPrime65521Ext8/X^8-17/v1";  // full_extension.rs:130
 
 void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)>& body) {
   unsigned hw = std::thread::hardware_concurrency();
@@ -104,7 +103,7 @@ static E8 generator_element(const Hasher& prefix, uint64_t row) {
     h.frame_u64("attempt", attempt);
     Digest d = h.finish();
     E8 v;
-    if (e8_from_bytes(d.data(), v) || e8_from_bytes(d.data() + 16, v)) return v;
+    if (e8_from_uniform(d.data(), v) || e8_from_uniform(d.data() + 16, v)) return v;
   }
 }
 
@@ -180,7 +179,7 @@ std::vector<std::vector<Digest>> merkle_levels(std::vector<Digest> leaves) {
 PcsParams make_pcs_params(int log_n, uint64_t security_bits, const uint8_t seed32[32]) {
   require(log_n >= 4 && log_n <= 30, QF_ERR_INVALID_VARIABLES, "packed log_n out of range");
   PcsParams pp;
-  pp.n_witness = 1ull << (log_n - 3);
+  pp.n_witness = 1ull << (log_n - LOG_LANES);
   pp.security_bits = security_bits;
   require(pp.n_witness % pp.ext_cols == 0, QF_ERR_ENCODING, "witness not divisible by columns");
   // StdRng::from_seed(seed).fill_bytes(32): the first 32 bytes of the ChaCha12

@@ -1,4 +1,4 @@
-// Public parameters of the F65521/Ext8 QAFold prover: BVA layout, QA code
+// Public parameters of the QAFold prover: BVA layout, QA code
 // sequence, setup commitments and the parameter digest. Computed once per
 // context on the host (setup time is outside the prover's timed phases, as in
 // the reference's Parameters::new, protocol.rs:109-130).

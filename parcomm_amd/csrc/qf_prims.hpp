@@ -1,12 +1,21 @@
 // Field and hash primitives shared by the HIP kernels and the host protocol code.
 //
-// F65521 = Z/(2^16 - 15) (qafold_short.rs:325-477); its octic extension
-// F65521[X]/(X^8 - 17) (qafold_short.rs:482-660) is stored as eight canonical
-// u16 coefficients (16 bytes, coefficient i = X^i), which is also the
-// canonical wire encoding (full_extension.rs:157-164).
+// The library is built once per field backend (the reference ships one crate
+// per field):
+//   default  F65521 = Z/(2^16 - 15) (qafold_short.rs:325-477) with the octic
+//            extension F65521[X]/(X^8 - 17) (qafold_short.rs:482-660): eight
+//            canonical u16 coefficients; BVA entries are i8.
+//   QF_KB    KoalaBear = Z/(2^31 - 2^24 + 1) with the quartic extension
+//            KoalaBear[X]/(X^4 - 3) (koalabear.rs:1-2, :131-145): four
+//            canonical u32 coefficients; BVA entries are i16.
+// Either way one extension element is 16 bytes, coefficient i = X^i, which is
+// also its canonical wire encoding (full_extension.rs:157-164,
+// koalabear.rs:229-246). The type keeps the name E8 in both builds.
 //
-// Device code keeps coefficients unpacked in 32-bit lanes and reduces with the
+// Device code keeps coefficients unpacked in 32-bit lanes. Sums use the
 // unsigned-min trick: for s < 2P, min(s, s - P) (wrapping) is s mod P.
+// F65521 products reduce with 2^16 = 15 (mod P); KoalaBear products use
+// Montgomery reduction with R = 2^32.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -14,11 +23,26 @@
 
 namespace qf {
 
+#ifdef QF_KB
+constexpr uint32_t P = 2130706433;
+constexpr int LANES = 4, LOG_LANES = 2;  // extension degree = BVA entries per packed element
+using Coef = uint32_t;
+using Val = int16_t;
+constexpr uint32_t EXT_W = 3;             // X^4 = 3
+constexpr uint32_t FROB_ROOT = 2113994754;  // x^p scales coefficient i by FROB_ROOT^i (koalabear.rs:155)
+#else
 constexpr uint32_t P = 65521;
+constexpr int LANES = 8, LOG_LANES = 3;
+using Coef = uint16_t;
+using Val = int8_t;
+constexpr uint32_t EXT_W = 17;            // X^8 = 17
+constexpr uint32_t FROB_ROOT = 57852;     // 17^((p-1)/8) (qafold_short.rs:486-525)
+#endif
 
 struct alignas(16) E8 {
-  uint16_t c[8];
+  Coef c[LANES];
 };
+static_assert(sizeof(E8) == 16 && LANES * sizeof(Val) == 8, "16-byte elements packed from 8 BVA bytes");
 
 __host__ __device__ inline uint32_t madd(uint32_t a, uint32_t b) {
   uint32_t s = a + b;
@@ -30,6 +54,34 @@ __host__ __device__ inline uint32_t msub(uint32_t a, uint32_t b) {
   uint32_t e = d + P;
   return d < e ? d : e;
 }
+// Canonical residue of a signed BVA entry.
+__host__ __device__ inline uint32_t from_val(int32_t v) { return uint32_t(v < 0 ? v + int32_t(P) : v); }
+
+#ifdef QF_KB
+// Montgomery constants: PINV = -P^-1 mod 2^32, R2 = 2^64 mod P, R32 = 2^32 mod P.
+constexpr uint32_t MONT_PINV = 2130706431u;
+constexpr uint32_t MONT_R2 = 402124772u;
+constexpr uint32_t R32 = 33554430u;
+// x < P * 2^32 -> x * 2^-32 mod P (canonical).
+__host__ __device__ inline uint32_t redc(uint64_t x) {
+  const uint32_t m = uint32_t(x) * MONT_PINV;
+  const uint64_t t = (x + uint64_t(m) * P) >> 32;  // < 2P
+  const uint32_t r = uint32_t(t);
+  const uint32_t w = r - P;
+  return r < w ? r : w;
+}
+// a -> a * 2^32 mod P (Montgomery form of a canonical a)
+__host__ __device__ inline uint32_t to_mont(uint32_t a) { return redc(uint64_t(a) * MONT_R2); }
+// any 64-bit x -> x mod P
+__host__ __device__ inline uint32_t mred64(uint64_t x) {
+  const uint64_t y = (x >> 32) * R32 + (x & 0xffffffffu);  // < 2^57 + 2^32 < P * 2^32
+  return redc(uint64_t(redc(y)) * MONT_R2);
+}
+__host__ __device__ inline uint32_t mred32(uint32_t x) {
+  x = x >= 2 * P ? x - 2 * P : x;
+  return x >= P ? x - P : x;
+}
+#else
 // x < 2^32 -> x mod P (2^16 = 15 mod P)
 __host__ __device__ inline uint32_t mred32(uint32_t x) {
   uint32_t y = (x & 0xffffu) + 15u * (x >> 16);  // < 2^20
@@ -37,39 +89,79 @@ __host__ __device__ inline uint32_t mred32(uint32_t x) {
   uint32_t t = y - P;
   return y < t ? y : t;
 }
-__host__ __device__ inline uint32_t mred64(uint64_t x) {
-  uint64_t y = (x & 0xffffu) + 15u * (x >> 16);   // < 2^44 for x < 2^64? x < 2^40 here
+__host__ __device__ inline uint32_t mred64(uint64_t x) {  // x < 2^48
+  uint64_t y = (x & 0xffffu) + 15u * (x >> 16);   // < 2^36
   y = (y & 0xffffu) + 15u * (y >> 16);
   return mred32(uint32_t(y));
 }
+#endif
 
-// Unpacked register form of one Ext8 element.
+// Unpacked register form of one extension element.
 struct V8 {
-  uint32_t c[8];
+  uint32_t c[LANES];
 };
 __host__ __device__ inline V8 v8_load(const E8& e) {
   V8 r;
+#ifdef QF_KB
+#pragma unroll
+  for (int i = 0; i < 4; i++) r.c[i] = e.c[i];
+#else
   const uint32_t* w = reinterpret_cast<const uint32_t*>(&e);
 #pragma unroll
   for (int i = 0; i < 4; i++) { r.c[2 * i] = w[i] & 0xffffu; r.c[2 * i + 1] = w[i] >> 16; }
+#endif
   return r;
 }
 __host__ __device__ inline E8 v8_store(const V8& v) {
   E8 e;
+#ifdef QF_KB
+#pragma unroll
+  for (int i = 0; i < 4; i++) e.c[i] = v.c[i];
+#else
   uint32_t* w = reinterpret_cast<uint32_t*>(&e);
 #pragma unroll
   for (int i = 0; i < 4; i++) w[i] = v.c[2 * i] | (v.c[2 * i + 1] << 16);
+#endif
   return e;
 }
 __host__ __device__ inline V8 v8_add(const V8& a, const V8& b) { V8 r;
 #pragma unroll
-  for (int i = 0; i < 8; i++) r.c[i] = madd(a.c[i], b.c[i]); return r; }
+  for (int i = 0; i < LANES; i++) r.c[i] = madd(a.c[i], b.c[i]); return r; }
 __host__ __device__ inline V8 v8_sub(const V8& a, const V8& b) { V8 r;
 #pragma unroll
-  for (int i = 0; i < 8; i++) r.c[i] = msub(a.c[i], b.c[i]); return r; }
+  for (int i = 0; i < LANES; i++) r.c[i] = msub(a.c[i], b.c[i]); return r; }
 __host__ __device__ inline V8 v8_zero() { V8 r;
 #pragma unroll
-  for (int i = 0; i < 8; i++) r.c[i] = 0; return r; }
+  for (int i = 0; i < LANES; i++) r.c[i] = 0; return r; }
+#ifdef QF_KB
+// Product in KoalaBear[X]/(X^4 - 3) with b given in Montgomery form: the
+// X^4 = 3 wrap is folded into the sequence E = (3 b_1, 3 b_2, 3 b_3, b_0, ..,
+// b_3), so coefficient i = sum_j a_j E[3 + i - j]; each pair of products
+// (< 2 P^2 < P 2^32) takes one Montgomery reduction.
+__host__ __device__ inline V8 v8_mul_mont(const V8& a, const V8& bm) {
+  uint32_t E[7];
+#pragma unroll
+  for (int j = 0; j < 4; j++) E[3 + j] = bm.c[j];
+#pragma unroll
+  for (int j = 1; j < 4; j++) { const uint32_t d = madd(bm.c[j], bm.c[j]); E[j - 1] = madd(d, bm.c[j]); }
+  V8 r;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint64_t s0 = uint64_t(a.c[0]) * E[3 + i] + uint64_t(a.c[1]) * E[2 + i];
+    const uint64_t s1 = uint64_t(a.c[2]) * E[1 + i] + uint64_t(a.c[3]) * E[i];
+    r.c[i] = madd(redc(s0), redc(s1));
+  }
+  return r;
+}
+__host__ __device__ inline V8 v8_to_mont(const V8& b) { V8 r;
+#pragma unroll
+  for (int i = 0; i < 4; i++) r.c[i] = to_mont(b.c[i]); return r; }
+__host__ __device__ inline V8 v8_mul(const V8& a, const V8& b) { return v8_mul_mont(a, v8_to_mont(b)); }
+// The QA generator alpha_1 is kept on the device in an "encode form" for the
+// encoding kernels: Montgomery form here, canonical for F65521.
+__host__ __device__ inline V8 v8_encode_form(const V8& b) { return v8_to_mont(b); }
+__host__ __device__ inline V8 v8_mul_enc(const V8& a, const V8& b_enc) { return v8_mul_mont(a, b_enc); }
+#else
 // Product in F65521[X]/(X^8 - 17); 64-bit accumulation, one reduction per coefficient.
 __host__ __device__ inline V8 v8_mul(const V8& a, const V8& b) {
   uint64_t t[15];
@@ -84,6 +176,9 @@ __host__ __device__ inline V8 v8_mul(const V8& a, const V8& b) {
   for (int i = 0; i < 8; i++) r.c[i] = mred64(t[i] + (i < 7 ? 17ull * t[i + 8] : 0ull));
   return r;
 }
+__host__ __device__ inline V8 v8_encode_form(const V8& b) { return b; }
+__host__ __device__ inline V8 v8_mul_enc(const V8& a, const V8& b_enc) { return v8_mul(a, b_enc); }
+#endif
 
 // ---------------------------------------------------------------- BLAKE2s-256 (RFC 7693)
 __host__ __device__ inline uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }

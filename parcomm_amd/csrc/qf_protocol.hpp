@@ -28,7 +28,7 @@ inline std::string js_scalars(const std::vector<E8>& v) {
 struct Outer {  // transcript.rs:8-70
   Digest state;
   uint64_t counter = 0;
-  Outer() { state = blake2s("parcomm-qafold/public/v1", 24); }
+  Outer() { state = blake2s(OUTER_DOMAIN, std::strlen(OUTER_DOMAIN)); }
   void bytes(const char* label, const void* d, size_t n) {
     Hasher h;
     h.update(state.data(), 32);
@@ -47,7 +47,7 @@ struct Outer {  // transcript.rs:8-70
       for (int i = 0; i < 8; i++) b[i] = uint8_t(ct >> (8 * i));
       bytes(label, b, 8);
       E8 v;
-      if (e8_from_bytes(state.data(), v)) return v;
+      if (e8_from_uniform(state.data(), v)) return v;
     }
   }
   E8 nonzero(const char* label) { for (;;) { E8 v = challenge(label); if (!e8_is_zero(v)) return v; } }
@@ -97,7 +97,7 @@ struct Pcs {  // qafold_short_pc.rs:2011-2229
     uint64_t ctr = counter;
     for (uint64_t outer = 0;; outer++) {
       E8 v;
-      for (uint64_t a = outer;; a++) if (e8_from_bytes(derive(label, ctr, a).data(), v)) break;
+      for (uint64_t a = outer;; a++) if (e8_from_uniform(derive(label, ctr, a).data(), v)) break;
       if (!nonzero || !e8_is_zero(v)) {
         counter++;
         bytes("field-challenge-label", label);
@@ -119,7 +119,7 @@ struct Packed {  // packed.rs:754-779
       c.u64le(ctr);
       Digest d = c.finish();
       E8 v;
-      if (e8_from_bytes(d.data(), v)) { bytes(d.data(), 32); return v; }
+      if (e8_from_uniform(d.data(), v)) { bytes(d.data(), 32); return v; }
     }
   }
 };
@@ -203,7 +203,7 @@ inline bool bound_at(uint64_t n, uint64_t index, double d, double ln_p, double& 
 }
 inline uint64_t num_queries(const QaCode& code, uint64_t bits, uint64_t rows) {
   unsigned __int128 card = 1;
-  for (int i = 0; i < 8; i++) card *= 65521;
+  for (int i = 0; i < LANES; i++) card *= P;
   double ln_p = std::log2(double(card)) * M_LN2;
   double upper = 1.0 - 1.0 / double(code.index), target = -double(bits) * M_LN2;
   auto valid = [&](double d) { double lf; return bound_at(code.message_len, code.index, d, ln_p, lf) && lf <= target; };
@@ -383,20 +383,23 @@ inline std::vector<E8> tbv_w(const std::array<E8, 2>* f, size_t n) {
   }
   return r;
 }
-inline E8 small_byte(int v) { return e8(uint32_t(v < 0 ? v + int(P) : v)); }
+inline E8 small_val(int v) { return e8(from_val(v)); }
 
-inline E8 projected_claim(const std::array<E8, 8>& ev, const std::vector<E8>& low) {  // packed.rs:502-523
+// One extension element per packing lane (the conjugate evaluations, weight images, ...).
+using Lanes = std::array<E8, LANES>;
+// packed.rs:502-523, packed_koalabear.rs:507-528: inverse DFT of the conjugates.
+inline E8 projected_claim(const Lanes& ev, const std::vector<E8>& low) {
   E8 x1 = e8_inv(e8_basis(1));
-  std::array<E8, 8> ib;
+  Lanes ib;
   ib[0] = e8_one();
-  for (int i = 1; i < 8; i++) ib[i] = ib[i - 1] * x1;
-  uint32_t omega_inv = fp_inv(57852), inv8 = fp_inv(8), step = 1;
+  for (int i = 1; i < LANES; i++) ib[i] = ib[i - 1] * x1;
+  uint32_t omega_inv = fp_inv(FROB_ROOT), inv_lanes = fp_inv(LANES), step = 1;
   E8 claim{};
-  for (int j = 0; j < 8; j++) {
+  for (int j = 0; j < LANES; j++) {
     uint32_t phase = 1;
     E8 sum{};
-    for (int i = 0; i < 8; i++) { sum += e8_scale(e8_frob(ev[i], i), phase); phase = fp_mul(phase, step); }
-    claim += low[j] * e8_scale(sum * ib[j], inv8);
+    for (int i = 0; i < LANES; i++) { sum += e8_scale(e8_frob(ev[i], i), phase); phase = fp_mul(phase, step); }
+    claim += low[j] * e8_scale(sum * ib[j], inv_lanes);
     step = fp_mul(step, omega_inv);
   }
   return claim;
@@ -404,7 +407,7 @@ inline E8 projected_claim(const std::array<E8, 8>& ev, const std::vector<E8>& lo
 inline Packed packed_transcript(const ParamsState& p, const Digest& root, const std::vector<E8>& point, const E8& value,
                          const Bytes& context) {
   Packed t;
-  const char* dom = "mikros/qafold-f65521-ext8/packed-boolean-mle/v1";
+  const char* dom = PACKED_DOMAIN;
   t.bytes(dom, std::strlen(dom));
   t.bytes(p.pp.parameter_digest.data(), 32);
   uint8_t ln[8];
@@ -416,13 +419,13 @@ inline Packed packed_transcript(const ParamsState& p, const Digest& root, const 
   t.field(value);
   return t;
 }
-inline std::array<E8, 8> powers8(const E8& v) { std::array<E8, 8> o; o[0] = e8_one(); for (int i = 1; i < 8; i++) o[i] = o[i - 1] * v; return o; }
-inline E8 batch_weight(const E8* point, size_t n, const std::vector<E8>& red, const std::array<E8, 8>& powers) {
+inline Lanes lane_powers(const E8& v) { Lanes o; o[0] = e8_one(); for (int i = 1; i < LANES; i++) o[i] = o[i - 1] * v; return o; }
+inline E8 batch_weight(const E8* point, size_t n, const std::vector<E8>& red, const Lanes& powers) {
   E8 sum{};
-  for (int i = 0; i < 8; i++) {
+  for (int i = 0; i < LANES; i++) {
     E8 w = e8_one();
     for (size_t k = 0; k < n; k++) {
-      E8 s = e8_frob(point[k], (8 - i) % 8);
+      E8 s = e8_frob(point[k], (LANES - i) % LANES);
       w = w * ((e8_one() - s) * (e8_one() - red[k]) + s * red[k]);
     }
     sum += powers[i] * w;

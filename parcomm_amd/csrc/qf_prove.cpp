@@ -23,6 +23,21 @@ const E8* ParamsState::alpha_device(size_t level) const {
   }
   return d_alpha_levels[level];
 }
+const E8* ParamsState::alpha_encode(size_t level) const {
+  const E8* a = alpha_device(level);
+  if (LANES == 8) return a;  // F65521 encodes with the canonical form
+  std::lock_guard<std::mutex> g(mu);
+  if (d_alpha_enc.size() != pp.codes.size()) d_alpha_enc.assign(pp.codes.size(), nullptr);
+  if (!d_alpha_enc[level]) {
+    const uint64_t n = pp.codes[level].alpha1.size();
+    E8* d = nullptr;
+    QF_HIP(hipMalloc(&d, n * 16));
+    launch_encode_form(a, d, n, nullptr);
+    QF_HIP(hipDeviceSynchronize());
+    d_alpha_enc[level] = d;
+  }
+  return d_alpha_enc[level];
+}
 
 namespace {
 
@@ -94,7 +109,7 @@ DCode alloc_code(uint64_t R) {
 DCode commit_extension(DevCtx& c, const ParamsState& p, size_t level, const E8* values) {
   const QaCode& code = p.pp.codes[level];
   DCode d = alloc_code(code.codeword_len());
-  launch_commit_extension(values, d.cw.p, ilog2u(code.message_len), p.alpha_device(level), p.leaf_ext, p.geom.inner,
+  launch_commit_extension(values, d.cw.p, ilog2u(code.message_len), p.alpha_encode(level), p.leaf_ext, p.geom.inner,
                           d.tree, c.st);
   QF_HIP(hipGetLastError());
   d.root = download_root(c, d.tree, d.R);
@@ -118,7 +133,7 @@ DVec generator_right(DevCtx& c, const ParamsState& p, size_t level, const DVec& 
   uint64_t n = code.message_len;
   DVec out(2 * n);
   dv::copy(c, out.p, right.p, n);
-  launch_encode_parity(out.p, 1, ilog2u(n), p.alpha_device(level), c.st);
+  launch_encode_parity(out.p, 1, ilog2u(n), p.alpha_encode(level), c.st);
   QF_HIP(hipGetLastError());
   return out;
 }
@@ -558,17 +573,17 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
   return w;
 }
 
-std::vector<int8_t> read_bytes(DevCtx& c, const int8_t* bva, uint64_t active, uint64_t start, uint64_t n) {
-  std::vector<int8_t> out(n, 0);
+std::vector<Val> read_values(DevCtx& c, const Val* bva, uint64_t active, uint64_t start, uint64_t n) {
+  std::vector<Val> out(n, 0);
   if (start < active) {
     uint64_t m = std::min(n, active - start);
-    QF_HIP(hipMemcpyAsync(out.data(), bva + start, m, hipMemcpyDeviceToHost, c.st));
+    QF_HIP(hipMemcpyAsync(out.data(), bva + start, m * sizeof(Val), hipMemcpyDeviceToHost, c.st));
     c.sync();
   }
   return out;
 }
 // Lane projections of the terms with len >= s on the device (tbv.rs:288-352).
-std::vector<std::vector<E8>> project_terms(DevCtx& c, const Query& q, const int8_t* bva, uint64_t active, uint64_t s,
+std::vector<std::vector<E8>> project_terms(DevCtx& c, const Query& q, const Val* bva, uint64_t active, uint64_t s,
                                            std::vector<size_t>& which) {
   std::vector<ProjTask> tasks;
   std::vector<std::array<E8, 2>> pairs;
@@ -594,7 +609,7 @@ std::vector<std::vector<E8>> project_terms(DevCtx& c, const Query& q, const int8
 
 // ================================================================ EO (eo.rs:97-181)
 struct EoOut { std::vector<E8> pos, neg; std::vector<std::array<E8, 2>> rounds; E8 value, query; std::vector<E8> point; E8 scale; };
-EoOut eo_prove(DevCtx& c, const Layout& l, const int8_t* bva, uint64_t s, const Query& q, Outer& tr) {
+EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Query& q, Outer& tr) {
   EoOut o;
   o.pos.assign(s - 1, E8{});
   o.neg.assign(s - 1, E8{});
@@ -613,13 +628,13 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const int8_t* bva, uint64_t s, const 
   for (auto& t : q.terms) {
     if (t.len() >= s) continue;
     uint64_t start = t.offset / s * s;
-    auto vals = read_bytes(c, bva, l.active, start, s);
+    auto vals = read_values(c, bva, l.active, start, s);
     std::vector<E8> query(s, E8{});
     for (uint64_t i = 0; i < t.len(); i++) query[t.offset - start + i] = t.weight(i);
     for (uint64_t d = 1; d < s; d++)
       for (uint64_t lane = 0; lane + d < s; lane++) {
-        o.pos[d - 1] += query[lane + d] * small_byte(vals[lane]);
-        o.neg[d - 1] += query[lane] * small_byte(vals[lane + d]);
+        o.pos[d - 1] += query[lane + d] * small_val(vals[lane]);
+        o.neg[d - 1] += query[lane] * small_val(vals[lane + d]);
       }
   }
   tr.object("packed-positive", js_scalars(o.pos));
@@ -716,25 +731,25 @@ void sumcheck(DevCtx& c, DVec& v, DVec& w, E8& claim, Packed& t, std::vector<std
   fval = dv::download(c, v.p, 1)[0];
   fwt = dv::download(c, w.p, 1)[0];
 }
-// Lanes of every block against eq(outer): lanes[b][j] = sum_i byte(b, 8i + j) * eq_i
-std::vector<std::array<E8, 8>> block_lanes(DevCtx& c, const ParamsState& p, const int8_t* bva, uint64_t active,
+// Lanes of every block against eq(outer): lanes[b][j] = sum_i entry(b, LANES i + j) * eq_i
+std::vector<Lanes> block_lanes(DevCtx& c, const ParamsState& p, const Val* bva, uint64_t active,
                                            uint64_t blocks, const std::vector<E8>& outer) {
   std::vector<std::array<E8, 2>> pairs;
   size_t m = outer.size();
   for (size_t b = 0; b < m; b++) { const E8& pc = outer[m - 1 - b]; pairs.push_back({e8_one() - pc, pc}); }
   std::vector<ProjTask> tasks;
   for (uint64_t b = 0; b < blocks; b++) tasks.push_back({b * p.block_len, uint64_t(1) << m, uint32_t(m), 0, e8_one()});
-  auto flat = tensor_project(c, bva, active, 8, tasks, pairs);
-  std::vector<std::array<E8, 8>> out(blocks);
+  auto flat = tensor_project(c, bva, active, LANES, tasks, pairs);
+  std::vector<Lanes> out(blocks);
   for (uint64_t b = 0; b < blocks; b++)
-    for (int j = 0; j < 8; j++) out[b][j] = flat[b * 8 + j];
+    for (int j = 0; j < LANES; j++) out[b][j] = flat[b * LANES + j];
   return out;
 }
-std::array<E8, 8> conjugates(const std::array<E8, 8>& lanes) {
-  std::array<E8, 8> e;
-  for (int img = 0; img < 8; img++) {
+Lanes conjugates(const Lanes& lanes) {
+  Lanes e;
+  for (int img = 0; img < LANES; img++) {
     E8 s{};
-    for (int j = 0; j < 8; j++) s += e8_basis(j) * e8_frob(lanes[j], (8 - img) % 8);
+    for (int j = 0; j < LANES; j++) s += e8_basis(j) * e8_frob(lanes[j], (LANES - img) % LANES);
     e[img] = s;
   }
   return e;
@@ -755,7 +770,7 @@ DCode block_code(DevCtx& c, const ParamsState& p, const CommittedState& cs, uint
   }
   if (!workspace.cw.p) workspace = alloc_code(R);
   launch_commit_block(p.geom, cs.d_bva, b * p.block_len, p.layout.active, workspace.cw.p,
-                      p.alpha_device(p.pp.codes.size() - 1), workspace.tree, c.st);
+                      p.alpha_encode(p.pp.codes.size() - 1), workspace.tree, c.st);
   QF_HIP(hipGetLastError());
   Digest r = download_root(c, workspace.tree, R);
   require(r == cs.roots[b], QF_ERR_INVALID_COMMITMENT, "recomputed block root mismatch");
@@ -768,18 +783,18 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
                  const Bytes& context, E8& claimed, ProveTimings& tm) {
   auto t0 = Clock::now();
   uint64_t blocks = cs.roots.size();
-  size_t split = ilog2u(blocks), n_outer = point.size() - 3 - split;
+  size_t split = ilog2u(blocks), n_outer = point.size() - LOG_LANES - split;
   std::vector<E8> outer(point.begin() + split, point.begin() + split + n_outer);
-  auto low = eq_host(point.data() + point.size() - 3, 3);
+  auto low = eq_host(point.data() + point.size() - LOG_LANES, LOG_LANES);
   auto high = eq_host(point.data(), split);
   uint64_t plen = uint64_t(1) << n_outer;
   auto lanes = block_lanes(c, p, cs.d_bva, p.layout.active, blocks, outer);
-  std::vector<std::array<E8, 8>> ebb(blocks);
+  std::vector<Lanes> ebb(blocks);
   for (uint64_t b = 0; b < blocks; b++) ebb[b] = conjugates(lanes[b]);
-  std::array<E8, 8> ev;
-  for (int img = 0; img < 8; img++) {
+  Lanes ev;
+  for (int img = 0; img < LANES; img++) {
     E8 s{};
-    for (uint64_t b = 0; b < blocks; b++) s += e8_frob(high[b], (8 - img) % 8) * ebb[b][img];
+    for (uint64_t b = 0; b < blocks; b++) s += e8_frob(high[b], (LANES - img) % LANES) * ebb[b][img];
     ev[img] = s;
   }
   claimed = projected_claim(ev, low);
@@ -791,12 +806,12 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
   t.bytes(nb, 8);
   for (auto& r : cs.roots) t.bytes(r.data(), 32);
   for (auto& v : ev) t.field(v);
-  auto powers = powers8(t.challenge());
+  auto powers = lane_powers(t.challenge());
   E8 claim{};
-  for (int i = 0; i < 8; i++) claim += powers[i] * ev[i];
+  for (int i = 0; i < LANES; i++) claim += powers[i] * ev[i];
   auto cross = [&](uint64_t w, uint64_t q) {
     E8 s{};
-    for (int i = 0; i < 8; i++) s += powers[i] * e8_frob(high[q], (8 - i) % 8) * ebb[w][i];
+    for (int i = 0; i < LANES; i++) s += powers[i] * e8_frob(high[q], (LANES - i) % LANES) * ebb[w][i];
     return s;
   };
   std::vector<E8> pos, neg;
@@ -820,16 +835,16 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
   fw[0] = e8_one();
   E8 oinv = e8_inv(omega);
   for (uint64_t b = 1; b < blocks; b++) fw[b] = fw[b - 1] * oinv;
-  std::array<E8, 8> coefs;
-  for (int img = 0; img < 8; img++) {
+  Lanes coefs;
+  for (int img = 0; img < LANES; img++) {
     E8 s{}, f = e8_one();
-    for (uint64_t b = 0; b < blocks; b++) { s += f * e8_frob(high[b], (8 - img) % 8); f = f * omega; }
+    for (uint64_t b = 0; b < blocks; b++) { s += f * e8_frob(high[b], (LANES - img) % LANES); f = f * omega; }
     coefs[img] = powers[img] * s;
   }
-  std::array<E8, 8> images;
-  for (int lane = 0; lane < 8; lane++) {
+  Lanes images;
+  for (int lane = 0; lane < LANES; lane++) {
     E8 s{};
-    for (int img = 0; img < 8; img++) s += coefs[img] * e8_frob(e8_basis(lane), (8 - img) % 8);
+    for (int img = 0; img < LANES; img++) s += coefs[img] * e8_frob(e8_basis(lane), (LANES - img) % LANES);
     images[lane] = s;
   }
   std::vector<E8> ehi, elo;
@@ -841,7 +856,7 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
   block_fold(c, cs.d_bva, p.layout.active, p.block_len, blocks, fw, values.p, plen);
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode folded = alloc_code(R);
-  launch_commit_values(p.geom, values.p, folded.cw.p, p.alpha_device(p.pp.codes.size() - 1), folded.tree, c.st);
+  launch_commit_values(p.geom, values.p, folded.cw.p, p.alpha_encode(p.pp.codes.size() - 1), folded.tree, c.st);
   QF_HIP(hipGetLastError());
   folded.root = download_root(c, folded.tree, R);
   t.bytes(folded.root.data(), 32);
@@ -906,24 +921,24 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
 Bytes open_single(DevCtx& c, const ParamsState& p, const CommittedState& cs, const std::vector<E8>& point,
                   const Bytes& context, E8& claimed, ProveTimings& tm) {
   auto t0 = Clock::now();
-  size_t n_outer = point.size() - 3;
+  size_t n_outer = point.size() - LOG_LANES;
   std::vector<E8> outer(point.begin(), point.begin() + n_outer);
   uint64_t plen = uint64_t(1) << n_outer;
   auto lanes = block_lanes(c, p, cs.d_bva, p.layout.active, 1, outer)[0];
-  auto low = eq_host(point.data() + n_outer, 3);
+  auto low = eq_host(point.data() + n_outer, LOG_LANES);
   claimed = E8{};
-  for (int j = 0; j < 8; j++) claimed += lanes[j] * low[j];
+  for (int j = 0; j < LANES; j++) claimed += lanes[j] * low[j];
   auto ev = conjugates(lanes);
   require(projected_claim(ev, low) == claimed, QF_ERR_INVALID_COMMITMENT, "projected claim");
   Packed t = packed_transcript(p, cs.roots[0], point, claimed, context);
   for (auto& v : ev) t.field(v);
-  auto powers = powers8(t.challenge());
+  auto powers = lane_powers(t.challenge());
   E8 claim{};
-  for (int i = 0; i < 8; i++) claim += powers[i] * ev[i];
-  std::array<E8, 8> images;
-  for (int j = 0; j < 8; j++) {
+  for (int i = 0; i < LANES; i++) claim += powers[i] * ev[i];
+  Lanes images;
+  for (int j = 0; j < LANES; j++) {
     E8 s{};
-    for (int i = 0; i < 8; i++) s += powers[i] * e8_frob(e8_basis(j), (8 - i) % 8);
+    for (int i = 0; i < LANES; i++) s += powers[i] * e8_frob(e8_basis(j), (LANES - i) % LANES);
     images[j] = s;
   }
   std::vector<E8> ehi, elo;
@@ -952,7 +967,7 @@ Bytes open_single(DevCtx& c, const ParamsState& p, const CommittedState& cs, con
   Bytes decide = prove_decide(c, p, pending);
   tm.decide_ms = ms_since(t0);
   Bytes w;
-  put_tag(w, "QA655PK1");
+  put_tag(w, PACKED_MAGIC);
   put_e8(w, claimed);
   for (auto& v : ev) put_e8(w, v);
   put_u64(w, rounds.size());
@@ -1046,9 +1061,9 @@ void evaluate_claims(const ParamsState& p, const CommittedState& cs, std::vector
   for (size_t i = 0; i < q.terms.size(); i++) {
     const Term& tm = q.terms[i];
     if (done[i] || tm.len() >= 16) continue;  // big terms past the active data project to zero
-    auto vals = read_bytes(c, cs.d_bva, p.layout.active, tm.offset, tm.len());
+    auto vals = read_values(c, cs.d_bva, p.layout.active, tm.offset, tm.len());
     E8 v{};
-    for (uint64_t j = 0; j < tm.len(); j++) v += tm.weight(j) * small_byte(vals[j]);
+    for (uint64_t j = 0; j < tm.len(); j++) v += tm.weight(j) * small_val(vals[j]);
     claims[p.layout.buckets[i].matrix].value += v;
   }
 }
@@ -1076,7 +1091,7 @@ Bytes prove(const ParamsState& p, const CommittedState& cs, const std::vector<Cl
   EoOut eo = eo_prove(c, p.layout, cs.d_bva, s, q, tr);
   tm.eo_ms = ms_since(t0);
   Bytes context;
-  const char* cx = "parcomm/qafold/terminal/v2";
+  const char* cx = TERMINAL_DOMAIN;
   context.insert(context.end(), cx, cx + std::strlen(cx));
   context.insert(context.end(), tr.state.begin(), tr.state.end());
   put_u64(context, 0);

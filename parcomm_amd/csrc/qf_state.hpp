@@ -18,13 +18,19 @@ struct ParamsState {
   LeafPrefix leaf_ext{};
   mutable E8* d_alpha = nullptr;              // top-code alpha_1, uploaded on first use
   mutable std::vector<E8*> d_alpha_levels;    // alpha_1 of every level (top included), on first prove
+  mutable std::vector<E8*> d_alpha_enc;       // the same in encode form (KoalaBear: Montgomery)
   mutable std::vector<uint64_t> zero_ranges;  // padding intervals (start, len) chunked for the device check
   mutable std::mutex mu;                      // guards the lazily filled members above
   ~ParamsState() {
+    for (auto* a : d_alpha_enc)
+      if (a && std::find(d_alpha_levels.begin(), d_alpha_levels.end(), a) == d_alpha_levels.end()) (void)hipFree(a);
     for (auto* a : d_alpha_levels) if (a && a != d_alpha) (void)hipFree(a);
     if (d_alpha) (void)hipFree(d_alpha);
   }
+  // alpha_1 of code `level`, canonical (for the opening algebra)
   const E8* alpha_device(size_t level) const;
+  // alpha_1 of code `level` in the encoding kernels' form (v8_encode_form)
+  const E8* alpha_encode(size_t level) const;
 };
 
 struct CommittedState {
@@ -33,7 +39,7 @@ struct CommittedState {
   std::vector<Digest> roots;
   std::vector<E8*> d_cw;        // per block (cached) or one workspace
   std::vector<uint32_t*> d_tree;
-  const int8_t* d_bva = nullptr;  // the committed BVA (active bytes) in HBM
+  const Val* d_bva = nullptr;  // the committed BVA (active entries) in HBM
   bool owns_bva = false;
   double total_ms = 0, leaf_ms = 0, encode_ms = 0;
   ~CommittedState() {
@@ -43,7 +49,7 @@ struct CommittedState {
     };
     for (auto* q : d_cw) release(q);
     for (auto* q : d_tree) release(q);
-    if (owns_bva && d_bva) (void)hipFree(const_cast<int8_t*>(d_bva));
+    if (owns_bva && d_bva) (void)hipFree(const_cast<Val*>(d_bva));
   }
 };
 

@@ -6,12 +6,14 @@
 // copy then WHT . alpha . WHT. One
 // workgroup (512 threads) owns a tile of 4096 elements: C contiguous positions
 // x 2^t positions spaced 2^s apart (x 2^logV whole vectors when a vector is
-// shorter than a tile). Each thread keeps 8 elements as eight signed 32-bit
-// lanes without modular reduction (12 stages grow a canonical value by at
-// most 2^12, so |x| < 2^28), runs three butterfly stages in registers, and
+// shorter than a tile). Each thread keeps 8 elements in registers, one 32-bit
+// lane per coefficient, runs three butterfly stages in registers, and
 // trades elements through LDS to bring the next three tile bits into its
-// registers (mappings with register bits 9..11, 6..8, 3..5, 0..2). Values are reduced to canonical u16 only
-// before they are stored or multiplied. Workgroups are persistent: each walks
+// registers (mappings with register bits 9..11, 6..8, 3..5, 0..2). F65521
+// lanes are signed and unreduced (12 stages grow a canonical value by at most
+// 2^12, so |x| < 2^28) and are reduced to canonical u16 only before they are
+// stored or multiplied; KoalaBear lanes (p ~ 2^31) stay canonical through
+// modular butterflies. Workgroups are persistent: each walks
 // a contiguous range of tiles (neighbouring tiles share 128-byte lines, so
 // they stay on one XCD's L2) and prefetches the next tile's input while the
 // current one is transformed.
@@ -25,7 +27,7 @@ constexpr int TQB = 3;                 // register bits per phase
 constexpr int TQ = 1 << TQB;           // elements per thread
 constexpr int TN = TWG * TQ;           // elements per tile
 constexpr int TPAD = TN + TN / 8;      // LDS row with one pad slot per 8 elements
-constexpr int32_t BIAS = 32768 * int32_t(P);  // multiple of P above every lazy magnitude
+constexpr int LROWS = LANES / 2;       // LDS rows of uint2 (two lanes each)
 
 enum : int { TM_M = 0, TM_M_W = 1, TM_W = 2, TM_WAW = 3, TM_COPY_W = 4, TM_COPY_WAW = 5 };
 // Mappings are named by the first tile bit held in registers.
@@ -74,8 +76,17 @@ __device__ inline uint32_t tile_off(const TileArgs& a, int e, uint64_t vs) {
 // stages carry no branches), else the runtime range [lo, hi). The scheduler
 // barriers keep the compiler from interleaving whole phases, which would
 // lengthen live ranges.
+// Butterfly arithmetic on register lanes.
+#ifdef QF_KB
+__device__ inline int32_t bf_add(int32_t u, int32_t w) { return int32_t(madd(uint32_t(u), uint32_t(w))); }
+__device__ inline int32_t bf_sub(int32_t u, int32_t w) { return int32_t(msub(uint32_t(u), uint32_t(w))); }
+#else
+__device__ inline int32_t bf_add(int32_t u, int32_t w) { return u + w; }
+__device__ inline int32_t bf_sub(int32_t u, int32_t w) { return u - w; }
+#endif
+
 template <bool WHT, int MAP, int LO>
-__device__ inline void phase(int32_t (&x)[TQ][8], int lo, int hi) {
+__device__ inline void phase(int32_t (&x)[TQ][LANES], int lo, int hi) {
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int b = 0; b < TQB; b++) {
@@ -90,10 +101,10 @@ __device__ inline void phase(int32_t (&x)[TQ][8], int lo, int hi) {
       if (q & (1 << b)) continue;
       const int r = q | (1 << b);
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
+      for (int k = 0; k < LANES; k++) {
         const int32_t u = x[q][k], w = x[r][k];
-        if (WHT) { x[q][k] = u + w; x[r][k] = u - w; }
-        else x[r][k] = w - u;
+        if (WHT) { x[q][k] = bf_add(u, w); x[r][k] = bf_sub(u, w); }
+        else x[r][k] = bf_sub(w, u);
       }
     }
   }
@@ -101,27 +112,57 @@ __device__ inline void phase(int32_t (&x)[TQ][8], int lo, int hi) {
 }
 
 template <int FROM, int TO>
-__device__ inline void exchange(int32_t (&x)[TQ][8], uint2 (*lds)[TPAD]) {
+__device__ inline void exchange(int32_t (&x)[TQ][LANES], uint2 (*lds)[TPAD]) {
   const int tid = threadIdx.x;
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < TQ; q++) {
     const int i = lds_slot(tile_e<FROM>(tid, q));
 #pragma unroll
-    for (int p = 0; p < 4; p++) lds[p][i] = make_uint2(uint32_t(x[q][2 * p]), uint32_t(x[q][2 * p + 1]));
+    for (int p = 0; p < LROWS; p++) lds[p][i] = make_uint2(uint32_t(x[q][2 * p]), uint32_t(x[q][2 * p + 1]));
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < TQ; q++) {
     const int i = lds_slot(tile_e<TO>(tid, q));
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
+    for (int p = 0; p < LROWS; p++) {
       const uint2 v = lds[p][i];
       x[q][2 * p] = int32_t(v.x);
       x[q][2 * p + 1] = int32_t(v.y);
     }
   }
 }
+
+#ifdef QF_KB
+// KoalaBear lanes are canonical throughout.
+__device__ inline void reduce_all(int32_t (&)[TQ][LANES]) {}
+
+__device__ inline void unpack_e8(const uint4& w, int32_t (&c)[LANES]) {
+  c[0] = int32_t(w.x); c[1] = int32_t(w.y); c[2] = int32_t(w.z); c[3] = int32_t(w.w);
+}
+__device__ inline uint4 pack_e8(const int32_t (&c)[LANES]) {
+  return make_uint4(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]), uint32_t(c[3]));
+}
+// Four little-endian i16 BVA entries -> canonical lanes.
+__device__ inline void unpack_bva(const uint2& w, int32_t (&c)[LANES]) {
+  c[0] = int32_t(from_val(int16_t(w.x & 0xffffu)));
+  c[1] = int32_t(from_val(int16_t(w.x >> 16)));
+  c[2] = int32_t(from_val(int16_t(w.y & 0xffffu)));
+  c[3] = int32_t(from_val(int16_t(w.y >> 16)));
+}
+// a * alpha in KoalaBear[X]/(X^4 - 3), alpha in Montgomery form.
+__device__ inline void mul_alpha(int32_t (&a)[LANES], const uint4& bw) {
+  V8 x, b;
+#pragma unroll
+  for (int k = 0; k < 4; k++) x.c[k] = uint32_t(a[k]);
+  b.c[0] = bw.x; b.c[1] = bw.y; b.c[2] = bw.z; b.c[3] = bw.w;
+  const V8 r = v8_mul_mont(x, b);
+#pragma unroll
+  for (int k = 0; k < 4; k++) a[k] = int32_t(r.c[k]);
+}
+#else
+constexpr int32_t BIAS = 32768 * int32_t(P);  // multiple of P above every lazy magnitude
 
 // Lazy signed value (|v| < 2^31 - BIAS margin) -> canonical residue mod P.
 __device__ inline uint32_t reduce_lazy(int32_t v) {
@@ -149,7 +190,7 @@ __device__ inline uint4 pack_e8(const int32_t (&c)[8]) {
   return make_uint4(uint32_t(c[0]) | (uint32_t(c[1]) << 16), uint32_t(c[2]) | (uint32_t(c[3]) << 16),
                     uint32_t(c[4]) | (uint32_t(c[5]) << 16), uint32_t(c[6]) | (uint32_t(c[7]) << 16));
 }
-__device__ inline void unpack_i8(const uint2& w, int32_t (&c)[8]) {
+__device__ inline void unpack_bva(const uint2& w, int32_t (&c)[8]) {
 #pragma unroll
   for (int k = 0; k < 8; k++) c[k] = int32_t(int8_t(((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 0xffu));
 }
@@ -204,6 +245,7 @@ __device__ inline void mul_alpha(int32_t (&a)[8], const uint4& bw) {
     a[i] = int32_t(r < w ? r : w);
   }
 }
+#endif
 
 // Per-thread byte offsets of the mapping-A slots from the tile's first
 // element: slot q = thread part + q part (the q part is uniform). 32-bit
@@ -227,12 +269,12 @@ __device__ inline void load_tile(const TileArgs& a, uint64_t T, const SlotOffs& 
       uint2 w;
       if (byte + 8 <= a.available) {
         w = *reinterpret_cast<const uint2*>(base + off);
-      } else {
-        const int8_t* bva = static_cast<const int8_t*>(a.in);
+      } else {  // bytes at or past the active end read as zero
+        const uint8_t* bva = static_cast<const uint8_t*>(a.in);
         w = make_uint2(0, 0);
         for (int j = 0; j < 8; j++)
           if (byte + j < a.available) {
-            const uint32_t v = uint32_t(uint8_t(bva[byte + j])) << (8 * (j & 3));
+            const uint32_t v = uint32_t(bva[byte + j]) << (8 * (j & 3));
             if (j < 4) w.x |= v; else w.y |= v;
           }
       }
@@ -246,7 +288,7 @@ __device__ inline void load_tile(const TileArgs& a, uint64_t T, const SlotOffs& 
   }
 }
 
-__device__ inline void store_tile(const TileArgs& a, uint64_t T, const SlotOffs& so, E8* out, const int32_t (&x)[TQ][8]) {
+__device__ inline void store_tile(const TileArgs& a, uint64_t T, const SlotOffs& so, E8* out, const int32_t (&x)[TQ][LANES]) {
   uint64_t vec0, pos0;
   tile_base(a, T, vec0, pos0);
   char* base = reinterpret_cast<char*>(out + vec0 * a.out_vstride + pos0);
@@ -257,7 +299,7 @@ __device__ inline void store_tile(const TileArgs& a, uint64_t T, const SlotOffs&
 
 template <int MODE, bool PACK, int LO>
 __global__ __launch_bounds__(TWG) void k_tile(TileArgs a) {
-  __shared__ uint2 lds[4][TPAD];
+  __shared__ uint2 lds[LROWS][TPAD];
   const uint64_t first = uint64_t(blockIdx.x) * a.per_wg;
   const uint64_t last = first + a.per_wg < a.tiles ? first + a.per_wg : a.tiles;
   if (first >= last) return;
@@ -268,10 +310,10 @@ __global__ __launch_bounds__(TWG) void k_tile(TileArgs a) {
   uint4 raw[TQ];
   load_tile<PACK>(a, first, so, raw);
   for (uint64_t T = first; T < last; T++) {
-    int32_t x[TQ][8];
+    int32_t x[TQ][LANES];
 #pragma unroll
     for (int q = 0; q < TQ; q++) {
-      if (PACK) unpack_i8(make_uint2(raw[q].x, raw[q].y), x[q]);
+      if (PACK) unpack_bva(make_uint2(raw[q].x, raw[q].y), x[q]);
       else unpack_e8(raw[q], x[q]);
     }
     if (MODE == TM_COPY_W || MODE == TM_COPY_WAW) store_tile(a, T, so, a.out1, x);

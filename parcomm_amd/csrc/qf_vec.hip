@@ -37,14 +37,14 @@ __global__ __launch_bounds__(WG) void k_reduce(const E8* __restrict__ a, const E
     }
   }
 #pragma unroll
-  for (int k = 0; k < 8; k++) { red[threadIdx.x][k] = s0.c[k]; red[threadIdx.x][8 + k] = s1.c[k]; }
+  for (int k = 0; k < LANES; k++) { red[threadIdx.x][k] = s0.c[k]; red[threadIdx.x][LANES + k] = s1.c[k]; }
   __syncthreads();
   for (int w = WG / 2; w > 0; w /= 2) {
     if (int(threadIdx.x) < w)
-      for (int k = 0; k < 16; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + w][k]);
+      for (int k = 0; k < 2 * LANES; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + w][k]);
     __syncthreads();
   }
-  if (threadIdx.x < 16) partials[blockIdx.x * 16 + threadIdx.x] = red[0][threadIdx.x];
+  if (threadIdx.x < 2 * LANES) partials[blockIdx.x * 16 + threadIdx.x] = red[0][threadIdx.x];
 }
 
 uint32_t groups_for(uint64_t n) {
@@ -58,9 +58,9 @@ std::pair<E8, E8> finish_partials(DevCtx& c, uint32_t groups) {
   c.sync();
   V8 s0 = v8_zero(), s1 = v8_zero();
   for (uint32_t g = 0; g < groups; g++)
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < LANES; k++) {
       s0.c[k] = madd(s0.c[k], c.host_partials[g * 16 + k]);
-      s1.c[k] = madd(s1.c[k], c.host_partials[g * 16 + 8 + k]);
+      s1.c[k] = madd(s1.c[k], c.host_partials[g * 16 + LANES + k]);
     }
   return {v8_store(s0), v8_store(s1)};
 }
@@ -120,22 +120,23 @@ __global__ void k_scale(E8* y, uint64_t n, E8 s) {
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
     y[i] = v8_store(v8_mul(v8_load(y[i]), vs));
 }
-// Column projection: grid (groups, cols); partials[(col * groups + g)][0..8)
+// Column projection: grid (groups, cols); partials[(col * groups + g)][0..LANES)
 __global__ __launch_bounds__(WG) void k_col_projection(const E8* __restrict__ cw, uint64_t R, uint64_t n,
                                                        const E8* __restrict__ w, uint32_t* __restrict__ partials) {
-  __shared__ uint32_t red[WG][8];
+  __shared__ uint32_t red[WG][LANES];
   const E8* col = cw + uint64_t(blockIdx.y) * R;
   V8 s = v8_zero();
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
     s = v8_add(s, v8_mul(v8_load(col[i]), v8_load(w[i])));
-  for (int k = 0; k < 8; k++) red[threadIdx.x][k] = s.c[k];
+  for (int k = 0; k < LANES; k++) red[threadIdx.x][k] = s.c[k];
   __syncthreads();
   for (int h = WG / 2; h > 0; h /= 2) {
     if (int(threadIdx.x) < h)
-      for (int k = 0; k < 8; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + h][k]);
+      for (int k = 0; k < LANES; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + h][k]);
     __syncthreads();
   }
-  if (threadIdx.x < 8) partials[(uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * 8 + threadIdx.x] = red[0][threadIdx.x];
+  if (threadIdx.x < LANES)
+    partials[(uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * LANES + threadIdx.x] = red[0][threadIdx.x];
 }
 __global__ __launch_bounds__(WG) void k_col_fold(const E8* __restrict__ cw, uint64_t R, uint64_t n, uint64_t cols,
                                                  const E8* __restrict__ f, E8* __restrict__ out) {
@@ -196,22 +197,42 @@ __device__ inline void build_table(E8* T, const E8 (*pairs)[2], int k) {
   }
 }
 
-// Q (2 or 4) consecutive BVA bytes at `byte` as a little-endian word; bytes at
-// or past `available` read as zero. `byte` is a multiple of Q.
+// Signed products of BVA entries with canonical coefficients accumulate
+// exactly in Acc before one reduction: i8 x u16 products (< 2^23) fit 32 bits
+// over the short runs summed here; i16 x u31 products (< 2^46) need 64.
+#ifdef QF_KB
+using Acc = int64_t;
+#else
+using Acc = int32_t;
+#endif
+
+// Q (2 or 4) consecutive BVA entries from entry `idx` (a multiple of Q), one
+// aligned load; entries at or past `available` read as zero.
 template <int Q>
-__device__ inline uint32_t load_lanes(const int8_t* __restrict__ bva, uint64_t byte, uint64_t available) {
-  if (byte + Q <= available) {
-    if constexpr (Q == 4) return *reinterpret_cast<const uint32_t*>(bva + byte);
-    else return *reinterpret_cast<const uint16_t*>(bva + byte);
+__device__ inline void load_vals(const Val* __restrict__ bva, uint64_t idx, uint64_t available, int32_t (&v)[Q]) {
+  if (idx + Q <= available) {
+    if constexpr (Q * sizeof(Val) == 8) {
+      const uint2 w = *reinterpret_cast<const uint2*>(bva + idx);
+#pragma unroll
+      for (int j = 0; j < Q; j++) v[j] = int16_t((j < 2 ? w.x : w.y) >> (16 * (j & 1)));
+    } else if constexpr (Q * sizeof(Val) == 4) {
+      const uint32_t w = *reinterpret_cast<const uint32_t*>(bva + idx);
+#pragma unroll
+      for (int j = 0; j < Q; j++) v[j] = sizeof(Val) == 1 ? int32_t(int8_t(w >> (8 * j))) : int32_t(int16_t(w >> (16 * j)));
+    } else {
+      const uint16_t w = *reinterpret_cast<const uint16_t*>(bva + idx);
+#pragma unroll
+      for (int j = 0; j < Q; j++) v[j] = int8_t(w >> (8 * j));
+    }
+    return;
   }
-  uint32_t w = 0;
-  for (int j = 0; j < Q; j++) if (byte + j < available) w |= uint32_t(uint8_t(bva[byte + j])) << (8 * j);
-  return w;
+#pragma unroll
+  for (int j = 0; j < Q; j++) v[j] = idx + j < available ? int32_t(bva[idx + j]) : 0;
 }
 
 // One workgroup = one chunk of one projection task (2^chunk_rows_log rows).
 template <int L>
-__global__ __launch_bounds__(WG) void k_tensor_project(const int8_t* __restrict__ bva, uint64_t available,
+__global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ bva, uint64_t available,
                                                        const ProjTaskDev* __restrict__ tasks,
                                                        const E8 (*__restrict__ pairs)[2], E8* __restrict__ out) {
   constexpr int Q = L < 4 ? L : 4;       // lanes per thread
@@ -237,34 +258,33 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const int8_t* __restrict_
       C = v8_store(c);
     }
     __syncthreads();
-    int32_t ai[Q][8];
+    Acc ai[Q][LANES];
     for (int l = 0; l < Q; l++)
-      for (int k = 0; k < 8; k++) ai[l][k] = 0;
+      for (int k = 0; k < LANES; k++) ai[l][k] = 0;
     for (uint64_t lo = r0; lo < lo_rows; lo += RPP) {
-      const uint32_t w = load_lanes<Q>(bva, t.base + (row_hi + lo) * L + Q * q, available);
+      int32_t v[Q];
+      load_vals<Q>(bva, t.base + (row_hi + lo) * L + Q * q, available, v);
       const E8 wt = T[lo];
 #pragma unroll
-      for (int l = 0; l < Q; l++) {
-        const int32_t v = int8_t(w >> (8 * l));
+      for (int l = 0; l < Q; l++)
 #pragma unroll
-        for (int k = 0; k < 8; k++) ai[l][k] += v * int32_t(wt.c[k]);
-      }
+        for (int k = 0; k < LANES; k++) ai[l][k] += Acc(v[l]) * Acc(wt.c[k]);
     }
     V8 c = v8_load(C);
 #pragma unroll
     for (int l = 0; l < Q; l++) {
       V8 x;
 #pragma unroll
-      for (int k = 0; k < 8; k++) x.c[k] = mod_signed(ai[l][k]);
+      for (int k = 0; k < LANES; k++) x.c[k] = mod_signed(ai[l][k]);
       acc[l] = v8_add(acc[l], v8_mul(x, c));
     }
   }
   __syncthreads();
   // reduce acc over threads sharing the same lane quad; reuse the table as scratch
-  uint32_t* red = reinterpret_cast<uint32_t*>(T);  // [WG][8 Q]
-  constexpr int RW = 8 * Q;
+  uint32_t* red = reinterpret_cast<uint32_t*>(T);  // [WG][LANES Q]
+  constexpr int RW = LANES * Q;
   for (int l = 0; l < Q; l++)
-    for (int k = 0; k < 8; k++) red[threadIdx.x * RW + l * 8 + k] = acc[l].c[k];
+    for (int k = 0; k < LANES; k++) red[threadIdx.x * RW + l * LANES + k] = acc[l].c[k];
   __syncthreads();
   for (int h = WG / 2; h >= TPR; h /= 2) {
     if (int(threadIdx.x) < h)
@@ -274,35 +294,37 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const int8_t* __restrict_
   if (int(threadIdx.x) < TPR) {
     for (int l = 0; l < Q; l++) {
       V8 x;
-      for (int k = 0; k < 8; k++) x.c[k] = red[threadIdx.x * RW + l * 8 + k];
+      for (int k = 0; k < LANES; k++) x.c[k] = red[threadIdx.x * RW + l * LANES + k];
       out[uint64_t(blockIdx.x) * L + Q * threadIdx.x + l] = v8_store(x);
     }
   }
 }
 
 template <int S>
-__global__ __launch_bounds__(WG) void k_eo_witness(const int8_t* __restrict__ bva, uint64_t available, uint64_t n_rows,
+__global__ __launch_bounds__(WG) void k_eo_witness(const Val* __restrict__ bva, uint64_t available, uint64_t n_rows,
                                                    const E8* __restrict__ pinv_g, E8* __restrict__ w) {
   __shared__ E8 pinv[S];
   for (int i = threadIdx.x; i < S; i += WG) pinv[i] = pinv_g[i];
   __syncthreads();
   for (uint64_t row = uint64_t(blockIdx.x) * WG + threadIdx.x; row < n_rows; row += uint64_t(gridDim.x) * WG) {
-    int32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    Acc acc[LANES];
+#pragma unroll
+    for (int k = 0; k < LANES; k++) acc[k] = 0;
     constexpr int Q = S < 4 ? S : 4;
-    const uint64_t byte = row * S;
+    const uint64_t first = row * S;
     for (int g = 0; g < S / Q; g++) {
-      const uint32_t x = load_lanes<Q>(bva, byte + Q * g, available);
+      int32_t v[Q];
+      load_vals<Q>(bva, first + Q * g, available, v);
 #pragma unroll
       for (int j = 0; j < Q; j++) {
-        const int32_t v = int8_t(x >> (8 * j));
         const E8 p = pinv[Q * g + j];
 #pragma unroll
-        for (int k = 0; k < 8; k++) acc[k] += v * int32_t(p.c[k]);
+        for (int k = 0; k < LANES; k++) acc[k] += Acc(v[j]) * Acc(p.c[k]);
       }
     }
     V8 o;
 #pragma unroll
-    for (int k = 0; k < 8; k++) o.c[k] = mod_signed(acc[k]);
+    for (int k = 0; k < LANES; k++) o.c[k] = mod_signed(acc[k]);
     w[row] = v8_store(o);
   }
 }
@@ -334,65 +356,74 @@ __global__ __launch_bounds__(WG) void k_tensor_fill_add(E8* __restrict__ out, co
   }
 }
 
-__global__ __launch_bounds__(WG) void k_block_fold(const int8_t* __restrict__ bva, uint64_t available,
-                                                   uint64_t block_len, uint32_t blocks, const E8* __restrict__ fw_g,
+// values[i] = sum_b fw[b] * packed(b, i); `available` and `block_bytes` in bytes.
+__global__ __launch_bounds__(WG) void k_block_fold(const uint8_t* __restrict__ bva, uint64_t available,
+                                                   uint64_t block_bytes, uint32_t blocks, const E8* __restrict__ fw_g,
                                                    E8* __restrict__ values, uint64_t n) {
   __shared__ E8 fw[128];
   for (uint32_t i = threadIdx.x; i < blocks; i += WG) fw[i] = fw_g[i];
   __syncthreads();
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG) {
-    int32_t S[8][8];
-    for (int j = 0; j < 8; j++)
-      for (int k = 0; k < 8; k++) S[j][k] = 0;
+    Acc S[LANES][LANES];
+    for (int j = 0; j < LANES; j++)
+      for (int k = 0; k < LANES; k++) S[j][k] = 0;
     for (uint32_t b = 0; b < blocks; b++) {
-      const uint64_t byte = uint64_t(b) * block_len + 8 * i;
+      const uint64_t byte = uint64_t(b) * block_bytes + 8 * i;
       uint32_t x[2] = {0, 0};
       if (byte + 8 <= available) {
         uint2 u = *reinterpret_cast<const uint2*>(bva + byte);
         x[0] = u.x;
         x[1] = u.y;
       } else {
-        for (int j = 0; j < 8; j++) if (byte + j < available) x[j >> 2] |= uint32_t(uint8_t(bva[byte + j])) << (8 * (j & 3));
+        for (int j = 0; j < 8; j++) if (byte + j < available) x[j >> 2] |= uint32_t(bva[byte + j]) << (8 * (j & 3));
       }
       const E8 f = fw[b];
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int32_t v = int8_t(x[j >> 2] >> (8 * (j & 3)));
+      for (int j = 0; j < LANES; j++) {
+#ifdef QF_KB
+        const Acc v = int16_t(x[j >> 1] >> (16 * (j & 1)));
+#else
+        const Acc v = int8_t(x[j >> 2] >> (8 * (j & 3)));
+#endif
 #pragma unroll
-        for (int k = 0; k < 8; k++) S[j][k] += v * int32_t(f.c[k]);
+        for (int k = 0; k < LANES; k++) S[j][k] += v * Acc(f.c[k]);
       }
     }
-    // value = sum_j X^j * S_j with X^8 = 17
-    uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // value = sum_j X^j * S_j with X^LANES = EXT_W
+    uint64_t acc[LANES];
 #pragma unroll
-    for (int j = 0; j < 8; j++)
+    for (int k = 0; k < LANES; k++) acc[k] = 0;
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
+    for (int j = 0; j < LANES; j++)
+#pragma unroll
+      for (int k = 0; k < LANES; k++) {
         const uint32_t s = mod_signed(S[j][k]);
         const int d = j + k;
-        if (d < 8) acc[d] += s; else acc[d - 8] += 17ull * s;
+        if (d < LANES) acc[d] += s; else acc[d - LANES] += uint64_t(EXT_W) * s;
       }
     V8 o;
 #pragma unroll
-    for (int k = 0; k < 8; k++) o.c[k] = mred64(acc[k]);
+    for (int k = 0; k < LANES; k++) o.c[k] = mred64(acc[k]);
     values[i] = v8_store(o);
   }
 }
 
-struct Images { E8 m[8]; };
+struct Images { E8 m[LANES]; };
 __global__ __launch_bounds__(WG) void k_weight_images(E8* __restrict__ out, uint64_t n, const E8* __restrict__ hi,
                                                       const E8* __restrict__ lo, int a, Images im) {
   const uint64_t mask = (uint64_t(1) << a) - 1;
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG) {
     V8 e = v8_mul(v8_load(hi[i >> a]), v8_load(lo[i & mask]));
-    uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t acc[LANES];  // LANES products of canonical coefficients: < 2^35 (F65521), < 2^64 (KoalaBear)
 #pragma unroll
-    for (int j = 0; j < 8; j++)
+    for (int k = 0; k < LANES; k++) acc[k] = 0;
 #pragma unroll
-      for (int k = 0; k < 8; k++) acc[k] += uint64_t(im.m[j].c[k]) * e.c[j];
+    for (int j = 0; j < LANES; j++)
+#pragma unroll
+      for (int k = 0; k < LANES; k++) acc[k] += uint64_t(im.m[j].c[k]) * e.c[j];
     V8 o;
 #pragma unroll
-    for (int k = 0; k < 8; k++) o.c[k] = mred64(acc[k]);
+    for (int k = 0; k < LANES; k++) o.c[k] = mred64(acc[k]);
     out[i] = v8_store(o);
   }
 }
@@ -598,7 +629,7 @@ std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, 
   for (uint64_t col = 0; col < cols; col++) {
     V8 s = v8_zero();
     for (uint32_t i = 0; i < g; i++)
-      for (int k = 0; k < 8; k++) s.c[k] = madd(s.c[k], c.host_partials[(col * g + i) * 8 + k]);
+      for (int k = 0; k < LANES; k++) s.c[k] = madd(s.c[k], c.host_partials[(col * g + i) * LANES + k]);
     out[col] = v8_store(s);
   }
   return out;
@@ -666,7 +697,7 @@ std::atomic<int> g_launch_mode{LAUNCH_BATCHED};
 void set_launch_mode(int mode) { g_launch_mode = mode; }
 int launch_mode() { return g_launch_mode; }
 
-std::vector<E8> tensor_project(DevCtx& c, const int8_t* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
+std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
                                const std::vector<std::array<E8, 2>>& pairs) {
   require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
   if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
@@ -727,7 +758,7 @@ std::vector<E8> tensor_project(DevCtx& c, const int8_t* bva, uint64_t available,
   return result;
 }
 
-void eo_witness(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
+void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
                 E8* w) {
   require(s >= 2 && s <= 64 && pow2(s) && int(pinv.size()) == s, QF_ERR_ARGUMENT, "witness stride");
   E8* t = put_table(c, scratch_tab(), pinv.data(), pinv.size() * 16);
@@ -778,22 +809,23 @@ void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, con
   dfree(d_tasks, c.st);
 }
 
-void block_fold(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
+void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
                 const std::vector<E8>& fw, E8* values, uint64_t n) {
   require(blocks <= 128 && fw.size() == blocks, QF_ERR_ARGUMENT, "block fold weights");
   E8* t = put_table(c, scratch_tab(), fw.data(), fw.size() * 16);
-  hipLaunchKernelGGL(k_block_fold, grid_for(n), dim3(WG), 0, c.st, bva, available, block_len, uint32_t(blocks), t,
-                     values, n);
+  hipLaunchKernelGGL(k_block_fold, grid_for(n), dim3(WG), 0, c.st, reinterpret_cast<const uint8_t*>(bva),
+                     available * sizeof(Val), block_len * sizeof(Val), uint32_t(blocks), t, values, n);
   QF_HIP(hipGetLastError());
   c.sync();
 }
 
-void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo, const std::array<E8, 8>& images) {
+void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo,
+                   const std::array<E8, LANES>& images) {
   E8* t = scratch_tab();
   put_table(c, t, hi.data(), hi.size() * 16);
   put_table(c, t + hi.size(), lo.data(), lo.size() * 16);
   Images im;
-  for (int j = 0; j < 8; j++) im.m[j] = images[j];
+  for (int j = 0; j < LANES; j++) im.m[j] = images[j];
   uint64_t n = hi.size() * lo.size();
   hipLaunchKernelGGL(k_weight_images, grid_for(n), dim3(WG), 0, c.st, out, n, t, t + hi.size(), ilog2u(lo.size()), im);
   QF_HIP(hipGetLastError());

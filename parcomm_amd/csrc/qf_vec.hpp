@@ -1,4 +1,4 @@
-// Device vectors of Ext8 elements and the vector operations of the opening
+// Device vectors of extension elements and the vector operations of the opening
 // (LiteBullet, even/odd and sumcheck rounds, tensor tables, codeword column
 // projections). Reductions return their result to the host: each workgroup
 // writes a partial sum and the host adds at most a few thousand partials.
@@ -9,7 +9,7 @@
 
 namespace qf {
 
-// Owning device buffer of `n` Ext8 elements.
+// Owning device buffer of `n` extension elements.
 // Allocated and released in order on the thread's current stream.
 struct DVec {
   E8* p = nullptr;
@@ -31,7 +31,7 @@ struct DVec {
 struct DevCtx {
   hipStream_t st = nullptr;
   hipStream_t prev = nullptr;
-  uint32_t* partials = nullptr;  // [max_groups][2][8] u32
+  uint32_t* partials = nullptr;  // [max_groups][16] u32: two sums of LANES coefficients
   uint32_t* host_partials = nullptr;
   static constexpr uint32_t MAX_GROUPS = 1024;
   DevCtx();
@@ -85,27 +85,27 @@ int launch_mode();
 // ---------------------------------------------------------------- BVA-scale kernels of the opening
 // One projection task: lanes[l] = sum_row W(row) * v[base + row*L + l] over `rows`
 // rows, W(row) = scale * prod_b pair_b[(row >> b) & 1] for the task's bit pairs
-// (LSB first). Bytes at or past `available` read as zero.
+// (LSB first). Entries at or past `available` read as zero.
 struct ProjTask {
-  uint64_t base;      // byte offset of row 0
+  uint64_t base;      // entry offset of row 0
   uint64_t rows;      // power of two
   uint32_t nbits;     // log2(rows)
   uint32_t pair_off;  // index of this task's first pair in the pair table
   E8 scale;
 };
 // Returns tasks.size() x L lane sums.
-std::vector<E8> tensor_project(DevCtx& c, const int8_t* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
+std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
                                const std::vector<std::array<E8, 2>>& pairs);
 // Packed EO witness: w[row] = sum_lane v[row*s + lane] * pinv[lane] for rows < n_rows.
-void eo_witness(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
+void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
                 E8* w);
 // out[start + j] += scale * prod_b pair_b[(j >> b) & 1], j < 2^nbits, per task (tasks disjoint).
 struct FillTask { uint64_t start; uint32_t nbits, pair_off; E8 scale; };
 void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, const std::vector<std::array<E8, 2>>& pairs);
-// values[i] = sum_b fw[b] * packed(b, i), packed(b,i) lanes = bytes (b*block_len + 8i + j)
-void block_fold(DevCtx& c, const int8_t* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
+// values[i] = sum_b fw[b] * packed(b, i), lane j of packed(b, i) = entry b*block_len + LANES*i + j
+void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
                 const std::vector<E8>& fw, E8* values, uint64_t n);
 // weights[i] = sum_j eq(i).c[j] * images[j], eq = hi (x) lo tables
-void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo, const std::array<E8, 8>& images);
+void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo, const std::array<E8, LANES>& images);
 
 }  // namespace qf

@@ -552,18 +552,18 @@ void verify_decide(const ParamsState& p, const std::vector<Pending>& pending, Re
 // packed.rs:396-458 (one block)
 void verify_single(const ParamsState& p, const Digest& root, const std::vector<E8>& point, E8 claimed, Reader r,
                    const Bytes& context) {
-  const size_t n_outer = size_t(p.chunk_log) - 3;
+  const size_t n_outer = size_t(p.chunk_log) - LOG_LANES;
   check(point.size() == size_t(p.chunk_log), "single opening point size");
-  check(r.tag("QA655PK1") && r.field() == claimed, "single opening header");
-  std::array<E8, 8> ev;
+  check(r.tag(PACKED_MAGIC) && r.field() == claimed, "single opening header");
+  Lanes ev;
   for (auto& v : ev) v = r.field();
-  auto low = eq_host(point.data() + n_outer, 3);
+  auto low = eq_host(point.data() + n_outer, LOG_LANES);
   check(projected_claim(ev, low) == claimed, "projected claim");
   Packed t = packed_transcript(p, root, point, claimed, context);
   for (auto& v : ev) t.field(v);
-  auto powers = powers8(t.challenge());
+  auto powers = lane_powers(t.challenge());
   E8 claim{};
-  for (int i = 0; i < 8; i++) claim += powers[i] * ev[i];
+  for (int i = 0; i < LANES; i++) claim += powers[i] * ev[i];
   check(r.u64() == n_outer, "sumcheck round count");
   std::vector<E8> reduced;
   for (size_t i = 0; i < n_outer; i++) {
@@ -587,13 +587,13 @@ void verify_batch(const ParamsState& p, const std::vector<Digest>& roots, const 
                   Reader r, const Bytes& context) {
   const uint64_t blocks = roots.size();
   check(blocks >= 2 && blocks <= 128 && pow2(blocks), "block count");
-  const size_t split = ilog2u(blocks), n_outer = size_t(p.chunk_log) - 3;
-  check(point.size() == split + n_outer + 3, "batch point size");
+  const size_t split = ilog2u(blocks), n_outer = size_t(p.chunk_log) - LOG_LANES;
+  check(point.size() == split + n_outer + LOG_LANES, "batch point size");
   std::vector<E8> outer(point.begin() + split, point.begin() + split + n_outer);
-  auto low = eq_host(point.data() + point.size() - 3, 3);
+  auto low = eq_host(point.data() + point.size() - LOG_LANES, LOG_LANES);
   auto high = eq_host(point.data(), split);
   check(r.tag("QAPKBAT1") && r.field() == claimed, "batch header");
-  std::array<E8, 8> ev;
+  Lanes ev;
   for (auto& v : ev) v = r.field();
   check(projected_claim(ev, low) == claimed, "projected claim");
   Packed t = packed_transcript(p, roots[0], point, claimed, context);
@@ -604,9 +604,9 @@ void verify_batch(const ParamsState& p, const std::vector<Digest>& roots, const 
   t.bytes(nb, 8);
   for (auto& rt : roots) t.bytes(rt.data(), 32);
   for (auto& v : ev) t.field(v);
-  auto powers = powers8(t.challenge());
+  auto powers = lane_powers(t.challenge());
   E8 claim{};
-  for (int i = 0; i < 8; i++) claim += powers[i] * ev[i];
+  for (int i = 0; i < LANES; i++) claim += powers[i] * ev[i];
   std::vector<E8> pos, neg;
   for (uint64_t d = 1; d < blocks; d++) pos.push_back(r.field());
   for (uint64_t d = 1; d < blocks; d++) neg.push_back(r.field());
@@ -624,10 +624,10 @@ void verify_batch(const ParamsState& p, const std::vector<Digest>& roots, const 
   fw[0] = e8_one();
   E8 oinv = e8_inv(omega);
   for (uint64_t b = 1; b < blocks; b++) fw[b] = fw[b - 1] * oinv;
-  std::array<E8, 8> coefs;
-  for (int img = 0; img < 8; img++) {
+  Lanes coefs;
+  for (int img = 0; img < LANES; img++) {
     E8 s{}, f = e8_one();
-    for (uint64_t b = 0; b < blocks; b++) { s += f * e8_frob(high[b], (8 - img) % 8); f = f * omega; }
+    for (uint64_t b = 0; b < blocks; b++) { s += f * e8_frob(high[b], (LANES - img) % LANES); f = f * omega; }
     coefs[img] = powers[img] * s;
   }
   Digest folded = r.digest();
@@ -746,7 +746,7 @@ void verify(const ParamsState& p, const std::vector<Digest>& roots, const std::v
   }
   std::reverse(point.begin(), point.end());
   Bytes context;
-  const char* cx = "parcomm/qafold/terminal/v2";
+  const char* cx = TERMINAL_DOMAIN;
   context.insert(context.end(), cx, cx + std::strlen(cx));
   context.insert(context.end(), tr.state.begin(), tr.state.end());
   put_u64(context, 0);

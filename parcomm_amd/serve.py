@@ -17,15 +17,13 @@ from typing import List
 
 import numpy as np
 
-from . import Claim, _check, lib
-
-P = 65521
+from . import Claim, _check, field_info, lib, QF_FIELD_F65521
 
 
-def device_memory():
+def device_memory(field=QF_FIELD_F65521):
     """(free, total) bytes of the current device."""
     free, total = ctypes.c_size_t(), ctypes.c_size_t()
-    _check(lib().qf_device_memory(ctypes.byref(free), ctypes.byref(total)))
+    _check(lib(field).qf_device_memory(ctypes.byref(free), ctypes.byref(total)), field)
     return free.value, total.value
 
 
@@ -35,14 +33,16 @@ def request_bytes(params):
     the batched folded codeword and tree, and the sumcheck vectors."""
     s = 1 << params.config.packed_log
     eo = 3 * (params.logical // s) * 16
-    n_values = params.block_len // 8
+    n_values = params.block_len // field_info(params.field)["lanes"]
     folded = 2 * n_values * 16 + 2 * (n_values // 16) * 32
     return eo + folded + 2 * n_values * 16
 
 
-def _element(rng):
-    c = rng.integers(0, P, size=8, dtype=np.uint16)
-    return c.astype("<u2").tobytes()
+def _element(rng, field):
+    """A uniform extension element in its 16-byte canonical form."""
+    info = field_info(field)
+    c = rng.integers(0, info["p"], size=info["lanes"], dtype=np.uint64)
+    return c.astype(info["coef"]).tobytes()
 
 
 @dataclass
@@ -61,7 +61,7 @@ def request_claims(params, committed, base_claims, subset, rng):
     chosen = set(subset)
     for m, c in enumerate(base_claims):
         if m in chosen:
-            points.append(([_element(rng) for _ in c.row], [_element(rng) for _ in c.col]))
+            points.append(([_element(rng, params.field) for _ in c.row], [_element(rng, params.field) for _ in c.col]))
         else:
             points.append((list(c.row), list(c.col)))
     return params.evaluate_claims(committed, points)
@@ -75,7 +75,7 @@ def serve(params, committed, base_claims, n_requests=32, rate=50.0, seed=1, work
     reqs = [Request(i, 0.0, sorted(rng.choice(len(base_claims), size=int(k), replace=False).tolist()))
             for i, k in enumerate(sizes)]
     per = request_bytes(params)
-    free, _ = device_memory()
+    free, _ = device_memory(params.field)
     budget_slots = max(1, int((free * 0.9) // per))
     slots = max(1, min(workers or 4, budget_slots))
     admit = threading.Semaphore(slots)

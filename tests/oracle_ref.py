@@ -7,19 +7,23 @@ import numpy as np
 DEMO_SHAPES = [(31, 37), (65, 17), (1, 7), (9, 1)]
 
 
-def demo_bva(layout_encode):
-    """Demo matrices of the reference CLI: entry e of matrix i = (e*73 + i*19) as u8 as i8
-    (qafold/f65521/src/main.rs:133-144)."""
+def demo_bva(layout_encode, kb=False):
+    """Demo matrices of the reference CLI: entry e of matrix i = (e*73 + i*19)
+    as u8 as i8 (qafold/f65521/src/main.rs:133-144), or as u16 as i16 for the
+    KoalaBear crate (qafold/koalabear/src/main.rs:137-148)."""
     mats = []
     for i, (r, c) in enumerate(DEMO_SHAPES):
         e = np.arange(r * c, dtype=np.int64)
-        mats.append(((e * 73 + i * 19) % 256).astype(np.uint8).view(np.int8))
+        if kb:
+            mats.append(((e * 73 + i * 19) % 65536).astype(np.uint16).view(np.int16))
+        else:
+            mats.append(((e * 73 + i * 19) % 256).astype(np.uint8).view(np.int8))
     return layout_encode(mats)
 
 
 def layout_encode(params, mats):
     """layout.rs:115-135 on the host from the product's bucket table."""
-    out = np.zeros(params.active, dtype=np.int8)
+    out = np.zeros(params.active, dtype=getattr(params, "entry_dtype", "int8"))
     for (m, ro, co, rows, cols, pr, pc, off) in params.buckets():
         src = mats[m].reshape(params.shapes[m].rows, params.shapes[m].cols)
         block = src[ro:ro + pr, co:co + pc]
@@ -29,6 +33,10 @@ def layout_encode(params, mats):
 
 
 def oracle_commit(oracle, chunk_log, values, blocks, want_codeword=False):
+    """Per-block roots (and codewords) of the oracle; the entry type of `values`
+    selects the field (int8: F65521 oracle, int16: KoalaBear oracle)."""
+    kb = values.dtype == np.int16
+    lanes, coef = (4, np.uint32) if kb else (8, np.uint16)
     h = oracle.orc_pcs_new(chunk_log, 100, bytes([0x51] * 32))
     assert h, oracle.orc_last_error()
     try:
@@ -37,12 +45,12 @@ def oracle_commit(oracle, chunk_log, values, blocks, want_codeword=False):
         for b in range(blocks):
             first = b * block_len
             avail = max(0, min(block_len, len(values) - first))
-            chunk = np.zeros(block_len, dtype=np.int8)
+            chunk = np.zeros(block_len, dtype=values.dtype)
             chunk[:avail] = values[first:first + avail]
             root = (ctypes.c_uint8 * 32)()
             cw = None
             if want_codeword:
-                cw = np.empty((32, 2 * (block_len // 256), 8), dtype=np.uint16)
+                cw = np.empty((32, 2 * (block_len // (32 * lanes)), lanes), dtype=coef)
             rc = oracle.orc_commit_block(ctypes.c_void_p(h), chunk.ctypes.data_as(ctypes.c_void_p),
                                          ctypes.c_uint64(block_len), root, None,
                                          cw.ctypes.data_as(ctypes.c_void_p) if cw is not None else None)
@@ -58,6 +66,23 @@ def synthetic_int8(shape_count, seed, sigma=20.0):
     """round(N(0, sigma)) clipped to [-128, 127], seeded."""
     rng = np.random.default_rng(seed)
     return np.clip(np.rint(rng.normal(0.0, sigma, size=shape_count)), -128, 127).astype(np.int8)
+
+
+def synthetic_bf16(shape_count, seed, sigma=0.02):
+    """Raw bfloat16 bit patterns (as int16) of N(0, sigma) weights, seeded;
+    round to nearest even from float32."""
+    rng = np.random.default_rng(seed)
+    b = rng.normal(0.0, sigma, size=shape_count).astype(np.float32).view(np.uint32).astype(np.uint64)
+    r = (b + 0x7FFF + ((b >> 16) & 1)) >> 16
+    return r.astype(np.uint16).view(np.int16)
+
+
+def synthetic_int16(shape_count, seed):
+    """Uniform int16 entries over the full range (extreme values included), seeded."""
+    rng = np.random.default_rng(seed)
+    v = rng.integers(-32768, 32768, size=shape_count, dtype=np.int64)
+    v[: min(4, shape_count)] = [-32768, 32767, -1, 0][: min(4, shape_count)]
+    return v.astype(np.int16)
 
 
 def shapes_flat(shapes):

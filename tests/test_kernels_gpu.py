@@ -1,29 +1,33 @@
 """Kernel-level parity: the register-radix transform passes (qf_tile.hip)
-against numpy restatements of the same 2x2 tensor maps over F65521/Ext8."""
+against numpy restatements of the same 2x2 tensor maps, over F65521/Ext8
+(libparcomm_amd.so) and KoalaBear/Ext4 (libparcomm_amd_kb.so)."""
 import ctypes
 
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
-P = 65521
+# field -> (p, extension degree, X^degree wrap constant, coefficient dtype)
+FIELDS = {0: (65521, 8, 17, np.uint16), 1: (2130706433, 4, 3, np.uint32)}
+FIELD_IDS = {0: "f65521", 1: "koalabear"}
 
 
-def e8_mul(a, b):
-    """Rows of (..., 8) canonical coefficients: a * b in F_p[X]/(X^8 - 17)."""
+def e8_mul(a, b, field):
+    """Rows of (..., lanes) canonical coefficients: a * b in F_p[X]/(X^lanes - w)."""
+    P, lanes, wrap, _ = FIELDS[field]
     a = a.astype(np.int64)
     b = b.astype(np.int64)
-    t = np.zeros(a.shape[:-1] + (15,), dtype=np.int64)
-    for i in range(8):
-        for j in range(8):
+    t = np.zeros(a.shape[:-1] + (2 * lanes - 1,), dtype=np.int64)
+    for i in range(lanes):
+        for j in range(lanes):
             t[..., i + j] = (t[..., i + j] + a[..., i] * b[..., j]) % P
-    out = t[..., :8].copy()
-    out[..., :7] = (out[..., :7] + 17 * t[..., 8:]) % P
+    out = t[..., :lanes].copy()
+    out[..., :lanes - 1] = (out[..., :lanes - 1] + wrap * t[..., lanes:]) % P
     return out % P
 
 
-def transform(x, bits, wht):
-    """x: (vectors, n, 8); 2x2 maps on the given index bits."""
+def transform(x, bits, wht, P):
+    """x: (vectors, n, lanes); 2x2 maps on the given index bits."""
     y = x.astype(np.int64).copy()
     n = y.shape[1]
     idx = np.arange(n)
@@ -39,24 +43,26 @@ def transform(x, bits, wht):
     return y % P
 
 
-def run(mode, x, alpha, L, s, t, vstride_in=None, vstride_out=None, xvec=False):
+def run(field, mode, x, alpha, L, s, t, vstride_in=None, vstride_out=None, xvec=False):
     import parcomm_amd as pa
+    _, lanes, _, dt = FIELDS[field]
     vectors = x.shape[0]
     n = 1 << L
     vin = vstride_in or n
     vout = vstride_out or n
-    host_in = np.zeros((vectors * vin, 8), dtype=np.uint16)
+    host_in = np.zeros((vectors * vin, lanes), dtype=dt)
     for v in range(vectors):
         host_in[v * vin:v * vin + n] = x[v]
-    d_in = pa.DeviceBuffer.from_host(host_in)
-    d_o1 = pa.DeviceBuffer(vectors * vout * 16)
-    d_o2 = pa.DeviceBuffer(vectors * vout * 16)
-    d_a = pa.DeviceBuffer.from_host(alpha.astype(np.uint16)) if alpha is not None else None
-    rc = pa.lib().qf_test_tile_pass(mode, d_in.ptr, d_o1.ptr, d_o2.ptr, d_a.ptr if d_a else None, vectors, vin, vout,
-                                    L, s, t, int(xvec))
-    assert rc == 0, pa.lib().qf_last_error()
-    o1 = d_o1.to_host("uint16").reshape(vectors * vout, 8)
-    o2 = d_o2.to_host("uint16").reshape(vectors * vout, 8)
+    d_in = pa.DeviceBuffer.from_host(host_in, field)
+    d_o1 = pa.DeviceBuffer(vectors * vout * 16, field)
+    d_o2 = pa.DeviceBuffer(vectors * vout * 16, field)
+    d_a = pa.DeviceBuffer.from_host(alpha.astype(dt), field) if alpha is not None else None
+    L_ = pa.lib(field)
+    rc = L_.qf_test_tile_pass(mode, d_in.ptr, d_o1.ptr, d_o2.ptr, d_a.ptr if d_a else None, vectors, vin, vout,
+                              L, s, t, int(xvec))
+    assert rc == 0, L_.qf_last_error()
+    o1 = d_o1.to_host(dt).reshape(vectors * vout, lanes)
+    o2 = d_o2.to_host(dt).reshape(vectors * vout, lanes)
     take = lambda o: np.stack([o[v * vout:v * vout + n] for v in range(vectors)])
     return take(o1), take(o2)
 
@@ -65,36 +71,48 @@ GEOMS = [(32, 12, 0, 12), (2, 14, 12, 2), (1, 16, 12, 4), (16, 8, 0, 8), (1, 22,
          (32, 17, 12, 5), (2, 13, 0, 12)]
 
 
+def field_values(rng, P, shape):
+    """Uniform canonical coefficients with the extremes 0, 1, P - 1 planted."""
+    x = rng.integers(0, P, size=shape, dtype=np.int64)
+    flat = x.reshape(-1)
+    flat[:3] = [0, 1, P - 1]
+    flat[-3:] = [P - 1, P - 1, P - 2]
+    return x
+
+
+@pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
 @pytest.mark.parametrize("vectors,L,s,t", GEOMS)
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
-def test_tile_pass_matches_numpy(vectors, L, s, t, mode):
-    rng = np.random.default_rng(1000 * L + 10 * mode + s)
-    x = rng.integers(0, P, size=(vectors, 1 << L, 8), dtype=np.int64)
-    alpha = rng.integers(0, P, size=(1 << L, 8), dtype=np.int64)
-    print("tile pass mode", mode, "vectors", vectors, "L", L, "bits", s, s + t, flush=True)
-    o1, o2 = run(mode, x, alpha if mode in (3, 5) else None, L, s, t, vstride_in=(1 << L) + 64 if L < 20 else None)
+def test_tile_pass_matches_numpy(vectors, L, s, t, mode, field):
+    P, lanes, _, _ = FIELDS[field]
+    rng = np.random.default_rng(1000 * L + 10 * mode + s + 7 * field)
+    x = field_values(rng, P, (vectors, 1 << L, lanes))
+    alpha = field_values(rng, P, (1 << L, lanes))
+    print("tile pass", FIELD_IDS[field], "mode", mode, "vectors", vectors, "L", L, "bits", s, s + t, flush=True)
+    o1, o2 = run(field, mode, x, alpha if mode in (3, 5) else None, L, s, t,
+                 vstride_in=(1 << L) + 64 if L < 20 else None)
     bits = list(range(s, s + t))
     if mode == 0:
-        assert np.array_equal(o1, transform(x, bits, False))
+        assert np.array_equal(o1, transform(x, bits, False, P))
     elif mode == 1:
-        m = transform(x, bits, False)
+        m = transform(x, bits, False, P)
         assert np.array_equal(o1, m)
-        assert np.array_equal(o2, transform(m, bits, True))
+        assert np.array_equal(o2, transform(m, bits, True, P))
     elif mode == 2:
-        assert np.array_equal(o1, transform(x, bits, True))
+        assert np.array_equal(o1, transform(x, bits, True, P))
     elif mode == 3:
-        w = transform(x, bits, True)
-        assert np.array_equal(o1, transform(e8_mul(w, alpha[None]), bits, True))
+        w = transform(x, bits, True, P)
+        assert np.array_equal(o1, transform(e8_mul(w, alpha[None], field), bits, True, P))
     elif mode == 4:
         assert np.array_equal(o1, x)
-        assert np.array_equal(o2, transform(x, bits, True))
+        assert np.array_equal(o2, transform(x, bits, True, P))
     else:
-        w = transform(x, bits, True)
+        w = transform(x, bits, True, P)
         assert np.array_equal(o1, x)
-        assert np.array_equal(o2, transform(e8_mul(w, alpha[None]), bits, True))
+        assert np.array_equal(o2, transform(e8_mul(w, alpha[None], field), bits, True, P))
 
 
-def transform_vectors(x, nbits, wht):
+def transform_vectors(x, nbits, wht, P):
     """2x2 maps on the low nbits of the vector index (axis 0)."""
     y = x.astype(np.int64).copy()
     idx = np.arange(y.shape[0])
@@ -110,14 +128,16 @@ def transform_vectors(x, nbits, wht):
     return y % P
 
 
+@pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
 @pytest.mark.parametrize("L", [12, 15])
 @pytest.mark.parametrize("mode", [0, 2])
-def test_tile_pass_over_column_bits(L, mode):
+def test_tile_pass_over_column_bits(L, mode, field):
     # 32 columns x 128 rows per tile: the first commit pass (column bits + row bits 0..6)
-    rng = np.random.default_rng(77 + L + mode)
-    x = rng.integers(0, P, size=(32, 1 << L, 8), dtype=np.int64)
-    print("column-bit tile pass mode", mode, "L", L, flush=True)
-    o1, _ = run(mode, x, None, L, 0, 7, vstride_in=(1 << L) + 16, vstride_out=2 << L, xvec=True)
+    P, lanes, _, _ = FIELDS[field]
+    rng = np.random.default_rng(77 + L + mode + field)
+    x = field_values(rng, P, (32, 1 << L, lanes))
+    print("column-bit tile pass", FIELD_IDS[field], "mode", mode, "L", L, flush=True)
+    o1, _ = run(field, mode, x, None, L, 0, 7, vstride_in=(1 << L) + 16, vstride_out=2 << L, xvec=True)
     wht = mode == 2
-    assert np.array_equal(o1, transform_vectors(transform(x, range(7), wht), 5, wht))
+    assert np.array_equal(o1, transform_vectors(transform(x, range(7), wht, P), 5, wht, P))
 
