@@ -1,5 +1,6 @@
 """Benchmark: QAFold F65521 (8-bit) prover, commit + open, on Llama-3.1-8B-shaped
-synthetic int8 weights.
+synthetic int8 weights; the KoalaBear (16-bit) prover on the same shapes is
+measured beside it.
 
 Workload: 32 layers x {q 4096x4096, k 1024x4096, v 1024x4096, o 4096x4096,
 gate 14336x4096, up 14336x4096, down 4096x14336} + embed and lm_head
@@ -15,6 +16,11 @@ qf_synthetic_bva) and stays resident in HBM, so the timed region starts with
 inputs in HBM. Every prover call returns after its stream has drained, so the
 host clock brackets completed device work. Setup (Parameters::new) is outside
 the timed region, as in the reference.
+
+The KoalaBear line (`config.koalabear`, or the whole line with --field
+koalabear) runs the same step on the same 226 shapes with i16 entries that
+are the raw bfloat16 bit patterns of N(0, 0.02) weights, chunk log 30, packed
+log 4, all matrices batched per phase.
 
 Prints one JSON line (see `main`).
 """
@@ -36,61 +42,55 @@ def llama8b_shapes():
     return [(128256, 4096), (128256, 4096)] + layer * 32
 
 
-def cpu_baseline(seed=5):
+def cpu_baseline(seed=5, kb=False):
     """The CPU oracle (a parity-checked restatement, not the reference binary)
     running the whole CLI flow (commit, benchmark claims, prove) on a bounded
-    sample of the workload: two 4096x4096 int8 matrices (2^25 bytes, two PCS
+    sample of the workload: two 4096x4096 matrices (2^25 entries, two PCS
     blocks of 2^24, so the batched opening path runs as in the 8B case) on the
-    host cores. Scaled to the 2^33-entry 8B BVA by byte count."""
-    from tests.conftest import load_oracle
-    from tests.oracle_ref import oracle_prove, shapes_flat, synthetic_int8
+    host cores. Scaled to the 2^33-entry 8B BVA by entry count."""
+    from tests.conftest import ORACLE_KB_LIB, load_oracle
+    from tests.oracle_ref import oracle_prove, shapes_flat, synthetic_bf16, synthetic_int8
     import numpy as np
-    L = load_oracle()
+    L = load_oracle(ORACLE_KB_LIB) if kb else load_oracle()
+    make = synthetic_bf16 if kb else synthetic_int8
     shapes = [(4096, 4096), (4096, 4096)]
-    values = np.concatenate([synthetic_int8(4096 * 4096, seed + i) for i in range(2)])
+    values = np.concatenate([make(4096 * 4096, seed + i) for i in range(2)])
     t0 = time.perf_counter()
     oracle_prove(L, shapes_flat(shapes), len(shapes), 24, 4, values)
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     scale = (1 << 33) // len(values)
+    kind = "bf16-pattern int16" if kb else "int8"
     return {
         "value": round(dt * scale, 2),
         "unit": "s (commit+open, extrapolated to the 8B BVA)",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle commit+claims+prove of 2 x 4096x4096 int8 (2^25 B, chunk_log 24, 2 blocks) "
-                  f"in {dt:.2f} s on {threads} threads, x{scale} by bytes",
+        "sample": f"oracle commit+claims+prove of 2 x 4096x4096 {kind} (2^25 entries, chunk_log 24, 2 blocks) "
+                  f"in {dt:.2f} s on {threads} threads, x{scale} by entries",
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--chunk-log", type=int, default=30)
-    ap.add_argument("--seed", type=int, default=20260)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=32)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is designed, not built"}))
-        sys.exit(2)
-    import parcomm_amd as pa
-
+def measure(pa, field, args, per_matrix=True, serve_requests=0):
+    """Setup, resident synthetic BVA, warmup + timed commit+open steps, an
+    untimed verify, optionally the per-matrix form and the serving loop."""
+    kb = field == pa.KOALABEAR
+    tag = "koalabear" if kb else "f65521"
     shapes = llama8b_shapes()
     t0 = time.perf_counter()
-    params = pa.Parameters([pa.Shape(r, c) for r, c in shapes], pa.Config(chunk_log=args.chunk_log, packed_log=4))
+    params = pa.Parameters([pa.Shape(r, c) for r, c in shapes], pa.Config(chunk_log=args.chunk_log, packed_log=4),
+                           field=field)
     setup_s = time.perf_counter() - t0
-    print(f"# setup {setup_s:.2f} s, active {params.active}, blocks {params.blocks}", file=sys.stderr, flush=True)
-    bva = params.synthetic_bva(args.seed, 20.0)
-    print("# bva resident", file=sys.stderr, flush=True)
+    print(f"# {tag}: setup {setup_s:.2f} s, active {params.active}, blocks {params.blocks}", file=sys.stderr,
+          flush=True)
+    bva = params.synthetic_bva(args.seed)
+    print(f"# {tag}: bva resident", file=sys.stderr, flush=True)
 
     committed = params.commit(bva)
     claims = params.benchmark_claims(committed)
     roots = committed.statement.roots
     committed.close()
-    print("# claims ready", file=sys.stderr, flush=True)
+    print(f"# {tag}: claims ready", file=sys.stderr, flush=True)
 
     def step():
         t0 = time.perf_counter()
@@ -105,19 +105,88 @@ def main():
 
     for i in range(args.warmup):
         r = step()
-        print(f"# warmup {i}: commit {r['commit_s']:.3f} s open {r['open_s']:.3f} s", file=sys.stderr, flush=True)
+        print(f"# {tag} warmup {i}: commit {r['commit_s']:.3f} s open {r['open_s']:.3f} s", file=sys.stderr,
+              flush=True)
     t0 = time.perf_counter()
     timings = []
     for i in range(args.steps):
         timings.append(step())
         r = timings[-1]
-        print(f"# step {i}: commit {r['commit_s']:.3f} s open {r['open_s']:.3f} s "
+        print(f"# {tag} step {i}: commit {r['commit_s']:.3f} s open {r['open_s']:.3f} s "
               f"profile {json.dumps({k: round(v, 4) for k, v in r['profile'].items()})}", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
-    s_per_step = elapsed / args.steps
+    res = {"params": params, "timings": timings, "s_per_step": elapsed / args.steps, "setup_s": setup_s}
 
+    # the witness-free verifier on the last proof (host, untimed; reference: 0.157 s on one thread)
+    c = params.commit(bva)
+    proof, _ = params.prove(c, claims)
+    tv = time.perf_counter()
+    params.verify(c.statement, claims, proof)
+    res["verify_s"] = time.perf_counter() - tv
+    c.close()
+    print(f"# {tag}: proof verified in {res['verify_s']:.3f} s", file=sys.stderr, flush=True)
+
+    if per_matrix:  # the plain per-matrix GPU form: one launch and host round trip per bucket term
+        pa.set_launch_mode(pa.LAUNCH_PER_MATRIX, field)
+        try:
+            pm = step()
+        finally:
+            pa.set_launch_mode(pa.LAUNCH_BATCHED, field)
+        res["per_matrix_s"] = pm["commit_s"] + pm["open_s"]
+        print(f"# {tag} per-matrix form: commit {pm['commit_s']:.3f} s open {pm['open_s']:.3f} s", file=sys.stderr,
+              flush=True)
+    if serve_requests:  # resident commitment, Poisson opening requests (mean 50/s), concurrent workers
+        from parcomm_amd.serve import serve
+        c = params.commit(bva)
+        res["serving"], _ = serve(params, c, claims, n_requests=serve_requests, rate=50.0, seed=args.seed,
+                                  workers=4)
+        c.close()
+        print(f"# {tag} serving {res['serving']}", file=sys.stderr, flush=True)
+    bva.close()
+    return res
+
+
+def summary(res):
+    timings = res["timings"]
+    mean = lambda k: sum(t[k] for t in timings) / len(timings)
+    phases = {k: round(sum(t["profile"][k] for t in timings) / len(timings), 4) for k in timings[0]["profile"]}
+    out = {"s_per_step": round(res["s_per_step"], 4), "setup_s": round(res["setup_s"], 2),
+           "commit_s": round(mean("commit_s"), 4), "open_s": round(mean("open_s"), 4), "open_phases_s": phases,
+           "proof_bytes": timings[0]["proof_bytes"], "verified": True, "verify_s": round(res["verify_s"], 4)}
+    if "per_matrix_s" in res:
+        out["per_matrix_s"] = round(res["per_matrix_s"], 4)
+        out["batched_speedup"] = round(res["per_matrix_s"] / (mean("commit_s") + mean("open_s")), 3)
+    if "serving" in res:
+        out["serving"] = res["serving"]
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--chunk-log", type=int, default=30)
+    ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--field", choices=["f65521", "koalabear"], default="f65521",
+                    help="headline backend; the default run also measures KoalaBear beside it")
+    ap.add_argument("--no-koalabear", action="store_true", help="skip the KoalaBear line of the default run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is designed, not built"}))
+        sys.exit(2)
+    import parcomm_amd as pa
+
+    kb_head = args.field == "koalabear"
+    field = pa.KOALABEAR if kb_head else pa.F65521
+    res = measure(pa, field, args, per_matrix=True, serve_requests=args.serve_requests)
+    params, timings = res["params"], res["timings"]
+    s_per_step = res["s_per_step"]
     blocks = params.blocks
-    R = 2 * (params.block_len // 256)
+    lanes = pa.field_info(field)["lanes"]
+    R = 2 * (params.block_len // (32 * lanes))
     leaf_ms = sum(t["commit"]["leaf_ms"] for t in timings) / (len(timings) * blocks)
     leaf_bytes = R * 32 * 16 + R * 32  # read the codeword rows, write the leaves
     roofline = None
@@ -125,34 +194,26 @@ def main():
         achieved = leaf_bytes / (leaf_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_leaf_hash"}
-    # the witness-free verifier on the last proof (host, untimed; reference: 0.157 s on one thread)
-    c = params.commit(bva)
-    proof, _ = params.prove(c, claims)
-    tv = time.perf_counter()
-    params.verify(c.statement, claims, proof)
-    verify_s = time.perf_counter() - tv
-    c.close()
-    print(f"# proof verified in {verify_s:.3f} s", file=sys.stderr, flush=True)
-
-    # the plain per-matrix GPU form: one launch and host round trip per bucket term
-    pa.set_launch_mode(pa.LAUNCH_PER_MATRIX)
-    try:
-        pm = step()
-    finally:
-        pa.set_launch_mode(pa.LAUNCH_BATCHED)
-    per_matrix_s = pm["commit_s"] + pm["open_s"]
-    print(f"# per-matrix form: commit {pm['commit_s']:.3f} s open {pm['open_s']:.3f} s", file=sys.stderr, flush=True)
-
-    # serving: resident commitment, Poisson opening requests (mean 50/s), concurrent workers
-    from parcomm_amd.serve import serve
-    c = params.commit(bva)
-    serving, _ = serve(params, c, claims, n_requests=args.serve_requests, rate=50.0, seed=args.seed, workers=4)
-    c.close()
-    print(f"# serving {serving}", file=sys.stderr, flush=True)
-    mean = lambda k: sum(t[k] for t in timings) / len(timings)
-    phases = {k: round(sum(t["profile"][k] for t in timings) / len(timings), 4) for k in timings[0]["profile"]}
+    summ = summary(res)
+    if kb_head:
+        workload = "llama3.1-8b-shaped bf16-pattern int16 BVA (KoalaBear), 226 matrices, chunk_log 30, packed_log 4"
+        dtype = "u32 (KoalaBear / Ext4 exact integer)"
+        metric = "8B-16bit KoalaBear commit+open prover s (batched)"
+    else:
+        workload = "llama3.1-8b-shaped int8 BVA, 226 matrices, chunk_log 30, packed_log 4"
+        dtype = "u16 (F65521 / Ext8 exact integer)"
+        metric = "8B-8bit commit+open prover s: batched vs per-matrix GPU; p99 open latency"
+    config = {"workload": workload, "active_entries": params.active, "blocks": blocks,
+              "phases_timed": ["commit", "open"], "cache_mode": "cached"}
+    config.update({k: v for k, v in summ.items() if k != "s_per_step"})
+    del res
+    if not kb_head and not args.no_koalabear:
+        kb = summary(measure(pa, pa.KOALABEAR, args, per_matrix=False, serve_requests=0))
+        if not args.no_cpu_baseline:
+            kb["cpu_baseline"] = cpu_baseline(kb=True)
+        config["koalabear"] = kb
     out = {
-        "metric": "8B-8bit commit+open prover s: batched vs per-matrix GPU; p99 open latency",
+        "metric": metric,
         "value": round(s_per_step, 4),
         "unit": "s",
         "n_gpus": 1,
@@ -162,21 +223,14 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16 (F65521 / Ext8 exact integer)",
+        "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": "llama3.1-8b-shaped int8 BVA, 226 matrices, chunk_log 30, packed_log 4",
-                   "active_entries": params.active, "blocks": blocks, "phases_timed": ["commit", "open"],
-                   "cache_mode": "cached", "setup_s": round(setup_s, 2), "commit_s": round(mean("commit_s"), 4),
-                   "open_s": round(mean("open_s"), 4), "open_phases_s": phases,
-                   "proof_bytes": timings[0]["proof_bytes"], "verified": True, "verify_s": round(verify_s, 4),
-                   "per_matrix_s": round(per_matrix_s, 4),
-                   "batched_speedup": round(per_matrix_s / (mean("commit_s") + mean("open_s")), 3),
-                   "serving": serving},
+        "config": config,
         "roofline": roofline,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(kb=kb_head)
     print(json.dumps(out), flush=True)
 
 

@@ -141,11 +141,27 @@ class DeviceBuffer:
         _check(buf._L.qf_device_upload(buf.ptr, arr.ctypes.data, arr.nbytes), field)
         return buf
 
-    def to_host(self, dtype="int8"):
+    def to_host(self, dtype="int8", offset=0, count=None):
+        """Copy `count` items of `dtype` from byte `offset` (default: all) to host."""
         import numpy as np
-        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
-        _check(self._L.qf_device_download(out.ctypes.data, self.ptr, out.nbytes), self.field)
+        item = np.dtype(dtype).itemsize
+        if count is None:
+            count = (self.nbytes - offset) // item
+        if offset < 0 or offset + count * item > self.nbytes:
+            raise ValueError("range outside the device buffer")
+        out = np.empty(count, dtype=dtype)
+        _check(self._L.qf_device_download(out.ctypes.data, ctypes.c_void_p(self.ptr.value + offset), out.nbytes),
+               self.field)
         return out
+
+    def upload(self, array, offset=0):
+        """Copy a host array into the buffer at byte `offset`."""
+        import numpy as np
+        arr = np.ascontiguousarray(array)
+        if offset < 0 or offset + arr.nbytes > self.nbytes:
+            raise ValueError("range outside the device buffer")
+        _check(self._L.qf_device_upload(ctypes.c_void_p(self.ptr.value + offset), arr.ctypes.data, arr.nbytes),
+               self.field)
 
     def close(self):
         if self.ptr:

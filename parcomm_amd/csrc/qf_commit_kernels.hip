@@ -304,20 +304,25 @@ static std::vector<std::pair<int, int>> hi_groups(int L) {
   return gs;
 }
 
-// Tile-pass schedules (qf_tile.hip); each returns false when the geometry needs
-// the generic LDS-stage passes instead (vectors shorter than a tile).
+// Tile-pass schedules (qf_tile.hip); each returns false when its first pass
+// declines the geometry (vectors shorter than a tile), and the caller then
+// runs the generic LDS-stage passes. Once a schedule has started, every later
+// pass must run: a pass that declines would leave the data half transformed.
+static void must(bool ran) { require(ran, QF_ERR_ARGUMENT, "tile pass declined a geometry mid-schedule"); }
 static bool tiled_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream_t st) {
   if (!tile_pass(TP_W, false, v, v, nullptr, nullptr, vectors, stride, stride, L, 0, std::min(L, 12), 0, 0, st))
     return false;
-  for (auto& g : hi_groups(L)) tile_pass(TP_W, false, v, v, nullptr, nullptr, vectors, stride, stride, L, g.first, g.second, 0, 0, st);
+  for (auto& g : hi_groups(L))
+    must(tile_pass(TP_W, false, v, v, nullptr, nullptr, vectors, stride, stride, L, g.first, g.second, 0, 0, st));
   return true;
 }
 // Second half of the parity, in place: WHT on the low bits, alpha, WHT on the
 // low bits (one pass), then WHT on the high groups. Precondition: the parity
 // region holds the message with the WHT already applied on every high group.
 static void tiled_parity_tail(E8* par, uint64_t cols, uint64_t R, int L, const E8* alpha, hipStream_t st) {
-  tile_pass(TP_WAW, false, par, par, nullptr, alpha, cols, R, R, L, 0, std::min(L, 12), 0, 0, st);
-  for (auto& g : hi_groups(L)) tile_pass(TP_W, false, par, par, nullptr, nullptr, cols, R, R, L, g.first, g.second, 0, 0, st);
+  must(tile_pass(TP_WAW, false, par, par, nullptr, alpha, cols, R, R, L, 0, std::min(L, 12), 0, 0, st));
+  for (auto& g : hi_groups(L))
+    must(tile_pass(TP_W, false, par, par, nullptr, nullptr, cols, R, R, L, g.first, g.second, 0, 0, st));
 }
 // Parity halves WHT(alpha . WHT(m)) from the message halves of `cols` columns.
 static bool tiled_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStream_t st) {
@@ -328,7 +333,7 @@ static bool tiled_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, h
   if (!tile_pass(TP_W, false, cw, cw + n, nullptr, nullptr, cols, R, R, L, gs[0].first, gs[0].second, 0, 0, st))
     return false;
   for (size_t i = 1; i < gs.size(); i++)
-    tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, cols, R, R, L, gs[i].first, gs[i].second, 0, 0, st);
+    must(tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, cols, R, R, L, gs[i].first, gs[i].second, 0, 0, st));
   tiled_parity_tail(cw + n, cols, R, L, alpha, st);
   return true;
 }
@@ -341,17 +346,20 @@ static bool tiled_commit_rows(bool pack, const void* src, uint64_t block_first, 
                               const E8* alpha, hipStream_t st) {
   if (L < 12) return false;
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  tile_pass(TP_M, pack, src, cw, nullptr, nullptr, 32, n, R, L, 0, 7, block_first, available, st, true);
-  tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, 7, 5, 0, 0, st);
+  if (!tile_pass(TP_M, pack, src, cw, nullptr, nullptr, 32, n, R, L, 0, 7, block_first, available, st, true))
+    return false;
+  must(tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, 7, 5, 0, 0, st));
   auto gs = hi_groups(L);
-  if (gs.empty())  // L = 12: message complete; one pass gives the parity
-    return tile_pass(TP_WAW, false, cw, cw + n, nullptr, alpha, 32, R, R, L, 0, 12, 0, 0, st);
+  if (gs.empty()) {  // L = 12: message complete; one pass gives the parity
+    must(tile_pass(TP_WAW, false, cw, cw + n, nullptr, alpha, 32, R, R, L, 0, 12, 0, 0, st));
+    return true;
+  }
   for (size_t i = 0; i + 1 < gs.size(); i++)
-    tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st);
+    must(tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st));
   // last Mobius group completes the message; the same pass starts the WHT in the parity half
-  tile_pass(TP_M_W, false, cw, cw, cw + n, nullptr, 32, R, R, L, gs.back().first, gs.back().second, 0, 0, st);
+  must(tile_pass(TP_M_W, false, cw, cw, cw + n, nullptr, 32, R, R, L, gs.back().first, gs.back().second, 0, 0, st));
   for (size_t i = 0; i + 1 < gs.size(); i++)
-    tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st);
+    must(tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st));
   tiled_parity_tail(cw + n, 32, R, L, alpha, st);
   return true;
 }
@@ -434,7 +442,7 @@ void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, c
     tiled = tile_pass(TP_COPY_W, false, values, cw, cw + n, nullptr, 32, n, R, L, gs[0].first, gs[0].second, 0, 0, st);
     if (tiled) {
       for (size_t i = 1; i < gs.size(); i++)
-        tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st);
+        must(tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st));
       tiled_parity_tail(cw + n, 32, R, L, alpha, st);
     }
   }

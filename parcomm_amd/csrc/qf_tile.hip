@@ -63,7 +63,15 @@ __device__ inline void tile_base(const TileArgs& a, uint64_t T, uint64_t& vec0, 
 }
 // Element offset of tile element e from element 0 for vector stride vs. The
 // map is additive over disjoint bit fields of e, so off(x + y) = off(x) + off(y)
-// whenever x and y share no set bits; offsets stay below 2^32 elements.
+// whenever x and y share no set bits. The 32-bit form serves the per-thread
+// parts, which the host bounds below 2^32 bytes; the uniform per-slot parts
+// use the 64-bit form.
+__host__ __device__ inline uint64_t tile_off64(const TileArgs& a, int e, uint64_t vs) {
+  const uint64_t lowc = uint64_t(e) & ((uint64_t(1) << a.logC) - 1);
+  const uint64_t h = (uint64_t(e) >> a.logC) & ((uint64_t(1) << a.t) - 1);
+  const uint64_t v = uint64_t(e) >> (a.logC + a.t);
+  return lowc + (h << a.s) + v * vs;
+}
 __device__ inline uint32_t tile_off(const TileArgs& a, int e, uint64_t vs) {
   const uint32_t lowc = uint32_t(e) & ((1u << a.logC) - 1);
   const uint32_t h = (uint32_t(e) >> a.logC) & ((1u << a.t) - 1);
@@ -248,12 +256,13 @@ __device__ inline void mul_alpha(int32_t (&a)[8], const uint4& bw) {
 #endif
 
 // Per-thread byte offsets of the mapping-A slots from the tile's first
-// element: slot q = thread part + q part (the q part is uniform). 32-bit
-// offsets against a uniform base keep addresses out of 64-bit registers.
+// element: slot q = thread part + q part. The q part is uniform (64-bit
+// scalar arithmetic); the thread part is a 32-bit offset against that uniform
+// base, which keeps per-slot addresses out of 64-bit vector registers.
 struct SlotOffs {
   uint32_t in_t, out_t;
 };
-__device__ inline uint32_t q_part(const TileArgs& a, int q, uint64_t vs) { return tile_off(a, TWG * q, vs); }
+__device__ inline uint64_t q_part(const TileArgs& a, int q, uint64_t vs) { return tile_off64(a, TWG * q, vs); }
 
 template <bool PACK>
 __device__ inline void load_tile(const TileArgs& a, uint64_t T, const SlotOffs& so, uint4 (&raw)[TQ]) {
@@ -264,11 +273,11 @@ __device__ inline void load_tile(const TileArgs& a, uint64_t T, const SlotOffs& 
     const char* base = static_cast<const char*>(a.in) + b0;
 #pragma unroll
     for (int q = 0; q < TQ; q++) {
-      const uint32_t off = so.in_t + 8u * q_part(a, q, a.pack_n);
-      const uint64_t byte = b0 + off;
+      const char* bq = base + 8 * q_part(a, q, a.pack_n);
+      const uint64_t byte = uint64_t(bq - static_cast<const char*>(a.in)) + so.in_t;
       uint2 w;
       if (byte + 8 <= a.available) {
-        w = *reinterpret_cast<const uint2*>(base + off);
+        w = *reinterpret_cast<const uint2*>(bq + so.in_t);
       } else {  // bytes at or past the active end read as zero
         const uint8_t* bva = static_cast<const uint8_t*>(a.in);
         w = make_uint2(0, 0);
@@ -284,7 +293,7 @@ __device__ inline void load_tile(const TileArgs& a, uint64_t T, const SlotOffs& 
     const char* base = reinterpret_cast<const char*>(static_cast<const E8*>(a.in) + vec0 * a.in_vstride + pos0);
 #pragma unroll
     for (int q = 0; q < TQ; q++)
-      raw[q] = *reinterpret_cast<const uint4*>(base + so.in_t + 16u * q_part(a, q, a.in_vstride));
+      raw[q] = *reinterpret_cast<const uint4*>(base + 16 * q_part(a, q, a.in_vstride) + so.in_t);
   }
 }
 
@@ -294,7 +303,7 @@ __device__ inline void store_tile(const TileArgs& a, uint64_t T, const SlotOffs&
   char* base = reinterpret_cast<char*>(out + vec0 * a.out_vstride + pos0);
 #pragma unroll
   for (int q = 0; q < TQ; q++)
-    *reinterpret_cast<uint4*>(base + so.out_t + 16u * q_part(a, q, a.out_vstride)) = pack_e8(x[q]);
+    *reinterpret_cast<uint4*>(base + 16 * q_part(a, q, a.out_vstride) + so.out_t) = pack_e8(x[q]);
 }
 
 template <int MODE, bool PACK, int LO>
@@ -428,10 +437,11 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
     a.log_lo_chunks = s - a.logC;
   }
   a.log_hi_count = L - s - t;
-  // byte offsets inside a tile must fit 32 bits
-  const uint64_t span_pos = (uint64_t(1) << L) * 16;
-  const uint64_t vmax = (uint64_t(1) << a.logV) - 1;
-  if (vmax * std::max(in_vstride * (pack ? 8 : 16), out_vstride * 16) + span_pos >= (1ull << 32)) return false;
+  // the per-thread byte offsets (tile bits 0..8 in mapping A, largest at
+  // element 511 since the offset map is additive) must fit 32 bits
+  const uint64_t thread_max = std::max(tile_off64(a, TWG - 1, in_vstride) * (pack ? 8 : 16),
+                                       tile_off64(a, TWG - 1, out_vstride) * 16);
+  if (thread_max + 16 >= (1ull << 32)) return false;
   a.xlo = a.logC;
   a.xhi = xform_vectors ? 12 : a.logC + a.t;
   const uint64_t tiles = (vectors << L) / TN;

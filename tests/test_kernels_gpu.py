@@ -68,7 +68,7 @@ def run(field, mode, x, alpha, L, s, t, vstride_in=None, vstride_out=None, xvec=
 
 
 GEOMS = [(32, 12, 0, 12), (2, 14, 12, 2), (1, 16, 12, 4), (16, 8, 0, 8), (1, 22, 12, 10), (1, 20, 12, 8),
-         (32, 17, 12, 5), (2, 13, 0, 12)]
+         (32, 17, 12, 5), (2, 13, 0, 12), (1, 23, 22, 1)]
 
 
 def field_values(rng, P, shape):
@@ -141,3 +141,30 @@ def test_tile_pass_over_column_bits(L, mode, field):
     wht = mode == 2
     assert np.array_equal(o1, transform_vectors(transform(x, range(7), wht, P), 5, wht, P))
 
+
+
+@pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
+@pytest.mark.parametrize("mode", [0, 2])
+def test_column_bit_pass_with_wide_vector_stride(mode, field):
+    # The first commit pass of a 2^30-entry KoalaBear block: 32 codeword
+    # columns 2^24 elements (256 MiB) apart, so a tile spans 8 GiB and the
+    # per-slot offsets need 64 bits. Only the first 2^12 elements of every
+    # column are transformed and copied back.
+    import parcomm_amd as pa
+    P, lanes, _, dt = FIELDS[field]
+    L, vs = 12, 1 << 24
+    rng = np.random.default_rng(91 + mode + field)
+    x = field_values(rng, P, (32, 1 << L, lanes))
+    d_in = pa.DeviceBuffer(32 * vs * 16, field)
+    d_o1 = pa.DeviceBuffer(32 * vs * 16, field)
+    for v in range(32):
+        d_in.upload(x[v].astype(dt), v * vs * 16)
+    print("wide-stride column-bit pass", FIELD_IDS[field], "mode", mode, flush=True)
+    L_ = pa.lib(field)
+    rc = L_.qf_test_tile_pass(mode, d_in.ptr, d_o1.ptr, None, None, 32, vs, vs, L, 0, 7, 1)
+    assert rc == 0, L_.qf_last_error()
+    o1 = np.stack([d_o1.to_host(dt, v * vs * 16, (1 << L) * lanes).reshape(1 << L, lanes) for v in range(32)])
+    wht = mode == 2
+    assert np.array_equal(o1, transform_vectors(transform(x, range(7), wht, P), 5, wht, P))
+    d_in.close()
+    d_o1.close()
