@@ -14,21 +14,27 @@ def declared_symbols():
     return sorted(set(re.findall(r"\b(qf_[a-z_]+)\s*\(", text)))
 
 
-def test_every_declared_symbol_is_exported(product):
+@pytest.mark.parametrize("field", [0, 1], ids=["f65521", "koalabear"])
+def test_every_declared_symbol_is_exported(field):
+    import parcomm_amd as pa
+    product = pa.lib(field)
     names = declared_symbols()
     assert len(names) >= 10
     for name in names:
         assert hasattr(product, name), name
+    assert product.qf_field() == field
 
 
+@pytest.mark.parametrize("field", [0, 1], ids=["f65521", "koalabear"])
 @pytest.mark.parametrize("chunk_log", [10, 12, 14, 17])
-def test_parameter_digest_matches_oracle(oracle, chunk_log):
+def test_parameter_digest_matches_oracle(oracle, oracle_kb, chunk_log, field):
     import parcomm_amd as pa
-    h = oracle.orc_pcs_new(chunk_log, 100, bytes([0x51] * 32))
+    orc = oracle_kb if field else oracle
+    h = orc.orc_pcs_new(chunk_log, 100, bytes([0x51] * 32))
     d = (ctypes.c_uint8 * 32)()
-    oracle.orc_pcs_parameter_digest(ctypes.c_void_p(h), d)
-    oracle.orc_pcs_free(h)
-    p = pa.Parameters([pa.Shape(1, 1 << chunk_log)], pa.Config(chunk_log=chunk_log))
+    orc.orc_pcs_parameter_digest(ctypes.c_void_p(h), d)
+    orc.orc_pcs_free(h)
+    p = pa.Parameters([pa.Shape(1, 1 << chunk_log)], pa.Config(chunk_log=chunk_log), field=field)
     assert p.parameter_digest() == bytes(d)
 
 
