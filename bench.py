@@ -173,6 +173,7 @@ def main():
     ap.add_argument("--field", choices=["f65521", "koalabear"], default="f65521",
                     help="headline backend; the default run also measures KoalaBear beside it")
     ap.add_argument("--no-koalabear", action="store_true", help="skip the KoalaBear line of the default run")
+    ap.add_argument("--no-per-matrix", action="store_true", help="skip the per-matrix comparison step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is designed, not built"}))
@@ -181,7 +182,7 @@ def main():
 
     kb_head = args.field == "koalabear"
     field = pa.KOALABEAR if kb_head else pa.F65521
-    res = measure(pa, field, args, per_matrix=True, serve_requests=args.serve_requests)
+    res = measure(pa, field, args, per_matrix=not args.no_per_matrix, serve_requests=args.serve_requests)
     params, timings = res["params"], res["timings"]
     s_per_step = res["s_per_step"]
     blocks = params.blocks

@@ -162,8 +162,62 @@ __host__ __device__ inline V8 v8_mul(const V8& a, const V8& b) { return v8_mul_m
 __host__ __device__ inline V8 v8_encode_form(const V8& b) { return v8_to_mont(b); }
 __host__ __device__ inline V8 v8_mul_enc(const V8& a, const V8& b_enc) { return v8_mul_mont(a, b_enc); }
 #else
-// Product in F65521[X]/(X^8 - 17); 64-bit accumulation, one reduction per coefficient.
+#ifdef __HIP_DEVICE_COMPILE__
+// Device product in F65521[X]/(X^8 - 17). Coefficient i sums a_j E[7 + i - j]
+// over the sequence E = (17 b_1, ..., 17 b_7, b_0, ..., b_7), which folds the
+// X^8 = 17 wrap. a is split into 8-bit limbs; each v_dot2_u32_u16 adds two
+// limb x E products, so every partial sum of eight stays below 2^27.
+__device__ inline uint32_t dot2_u16(uint32_t a, uint32_t b, uint32_t c) {
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b), c, false);
+}
+__device__ inline uint32_t fold16(uint32_t u) {  // u < 2^31 -> (u & 0xffff) + 15 (u >> 16)
+  return u - uint32_t(__mul24(int32_t(u >> 16), int32_t(P)));
+}
+__device__ inline V8 v8_mul_dot2(const V8& a, const V8& b) {
+  uint32_t E[15];
+#pragma unroll
+  for (int i = 0; i < 8; i++) E[7 + i] = b.c[i];
+#pragma unroll
+  for (int j = 1; j < 8; j++) {
+    const uint32_t u = fold16(__umul24(b.c[j], 17u));  // < 2^16 + 255
+    const uint32_t w = u - P;
+    E[j - 1] = u < w ? u : w;
+  }
+  uint32_t D[15];  // D[m] = (E[m], E[m-1]) as a u16 pair
+#pragma unroll
+  for (int m = 1; m < 15; m++) D[m] = E[m] | (E[m - 1] << 16);
+  uint32_t lo[4], hi[4];
+#pragma unroll
+  for (int jp = 0; jp < 4; jp++) {
+    const uint32_t pr = a.c[2 * jp] | (a.c[2 * jp + 1] << 16);
+    lo[jp] = pr & 0x00ff00ffu;
+    hi[jp] = (pr >> 8) & 0x00ff00ffu;
+  }
+  V8 r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    uint32_t sl = 0, sh = 0;
+#pragma unroll
+    for (int jp = 0; jp < 4; jp++) {
+      sl = dot2_u16(lo[jp], D[7 + i - 2 * jp], sl);
+      sh = dot2_u16(hi[jp], D[7 + i - 2 * jp], sh);
+    }
+    uint32_t x = fold16(sh);  // < 2^16 + 15 * 2^11
+    x = (x << 8) + sl;        // < 2^25 + 2^27
+    x = fold16(fold16(x));    // < 2^16 + 15
+    const uint32_t w = x - P;
+    r.c[i] = x < w ? x : w;
+  }
+  return r;
+}
+#endif
+// Product in F65521[X]/(X^8 - 17): the dot2 form on the device, 64-bit
+// schoolbook sums with one reduction per coefficient on the host.
 __host__ __device__ inline V8 v8_mul(const V8& a, const V8& b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return v8_mul_dot2(a, b);
+#endif
   uint64_t t[15];
 #pragma unroll
   for (int k = 0; k < 15; k++) t[k] = 0;

@@ -680,21 +680,34 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   }
   E8 claim = q.claim, fwd = ch, bwd = inv;
   for (uint64_t d = 0; d + 1 < s; d++) { claim += o.pos[d] * fwd + o.neg[d] * bwd; fwd = fwd * ch; bwd = bwd * inv; }
-  require(dv::inner(c, w.p, qv.p, rows) == claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
+  // The consistency check shares one pass with the first round's cross terms,
+  // and every fold also returns the next round's cross terms (rows is a power of two).
+  E8 c0{}, c1{};
+  if (rows >= 2) {
+    auto first = dv::eo_first(c, w.p, qv.p, rows / 2);
+    require(first[0] == claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
+    c0 = first[1];
+    c1 = first[2];
+  } else {
+    require(dv::inner(c, w.p, qv.p, rows) == claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
+  }
   DVec w2(rows / 2 ? rows / 2 : 1), q2(rows / 2 ? rows / 2 : 1);
   DVec *cw = &w, *cq = &qv, *nw = &w2, *nq = &q2;
   uint64_t n = rows;
   while (n > 1) {
     uint64_t h = n / 2;
-    auto [c0, c1] = dv::eo_cross(c, cw->p, cq->p, h);
     tr.object("binary-cross", js_scalars({c0, c1}));
     E8 r = tr.fold_challenge(1), ri = e8_inv(r);
     claim += c0 * r + c1 * ri;
     challenges.push_back(r);
-    dv::eo_fold(c, cw->p, cq->p, nw->p, nq->p, h, ri, r);
+    o.rounds.push_back({c0, c1});
+    if (h >= 2) {
+      std::tie(c0, c1) = dv::eo_fold_cross(c, cw->p, cq->p, nw->p, nq->p, h, ri, r);
+    } else {
+      dv::eo_fold(c, cw->p, cq->p, nw->p, nq->p, h, ri, r);
+    }
     std::swap(cw, nw);
     std::swap(cq, nq);
-    o.rounds.push_back({c0, c1});
     n = h;
   }
   o.value = dv::download(c, cw->p, 1)[0];

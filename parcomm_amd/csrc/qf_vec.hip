@@ -1,6 +1,7 @@
 // Device vector operations and BVA-scale kernels of the opening (see qf_vec.hpp).
 #include "qf_vec.hpp"
 #include <atomic>
+#include <type_traits>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -47,6 +48,62 @@ __global__ __launch_bounds__(WG) void k_reduce(const E8* __restrict__ a, const E
   if (threadIdx.x < 2 * LANES) partials[blockIdx.x * 16 + threadIdx.x] = red[0][threadIdx.x];
 }
 
+// Block-wide sums of NS extension elements per thread; partials[group][32] u32.
+constexpr int PSTRIDE = 32;
+template <int NS>
+__device__ inline void block_sums(const V8 (&s)[NS], uint32_t* __restrict__ partials) {
+  __shared__ uint32_t red[WG][NS * LANES];
+#pragma unroll
+  for (int j = 0; j < NS; j++)
+#pragma unroll
+    for (int k = 0; k < LANES; k++) red[threadIdx.x][j * LANES + k] = s[j].c[k];
+  __syncthreads();
+  for (int w = WG / 2; w > 0; w /= 2) {
+    if (int(threadIdx.x) < w)
+      for (int k = 0; k < NS * LANES; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + w][k]);
+    __syncthreads();
+  }
+  if (threadIdx.x < NS * LANES) partials[blockIdx.x * PSTRIDE + threadIdx.x] = red[0][threadIdx.x];
+}
+
+// The packed reduction's consistency sum <w, q> together with the first
+// binary round's cross terms (eo.rs:135-150): one read of both vectors.
+__global__ __launch_bounds__(WG) void k_eo_first(const E8* __restrict__ w, const E8* __restrict__ q, uint64_t h,
+                                                 uint32_t* __restrict__ partials) {
+  V8 s[3] = {v8_zero(), v8_zero(), v8_zero()};
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
+    const V8 w0 = v8_load(w[2 * i]), w1 = v8_load(w[2 * i + 1]), q0 = v8_load(q[2 * i]), q1 = v8_load(q[2 * i + 1]);
+    s[0] = v8_add(s[0], v8_add(v8_mul(w0, q0), v8_mul(w1, q1)));
+    s[1] = v8_add(s[1], v8_mul(w0, q1));
+    s[2] = v8_add(s[2], v8_mul(w1, q0));
+  }
+  block_sums<3>(s, partials);
+}
+
+// One binary EO fold (eo.rs:57-79: w'[i] = w[2i] + w[2i+1] r^-1,
+// q'[i] = q[2i] + q[2i+1] r) of length-2h vectors into length-h outputs,
+// fused with the next round's cross terms over the outputs:
+// sum_j w'[2j] q'[2j+1] and sum_j w'[2j+1] q'[2j] (h even).
+__global__ __launch_bounds__(WG) void k_eo_fold_cross(const E8* __restrict__ w, const E8* __restrict__ q,
+                                                      E8* __restrict__ ow, E8* __restrict__ oq, uint64_t h, E8 ri,
+                                                      E8 r, uint32_t* __restrict__ partials) {
+  const V8 vri = v8_encode_form(v8_load(ri)), vr = v8_encode_form(v8_load(r));
+  V8 s[2] = {v8_zero(), v8_zero()};
+  for (uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x; j < h / 2; j += uint64_t(gridDim.x) * WG) {
+    const V8 a0 = v8_add(v8_load(w[4 * j]), v8_mul_enc(v8_load(w[4 * j + 1]), vri));
+    const V8 a1 = v8_add(v8_load(w[4 * j + 2]), v8_mul_enc(v8_load(w[4 * j + 3]), vri));
+    const V8 b0 = v8_add(v8_load(q[4 * j]), v8_mul_enc(v8_load(q[4 * j + 1]), vr));
+    const V8 b1 = v8_add(v8_load(q[4 * j + 2]), v8_mul_enc(v8_load(q[4 * j + 3]), vr));
+    ow[2 * j] = v8_store(a0);
+    ow[2 * j + 1] = v8_store(a1);
+    oq[2 * j] = v8_store(b0);
+    oq[2 * j + 1] = v8_store(b1);
+    s[0] = v8_add(s[0], v8_mul(a0, b1));
+    s[1] = v8_add(s[1], v8_mul(a1, b0));
+  }
+  block_sums<2>(s, partials);
+}
+
 uint32_t groups_for(uint64_t n) {
   uint64_t g = (n + WG - 1) / WG;
   if (g < 1) g = 1;
@@ -63,6 +120,21 @@ std::pair<E8, E8> finish_partials(DevCtx& c, uint32_t groups) {
       s1.c[k] = madd(s1.c[k], c.host_partials[g * 16 + LANES + k]);
     }
   return {v8_store(s0), v8_store(s1)};
+}
+
+// Sums of the first NS extension elements over `groups` partial rows (stride PSTRIDE).
+template <int NS>
+std::array<E8, NS> finish_sums(DevCtx& c, uint32_t groups) {
+  QF_HIP(hipMemcpyAsync(c.host_partials, c.partials, size_t(groups) * PSTRIDE * 4, hipMemcpyDeviceToHost, c.st));
+  c.sync();
+  std::array<E8, NS> out;
+  for (int j = 0; j < NS; j++) {
+    V8 s = v8_zero();
+    for (uint32_t g = 0; g < groups; g++)
+      for (int k = 0; k < LANES; k++) s.c[k] = madd(s.c[k], c.host_partials[g * PSTRIDE + j * LANES + k]);
+    out[j] = v8_store(s);
+  }
+  return out;
 }
 
 template <int MODE>
@@ -83,42 +155,44 @@ __global__ void k_tensor(E8* out, uint64_t n, const E8* hi, const E8* lo, int a)
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
     out[i] = v8_store(v8_mul(v8_load(hi[i >> a]), v8_load(lo[i & mask])));
 }
+// Kernels with a constant multiplier convert it once to the encode form
+// (Montgomery for KoalaBear) and multiply with v8_mul_enc.
 __global__ void k_lb_fold(E8* a, E8* b, uint64_t h, E8 x, E8 y) {
-  V8 vx = v8_load(x), vy = v8_load(y);
+  const V8 vx = v8_encode_form(v8_load(x)), vy = v8_encode_form(v8_load(y));
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
-    a[i] = v8_store(v8_add(v8_load(a[i]), v8_mul(v8_load(a[h + i]), vx)));
-    b[i] = v8_store(v8_add(v8_load(b[i]), v8_mul(v8_load(b[h + i]), vy)));
+    a[i] = v8_store(v8_add(v8_load(a[i]), v8_mul_enc(v8_load(a[h + i]), vx)));
+    b[i] = v8_store(v8_add(v8_load(b[i]), v8_mul_enc(v8_load(b[h + i]), vy)));
   }
 }
 __global__ void k_eo_fold(const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, E8 ri, E8 r) {
-  V8 vri = v8_load(ri), vr = v8_load(r);
+  const V8 vri = v8_encode_form(v8_load(ri)), vr = v8_encode_form(v8_load(r));
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
     V8 w0 = v8_load(w[2 * i]), w1 = v8_load(w[2 * i + 1]), q0 = v8_load(q[2 * i]), q1 = v8_load(q[2 * i + 1]);
-    ow[i] = v8_store(v8_add(w0, v8_mul(w1, vri)));
-    oq[i] = v8_store(v8_add(q0, v8_mul(q1, vr)));
+    ow[i] = v8_store(v8_add(w0, v8_mul_enc(w1, vri)));
+    oq[i] = v8_store(v8_add(q0, v8_mul_enc(q1, vr)));
   }
 }
 __global__ void k_sc_fold(E8* v, E8* w, uint64_t h, E8 r) {
-  V8 vr = v8_load(r);
+  const V8 vr = v8_encode_form(v8_load(r));
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
     V8 vl = v8_load(v[i]), wl = v8_load(w[i]);
-    v[i] = v8_store(v8_add(vl, v8_mul(vr, v8_sub(v8_load(v[h + i]), vl))));
-    w[i] = v8_store(v8_add(wl, v8_mul(vr, v8_sub(v8_load(w[h + i]), wl))));
+    v[i] = v8_store(v8_add(vl, v8_mul_enc(v8_sub(v8_load(v[h + i]), vl), vr)));
+    w[i] = v8_store(v8_add(wl, v8_mul_enc(v8_sub(v8_load(w[h + i]), wl), vr)));
   }
 }
 __global__ void k_axpy(E8* y, const E8* x, uint64_t n, E8 s) {
-  V8 vs = v8_load(s);
+  const V8 vs = v8_encode_form(v8_load(s));
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
-    y[i] = v8_store(v8_add(v8_load(y[i]), v8_mul(vs, v8_load(x[i]))));
+    y[i] = v8_store(v8_add(v8_load(y[i]), v8_mul_enc(v8_load(x[i]), vs)));
 }
 __global__ void k_mul(E8* y, const E8* x, uint64_t n) {
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
     y[i] = v8_store(v8_mul(v8_load(y[i]), v8_load(x[i])));
 }
 __global__ void k_scale(E8* y, uint64_t n, E8 s) {
-  V8 vs = v8_load(s);
+  const V8 vs = v8_encode_form(v8_load(s));
   for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG)
-    y[i] = v8_store(v8_mul(v8_load(y[i]), vs));
+    y[i] = v8_store(v8_mul_enc(v8_load(y[i]), vs));
 }
 // Column projection: grid (groups, cols); partials[(col * groups + g)][0..LANES)
 __global__ __launch_bounds__(WG) void k_col_projection(const E8* __restrict__ cw, uint64_t R, uint64_t n,
@@ -206,24 +280,36 @@ using Acc = int64_t;
 using Acc = int32_t;
 #endif
 
-// Q (2 or 4) consecutive BVA entries from entry `idx` (a multiple of Q), one
-// aligned load; entries at or past `available` read as zero.
+// Q (2 or 4) consecutive BVA entries as one aligned word (idx a multiple of Q).
+// The caller guarantees idx + Q <= available; there is no branch, so a
+// thread can issue several of these loads before it consumes any.
+template <int Q>
+struct RawVals {
+  using T = typename std::conditional<Q * sizeof(Val) == 8, uint2,
+            typename std::conditional<Q * sizeof(Val) == 4, uint32_t, uint16_t>::type>::type;
+};
+template <int Q>
+__device__ inline typename RawVals<Q>::T load_raw(const Val* __restrict__ bva, uint64_t idx) {
+  return *reinterpret_cast<const typename RawVals<Q>::T*>(bva + idx);
+}
+template <int Q>
+__device__ inline void unpack_vals(const typename RawVals<Q>::T& w, int32_t (&v)[Q]) {
+  if constexpr (Q * sizeof(Val) == 8) {
+#pragma unroll
+    for (int j = 0; j < Q; j++) v[j] = int16_t((j < 2 ? w.x : w.y) >> (16 * (j & 1)));
+  } else if constexpr (Q * sizeof(Val) == 4) {
+#pragma unroll
+    for (int j = 0; j < Q; j++) v[j] = sizeof(Val) == 1 ? int32_t(int8_t(w >> (8 * j))) : int32_t(int16_t(w >> (16 * j)));
+  } else {
+#pragma unroll
+    for (int j = 0; j < Q; j++) v[j] = int8_t(w >> (8 * j));
+  }
+}
+// Guarded form: entries at or past `available` read as zero.
 template <int Q>
 __device__ inline void load_vals(const Val* __restrict__ bva, uint64_t idx, uint64_t available, int32_t (&v)[Q]) {
   if (idx + Q <= available) {
-    if constexpr (Q * sizeof(Val) == 8) {
-      const uint2 w = *reinterpret_cast<const uint2*>(bva + idx);
-#pragma unroll
-      for (int j = 0; j < Q; j++) v[j] = int16_t((j < 2 ? w.x : w.y) >> (16 * (j & 1)));
-    } else if constexpr (Q * sizeof(Val) == 4) {
-      const uint32_t w = *reinterpret_cast<const uint32_t*>(bva + idx);
-#pragma unroll
-      for (int j = 0; j < Q; j++) v[j] = sizeof(Val) == 1 ? int32_t(int8_t(w >> (8 * j))) : int32_t(int16_t(w >> (16 * j)));
-    } else {
-      const uint16_t w = *reinterpret_cast<const uint16_t*>(bva + idx);
-#pragma unroll
-      for (int j = 0; j < Q; j++) v[j] = int8_t(w >> (8 * j));
-    }
+    unpack_vals<Q>(load_raw<Q>(bva, idx), v);
     return;
   }
 #pragma unroll
@@ -238,39 +324,61 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
   constexpr int Q = L < 4 ? L : 4;       // lanes per thread
   constexpr int TPR = L / Q;             // threads per row
   constexpr int RPP = WG / TPR;          // rows per pass
+  constexpr int U = 8;                   // row loads in flight per thread
   __shared__ E8 T[4096];
-  __shared__ E8 C;
+  __shared__ E8 Chi[16];                 // scale x high-bit weights of each hi group
   const ProjTaskDev t = tasks[blockIdx.x];
   const E8(*tp)[2] = pairs + t.pair_off;
   build_table(T, tp, int(t.k));
   const int q = threadIdx.x % TPR;
   const int r0 = threadIdx.x / TPR;
-  const uint64_t lo_rows = uint64_t(1) << t.k;
-  const uint64_t chunk_rows = uint64_t(1) << t.chunk_rows_log;
+  const uint32_t lo_rows = 1u << t.k;
+  const uint32_t hi_count = 1u << (t.chunk_rows_log - t.k);  // <= 16: chunks hold <= 2^16 rows, k = 12 then
+  if (threadIdx.x < hi_count) {  // the hi-group scales, one per thread
+    const uint64_t row_hi = t.chunk_first + uint64_t(threadIdx.x) * lo_rows;
+    V8 c = v8_load(t.scale);
+    for (uint32_t b = t.k; b < t.nbits; b++) c = v8_mul(c, v8_load(tp[b][(row_hi >> b) & 1]));
+    Chi[threadIdx.x] = v8_store(c);
+  }
+  __syncthreads();
   V8 acc[Q];
   for (int l = 0; l < Q; l++) acc[l] = v8_zero();
-  for (uint64_t hs = 0; hs < chunk_rows; hs += lo_rows) {
-    const uint64_t row_hi = t.chunk_first + hs;  // first row of this hi group
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      V8 c = v8_load(t.scale);
-      for (uint32_t b = t.k; b < t.nbits; b++) c = v8_mul(c, v8_load(tp[b][(row_hi >> b) & 1]));
-      C = v8_store(c);
-    }
-    __syncthreads();
+  for (uint32_t h = 0; h < hi_count; h++) {
+    const uint64_t row_hi = t.chunk_first + uint64_t(h) * lo_rows;  // first row of this hi group
+    const uint64_t first = t.base + row_hi * L + Q * q;
     Acc ai[Q][LANES];
     for (int l = 0; l < Q; l++)
       for (int k = 0; k < LANES; k++) ai[l][k] = 0;
-    for (uint64_t lo = r0; lo < lo_rows; lo += RPP) {
-      int32_t v[Q];
-      load_vals<Q>(bva, t.base + (row_hi + lo) * L + Q * q, available, v);
-      const E8 wt = T[lo];
+    const bool in_range = t.base + (row_hi + lo_rows) * L <= available;  // uniform: all but the last groups
+    if (in_range && lo_rows >= uint32_t(RPP * U)) {
+      // U unguarded row loads in flight per thread, then U rows of products
+      for (uint32_t lo0 = r0; lo0 < lo_rows; lo0 += RPP * U) {
+        typename RawVals<Q>::T raw[U];
 #pragma unroll
-      for (int l = 0; l < Q; l++)
+        for (int u = 0; u < U; u++) raw[u] = load_raw<Q>(bva, first + uint64_t(lo0 + u * RPP) * L);
 #pragma unroll
-        for (int k = 0; k < LANES; k++) ai[l][k] += Acc(v[l]) * Acc(wt.c[k]);
+        for (int u = 0; u < U; u++) {
+          int32_t v[Q];
+          unpack_vals<Q>(raw[u], v);
+          const E8 wt = T[lo0 + u * RPP];
+#pragma unroll
+          for (int l = 0; l < Q; l++)
+#pragma unroll
+            for (int k = 0; k < LANES; k++) ai[l][k] += Acc(v[l]) * Acc(wt.c[k]);
+        }
+      }
+    } else {
+      for (uint32_t lo = r0; lo < lo_rows; lo += RPP) {
+        int32_t v[Q];
+        load_vals<Q>(bva, first + uint64_t(lo) * L, available, v);
+        const E8 wt = T[lo];
+#pragma unroll
+        for (int l = 0; l < Q; l++)
+#pragma unroll
+          for (int k = 0; k < LANES; k++) ai[l][k] += Acc(v[l]) * Acc(wt.c[k]);
+      }
     }
-    V8 c = v8_load(C);
+    V8 c = v8_load(Chi[h]);
 #pragma unroll
     for (int l = 0; l < Q; l++) {
       V8 x;
@@ -312,14 +420,31 @@ __global__ __launch_bounds__(WG) void k_eo_witness(const Val* __restrict__ bva, 
     for (int k = 0; k < LANES; k++) acc[k] = 0;
     constexpr int Q = S < 4 ? S : 4;
     const uint64_t first = row * S;
-    for (int g = 0; g < S / Q; g++) {
-      int32_t v[Q];
-      load_vals<Q>(bva, first + Q * g, available, v);
+    if (first + S <= available) {  // all loads of the row issued before any product
+      typename RawVals<Q>::T raw[S / Q];
 #pragma unroll
-      for (int j = 0; j < Q; j++) {
-        const E8 p = pinv[Q * g + j];
+      for (int g = 0; g < S / Q; g++) raw[g] = load_raw<Q>(bva, first + Q * g);
 #pragma unroll
-        for (int k = 0; k < LANES; k++) acc[k] += Acc(v[j]) * Acc(p.c[k]);
+      for (int g = 0; g < S / Q; g++) {
+        int32_t v[Q];
+        unpack_vals<Q>(raw[g], v);
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const E8 p = pinv[Q * g + j];
+#pragma unroll
+          for (int k = 0; k < LANES; k++) acc[k] += Acc(v[j]) * Acc(p.c[k]);
+        }
+      }
+    } else {
+      for (int g = 0; g < S / Q; g++) {
+        int32_t v[Q];
+        load_vals<Q>(bva, first + Q * g, available, v);
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const E8 p = pinv[Q * g + j];
+#pragma unroll
+          for (int k = 0; k < LANES; k++) acc[k] += Acc(v[j]) * Acc(p.c[k]);
+        }
       }
     }
     V8 o;
@@ -367,17 +492,7 @@ __global__ __launch_bounds__(WG) void k_block_fold(const uint8_t* __restrict__ b
     Acc S[LANES][LANES];
     for (int j = 0; j < LANES; j++)
       for (int k = 0; k < LANES; k++) S[j][k] = 0;
-    for (uint32_t b = 0; b < blocks; b++) {
-      const uint64_t byte = uint64_t(b) * block_bytes + 8 * i;
-      uint32_t x[2] = {0, 0};
-      if (byte + 8 <= available) {
-        uint2 u = *reinterpret_cast<const uint2*>(bva + byte);
-        x[0] = u.x;
-        x[1] = u.y;
-      } else {
-        for (int j = 0; j < 8; j++) if (byte + j < available) x[j >> 2] |= uint32_t(bva[byte + j]) << (8 * (j & 3));
-      }
-      const E8 f = fw[b];
+    auto add_block = [&](const uint32_t (&x)[2], const E8& f) {
 #pragma unroll
       for (int j = 0; j < LANES; j++) {
 #ifdef QF_KB
@@ -388,6 +503,30 @@ __global__ __launch_bounds__(WG) void k_block_fold(const uint8_t* __restrict__ b
 #pragma unroll
         for (int k = 0; k < LANES; k++) S[j][k] += v * Acc(f.c[k]);
       }
+    };
+    // Blocks [0, full) hold this element's 8 bytes entirely below `available`:
+    // their loads are unguarded and issued four at a time.
+    const uint64_t off = 8 * i;
+    const uint32_t full = available >= off + 8
+                              ? uint32_t(std::min<uint64_t>(blocks, (available - off - 8) / block_bytes + 1))
+                              : 0u;
+    uint32_t b = 0;
+    for (; b + 4 <= full; b += 4) {
+      uint2 u[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) u[k] = *reinterpret_cast<const uint2*>(bva + uint64_t(b + k) * block_bytes + off);
+#pragma unroll
+      for (int k = 0; k < 4; k++) add_block({u[k].x, u[k].y}, fw[b + k]);
+    }
+    for (; b < full; b++) {
+      const uint2 u = *reinterpret_cast<const uint2*>(bva + uint64_t(b) * block_bytes + off);
+      add_block({u.x, u.y}, fw[b]);
+    }
+    if (b < blocks) {  // the block that straddles `available`; later blocks read as zero
+      const uint64_t byte = uint64_t(b) * block_bytes + off;
+      uint32_t x[2] = {0, 0};
+      for (int j = 0; j < 8; j++) if (byte + j < available) x[j >> 2] |= uint32_t(bva[byte + j]) << (8 * (j & 3));
+      add_block(x, fw[b]);
     }
     // value = sum_j X^j * S_j with X^LANES = EXT_W
     uint64_t acc[LANES];
@@ -607,6 +746,21 @@ void lb_fold(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y) {
 std::pair<E8, E8> eo_cross(DevCtx& c, const E8* w, const E8* q, uint64_t h) { return reduce<R_EO>(c, w, q, h); }
 void eo_fold(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri, const E8& r) {
   hipLaunchKernelGGL(k_eo_fold, grid_for(h), dim3(WG), 0, c.st, w, q, ow, oq, h, ri, r);
+}
+std::array<E8, 3> eo_first(DevCtx& c, const E8* w, const E8* q, uint64_t h) {
+  const uint32_t g = groups_for(h);
+  hipLaunchKernelGGL(k_eo_first, dim3(g), dim3(WG), 0, c.st, w, q, h, c.partials);
+  QF_HIP(hipGetLastError());
+  return finish_sums<3>(c, g);
+}
+std::pair<E8, E8> eo_fold_cross(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri,
+                                const E8& r) {
+  require(h >= 2 && h % 2 == 0, QF_ERR_ARGUMENT, "fold length");
+  const uint32_t g = groups_for(h / 2);
+  hipLaunchKernelGGL(k_eo_fold_cross, dim3(g), dim3(WG), 0, c.st, w, q, ow, oq, h, ri, r, c.partials);
+  QF_HIP(hipGetLastError());
+  auto s = finish_sums<2>(c, g);
+  return {s[0], s[1]};
 }
 std::pair<E8, E8> sc_msg(DevCtx& c, const E8* v, const E8* w, uint64_t h) { return reduce<R_SC>(c, v, w, h); }
 void sc_fold(DevCtx& c, E8* v, E8* w, uint64_t h, const E8& r) {

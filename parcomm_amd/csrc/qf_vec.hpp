@@ -56,6 +56,11 @@ void lb_fold(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y);
 std::pair<E8, E8> eo_cross(DevCtx& c, const E8* w, const E8* q, uint64_t h);
 // ow[i] = w[2i] + w[2i+1] ri; oq[i] = q[2i] + q[2i+1] r
 void eo_fold(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri, const E8& r);
+// <w, q> over 2h elements and the first round's cross terms, in one pass: {inner, cross0, cross1}
+std::array<E8, 3> eo_first(DevCtx& c, const E8* w, const E8* q, uint64_t h);
+// eo_fold into (ow, oq) of length h, returning the next round's cross terms over the outputs (h even)
+std::pair<E8, E8> eo_fold_cross(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri,
+                                const E8& r);
 // (sum v_lo w_lo, sum (v_hi - v_lo)(w_hi - w_lo))
 std::pair<E8, E8> sc_msg(DevCtx& c, const E8* v, const E8* w, uint64_t h);
 // v_lo += r (v_hi - v_lo), same for w
