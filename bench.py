@@ -174,11 +174,22 @@ def main():
                     help="headline backend; the default run also measures KoalaBear beside it")
     ap.add_argument("--no-koalabear", action="store_true", help="skip the KoalaBear line of the default run")
     ap.add_argument("--no-per-matrix", action="store_true", help="skip the per-matrix comparison step")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (a plain copy kernel of known size, for the "
+                         "HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is designed, not built"}))
         sys.exit(2)
     import parcomm_amd as pa
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src, dst = pa.DeviceBuffer(nbytes), pa.DeviceBuffer(nbytes)
+        dst.copy_from(src)
+        src.close()
+        dst.close()
+        print(f"# copied {nbytes} bytes device to device", file=sys.stderr, flush=True)
 
     kb_head = args.field == "koalabear"
     field = pa.KOALABEAR if kb_head else pa.F65521

@@ -146,6 +146,9 @@ int qf_device_alloc(size_t bytes, void** device_out);
 int qf_device_upload(void* device_dst, const void* host_src, size_t bytes);
 int qf_device_download(void* host_dst, const void* device_src, size_t bytes);
 int qf_device_free(void* device_ptr);
+/* Device-to-device copy of `bytes` (synchronous). Used to calibrate the HBM
+ * traffic counters against a plain copy of known size. */
+int qf_device_copy(void* device_dst, const void* device_src, size_t bytes);
 
 /* Free and total bytes of the current device (admission control of the
  * serving loop). */
@@ -171,7 +174,8 @@ int qf_set_launch_mode(int mode);
 /* Kernel-level entry point for the parity tests (no reference counterpart):
  * one register-radix transform pass (mode 0 Mobius, 1 Mobius then WHT into
  * out2, 2 WHT, 3 WHT . alpha . WHT, 4 copy into out1 and WHT into out2, 5 copy
- * into out1 and WHT . alpha . WHT into out2) over
+ * into out1 and WHT . alpha . WHT into out2, 6 alpha . (elementwise) then WHT)
+ * over
  * `vectors` device vectors of 2^L extension elements, on index bits [s, s+t) and,
  * when xform_vectors is set (s = 0), also on the index bits of the 2^(12-t)
  * vectors that share a tile. Returns QF_ERR_ARGUMENT when the geometry does

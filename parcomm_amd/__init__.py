@@ -88,6 +88,7 @@ def lib(field=QF_FIELD_F65521):
         L.qf_device_upload.argtypes = [vp, vp, sz]
         L.qf_device_download.argtypes = [vp, vp, sz]
         L.qf_device_free.argtypes = [vp]
+        L.qf_device_copy.argtypes = [vp, vp, sz]
         L.qf_synthetic_bva.argtypes = [vp, u64, ctypes.c_float, vp]
         L.qf_claims_size.argtypes = [vp, P(sz)]
         L.qf_benchmark_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
@@ -162,6 +163,13 @@ class DeviceBuffer:
             raise ValueError("range outside the device buffer")
         _check(self._L.qf_device_upload(ctypes.c_void_p(self.ptr.value + offset), arr.ctypes.data, arr.nbytes),
                self.field)
+
+    def copy_from(self, other, nbytes=None):
+        """Device-to-device copy of `nbytes` (default: all of `other`)."""
+        n = other.nbytes if nbytes is None else int(nbytes)
+        if n > self.nbytes or n > other.nbytes:
+            raise ValueError("copy larger than a buffer")
+        _check(self._L.qf_device_copy(self.ptr, other.ptr, n), self.field)
 
     def close(self):
         if self.ptr:

@@ -172,45 +172,61 @@ __global__ __launch_bounds__(WG) void k_mobius_columns(E8* __restrict__ cw, uint
 // Leaf hash of one 32-column codeword row: BLAKE2s(prefix || row bytes), with
 // the framed domain + width prefix of PLEN bytes (75 for base rows, 80 for
 // extension rows). The first 64 prefix bytes are compressed on the host.
+//
+// The 512 row bytes stream through a register window of five columns (20
+// words): message block b (b >= 1) covers row words 16 b - QB .. 16 b - QB + 16,
+// which start at word A of the columns 4b - 5 .. 4b - 1, so each block loads
+// four new columns and keeps the last one of the previous block. No array is
+// indexed at run time, so nothing spills to scratch.
+template <int PLEN>
+__device__ inline void leaf_window_block(uint32_t (&W)[20], uint32_t (&h)[8], const uint4* __restrict__ col,
+                                         uint64_t rows, const LeafPrefix& lp, int b, bool first, bool last) {
+  constexpr int TOTAL = PLEN + 512;
+  constexpr int OFF = (4 - PLEN % 4) % 4;    // byte offset of message words inside row words
+  constexpr int QB = (PLEN + OFF) / 4;       // block b starts at row word 16 b - QB
+  constexpr int A = ((-QB) % 4 + 4) % 4;     // ... which is word A of the window
+  const int c0 = (16 * b - QB - A) / 4;      // first window column (exact division)
+#pragma unroll
+  for (int k = 0; k < 4; k++) W[k] = W[16 + k];  // column c0 was the previous block's last column
+#pragma unroll
+  for (int j = 1; j < 5; j++) {
+    const int c = c0 + j;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (c < 32) v = col[uint64_t(c) * rows];
+    W[4 * j] = v.x;
+    W[4 * j + 1] = v.y;
+    W[4 * j + 2] = v.z;
+    W[4 * j + 3] = v.w;
+  }
+  uint32_t m[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t lo = W[A + i], hi = W[A + i + 1];
+    uint32_t x = OFF ? (lo >> (8 * OFF)) | (hi << (32 - 8 * OFF)) : lo;
+    if (first && 64 + 4 * i < PLEN) x |= lp.tail[i];  // prefix bytes 64 .. PLEN - 1
+    m[i] = x;
+  }
+  b2s_compress(h, m, last ? uint32_t(TOTAL) : uint32_t(64 * (b + 1)), last);
+}
+
 template <int PLEN>
 __global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uint64_t rows, LeafPrefix lp,
                                                   uint32_t* __restrict__ leaves) {
-  constexpr int TOTAL = PLEN + 512;
-  constexpr int BLOCKS = (TOTAL + 63) / 64;
-  uint64_t r = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  constexpr int BLOCKS = (PLEN + 512 + 63) / 64;
+  static_assert(BLOCKS == 10 && (16 * 1 - (PLEN + (4 - PLEN % 4) % 4) / 4) < 0, "prefix ends inside block 1");
+  const uint64_t r = uint64_t(blockIdx.x) * WG + threadIdx.x;
   if (r >= rows) return;
-  uint32_t R[129];
-#pragma unroll
-  for (int k = 0; k < 32; k++) {
-    uint4 w = *reinterpret_cast<const uint4*>(cw + k * rows + r);
-    R[4 * k] = w.x; R[4 * k + 1] = w.y; R[4 * k + 2] = w.z; R[4 * k + 3] = w.w;
-  }
-  R[128] = 0;
+  const uint4* col = reinterpret_cast<const uint4*>(cw) + r;  // column k of this row at col[k * rows]
   uint32_t h[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) h[i] = lp.h[i];
+  uint32_t W[20];
 #pragma unroll
-  for (int b = 1; b < BLOCKS; b++) {
-    uint32_t m[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      const int w = 16 * b + i;
-      uint32_t x;
-      if (4 * w + 4 <= PLEN) {
-        x = lp.tail[w - 16];
-      } else if (4 * w < PLEN) {
-        x = lp.tail[w - 16] | (R[0] << (8 * (PLEN - 4 * w)));
-      } else if (4 * w >= TOTAL) {
-        x = 0;
-      } else {
-        const int o = 4 * w - PLEN, q = o >> 2, sh = o & 3;
-        const uint32_t lo = q < 129 ? R[q] : 0u, hi = q + 1 < 129 ? R[q + 1] : 0u;
-        x = sh == 0 ? lo : (lo >> (8 * sh)) | (hi << (32 - 8 * sh));
-      }
-      m[i] = x;
-    }
-    b2s_compress(h, m, b == BLOCKS - 1 ? uint32_t(TOTAL) : uint32_t(64 * (b + 1)), b == BLOCKS - 1);
-  }
+  for (int k = 16; k < 20; k++) W[k] = 0;  // column -1: before the row
+  leaf_window_block<PLEN>(W, h, col, rows, lp, 1, true, false);
+#pragma unroll 1
+  for (int b = 2; b < BLOCKS - 1; b++) leaf_window_block<PLEN>(W, h, col, rows, lp, b, false, false);
+  leaf_window_block<PLEN>(W, h, col, rows, lp, BLOCKS - 1, false, true);
   uint4* o = reinterpret_cast<uint4*>(leaves + 8 * r);
   o[0] = make_uint4(h[0], h[1], h[2], h[3]);
   o[1] = make_uint4(h[4], h[5], h[6], h[7]);
@@ -319,8 +335,16 @@ static bool tiled_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream
 // Second half of the parity, in place: WHT on the low bits, alpha, WHT on the
 // low bits (one pass), then WHT on the high groups. Precondition: the parity
 // region holds the message with the WHT already applied on every high group.
+// F65521's one-pass WHT . alpha . WHT needs more registers than two workgroups
+// per CU allow; on full-size blocks it runs as two lean passes (WHT, then
+// alpha . WHT), which is faster despite the extra sweep.
 static void tiled_parity_tail(E8* par, uint64_t cols, uint64_t R, int L, const E8* alpha, hipStream_t st) {
-  must(tile_pass(TP_WAW, false, par, par, nullptr, alpha, cols, R, R, L, 0, std::min(L, 12), 0, 0, st));
+  if (LANES == 8 && L >= 16) {
+    must(tile_pass(TP_W, false, par, par, nullptr, nullptr, cols, R, R, L, 0, 12, 0, 0, st));
+    must(tile_pass(TP_AW, false, par, par, nullptr, alpha, cols, R, R, L, 0, 12, 0, 0, st));
+  } else {
+    must(tile_pass(TP_WAW, false, par, par, nullptr, alpha, cols, R, R, L, 0, std::min(L, 12), 0, 0, st));
+  }
   for (auto& g : hi_groups(L))
     must(tile_pass(TP_W, false, par, par, nullptr, nullptr, cols, R, R, L, g.first, g.second, 0, 0, st));
 }

@@ -78,6 +78,12 @@ int qf_device_download(void* dst, const void* src, size_t bytes) {
 int qf_device_free(void* p) {
   return guard([&] { if (p) QF_HIP(hipFree(p)); });
 }
+int qf_device_copy(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    require(dst && src, QF_ERR_ARGUMENT, "null argument");
+    QF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  });
+}
 
 int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* device_out) {
   return guard([&] {

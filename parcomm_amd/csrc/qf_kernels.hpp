@@ -56,7 +56,7 @@ void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E
 
 // One register-radix tile pass (qf_tile.hip); false when the geometry does not
 // fit a 4096-element tile.
-enum : int { TP_M = 0, TP_M_W = 1, TP_W = 2, TP_WAW = 3, TP_COPY_W = 4, TP_COPY_WAW = 5 };
+enum : int { TP_M = 0, TP_M_W = 1, TP_W = 2, TP_WAW = 3, TP_COPY_W = 4, TP_COPY_WAW = 5, TP_AW = 6 };
 // With xform_vectors the pass also transforms the index bits of the vectors a
 // tile holds (s = 0, t < 12: 2^(12-t) vectors per tile).
 bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8* alpha, uint64_t vectors,

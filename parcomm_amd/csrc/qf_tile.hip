@@ -3,7 +3,7 @@
 // A pass applies a tensor product of 2x2 maps (Mobius [[1,0],[-1,1]] or
 // Walsh-Hadamard [[1,1],[1,-1]]) on up to 12 index bits of many vectors.
 // Modes: M; M then WHT (second output); WHT; WHT . alpha . WHT; copy then WHT;
-// copy then WHT . alpha . WHT. One
+// copy then WHT . alpha . WHT; alpha then WHT. One
 // workgroup (512 threads) owns a tile of 4096 elements: C contiguous positions
 // x 2^t positions spaced 2^s apart (x 2^logV whole vectors when a vector is
 // shorter than a tile). Each thread keeps 8 elements in registers, one 32-bit
@@ -27,9 +27,13 @@ constexpr int TQB = 3;                 // register bits per phase
 constexpr int TQ = 1 << TQB;           // elements per thread
 constexpr int TN = TWG * TQ;           // elements per tile
 constexpr int TPAD = TN + TN / 8;      // LDS row with one pad slot per 8 elements
-constexpr int LROWS = LANES / 2;       // LDS rows of uint2 (two lanes each)
+// An exchange moves the LANES 32-bit lanes of every tile element through NR
+// LDS rows of uint2 (72 KiB for NR = 2, so two workgroups can share a CU);
+// F65521's eight lanes cross in two rounds. A single round through four rows
+// needs more registers than the alpha passes have and spills.
+constexpr int NR = 2;
 
-enum : int { TM_M = 0, TM_M_W = 1, TM_W = 2, TM_WAW = 3, TM_COPY_W = 4, TM_COPY_WAW = 5 };
+enum : int { TM_M = 0, TM_M_W = 1, TM_W = 2, TM_WAW = 3, TM_COPY_W = 4, TM_COPY_WAW = 5, TM_AW = 6 };
 // Mappings are named by the first tile bit held in registers.
 enum : int { MAP_A = 9, MAP_B = 6, MAP_C = 3, MAP_D = 0 };
 
@@ -119,25 +123,33 @@ __device__ inline void phase(int32_t (&x)[TQ][LANES], int lo, int hi) {
   __builtin_amdgcn_sched_barrier(0);
 }
 
-template <int FROM, int TO>
+template <int FROM, int TO, int LROWS>
 __device__ inline void exchange(int32_t (&x)[TQ][LANES], uint2 (*lds)[TPAD]) {
+  constexpr int XROUNDS = LANES / (2 * LROWS);
   const int tid = threadIdx.x;
-  __syncthreads();
 #pragma unroll
-  for (int q = 0; q < TQ; q++) {
-    const int i = lds_slot(tile_e<FROM>(tid, q));
+  for (int rnd = 0; rnd < XROUNDS; rnd++) {
+    __syncthreads();
 #pragma unroll
-    for (int p = 0; p < LROWS; p++) lds[p][i] = make_uint2(uint32_t(x[q][2 * p]), uint32_t(x[q][2 * p + 1]));
-  }
-  __syncthreads();
+    for (int q = 0; q < TQ; q++) {
+      const int i = lds_slot(tile_e<FROM>(tid, q));
 #pragma unroll
-  for (int q = 0; q < TQ; q++) {
-    const int i = lds_slot(tile_e<TO>(tid, q));
+      for (int p = 0; p < LROWS; p++) {
+        const int k = 2 * (rnd * LROWS + p);
+        lds[p][i] = make_uint2(uint32_t(x[q][k]), uint32_t(x[q][k + 1]));
+      }
+    }
+    __syncthreads();
 #pragma unroll
-    for (int p = 0; p < LROWS; p++) {
-      const uint2 v = lds[p][i];
-      x[q][2 * p] = int32_t(v.x);
-      x[q][2 * p + 1] = int32_t(v.y);
+    for (int q = 0; q < TQ; q++) {
+      const int i = lds_slot(tile_e<TO>(tid, q));
+#pragma unroll
+      for (int p = 0; p < LROWS; p++) {
+        const int k = 2 * (rnd * LROWS + p);
+        const uint2 v = lds[p][i];
+        x[q][k] = int32_t(v.x);
+        x[q][k + 1] = int32_t(v.y);
+      }
     }
   }
 }
@@ -306,9 +318,12 @@ __device__ inline void store_tile(const TileArgs& a, uint64_t T, const SlotOffs&
     *reinterpret_cast<uint4*>(base + 16 * q_part(a, q, a.out_vstride) + so.out_t) = pack_e8(x[q]);
 }
 
-template <int MODE, bool PACK, int LO>
-__global__ __launch_bounds__(TWG) void k_tile(TileArgs a) {
-  __shared__ uint2 lds[LROWS][TPAD];
+// PF: prefetch the next tile's input while the current one is transformed
+// (one workgroup per CU); without it each tile loads at the top of its
+// iteration and the other workgroup on the CU hides the latency.
+template <int MODE, bool PACK, int LO, bool PF>
+__device__ inline void tile_body(const TileArgs& a) {
+  __shared__ uint2 lds[NR][TPAD];
   const uint64_t first = uint64_t(blockIdx.x) * a.per_wg;
   const uint64_t last = first + a.per_wg < a.tiles ? first + a.per_wg : a.tiles;
   if (first >= last) return;
@@ -317,9 +332,10 @@ __global__ __launch_bounds__(TWG) void k_tile(TileArgs a) {
   so.in_t = (PACK ? 8u : 16u) * tile_off(a, tile_e<MAP_A>(threadIdx.x, 0), PACK ? a.pack_n : a.in_vstride);
   so.out_t = 16u * tile_off(a, tile_e<MAP_A>(threadIdx.x, 0), a.out_vstride);
   uint4 raw[TQ];
-  load_tile<PACK>(a, first, so, raw);
+  if (PF) load_tile<PACK>(a, first, so, raw);
   for (uint64_t T = first; T < last; T++) {
     int32_t x[TQ][LANES];
+    if (!PF) load_tile<PACK>(a, T, so, raw);
 #pragma unroll
     for (int q = 0; q < TQ; q++) {
       if (PACK) unpack_bva(make_uint2(raw[q].x, raw[q].y), x[q]);
@@ -327,67 +343,105 @@ __global__ __launch_bounds__(TWG) void k_tile(TileArgs a) {
     }
     if (MODE == TM_COPY_W || MODE == TM_COPY_WAW) store_tile(a, T, so, a.out1, x);
     constexpr bool WAW = MODE == TM_WAW || MODE == TM_COPY_WAW;
-    if (!WAW && T + 1 < last) load_tile<PACK>(a, T + 1, so, raw);
+    if (PF && !WAW && T + 1 < last) load_tile<PACK>(a, T + 1, so, raw);
+    if (MODE == TM_AW) {  // x <- alpha . x (x canonical from memory), then the WHT below
+      uint64_t vec0, pos0;
+      tile_base(a, T, vec0, pos0);
+      const char* ab = reinterpret_cast<const char*>(a.alpha + pos0);
+      const uint32_t dt = 16u * tile_off(a, tile_e<MAP_A>(threadIdx.x, 0), 0);
+      auto alpha_at = [&](int q) {
+        return *reinterpret_cast<const uint4*>(ab + dt + 16u * tile_off(a, TWG * q, 0));
+      };
+      uint4 al0 = alpha_at(0), al1 = alpha_at(1);
+#pragma unroll
+      for (int q = 0; q < TQ; q += 2) {
+        const uint4 c0 = al0, c1 = al1;
+        if (q + 2 < TQ) { al0 = alpha_at(q + 2); al1 = alpha_at(q + 3); }
+        mul_alpha(x[q], c0);
+        __builtin_amdgcn_sched_barrier(0);
+        mul_alpha(x[q + 1], c1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
     if (MODE == TM_M || MODE == TM_M_W) {
       phase<false, MAP_A, LO>(x, lo, hi);
-      exchange<MAP_A, MAP_B>(x, lds);
+      exchange<MAP_A, MAP_B, NR>(x, lds);
       phase<false, MAP_B, LO>(x, lo, hi);
-      exchange<MAP_B, MAP_C>(x, lds);
+      exchange<MAP_B, MAP_C, NR>(x, lds);
       phase<false, MAP_C, LO>(x, lo, hi);
-      exchange<MAP_C, MAP_D>(x, lds);
+      exchange<MAP_C, MAP_D, NR>(x, lds);
       phase<false, MAP_D, LO>(x, lo, hi);
-      exchange<MAP_D, MAP_A>(x, lds);
+      exchange<MAP_D, MAP_A, NR>(x, lds);
       reduce_all(x);
       store_tile(a, T, so, a.out1, x);
     }
-    if (MODE == TM_M_W || MODE == TM_W || MODE == TM_COPY_W) {
+    if (MODE == TM_M_W || MODE == TM_W || MODE == TM_COPY_W || MODE == TM_AW) {
       phase<true, MAP_A, LO>(x, lo, hi);
-      exchange<MAP_A, MAP_B>(x, lds);
+      exchange<MAP_A, MAP_B, NR>(x, lds);
       phase<true, MAP_B, LO>(x, lo, hi);
-      exchange<MAP_B, MAP_C>(x, lds);
+      exchange<MAP_B, MAP_C, NR>(x, lds);
       phase<true, MAP_C, LO>(x, lo, hi);
-      exchange<MAP_C, MAP_D>(x, lds);
+      exchange<MAP_C, MAP_D, NR>(x, lds);
       phase<true, MAP_D, LO>(x, lo, hi);
-      exchange<MAP_D, MAP_A>(x, lds);
+      exchange<MAP_D, MAP_A, NR>(x, lds);
       reduce_all(x);
-      store_tile(a, T, so, MODE == TM_W ? a.out1 : a.out2, x);
+      store_tile(a, T, so, MODE == TM_W || MODE == TM_AW ? a.out1 : a.out2, x);
     }
     if (WAW) {
       phase<true, MAP_A, LO>(x, lo, hi);
-      exchange<MAP_A, MAP_B>(x, lds);
+      exchange<MAP_A, MAP_B, NR>(x, lds);
       phase<true, MAP_B, LO>(x, lo, hi);
-      exchange<MAP_B, MAP_C>(x, lds);
-      uint4 al[TQ];  // alpha_1 of the elements held in mapping D
+      exchange<MAP_B, MAP_C, NR>(x, lds);
+      // alpha_1 of the elements held in mapping D, two slots in flight
+      const char* ab;
+      uint32_t dt;
       {
         uint64_t vec0, pos0;
         tile_base(a, T, vec0, pos0);
-        const char* ab = reinterpret_cast<const char*>(a.alpha + pos0);
-        const uint32_t dt = 16u * tile_off(a, tile_e<MAP_D>(threadIdx.x, 0), 0);
-#pragma unroll
-        for (int q = 0; q < TQ; q++) al[q] = *reinterpret_cast<const uint4*>(ab + dt + 16u * tile_off(a, q, 0));
+        ab = reinterpret_cast<const char*>(a.alpha + pos0);
+        dt = 16u * tile_off(a, tile_e<MAP_D>(threadIdx.x, 0), 0);
       }
+      auto alpha_at = [&](int q) { return *reinterpret_cast<const uint4*>(ab + dt + 16u * tile_off(a, q, 0)); };
+      uint4 al0 = alpha_at(0), al1 = alpha_at(1);
       phase<true, MAP_C, LO>(x, lo, hi);
-      exchange<MAP_C, MAP_D>(x, lds);
+      exchange<MAP_C, MAP_D, NR>(x, lds);
       phase<true, MAP_D, LO>(x, lo, hi);
       reduce_all(x);
 #pragma unroll
-      for (int q = 0; q < TQ; q++) {
-        mul_alpha(x[q], al[q]);
+      for (int q = 0; q < TQ; q += 2) {
+        const uint4 c0 = al0, c1 = al1;
+        if (q + 2 < TQ) { al0 = alpha_at(q + 2); al1 = alpha_at(q + 3); }
+        mul_alpha(x[q], c0);
+        __builtin_amdgcn_sched_barrier(0);
+        mul_alpha(x[q + 1], c1);
         __builtin_amdgcn_sched_barrier(0);
       }
       phase<true, MAP_D, LO>(x, lo, hi);
-      exchange<MAP_D, MAP_C>(x, lds);
+      exchange<MAP_D, MAP_C, NR>(x, lds);
       phase<true, MAP_C, LO>(x, lo, hi);
-      exchange<MAP_C, MAP_B>(x, lds);
+      exchange<MAP_C, MAP_B, NR>(x, lds);
       phase<true, MAP_B, LO>(x, lo, hi);
-      exchange<MAP_B, MAP_A>(x, lds);
+      exchange<MAP_B, MAP_A, NR>(x, lds);
       phase<true, MAP_A, LO>(x, lo, hi);
       reduce_all(x);
       store_tile(a, T, so, MODE == TM_WAW ? a.out1 : a.out2, x);
-      if (T + 1 < last) load_tile<PACK>(a, T + 1, so, raw);
+      if (PF && T + 1 < last) load_tile<PACK>(a, T + 1, so, raw);
     }
     __syncthreads();  // the next tile's first exchange reuses the LDS rows
   }
+}
+
+// Two workgroups share a CU (72 KiB of LDS each) when a wave stays within 128
+// VGPRs. F65521's WHT . alpha . WHT modes need more registers and run one
+// workgroup per CU rather than spill; KoalaBear's fit.
+constexpr bool wide_mode(int mode) { return LANES == 8 && (mode == TM_WAW || mode == TM_COPY_WAW); }
+template <int MODE, bool PACK, int LO>
+__global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(4))) void k_tile(TileArgs a) {
+  tile_body<MODE, PACK, LO, false>(a);
+}
+template <int MODE, bool PACK, int LO>
+__global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_tile_wide(TileArgs a) {
+  tile_body<MODE, PACK, LO, true>(a);
 }
 
 int device_cus() {
@@ -447,13 +501,21 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
   const uint64_t tiles = (vectors << L) / TN;
   if (tiles == 0 || tiles >= (1ull << 31)) return false;
   a.tiles = uint32_t(tiles);
-  const uint64_t wgs = std::min<uint64_t>(tiles, uint64_t(device_cus()));
+  // persistent workgroups: two per CU for the lean kernels, one for the wide ones
+  const uint64_t per_cu = wide_mode(mode) ? 1 : 2;
+  const uint64_t wgs = std::min<uint64_t>(tiles, per_cu * uint64_t(device_cus()));
   a.per_wg = uint32_t((tiles + wgs - 1) / wgs);
   const uint32_t grid = uint32_t((tiles + a.per_wg - 1) / a.per_wg);
   dim3 g{grid, 1, 1}, b{uint32_t(TWG), 1, 1};
   // compile-time stage ranges for the full-width tiles ([logC, 12)), runtime otherwise
   const int lo_ct = a.xhi == 12 && (a.xlo == 0 || a.xlo == 2 || a.xlo == 7) ? a.xlo : -1;
-#define QF_TILE_LAUNCH_LO(M, PK, LOV) hipLaunchKernelGGL((k_tile<M, PK, LOV>), g, b, 0, st, a)
+#define QF_TILE_LAUNCH_LO(M, PK, LOV)                                                 \
+  do {                                                                                \
+    if constexpr (wide_mode(M))                                                       \
+      hipLaunchKernelGGL((k_tile_wide<M, PK, LOV>), g, b, 0, st, a);                   \
+    else                                                                              \
+      hipLaunchKernelGGL((k_tile<M, PK, LOV>), g, b, 0, st, a);                        \
+  } while (0)
 #define QF_TILE_LAUNCH(M, PK)                        \
   do {                                               \
     switch (lo_ct) {                                 \
@@ -474,6 +536,7 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
       case TM_WAW: QF_TILE_LAUNCH(TM_WAW, false); break;
       case TM_COPY_W: QF_TILE_LAUNCH(TM_COPY_W, false); break;
       case TM_COPY_WAW: QF_TILE_LAUNCH(TM_COPY_WAW, false); break;
+      case TM_AW: QF_TILE_LAUNCH(TM_AW, false); break;
       default: return false;
     }
   }

@@ -82,14 +82,14 @@ def field_values(rng, P, shape):
 
 @pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
 @pytest.mark.parametrize("vectors,L,s,t", GEOMS)
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6])
 def test_tile_pass_matches_numpy(vectors, L, s, t, mode, field):
     P, lanes, _, _ = FIELDS[field]
     rng = np.random.default_rng(1000 * L + 10 * mode + s + 7 * field)
     x = field_values(rng, P, (vectors, 1 << L, lanes))
     alpha = field_values(rng, P, (1 << L, lanes))
     print("tile pass", FIELD_IDS[field], "mode", mode, "vectors", vectors, "L", L, "bits", s, s + t, flush=True)
-    o1, o2 = run(field, mode, x, alpha if mode in (3, 5) else None, L, s, t,
+    o1, o2 = run(field, mode, x, alpha if mode in (3, 5, 6) else None, L, s, t,
                  vstride_in=(1 << L) + 64 if L < 20 else None)
     bits = list(range(s, s + t))
     if mode == 0:
@@ -106,6 +106,8 @@ def test_tile_pass_matches_numpy(vectors, L, s, t, mode, field):
     elif mode == 4:
         assert np.array_equal(o1, x)
         assert np.array_equal(o2, transform(x, bits, True, P))
+    elif mode == 6:
+        assert np.array_equal(o1, transform(e8_mul(x, alpha[None], field), bits, True, P))
     else:
         w = transform(x, bits, True, P)
         assert np.array_equal(o1, x)
