@@ -35,6 +35,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_mw_8b_traffic.md")
+
+
+def measured_traffic(field_tag):
+    """HBM bytes per launch of the dominant pass, from the committed PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy kernel), or
+    None when that summary has no line for this field."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith(f"traffic_bytes_per_launch[{field_tag}]:"):
+            return int(line.split(":", 1)[1].strip())
+    return None
 
 
 def llama8b_shapes():
@@ -146,6 +159,68 @@ def measure(pa, field, args, per_matrix=True, serve_requests=0):
     return res
 
 
+def roofline_of(pa, res, field_tag):
+    """Roofline object of the dominant kernel: the pass that finishes the
+    message (Mobius on the last high row group) and starts the parity (WHT on
+    the same bits). Per launch it reads the 32-column message half and writes
+    the message and parity halves: 3 x 32 n elements of 16 bytes. Its launch
+    time comes from HIP events on the commit stream around every launch."""
+    params, timings = res["params"], res["timings"]
+    n = params.block_len // (32 * pa.field_info(params.field)["lanes"])  # message rows per column
+    launches = sum(t["commit"]["timed_blocks"] for t in timings)
+    mw_ms = sum(t["commit"]["mw_ms"] for t in timings) / max(launches, 1)
+    leaf_ms = sum(t["commit"]["leaf_ms"] for t in timings) / max(launches, 1)
+    if not mw_ms:
+        return None
+    mw_bytes = 3 * 32 * n * 16
+    achieved = mw_bytes / (mw_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(field_tag),
+            "kernel": "k_tile<M_W> (Mobius + WHT, last high row group)",
+            "algorithmic_bytes": mw_bytes, "avg_launch_ms": round(mw_ms, 4), "launches": launches,
+            "leaf_hash_ms": round(leaf_ms, 4)}
+
+
+def phase_hbm(pa, res):
+    """Compulsory HBM bytes of the bandwidth-bound phases (each forced pass
+    reads its inputs once and writes its outputs once) over their measured
+    wall time, against the HBM peak. Per encoded block of 2^c entries of w
+    bytes, L lanes, 16-byte elements and R = 2^(c+1) / (32 L) codeword rows:
+    commit 2^c w + 4 (2^c / L) 16 + 2 R 32 (input, codeword write and
+    leaf-hash read, digests); packed EO 2 active w + 2 (logical/s) 16 + the
+    rounds' 3 n 16 for n = logical/s, /2, ...; projection B 2^c w + 3 (2^c/L)
+    16 (lane sums, weights and weight map); folding B 2^c w + (2^c/L) 16 plus
+    one block's commit; sumcheck 6 (2^c/L) 16."""
+    params, timings = res["params"], res["timings"]
+    info = pa.field_info(params.field)
+    w, L = (2 if info["lanes"] == 4 else 1), info["lanes"]
+    c2 = params.block_len
+    B = params.blocks
+    rows = 2 * c2 // (32 * L)
+    commit_block = c2 * w + 4 * (c2 // L) * 16 + 2 * rows * 32
+    N = params.logical >> params.config.packed_log
+    eo_rounds, n = 0, N
+    while n > 1:
+        eo_rounds += 3 * n * 16
+        n //= 2
+    k = len(timings)
+    phases = {
+        "commit": (sum(t["commit"]["timed_blocks"] for t in timings) / k * commit_block,
+                   sum(t["commit_s"] for t in timings) / k),
+        "packed_eo": (2 * params.active * w + 2 * N * 16 + eo_rounds, "packed_eo_seconds"),
+        "projection": (B * c2 * w + 3 * (c2 // L) * 16, "pcs_projection_seconds"),
+        "folding": (B * c2 * w + (c2 // L) * 16 + commit_block, "terminal_batch_folding_seconds"),
+        "sumcheck": (6 * (c2 // L) * 16, "pcs_sumcheck_seconds"),
+    }
+    out = {}
+    for name, (nbytes, t) in phases.items():
+        secs = t if isinstance(t, float) else sum(x["profile"][t] for x in timings) / k
+        gbs = nbytes / secs / 1e9 if secs else 0.0
+        out[name] = {"algorithmic_GB": round(nbytes / 1e9, 2), "s": round(secs, 4), "GB_per_s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def summary(res):
     timings = res["timings"]
     mean = lambda k: sum(t[k] for t in timings) / len(timings)
@@ -187,26 +262,22 @@ def main():
         nbytes = args.copy_calibration << 20
         src, dst = pa.DeviceBuffer(nbytes), pa.DeviceBuffer(nbytes)
         dst.copy_from(src)
+        dst.copy_from(src, half=0)
+        dst.copy_from(src, half=1)
         src.close()
         dst.close()
-        print(f"# copied {nbytes} bytes device to device", file=sys.stderr, flush=True)
+        print(f"# copied {nbytes} bytes device to device: whole, then each 64-byte half of its 128-byte lines",
+              file=sys.stderr, flush=True)
 
     kb_head = args.field == "koalabear"
     field = pa.KOALABEAR if kb_head else pa.F65521
     res = measure(pa, field, args, per_matrix=not args.no_per_matrix, serve_requests=args.serve_requests)
-    params, timings = res["params"], res["timings"]
+    params = res["params"]
     s_per_step = res["s_per_step"]
     blocks = params.blocks
-    lanes = pa.field_info(field)["lanes"]
-    R = 2 * (params.block_len // (32 * lanes))
-    leaf_ms = sum(t["commit"]["leaf_ms"] for t in timings) / (len(timings) * blocks)
-    leaf_bytes = R * 32 * 16 + R * 32  # read the codeword rows, write the leaves
-    roofline = None
-    if leaf_ms:
-        achieved = leaf_bytes / (leaf_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_leaf_hash"}
+    roofline = roofline_of(pa, res, args.field)
     summ = summary(res)
+    summ["phase_hbm"] = phase_hbm(pa, res)
     if kb_head:
         workload = "llama3.1-8b-shaped bf16-pattern int16 BVA (KoalaBear), 226 matrices, chunk_log 30, packed_log 4"
         dtype = "u32 (KoalaBear / Ext4 exact integer)"
@@ -220,7 +291,11 @@ def main():
     config.update({k: v for k, v in summ.items() if k != "s_per_step"})
     del res
     if not kb_head and not args.no_koalabear:
-        kb = summary(measure(pa, pa.KOALABEAR, args, per_matrix=False, serve_requests=0))
+        kb_res = measure(pa, pa.KOALABEAR, args, per_matrix=False, serve_requests=0)
+        kb = summary(kb_res)
+        kb["phase_hbm"] = phase_hbm(pa, kb_res)
+        kb["roofline"] = roofline_of(pa, kb_res, "koalabear")
+        del kb_res
         if not args.no_cpu_baseline:
             kb["cpu_baseline"] = cpu_baseline(kb=True)
         config["koalabear"] = kb

@@ -91,9 +91,12 @@ int qf_committed_roots(const qf_committed* c, uint8_t* out, size_t cap_blocks);
  * bytes each) to host memory; only for cached commitments. `cap` is in bytes. */
 int qf_committed_codeword(const qf_committed* c, uint64_t block, void* out, size_t cap);
 
-/* Device time in milliseconds of the kernels of the last qf_commit*, and the
- * mean duration of its leaf-hash kernel (HIP events on the commit stream). */
-int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms, double* encode_ms);
+/* Device time in milliseconds of the commit's kernels (HIP events on the
+ * commit stream), summed over the encoded blocks: all work, the leaf-hash
+ * kernel, the encoding before the leaves, and the Mobius+WHT pass of the last
+ * high row group; timed_blocks counts those blocks. Any pointer may be null. */
+int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms, double* encode_ms, double* mw_ms,
+                        uint64_t* timed_blocks);
 
 typedef struct qf_proof qf_proof;
 
@@ -146,9 +149,14 @@ int qf_device_alloc(size_t bytes, void** device_out);
 int qf_device_upload(void* device_dst, const void* host_src, size_t bytes);
 int qf_device_download(void* host_dst, const void* device_src, size_t bytes);
 int qf_device_free(void* device_ptr);
-/* Device-to-device copy of `bytes` (synchronous). Used to calibrate the HBM
- * traffic counters against a plain copy of known size. */
-int qf_device_copy(void* device_dst, const void* device_src, size_t bytes);
+/* Device-to-device copy by a plain 16-bytes-per-thread kernel (synchronous;
+ * 16-byte aligned buffers and sizes). pattern 0 copies all `bytes`; pattern 1
+ * (2) copies only the first (second) 64-byte half of every 128-byte line of
+ * the range (128-byte aligned buffers and sizes) and leaves the rest of dst
+ * as it was. Used to calibrate the HBM traffic counters against copies of
+ * known size, whole lines and the isolated half lines of the commit kernels'
+ * 64-byte tile rows. */
+int qf_device_copy(void* device_dst, const void* device_src, size_t bytes, int pattern);
 
 /* Free and total bytes of the current device (admission control of the
  * serving loop). */

@@ -83,12 +83,13 @@ def lib(field=QF_FIELD_F65521):
         L.qf_committed_free.restype = None
         L.qf_committed_roots.argtypes = [vp, ctypes.c_char_p, sz]
         L.qf_committed_codeword.argtypes = [vp, u64, vp, sz]
-        L.qf_committed_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)]
+        L.qf_committed_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
+                                          P(ctypes.c_double), P(u64)]
         L.qf_device_alloc.argtypes = [sz, P(vp)]
         L.qf_device_upload.argtypes = [vp, vp, sz]
         L.qf_device_download.argtypes = [vp, vp, sz]
         L.qf_device_free.argtypes = [vp]
-        L.qf_device_copy.argtypes = [vp, vp, sz]
+        L.qf_device_copy.argtypes = [vp, vp, sz, i32]
         L.qf_synthetic_bva.argtypes = [vp, u64, ctypes.c_float, vp]
         L.qf_claims_size.argtypes = [vp, P(sz)]
         L.qf_benchmark_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
@@ -164,12 +165,14 @@ class DeviceBuffer:
         _check(self._L.qf_device_upload(ctypes.c_void_p(self.ptr.value + offset), arr.ctypes.data, arr.nbytes),
                self.field)
 
-    def copy_from(self, other, nbytes=None):
-        """Device-to-device copy of `nbytes` (default: all of `other`)."""
+    def copy_from(self, other, nbytes=None, half=None):
+        """Device-to-device copy of `nbytes` (default: all of `other`); with
+        half = 0 or 1 only that 64-byte half of every 128-byte line."""
         n = other.nbytes if nbytes is None else int(nbytes)
         if n > self.nbytes or n > other.nbytes:
             raise ValueError("copy larger than a buffer")
-        _check(self._L.qf_device_copy(self.ptr, other.ptr, n), self.field)
+        pattern = 0 if half is None else 1 + int(half)
+        _check(self._L.qf_device_copy(self.ptr, other.ptr, n, pattern), self.field)
 
     def close(self):
         if self.ptr:
@@ -249,10 +252,13 @@ class Committed:
         return buf.value.decode()
 
     def timing(self):
-        t, l, e = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        _check(self._L.qf_committed_timing(self._h, ctypes.byref(t), ctypes.byref(l), ctypes.byref(e)),
-               self._params.field)
-        return {"total_ms": t.value, "leaf_ms": l.value, "encode_ms": e.value}
+        """Device milliseconds of the commit (HIP events), summed over encoded blocks."""
+        t, l, e, m = (ctypes.c_double() for _ in range(4))
+        nb = ctypes.c_uint64()
+        _check(self._L.qf_committed_timing(self._h, ctypes.byref(t), ctypes.byref(l), ctypes.byref(e),
+                                           ctypes.byref(m), ctypes.byref(nb)), self._params.field)
+        return {"total_ms": t.value, "leaf_ms": l.value, "encode_ms": e.value, "mw_ms": m.value,
+                "timed_blocks": nb.value}
 
     def close(self):
         if self._h:

@@ -95,7 +95,7 @@ void commit_impl(const qf_params* p, const Val* d_bva, size_t len, int cache_mod
     const uint64_t n = uint64_t(1) << p->geom.log_n, R = 2 * n;
     const size_t cw_bytes = size_t(32 * R) * 16, tree_bytes = size_t(2 * R - 1) * 32;
     const E8* alpha = p->alpha_encode(p->pp.codes.size() - 1);
-    hipEvent_t e0, e1, ev[4];
+    hipEvent_t e0, e1, ev[6];
     QF_HIP(hipEventCreate(&e0)); QF_HIP(hipEventCreate(&e1));
     for (auto& e : ev) QF_HIP(hipEventCreate(&e));
     QF_HIP(hipEventRecord(e0, st));
@@ -123,11 +123,14 @@ void commit_impl(const qf_params* p, const Val* d_bva, size_t len, int cache_mod
       Digest root;
       QF_HIP(hipMemcpyAsync(root.data(), tree + 8 * (2 * R - 2), 32, hipMemcpyDeviceToHost, st));
       QF_HIP(hipStreamSynchronize(st));
-      float a = 0, b2 = 0;
+      float a = 0, b2 = 0, m = 0;
       QF_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
       QF_HIP(hipEventElapsedTime(&b2, ev[1], ev[2]));
+      QF_HIP(hipEventElapsedTime(&m, ev[4], ev[5]));
       c->encode_ms += a;
       c->leaf_ms += b2;
+      c->mw_ms += m;
+      c->timed_blocks++;
       c->roots.push_back(root);
       if (first >= len) zero_block = int(b);
     }
@@ -283,12 +286,15 @@ int qf_committed_codeword(const qf_committed* c, uint64_t block, void* out, size
   });
 }
 
-int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms, double* encode_ms) {
+int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms, double* encode_ms, double* mw_ms,
+                        uint64_t* timed_blocks) {
   return guarded([&] {
     require(c, QF_ERR_ARGUMENT, "null argument");
     if (total_ms) *total_ms = c->total_ms;
     if (leaf_ms) *leaf_ms = c->leaf_ms;
     if (encode_ms) *encode_ms = c->encode_ms;
+    if (mw_ms) *mw_ms = c->mw_ms;
+    if (timed_blocks) *timed_blocks = c->timed_blocks;
   });
 }
 

@@ -366,8 +366,9 @@ static bool tiled_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, h
 // parity. Pass 1 holds 32 columns x 128 rows per tile (column bits and row bits
 // 0..6), pass 2 row bits 7..11, then the high groups, the last of which also
 // starts the parity WHT.
+// mw (optional, 2 events): recorded around the last-high-group Mobius+WHT pass.
 static bool tiled_commit_rows(bool pack, const void* src, uint64_t block_first, uint64_t available, E8* cw, int L,
-                              const E8* alpha, hipStream_t st) {
+                              const E8* alpha, hipStream_t st, hipEvent_t* mw = nullptr) {
   if (L < 12) return false;
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
   if (!tile_pass(TP_M, pack, src, cw, nullptr, nullptr, 32, n, R, L, 0, 7, block_first, available, st, true))
@@ -381,7 +382,9 @@ static bool tiled_commit_rows(bool pack, const void* src, uint64_t block_first, 
   for (size_t i = 0; i + 1 < gs.size(); i++)
     must(tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st));
   // last Mobius group completes the message; the same pass starts the WHT in the parity half
+  if (mw) QF_HIP(hipEventRecord(mw[0], st));
   must(tile_pass(TP_M_W, false, cw, cw, cw + n, nullptr, 32, R, R, L, gs.back().first, gs.back().second, 0, 0, st));
+  if (mw) QF_HIP(hipEventRecord(mw[1], st));
   for (size_t i = 0; i + 1 < gs.size(); i++)
     must(tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st));
   tiled_parity_tail(cw + n, 32, R, L, alpha, st);
@@ -418,7 +421,7 @@ void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStre
 }
 
 void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
-                 hipStream_t st) {
+                 hipStream_t st, hipEvent_t after_leaf) {
   const dim3 grid{uint32_t((R + WG - 1) / WG), 1, 1}, block{uint32_t(WG), 1, 1};
   if (leaf.prefix_len == 75)
     hipLaunchKernelGGL(k_leaf_hash<75>, grid, block, 0, st, cw, R, leaf, tree);
@@ -426,6 +429,7 @@ void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPr
     hipLaunchKernelGGL(k_leaf_hash<80>, grid, block, 0, st, cw, R, leaf, tree);
   else
     throw QfError(QF_ERR_ARGUMENT, "unsupported leaf prefix");
+  if (after_leaf) QF_HIP(hipEventRecord(after_leaf, st));
   uint32_t* level = tree;
   for (uint64_t cnt = R; cnt > 1; cnt /= 2) {
     uint32_t* next = level + 8 * cnt;
@@ -490,7 +494,9 @@ void launch_commit_block(const CommitGeom& g, const Val* values, uint64_t first_
   const uint64_t block_first = first_entry * sizeof(Val), available = active * sizeof(Val);
   if (ev) QF_HIP(hipEventRecord(ev[0], st));
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  if (!tiled_commit_rows(true, bva, block_first, available, cw, L, alpha, st)) {
+  if (ev) QF_HIP(hipEventRecord(ev[4], st));  // both M_W events stay here if no tiled M_W pass runs
+  if (ev) QF_HIP(hipEventRecord(ev[5], st));
+  if (!tiled_commit_rows(true, bva, block_first, available, cw, L, alpha, st, ev ? ev + 4 : nullptr)) {
     const int tp = L < 12 ? L : 12;
     // 1. pack + Mobius on the low bits within each column
     uint64_t packed = n * 32;
@@ -505,8 +511,7 @@ void launch_commit_block(const CommitGeom& g, const Val* values, uint64_t first_
   }
   // 4. leaves and inner levels
   if (ev) QF_HIP(hipEventRecord(ev[1], st));
-  launch_tree(cw, R, g.leaf, g.inner, tree, st);
-  if (ev) QF_HIP(hipEventRecord(ev[2], st));
+  launch_tree(cw, R, g.leaf, g.inner, tree, st, ev ? ev[2] : nullptr);
   if (ev) QF_HIP(hipEventRecord(ev[3], st));
 }
 

@@ -45,6 +45,23 @@ __global__ void k_synthetic(Val* out, uint64_t n, uint64_t seed, float sigma) {
   }
 }
 
+// Plain copies, 16 bytes per thread per step: the calibration kernels of the
+// HBM traffic counters. k_copy moves the whole buffer contiguously;
+// k_copy_half moves only the 64-byte half `parity` of every 128-byte line (the
+// row shape of the tile passes over high index bits, 4 contiguous elements),
+// with the other half never touched in the launch, which shows what one
+// isolated 64-byte read costs.
+__global__ void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += uint64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+__global__ void k_copy_half(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16, int parity) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16 / 2; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t j = 4 * (2 * (i / 4) + parity) + (i % 4);  // run i/4 -> line i/4, half `parity`
+    dst[j] = src[j];
+  }
+}
+
 // Zero a rows x width rectangle of a row-major bucket (padding region).
 __global__ void k_zero_rect(Val* base, uint64_t rows, uint64_t stride, uint64_t width) {
   uint64_t total = rows * width;
@@ -78,10 +95,23 @@ int qf_device_download(void* dst, const void* src, size_t bytes) {
 int qf_device_free(void* p) {
   return guard([&] { if (p) QF_HIP(hipFree(p)); });
 }
-int qf_device_copy(void* dst, const void* src, size_t bytes) {
+int qf_device_copy(void* dst, const void* src, size_t bytes, int pattern) {
   return guard([&] {
     require(dst && src, QF_ERR_ARGUMENT, "null argument");
-    QF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    require(pattern >= 0 && pattern <= 2, QF_ERR_ARGUMENT, "copy pattern must be 0, 1 or 2");
+    require(bytes % 16 == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0,
+            QF_ERR_ARGUMENT, "copy needs 16-byte aligned buffers and sizes");
+    require(pattern == 0 || (bytes % 128 == 0 && reinterpret_cast<uintptr_t>(dst) % 128 == 0 &&
+                             reinterpret_cast<uintptr_t>(src) % 128 == 0),
+            QF_ERR_ARGUMENT, "half-line copies need 128-byte aligned buffers and sizes");
+    if (pattern)
+      hipLaunchKernelGGL(k_copy_half, dim3(65536), dim3(256), 0, 0, static_cast<uint4*>(dst),
+                         static_cast<const uint4*>(src), uint64_t(bytes / 16), pattern - 1);
+    else
+      hipLaunchKernelGGL(k_copy, dim3(65536), dim3(256), 0, 0, static_cast<uint4*>(dst), static_cast<const uint4*>(src),
+                         uint64_t(bytes / 16));
+    QF_HIP(hipGetLastError());
+    QF_HIP(hipDeviceSynchronize());
   });
 }
 

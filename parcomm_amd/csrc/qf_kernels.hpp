@@ -33,6 +33,8 @@ struct CommitGeom {
 // of `cw`, encode the parity half, hash the rows and build all Merkle levels
 // in `tree` (2R - 1 digests, leaves first). Everything is enqueued on `st`.
 // Every `alpha` below is alpha_1 in its encode form (v8_encode_form).
+// ev (optional, 6 events): [0] start, [1] before the leaf kernel, [2] after it,
+// [3] end, [4]/[5] around the Mobius+WHT pass of the last high row group.
 void launch_commit_block(const CommitGeom& g, const Val* bva, uint64_t first_entry, uint64_t active, E8* cw,
                          const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev = nullptr);
 // out[i] = encode form of in[i] (Montgomery form for KoalaBear, a copy for F65521).
@@ -46,7 +48,7 @@ void launch_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream_t st)
 void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStream_t st);
 // Leaves (prefix chosen by `leaf`) and all inner levels of a 32-column codeword.
 void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
-                 hipStream_t st);
+                 hipStream_t st, hipEvent_t after_leaf = nullptr);
 // Commit of an extension matrix: flat 32 x 2^L message, no Mobius, extension leaves.
 void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, const LeafPrefix& leaf,
                              const InnerPrefix& inner, uint32_t* tree, hipStream_t st);

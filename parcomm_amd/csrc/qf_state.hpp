@@ -41,7 +41,8 @@ struct CommittedState {
   std::vector<uint32_t*> d_tree;
   const Val* d_bva = nullptr;  // the committed BVA (active entries) in HBM
   bool owns_bva = false;
-  double total_ms = 0, leaf_ms = 0, encode_ms = 0;
+  double total_ms = 0, leaf_ms = 0, encode_ms = 0, mw_ms = 0;
+  uint64_t timed_blocks = 0;  // blocks actually encoded (zero blocks past the data reuse a root)
   ~CommittedState() {
     std::vector<void*> freed;
     auto release = [&](void* q) {  // codewords and trees come from the stream-ordered pool

@@ -13,10 +13,13 @@
 // lanes are signed and unreduced (12 stages grow a canonical value by at most
 // 2^12, so |x| < 2^28) and are reduced to canonical u16 only before they are
 // stored or multiplied; KoalaBear lanes (p ~ 2^31) stay canonical through
-// modular butterflies. Workgroups are persistent: each walks
-// a contiguous range of tiles (neighbouring tiles share 128-byte lines, so
-// they stay on one XCD's L2) and prefetches the next tile's input while the
-// current one is transformed.
+// modular butterflies. Workgroups are persistent. Each walks a contiguous
+// range of tiles, except in passes whose tile rows are 64 bytes (4 contiguous
+// positions): there the two tiles that share every 128-byte line are taken
+// at the same time by sibling workgroups on one XCD (workgroups are dealt to
+// the 8 XCDs round robin), so the second half of each line is still in that
+// XCD's L2 when its sibling asks for it; a line read by one tile long after
+// its partner has been evicted and is fetched from HBM twice.
 #include "qf_kernels.hpp"
 
 namespace qf {
@@ -45,6 +48,7 @@ struct TileArgs {
   uint64_t in_vstride, out_vstride;
   uint64_t block_first, available, pack_n;  // packing: byte offset, active bytes, positions per packed vector
   uint32_t tiles, per_wg;
+  uint32_t xcd_span;        // > 0: XCD-interleaved walk, tiles per XCD (even)
   int logC, t, s, logV, log_lo_chunks, log_hi_count;
   int xlo, xhi;             // transformed tile bits [xlo, xhi)
 };
@@ -324,8 +328,17 @@ __device__ inline void store_tile(const TileArgs& a, uint64_t T, const SlotOffs&
 template <int MODE, bool PACK, int LO, bool PF>
 __device__ inline void tile_body(const TileArgs& a) {
   __shared__ uint2 lds[NR][TPAD];
-  const uint64_t first = uint64_t(blockIdx.x) * a.per_wg;
-  const uint64_t last = first + a.per_wg < a.tiles ? first + a.per_wg : a.tiles;
+  uint64_t first, last, step;
+  if (a.xcd_span) {  // XCD g = blockIdx % 8 owns tiles [g span, (g+1) span); its workgroups take consecutive tiles
+    const uint64_t g = blockIdx.x % 8, slots = gridDim.x / 8;
+    first = g * a.xcd_span + blockIdx.x / 8;
+    last = (g + 1) * a.xcd_span < a.tiles ? (g + 1) * a.xcd_span : a.tiles;
+    step = slots;
+  } else {
+    first = uint64_t(blockIdx.x) * a.per_wg;
+    last = first + a.per_wg < a.tiles ? first + a.per_wg : a.tiles;
+    step = 1;
+  }
   if (first >= last) return;
   const int lo = a.xlo, hi = a.xhi;  // transformed tile bits (runtime form)
   SlotOffs so;
@@ -333,7 +346,7 @@ __device__ inline void tile_body(const TileArgs& a) {
   so.out_t = 16u * tile_off(a, tile_e<MAP_A>(threadIdx.x, 0), a.out_vstride);
   uint4 raw[TQ];
   if (PF) load_tile<PACK>(a, first, so, raw);
-  for (uint64_t T = first; T < last; T++) {
+  for (uint64_t T = first; T < last; T += step) {
     int32_t x[TQ][LANES];
     if (!PF) load_tile<PACK>(a, T, so, raw);
 #pragma unroll
@@ -343,7 +356,7 @@ __device__ inline void tile_body(const TileArgs& a) {
     }
     if (MODE == TM_COPY_W || MODE == TM_COPY_WAW) store_tile(a, T, so, a.out1, x);
     constexpr bool WAW = MODE == TM_WAW || MODE == TM_COPY_WAW;
-    if (PF && !WAW && T + 1 < last) load_tile<PACK>(a, T + 1, so, raw);
+    if (PF && !WAW && T + step < last) load_tile<PACK>(a, T + step, so, raw);
     if (MODE == TM_AW) {  // x <- alpha . x (x canonical from memory), then the WHT below
       uint64_t vec0, pos0;
       tile_base(a, T, vec0, pos0);
@@ -425,7 +438,7 @@ __device__ inline void tile_body(const TileArgs& a) {
       phase<true, MAP_A, LO>(x, lo, hi);
       reduce_all(x);
       store_tile(a, T, so, MODE == TM_WAW ? a.out1 : a.out2, x);
-      if (PF && T + 1 < last) load_tile<PACK>(a, T + 1, so, raw);
+      if (PF && T + step < last) load_tile<PACK>(a, T + step, so, raw);
     }
     __syncthreads();  // the next tile's first exchange reuses the LDS rows
   }
@@ -505,7 +518,14 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
   const uint64_t per_cu = wide_mode(mode) ? 1 : 2;
   const uint64_t wgs = std::min<uint64_t>(tiles, per_cu * uint64_t(device_cus()));
   a.per_wg = uint32_t((tiles + wgs - 1) / wgs);
-  const uint32_t grid = uint32_t((tiles + a.per_wg - 1) / a.per_wg);
+  uint32_t grid = uint32_t((tiles + a.per_wg - 1) / a.per_wg);
+  // 64-byte tile rows: pair the tiles that share 128-byte lines across sibling
+  // workgroups of one XCD (tiles 2m and 2m+1 differ in the lowest chunk bit)
+  if (a.logC == 2 && a.log_lo_chunks >= 1 && wgs % 16 == 0 && tiles >= 2 * wgs) {
+    const uint64_t span = ((tiles + 7) / 8 + 1) & ~uint64_t(1);
+    a.xcd_span = uint32_t(span);
+    grid = uint32_t(wgs);
+  }
   dim3 g{grid, 1, 1}, b{uint32_t(TWG), 1, 1};
   // compile-time stage ranges for the full-width tiles ([logC, 12)), runtime otherwise
   const int lo_ct = a.xhi == 12 && (a.xlo == 0 || a.xlo == 2 || a.xlo == 7) ? a.xlo : -1;

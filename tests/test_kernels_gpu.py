@@ -170,3 +170,22 @@ def test_column_bit_pass_with_wide_vector_stride(mode, field):
     assert np.array_equal(o1, transform_vectors(transform(x, range(7), wht, P), 5, wht, P))
     d_in.close()
     d_o1.close()
+
+
+@pytest.mark.parametrize("half", [None, 0, 1], ids=["whole", "half0", "half1"])
+def test_device_copy_calibration_kernels(half):
+    """The calibration copies: the whole range, or exactly one 64-byte half of
+    every 128-byte line with the other half of dst left as it was."""
+    import parcomm_amd as pa
+    rng = np.random.default_rng(7)
+    host = rng.integers(0, 1 << 32, size=(1 << 20) // 4, dtype=np.uint32)  # 1 MiB
+    src = pa.DeviceBuffer.from_host(host)
+    dst = pa.DeviceBuffer(host.nbytes)  # zeroed
+    dst.copy_from(src, half=half)
+    got = dst.to_host(np.uint32).reshape(-1, 32)  # 128-byte lines of 32 words
+    want = host.reshape(-1, 32).copy()
+    if half is not None:
+        want[:, 16 * (1 - half):16 * (2 - half)] = 0
+    assert np.array_equal(got, want)
+    with pytest.raises(Exception):
+        dst.copy_from(src, nbytes=48, half=0)  # not a whole number of lines
