@@ -162,18 +162,18 @@ __host__ __device__ inline V8 v8_mul(const V8& a, const V8& b) { return v8_mul_m
 __host__ __device__ inline V8 v8_encode_form(const V8& b) { return v8_to_mont(b); }
 __host__ __device__ inline V8 v8_mul_enc(const V8& a, const V8& b_enc) { return v8_mul_mont(a, b_enc); }
 #else
+// v_dot2_u32_u16: c + a.lo b.lo + a.hi b.hi.
+__device__ inline uint32_t dot2_u16(uint32_t a, uint32_t b, uint32_t c) {
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b), c, false);
+}
+// u -> (u & 0xffff) + 15 (u >> 16), congruent mod P; below 2^20.1 for any u32.
+__device__ inline uint32_t fold16(uint32_t u) { return u - __umul24(u >> 16, uint32_t(P)); }
 #ifdef __HIP_DEVICE_COMPILE__
 // Device product in F65521[X]/(X^8 - 17). Coefficient i sums a_j E[7 + i - j]
 // over the sequence E = (17 b_1, ..., 17 b_7, b_0, ..., b_7), which folds the
 // X^8 = 17 wrap. a is split into 8-bit limbs; each v_dot2_u32_u16 adds two
 // limb x E products, so every partial sum of eight stays below 2^27.
-__device__ inline uint32_t dot2_u16(uint32_t a, uint32_t b, uint32_t c) {
-  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-  return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b), c, false);
-}
-__device__ inline uint32_t fold16(uint32_t u) {  // u < 2^31 -> (u & 0xffff) + 15 (u >> 16)
-  return u - uint32_t(__mul24(int32_t(u >> 16), int32_t(P)));
-}
 __device__ inline V8 v8_mul_dot2(const V8& a, const V8& b) {
   uint32_t E[15];
 #pragma unroll

@@ -614,16 +614,33 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   o.pos.assign(s - 1, E8{});
   o.neg.assign(s - 1, E8{});
   int ls = ilog2u(s);
-  std::vector<size_t> which;
-  auto proj = project_terms(c, q, bva, l.active, s, which);
-  for (size_t k = 0; k < which.size(); k++) {
-    const Term& t = q.terms[which[k]];
-    auto low = tbv_w(t.f.data(), ls);
-    for (uint64_t d = 1; d < s; d++)
-      for (uint64_t lane = 0; lane + d < s; lane++) {
-        o.pos[d - 1] += proj[k][lane] * low[lane + d];
-        o.neg[d - 1] += proj[k][lane + d] * low[lane];
-      }
+  if (launch_mode() == LAUNCH_BATCHED) {
+    // one projection launch over every term; the lane correlations on the device
+    std::vector<ProjTask> tasks;
+    std::vector<std::array<E8, 2>> pairs;
+    std::vector<E8> lows;
+    for (const Term& t : q.terms) {
+      if (t.len() < s || t.offset >= l.active || e8_is_zero(t.scale)) continue;
+      tasks.push_back({t.offset, t.len() / s, uint32_t(t.f.size() - ls), uint32_t(pairs.size()), t.scale});
+      for (size_t b = ls; b < t.f.size(); b++) pairs.push_back(t.f[b]);
+      auto low = tbv_w(t.f.data(), ls);
+      lows.insert(lows.end(), low.begin(), low.end());
+    }
+    auto pn = tensor_project_cross(c, bva, l.active, int(s), tasks, pairs, lows);
+    o.pos = pn.first;
+    o.neg = pn.second;
+  } else {
+    std::vector<size_t> which;
+    auto proj = project_terms(c, q, bva, l.active, s, which);
+    for (size_t k = 0; k < which.size(); k++) {
+      const Term& t = q.terms[which[k]];
+      auto low = tbv_w(t.f.data(), ls);
+      for (uint64_t d = 1; d < s; d++)
+        for (uint64_t lane = 0; lane + d < s; lane++) {
+          o.pos[d - 1] += proj[k][lane] * low[lane + d];
+          o.neg[d - 1] += proj[k][lane + d] * low[lane];
+        }
+    }
   }
   for (auto& t : q.terms) {
     if (t.len() >= s) continue;
@@ -649,9 +666,8 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   for (uint64_t i = 1; i < s; i++) { pinv[i] = pinv[i - 1] * inv; pows[i] = pows[i - 1] * ch; }
   uint64_t rows = l.logical / s;
   DVec w(rows), qv(rows);
-  eo_witness(c, bva, l.active, rows, int(s), pinv, w.p);
-  dv::fill(c, qv.p, rows, E8{});
-  // query fold (tbv.rs:393-428): bucket terms, then padding/suffix terms; each phase has disjoint supports.
+  // query fold (tbv.rs:393-428): bucket terms (written, the rows outside them
+  // zeroed), then padding/suffix terms added; each phase has disjoint supports.
   std::map<uint64_t, E8> small;
   for (int phase = 0; phase < 2; phase++) {
     std::vector<FillTask> tasks;
@@ -670,7 +686,8 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
         for (uint64_t j = 0; j < t.len(); j++) small[t.offset / s] += t.weight(j) * pows[(t.offset + j) % s];
       }
     }
-    tensor_fill_add(c, qv.p, tasks, pairs);
+    if (phase == 0) tensor_fill(c, qv.p, rows, tasks, pairs);
+    else tensor_fill_add(c, qv.p, tasks, pairs);
   }
   {
     std::vector<uint64_t> idx;
@@ -680,15 +697,17 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   }
   E8 claim = q.claim, fwd = ch, bwd = inv;
   for (uint64_t d = 0; d + 1 < s; d++) { claim += o.pos[d] * fwd + o.neg[d] * bwd; fwd = fwd * ch; bwd = bwd * inv; }
-  // The consistency check shares one pass with the first round's cross terms,
-  // and every fold also returns the next round's cross terms (rows is a power of two).
+  // The witness is built in the same pass as the consistency check and the
+  // first round's cross terms, and every fold also returns the next round's
+  // cross terms (rows is a power of two).
   E8 c0{}, c1{};
   if (rows >= 2) {
-    auto first = dv::eo_first(c, w.p, qv.p, rows / 2);
+    auto first = eo_witness_first(c, bva, l.active, rows, int(s), pinv, qv.p, w.p);
     require(first[0] == claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
     c0 = first[1];
     c1 = first[2];
   } else {
+    eo_witness(c, bva, l.active, rows, int(s), pinv, w.p);
     require(dv::inner(c, w.p, qv.p, rows) == claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
   }
   DVec w2(rows / 2 ? rows / 2 : 1), q2(rows / 2 ? rows / 2 : 1);

@@ -1,5 +1,6 @@
 // Device vector operations and BVA-scale kernels of the opening (see qf_vec.hpp).
 #include "qf_vec.hpp"
+#include <algorithm>
 #include <atomic>
 #include <type_traits>
 #include <map>
@@ -11,6 +12,7 @@ namespace qf {
 namespace {
 constexpr int WG = 256;
 constexpr uint64_t TAB_CAP = 1u << 18;  // E8 entries of the small-table scratch
+constexpr uint32_t PROJ_TAB_LOG = 12;   // low-bit weight table of a projection chunk: 2^12 entries
 
 __device__ inline V8 v8_madd(const V8& acc, const V8& x) { return v8_add(acc, x); }
 
@@ -256,13 +258,14 @@ __device__ inline uint32_t mod_signed(int64_t x) {
 }
 
 // Builds T[i] = prod_{b<k} pairs[b][(i >> b) & 1] in LDS (k <= 12).
+template <int NT = WG>
 __device__ inline void build_table(E8* T, const E8 (*pairs)[2], int k) {
   if (threadIdx.x == 0) T[0] = v8_store(v8_zero()), T[0].c[0] = 1;
   __syncthreads();
   for (int b = 0; b < k; b++) {
     const int half = 1 << b;
     V8 p0 = v8_load(pairs[b][0]), p1 = v8_load(pairs[b][1]);
-    for (int i = threadIdx.x; i < half; i += WG) {
+    for (int i = threadIdx.x; i < half; i += NT) {
       V8 t = v8_load(T[i]);
       T[i + half] = v8_store(v8_mul(t, p1));
       T[i] = v8_store(v8_mul(t, p0));
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
   constexpr int TPR = L / Q;             // threads per row
   constexpr int RPP = WG / TPR;          // rows per pass
   constexpr int U = 8;                   // row loads in flight per thread
-  __shared__ E8 T[4096];
+  __shared__ E8 T[1 << PROJ_TAB_LOG];
   __shared__ E8 Chi[16];                 // scale x high-bit weights of each hi group
   const ProjTaskDev t = tasks[blockIdx.x];
   const E8(*tp)[2] = pairs + t.pair_off;
@@ -391,6 +394,7 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
   // reduce acc over threads sharing the same lane quad; reuse the table as scratch
   uint32_t* red = reinterpret_cast<uint32_t*>(T);  // [WG][LANES Q]
   constexpr int RW = LANES * Q;
+  static_assert(sizeof(T) >= sizeof(uint32_t) * WG * RW, "reduction scratch exceeds the table");
   for (int l = 0; l < Q; l++)
     for (int k = 0; k < LANES; k++) red[threadIdx.x * RW + l * LANES + k] = acc[l].c[k];
   __syncthreads();
@@ -408,6 +412,58 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
   }
 }
 
+// Witness rows row0 .. row0 + NR - 1 (eo.rs:119-132): w[row] = sum_lane
+// v[row S + lane] pinv[lane], from S consecutive BVA entries per row (zero at
+// or past `available`). On the fast path all loads of the NR rows are issued
+// before any product.
+template <int S, int NR>
+__device__ inline void witness_rows(const Val* __restrict__ bva, uint64_t row0, uint64_t available,
+                                    const E8 (&pinv)[S], V8 (&w)[NR]) {
+  constexpr int Q = S < 4 ? S : 4;
+  Acc acc[NR][LANES];
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int k = 0; k < LANES; k++) acc[r][k] = 0;
+  const uint64_t first = row0 * S;
+  if (first + NR * S <= available) {
+    typename RawVals<Q>::T raw[NR][S / Q];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int g = 0; g < S / Q; g++) raw[r][g] = load_raw<Q>(bva, first + r * S + Q * g);
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int g = 0; g < S / Q; g++) {
+        int32_t v[Q];
+        unpack_vals<Q>(raw[r][g], v);
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const E8 p = pinv[Q * g + j];
+#pragma unroll
+          for (int k = 0; k < LANES; k++) acc[r][k] += Acc(v[j]) * Acc(p.c[k]);
+        }
+      }
+  } else {
+    for (int r = 0; r < NR; r++)
+      for (int g = 0; g < S / Q; g++) {
+        int32_t v[Q];
+        load_vals<Q>(bva, first + r * S + Q * g, available, v);
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const E8 p = pinv[Q * g + j];
+#pragma unroll
+          for (int k = 0; k < LANES; k++) acc[r][k] += Acc(v[j]) * Acc(p.c[k]);
+        }
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int k = 0; k < LANES; k++) w[r].c[k] = mod_signed(acc[r][k]);
+}
+
 template <int S>
 __global__ __launch_bounds__(WG) void k_eo_witness(const Val* __restrict__ bva, uint64_t available, uint64_t n_rows,
                                                    const E8* __restrict__ pinv_g, E8* __restrict__ w) {
@@ -415,73 +471,376 @@ __global__ __launch_bounds__(WG) void k_eo_witness(const Val* __restrict__ bva, 
   for (int i = threadIdx.x; i < S; i += WG) pinv[i] = pinv_g[i];
   __syncthreads();
   for (uint64_t row = uint64_t(blockIdx.x) * WG + threadIdx.x; row < n_rows; row += uint64_t(gridDim.x) * WG) {
-    Acc acc[LANES];
-#pragma unroll
-    for (int k = 0; k < LANES; k++) acc[k] = 0;
-    constexpr int Q = S < 4 ? S : 4;
-    const uint64_t first = row * S;
-    if (first + S <= available) {  // all loads of the row issued before any product
-      typename RawVals<Q>::T raw[S / Q];
-#pragma unroll
-      for (int g = 0; g < S / Q; g++) raw[g] = load_raw<Q>(bva, first + Q * g);
-#pragma unroll
-      for (int g = 0; g < S / Q; g++) {
-        int32_t v[Q];
-        unpack_vals<Q>(raw[g], v);
-#pragma unroll
-        for (int j = 0; j < Q; j++) {
-          const E8 p = pinv[Q * g + j];
-#pragma unroll
-          for (int k = 0; k < LANES; k++) acc[k] += Acc(v[j]) * Acc(p.c[k]);
-        }
-      }
-    } else {
-      for (int g = 0; g < S / Q; g++) {
-        int32_t v[Q];
-        load_vals<Q>(bva, first + Q * g, available, v);
-#pragma unroll
-        for (int j = 0; j < Q; j++) {
-          const E8 p = pinv[Q * g + j];
-#pragma unroll
-          for (int k = 0; k < LANES; k++) acc[k] += Acc(v[j]) * Acc(p.c[k]);
-        }
-      }
-    }
-    V8 o;
-#pragma unroll
-    for (int k = 0; k < LANES; k++) o.c[k] = mod_signed(acc[k]);
-    w[row] = v8_store(o);
+    V8 o[1];
+    witness_rows<S, 1>(bva, row, available, pinv, o);
+    w[row] = v8_store(o[0]);
   }
 }
 
+// The witness together with the packed reduction's consistency sum <w, q>
+// and the first binary round's cross terms (eo.rs:119-150), with the query q
+// already built: one read of the BVA and of q, w written once and not read
+// back. Pair i covers rows 2i and 2i + 1; sums {<w,q>, sum w[2i] q[2i+1],
+// sum w[2i+1] q[2i]} go to the block partials.
+#ifndef QF_KB
+// Lazy sums of F65521 extension products. A product a b splits a into 8-bit
+// limbs and b into the wrap-folded sequence D of v8_mul_dot2; each product
+// adds at most 4 x 2 x 255 x 65520 < 2^27.1 to every low-limb (sl) and
+// high-limb (sh) coefficient sum, so a sum takes 31 products before it must
+// be folded into a reduced element.
+struct LazySum {
+  uint32_t sl[8], sh[8];
+};
+__device__ inline void lazy_zero(LazySum& s) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) s.sl[i] = s.sh[i] = 0;
+}
+__device__ inline void fold_seq(const V8& b, uint32_t (&D)[15]) {  // D[1..14]: u16 pairs of E (v8_mul_dot2)
+  uint32_t E[15];
+#pragma unroll
+  for (int i = 0; i < 8; i++) E[7 + i] = b.c[i];
+#pragma unroll
+  for (int j = 1; j < 8; j++) {
+    const uint32_t u = fold16(__umul24(b.c[j], 17u));
+    const uint32_t w = u - P;
+    E[j - 1] = u < w ? u : w;
+  }
+#pragma unroll
+  for (int m = 1; m < 15; m++) D[m] = E[m] | (E[m - 1] << 16);
+}
+__device__ inline void limb_split(const V8& a, uint32_t (&lo)[4], uint32_t (&hi)[4]) {
+#pragma unroll
+  for (int jp = 0; jp < 4; jp++) {
+    const uint32_t pr = a.c[2 * jp] | (a.c[2 * jp + 1] << 16);
+    lo[jp] = pr & 0x00ff00ffu;
+    hi[jp] = (pr >> 8) & 0x00ff00ffu;
+  }
+}
+__device__ inline void lazy_mac(LazySum& s, const uint32_t (&lo)[4], const uint32_t (&hi)[4], const uint32_t (&D)[15]) {
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int jp = 0; jp < 4; jp++) {
+      s.sl[i] = dot2_u16(lo[jp], D[7 + i - 2 * jp], s.sl[i]);
+      s.sh[i] = dot2_u16(hi[jp], D[7 + i - 2 * jp], s.sh[i]);
+    }
+}
+__device__ inline V8 lazy_value(const LazySum& s) {  // sl, sh < 2^32
+  V8 r;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    uint32_t x = (fold16(fold16(s.sh[i])) << 8) + fold16(s.sl[i]);  // < 2^24.1 + 2^20.1
+    x = fold16(fold16(x));                                          // < 2^16 + 15
+    const uint32_t w = x - P;
+    r.c[i] = x < w ? x : w;
+  }
+  return r;
+}
+
+// F65521 form of the fused witness pass. Lanes are biased to u = v + 128 (one
+// xor per four lanes), paired as u16 and multiplied by u16 pairs of pinv
+// coefficients with v_dot2_u32_u16; negc[k] = -128 sum_lane pinv[lane].c[k]
+// mod P removes the bias. The three sums accumulate lazily (flushed every 8
+// pairs: sum 0 takes two products per pair).
+template <int S>
+__global__ __launch_bounds__(WG) void k_eo_witness_first(const Val* __restrict__ bva, uint64_t available, uint64_t h,
+                                                         const E8* __restrict__ pinv_g, const E8* __restrict__ q,
+                                                         E8* __restrict__ w, uint32_t* __restrict__ partials) {
+  constexpr int G = S < 4 ? 1 : S / 4;  // BVA words (4 lanes) per row
+  constexpr int NP = S / 2;             // lane pairs per row
+  __shared__ E8 pinv[S];
+  __shared__ uint32_t PU[8][NP];
+  __shared__ uint32_t negc[8];
+  for (int i = threadIdx.x; i < S; i += WG) pinv[i] = pinv_g[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < 8 * NP; i += WG) {
+    const int k = i / NP, lp = i % NP;
+    PU[k][lp] = uint32_t(pinv[2 * lp].c[k]) | (uint32_t(pinv[2 * lp + 1].c[k]) << 16);
+  }
+  if (threadIdx.x < 8) {
+    uint32_t t = 0;
+    for (int l = 0; l < S; l++) t = madd(t, pinv[l].c[threadIdx.x]);
+    negc[threadIdx.x] = msub(0, uint32_t((uint64_t(t) * 128) % P));
+  }
+  __syncthreads();
+  auto witness = [&](const uint32_t (&raw)[G], V8& out) {
+    uint32_t up[NP];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const uint32_t x = raw[g] ^ 0x80808080u;
+      up[2 * g] = __builtin_amdgcn_perm(0u, x, 0x0c010c00u);      // (u0, u1)
+      if (2 * g + 1 < NP) up[2 * g + 1] = __builtin_amdgcn_perm(0u, x, 0x0c030c02u);  // (u2, u3)
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      uint32_t acc = negc[k];
+#pragma unroll
+      for (int lp = 0; lp < NP; lp++) acc = dot2_u16(up[lp], PU[k][lp], acc);  // < 2^28.1
+      uint32_t x = fold16(fold16(acc));
+      const uint32_t y = x - P;
+      out.c[k] = x < y ? x : y;
+    }
+  };
+  __shared__ uint32_t sred[3 * 8][WG];  // the reduced sums, touched once per flush
+  for (int j = 0; j < 24; j++) sred[j][threadIdx.x] = 0;
+  LazySum ls[3];
+  lazy_zero(ls[0]);
+  lazy_zero(ls[1]);
+  lazy_zero(ls[2]);
+  auto flush = [&]() {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const V8 v = lazy_value(ls[j]);
+#pragma unroll
+      for (int k = 0; k < 8; k++) sred[8 * j + k][threadIdx.x] = madd(sred[8 * j + k][threadIdx.x], v.c[k]);
+      lazy_zero(ls[j]);
+    }
+  };
+  uint32_t pending = 0;
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
+    // the coefficient pairs stay in LDS: reloading them costs less than 64 registers
+    asm volatile("" ::: "memory");
+    const V8 q0 = v8_load(q[2 * i]), q1 = v8_load(q[2 * i + 1]);
+    V8 wr[2];
+    const uint64_t first = 2 * i * S;
+    if (S >= 4 && first + 2 * S <= available) {
+      uint32_t raw[2][G];
+#pragma unroll
+      for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int g = 0; g < G; g++) raw[r][g] = *reinterpret_cast<const uint32_t*>(bva + first + r * S + 4 * g);
+      witness(raw[0], wr[0]);
+      witness(raw[1], wr[1]);
+    } else {
+      witness_rows<S, 2>(bva, 2 * i, available, pinv, wr);
+    }
+    w[2 * i] = v8_store(wr[0]);
+    w[2 * i + 1] = v8_store(wr[1]);
+    uint32_t lo0[4], hi0[4], lo1[4], hi1[4], D[15];
+    limb_split(wr[0], lo0, hi0);
+    limb_split(wr[1], lo1, hi1);
+    fold_seq(q0, D);
+    lazy_mac(ls[0], lo0, hi0, D);  // w0 q0
+    lazy_mac(ls[2], lo1, hi1, D);  // w1 q0
+    fold_seq(q1, D);
+    lazy_mac(ls[0], lo1, hi1, D);  // w1 q1
+    lazy_mac(ls[1], lo0, hi0, D);  // w0 q1
+    if (++pending == 12) {  // sum 0 holds at most 24 products < 2^31.3
+      flush();
+      pending = 0;
+    }
+  }
+  flush();
+  // block sums in place (the layout of block_sums<3>: sum j coefficient k at 8 j + k)
+  __syncthreads();
+  for (int half = WG / 2; half > 0; half /= 2) {
+    if (int(threadIdx.x) < half)
+      for (int j = 0; j < 24; j++) sred[j][threadIdx.x] = madd(sred[j][threadIdx.x], sred[j][threadIdx.x + half]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 24) partials[blockIdx.x * PSTRIDE + threadIdx.x] = sred[threadIdx.x][0];
+}
+#else
+template <int S>
+__global__ __launch_bounds__(WG) void k_eo_witness_first(const Val* __restrict__ bva, uint64_t available, uint64_t h,
+                                                         const E8* __restrict__ pinv_g, const E8* __restrict__ q,
+                                                         E8* __restrict__ w, uint32_t* __restrict__ partials) {
+  __shared__ E8 pinv[S];
+  for (int i = threadIdx.x; i < S; i += WG) pinv[i] = pinv_g[i];
+  __syncthreads();
+  V8 s[3] = {v8_zero(), v8_zero(), v8_zero()};
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
+    const V8 q0 = v8_load(q[2 * i]), q1 = v8_load(q[2 * i + 1]);
+    V8 wr[2];
+    witness_rows<S, 2>(bva, 2 * i, available, pinv, wr);
+    w[2 * i] = v8_store(wr[0]);
+    w[2 * i + 1] = v8_store(wr[1]);
+    s[0] = v8_add(s[0], v8_add(v8_mul(wr[0], q0), v8_mul(wr[1], q1)));
+    s[1] = v8_add(s[1], v8_mul(wr[0], q1));
+    s[2] = v8_add(s[2], v8_mul(wr[1], q0));
+  }
+  block_sums<3>(s, partials);
+}
+#endif
+
 struct FillTaskDev { uint64_t start, first; uint32_t nbits, pair_off, k, count_log; E8 scale; };
-// out[start + j] += scale * W(j), j in [first, first + 2^count_log)
-__global__ __launch_bounds__(WG) void k_tensor_fill_add(E8* __restrict__ out, const FillTaskDev* __restrict__ tasks,
-                                                        const E8 (*__restrict__ pairs)[2]) {
+// out[start + j] (+)= scale * W(j), j in [first, first + 2^count_log): ADD
+// accumulates, else the task's rows are written without being read. A chunk
+// is at most 16 groups of 2^k rows; group g's scale (the task scale times
+// the weights of the bits above k) is computed once, by thread g. 1024
+// threads share the 64 KiB low-bit table, so a CU runs 32 waves of the kernel.
+constexpr int FWG = 1024;
+template <bool ADD>
+__global__ __launch_bounds__(FWG) void k_tensor_fill(E8* __restrict__ out, const FillTaskDev* __restrict__ tasks,
+                                                     const E8 (*__restrict__ pairs)[2]) {
   __shared__ E8 T[4096];
-  __shared__ E8 C;
+  __shared__ E8 Chi[16];
   const FillTaskDev t = tasks[blockIdx.x];
   const E8(*tp)[2] = pairs + t.pair_off;
-  build_table(T, tp, int(t.k));
-  const uint64_t lo_n = uint64_t(1) << t.k, count = uint64_t(1) << t.count_log;
-  for (uint64_t hs = 0; hs < count; hs += lo_n) {
-    const uint64_t j_hi = t.first + hs;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      V8 c = v8_load(t.scale);
-      for (uint32_t b = t.k; b < t.nbits; b++) c = v8_mul(c, v8_load(tp[b][(j_hi >> b) & 1]));
-      C = v8_store(c);
+  build_table<FWG>(T, tp, int(t.k));
+  const uint32_t lo_n = 1u << t.k, groups = 1u << (t.count_log - t.k);
+  if (threadIdx.x < groups) {
+    const uint64_t j_hi = t.first + uint64_t(threadIdx.x) * lo_n;
+    V8 c = v8_load(t.scale);
+    for (uint32_t b = t.k; b < t.nbits; b++) c = v8_mul(c, v8_load(tp[b][(j_hi >> b) & 1]));
+    Chi[threadIdx.x] = v8_store(c);
+  }
+  __syncthreads();
+  for (uint32_t g = 0; g < groups; g++) {
+    const V8 c = v8_load(Chi[g]);  // the uniform operand second: its product setup stays out of the loop
+    E8* o = out + t.start + t.first + uint64_t(g) * lo_n;
+    for (uint32_t lo = threadIdx.x; lo < lo_n; lo += FWG) {
+      const V8 x = v8_mul(v8_load(T[lo]), c);
+      o[lo] = v8_store(ADD ? v8_add(v8_load(o[lo]), x) : x);
     }
+  }
+}
+
+// Zero out[start, start + len) for each (start, len) pair of the table, one
+// workgroup per pair.
+__global__ __launch_bounds__(WG) void k_zero_ranges(E8* __restrict__ out, const uint64_t* __restrict__ ranges) {
+  const uint64_t start = ranges[2 * blockIdx.x], len = ranges[2 * blockIdx.x + 1];
+  uint4* o = reinterpret_cast<uint4*>(out + start);
+  for (uint64_t i = threadIdx.x; i < len; i += WG) o[i] = make_uint4(0, 0, 0, 0);
+}
+
+// Shifted lane correlations of the term projections (tbv.rs:361-392): block b
+// < L-1 sums pos[d-1] = sum_ch sum_lane parts[ch][lane] low[owner][lane+d],
+// block L-1+b the mirror neg[d-1] = sum parts[ch][lane+d] low[owner][lane],
+// d = b + 1, over the projection chunks (parts) and the low-bit weights of
+// each chunk's term.
+__global__ __launch_bounds__(WG) void k_proj_cross(const E8* __restrict__ parts, const uint32_t* __restrict__ owner,
+                                                   uint32_t nchunks, int L, const E8* __restrict__ lows,
+                                                   E8* __restrict__ out) {
+  __shared__ uint32_t red[WG][LANES];
+  const int d = int(blockIdx.x % uint32_t(L - 1)) + 1;
+  const bool neg = blockIdx.x >= uint32_t(L - 1);
+  V8 acc = v8_zero();
+  for (uint32_t ch = threadIdx.x; ch < nchunks; ch += WG) {
+    const E8* pr = parts + uint64_t(ch) * L;
+    const E8* lw = lows + uint64_t(owner[ch]) * L;
+    for (int lane = 0; lane + d < L; lane++)
+      acc = v8_add(acc, neg ? v8_mul(v8_load(pr[lane + d]), v8_load(lw[lane]))
+                            : v8_mul(v8_load(pr[lane]), v8_load(lw[lane + d])));
+  }
+#pragma unroll
+  for (int k = 0; k < LANES; k++) red[threadIdx.x][k] = acc.c[k];
+  __syncthreads();
+  for (int h = WG / 2; h > 0; h /= 2) {
+    if (int(threadIdx.x) < h)
+      for (int k = 0; k < LANES; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + h][k]);
     __syncthreads();
-    const V8 c = v8_load(C);
-    for (uint64_t lo = threadIdx.x; lo < lo_n; lo += WG) {
-      E8* o = out + t.start + j_hi + lo;
-      *o = v8_store(v8_add(v8_load(*o), v8_mul(c, v8_load(T[lo]))));
-    }
+  }
+  if (threadIdx.x == 0) {
+    V8 x;
+    for (int k = 0; k < LANES; k++) x.c[k] = red[0][k];
+    out[blockIdx.x] = v8_store(x);
   }
 }
 
 // values[i] = sum_b fw[b] * packed(b, i); `available` and `block_bytes` in bytes.
+#ifndef QF_KB
+// F65521 form. packed(b, i) has the 8 signed bytes v_j of the element as
+// coefficients; with u_j = v_j + 128 (one xor per four bytes) each is a single
+// 8-bit limb of the v8_mul_dot2 product, so fw[b] u costs 32 v_dot2_u32_u16
+// against fw[b]'s wrap-folded pair sequence, built once per block in LDS.
+// Bytes at or past `available` count as v = 0 (u = 128). The bias of the
+// first nb blocks, sum_{b < nb} fw[b] (128, ..., 128), is removed at the end
+// (negb[nb]). Each block adds < 2^27.1 to a coefficient sum: folds every 16.
+__global__ __launch_bounds__(WG) void k_block_fold(const uint8_t* __restrict__ bva, uint64_t available,
+                                                   uint64_t block_bytes, uint32_t blocks, const E8* __restrict__ fw_g,
+                                                   E8* __restrict__ values, uint64_t n) {
+  __shared__ uint4 FD[128][4];       // D[0..15] of each block (D[0], D[15] unused)
+  __shared__ uint32_t negb[129][8];  // -(sum_{b < nb} fw[b] (128, ..., 128)) mod P
+  for (uint32_t b = threadIdx.x; b < blocks; b += WG) {
+    const V8 f = v8_load(fw_g[b]);
+    uint32_t D[15];
+    fold_seq(f, D);
+    uint32_t* fd = reinterpret_cast<uint32_t*>(FD[b]);
+    fd[0] = 0;
+    for (int m = 1; m < 15; m++) fd[m] = D[m];
+    fd[15] = 0;
+    V8 k128;
+    for (int k = 0; k < 8; k++) k128.c[k] = 128;
+    const V8 pb = v8_mul(f, k128);  // this block's bias, summed below
+    for (int k = 0; k < 8; k++) negb[b + 1][k] = pb.c[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {  // prefix sums per coefficient, negated
+    const int k = threadIdx.x;
+    uint32_t run = 0;
+    negb[0][k] = 0;
+    for (uint32_t b = 1; b <= blocks; b++) {
+      run = madd(run, negb[b][k]);
+      negb[b][k] = msub(0, run);
+    }
+  }
+  __syncthreads();
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < n; i += uint64_t(gridDim.x) * WG) {
+    uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    V8 total = v8_zero();
+    auto add_block = [&](uint32_t x0, uint32_t x1, uint32_t b) {
+      x0 ^= 0x80808080u;
+      x1 ^= 0x80808080u;
+      const uint32_t lo[4] = {__builtin_amdgcn_perm(0u, x0, 0x0c010c00u), __builtin_amdgcn_perm(0u, x0, 0x0c030c02u),
+                              __builtin_amdgcn_perm(0u, x1, 0x0c010c00u), __builtin_amdgcn_perm(0u, x1, 0x0c030c02u)};
+      uint32_t D[16];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint4 v = FD[b][r];
+        D[4 * r] = v.x; D[4 * r + 1] = v.y; D[4 * r + 2] = v.z; D[4 * r + 3] = v.w;
+      }
+#pragma unroll
+      for (int c = 0; c < 8; c++)
+#pragma unroll
+        for (int jp = 0; jp < 4; jp++) acc[c] = dot2_u16(lo[jp], D[7 + c - 2 * jp], acc[c]);
+    };
+    auto fold_acc = [&]() {
+      V8 r;
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        uint32_t x = fold16(fold16(acc[c]));
+        const uint32_t y = x - P;
+        r.c[c] = x < y ? x : y;
+        acc[c] = 0;
+      }
+      total = v8_add(total, r);
+    };
+    // Blocks [0, full) hold this element's 8 bytes entirely below `available`:
+    // their loads are unguarded and issued four at a time.
+    const uint64_t off = 8 * i;
+    const uint32_t full = available >= off + 8
+                              ? uint32_t(std::min<uint64_t>(blocks, (available - off - 8) / block_bytes + 1))
+                              : 0u;
+    uint32_t b = 0;
+    for (; b + 4 <= full; b += 4) {
+      uint2 u[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) u[k] = *reinterpret_cast<const uint2*>(bva + uint64_t(b + k) * block_bytes + off);
+#pragma unroll
+      for (int k = 0; k < 4; k++) add_block(u[k].x, u[k].y, b + k);
+      if ((b + 4) % 16 == 0) fold_acc();
+    }
+    for (; b < full; b++) {
+      const uint2 u = *reinterpret_cast<const uint2*>(bva + uint64_t(b) * block_bytes + off);
+      add_block(u.x, u.y, b);
+    }
+    uint32_t nb = full;
+    if (b < blocks) {  // the block that straddles `available`; later blocks are zero and skipped
+      const uint64_t byte = uint64_t(b) * block_bytes + off;
+      uint32_t x[2] = {0, 0};
+      for (int j = 0; j < 8; j++) if (byte + j < available) x[j >> 2] |= uint32_t(bva[byte + j]) << (8 * (j & 3));
+      add_block(x[0], x[1], b);
+      nb = b + 1;
+    }
+    fold_acc();
+    V8 bias;
+#pragma unroll
+    for (int c = 0; c < 8; c++) bias.c[c] = negb[nb][c];
+    values[i] = v8_store(v8_add(total, bias));
+  }
+}
+#else
 __global__ __launch_bounds__(WG) void k_block_fold(const uint8_t* __restrict__ bva, uint64_t available,
                                                    uint64_t block_bytes, uint32_t blocks, const E8* __restrict__ fw_g,
                                                    E8* __restrict__ values, uint64_t n) {
@@ -546,6 +905,7 @@ __global__ __launch_bounds__(WG) void k_block_fold(const uint8_t* __restrict__ b
     values[i] = v8_store(o);
   }
 }
+#endif
 
 struct Images { E8 m[LANES]; };
 __global__ __launch_bounds__(WG) void k_weight_images(E8* __restrict__ out, uint64_t n, const E8* __restrict__ hi,
@@ -851,42 +1211,27 @@ std::atomic<int> g_launch_mode{LAUNCH_BATCHED};
 void set_launch_mode(int mode) { g_launch_mode = mode; }
 int launch_mode() { return g_launch_mode; }
 
-std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
-                               const std::vector<std::array<E8, 2>>& pairs) {
-  require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
-  if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
-    // the plain per-matrix form: one launch (and one round trip) per bucket term
-    std::vector<E8> result;
-    for (const ProjTask& t : tasks) {
-      ProjTask one = t;
-      one.pair_off = 0;
-      std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
-      set_launch_mode(LAUNCH_BATCHED);
-      auto r = tensor_project(c, bva, available, L, {one}, own);
-      set_launch_mode(LAUNCH_PER_MATRIX);
-      result.insert(result.end(), r.begin(), r.end());
-    }
-    return result;
-  }
+// Launches the batched projection: one workgroup per chunk of <= 2^16 rows of
+// a task; returns the chunk lane sums (chunks x L, on the device) and each
+// chunk's task in `owner`.
+static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
+                           const std::vector<std::array<E8, 2>>& pairs, std::vector<uint32_t>& owner) {
   std::vector<ProjTaskDev> chunks;
-  std::vector<size_t> owner;
   for (size_t i = 0; i < tasks.size(); i++) {
     const ProjTask& t = tasks[i];
     require(pow2(t.rows) && uint64_t(1) << t.nbits == t.rows, QF_ERR_ARGUMENT, "projection task rows");
-    uint32_t k = std::min<uint32_t>(t.nbits, 12);
+    uint32_t k = std::min<uint32_t>(t.nbits, PROJ_TAB_LOG);
     uint32_t clog = std::min<uint32_t>(t.nbits, 16);
     for (uint64_t first = 0; first < t.rows; first += uint64_t(1) << clog) {
       if (t.base + first * L >= available) break;  // rows past the active data contribute zero
       chunks.push_back({t.base, t.rows, first, t.nbits, t.pair_off, k, clog, t.scale});
-      owner.push_back(i);
+      owner.push_back(uint32_t(i));
     }
   }
-  std::vector<E8> result(tasks.size() * L, E8{});
-  if (chunks.empty()) return result;
+  if (chunks.empty()) return DVec();
   E8(*d_pairs)[2] = nullptr;
-  ProjTaskDev* d_tasks = nullptr;
   upload_pairs(c, pairs, &d_pairs);
-  d_tasks = static_cast<ProjTaskDev*>(dmalloc(chunks.size() * sizeof(ProjTaskDev), c.st));
+  auto* d_tasks = static_cast<ProjTaskDev*>(dmalloc(chunks.size() * sizeof(ProjTaskDev), c.st));
   QF_HIP(hipMemcpyAsync(d_tasks, chunks.data(), chunks.size() * sizeof(ProjTaskDev), hipMemcpyHostToDevice, c.st));
   DVec out(chunks.size() * L);
   for (size_t first = 0; first < chunks.size(); first += 65535) {
@@ -904,10 +1249,60 @@ std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, in
     }
   }
   QF_HIP(hipGetLastError());
-  auto parts = dv::download(c, out);
+  // the host vectors of pairs and chunks are copied before the call returns
+  c.sync();
   dfree(d_pairs, c.st);
   dfree(d_tasks, c.st);
-  for (size_t ch = 0; ch < chunks.size(); ch++)
+  return out;
+}
+
+std::pair<std::vector<E8>, std::vector<E8>> tensor_project_cross(DevCtx& c, const Val* bva, uint64_t available, int L,
+                                                                 const std::vector<ProjTask>& tasks,
+                                                                 const std::vector<std::array<E8, 2>>& pairs,
+                                                                 const std::vector<E8>& lows) {
+  require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
+  require(lows.size() == tasks.size() * size_t(L), QF_ERR_ARGUMENT, "projection low weights");
+  std::vector<E8> pos(L - 1, E8{}), neg(L - 1, E8{});
+  std::vector<uint32_t> owner;
+  DVec parts = project_chunks(c, bva, available, L, tasks, pairs, owner);
+  if (owner.empty()) return {pos, neg};
+  DVec d_lows = dv::upload(c, lows);
+  auto* d_owner = static_cast<uint32_t*>(dmalloc(owner.size() * 4, c.st));
+  QF_HIP(hipMemcpyAsync(d_owner, owner.data(), owner.size() * 4, hipMemcpyHostToDevice, c.st));
+  DVec out(2 * (L - 1));
+  hipLaunchKernelGGL(k_proj_cross, dim3(2 * (L - 1)), dim3(WG), 0, c.st, parts.p, d_owner, uint32_t(owner.size()), L,
+                     d_lows.p, out.p);
+  QF_HIP(hipGetLastError());
+  auto r = dv::download(c, out);
+  dfree(d_owner, c.st);
+  std::copy(r.begin(), r.begin() + (L - 1), pos.begin());
+  std::copy(r.begin() + (L - 1), r.end(), neg.begin());
+  return {pos, neg};
+}
+
+std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
+                               const std::vector<std::array<E8, 2>>& pairs) {
+  require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
+  if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
+    // the plain per-matrix form: one launch (and one round trip) per bucket term
+    std::vector<E8> result;
+    for (const ProjTask& t : tasks) {
+      ProjTask one = t;
+      one.pair_off = 0;
+      std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
+      set_launch_mode(LAUNCH_BATCHED);
+      auto r = tensor_project(c, bva, available, L, {one}, own);
+      set_launch_mode(LAUNCH_PER_MATRIX);
+      result.insert(result.end(), r.begin(), r.end());
+    }
+    return result;
+  }
+  std::vector<uint32_t> owner;
+  DVec out = project_chunks(c, bva, available, L, tasks, pairs, owner);
+  std::vector<E8> result(tasks.size() * L, E8{});
+  if (owner.empty()) return result;
+  auto parts = dv::download(c, out);
+  for (size_t ch = 0; ch < owner.size(); ch++)
     for (int l = 0; l < L; l++) result[owner[ch] * L + l] = e8_add(result[owner[ch] * L + l], parts[ch * L + l]);
   return result;
 }
@@ -928,14 +1323,16 @@ void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, 
   c.sync();
 }
 
-void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, const std::vector<std::array<E8, 2>>& pairs) {
+template <bool ADD>
+static void fill_tasks(DevCtx& c, E8* out, const std::vector<FillTask>& tasks,
+                       const std::vector<std::array<E8, 2>>& pairs) {
   if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
     for (const FillTask& t : tasks) {
       FillTask one = t;
       one.pair_off = 0;
       std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
       set_launch_mode(LAUNCH_BATCHED);
-      tensor_fill_add(c, out, {one}, own);
+      fill_tasks<ADD>(c, out, {one}, own);
       set_launch_mode(LAUNCH_PER_MATRIX);
     }
     return;
@@ -955,12 +1352,69 @@ void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, con
   QF_HIP(hipMemcpyAsync(d_tasks, chunks.data(), chunks.size() * sizeof(FillTaskDev), hipMemcpyHostToDevice, c.st));
   for (size_t first = 0; first < chunks.size(); first += 65535) {
     uint32_t n = uint32_t(std::min<size_t>(65535, chunks.size() - first));
-    hipLaunchKernelGGL(k_tensor_fill_add, dim3(n), dim3(WG), 0, c.st, out, d_tasks + first, (const E8(*)[2])d_pairs);
+    hipLaunchKernelGGL(k_tensor_fill<ADD>, dim3(n), dim3(FWG), 0, c.st, out, d_tasks + first, (const E8(*)[2])d_pairs);
   }
   QF_HIP(hipGetLastError());
-  c.sync();
+  c.sync();  // the host vectors of pairs and chunks are copied before the call returns
   dfree(d_pairs, c.st);
   dfree(d_tasks, c.st);
+}
+
+void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, const std::vector<std::array<E8, 2>>& pairs) {
+  fill_tasks<true>(c, out, tasks, pairs);
+}
+
+void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& tasks,
+                 const std::vector<std::array<E8, 2>>& pairs) {
+  // the rows of [0, n) outside every task, in chunks of at most 2^16 rows
+  std::vector<std::pair<uint64_t, uint64_t>> spans;
+  for (const FillTask& t : tasks) spans.push_back({t.start, t.start + (uint64_t(1) << t.nbits)});
+  std::sort(spans.begin(), spans.end());
+  std::vector<uint64_t> gaps;
+  uint64_t at = 0;
+  auto gap = [&](uint64_t a, uint64_t b) {
+    for (; a < b; a += std::min<uint64_t>(b - a, uint64_t(1) << 16)) {
+      gaps.push_back(a);
+      gaps.push_back(std::min<uint64_t>(b - a, uint64_t(1) << 16));
+    }
+  };
+  for (auto& [a, b] : spans) {
+    require(a >= at && b <= n, QF_ERR_ARGUMENT, "fill tasks overlap or leave the vector");
+    gap(at, a);
+    at = b;
+  }
+  gap(at, n);
+  if (!gaps.empty()) {
+    auto* d_gaps = static_cast<uint64_t*>(dmalloc(gaps.size() * 8, c.st));
+    QF_HIP(hipMemcpyAsync(d_gaps, gaps.data(), gaps.size() * 8, hipMemcpyHostToDevice, c.st));
+    for (size_t first = 0; first < gaps.size() / 2; first += 65535) {
+      uint32_t cnt = uint32_t(std::min<size_t>(65535, gaps.size() / 2 - first));
+      hipLaunchKernelGGL(k_zero_ranges, dim3(cnt), dim3(WG), 0, c.st, out, d_gaps + 2 * first);
+    }
+    QF_HIP(hipGetLastError());
+    c.sync();
+    dfree(d_gaps, c.st);
+  }
+  fill_tasks<false>(c, out, tasks, pairs);
+}
+
+std::array<E8, 3> eo_witness_first(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s,
+                                   const std::vector<E8>& pinv, const E8* q, E8* w) {
+  require(s >= 2 && s <= 64 && pow2(s) && int(pinv.size()) == s, QF_ERR_ARGUMENT, "witness stride");
+  require(n_rows >= 2 && n_rows % 2 == 0, QF_ERR_ARGUMENT, "witness rows");
+  E8* t = put_table(c, scratch_tab(), pinv.data(), pinv.size() * 16);
+  const uint64_t h = n_rows / 2;
+  const uint32_t g = groups_for(h);
+  switch (s) {
+    case 2: hipLaunchKernelGGL(k_eo_witness_first<2>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
+    case 4: hipLaunchKernelGGL(k_eo_witness_first<4>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
+    case 8: hipLaunchKernelGGL(k_eo_witness_first<8>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
+    case 16: hipLaunchKernelGGL(k_eo_witness_first<16>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
+    case 32: hipLaunchKernelGGL(k_eo_witness_first<32>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
+    default: hipLaunchKernelGGL(k_eo_witness_first<64>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
+  }
+  QF_HIP(hipGetLastError());
+  return finish_sums<3>(c, g);
 }
 
 void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,

@@ -101,12 +101,27 @@ struct ProjTask {
 // Returns tasks.size() x L lane sums.
 std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
                                const std::vector<std::array<E8, 2>>& pairs);
+// The shifted correlations of the projections with per-task low-bit weights
+// (lows: tasks x L), without bringing the projections to the host:
+// pos[d-1] = sum_task sum_lane proj[lane] low[lane+d], neg[d-1] = sum proj[lane+d] low[lane].
+std::pair<std::vector<E8>, std::vector<E8>> tensor_project_cross(DevCtx& c, const Val* bva, uint64_t available, int L,
+                                                                 const std::vector<ProjTask>& tasks,
+                                                                 const std::vector<std::array<E8, 2>>& pairs,
+                                                                 const std::vector<E8>& lows);
 // Packed EO witness: w[row] = sum_lane v[row*s + lane] * pinv[lane] for rows < n_rows.
 void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
                 E8* w);
 // out[start + j] += scale * prod_b pair_b[(j >> b) & 1], j < 2^nbits, per task (tasks disjoint).
 struct FillTask { uint64_t start; uint32_t nbits, pair_off; E8 scale; };
 void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, const std::vector<std::array<E8, 2>>& pairs);
+// The same with assignment, over out[0, n): task rows are written without
+// being read and the rows outside every task are zeroed.
+void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& tasks,
+                 const std::vector<std::array<E8, 2>>& pairs);
+// eo_witness for n_rows (even) rows fused with dv::eo_first against the built
+// query q: {<w, q>, sum w[2i] q[2i+1], sum w[2i+1] q[2i]}.
+std::array<E8, 3> eo_witness_first(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s,
+                                   const std::vector<E8>& pinv, const E8* q, E8* w);
 // values[i] = sum_b fw[b] * packed(b, i), lane j of packed(b, i) = entry b*block_len + LANES*i + j
 void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
                 const std::vector<E8>& fw, E8* values, uint64_t n);

@@ -61,6 +61,7 @@ def test_demo_proof_matches_reference_digest(oracle, recompute):
     (16, [(513, 129), (33, 65)], 5, 5),  # ragged shapes, packed_log 5
     (12, [(64, 64)], 6, 1),              # packed_log 1 (two lanes)
     (14, [(300, 200)], 7, 6),            # packed_log 6
+    (10, [(150, 150)], 8, 4),            # 22 blocks: block sums folded every 16 blocks
 ])
 def test_random_proofs_match_oracle(oracle, chunk_log, shapes, seed, packed_log):
     p = params(shapes, chunk_log, packed_log)
