@@ -247,6 +247,7 @@ int qf_params_bucket(const qf_params* p, uint64_t i, uint64_t out[8]) {
 int qf_commit(const qf_params* p, const void* values, size_t len, int cache_mode, qf_committed** out) {
   return guarded([&] {
     require(p && out && (values || len == 0), QF_ERR_ARGUMENT, "null argument");
+    device_setup();
     Val* d = nullptr;
     QF_HIP(hipMalloc(&d, std::max<size_t>(len * sizeof(Val), 16)));
     try {
@@ -387,6 +388,7 @@ int qf_test_tile_pass(int mode, const void* device_in, void* device_out1, void* 
                       int s, int t, int xform_vectors) {
   return guarded([&] {
     require(device_in && device_out1 && L >= 0 && L < 40, QF_ERR_ARGUMENT, "null argument");
+    device_setup();
     const E8* alpha = static_cast<const E8*>(device_alpha);
     E8* enc = nullptr;  // alpha_1 in the encoding kernels' form
     if (alpha && LANES != 8) {
@@ -413,6 +415,7 @@ int qf_set_launch_mode(int mode) {
 int qf_device_memory(size_t* free_bytes, size_t* total_bytes) {
   return guarded([&] {
     require(free_bytes && total_bytes, QF_ERR_ARGUMENT, "null argument");
+    device_setup();
     QF_HIP(hipMemGetInfo(free_bytes, total_bytes));
   });
 }

@@ -81,6 +81,7 @@ extern "C" {
 
 int qf_device_alloc(size_t bytes, void** out) {
   return guard([&] {
+    device_setup();
     require(out != nullptr, QF_ERR_ARGUMENT, "null argument");
     QF_HIP(hipMalloc(out, bytes ? bytes : 1));
     QF_HIP(hipMemset(*out, 0, bytes ? bytes : 1));

@@ -21,6 +21,10 @@ hipStream_t current_stream();
 // kept for the life of the process): commits and proofs issued from one
 // thread run on it, so pooled blocks are reused in stream order.
 hipStream_t thread_stream();
+// Once per process, before the library's first device call: host waits on
+// the device spin instead of sleeping (the opening's round trips are short).
+// Has no effect when the process already holds a device context.
+void device_setup();
 
 struct CommitGeom {
   int log_n;          // log2 of message rows per column (chunk_log - 8)

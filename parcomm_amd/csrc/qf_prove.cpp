@@ -23,6 +23,18 @@ const E8* ParamsState::alpha_device(size_t level) const {
   }
   return d_alpha_levels[level];
 }
+const E8* ParamsState::setup_device(size_t level) const {
+  std::lock_guard<std::mutex> g(mu);
+  if (d_setup_enc.size() != pp.setup.size()) d_setup_enc.assign(pp.setup.size(), nullptr);
+  if (!d_setup_enc[level]) {
+    const auto& e = pp.setup[level].encoded;
+    E8* d = nullptr;
+    QF_HIP(hipMalloc(&d, std::max<size_t>(e.size(), 1) * 16));
+    if (!e.empty()) QF_HIP(hipMemcpy(d, e.data(), e.size() * 16, hipMemcpyHostToDevice));
+    d_setup_enc[level] = d;
+  }
+  return d_setup_enc[level];
+}
 const E8* ParamsState::alpha_encode(size_t level) const {
   const E8* a = alpha_device(level);
   if (LANES == 8) return a;  // F65521 encodes with the canonical form
@@ -268,13 +280,8 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
     DVec rv(R);
     cache.fold(c, R, fr, rv.p);
     sampled = dv::download(c, rv);
-  } else {
-    auto rows = cache.rows(c, sel);
-    for (size_t k = 0; k < sel.size(); k++) {
-      E8 s{};
-      for (int col = 0; col < 32; col++) s += fr[col] * rows[k * 32 + col];
-      sampled.push_back(s);
-    }
+  } else {  // sampled[k] = sum_col fr[col] codeword[col][sel[k]]
+    sampled = dv::gather_weighted(c, cache.data(), R, cache.cols, nullptr, 0, 0, fr, sel);
   }
   tr.exts("sampled-values", sampled);
   return round_reductions(c, p, level, point, fv, folded, row_w, sel, sampled, tr);
@@ -465,16 +472,10 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     tr.u64("iota-carry-cols", phys_carry);
     absorb_sel(tr, sel);
     tr.exts("iota-column-weights", iw);
-    std::vector<E8> carry_rows;
-    if (have_carry) carry_rows = carry.rows(c, srows);
-    std::vector<E8> iota;
-    for (size_t k = 0; k < srows.size(); k++) {
-      E8 v{};
-      for (uint64_t col = 0; col < scols; col++) v += iw[col] * sl.encoded[col * n_enc + srows[k]];
-      if (have_carry)
-        for (uint64_t col = 0; col < phys_carry; col++) v += iw[scols + col] * carry_rows[k * 32 + col];
-      iota.push_back(v);
-    }
+    // iota[k] = sum_col iw[col] setup[col][row_k] + sum_col iw[scols + col] carry[col][row_k], on the device
+    const E8* d_setup = p.setup_device(level);
+    std::vector<E8> iota = dv::gather_weighted(c, d_setup, n_enc, scols, have_carry ? carry.data() : nullptr,
+                                               have_carry ? carry.R : 0, have_carry ? phys_carry : 0, iw, srows);
     for (auto& v : iota) tr.respond(v);
     Bytes round;
     put_u64(round, level);
@@ -498,10 +499,12 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     } else {
       round.push_back(0);
       put_u64(round, sel.rows.size());
-      for (uint64_t r : sel.rows) {
+      auto srow_vals = dv::gather_rows(c, d_setup, n_enc, scols, sel.rows);  // row-major, contiguous
+      for (size_t k = 0; k < sel.rows.size(); k++) {
+        const uint64_t r = sel.rows[k];
         put_u64(round, r);
         put_u64(round, scols);
-        for (uint64_t col = 0; col < scols; col++) put_e8(round, sl.encoded[col * n_enc + r]);
+        for (uint64_t col = 0; col < scols; col++) put_e8(round, srow_vals[k * scols + col]);
         uint64_t idx = r;
         put_u64(round, sl.tree.size() - 1);
         for (size_t l = 0; l + 1 < sl.tree.size(); l++) { put_digest(round, sl.tree[l][idx ^ 1]); idx >>= 1; }

@@ -19,9 +19,11 @@ struct ParamsState {
   mutable E8* d_alpha = nullptr;              // top-code alpha_1, uploaded on first use
   mutable std::vector<E8*> d_alpha_levels;    // alpha_1 of every level (top included), on first prove
   mutable std::vector<E8*> d_alpha_enc;       // the same in encode form (KoalaBear: Montgomery)
+  mutable std::vector<E8*> d_setup_enc;       // setup level l's encoded matrix (column-major), on first Decide
   mutable std::vector<uint64_t> zero_ranges;  // padding intervals (start, len) chunked for the device check
   mutable std::mutex mu;                      // guards the lazily filled members above
   ~ParamsState() {
+    for (auto* a : d_setup_enc) if (a) (void)hipFree(a);
     for (auto* a : d_alpha_enc)
       if (a && std::find(d_alpha_levels.begin(), d_alpha_levels.end(), a) == d_alpha_levels.end()) (void)hipFree(a);
     for (auto* a : d_alpha_levels) if (a && a != d_alpha) (void)hipFree(a);
@@ -31,6 +33,8 @@ struct ParamsState {
   const E8* alpha_device(size_t level) const;
   // alpha_1 of code `level` in the encoding kernels' form (v8_encode_form)
   const E8* alpha_encode(size_t level) const;
+  // setup level `level`'s encoded matrix in device memory (same layout as pp.setup[level].encoded)
+  const E8* setup_device(size_t level) const;
 };
 
 struct CommittedState {

@@ -228,6 +228,17 @@ __global__ void k_gather_rows(const E8* cw, uint64_t R, uint64_t cols, const uin
   uint64_t k = i / cols, c = i % cols;
   out[i] = cw[c * R + rows[k]];
 }
+__global__ void k_gather_weighted(const E8* __restrict__ a, uint64_t Ra, uint64_t ca, const E8* __restrict__ b,
+                                  uint64_t Rb, uint64_t cb, const E8* __restrict__ w, const uint64_t* __restrict__ rows,
+                                  uint64_t count, E8* __restrict__ out) {
+  const uint64_t k = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (k >= count) return;
+  const uint64_t r = rows[k];
+  V8 s = v8_zero();
+  for (uint64_t col = 0; col < ca; col++) s = v8_add(s, v8_mul(v8_load(a[col * Ra + r]), v8_load(w[col])));
+  for (uint64_t col = 0; col < cb; col++) s = v8_add(s, v8_mul(v8_load(b[col * Rb + r]), v8_load(w[ca + col])));
+  out[k] = v8_store(s);
+}
 __global__ void k_gather_paths(const uint32_t* tree, uint64_t R, int depth, const uint64_t* rows, uint64_t count,
                                uint32_t* out) {
   uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
@@ -989,12 +1000,20 @@ BlockCache& cache() {
 }
 }  // namespace
 hipStream_t current_stream() { return g_current_stream; }
+void device_setup() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
+  });
+}
 hipStream_t thread_stream() {
+  device_setup();
   thread_local hipStream_t st = nullptr;
   if (!st) QF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   return st;
 }
 void* dmalloc(size_t bytes, hipStream_t st) {
+  device_setup();
   BlockCache& bc = cache();
   const size_t size = BlockCache::round(bytes);
   {
@@ -1164,6 +1183,22 @@ std::vector<E8> gather_rows(DevCtx& c, const E8* cw, uint64_t R, uint64_t cols, 
   auto out = download(c, tmp.p, total);
   dfree(d_rows, c.st);
   return out;
+}
+std::vector<E8> gather_weighted(DevCtx& c, const E8* a, uint64_t Ra, uint64_t ca, const E8* b, uint64_t Rb, uint64_t cb,
+                                const std::vector<E8>& w, const std::vector<uint64_t>& rows) {
+  require(w.size() >= ca + cb && (cb == 0 || b), QF_ERR_ARGUMENT, "gather weights");
+  if (rows.empty()) return {};
+  for (uint64_t r : rows) require(r < Ra && (cb == 0 || r < Rb), QF_ERR_ARGUMENT, "gather row out of range");
+  uint64_t* d_rows = static_cast<uint64_t*>(dmalloc(rows.size() * 8, c.st));
+  QF_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, c.st));
+  DVec d_w = upload(c, std::vector<E8>(w.begin(), w.begin() + (ca + cb)));
+  DVec out(rows.size());
+  hipLaunchKernelGGL(k_gather_weighted, dim3(uint32_t((rows.size() + WG - 1) / WG)), dim3(WG), 0, c.st, a, Ra, ca, b,
+                     Rb, cb, d_w.p, d_rows, uint64_t(rows.size()), out.p);
+  QF_HIP(hipGetLastError());
+  auto r = download(c, out);
+  dfree(d_rows, c.st);
+  return r;
 }
 std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, uint64_t R,
                                               const std::vector<uint64_t>& rows) {

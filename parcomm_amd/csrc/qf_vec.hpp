@@ -74,6 +74,10 @@ std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, 
 void col_fold(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& f, E8* out);
 // Gather full rows (cols entries each) of a column-major matrix: out[k*cols + col].
 std::vector<E8> gather_rows(DevCtx& c, const E8* cw, uint64_t R, uint64_t cols, const std::vector<uint64_t>& rows);
+// out[k] = sum_{col < ca} w[col] a[col*Ra + rows[k]] + sum_{col < cb} w[ca + col] b[col*Rb + rows[k]]
+// (column-major a and b; b may be null when cb = 0).
+std::vector<E8> gather_weighted(DevCtx& c, const E8* a, uint64_t Ra, uint64_t ca, const E8* b, uint64_t Rb, uint64_t cb,
+                                const std::vector<E8>& w, const std::vector<uint64_t>& rows);
 // Merkle paths (siblings leaf to root) of `rows` in a device tree of R leaves.
 std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, uint64_t R,
                                               const std::vector<uint64_t>& rows);
