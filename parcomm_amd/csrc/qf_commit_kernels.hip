@@ -313,10 +313,20 @@ __global__ void k_encode_form(const E8* in, E8* out, uint64_t n) {
     out[i] = v8_store(v8_encode_form(v8_load(in[i])));
 }
 
-// High index bits [12, L) in groups of at most 10 (tile passes with C = 4096 >> t contiguous positions).
+// High index bits [12, L) in the fewest groups of at most 10, split as evenly
+// as possible (tile passes with C = 4096 >> t contiguous positions): fewer
+// bits per pass mean longer contiguous runs per tile row at the same pass
+// count, e.g. L = 23 as (12, 6) + (18, 5) rather than (12, 10) + (22, 1).
 static std::vector<std::pair<int, int>> hi_groups(int L) {
   std::vector<std::pair<int, int>> gs;
-  for (int s = 12; s < L;) { int t = L - s < 10 ? L - s : 10; gs.push_back({s, t}); s += t; }
+  const int m = L - 12;
+  if (m <= 0) return gs;
+  const int passes = (m + 9) / 10, base = m / passes, extra = m % passes;
+  for (int i = 0, s = 12; i < passes; i++) {
+    const int t = base + (i < extra ? 1 : 0);
+    gs.push_back({s, t});
+    s += t;
+  }
   return gs;
 }
 
