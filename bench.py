@@ -55,6 +55,17 @@ def llama8b_shapes():
     return [(128256, 4096), (128256, 4096)] + layer * 32
 
 
+def llama70b_shapes():
+    layer = [(8192, 8192), (1024, 8192), (1024, 8192), (8192, 8192), (28672, 8192), (28672, 8192), (8192, 28672)]
+    return [(128256, 8192), (128256, 8192)] + layer * 80
+
+
+MODELS = {  # shapes, packed_log, commit form (the 70B codewords, 512 GiB, do not fit HBM: roots only)
+    "8b": (llama8b_shapes, 4, "commit"),
+    "70b": (llama70b_shapes, 6, "commit_recomputed"),
+}
+
+
 def cpu_baseline(seed=5, kb=False):
     """The CPU oracle (a parity-checked restatement, not the reference binary)
     running the whole CLI flow (commit, benchmark claims, prove) on a bounded
@@ -84,22 +95,26 @@ def cpu_baseline(seed=5, kb=False):
     }
 
 
-def measure(pa, field, args, per_matrix=True, serve_requests=0):
+def measure(pa, field, args, per_matrix=True, serve_requests=0, model="8b", steps=None, warmup=None):
     """Setup, resident synthetic BVA, warmup + timed commit+open steps, an
     untimed verify, optionally the per-matrix form and the serving loop."""
     kb = field == pa.KOALABEAR
-    tag = "koalabear" if kb else "f65521"
-    shapes = llama8b_shapes()
+    tag = ("koalabear" if kb else "f65521") + "-" + model
+    make_shapes, packed_log, commit_form = MODELS[model]
+    shapes = make_shapes()
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     t0 = time.perf_counter()
-    params = pa.Parameters([pa.Shape(r, c) for r, c in shapes], pa.Config(chunk_log=args.chunk_log, packed_log=4),
-                           field=field)
+    params = pa.Parameters([pa.Shape(r, c) for r, c in shapes],
+                           pa.Config(chunk_log=args.chunk_log, packed_log=packed_log), field=field)
     setup_s = time.perf_counter() - t0
     print(f"# {tag}: setup {setup_s:.2f} s, active {params.active}, blocks {params.blocks}", file=sys.stderr,
           flush=True)
     bva = params.synthetic_bva(args.seed)
     print(f"# {tag}: bva resident", file=sys.stderr, flush=True)
 
-    committed = params.commit(bva)
+    commit = getattr(params, commit_form)
+    committed = commit(bva)
     claims = params.benchmark_claims(committed)
     roots = committed.statement.roots
     committed.close()
@@ -107,7 +122,7 @@ def measure(pa, field, args, per_matrix=True, serve_requests=0):
 
     def step():
         t0 = time.perf_counter()
-        c = params.commit(bva)
+        c = commit(bva)
         t1 = time.perf_counter()
         assert c.statement.roots == roots
         proof, profile = params.prove(c, claims)
@@ -116,22 +131,23 @@ def measure(pa, field, args, per_matrix=True, serve_requests=0):
         c.close()
         return {"commit_s": t1 - t0, "open_s": t2 - t1, "proof_bytes": len(proof), "commit": t, "profile": profile}
 
-    for i in range(args.warmup):
+    for i in range(warmup):
         r = step()
         print(f"# {tag} warmup {i}: commit {r['commit_s']:.3f} s open {r['open_s']:.3f} s", file=sys.stderr,
               flush=True)
     t0 = time.perf_counter()
     timings = []
-    for i in range(args.steps):
+    for i in range(steps):
         timings.append(step())
         r = timings[-1]
         print(f"# {tag} step {i}: commit {r['commit_s']:.3f} s open {r['open_s']:.3f} s "
               f"profile {json.dumps({k: round(v, 4) for k, v in r['profile'].items()})}", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
-    res = {"params": params, "timings": timings, "s_per_step": elapsed / args.steps, "setup_s": setup_s}
+    res = {"params": params, "timings": timings, "s_per_step": elapsed / steps, "setup_s": setup_s,
+           "model": model, "cache_mode": "cached" if commit_form == "commit" else "recompute"}
 
     # the witness-free verifier on the last proof (host, untimed; reference: 0.157 s on one thread)
-    c = params.commit(bva)
+    c = commit(bva)
     proof, _ = params.prove(c, claims)
     tv = time.perf_counter()
     params.verify(c.statement, claims, proof)
@@ -249,6 +265,7 @@ def main():
                     help="headline backend; the default run also measures KoalaBear beside it")
     ap.add_argument("--no-koalabear", action="store_true", help="skip the KoalaBear line of the default run")
     ap.add_argument("--no-per-matrix", action="store_true", help="skip the per-matrix comparison step")
+    ap.add_argument("--no-70b", action="store_true", help="skip the 70B-shaped line of the default run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a plain copy kernel of known size, for the "
                          "HBM traffic counters)")
@@ -299,6 +316,16 @@ def main():
         if not args.no_cpu_baseline:
             kb["cpu_baseline"] = cpu_baseline(kb=True)
         config["koalabear"] = kb
+    if not kb_head and not args.no_70b:
+        # the 70B-shaped F65521 configuration: 562 matrices, packed_log 6, roots-only commitment
+        r70 = measure(pa, pa.F65521, args, per_matrix=False, serve_requests=0, model="70b", steps=1, warmup=0)
+        l70 = summary(r70)
+        l70["phase_hbm"] = phase_hbm(pa, r70)
+        l70.update({"workload": "llama3.1-70b-shaped int8 BVA, 562 matrices, chunk_log 30, packed_log 6",
+                    "active_entries": r70["params"].active, "blocks": r70["params"].blocks,
+                    "cache_mode": "recompute", "steps": 1, "warmup": 0})
+        del r70
+        config["llama70b"] = l70
     out = {
         "metric": metric,
         "value": round(s_per_step, 4),
