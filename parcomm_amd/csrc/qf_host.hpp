@@ -90,10 +90,21 @@ inline uint32_t fp_inv(uint32_t a) { require(a != 0, QF_ERR_ENCODING, "zero inve
 inline E8 e8_scale(const E8& a, uint32_t s) { E8 r; for (int i = 0; i < LANES; i++) r.c[i] = Coef(fp_mul(a.c[i], s)); return r; }
 // x -> x^(p^k): coefficient i times FROB_ROOT^(i k mod LANES) (qafold_short.rs:486-525,
 // koalabear.rs:153-163)
+// frob_table()[k][i] = FROB_ROOT^((i k) mod LANES): the k-th Frobenius power scales coefficient i by it.
+inline const std::array<std::array<uint32_t, LANES>, LANES>& frob_table() {
+  static const auto t = [] {
+    std::array<std::array<uint32_t, LANES>, LANES> m{};
+    for (int k = 0; k < LANES; k++)
+      for (int i = 0; i < LANES; i++) m[k][i] = uint32_t(fp_pow(FROB_ROOT, uint64_t((i * k) % LANES)));
+    return m;
+  }();
+  return t;
+}
 inline E8 e8_frob(const E8& a, int k) {
   k = ((k % LANES) + LANES) % LANES;
+  const auto& m = frob_table()[k];
   E8 r;
-  for (int i = 0; i < LANES; i++) r.c[i] = Coef(fp_mul(a.c[i], fp_pow(FROB_ROOT, uint64_t((i * k) % LANES))));
+  for (int i = 0; i < LANES; i++) r.c[i] = Coef(fp_mul(a.c[i], m[i]));
   return r;
 }
 inline E8 e8_inv(const E8& a) {  // (product of the other conjugates) / norm

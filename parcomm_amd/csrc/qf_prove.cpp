@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <tuple>
 
 namespace qf {
 
@@ -165,14 +166,16 @@ LB litebullet(DevCtx& c, E8 claimed, DVec& left, DVec& right, Pcs& tr, const cha
   if (label) tr.bytes("litebullet-domain", label);
   LB lb;
   E8 rhs = claimed;
-  while (n > 1) {
+  E8 lc{}, rc{};
+  if (n > 1) std::tie(lc, rc) = dv::cross_halves(c, left.p, right.p, n / 2);
+  while (n > 1) {  // each fold also returns the next round's cross terms
     uint64_t h = n / 2;
-    auto [lc, rc] = dv::cross_halves(c, left.p, right.p, h);
     tr.respond(lc);
     tr.respond(rc);
     E8 ch = tr.challenge(fold_label, true), ci = e8_inv(ch);
     rhs += lc * ch + rc * ci;
-    dv::lb_fold(c, left.p, right.p, h, ci, ch);
+    if (h >= 2) std::tie(lc, rc) = dv::lb_fold_cross(c, left.p, right.p, h, ci, ch);
+    else dv::lb_fold(c, left.p, right.p, h, ci, ch);
     lb.a_point.push_back(ci);
     lb.b_point.push_back(ch);
     n = h;
@@ -424,14 +427,16 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     std::vector<E8> red_point;
     DVec folded_vector(rows);
     uint64_t n = padded * rows;
+    E8 lc{}, rc{};
+    if (n > 1) std::tie(lc, rc) = dv::cross_halves(c, left.p, right.p, n / 2);
     while (n > 1) {
       uint64_t hh = n / 2;
-      auto [lc, rc] = dv::cross_halves(c, left.p, right.p, hh);
       tr.respond(lc);
       tr.respond(rc);
       E8 ch = tr.challenge("multi-eval-fold", true), ci = e8_inv(ch);
       rhs += lc * ch + rc * ci;
-      dv::lb_fold(c, left.p, right.p, hh, ci, ch);
+      if (hh >= 2) std::tie(lc, rc) = dv::lb_fold_cross(c, left.p, right.p, hh, ci, ch);
+      else dv::lb_fold(c, left.p, right.p, hh, ci, ch);
       red_point.push_back(ci);
       n = hh;
       if (n == rows) dv::copy(c, folded_vector.p, left.p, rows);
@@ -751,15 +756,17 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
 void sumcheck(DevCtx& c, DVec& v, DVec& w, E8& claim, Packed& t, std::vector<std::array<E8, 2>>& rounds,
               std::vector<E8>& reduced, E8& fval, E8& fwt) {
   uint64_t n = v.n;
-  while (n > 1) {
+  E8 m0{}, m1{};
+  if (n > 1) std::tie(m0, m1) = dv::sc_msg(c, v.p, w.p, n / 2);
+  while (n > 1) {  // each fold also returns the next round's message
     uint64_t h = n / 2;
-    auto [m0, m1] = dv::sc_msg(c, v.p, w.p, h);
     t.field(m0);
     t.field(m1);
     E8 r = t.challenge();
     claim = m0 + r * ((claim - m0 - m0 - m1) + r * m1);
-    dv::sc_fold(c, v.p, w.p, h, r);
     rounds.push_back({m0, m1});
+    if (h >= 2) std::tie(m0, m1) = dv::sc_fold_msg(c, v.p, w.p, h, r);
+    else dv::sc_fold(c, v.p, w.p, h, r);
     reduced.push_back(r);
     n = h;
   }

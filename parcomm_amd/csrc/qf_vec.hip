@@ -82,6 +82,31 @@ __global__ __launch_bounds__(WG) void k_eo_first(const E8* __restrict__ w, const
   block_sums<3>(s, partials);
 }
 
+// One LiteBullet fold in place (a[i] += a[h+i] x, b[i] += b[h+i] y, i < h;
+// qafold_short.rs:2901-2926) fused with the next round's cross terms over the
+// folded halves (h even, g = h/2): sum_{i<g} a'[i] b'[g+i] and sum a'[g+i] b'[i]
+// (qafold_short.rs:2876-2898). Each thread owns positions i and g+i of both
+// halves, so the in-place update reads nothing another thread writes.
+__global__ __launch_bounds__(WG) void k_lb_fold_cross(E8* __restrict__ a, E8* __restrict__ b, uint64_t h, E8 x, E8 y,
+                                                      uint32_t* __restrict__ partials) {
+  const V8 vx = v8_encode_form(v8_load(x)), vy = v8_encode_form(v8_load(y));
+  const uint64_t g = h / 2;
+  V8 s[2] = {v8_zero(), v8_zero()};
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < g; i += uint64_t(gridDim.x) * WG) {
+    const V8 a0 = v8_add(v8_load(a[i]), v8_mul_enc(v8_load(a[h + i]), vx));
+    const V8 a1 = v8_add(v8_load(a[g + i]), v8_mul_enc(v8_load(a[h + g + i]), vx));
+    const V8 b0 = v8_add(v8_load(b[i]), v8_mul_enc(v8_load(b[h + i]), vy));
+    const V8 b1 = v8_add(v8_load(b[g + i]), v8_mul_enc(v8_load(b[h + g + i]), vy));
+    a[i] = v8_store(a0);
+    a[g + i] = v8_store(a1);
+    b[i] = v8_store(b0);
+    b[g + i] = v8_store(b1);
+    s[0] = v8_add(s[0], v8_mul(a0, b1));
+    s[1] = v8_add(s[1], v8_mul(a1, b0));
+  }
+  block_sums<2>(s, partials);
+}
+
 // One binary EO fold (eo.rs:57-79: w'[i] = w[2i] + w[2i+1] r^-1,
 // q'[i] = q[2i] + q[2i+1] r) of length-2h vectors into length-h outputs,
 // fused with the next round's cross terms over the outputs:
@@ -113,8 +138,7 @@ uint32_t groups_for(uint64_t n) {
 }
 
 std::pair<E8, E8> finish_partials(DevCtx& c, uint32_t groups) {
-  QF_HIP(hipMemcpyAsync(c.host_partials, c.partials, size_t(groups) * 64, hipMemcpyDeviceToHost, c.st));
-  c.sync();
+  c.sync();  // the block sums are in the pinned buffer once the kernel has finished
   V8 s0 = v8_zero(), s1 = v8_zero();
   for (uint32_t g = 0; g < groups; g++)
     for (int k = 0; k < LANES; k++) {
@@ -127,7 +151,6 @@ std::pair<E8, E8> finish_partials(DevCtx& c, uint32_t groups) {
 // Sums of the first NS extension elements over `groups` partial rows (stride PSTRIDE).
 template <int NS>
 std::array<E8, NS> finish_sums(DevCtx& c, uint32_t groups) {
-  QF_HIP(hipMemcpyAsync(c.host_partials, c.partials, size_t(groups) * PSTRIDE * 4, hipMemcpyDeviceToHost, c.st));
   c.sync();
   std::array<E8, NS> out;
   for (int j = 0; j < NS; j++) {
@@ -181,6 +204,32 @@ __global__ void k_sc_fold(E8* v, E8* w, uint64_t h, E8 r) {
     v[i] = v8_store(v8_add(vl, v8_mul_enc(v8_sub(v8_load(v[h + i]), vl), vr)));
     w[i] = v8_store(v8_add(wl, v8_mul_enc(v8_sub(v8_load(w[h + i]), wl), vr)));
   }
+}
+// One sumcheck fold in place (x_lo += r (x_hi - x_lo), i < h; packed.rs:674-696)
+// fused with the next round's message over the folded length h (h even, g =
+// h/2): m0 = sum_{i<g} v'[i] w'[i], m1 = sum (v'[g+i] - v'[i]) (w'[g+i] - w'[i])
+// (packed.rs:652-667). Each thread owns positions i and g+i of both vectors.
+__global__ __launch_bounds__(WG) void k_sc_fold_msg(E8* __restrict__ v, E8* __restrict__ w, uint64_t h, E8 r,
+                                                    uint32_t* __restrict__ partials) {
+  const V8 vr = v8_encode_form(v8_load(r));
+  const uint64_t g = h / 2;
+  V8 s[2] = {v8_zero(), v8_zero()};
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < g; i += uint64_t(gridDim.x) * WG) {
+    V8 x = v8_load(v[i]), y = v8_load(w[i]);
+    const V8 v0 = v8_add(x, v8_mul_enc(v8_sub(v8_load(v[h + i]), x), vr));
+    const V8 w0 = v8_add(y, v8_mul_enc(v8_sub(v8_load(w[h + i]), y), vr));
+    x = v8_load(v[g + i]);
+    y = v8_load(w[g + i]);
+    const V8 v1 = v8_add(x, v8_mul_enc(v8_sub(v8_load(v[h + g + i]), x), vr));
+    const V8 w1 = v8_add(y, v8_mul_enc(v8_sub(v8_load(w[h + g + i]), y), vr));
+    v[i] = v8_store(v0);
+    v[g + i] = v8_store(v1);
+    w[i] = v8_store(w0);
+    w[g + i] = v8_store(w1);
+    s[0] = v8_add(s[0], v8_mul(v0, w0));
+    s[1] = v8_add(s[1], v8_mul(v8_sub(v1, v0), v8_sub(w1, w0)));
+  }
+  block_sums<2>(s, partials);
 }
 __global__ void k_axpy(E8* y, const E8* x, uint64_t n, E8 s) {
   const V8 vs = v8_encode_form(v8_load(s));
@@ -1076,17 +1125,27 @@ DVec& DVec::operator=(DVec&& o) noexcept {
   return *this;
 }
 DVec::~DVec() { dfree(p, st); }
+// Reduction partials live in this thread's coherent pinned host buffer: the
+// kernels write their block sums there directly, so a reduction costs one
+// stream synchronisation and no copy.
+namespace {
+struct PinnedPartials {
+  uint32_t* host = nullptr;
+  ~PinnedPartials() { if (host) (void)hipHostFree(host); }
+};
+thread_local PinnedPartials g_pinned;
+}  // namespace
 DevCtx::DevCtx() {
   st = thread_stream();
   prev = g_current_stream;
   g_current_stream = st;
-  partials = static_cast<uint32_t*>(dmalloc(size_t(MAX_GROUPS) * 64 * 32, st));
-  QF_HIP(hipHostMalloc(&host_partials, size_t(MAX_GROUPS) * 64 * 32, 0));
+  if (!g_pinned.host)
+    QF_HIP(hipHostMalloc(&g_pinned.host, size_t(MAX_GROUPS) * 64 * 32, hipHostMallocMapped | hipHostMallocCoherent));
+  host_partials = g_pinned.host;
+  QF_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&partials), host_partials, 0));
 }
 DevCtx::~DevCtx() {
-  dfree(partials, st);
   if (st) (void)hipStreamSynchronize(st);
-  if (host_partials) (void)hipHostFree(host_partials);
   g_current_stream = prev;
 }
 void DevCtx::sync() { QF_HIP(hipStreamSynchronize(st)); }
@@ -1122,6 +1181,14 @@ std::pair<E8, E8> cross_halves(DevCtx& c, const E8* a, const E8* b, uint64_t h) 
 void lb_fold(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y) {
   hipLaunchKernelGGL(k_lb_fold, grid_for(h), dim3(WG), 0, c.st, a, b, h, x, y);
 }
+std::pair<E8, E8> lb_fold_cross(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y) {
+  require(h >= 2 && h % 2 == 0, QF_ERR_ARGUMENT, "fold length");
+  const uint32_t g = groups_for(h / 2);
+  hipLaunchKernelGGL(k_lb_fold_cross, dim3(g), dim3(WG), 0, c.st, a, b, h, x, y, c.partials);
+  QF_HIP(hipGetLastError());
+  auto s = finish_sums<2>(c, g);
+  return {s[0], s[1]};
+}
 std::pair<E8, E8> eo_cross(DevCtx& c, const E8* w, const E8* q, uint64_t h) { return reduce<R_EO>(c, w, q, h); }
 void eo_fold(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri, const E8& r) {
   hipLaunchKernelGGL(k_eo_fold, grid_for(h), dim3(WG), 0, c.st, w, q, ow, oq, h, ri, r);
@@ -1145,6 +1212,14 @@ std::pair<E8, E8> sc_msg(DevCtx& c, const E8* v, const E8* w, uint64_t h) { retu
 void sc_fold(DevCtx& c, E8* v, E8* w, uint64_t h, const E8& r) {
   hipLaunchKernelGGL(k_sc_fold, grid_for(h), dim3(WG), 0, c.st, v, w, h, r);
 }
+std::pair<E8, E8> sc_fold_msg(DevCtx& c, E8* v, E8* w, uint64_t h, const E8& r) {
+  require(h >= 2 && h % 2 == 0, QF_ERR_ARGUMENT, "fold length");
+  const uint32_t g = groups_for(h / 2);
+  hipLaunchKernelGGL(k_sc_fold_msg, dim3(g), dim3(WG), 0, c.st, v, w, h, r, c.partials);
+  QF_HIP(hipGetLastError());
+  auto s = finish_sums<2>(c, g);
+  return {s[0], s[1]};
+}
 void axpy(DevCtx& c, E8* y, const E8* x, uint64_t n, const E8& s) {
   hipLaunchKernelGGL(k_axpy, grid_for(n), dim3(WG), 0, c.st, y, x, n, s);
 }
@@ -1156,7 +1231,6 @@ std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, 
   uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(DevCtx::MAX_GROUPS / uint32_t(cols), groups_for(n)));
   hipLaunchKernelGGL(k_col_projection, dim3(g, uint32_t(cols)), dim3(WG), 0, c.st, cw, R, n, w, c.partials);
   QF_HIP(hipGetLastError());
-  QF_HIP(hipMemcpyAsync(c.host_partials, c.partials, size_t(g) * cols * 32, hipMemcpyDeviceToHost, c.st));
   c.sync();
   std::vector<E8> out(cols);
   for (uint64_t col = 0; col < cols; col++) {

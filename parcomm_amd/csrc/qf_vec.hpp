@@ -31,8 +31,8 @@ struct DVec {
 struct DevCtx {
   hipStream_t st = nullptr;
   hipStream_t prev = nullptr;
-  uint32_t* partials = nullptr;  // [max_groups][16] u32: two sums of LANES coefficients
-  uint32_t* host_partials = nullptr;
+  uint32_t* partials = nullptr;       // device address of host_partials: kernels write block sums here
+  uint32_t* host_partials = nullptr;  // pinned, coherent; the thread's buffer, read after sync()
   static constexpr uint32_t MAX_GROUPS = 1024;
   DevCtx();
   ~DevCtx();
@@ -52,6 +52,8 @@ E8 inner(DevCtx& c, const E8* a, const E8* b, uint64_t n);
 std::pair<E8, E8> cross_halves(DevCtx& c, const E8* a, const E8* b, uint64_t h);
 // a[i] += a[h+i] x; b[i] += b[h+i] y
 void lb_fold(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y);
+// lb_fold, returning the next round's cross_halves over the folded length h (h even)
+std::pair<E8, E8> lb_fold_cross(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y);
 // (sum w[2i] q[2i+1], sum w[2i+1] q[2i]), i < h
 std::pair<E8, E8> eo_cross(DevCtx& c, const E8* w, const E8* q, uint64_t h);
 // ow[i] = w[2i] + w[2i+1] ri; oq[i] = q[2i] + q[2i+1] r
@@ -65,6 +67,8 @@ std::pair<E8, E8> eo_fold_cross(DevCtx& c, const E8* w, const E8* q, E8* ow, E8*
 std::pair<E8, E8> sc_msg(DevCtx& c, const E8* v, const E8* w, uint64_t h);
 // v_lo += r (v_hi - v_lo), same for w
 void sc_fold(DevCtx& c, E8* v, E8* w, uint64_t h, const E8& r);
+// sc_fold, returning the next round's sc_msg over the folded length h (h even)
+std::pair<E8, E8> sc_fold_msg(DevCtx& c, E8* v, E8* w, uint64_t h, const E8& r);
 void axpy(DevCtx& c, E8* y, const E8* x, uint64_t n, const E8& s);  // y += s x
 void mul(DevCtx& c, E8* y, const E8* x, uint64_t n);                // y *= x
 void scale(DevCtx& c, E8* y, uint64_t n, const E8& s);              // y *= s
