@@ -413,6 +413,70 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
     for (int l = 0; l < Q; l++)
       for (int k = 0; k < LANES; k++) ai[l][k] = 0;
     const bool in_range = t.base + (row_hi + lo_rows) * L <= available;  // uniform: all but the last groups
+#ifndef QF_KB
+    // F65521 fast path over row pairs (2p, 2p+1): lanes biased to u = v + 128
+    // and paired across the two rows, weights paired likewise from the table,
+    // one v_dot2_u32_u16 per (lane, coefficient) and row pair; ts holds each
+    // coefficient's weight sum over the thread's rows and removes the bias.
+    // Per thread a group holds at most 32 row pairs: au < 2^30, 128 ts < 2^29.1.
+    if constexpr (Q == 4) {
+      constexpr int UP = 4;  // row pairs in flight per thread
+      if (in_range && lo_rows >= uint32_t(2 * RPP * UP)) {
+        uint32_t au[Q][LANES], ts[LANES];
+#pragma unroll
+        for (int k = 0; k < LANES; k++) {
+          ts[k] = 0;
+#pragma unroll
+          for (int l = 0; l < Q; l++) au[l][k] = 0;
+        }
+        for (uint32_t p0 = r0; 2 * p0 < lo_rows; p0 += RPP * UP) {
+          uint32_t raw0[UP], raw1[UP];
+#pragma unroll
+          for (int u = 0; u < UP; u++) {
+            const uint64_t row = 2 * uint64_t(p0 + u * RPP);
+            raw0[u] = load_raw<Q>(bva, first + row * L);
+            raw1[u] = load_raw<Q>(bva, first + (row + 1) * L);
+          }
+#pragma unroll
+          for (int u = 0; u < UP; u++) {
+            const uint32_t pr = p0 + u * RPP;
+            const uint4 t0 = *reinterpret_cast<const uint4*>(&T[2 * pr]);
+            const uint4 t1 = *reinterpret_cast<const uint4*>(&T[2 * pr + 1]);
+            const uint32_t w0[4] = {t0.x, t0.y, t0.z, t0.w}, w1[4] = {t1.x, t1.y, t1.z, t1.w};
+            uint32_t tp[LANES];  // (T[2p].c[k], T[2p+1].c[k]) as a u16 pair
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              tp[2 * j] = __builtin_amdgcn_perm(w1[j], w0[j], 0x05040100u);
+              tp[2 * j + 1] = __builtin_amdgcn_perm(w1[j], w0[j], 0x07060302u);
+            }
+            const uint32_t x0 = raw0[u] ^ 0x80808080u, x1 = raw1[u] ^ 0x80808080u;
+#pragma unroll
+            for (int l = 0; l < Q; l++) {
+              const uint32_t up = __builtin_amdgcn_perm(x1, x0, uint32_t(l) | 0x0c00u | (uint32_t(4 + l) << 16) | 0x0c000000u);
+#pragma unroll
+              for (int k = 0; k < LANES; k++) au[l][k] = dot2_u16(up, tp[k], au[l][k]);
+            }
+#pragma unroll
+            for (int k = 0; k < LANES; k++) ts[k] = dot2_u16(0x00010001u, tp[k], ts[k]);
+          }
+        }
+        const V8 c = v8_load(Chi[h]);
+        constexpr uint32_t OFF = uint32_t(P) * 16384u;  // a multiple of P above 128 ts
+#pragma unroll
+        for (int l = 0; l < Q; l++) {
+          V8 x;
+#pragma unroll
+          for (int k = 0; k < LANES; k++) {
+            const uint32_t y = fold16(fold16(au[l][k] + (OFF - (ts[k] << 7))));
+            const uint32_t z = y - P;
+            x.c[k] = y < z ? y : z;
+          }
+          acc[l] = v8_add(acc[l], v8_mul(x, c));
+        }
+        continue;
+      }
+    }
+#endif
     if (in_range && lo_rows >= uint32_t(RPP * U)) {
       // U unguarded row loads in flight per thread, then U rows of products
       for (uint32_t lo0 = r0; lo0 < lo_rows; lo0 += RPP * U) {
