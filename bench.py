@@ -237,6 +237,22 @@ def phase_hbm(pa, res):
     return out
 
 
+def autotune_summary(pa, field):
+    """The tile-pass autotuner's picks: per pass geometry the shape kept, its
+    time and the spread over the candidates tried (first use, scratch data)."""
+    out = []
+    for e in pa.autotune_report(field):
+        if e["vectors"] << e["L"] < (1 << 24):
+            continue  # test-sized geometries
+        b = e["best"]
+        times = [t["ms"] for t in e["tried"]]
+        out.append({"pass": f"mode {e['mode']} bits {e['s']}+{e['t']} of L={e['L']} x{e['vectors']}"
+                            + (" packed" if e["pack"] else ""),
+                    "kept": ("wide" if b["wide"] else "lean") + f" {b['per_cu']}/CU" + ("" if b["xcd"] else " no-xcd"),
+                    "ms": round(min(times), 4), "slowest_ms": round(max(times), 4)})
+    return out
+
+
 def summary(res):
     timings = res["timings"]
     mean = lambda k: sum(t[k] for t in timings) / len(timings)
@@ -295,6 +311,7 @@ def main():
     roofline = roofline_of(pa, res, args.field)
     summ = summary(res)
     summ["phase_hbm"] = phase_hbm(pa, res)
+    summ["autotune"] = autotune_summary(pa, field)
     if kb_head:
         workload = "llama3.1-8b-shaped bf16-pattern int16 BVA (KoalaBear), 226 matrices, chunk_log 30, packed_log 4"
         dtype = "u32 (KoalaBear / Ext4 exact integer)"
@@ -312,6 +329,7 @@ def main():
         kb = summary(kb_res)
         kb["phase_hbm"] = phase_hbm(pa, kb_res)
         kb["roofline"] = roofline_of(pa, kb_res, "koalabear")
+        kb["autotune"] = autotune_summary(pa, pa.KOALABEAR)
         del kb_res
         if not args.no_cpu_baseline:
             kb["cpu_baseline"] = cpu_baseline(kb=True)

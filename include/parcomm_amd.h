@@ -175,6 +175,15 @@ int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* devic
  * per phase over every bucket term of every matrix; QF_LAUNCH_PER_MATRIX issues
  * one launch and one host round trip per term, the plain per-matrix GPU form
  * the benchmark compares against. Proofs are identical in both. Process-wide. */
+/* The tile-pass autotuner's choices so far in this process (no reference
+ * counterpart): a JSON array with, per pass geometry (mode, packing, L, s, t,
+ * vectors), the launch shapes tried (kernel variant, workgroups per CU, XCD
+ * pairing), their times in ms and the one kept. The first pass of each
+ * geometry times the candidates on scratch buffers of the same layout; the
+ * environment variable QF_AUTOTUNE=0 keeps the built-in shapes. Writes at
+ * most cap bytes including the NUL; *len receives the full length. */
+int qf_autotune_report(char* out, size_t cap, size_t* len);
+
 #define QF_LAUNCH_BATCHED 0
 #define QF_LAUNCH_PER_MATRIX 1
 int qf_set_launch_mode(int mode);

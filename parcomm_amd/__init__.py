@@ -102,6 +102,7 @@ def lib(field=QF_FIELD_F65521):
         L.qf_proof_free.restype = None
         L.qf_statement_json.argtypes = [vp, vp, ctypes.c_char_p, sz, P(sz)]
         L.qf_set_launch_mode.argtypes = [i32]
+        L.qf_autotune_report.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.qf_device_memory.argtypes = [P(sz), P(sz)]
         L.qf_test_tile_pass.argtypes = [i32, vp, vp, vp, vp, u64, u64, u64, i32, i32, i32, i32]
         if L.qf_field() != field:
@@ -122,6 +123,18 @@ LAUNCH_PER_MATRIX = 1
 def set_launch_mode(mode, field=QF_FIELD_F65521):
     """Batched (one launch per phase over all matrices) or plain per-matrix launches."""
     _check(lib(field).qf_set_launch_mode(mode), field)
+
+
+def autotune_report(field=QF_FIELD_F65521):
+    """The tile-pass autotuner's choices in this process: one dict per pass
+    geometry with the launch shapes tried, their times (ms) and the best."""
+    import json
+    L = lib(field)
+    n = ctypes.c_size_t()
+    _check(L.qf_autotune_report(None, 0, ctypes.byref(n)), field)
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _check(L.qf_autotune_report(buf, n.value + 1, ctypes.byref(n)), field)
+    return json.loads(buf.value.decode())
 
 
 class DeviceBuffer:

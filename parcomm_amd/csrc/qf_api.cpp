@@ -1,6 +1,8 @@
 // C ABI of the product library (declared in include/parcomm_amd.h).
 #include "../../include/parcomm_amd.h"
 #include "qf_state.hpp"
+#include <algorithm>
+#include <cstring>
 #include "qf_vec.hpp"
 #include <memory>
 
@@ -402,6 +404,19 @@ int qf_test_tile_pass(int mode, const void* device_in, void* device_out1, void* 
     if (enc) (void)hipFree(enc);
     QF_HIP(e);
     require(ok, QF_ERR_ARGUMENT, "geometry does not fit a tile pass");
+  });
+}
+
+int qf_autotune_report(char* out, size_t cap, size_t* len) {
+  return guarded([&] {
+    require(len != nullptr, QF_ERR_ARGUMENT, "null argument");
+    const std::string r = autotune_report();
+    *len = r.size();
+    if (out && cap) {
+      const size_t m = std::min(cap - 1, r.size());
+      std::memcpy(out, r.data(), m);
+      out[m] = 0;
+    }
   });
 }
 

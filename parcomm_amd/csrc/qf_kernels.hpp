@@ -1,6 +1,7 @@
 // Launch interfaces of the HIP kernels (device code lives in *.hip files).
 #pragma once
 #include "qf_params.hpp"
+#include <string>
 
 namespace qf {
 
@@ -60,8 +61,12 @@ void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, c
 void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E8* alpha, uint32_t* tree,
                           hipStream_t st);
 
+// The tile-pass autotuner's choices so far, as a JSON array (qf_tile.hip).
+std::string autotune_report();
+
 // One register-radix tile pass (qf_tile.hip); false when the geometry does not
-// fit a 4096-element tile.
+// fit a 4096-element tile. Its launch shape is the autotuner's pick for the
+// geometry (timed on scratch buffers the first time the geometry runs).
 enum : int { TP_M = 0, TP_M_W = 1, TP_W = 2, TP_WAW = 3, TP_COPY_W = 4, TP_COPY_WAW = 5, TP_AW = 6 };
 // With xform_vectors the pass also transforms the index bits of the vectors a
 // tile holds (s = 0, t < 12: 2^(12-t) vectors per tile).

@@ -21,6 +21,13 @@
 // XCD's L2 when its sibling asks for it; a line read by one tile long after
 // its partner has been evicted and is fetched from HBM twice.
 #include "qf_kernels.hpp"
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
 
 namespace qf {
 namespace {
@@ -470,6 +477,185 @@ int device_cus() {
 
 }  // namespace
 
+namespace {
+// Launch shape of a tile pass: kernel variant (lean: at most 128 registers, no
+// prefetch; wide: prefetches the next tile), persistent workgroups per CU, and
+// the XCD-paired walk of 64-byte-row passes.
+struct TileShape {
+  bool wide;
+  int per_cu;
+  bool xcd;
+};
+
+bool xcd_applicable(const TileArgs& a, uint64_t wgs) {
+  return a.logC == 2 && a.log_lo_chunks >= 1 && wgs % 16 == 0 && a.tiles >= 2 * wgs;
+}
+
+void launch_shape(int mode, bool pack, TileArgs a, const TileShape& sh, hipStream_t st) {
+  const uint64_t tiles = a.tiles;
+  const uint64_t wgs = std::min<uint64_t>(tiles, uint64_t(sh.per_cu) * uint64_t(device_cus()));
+  a.per_wg = uint32_t((tiles + wgs - 1) / wgs);
+  uint32_t grid = uint32_t((tiles + a.per_wg - 1) / a.per_wg);
+  a.xcd_span = 0;
+  // 64-byte tile rows: pair the tiles that share 128-byte lines across sibling
+  // workgroups of one XCD (tiles 2m and 2m+1 differ in the lowest chunk bit)
+  if (sh.xcd && xcd_applicable(a, wgs)) {
+    a.xcd_span = uint32_t(((tiles + 7) / 8 + 1) & ~uint64_t(1));
+    grid = uint32_t(wgs);
+  }
+  dim3 g{grid, 1, 1}, b{uint32_t(TWG), 1, 1};
+  // compile-time stage ranges for the full-width tiles ([logC, 12)), runtime otherwise
+  const int lo_ct = a.xhi == 12 && (a.xlo == 0 || a.xlo == 2 || a.xlo == 7) ? a.xlo : -1;
+#define QF_TILE_LAUNCH_LO(M, PK, LOV)                                   \
+  do {                                                                  \
+    if (sh.wide)                                                        \
+      hipLaunchKernelGGL((k_tile_wide<M, PK, LOV>), g, b, 0, st, a);     \
+    else                                                                \
+      hipLaunchKernelGGL((k_tile<M, PK, LOV>), g, b, 0, st, a);          \
+  } while (0)
+#define QF_TILE_LAUNCH(M, PK)                        \
+  do {                                               \
+    switch (lo_ct) {                                 \
+      case 0: QF_TILE_LAUNCH_LO(M, PK, 0); break;    \
+      case 2: QF_TILE_LAUNCH_LO(M, PK, 2); break;    \
+      case 7: QF_TILE_LAUNCH_LO(M, PK, 7); break;    \
+      default: QF_TILE_LAUNCH_LO(M, PK, -1); break;  \
+    }                                                \
+  } while (0)
+  if (pack) {
+    QF_TILE_LAUNCH(TM_M, true);
+  } else {
+    switch (mode) {
+      case TM_M: QF_TILE_LAUNCH(TM_M, false); break;
+      case TM_M_W: QF_TILE_LAUNCH(TM_M_W, false); break;
+      case TM_W: QF_TILE_LAUNCH(TM_W, false); break;
+      case TM_WAW: QF_TILE_LAUNCH(TM_WAW, false); break;
+      case TM_COPY_W: QF_TILE_LAUNCH(TM_COPY_W, false); break;
+      case TM_COPY_WAW: QF_TILE_LAUNCH(TM_COPY_WAW, false); break;
+      default: QF_TILE_LAUNCH(TM_AW, false); break;
+    }
+  }
+#undef QF_TILE_LAUNCH
+#undef QF_TILE_LAUNCH_LO
+}
+
+TileShape default_shape(int mode) {
+  const bool w = wide_mode(mode);
+  return {w, w ? 1 : 2, true};
+}
+
+// ---------------------------------------------------------------- autotuner
+// The first pass of each geometry in the process times the candidate launch
+// shapes on scratch buffers laid out like the caller's (same strides, same
+// overlap of input and outputs, zeroed), with HIP events on the pass's
+// stream, and keeps the fastest; every candidate computes the same transform,
+// so the choice never changes a result. QF_AUTOTUNE=0 keeps the defaults.
+struct TuneKey {
+  int mode, L, s, t;
+  bool pack, xvec;
+  uint64_t vectors, in_vs, out_vs;
+  int out2_rel;  // out2 absent (0), out2 == out1 + 2^L (1), elsewhere (2)
+  bool in_place;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(mode, L, s, t, pack, xvec, vectors, in_vs, out_vs, out2_rel, in_place) <
+           std::tie(o.mode, o.L, o.s, o.t, o.pack, o.xvec, o.vectors, o.in_vs, o.out_vs, o.out2_rel, o.in_place);
+  }
+};
+struct TuneEntry {
+  TileShape best;
+  std::vector<std::pair<TileShape, float>> tried;  // candidate and its best time in ms
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry>& tune_table() {
+  static auto* t = new std::map<TuneKey, TuneEntry>;  // lives until process exit
+  return *t;
+}
+bool autotune_on() {
+  const char* e = std::getenv("QF_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+
+TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L, const TuneKey& key, hipStream_t st) {
+  TuneEntry out{default_shape(mode), {}};
+  const uint64_t n = uint64_t(1) << L;
+  const uint64_t out_ext = ((vectors - 1) * a0.out_vstride + n) * 16;
+  // one scratch span for the E8 buffers, at the callers' relative offsets
+  std::vector<std::pair<const char*, uint64_t>> bufs = {{reinterpret_cast<const char*>(a0.out1), out_ext}};
+  if (a0.out2) bufs.push_back({reinterpret_cast<const char*>(a0.out2), out_ext});
+  if (!pack) bufs.push_back({static_cast<const char*>(a0.in), ((vectors - 1) * a0.in_vstride + n) * 16});
+  const char* lo = bufs[0].first;
+  const char* hi = bufs[0].first + bufs[0].second;
+  for (auto& b : bufs) { lo = std::min(lo, b.first); hi = std::max(hi, b.first + b.second); }
+  const uint64_t span = uint64_t(hi - lo);
+  const uint64_t pack_bytes = pack ? vectors * n * 8 : 0;
+  if (span > (uint64_t(24) << 30)) return out;  // no scratch that large: keep the default
+  char* scratch = nullptr;
+  char* aux = nullptr;  // packed input bytes, or alpha
+  const uint64_t aux_bytes = std::max<uint64_t>(pack_bytes, n * 16);
+  if (hipMalloc(&scratch, span) != hipSuccess || hipMalloc(&aux, aux_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (scratch) (void)hipFree(scratch);
+    return out;
+  }
+  QF_HIP(hipMemsetAsync(scratch, 0, span, st));
+  QF_HIP(hipMemsetAsync(aux, 0, aux_bytes, st));
+  auto rebase = [&](const void* p) { return scratch + (static_cast<const char*>(p) - lo); };
+  TileArgs a = a0;
+  a.out1 = reinterpret_cast<E8*>(rebase(a0.out1));
+  a.out2 = a0.out2 ? reinterpret_cast<E8*>(rebase(a0.out2)) : nullptr;
+  a.alpha = reinterpret_cast<const E8*>(aux);
+  if (pack) {
+    a.in = aux;
+    a.block_first = 0;
+    a.available = pack_bytes;
+  } else {
+    a.in = rebase(a0.in);
+  }
+  std::vector<TileShape> cands = {{false, 2, true}, {false, 1, true}, {true, 1, true}, {true, 2, true}};
+  const uint64_t wgs2 = std::min<uint64_t>(a.tiles, 2 * uint64_t(device_cus()));
+  if (xcd_applicable(a, wgs2)) cands.push_back({false, 2, false});
+  hipEvent_t e0, e1;
+  QF_HIP(hipEventCreate(&e0));
+  QF_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  for (const TileShape& sh : cands) {
+    launch_shape(mode, pack, a, sh, st);  // warm-up
+    float t_best = 1e30f;
+    for (int rep = 0; rep < 2; rep++) {
+      QF_HIP(hipEventRecord(e0, st));
+      launch_shape(mode, pack, a, sh, st);
+      QF_HIP(hipEventRecord(e1, st));
+      QF_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      QF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      t_best = std::min(t_best, ms);
+    }
+    QF_HIP(hipGetLastError());
+    out.tried.push_back({sh, t_best});
+    if (t_best < best) { best = t_best; out.best = sh; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  QF_HIP(hipStreamSynchronize(st));
+  (void)hipFree(scratch);
+  (void)hipFree(aux);
+  (void)key;
+  return out;
+}
+
+TileShape shape_for(int mode, bool pack, const TileArgs& a, uint64_t vectors, int L, bool xvec, hipStream_t st) {
+  if (!autotune_on()) return default_shape(mode);
+  const uint64_t n = uint64_t(1) << L;
+  TuneKey key{mode, L, a.s, a.t, pack, xvec, vectors, a.in_vstride, a.out_vstride,
+              !a.out2 ? 0 : (a.out2 == a.out1 + n ? 1 : 2), !pack && a.in == static_cast<const void*>(a.out1)};
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto& tab = tune_table();
+  auto it = tab.find(key);
+  if (it == tab.end()) it = tab.emplace(key, tune(mode, pack, a, vectors, L, key, st)).first;
+  return it->second.best;
+}
+}  // namespace
+
 // One tile pass over `vectors` vectors of length 2^L, transforming bits [s, s+t)
 // (and, with xform_vectors, also the index bits of the 2^logV vectors a tile
 // holds). A tile holds 4096 elements: t + logC + logV = 12 with C = 2^logC
@@ -477,6 +663,7 @@ int device_cus() {
 bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8* alpha, uint64_t vectors,
                uint64_t in_vstride, uint64_t out_vstride, int L, int s, int t, uint64_t block_first,
                uint64_t available, hipStream_t st, bool xform_vectors) {
+  if (mode < TM_M || mode > TM_AW || (pack && mode != TM_M)) return false;
   TileArgs a{};
   a.in = in;
   a.out1 = out1;
@@ -514,56 +701,34 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
   const uint64_t tiles = (vectors << L) / TN;
   if (tiles == 0 || tiles >= (1ull << 31)) return false;
   a.tiles = uint32_t(tiles);
-  // persistent workgroups: two per CU for the lean kernels, one for the wide ones
-  const uint64_t per_cu = wide_mode(mode) ? 1 : 2;
-  const uint64_t wgs = std::min<uint64_t>(tiles, per_cu * uint64_t(device_cus()));
-  a.per_wg = uint32_t((tiles + wgs - 1) / wgs);
-  uint32_t grid = uint32_t((tiles + a.per_wg - 1) / a.per_wg);
-  // 64-byte tile rows: pair the tiles that share 128-byte lines across sibling
-  // workgroups of one XCD (tiles 2m and 2m+1 differ in the lowest chunk bit)
-  if (a.logC == 2 && a.log_lo_chunks >= 1 && wgs % 16 == 0 && tiles >= 2 * wgs) {
-    const uint64_t span = ((tiles + 7) / 8 + 1) & ~uint64_t(1);
-    a.xcd_span = uint32_t(span);
-    grid = uint32_t(wgs);
-  }
-  dim3 g{grid, 1, 1}, b{uint32_t(TWG), 1, 1};
-  // compile-time stage ranges for the full-width tiles ([logC, 12)), runtime otherwise
-  const int lo_ct = a.xhi == 12 && (a.xlo == 0 || a.xlo == 2 || a.xlo == 7) ? a.xlo : -1;
-#define QF_TILE_LAUNCH_LO(M, PK, LOV)                                                 \
-  do {                                                                                \
-    if constexpr (wide_mode(M))                                                       \
-      hipLaunchKernelGGL((k_tile_wide<M, PK, LOV>), g, b, 0, st, a);                   \
-    else                                                                              \
-      hipLaunchKernelGGL((k_tile<M, PK, LOV>), g, b, 0, st, a);                        \
-  } while (0)
-#define QF_TILE_LAUNCH(M, PK)                        \
-  do {                                               \
-    switch (lo_ct) {                                 \
-      case 0: QF_TILE_LAUNCH_LO(M, PK, 0); break;    \
-      case 2: QF_TILE_LAUNCH_LO(M, PK, 2); break;    \
-      case 7: QF_TILE_LAUNCH_LO(M, PK, 7); break;    \
-      default: QF_TILE_LAUNCH_LO(M, PK, -1); break;  \
-    }                                                \
-  } while (0)
-  if (pack) {
-    if (mode != TM_M) return false;
-    QF_TILE_LAUNCH(TM_M, true);
-  } else {
-    switch (mode) {
-      case TM_M: QF_TILE_LAUNCH(TM_M, false); break;
-      case TM_M_W: QF_TILE_LAUNCH(TM_M_W, false); break;
-      case TM_W: QF_TILE_LAUNCH(TM_W, false); break;
-      case TM_WAW: QF_TILE_LAUNCH(TM_WAW, false); break;
-      case TM_COPY_W: QF_TILE_LAUNCH(TM_COPY_W, false); break;
-      case TM_COPY_WAW: QF_TILE_LAUNCH(TM_COPY_WAW, false); break;
-      case TM_AW: QF_TILE_LAUNCH(TM_AW, false); break;
-      default: return false;
-    }
-  }
-#undef QF_TILE_LAUNCH
-#undef QF_TILE_LAUNCH_LO
+  launch_shape(mode, pack, a, shape_for(mode, pack, a, vectors, L, xform_vectors, st), st);
   QF_HIP(hipGetLastError());
   return true;
+}
+
+// JSON array of the autotuner's choices so far (this library build).
+std::string autotune_report() {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  std::string o = "[";
+  auto shape = [](const TileShape& s) {
+    return std::string("{\"wide\":") + (s.wide ? "true" : "false") + ",\"per_cu\":" + std::to_string(s.per_cu) +
+           ",\"xcd\":" + (s.xcd ? "true" : "false") + "}";
+  };
+  bool first = true;
+  for (auto& [k, e] : tune_table()) {
+    if (!first) o += ",";
+    first = false;
+    o += "{\"mode\":" + std::to_string(k.mode) + ",\"pack\":" + (k.pack ? "true" : "false") +
+         ",\"L\":" + std::to_string(k.L) + ",\"s\":" + std::to_string(k.s) + ",\"t\":" + std::to_string(k.t) +
+         ",\"vectors\":" + std::to_string(k.vectors) + ",\"best\":" + shape(e.best) + ",\"tried\":[";
+    for (size_t i = 0; i < e.tried.size(); i++) {
+      char ms[32];
+      std::snprintf(ms, sizeof ms, "%.4f", e.tried[i].second);
+      o += (i ? "," : "") + std::string("{\"shape\":") + shape(e.tried[i].first) + ",\"ms\":" + ms + "}";
+    }
+    o += "]}";
+  }
+  return o + "]";
 }
 
 }  // namespace qf
