@@ -1,6 +1,7 @@
 // Device vector operations and BVA-scale kernels of the opening (see qf_vec.hpp).
 #include "qf_vec.hpp"
 #include <algorithm>
+#include <cstring>
 #include <atomic>
 #include <type_traits>
 #include <map>
@@ -1058,11 +1059,52 @@ dim3 grid_for(uint64_t n) {
   return dim3(uint32_t(g));
 }
 
-// Small host tables go to scratch in stream order; the synchronous copy keeps
-// the host buffer valid only for the duration of the call.
+// Host-to-device copies of the opening's small tables and index lists go
+// through this thread's pinned staging ring: the host data is copied into the
+// ring at once and the device copy runs asynchronously in stream order, so
+// the caller's buffer may go away and no synchronisation is needed. Only a
+// wrap of the ring waits for the stream (everything staged before has then
+// been consumed); copies larger than the ring are synchronous.
+struct StagingRing {
+  char* base = nullptr;
+  size_t cap = 0, head = 0;
+  hipStream_t st = nullptr;  // the stream the staged copies were issued on
+  ~StagingRing() { if (base) (void)hipHostFree(base); }
+};
+thread_local StagingRing g_ring;
+void h2d(DevCtx& c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  StagingRing& r = g_ring;
+  constexpr size_t CAP = size_t(16) << 20;
+  if (bytes > CAP) {
+    QF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.st));
+    c.sync();
+    return;
+  }
+  if (!r.base) {
+    QF_HIP(hipHostMalloc(&r.base, CAP, 0));
+    r.cap = CAP;
+  }
+  if (r.st && r.st != c.st) {  // staged on another stream: let those copies finish first
+    QF_HIP(hipStreamSynchronize(r.st));
+    r.head = 0;
+  }
+  r.st = c.st;
+  const size_t off = (r.head + 255) & ~size_t(255);
+  if (off + bytes > r.cap) {
+    QF_HIP(hipStreamSynchronize(c.st));
+    r.head = 0;
+    return h2d(c, dst, src, bytes);
+  }
+  std::memcpy(r.base + off, src, bytes);
+  QF_HIP(hipMemcpyAsync(dst, r.base + off, bytes, hipMemcpyHostToDevice, c.st));
+  r.head = off + bytes;
+}
+
+// Small host tables go to the scratch table in stream order (the next call's
+// copy runs after this call's kernels).
 E8* put_table(DevCtx& c, E8* scratch, const void* host, size_t bytes) {
-  QF_HIP(hipMemcpyAsync(scratch, host, bytes, hipMemcpyHostToDevice, c.st));
-  c.sync();
+  h2d(c, scratch, host, bytes);
   return scratch;
 }
 
@@ -1217,8 +1259,7 @@ void DevCtx::sync() { QF_HIP(hipStreamSynchronize(st)); }
 namespace dv {
 DVec upload(DevCtx& c, const std::vector<E8>& v) {
   DVec d(v.size());
-  if (!v.empty()) QF_HIP(hipMemcpyAsync(d.p, v.data(), v.size() * 16, hipMemcpyHostToDevice, c.st));
-  c.sync();
+  h2d(c, d.p, v.data(), v.size() * 16);
   return d;
 }
 std::vector<E8> download(DevCtx& c, const E8* p, uint64_t n) {
@@ -1238,7 +1279,6 @@ void tensor(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>
   put_table(c, t + hi.size(), lo.data(), lo.size() * 16);
   uint64_t n = hi.size() * lo.size();
   hipLaunchKernelGGL(k_tensor, grid_for(n), dim3(WG), 0, c.st, out, n, t, t + hi.size(), ilog2u(lo.size()));
-  c.sync();  // the scratch tables are reused by the next call
 }
 E8 inner(DevCtx& c, const E8* a, const E8* b, uint64_t n) { return reduce<R_INNER>(c, a, b, n).first; }
 std::pair<E8, E8> cross_halves(DevCtx& c, const E8* a, const E8* b, uint64_t h) { return reduce<R_CROSS>(c, a, b, h); }
@@ -1308,13 +1348,12 @@ std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, 
 void col_fold(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& f, E8* out) {
   E8* t = put_table(c, scratch_tab(), f.data(), f.size() * 16);
   hipLaunchKernelGGL(k_col_fold, grid_for(n), dim3(WG), 0, c.st, cw, R, n, cols, t, out);
-  c.sync();
 }
 std::vector<E8> gather_rows(DevCtx& c, const E8* cw, uint64_t R, uint64_t cols, const std::vector<uint64_t>& rows) {
   if (rows.empty()) return {};
   uint64_t* d_rows = static_cast<uint64_t*>(dmalloc(rows.size() * 8, c.st));
   DVec tmp(rows.size() * cols);
-  QF_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, c.st));
+  h2d(c, d_rows, rows.data(), rows.size() * 8);
   uint64_t total = rows.size() * cols;
   hipLaunchKernelGGL(k_gather_rows, dim3(uint32_t((total + WG - 1) / WG)), dim3(WG), 0, c.st, cw, R, cols, d_rows,
                      uint64_t(rows.size()), tmp.p);
@@ -1328,7 +1367,7 @@ std::vector<E8> gather_weighted(DevCtx& c, const E8* a, uint64_t Ra, uint64_t ca
   if (rows.empty()) return {};
   for (uint64_t r : rows) require(r < Ra && (cb == 0 || r < Rb), QF_ERR_ARGUMENT, "gather row out of range");
   uint64_t* d_rows = static_cast<uint64_t*>(dmalloc(rows.size() * 8, c.st));
-  QF_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, c.st));
+  h2d(c, d_rows, rows.data(), rows.size() * 8);
   DVec d_w = upload(c, std::vector<E8>(w.begin(), w.begin() + (ca + cb)));
   DVec out(rows.size());
   hipLaunchKernelGGL(k_gather_weighted, dim3(uint32_t((rows.size() + WG - 1) / WG)), dim3(WG), 0, c.st, a, Ra, ca, b,
@@ -1346,7 +1385,7 @@ std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, u
   uint64_t total = rows.size() * depth;
   uint64_t* d_rows = static_cast<uint64_t*>(dmalloc(rows.size() * 8, c.st));
   uint32_t* d_out = static_cast<uint32_t*>(dmalloc(total * 32, c.st));
-  QF_HIP(hipMemcpyAsync(d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice, c.st));
+  h2d(c, d_rows, rows.data(), rows.size() * 8);
   hipLaunchKernelGGL(k_gather_paths, dim3(uint32_t((total + WG - 1) / WG)), dim3(WG), 0, c.st, tree, R, depth, d_rows,
                      uint64_t(rows.size()), d_out);
   std::vector<uint8_t> host(total * 32);
@@ -1364,10 +1403,9 @@ void scatter_add(DevCtx& c, E8* dst, const std::vector<uint64_t>& idx, const std
   if (idx.empty()) return;
   uint64_t* d_idx = static_cast<uint64_t*>(dmalloc(idx.size() * 8, c.st));
   DVec v = upload(c, vals);
-  QF_HIP(hipMemcpyAsync(d_idx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, c.st));
+  h2d(c, d_idx, idx.data(), idx.size() * 8);
   hipLaunchKernelGGL(k_scatter_add, dim3(uint32_t((idx.size() + WG - 1) / WG)), dim3(WG), 0, c.st, dst, d_idx, v.p,
                      uint64_t(idx.size()));
-  c.sync();
   dfree(d_idx, c.st);
 }
 }  // namespace dv
@@ -1375,7 +1413,7 @@ void scatter_add(DevCtx& c, E8* dst, const std::vector<uint64_t>& idx, const std
 // ---------------------------------------------------------------- BVA-scale wrappers
 static void upload_pairs(DevCtx& c, const std::vector<std::array<E8, 2>>& pairs, E8 (**out)[2]) {
   *out = reinterpret_cast<E8(*)[2]>(dmalloc(std::max<size_t>(pairs.size(), 1) * 32, c.st));
-  if (!pairs.empty()) QF_HIP(hipMemcpyAsync(*out, pairs.data(), pairs.size() * 32, hipMemcpyHostToDevice, c.st));
+  h2d(c, *out, pairs.data(), pairs.size() * 32);
 }
 
 namespace {
@@ -1405,7 +1443,7 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
   E8(*d_pairs)[2] = nullptr;
   upload_pairs(c, pairs, &d_pairs);
   auto* d_tasks = static_cast<ProjTaskDev*>(dmalloc(chunks.size() * sizeof(ProjTaskDev), c.st));
-  QF_HIP(hipMemcpyAsync(d_tasks, chunks.data(), chunks.size() * sizeof(ProjTaskDev), hipMemcpyHostToDevice, c.st));
+  h2d(c, d_tasks, chunks.data(), chunks.size() * sizeof(ProjTaskDev));
   DVec out(chunks.size() * L);
   for (size_t first = 0; first < chunks.size(); first += 65535) {
     uint32_t n = uint32_t(std::min<size_t>(65535, chunks.size() - first));
@@ -1422,8 +1460,6 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
     }
   }
   QF_HIP(hipGetLastError());
-  // the host vectors of pairs and chunks are copied before the call returns
-  c.sync();
   dfree(d_pairs, c.st);
   dfree(d_tasks, c.st);
   return out;
@@ -1441,7 +1477,7 @@ std::pair<std::vector<E8>, std::vector<E8>> tensor_project_cross(DevCtx& c, cons
   if (owner.empty()) return {pos, neg};
   DVec d_lows = dv::upload(c, lows);
   auto* d_owner = static_cast<uint32_t*>(dmalloc(owner.size() * 4, c.st));
-  QF_HIP(hipMemcpyAsync(d_owner, owner.data(), owner.size() * 4, hipMemcpyHostToDevice, c.st));
+  h2d(c, d_owner, owner.data(), owner.size() * 4);
   DVec out(2 * (L - 1));
   hipLaunchKernelGGL(k_proj_cross, dim3(2 * (L - 1)), dim3(WG), 0, c.st, parts.p, d_owner, uint32_t(owner.size()), L,
                      d_lows.p, out.p);
@@ -1493,7 +1529,6 @@ void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, 
     default: hipLaunchKernelGGL(k_eo_witness<64>, grid_for(n_rows), dim3(WG), 0, c.st, bva, available, n_rows, t, w); break;
   }
   QF_HIP(hipGetLastError());
-  c.sync();
 }
 
 template <bool ADD>
@@ -1522,13 +1557,12 @@ static void fill_tasks(DevCtx& c, E8* out, const std::vector<FillTask>& tasks,
   FillTaskDev* d_tasks = nullptr;
   upload_pairs(c, pairs, &d_pairs);
   d_tasks = static_cast<FillTaskDev*>(dmalloc(chunks.size() * sizeof(FillTaskDev), c.st));
-  QF_HIP(hipMemcpyAsync(d_tasks, chunks.data(), chunks.size() * sizeof(FillTaskDev), hipMemcpyHostToDevice, c.st));
+  h2d(c, d_tasks, chunks.data(), chunks.size() * sizeof(FillTaskDev));
   for (size_t first = 0; first < chunks.size(); first += 65535) {
     uint32_t n = uint32_t(std::min<size_t>(65535, chunks.size() - first));
     hipLaunchKernelGGL(k_tensor_fill<ADD>, dim3(n), dim3(FWG), 0, c.st, out, d_tasks + first, (const E8(*)[2])d_pairs);
   }
   QF_HIP(hipGetLastError());
-  c.sync();  // the host vectors of pairs and chunks are copied before the call returns
   dfree(d_pairs, c.st);
   dfree(d_tasks, c.st);
 }
@@ -1559,13 +1593,12 @@ void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& ta
   gap(at, n);
   if (!gaps.empty()) {
     auto* d_gaps = static_cast<uint64_t*>(dmalloc(gaps.size() * 8, c.st));
-    QF_HIP(hipMemcpyAsync(d_gaps, gaps.data(), gaps.size() * 8, hipMemcpyHostToDevice, c.st));
+    h2d(c, d_gaps, gaps.data(), gaps.size() * 8);
     for (size_t first = 0; first < gaps.size() / 2; first += 65535) {
       uint32_t cnt = uint32_t(std::min<size_t>(65535, gaps.size() / 2 - first));
       hipLaunchKernelGGL(k_zero_ranges, dim3(cnt), dim3(WG), 0, c.st, out, d_gaps + 2 * first);
     }
     QF_HIP(hipGetLastError());
-    c.sync();
     dfree(d_gaps, c.st);
   }
   fill_tasks<false>(c, out, tasks, pairs);
@@ -1597,7 +1630,6 @@ void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_le
   hipLaunchKernelGGL(k_block_fold, grid_for(n), dim3(WG), 0, c.st, reinterpret_cast<const uint8_t*>(bva),
                      available * sizeof(Val), block_len * sizeof(Val), uint32_t(blocks), t, values, n);
   QF_HIP(hipGetLastError());
-  c.sync();
 }
 
 void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo,
@@ -1610,7 +1642,6 @@ void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vec
   uint64_t n = hi.size() * lo.size();
   hipLaunchKernelGGL(k_weight_images, grid_for(n), dim3(WG), 0, c.st, out, n, t, t + hi.size(), ilog2u(lo.size()), im);
   QF_HIP(hipGetLastError());
-  c.sync();
 }
 
 }  // namespace qf
