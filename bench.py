@@ -245,11 +245,12 @@ def autotune_summary(pa, field):
         if e["vectors"] << e["L"] < (1 << 24):
             continue  # test-sized geometries
         b = e["best"]
-        times = [t["ms"] for t in e["tried"]]
+        times = [t["ms"] for t in e["tried"]]  # empty when the pick came from QF_TUNE_FILE
         out.append({"pass": f"mode {e['mode']} bits {e['s']}+{e['t']} of L={e['L']} x{e['vectors']}"
                             + (" packed" if e["pack"] else ""),
                     "kept": ("wide" if b["wide"] else "lean") + f" {b['per_cu']}/CU" + ("" if b["xcd"] else " no-xcd"),
-                    "ms": round(min(times), 4), "slowest_ms": round(max(times), 4)})
+                    "ms": round(min(times), 4) if times else None,
+                    "slowest_ms": round(max(times), 4) if times else None})
     return out
 
 
@@ -336,12 +337,12 @@ def main():
         config["koalabear"] = kb
     if not kb_head and not args.no_70b:
         # the 70B-shaped F65521 configuration: 562 matrices, packed_log 6, roots-only commitment
-        r70 = measure(pa, pa.F65521, args, per_matrix=False, serve_requests=0, model="70b", steps=1, warmup=0)
+        r70 = measure(pa, pa.F65521, args, per_matrix=False, serve_requests=0, model="70b", steps=1, warmup=1)
         l70 = summary(r70)
         l70["phase_hbm"] = phase_hbm(pa, r70)
         l70.update({"workload": "llama3.1-70b-shaped int8 BVA, 562 matrices, chunk_log 30, packed_log 6",
                     "active_entries": r70["params"].active, "blocks": r70["params"].blocks,
-                    "cache_mode": "recompute", "steps": 1, "warmup": 0})
+                    "cache_mode": "recompute", "steps": 1, "warmup": 1})
         del r70
         config["llama70b"] = l70
     out = {

@@ -159,7 +159,8 @@ int qf_device_free(void* device_ptr);
 int qf_device_copy(void* device_dst, const void* device_src, size_t bytes, int pattern);
 
 /* Free and total bytes of the current device (admission control of the
- * serving loop). */
+ * serving loop); free includes the blocks the library's caching allocator
+ * holds idle, which its next allocations reuse. */
 int qf_device_memory(size_t* free_bytes, size_t* total_bytes);
 
 /* Synthetic workload generator (no reference counterpart; the reference CLI
@@ -180,8 +181,10 @@ int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* devic
  * vectors), the launch shapes tried (kernel variant, workgroups per CU, XCD
  * pairing), their times in ms and the one kept. The first pass of each
  * geometry times the candidates on scratch buffers of the same layout; the
- * environment variable QF_AUTOTUNE=0 keeps the built-in shapes. Writes at
- * most cap bytes including the NUL; *len receives the full length. */
+ * environment variable QF_AUTOTUNE=0 keeps the built-in shapes, and
+ * QF_TUNE_FILE=path reads earlier picks from that file and records new ones
+ * there. Writes at most cap bytes including the NUL; *len receives the full
+ * length. */
 int qf_autotune_report(char* out, size_t cap, size_t* len);
 
 #define QF_LAUNCH_BATCHED 0

@@ -251,7 +251,13 @@ int qf_commit(const qf_params* p, const void* values, size_t len, int cache_mode
     require(p && out && (values || len == 0), QF_ERR_ARGUMENT, "null argument");
     device_setup();
     Val* d = nullptr;
-    QF_HIP(hipMalloc(&d, std::max<size_t>(len * sizeof(Val), 16)));
+    hipError_t e = hipMalloc(&d, std::max<size_t>(len * sizeof(Val), 16));
+    if (e == hipErrorOutOfMemory) {  // give the pool's idle blocks back and retry once
+      (void)hipGetLastError();
+      release_cached_blocks();
+      e = hipMalloc(&d, std::max<size_t>(len * sizeof(Val), 16));
+    }
+    QF_HIP(e);
     try {
       QF_HIP(hipMemcpy(d, values, len * sizeof(Val), hipMemcpyHostToDevice));
       commit_impl(p, d, len, cache_mode, out);
@@ -432,6 +438,7 @@ int qf_device_memory(size_t* free_bytes, size_t* total_bytes) {
     require(free_bytes && total_bytes, QF_ERR_ARGUMENT, "null argument");
     device_setup();
     QF_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    *free_bytes += cached_idle_bytes();  // the pool hands its idle blocks out again
   });
 }
 

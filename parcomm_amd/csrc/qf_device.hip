@@ -83,7 +83,13 @@ int qf_device_alloc(size_t bytes, void** out) {
   return guard([&] {
     device_setup();
     require(out != nullptr, QF_ERR_ARGUMENT, "null argument");
-    QF_HIP(hipMalloc(out, bytes ? bytes : 1));
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e == hipErrorOutOfMemory) {  // the pool's idle blocks may hold the memory: give them back and retry
+      (void)hipGetLastError();
+      release_cached_blocks();
+      e = hipMalloc(out, bytes ? bytes : 1);
+    }
+    QF_HIP(e);
     QF_HIP(hipMemset(*out, 0, bytes ? bytes : 1));
   });
 }

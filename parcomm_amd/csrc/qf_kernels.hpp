@@ -26,6 +26,11 @@ hipStream_t thread_stream();
 // the device spin instead of sleeping (the opening's round trips are short).
 // Has no effect when the process already holds a device context.
 void device_setup();
+// Returns every idle block of the caching allocator to the device (before a
+// large allocation outside the pool, e.g. a resident BVA).
+void release_cached_blocks();
+// Bytes held idle by the caching allocator (free for the library's next allocations).
+size_t cached_idle_bytes();
 
 struct CommitGeom {
   int log_n;          // log2 of message rows per column (chunk_log - 8)

@@ -1,6 +1,7 @@
 // Device vector operations and BVA-scale kernels of the opening (see qf_vec.hpp).
 #include "qf_vec.hpp"
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <type_traits>
@@ -1158,8 +1159,22 @@ hipStream_t current_stream() { return g_current_stream; }
 void device_setup() {
   static std::once_flag once;
   std::call_once(once, [] {
-    if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
+    const char* w = std::getenv("QF_HOST_WAIT");  // "spin" (default), "yield" or "auto"
+    const unsigned flag = !w || std::strcmp(w, "spin") == 0 ? hipDeviceScheduleSpin
+                          : std::strcmp(w, "yield") == 0   ? hipDeviceScheduleYield
+                                                           : hipDeviceScheduleAuto;
+    if (hipSetDeviceFlags(flag) != hipSuccess) (void)hipGetLastError();
   });
+}
+size_t cached_idle_bytes() {
+  BlockCache& bc = cache();
+  std::lock_guard<std::mutex> g(bc.mu);
+  return bc.idle_bytes;
+}
+void release_cached_blocks() {
+  BlockCache& bc = cache();
+  std::lock_guard<std::mutex> g(bc.mu);
+  bc.trim_locked(0);
 }
 hipStream_t thread_stream() {
   device_setup();
