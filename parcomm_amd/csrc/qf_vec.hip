@@ -19,7 +19,7 @@ constexpr uint32_t PROJ_TAB_LOG = 12;   // low-bit weight table of a projection 
 __device__ inline V8 v8_madd(const V8& acc, const V8& x) { return v8_add(acc, x); }
 
 // ---------------------------------------------------------------- reductions
-enum { R_INNER = 0, R_CROSS = 1, R_EO = 2, R_SC = 3 };
+enum { R_INNER = 0, R_CROSS = 1, R_SC = 3 };
 
 template <int MODE>
 __global__ __launch_bounds__(WG) void k_reduce(const E8* __restrict__ a, const E8* __restrict__ b, uint64_t n,
@@ -32,9 +32,6 @@ __global__ __launch_bounds__(WG) void k_reduce(const E8* __restrict__ a, const E
     } else if (MODE == R_CROSS) {
       s0 = v8_add(s0, v8_mul(v8_load(a[i]), v8_load(b[n + i])));
       s1 = v8_add(s1, v8_mul(v8_load(a[n + i]), v8_load(b[i])));
-    } else if (MODE == R_EO) {
-      s0 = v8_add(s0, v8_mul(v8_load(a[2 * i]), v8_load(b[2 * i + 1])));
-      s1 = v8_add(s1, v8_mul(v8_load(a[2 * i + 1]), v8_load(b[2 * i])));
     } else {
       V8 vl = v8_load(a[i]), wl = v8_load(b[i]);
       s0 = v8_add(s0, v8_mul(vl, wl));
@@ -68,20 +65,6 @@ __device__ inline void block_sums(const V8 (&s)[NS], uint32_t* __restrict__ part
     __syncthreads();
   }
   if (threadIdx.x < NS * LANES) partials[blockIdx.x * PSTRIDE + threadIdx.x] = red[0][threadIdx.x];
-}
-
-// The packed reduction's consistency sum <w, q> together with the first
-// binary round's cross terms (eo.rs:135-150): one read of both vectors.
-__global__ __launch_bounds__(WG) void k_eo_first(const E8* __restrict__ w, const E8* __restrict__ q, uint64_t h,
-                                                 uint32_t* __restrict__ partials) {
-  V8 s[3] = {v8_zero(), v8_zero(), v8_zero()};
-  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
-    const V8 w0 = v8_load(w[2 * i]), w1 = v8_load(w[2 * i + 1]), q0 = v8_load(q[2 * i]), q1 = v8_load(q[2 * i + 1]);
-    s[0] = v8_add(s[0], v8_add(v8_mul(w0, q0), v8_mul(w1, q1)));
-    s[1] = v8_add(s[1], v8_mul(w0, q1));
-    s[2] = v8_add(s[2], v8_mul(w1, q0));
-  }
-  block_sums<3>(s, partials);
 }
 
 // One LiteBullet fold in place (a[i] += a[h+i] x, b[i] += b[h+i] y, i < h;
@@ -1308,15 +1291,8 @@ std::pair<E8, E8> lb_fold_cross(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x
   auto s = finish_sums<2>(c, g);
   return {s[0], s[1]};
 }
-std::pair<E8, E8> eo_cross(DevCtx& c, const E8* w, const E8* q, uint64_t h) { return reduce<R_EO>(c, w, q, h); }
 void eo_fold(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri, const E8& r) {
   hipLaunchKernelGGL(k_eo_fold, grid_for(h), dim3(WG), 0, c.st, w, q, ow, oq, h, ri, r);
-}
-std::array<E8, 3> eo_first(DevCtx& c, const E8* w, const E8* q, uint64_t h) {
-  const uint32_t g = groups_for(h);
-  hipLaunchKernelGGL(k_eo_first, dim3(g), dim3(WG), 0, c.st, w, q, h, c.partials);
-  QF_HIP(hipGetLastError());
-  return finish_sums<3>(c, g);
 }
 std::pair<E8, E8> eo_fold_cross(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri,
                                 const E8& r) {

@@ -54,12 +54,8 @@ std::pair<E8, E8> cross_halves(DevCtx& c, const E8* a, const E8* b, uint64_t h);
 void lb_fold(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y);
 // lb_fold, returning the next round's cross_halves over the folded length h (h even)
 std::pair<E8, E8> lb_fold_cross(DevCtx& c, E8* a, E8* b, uint64_t h, const E8& x, const E8& y);
-// (sum w[2i] q[2i+1], sum w[2i+1] q[2i]), i < h
-std::pair<E8, E8> eo_cross(DevCtx& c, const E8* w, const E8* q, uint64_t h);
 // ow[i] = w[2i] + w[2i+1] ri; oq[i] = q[2i] + q[2i+1] r
 void eo_fold(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri, const E8& r);
-// <w, q> over 2h elements and the first round's cross terms, in one pass: {inner, cross0, cross1}
-std::array<E8, 3> eo_first(DevCtx& c, const E8* w, const E8* q, uint64_t h);
 // eo_fold into (ow, oq) of length h, returning the next round's cross terms over the outputs (h even)
 std::pair<E8, E8> eo_fold_cross(DevCtx& c, const E8* w, const E8* q, E8* ow, E8* oq, uint64_t h, const E8& ri,
                                 const E8& r);
@@ -126,7 +122,8 @@ void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, con
 // being read and the rows outside every task are zeroed.
 void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& tasks,
                  const std::vector<std::array<E8, 2>>& pairs);
-// eo_witness for n_rows (even) rows fused with dv::eo_first against the built
+// eo_witness for n_rows (even) rows fused with the packed reduction's
+// consistency sum and the first binary round's cross terms against the built
 // query q: {<w, q>, sum w[2i] q[2i+1], sum w[2i+1] q[2i]}.
 std::array<E8, 3> eo_witness_first(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s,
                                    const std::vector<E8>& pinv, const E8* q, E8* w);
