@@ -324,8 +324,14 @@ __device__ inline void build_table(E8* T, const E8 (*pairs)[2], int k) {
 // over the short runs summed here; i16 x u31 products (< 2^46) need 64.
 #ifdef QF_KB
 using Acc = int64_t;
+// acc + v c, exact: a small entry and a canonical coefficient (< 2^31) are
+// both signed 32-bit values, so the product is a sign-extended 32 x 32 one and
+// compiles to a single v_mad_i64_i32 (the zero-extended coefficient would take
+// two unsigned 64-bit multiply-adds).
+__device__ inline void acc_mac(Acc& acc, int32_t v, uint32_t c) { acc += int64_t(v) * int64_t(int32_t(c)); }
 #else
 using Acc = int32_t;
+__device__ inline void acc_mac(Acc& acc, int32_t v, uint32_t c) { acc += v * int32_t(c); }
 #endif
 
 // Q (2 or 4) consecutive BVA entries as one aligned word (idx a multiple of Q).
@@ -476,7 +482,7 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
 #pragma unroll
           for (int l = 0; l < Q; l++)
 #pragma unroll
-            for (int k = 0; k < LANES; k++) ai[l][k] += Acc(v[l]) * Acc(wt.c[k]);
+            for (int k = 0; k < LANES; k++) acc_mac(ai[l][k], v[l], wt.c[k]);
         }
       }
     } else {
@@ -487,7 +493,7 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
 #pragma unroll
         for (int l = 0; l < Q; l++)
 #pragma unroll
-          for (int k = 0; k < LANES; k++) ai[l][k] += Acc(v[l]) * Acc(wt.c[k]);
+          for (int k = 0; k < LANES; k++) acc_mac(ai[l][k], v[l], wt.c[k]);
       }
     }
     V8 c = v8_load(Chi[h]);
@@ -551,7 +557,7 @@ __device__ inline void witness_rows(const Val* __restrict__ bva, uint64_t row0, 
         for (int j = 0; j < Q; j++) {
           const E8 p = pinv[Q * g + j];
 #pragma unroll
-          for (int k = 0; k < LANES; k++) acc[r][k] += Acc(v[j]) * Acc(p.c[k]);
+          for (int k = 0; k < LANES; k++) acc_mac(acc[r][k], v[j], p.c[k]);
         }
       }
   } else {
@@ -563,7 +569,7 @@ __device__ inline void witness_rows(const Val* __restrict__ bva, uint64_t row0, 
         for (int j = 0; j < Q; j++) {
           const E8 p = pinv[Q * g + j];
 #pragma unroll
-          for (int k = 0; k < LANES; k++) acc[r][k] += Acc(v[j]) * Acc(p.c[k]);
+          for (int k = 0; k < LANES; k++) acc_mac(acc[r][k], v[j], p.c[k]);
         }
       }
   }
@@ -969,7 +975,7 @@ __global__ __launch_bounds__(WG) void k_block_fold(const uint8_t* __restrict__ b
         const Acc v = int8_t(x[j >> 2] >> (8 * (j & 3)));
 #endif
 #pragma unroll
-        for (int k = 0; k < LANES; k++) S[j][k] += v * Acc(f.c[k]);
+        for (int k = 0; k < LANES; k++) acc_mac(S[j][k], int32_t(v), f.c[k]);
       }
     };
     // Blocks [0, full) hold this element's 8 bytes entirely below `available`:
