@@ -36,9 +36,9 @@ constexpr int TWG = 512;               // threads per workgroup
 constexpr int TQB = 3;                 // register bits per phase
 constexpr int TQ = 1 << TQB;           // elements per thread
 constexpr int TN = TWG * TQ;           // elements per tile
-constexpr int TPAD = TN + TN / 8;      // LDS row with one pad slot per 8 elements
+constexpr int TPAD = TN + TN / 32;     // LDS row with one pad slot per 32 elements
 // An exchange moves the LANES 32-bit lanes of every tile element through NR
-// LDS rows of uint2 (72 KiB for NR = 2, so two workgroups can share a CU);
+// LDS rows of uint2 (66 KiB for NR = 2, so two workgroups can share a CU);
 // F65521's eight lanes cross in two rounds. A single round through four rows
 // needs more registers than the alpha passes have and spills.
 constexpr int NR = 2;
@@ -61,12 +61,25 @@ struct TileArgs {
 };
 
 // Tile element e held by thread `tid` in register slot q: q supplies tile bits
-// [MAP, MAP+3), tid the other nine bits in order.
+// [MAP, MAP+3), tid the other nine. Mappings A and B (the load/store layout
+// and the first exchange) give them in order, so a wave's lanes cover
+// consecutive elements; C and D, which only meet LDS, take them in the order
+// that keeps their exchanges free of bank conflicts (below).
 template <int MAP>
 __device__ inline int tile_e(int tid, int q) {
+  if (MAP == MAP_C)  // tid bits -> e bits 0 1 2 8 9 6 7 10 11
+    return (tid & 7) | (q << 3) | (((tid >> 3) & 3) << 8) | (((tid >> 5) & 3) << 6) | ((tid >> 7) << 10);
+  if (MAP == MAP_D)  // tid bits -> e bits 5 6 7 3 4 8 9 10 11
+    return q | ((tid & 7) << 5) | (((tid >> 3) & 3) << 3) | ((tid >> 5) << 8);
   return (tid & ((1 << MAP) - 1)) | (q << MAP) | ((tid >> MAP) << (MAP + TQB));
 }
-__device__ inline int lds_slot(int e) { return e + (e >> 3); }
+// LDS slot (8 bytes) of tile element e: one pad slot per 32 elements, so bit
+// i of e moves the slot by 2^i for i < 5 and by 2^i + 2^(i-5) above. The 8-byte
+// accesses of 16 (or 32) lanes fall in distinct bank pairs when the weights
+// of the element bits those lanes vary are 1, 2, 4, 8 (, 16) modulo 16 (32):
+// A and B vary bits 0..4; C varies 0, 1, 2, 8 (264), 9 (528); D varies
+// 5 (33), 6 (66), 7 (132), 3, 4.
+__device__ inline int lds_slot(int e) { return e + (e >> 5); }
 
 // Position of tile T's element 0.
 __device__ inline void tile_base(const TileArgs& a, uint64_t T, uint64_t& vec0, uint64_t& pos0) {
@@ -451,7 +464,7 @@ __device__ inline void tile_body(const TileArgs& a) {
   }
 }
 
-// Two workgroups share a CU (72 KiB of LDS each) when a wave stays within 128
+// Two workgroups share a CU (66 KiB of LDS each) when a wave stays within 128
 // VGPRs. F65521's WHT . alpha . WHT modes need more registers and run one
 // workgroup per CU rather than spill; KoalaBear's fit.
 constexpr bool wide_mode(int mode) { return LANES == 8 && (mode == TM_WAW || mode == TM_COPY_WAW); }
