@@ -313,21 +313,45 @@ __global__ void k_encode_form(const E8* in, E8* out, uint64_t n) {
     out[i] = v8_store(v8_encode_form(v8_load(in[i])));
 }
 
-// High index bits [12, L) in the fewest groups of at most 10, split as evenly
-// as possible (tile passes with C = 4096 >> t contiguous positions): fewer
-// bits per pass mean longer contiguous runs per tile row at the same pass
-// count, e.g. L = 23 as (12, 6) + (18, 5) rather than (12, 10) + (22, 1).
-static std::vector<std::pair<int, int>> hi_groups(int L) {
+// Index bits [first, last) in the fewest groups of at most 10, split as
+// evenly as possible (larger groups first): tile passes with C = 4096 >> t
+// contiguous positions, so fewer bits per pass mean longer contiguous runs per
+// tile row at the same pass count, e.g. [12, 23) as (12, 6) + (18, 5) rather
+// than (12, 10) + (22, 1).
+static std::vector<std::pair<int, int>> even_groups(int first, int last) {
   std::vector<std::pair<int, int>> gs;
-  const int m = L - 12;
+  const int m = last - first;
   if (m <= 0) return gs;
   const int passes = (m + 9) / 10, base = m / passes, extra = m % passes;
-  for (int i = 0, s = 12; i < passes; i++) {
+  for (int i = 0, s = first; i < passes; i++) {
     const int t = base + (i < extra ? 1 : 0);
     gs.push_back({s, t});
     s += t;
   }
   return gs;
+}
+// High index bits [12, L): the groups of every WHT over the row bits above a
+// 4096-element tile.
+static std::vector<std::pair<int, int>> hi_groups(int L) { return even_groups(12, L); }
+
+// Mobius groups of a commit over the row bits [7, L) left after its first pass
+// (row bits 0..6 with the column bits). The last group [a, L) also starts the
+// parity WHT, whose pre-alpha bits [12, a) then take passes of their own (the
+// second list). Either row bits 7..11 as one group and then the high groups
+// (no pre-alpha passes beyond the high groups), or [7, L) split evenly,
+// whichever needs fewer passes, with a >= 12 so that the low parity pass does
+// not transform those bits again: L = 23 takes (7, 8) + (15, 8) and one
+// pre-alpha pass over bits 12..14, seven passes per block instead of eight.
+static std::pair<std::vector<std::pair<int, int>>, std::vector<std::pair<int, int>>> mobius_plan(int L) {
+  std::vector<std::pair<int, int>> ma{{7, 5}};
+  for (auto& g : hi_groups(L)) ma.push_back(g);
+  std::vector<std::pair<int, int>> pa(ma.begin() + 1, ma.end() - 1);
+  auto mb = even_groups(7, L);
+  // the last group's WHT covers its own bits only once if it lies above the low tile
+  if (mb.back().first < 12) return {ma, pa};
+  auto pb = even_groups(12, mb.back().first);
+  if (mb.size() + pb.size() < ma.size() + pa.size()) return {mb, pb};
+  return {ma, pa};
 }
 
 // Tile-pass schedules (qf_tile.hip); each returns false when its first pass
@@ -374,29 +398,30 @@ static bool tiled_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, h
 // Message halves = Mobius over all 5 + L index bits of the packed block (from
 // the BVA when pack, else from a flat E8 vector with column stride n), then the
 // parity. Pass 1 holds 32 columns x 128 rows per tile (column bits and row bits
-// 0..6), pass 2 row bits 7..11, then the high groups, the last of which also
-// starts the parity WHT.
-// mw (optional, 2 events): recorded around the last-high-group Mobius+WHT pass.
+// 0..6), then the Mobius groups of mobius_plan, the last of which also starts
+// the parity WHT.
+// mw (optional, 2 events): recorded around that last Mobius+WHT pass.
 static bool tiled_commit_rows(bool pack, const void* src, uint64_t block_first, uint64_t available, E8* cw, int L,
                               const E8* alpha, hipStream_t st, hipEvent_t* mw = nullptr) {
   if (L < 12) return false;
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
   if (!tile_pass(TP_M, pack, src, cw, nullptr, nullptr, 32, n, R, L, 0, 7, block_first, available, st, true))
     return false;
-  must(tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, 7, 5, 0, 0, st));
-  auto gs = hi_groups(L);
-  if (gs.empty()) {  // L = 12: message complete; one pass gives the parity
+  if (L == 12) {  // message complete after row bits 7..11; one pass gives the parity
+    must(tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, 7, 5, 0, 0, st));
     must(tile_pass(TP_WAW, false, cw, cw + n, nullptr, alpha, 32, R, R, L, 0, 12, 0, 0, st));
     return true;
   }
-  for (size_t i = 0; i + 1 < gs.size(); i++)
-    must(tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st));
+  const auto plan = mobius_plan(L);
+  const auto& mg = plan.first;
+  for (size_t i = 0; i + 1 < mg.size(); i++)
+    must(tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, mg[i].first, mg[i].second, 0, 0, st));
   // last Mobius group completes the message; the same pass starts the WHT in the parity half
   if (mw) QF_HIP(hipEventRecord(mw[0], st));
-  must(tile_pass(TP_M_W, false, cw, cw, cw + n, nullptr, 32, R, R, L, gs.back().first, gs.back().second, 0, 0, st));
+  must(tile_pass(TP_M_W, false, cw, cw, cw + n, nullptr, 32, R, R, L, mg.back().first, mg.back().second, 0, 0, st));
   if (mw) QF_HIP(hipEventRecord(mw[1], st));
-  for (size_t i = 0; i + 1 < gs.size(); i++)
-    must(tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, gs[i].first, gs[i].second, 0, 0, st));
+  for (auto& g : plan.second)
+    must(tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, g.first, g.second, 0, 0, st));
   tiled_parity_tail(cw + n, 32, R, L, alpha, st);
   return true;
 }

@@ -197,20 +197,16 @@ void write_rows(DevCtx& c, Bytes& out, const DCode& d, bool all, const std::vect
     put_u64(out, d.R);
     put_u64(out, d.cols);
     put_u64(out, d.R * d.cols);
-    auto e = dv::download(c, d.data(), d.R * d.cols);  // column-major
-    for (auto& v : e) put_e8(out, v);
+    const size_t at = out.size();  // the column-major codeword, element bytes as stored
+    out.resize(at + d.R * d.cols * sizeof(E8));
+    QF_HIP(hipMemcpyAsync(out.data() + at, d.data(), d.R * d.cols * sizeof(E8), hipMemcpyDeviceToHost, c.st));
+    c.sync();
     return;
   }
   out.push_back(0);
   put_u64(out, rows.size());
-  auto vals = d.rows(c, rows);
-  auto paths = dv::gather_paths(c, d.tree, d.R, rows);
-  for (size_t k = 0; k < rows.size(); k++) {
-    put_u64(out, d.cols);
-    for (uint64_t col = 0; col < d.cols; col++) put_e8(out, vals[k * d.cols + col]);
-    put_u64(out, paths[k].size());
-    for (auto& s : paths[k]) put_digest(out, s);
-  }
+  // per row: u64 cols, the row, u64 depth, its Merkle siblings
+  dv::append_row_records(c, out, d.data(), d.R, d.cols, d.tree, rows);
 }
 
 // ================================================================ native opening

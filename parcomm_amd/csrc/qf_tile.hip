@@ -178,8 +178,35 @@ __device__ inline void exchange(int32_t (&x)[TQ][LANES], uint2 (*lds)[TPAD]) {
   }
 }
 
+// One transform over the tile bits [lo, hi): the register phases of mappings
+// A, B, C, D with LDS exchanges between them and back to A. Mappings whose
+// three bits all lie below the lowest transformed bit are skipped with their
+// exchanges: from bit 9 up A alone, from 6 A and B, from 3 A, B and C.
+template <bool WHT, int LO>
+__device__ inline void transform(int32_t (&x)[TQ][LANES], uint2 (*lds)[TPAD], int lo, int hi) {
+  const int first = LO >= 0 ? LO : lo;  // uniform across the workgroup
+  phase<WHT, MAP_A, LO>(x, lo, hi);
+  if (first >= MAP_A) return;
+  exchange<MAP_A, MAP_B, NR>(x, lds);
+  phase<WHT, MAP_B, LO>(x, lo, hi);
+  if (first >= MAP_B) {
+    exchange<MAP_B, MAP_A, NR>(x, lds);
+    return;
+  }
+  exchange<MAP_B, MAP_C, NR>(x, lds);
+  phase<WHT, MAP_C, LO>(x, lo, hi);
+  if (first >= MAP_C) {
+    exchange<MAP_C, MAP_A, NR>(x, lds);
+    return;
+  }
+  exchange<MAP_C, MAP_D, NR>(x, lds);
+  phase<WHT, MAP_D, LO>(x, lo, hi);
+  exchange<MAP_D, MAP_A, NR>(x, lds);
+}
+
 #ifdef QF_KB
 // KoalaBear lanes are canonical throughout.
+template <int LO>
 __device__ inline void reduce_all(int32_t (&)[TQ][LANES]) {}
 
 __device__ inline void unpack_e8(const uint4& w, int32_t (&c)[LANES]) {
@@ -216,12 +243,23 @@ __device__ inline uint32_t reduce_lazy(int32_t v) {
   const uint32_t w = u - P;
   return u < w ? u : w;
 }
+// After at most 10 butterfly stages from canonical values |v| <= 2^10 (P - 1):
+// v + 2^10 P < 2^27, and one fold leaves (u & 0xffff) + 15 (u >> 16) < 2 P.
+constexpr int32_t BIAS10 = 1024 * int32_t(P);
+__device__ inline uint32_t reduce_lazy10(int32_t v) {
+  uint32_t u = uint32_t(v) + uint32_t(BIAS10);
+  u = u - uint32_t(__mul24(int32_t(u >> 16), int32_t(P)));
+  const uint32_t w = u - P;
+  return u < w ? u : w;
+}
+// LO >= 2: the pass runs at most 10 stages between reductions.
+template <int LO>
 __device__ inline void reduce_all(int32_t (&x)[TQ][8]) {
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int q = 0; q < TQ; q++)
 #pragma unroll
-    for (int k = 0; k < 8; k++) x[q][k] = int32_t(reduce_lazy(x[q][k]));
+    for (int k = 0; k < 8; k++) x[q][k] = int32_t(LO >= 2 ? reduce_lazy10(x[q][k]) : reduce_lazy(x[q][k]));
   __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -397,27 +435,13 @@ __device__ inline void tile_body(const TileArgs& a) {
       }
     }
     if (MODE == TM_M || MODE == TM_M_W) {
-      phase<false, MAP_A, LO>(x, lo, hi);
-      exchange<MAP_A, MAP_B, NR>(x, lds);
-      phase<false, MAP_B, LO>(x, lo, hi);
-      exchange<MAP_B, MAP_C, NR>(x, lds);
-      phase<false, MAP_C, LO>(x, lo, hi);
-      exchange<MAP_C, MAP_D, NR>(x, lds);
-      phase<false, MAP_D, LO>(x, lo, hi);
-      exchange<MAP_D, MAP_A, NR>(x, lds);
-      reduce_all(x);
+      transform<false, LO>(x, lds, lo, hi);
+      reduce_all<LO>(x);
       store_tile(a, T, so, a.out1, x);
     }
     if (MODE == TM_M_W || MODE == TM_W || MODE == TM_COPY_W || MODE == TM_AW) {
-      phase<true, MAP_A, LO>(x, lo, hi);
-      exchange<MAP_A, MAP_B, NR>(x, lds);
-      phase<true, MAP_B, LO>(x, lo, hi);
-      exchange<MAP_B, MAP_C, NR>(x, lds);
-      phase<true, MAP_C, LO>(x, lo, hi);
-      exchange<MAP_C, MAP_D, NR>(x, lds);
-      phase<true, MAP_D, LO>(x, lo, hi);
-      exchange<MAP_D, MAP_A, NR>(x, lds);
-      reduce_all(x);
+      transform<true, LO>(x, lds, lo, hi);
+      reduce_all<LO>(x);
       store_tile(a, T, so, MODE == TM_W || MODE == TM_AW ? a.out1 : a.out2, x);
     }
     if (WAW) {
@@ -439,7 +463,7 @@ __device__ inline void tile_body(const TileArgs& a) {
       phase<true, MAP_C, LO>(x, lo, hi);
       exchange<MAP_C, MAP_D, NR>(x, lds);
       phase<true, MAP_D, LO>(x, lo, hi);
-      reduce_all(x);
+      reduce_all<LO>(x);
 #pragma unroll
       for (int q = 0; q < TQ; q += 2) {
         const uint4 c0 = al0, c1 = al1;
@@ -456,7 +480,7 @@ __device__ inline void tile_body(const TileArgs& a) {
       phase<true, MAP_B, LO>(x, lo, hi);
       exchange<MAP_B, MAP_A, NR>(x, lds);
       phase<true, MAP_A, LO>(x, lo, hi);
-      reduce_all(x);
+      reduce_all<LO>(x);
       store_tile(a, T, so, MODE == TM_WAW ? a.out1 : a.out2, x);
       if (PF && T + step < last) load_tile<PACK>(a, T + step, so, raw);
     }

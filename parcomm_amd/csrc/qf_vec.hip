@@ -96,16 +96,30 @@ __global__ __launch_bounds__(WG) void k_lb_fold_cross(E8* __restrict__ a, E8* __
 // q'[i] = q[2i] + q[2i+1] r) of length-2h vectors into length-h outputs,
 // fused with the next round's cross terms over the outputs:
 // sum_j w'[2j] q'[2j+1] and sum_j w'[2j+1] q'[2j] (h even).
-__global__ __launch_bounds__(WG) void k_eo_fold_cross(const E8* __restrict__ w, const E8* __restrict__ q,
-                                                      E8* __restrict__ ow, E8* __restrict__ oq, uint64_t h, E8 ri,
-                                                      E8 r, uint32_t* __restrict__ partials) {
+// At most 128 registers, so the 1024 workgroups of a full launch are all resident.
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_eo_fold_cross(
+    const E8* __restrict__ w, const E8* __restrict__ q, E8* __restrict__ ow, E8* __restrict__ oq, uint64_t h, E8 ri,
+    E8 r, uint32_t* __restrict__ partials) {
   const V8 vri = v8_encode_form(v8_load(ri)), vr = v8_encode_form(v8_load(r));
   V8 s[2] = {v8_zero(), v8_zero()};
-  for (uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x; j < h / 2; j += uint64_t(gridDim.x) * WG) {
-    const V8 a0 = v8_add(v8_load(w[4 * j]), v8_mul_enc(v8_load(w[4 * j + 1]), vri));
-    const V8 a1 = v8_add(v8_load(w[4 * j + 2]), v8_mul_enc(v8_load(w[4 * j + 3]), vri));
-    const V8 b0 = v8_add(v8_load(q[4 * j]), v8_mul_enc(v8_load(q[4 * j + 1]), vr));
-    const V8 b1 = v8_add(v8_load(q[4 * j + 2]), v8_mul_enc(v8_load(q[4 * j + 3]), vr));
+  const uint64_t n = h / 2, stride = uint64_t(gridDim.x) * WG;
+  // the next iteration's eight inputs load while this one computes
+  E8 nx[8];
+  auto fetch = [&](uint64_t j) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) { nx[k] = w[4 * j + k]; nx[4 + k] = q[4 * j + k]; }
+  };
+  uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (j < n) fetch(j);
+  for (; j < n; j += stride) {
+    E8 cx[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) cx[k] = nx[k];
+    if (j + stride < n) fetch(j + stride);
+    const V8 a0 = v8_add(v8_load(cx[0]), v8_mul_enc(v8_load(cx[1]), vri));
+    const V8 a1 = v8_add(v8_load(cx[2]), v8_mul_enc(v8_load(cx[3]), vri));
+    const V8 b0 = v8_add(v8_load(cx[4]), v8_mul_enc(v8_load(cx[5]), vr));
+    const V8 b1 = v8_add(v8_load(cx[6]), v8_mul_enc(v8_load(cx[7]), vr));
     ow[2 * j] = v8_store(a0);
     ow[2 * j + 1] = v8_store(a1);
     oq[2 * j] = v8_store(b0);
@@ -273,16 +287,36 @@ __global__ void k_gather_weighted(const E8* __restrict__ a, uint64_t Ra, uint64_
   for (uint64_t col = 0; col < cb; col++) s = v8_add(s, v8_mul(v8_load(b[col * Rb + r]), v8_load(w[ca + col])));
   out[k] = v8_store(s);
 }
-__global__ void k_gather_paths(const uint32_t* tree, uint64_t R, int depth, const uint64_t* rows, uint64_t count,
-                               uint32_t* out) {
-  uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
-  if (i >= count * uint64_t(depth)) return;
-  uint64_t k = i / depth;
-  int l = int(i % depth);
-  uint64_t off = 0, cnt = R;
-  for (int j = 0; j < l; j++) { off += cnt; cnt >>= 1; }
-  uint64_t sib = ((rows[k] >> l) ^ 1) + off;
-  for (int w = 0; w < 8; w++) out[i * 8 + w] = tree[sib * 8 + w];
+// Serialized query rows, record k at out + k * per (per = 16 + 16 cols + 32
+// depth): u64 cols, the cols elements of row rows[k] of a column-major
+// codeword, u64 depth, then the depth sibling digests from the leaf level up
+// (tree: the R leaf digests, then each level above). One thread per element,
+// digest or header pair; records are 16-byte aligned, their fields 8-byte.
+__global__ void k_row_records(const E8* __restrict__ cw, uint64_t R, uint64_t cols, const uint32_t* __restrict__ tree,
+                              int depth, const uint64_t* __restrict__ rows, uint64_t count, uint8_t* __restrict__ out) {
+  const uint64_t items = cols + uint64_t(depth) + 1, per = 16 + 16 * cols + 32 * uint64_t(depth);
+  const uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (i >= count * items) return;
+  const uint64_t k = i / items, j = i % items;
+  uint8_t* rec = out + k * per;
+  const uint64_t row = rows[k];
+  if (j < cols) {
+    const uint4 v = *reinterpret_cast<const uint4*>(&cw[j * R + row]);
+    uint2* o = reinterpret_cast<uint2*>(rec + 8 + 16 * j);
+    o[0] = make_uint2(v.x, v.y);
+    o[1] = make_uint2(v.z, v.w);
+  } else if (j < cols + uint64_t(depth)) {
+    const int l = int(j - cols);
+    uint64_t off = 0, cnt = R;
+    for (int t = 0; t < l; t++) { off += cnt; cnt >>= 1; }
+    const uint32_t* d = tree + 8 * (((row >> l) ^ 1) + off);
+    uint2* o = reinterpret_cast<uint2*>(rec + 16 + 16 * cols + 32 * uint64_t(l));
+#pragma unroll
+    for (int w = 0; w < 4; w++) o[w] = make_uint2(d[2 * w], d[2 * w + 1]);
+  } else {
+    *reinterpret_cast<uint64_t*>(rec) = cols;
+    *reinterpret_cast<uint64_t*>(rec + 8 + 16 * cols) = uint64_t(depth);
+  }
 }
 __global__ void k_scatter_add(E8* dst, const uint64_t* idx, const E8* vals, uint64_t n) {
   uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x;
@@ -1374,27 +1408,24 @@ std::vector<E8> gather_weighted(DevCtx& c, const E8* a, uint64_t Ra, uint64_t ca
   dfree(d_rows, c.st);
   return r;
 }
-std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, uint64_t R,
-                                              const std::vector<uint64_t>& rows) {
-  int depth = ilog2u(R);
-  std::vector<std::vector<Digest>> out(rows.size());
-  if (rows.empty() || depth == 0) return out;
-  uint64_t total = rows.size() * depth;
+void append_row_records(DevCtx& c, Bytes& out, const E8* cw, uint64_t R, uint64_t cols, const uint32_t* tree,
+                        const std::vector<uint64_t>& rows) {
+  if (rows.empty()) return;
+  for (uint64_t r : rows) require(r < R, QF_ERR_ARGUMENT, "query row out of range");
+  const int depth = ilog2u(R);
+  const uint64_t per = 16 + 16 * cols + 32 * uint64_t(depth), total = rows.size() * (cols + uint64_t(depth) + 1);
   uint64_t* d_rows = static_cast<uint64_t*>(dmalloc(rows.size() * 8, c.st));
-  uint32_t* d_out = static_cast<uint32_t*>(dmalloc(total * 32, c.st));
+  uint8_t* d_out = static_cast<uint8_t*>(dmalloc(rows.size() * per, c.st));
   h2d(c, d_rows, rows.data(), rows.size() * 8);
-  hipLaunchKernelGGL(k_gather_paths, dim3(uint32_t((total + WG - 1) / WG)), dim3(WG), 0, c.st, tree, R, depth, d_rows,
-                     uint64_t(rows.size()), d_out);
-  std::vector<uint8_t> host(total * 32);
-  QF_HIP(hipMemcpyAsync(host.data(), d_out, total * 32, hipMemcpyDeviceToHost, c.st));
+  hipLaunchKernelGGL(k_row_records, dim3(uint32_t((total + WG - 1) / WG)), dim3(WG), 0, c.st, cw, R, cols, tree, depth,
+                     d_rows, uint64_t(rows.size()), d_out);
+  QF_HIP(hipGetLastError());
+  const size_t at = out.size();
+  out.resize(at + rows.size() * per);
+  QF_HIP(hipMemcpyAsync(out.data() + at, d_out, rows.size() * per, hipMemcpyDeviceToHost, c.st));
   c.sync();
   dfree(d_rows, c.st);
   dfree(d_out, c.st);
-  for (size_t k = 0; k < rows.size(); k++) {
-    out[k].resize(depth);
-    for (int l = 0; l < depth; l++) std::memcpy(out[k][l].data(), host.data() + (k * depth + l) * 32, 32);
-  }
-  return out;
 }
 void scatter_add(DevCtx& c, E8* dst, const std::vector<uint64_t>& idx, const std::vector<E8>& vals) {
   if (idx.empty()) return;

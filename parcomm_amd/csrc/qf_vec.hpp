@@ -78,9 +78,11 @@ std::vector<E8> gather_rows(DevCtx& c, const E8* cw, uint64_t R, uint64_t cols, 
 // (column-major a and b; b may be null when cb = 0).
 std::vector<E8> gather_weighted(DevCtx& c, const E8* a, uint64_t Ra, uint64_t ca, const E8* b, uint64_t Rb, uint64_t cb,
                                 const std::vector<E8>& w, const std::vector<uint64_t>& rows);
-// Merkle paths (siblings leaf to root) of `rows` in a device tree of R leaves.
-std::vector<std::vector<Digest>> gather_paths(DevCtx& c, const uint32_t* tree, uint64_t R,
-                                              const std::vector<uint64_t>& rows);
+// Appends one record per query row to `out`, built on the device and copied
+// back once: u64 cols, the row's cols elements (column-major codeword with R
+// rows), u64 depth = log2 R, then its depth Merkle siblings leaf to root.
+void append_row_records(DevCtx& c, Bytes& out, const E8* cw, uint64_t R, uint64_t cols, const uint32_t* tree,
+                        const std::vector<uint64_t>& rows);
 void scatter_add(DevCtx& c, E8* dst, const std::vector<uint64_t>& idx, const std::vector<E8>& vals);
 }  // namespace dv
 

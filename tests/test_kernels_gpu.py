@@ -114,6 +114,32 @@ def test_tile_pass_matches_numpy(vectors, L, s, t, mode, field):
         assert np.array_equal(o2, transform(e8_mul(w, alpha[None], field), bits, True, P))
 
 
+# Geometries of the 2^23-row commit schedule: Mobius on row bits 7..14 and
+# 15..22, the pre-alpha parity pass on bits 12..14 (a pass without exchanges)
+# and the high parity groups.
+PLAN23 = [(1, 23, 7, 8), (1, 23, 15, 8), (1, 23, 12, 3), (1, 23, 12, 6), (1, 23, 18, 5)]
+
+
+@pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
+@pytest.mark.parametrize("vectors,L,s,t", PLAN23)
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_tile_pass_commit_plan_geometries(vectors, L, s, t, mode, field):
+    P, lanes, _, _ = FIELDS[field]
+    rng = np.random.default_rng(3000 + 10 * s + t + mode + 7 * field)
+    x = field_values(rng, P, (vectors, 1 << L, lanes))
+    print("plan tile pass", FIELD_IDS[field], "mode", mode, "L", L, "bits", s, s + t, flush=True)
+    o1, o2 = run(field, mode, x, None, L, s, t)
+    bits = list(range(s, s + t))
+    if mode == 0:
+        assert np.array_equal(o1, transform(x, bits, False, P))
+    elif mode == 1:
+        m = transform(x, bits, False, P)
+        assert np.array_equal(o1, m)
+        assert np.array_equal(o2, transform(m, bits, True, P))
+    else:
+        assert np.array_equal(o1, transform(x, bits, True, P))
+
+
 def transform_vectors(x, nbits, wht, P):
     """2x2 maps on the low nbits of the vector index (axis 0)."""
     y = x.astype(np.int64).copy()
