@@ -158,6 +158,69 @@ DVec tw_vec(DevCtx& c, const std::vector<E8>& point) {
 
 struct LB { E8 a_value, b_value; std::vector<E8> a_point, b_point; };
 // qafold_short_pc.rs:1903-1948 (label null: the multi-evaluation form without a domain frame)
+// Folds of two device vectors over their top halves, a' = a_lo + x a_hi and
+// b' = b_lo + y b_hi, with the cross terms (sum a_lo b_hi, sum a_hi b_lo) of
+// each result (LiteBullet and the Decide multi-evaluation). Once the vectors
+// hold at most HOST_TAIL elements they are copied to the host and the last
+// rounds run there: one download replaces a launch and a host round trip per
+// round, with the same field arithmetic.
+class TwoFold {
+ public:
+  static constexpr uint64_t HOST_TAIL = 256;
+  TwoFold(DevCtx& c, DVec& a, DVec& b) : c_(c), da_(a), db_(b) {}
+  // Cross terms of the length-n vectors (n even).
+  std::pair<E8, E8> cross(uint64_t n) {
+    if (n <= HOST_TAIL) to_host(n);
+    if (!host_) return dv::cross_halves(c_, da_.p, db_.p, n / 2);
+    return host_cross(n / 2);
+  }
+  // Folds length n to n / 2; the cross terms of the result (zero below length 2).
+  std::pair<E8, E8> fold(uint64_t n, const E8& x, const E8& y) {
+    const uint64_t h = n / 2;
+    if (n <= HOST_TAIL) to_host(n);
+    if (!host_) {
+      if (h >= 2) return dv::lb_fold_cross(c_, da_.p, db_.p, h, x, y);
+      dv::lb_fold(c_, da_.p, db_.p, h, x, y);
+      return {E8{}, E8{}};
+    }
+    for (uint64_t i = 0; i < h; i++) {
+      a_[i] = a_[i] + x * a_[h + i];
+      b_[i] = b_[i] + y * b_[h + i];
+    }
+    a_.resize(h);
+    b_.resize(h);
+    return h >= 2 ? host_cross(h / 2) : std::pair<E8, E8>{E8{}, E8{}};
+  }
+  // The first n elements of a into device memory.
+  void copy_a(E8* dst, uint64_t n) {
+    if (host_) QF_HIP(hipMemcpyAsync(dst, a_.data(), n * sizeof(E8), hipMemcpyHostToDevice, c_.st));
+    else dv::copy(c_, dst, da_.p, n);
+  }
+  E8 a0() { return host_ ? a_[0] : dv::download(c_, da_.p, 1)[0]; }
+  E8 b0() { return host_ ? b_[0] : dv::download(c_, db_.p, 1)[0]; }
+
+ private:
+  void to_host(uint64_t n) {
+    if (host_) return;
+    a_ = dv::download(c_, da_.p, n);
+    b_ = dv::download(c_, db_.p, n);
+    host_ = true;
+  }
+  std::pair<E8, E8> host_cross(uint64_t g) const {
+    E8 l{}, r{};
+    for (uint64_t i = 0; i < g; i++) {
+      l += a_[i] * b_[g + i];
+      r += a_[g + i] * b_[i];
+    }
+    return {l, r};
+  }
+  DevCtx& c_;
+  DVec& da_;
+  DVec& db_;
+  std::vector<E8> a_, b_;
+  bool host_ = false;
+};
+
 LB litebullet(DevCtx& c, E8 claimed, DVec& left, DVec& right, Pcs& tr, const char* label, const char* fold_label,
               bool check_claim = true) {
   uint64_t n = left.n;
@@ -167,22 +230,20 @@ LB litebullet(DevCtx& c, E8 claimed, DVec& left, DVec& right, Pcs& tr, const cha
   LB lb;
   E8 rhs = claimed;
   E8 lc{}, rc{};
-  if (n > 1) std::tie(lc, rc) = dv::cross_halves(c, left.p, right.p, n / 2);
+  TwoFold tf(c, left, right);
+  if (n > 1) std::tie(lc, rc) = tf.cross(n);
   while (n > 1) {  // each fold also returns the next round's cross terms
-    uint64_t h = n / 2;
     tr.respond(lc);
     tr.respond(rc);
     E8 ch = tr.challenge(fold_label, true), ci = e8_inv(ch);
     rhs += lc * ch + rc * ci;
-    if (h >= 2) std::tie(lc, rc) = dv::lb_fold_cross(c, left.p, right.p, h, ci, ch);
-    else dv::lb_fold(c, left.p, right.p, h, ci, ch);
+    std::tie(lc, rc) = tf.fold(n, ci, ch);
     lb.a_point.push_back(ci);
     lb.b_point.push_back(ch);
-    n = h;
+    n /= 2;
   }
-  auto a = dv::download(c, left.p, 1), b = dv::download(c, right.p, 1);
-  lb.a_value = a[0];
-  lb.b_value = b[0];
+  lb.a_value = tf.a0();
+  lb.b_value = tf.b0();
   tr.respond(lb.a_value);
   tr.respond(lb.b_value);
   require(lb.a_value * lb.b_value == rhs, QF_ERR_INVALID_COMMITMENT, "litebullet terminal");
@@ -424,20 +485,19 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     DVec folded_vector(rows);
     uint64_t n = padded * rows;
     E8 lc{}, rc{};
-    if (n > 1) std::tie(lc, rc) = dv::cross_halves(c, left.p, right.p, n / 2);
+    TwoFold tf(c, left, right);
+    if (n > 1) std::tie(lc, rc) = tf.cross(n);
     while (n > 1) {
-      uint64_t hh = n / 2;
       tr.respond(lc);
       tr.respond(rc);
       E8 ch = tr.challenge("multi-eval-fold", true), ci = e8_inv(ch);
       rhs += lc * ch + rc * ci;
-      if (hh >= 2) std::tie(lc, rc) = dv::lb_fold_cross(c, left.p, right.p, hh, ci, ch);
-      else dv::lb_fold(c, left.p, right.p, hh, ci, ch);
+      std::tie(lc, rc) = tf.fold(n, ci, ch);
       red_point.push_back(ci);
-      n = hh;
-      if (n == rows) dv::copy(c, folded_vector.p, left.p, rows);
+      n /= 2;
+      if (n == rows) tf.copy_a(folded_vector.p, rows);
     }
-    E8 red_value = dv::download(c, left.p, 1)[0], bval = dv::download(c, right.p, 1)[0];
+    E8 red_value = tf.a0(), bval = tf.b0();
     tr.respond(red_value);
     tr.respond(bval);
     require(red_value * bval == rhs, QF_ERR_INVALID_COMMITMENT, "multi-eval terminal");
