@@ -777,6 +777,8 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   DVec w2(rows / 2 ? rows / 2 : 1), q2(rows / 2 ? rows / 2 : 1);
   DVec *cw = &w, *cq = &qv, *nw = &w2, *nq = &q2;
   uint64_t n = rows;
+  // the last rounds run on host copies once the vectors are short (as TwoFold)
+  std::vector<E8> hw, hq;
   while (n > 1) {
     uint64_t h = n / 2;
     tr.object("binary-cross", js_scalars({c0, c1}));
@@ -784,17 +786,35 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     claim += c0 * r + c1 * ri;
     challenges.push_back(r);
     o.rounds.push_back({c0, c1});
-    if (h >= 2) {
-      std::tie(c0, c1) = dv::eo_fold_cross(c, cw->p, cq->p, nw->p, nq->p, h, ri, r);
-    } else {
-      dv::eo_fold(c, cw->p, cq->p, nw->p, nq->p, h, ri, r);
+    if (hw.empty() && n <= TwoFold::HOST_TAIL) {
+      hw = dv::download(c, cw->p, n);
+      hq = dv::download(c, cq->p, n);
     }
-    std::swap(cw, nw);
-    std::swap(cq, nq);
+    if (!hw.empty()) {  // w'[i] = w[2i] + w[2i+1] r^-1, q'[i] = q[2i] + q[2i+1] r, then the cross terms
+      for (uint64_t i = 0; i < h; i++) {
+        hw[i] = hw[2 * i] + ri * hw[2 * i + 1];
+        hq[i] = hq[2 * i] + r * hq[2 * i + 1];
+      }
+      hw.resize(h);
+      hq.resize(h);
+      c0 = c1 = E8{};
+      for (uint64_t j = 0; 2 * j + 1 < h; j++) {
+        c0 += hw[2 * j] * hq[2 * j + 1];
+        c1 += hw[2 * j + 1] * hq[2 * j];
+      }
+    } else {
+      if (h >= 2) {
+        std::tie(c0, c1) = dv::eo_fold_cross(c, cw->p, cq->p, nw->p, nq->p, h, ri, r);
+      } else {
+        dv::eo_fold(c, cw->p, cq->p, nw->p, nq->p, h, ri, r);
+      }
+      std::swap(cw, nw);
+      std::swap(cq, nq);
+    }
     n = h;
   }
-  o.value = dv::download(c, cw->p, 1)[0];
-  o.query = dv::download(c, cq->p, 1)[0];
+  o.value = hw.empty() ? dv::download(c, cw->p, 1)[0] : hw[0];
+  o.query = hq.empty() ? dv::download(c, cq->p, 1)[0] : hq[0];
   require(claim == o.value * o.query && query_eval(q, challenges) == o.query, QF_ERR_INVALID_COMMITMENT, "EO terminal mismatch");
   tr.scalar("terminal-value", o.value);
   tr.scalar("terminal-query", o.query);
@@ -814,6 +834,8 @@ void sumcheck(DevCtx& c, DVec& v, DVec& w, E8& claim, Packed& t, std::vector<std
   uint64_t n = v.n;
   E8 m0{}, m1{};
   if (n > 1) std::tie(m0, m1) = dv::sc_msg(c, v.p, w.p, n / 2);
+  // the last rounds run on host copies once the vectors are short (as TwoFold)
+  std::vector<E8> hv, hw;
   while (n > 1) {  // each fold also returns the next round's message
     uint64_t h = n / 2;
     t.field(m0);
@@ -821,13 +843,32 @@ void sumcheck(DevCtx& c, DVec& v, DVec& w, E8& claim, Packed& t, std::vector<std
     E8 r = t.challenge();
     claim = m0 + r * ((claim - m0 - m0 - m1) + r * m1);
     rounds.push_back({m0, m1});
-    if (h >= 2) std::tie(m0, m1) = dv::sc_fold_msg(c, v.p, w.p, h, r);
-    else dv::sc_fold(c, v.p, w.p, h, r);
+    if (hv.empty() && n <= TwoFold::HOST_TAIL) {
+      hv = dv::download(c, v.p, n);
+      hw = dv::download(c, w.p, n);
+    }
+    if (!hv.empty()) {  // v' = v_lo + r (v_hi - v_lo); message of the halves of v', w'
+      for (uint64_t i = 0; i < h; i++) {
+        hv[i] = hv[i] + r * (hv[h + i] - hv[i]);
+        hw[i] = hw[i] + r * (hw[h + i] - hw[i]);
+      }
+      hv.resize(h);
+      hw.resize(h);
+      m0 = m1 = E8{};
+      for (uint64_t g = h / 2, i = 0; i < g; i++) {
+        m0 += hv[i] * hw[i];
+        m1 += (hv[g + i] - hv[i]) * (hw[g + i] - hw[i]);
+      }
+    } else if (h >= 2) {
+      std::tie(m0, m1) = dv::sc_fold_msg(c, v.p, w.p, h, r);
+    } else {
+      dv::sc_fold(c, v.p, w.p, h, r);
+    }
     reduced.push_back(r);
     n = h;
   }
-  fval = dv::download(c, v.p, 1)[0];
-  fwt = dv::download(c, w.p, 1)[0];
+  fval = hv.empty() ? dv::download(c, v.p, 1)[0] : hv[0];
+  fwt = hw.empty() ? dv::download(c, w.p, 1)[0] : hw[0];
 }
 // Lanes of every block against eq(outer): lanes[b][j] = sum_i entry(b, LANES i + j) * eq_i
 std::vector<Lanes> block_lanes(DevCtx& c, const ParamsState& p, const Val* bva, uint64_t active,
