@@ -162,6 +162,7 @@ def measure(pa, field, args, per_matrix=True, serve_requests=0, model="8b", step
         finally:
             pa.set_launch_mode(pa.LAUNCH_BATCHED, field)
         res["per_matrix_s"] = pm["commit_s"] + pm["open_s"]
+        res["per_matrix_open_s"] = pm["open_s"]
         print(f"# {tag} per-matrix form: commit {pm['commit_s']:.3f} s open {pm['open_s']:.3f} s", file=sys.stderr,
               flush=True)
     if serve_requests:  # resident commitment, Poisson opening requests (mean 50/s), concurrent workers
@@ -177,8 +178,8 @@ def measure(pa, field, args, per_matrix=True, serve_requests=0, model="8b", step
 
 def roofline_of(pa, res, field_tag):
     """Roofline object of the dominant kernel: the pass that finishes the
-    message (Mobius on the last high row group) and starts the parity (WHT on
-    the same bits). Per launch it reads the 32-column message half and writes
+    message (Mobius on the last row group of the commit schedule) and starts
+    the parity (WHT on the same bits). Per launch it reads the 32-column message half and writes
     the message and parity halves: 3 x 32 n elements of 16 bytes. Its launch
     time comes from HIP events on the commit stream around every launch."""
     params, timings = res["params"], res["timings"]
@@ -192,7 +193,7 @@ def roofline_of(pa, res, field_tag):
     achieved = mw_bytes / (mw_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(field_tag),
-            "kernel": "k_tile<M_W> (Mobius + WHT, last high row group)",
+            "kernel": "k_tile<M_W> (Mobius + WHT, last row group)",
             "algorithmic_bytes": mw_bytes, "avg_launch_ms": round(mw_ms, 4), "launches": launches,
             "leaf_hash_ms": round(leaf_ms, 4)}
 
@@ -264,6 +265,9 @@ def summary(res):
     if "per_matrix_s" in res:
         out["per_matrix_s"] = round(res["per_matrix_s"], 4)
         out["batched_speedup"] = round(res["per_matrix_s"] / (mean("commit_s") + mean("open_s")), 3)
+        # the per-term launches are all in the opening; the commit is the same in both forms
+        out["per_matrix_open_s"] = round(res["per_matrix_open_s"], 4)
+        out["batched_open_speedup"] = round(res["per_matrix_open_s"] / mean("open_s"), 3)
     if "serving" in res:
         out["serving"] = res["serving"]
     return out
