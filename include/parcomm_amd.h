@@ -171,11 +171,6 @@ int qf_device_memory(size_t* free_bytes, size_t* total_bytes);
  * on boundary padding. */
 int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* device_out);
 
-/* Launch form of the opening's per-term kernels (no reference counterpart; the
- * reference runs on CPU threads): QF_LAUNCH_BATCHED (default) issues one launch
- * per phase over every bucket term of every matrix; QF_LAUNCH_PER_MATRIX issues
- * one launch and one host round trip per term, the plain per-matrix GPU form
- * the benchmark compares against. Proofs are identical in both. Process-wide. */
 /* The tile-pass autotuner's choices so far in this process (no reference
  * counterpart): a JSON array with, per pass geometry (mode, packing, L, s, t,
  * vectors), the launch shapes tried (kernel variant, workgroups per CU, XCD
@@ -187,6 +182,11 @@ int qf_synthetic_bva(const qf_params* p, uint64_t seed, float sigma, void* devic
  * length. */
 int qf_autotune_report(char* out, size_t cap, size_t* len);
 
+/* Launch form of the opening's per-term kernels (no reference counterpart; the
+ * reference runs on CPU threads): QF_LAUNCH_BATCHED (default) issues one launch
+ * per phase over every bucket term of every matrix; QF_LAUNCH_PER_MATRIX issues
+ * one launch and one host round trip per term, the plain per-matrix GPU form
+ * the benchmark compares against. Proofs are identical in both. Process-wide. */
 #define QF_LAUNCH_BATCHED 0
 #define QF_LAUNCH_PER_MATRIX 1
 int qf_set_launch_mode(int mode);
