@@ -15,6 +15,7 @@ namespace {
 constexpr int WG = 256;
 constexpr uint64_t TAB_CAP = 1u << 18;  // E8 entries of the small-table scratch
 constexpr uint32_t PROJ_TAB_LOG = 12;   // low-bit weight table of a projection chunk: 2^12 entries
+constexpr uint32_t MF_TAB_LOG = 11;     // the same for the MFMA projection: 32 KiB, three workgroups per CU
 
 __device__ inline V8 v8_madd(const V8& acc, const V8& x) { return v8_add(acc, x); }
 
@@ -560,6 +561,145 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
     }
   }
 }
+
+#ifndef QF_KB
+// Projection chunks of 16- and 8-lane rows (the EO cross-term and the batch
+// lane projections) as an exact i8 MFMA contraction of the BVA against the
+// weight table, over 16-byte records: one 16-lane row, or two consecutive
+// 8-lane rows 2r', 2r'+1, whose weights pairs[0][b] T'[r'] share the table T'
+// over the row bits above bit 0.
+// - Digits: a canonical table coefficient t is written as t' = t or t - p in
+//   [-32881, 32639]; h = (t' + 128) >> 8 and l = t' - 256 h then both lie in
+//   [-128, 127] and t' = 256 h + l. Record r of the digit table holds the low
+//   digits of coefficients 0..7 in bytes 0..7 and the high digits in 8..15.
+// - v_mfma_i32_16x16x32_i8 adds, per 32 records, the products entry x digit
+//   (|.| <= 2^14) into D[record byte][digit byte]; a hi group of at most 2^11
+//   records stays below 2^25 per wave and 2^27 per workgroup, exact in i32.
+// - Operands: ds_read_tr8_b64 on 8 consecutive 16-byte records in LDS gives
+//   lane i of each 16-lane group byte i of every record, which is the
+//   K-contiguous layout of both operands: BVA records (staged through LDS by
+//   the wave that loaded them, 4 KiB per round) for A, digit records for B.
+// - Each hi group's per-wave sums are combined through LDS, reduced mod p
+//   (low + 256 high), the two rows of an 8-lane record joined with pairs[0],
+//   multiplied by the group's scale and accumulated by one thread per lane.
+// The chunk's rows start 16-byte aligned and a hi group holds at least
+// MF_ROWS records per wave (the host routes other chunks to k_tensor_project).
+constexpr int MF_ROWS = 256;            // records per wave and round: 8 blocks of 32, 4 KiB
+constexpr int MF_WAVES = WG / 64;
+typedef int32_t qf_i32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t qf_i32x2 __attribute__((ext_vector_type(2)));
+__device__ inline long tr8_b64(const uint8_t* p) {
+  const qf_i32x2 v = __builtin_amdgcn_ds_read_tr8_b64_v2i32(
+      (__attribute__((address_space(3))) qf_i32x2*)(const_cast<uint8_t*>(p)));
+  return (long(uint32_t(v.y)) << 32) | long(uint32_t(v.x));
+}
+template <int L>
+__global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restrict__ bva, uint64_t available,
+                                                            const ProjTaskDev* __restrict__ tasks,
+                                                            const E8 (*__restrict__ pairs)[2], E8* __restrict__ out) {
+  static_assert(L == 16 || L == 8, "16-byte records of one or two rows");
+  constexpr int RBITS = L == 8 ? 1 : 0;            // row bits inside a record
+  __shared__ E8 T[1 << MF_TAB_LOG];                // record weights, then their digit bytes
+  __shared__ uint4 stage[MF_WAVES][MF_ROWS];       // a round of BVA records per wave; the combine reuses it
+  __shared__ uint32_t pv[16][8];                   // a group's record-byte sums mod p
+  __shared__ E8 Chi[32];                           // scale x high-bit weights of each hi group
+  const ProjTaskDev t = tasks[blockIdx.x];
+  const E8(*tp)[2] = pairs + t.pair_off;
+  build_table(T, tp + RBITS, int(t.k) - RBITS);
+  const uint32_t lo_rows = 1u << t.k, lo_recs = lo_rows >> RBITS;
+  const uint32_t hi_count = 1u << (t.chunk_rows_log - t.k);  // <= 32
+  if (threadIdx.x < hi_count) {
+    const uint64_t row_hi = t.chunk_first + uint64_t(threadIdx.x) * lo_rows;
+    V8 c = v8_load(t.scale);
+    for (uint32_t b = t.k; b < t.nbits; b++) c = v8_mul(c, v8_load(tp[b][(row_hi >> b) & 1]));
+    Chi[threadIdx.x] = v8_store(c);
+  }
+  for (uint32_t r = threadIdx.x; r < lo_recs; r += WG) {  // digits, in place
+    const E8 e = T[r];
+    uint32_t d[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int32_t tv = e.c[c] <= 32639 ? int32_t(e.c[c]) : int32_t(e.c[c]) - int32_t(P);
+      const int32_t hd = (tv + 128) >> 8, ld = tv - 256 * hd;
+      d[c / 4] |= uint32_t(uint8_t(ld)) << (8 * (c % 4));
+      d[2 + c / 4] |= uint32_t(uint8_t(hd)) << (8 * (c % 4));
+    }
+    reinterpret_cast<uint4*>(T)[r] = make_uint4(d[0], d[1], d[2], d[3]);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint8_t* tb = reinterpret_cast<const uint8_t*>(T);
+  uint8_t* st = reinterpret_cast<uint8_t*>(stage[wave]);
+  const uint32_t wrecs = lo_recs / MF_WAVES, rounds = wrecs / MF_ROWS, total = hi_count * rounds;
+  // round j: hi group j / rounds, this wave's records [r0, r0 + MF_ROWS) of it
+  auto fetch = [&](uint32_t j, uint4 (&w)[4]) {
+    const uint32_t r0 = wave * wrecs + (j % rounds) * MF_ROWS;
+    const uint64_t b0 = t.base + (t.chunk_first + uint64_t(j / rounds) * lo_rows) * L + uint64_t(r0) * 16;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint64_t b = b0 + 1024 * u + 16 * lane;
+      if (b + 16 <= available) {
+        w[u] = *reinterpret_cast<const uint4*>(bva + b);
+      } else {  // entries at or past the active end read as zero
+        uint32_t x[4] = {0, 0, 0, 0};
+        for (int q = 0; q < 16; q++)
+          if (b + q < available) x[q / 4] |= uint32_t(uint8_t(bva[b + q])) << (8 * (q % 4));
+        w[u] = make_uint4(x[0], x[1], x[2], x[3]);
+      }
+    }
+  };
+  V8 acc = v8_zero();  // threads 0..L-1: lane threadIdx.x of the chunk
+  qf_i32x4 dsum = {0, 0, 0, 0};
+  uint4 cur[4], nxt[4];
+  fetch(0, cur);
+  for (uint32_t j = 0; j < total; j++) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the previous round's reads of st are done
+#pragma unroll
+    for (int u = 0; u < 4; u++) *reinterpret_cast<uint4*>(st + 1024 * u + 16 * lane) = cur[u];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (j + 1 < total) fetch(j + 1, nxt);
+    const uint32_t r0 = wave * wrecs + (j % rounds) * MF_ROWS;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const long a = tr8_b64(st + 512 * b + 8 * lane);
+      const long w = tr8_b64(tb + 16 * (r0 + 32 * b) + 8 * lane);
+      dsum = __builtin_amdgcn_mfma_i32_16x16x32_i8(a, w, dsum, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) cur[u] = nxt[u];
+    if ((j + 1) % rounds) continue;
+    // end of hi group j / rounds: D[m][n] sits in lane n + 16 (m / 4), element m % 4
+    __syncthreads();  // every wave is done with its staging buffer
+    int32_t* red = reinterpret_cast<int32_t*>(&stage[0][0]);  // [wave][16 m][16 n]
+#pragma unroll
+    for (int i = 0; i < 4; i++) red[wave * 256 + (4 * (lane / 16) + i) * 16 + lane % 16] = dsum[i];
+    dsum = qf_i32x4{0, 0, 0, 0};
+    __syncthreads();
+    if (threadIdx.x < 128) {
+      const int m = threadIdx.x / 8, c = threadIdx.x % 8;
+      int64_t lo = 0, hi = 0;
+      for (int w2 = 0; w2 < MF_WAVES; w2++) {
+        lo += red[w2 * 256 + m * 16 + c];
+        hi += red[w2 * 256 + m * 16 + 8 + c];
+      }
+      pv[m][c] = mod_signed(lo + 256 * hi);
+    }
+    __syncthreads();
+    if (threadIdx.x < L) {
+      V8 x[2];
+#pragma unroll
+      for (int p = 0; p <= RBITS; p++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) x[p].c[c] = pv[p * L + threadIdx.x][c];
+      V8 y = x[0];
+      if (RBITS) y = v8_add(v8_mul(x[0], v8_load(tp[0][0])), v8_mul(x[1], v8_load(tp[0][1])));
+      acc = v8_add(acc, v8_mul(y, v8_load(Chi[j / rounds])));
+    }
+    __syncthreads();  // the staging buffers are free again
+  }
+  if (threadIdx.x < L) out[uint64_t(blockIdx.x) * L + threadIdx.x] = v8_store(acc);
+}
+#endif
 
 // Witness rows row0 .. row0 + NR - 1 (eo.rs:119-132): w[row] = sum_lane
 // v[row S + lane] pinv[lane], from S consecutive BVA entries per row (zero at
@@ -1455,37 +1595,64 @@ int launch_mode() { return g_launch_mode; }
 // chunk's task in `owner`.
 static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
                            const std::vector<std::array<E8, 2>>& pairs, std::vector<uint32_t>& owner) {
-  std::vector<ProjTaskDev> chunks;
+  // F65521 16- and 8-lane chunks with 16-byte aligned rows and at least 2^10
+  // records per hi group run on k_tensor_project_mfma (listed first), the rest
+  // on k_tensor_project.
+  // QF_PROJECT_MFMA=0 keeps every chunk on k_tensor_project (same results).
+  static const bool use_mfma = [] {
+    const char* e = std::getenv("QF_PROJECT_MFMA");
+    return !(e && e[0] == '0');
+  }();
+  std::vector<ProjTaskDev> chunks, vchunks;
+  std::vector<uint32_t> vowner;
   for (size_t i = 0; i < tasks.size(); i++) {
     const ProjTask& t = tasks[i];
     require(pow2(t.rows) && uint64_t(1) << t.nbits == t.rows, QF_ERR_ARGUMENT, "projection task rows");
-    uint32_t k = std::min<uint32_t>(t.nbits, PROJ_TAB_LOG);
+    // 16-byte records of 16 / L rows: 2^10 records per hi group at least, 2^11 at most
+    const uint32_t rbits = L == 8 ? 1 : 0;
+    const bool mfma = use_mfma && LANES == 8 && (L == 16 || L == 8) && t.base % 16 == 0 && t.nbits >= 10 + rbits;
+    uint32_t k = std::min<uint32_t>(t.nbits, mfma ? MF_TAB_LOG + rbits : PROJ_TAB_LOG);
     uint32_t clog = std::min<uint32_t>(t.nbits, 16);
     for (uint64_t first = 0; first < t.rows; first += uint64_t(1) << clog) {
       if (t.base + first * L >= available) break;  // rows past the active data contribute zero
-      chunks.push_back({t.base, t.rows, first, t.nbits, t.pair_off, k, clog, t.scale});
-      owner.push_back(uint32_t(i));
+      (mfma ? chunks : vchunks).push_back({t.base, t.rows, first, t.nbits, t.pair_off, k, clog, t.scale});
+      (mfma ? owner : vowner).push_back(uint32_t(i));
     }
   }
+  const size_t nm = chunks.size();
+  chunks.insert(chunks.end(), vchunks.begin(), vchunks.end());
+  owner.insert(owner.end(), vowner.begin(), vowner.end());
   if (chunks.empty()) return DVec();
   E8(*d_pairs)[2] = nullptr;
   upload_pairs(c, pairs, &d_pairs);
   auto* d_tasks = static_cast<ProjTaskDev*>(dmalloc(chunks.size() * sizeof(ProjTaskDev), c.st));
   h2d(c, d_tasks, chunks.data(), chunks.size() * sizeof(ProjTaskDev));
   DVec out(chunks.size() * L);
-  for (size_t first = 0; first < chunks.size(); first += 65535) {
-    uint32_t n = uint32_t(std::min<size_t>(65535, chunks.size() - first));
+  auto* pr = (const E8(*)[2])d_pairs;
+  for (size_t first = 0; first < chunks.size();) {  // at most 65535 workgroups per launch
+    const bool mf = first < nm;
+    const size_t end = std::min(mf ? nm : chunks.size(), first + 65535);
+    const uint32_t n = uint32_t(end - first);
     auto* tk = d_tasks + first;
-    auto* pr = (const E8(*)[2])d_pairs;
     E8* o = out.p + first * L;
-    switch (L) {
-      case 2: hipLaunchKernelGGL(k_tensor_project<2>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
-      case 4: hipLaunchKernelGGL(k_tensor_project<4>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
-      case 8: hipLaunchKernelGGL(k_tensor_project<8>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
-      case 16: hipLaunchKernelGGL(k_tensor_project<16>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
-      case 32: hipLaunchKernelGGL(k_tensor_project<32>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
-      default: hipLaunchKernelGGL(k_tensor_project<64>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+    if (mf) {
+#ifndef QF_KB
+      if (L == 16)
+        hipLaunchKernelGGL(k_tensor_project_mfma<16>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
+      else
+        hipLaunchKernelGGL(k_tensor_project_mfma<8>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
+#endif
+    } else {
+      switch (L) {
+        case 2: hipLaunchKernelGGL(k_tensor_project<2>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+        case 4: hipLaunchKernelGGL(k_tensor_project<4>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+        case 8: hipLaunchKernelGGL(k_tensor_project<8>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+        case 16: hipLaunchKernelGGL(k_tensor_project<16>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+        case 32: hipLaunchKernelGGL(k_tensor_project<32>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+        default: hipLaunchKernelGGL(k_tensor_project<64>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+      }
     }
+    first = end;
   }
   QF_HIP(hipGetLastError());
   dfree(d_pairs, c.st);
@@ -1559,20 +1726,11 @@ void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, 
   QF_HIP(hipGetLastError());
 }
 
+// One launch over every chunk of `tasks`. With ADD each chunk reads and
+// rewrites its rows, so the tasks of one launch must not share rows.
 template <bool ADD>
-static void fill_tasks(DevCtx& c, E8* out, const std::vector<FillTask>& tasks,
-                       const std::vector<std::array<E8, 2>>& pairs) {
-  if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
-    for (const FillTask& t : tasks) {
-      FillTask one = t;
-      one.pair_off = 0;
-      std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
-      set_launch_mode(LAUNCH_BATCHED);
-      fill_tasks<ADD>(c, out, {one}, own);
-      set_launch_mode(LAUNCH_PER_MATRIX);
-    }
-    return;
-  }
+static void fill_launch(DevCtx& c, E8* out, const std::vector<FillTask>& tasks,
+                        const std::vector<std::array<E8, 2>>& pairs) {
   std::vector<FillTaskDev> chunks;
   for (const FillTask& t : tasks) {
     uint32_t k = std::min<uint32_t>(t.nbits, 12);
@@ -1593,6 +1751,47 @@ static void fill_tasks(DevCtx& c, E8* out, const std::vector<FillTask>& tasks,
   QF_HIP(hipGetLastError());
   dfree(d_pairs, c.st);
   dfree(d_tasks, c.st);
+}
+
+template <bool ADD>
+static void fill_tasks(DevCtx& c, E8* out, const std::vector<FillTask>& tasks,
+                       const std::vector<std::array<E8, 2>>& pairs) {
+  if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
+    for (const FillTask& t : tasks) {
+      FillTask one = t;
+      one.pair_off = 0;
+      std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
+      set_launch_mode(LAUNCH_BATCHED);
+      fill_tasks<ADD>(c, out, {one}, own);
+      set_launch_mode(LAUNCH_PER_MATRIX);
+    }
+    return;
+  }
+  if (!ADD) {  // written tasks have disjoint rows (tensor_fill checks)
+    fill_launch<ADD>(c, out, tasks, pairs);
+    return;
+  }
+  // Added tasks can share rows: the column-padding rectangles of one bucket
+  // each span the bucket's rows, with zero weights outside their own columns.
+  // Tasks go to launches whose row ranges are pairwise disjoint (greedy
+  // interval colouring by start), run one after another on the stream.
+  std::vector<size_t> order(tasks.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return tasks[a].start < tasks[b].start; });
+  std::vector<std::vector<FillTask>> groups;
+  std::vector<uint64_t> group_end;
+  for (size_t i : order) {
+    const FillTask& t = tasks[i];
+    size_t g = 0;
+    while (g < groups.size() && group_end[g] > t.start) g++;
+    if (g == groups.size()) {
+      groups.emplace_back();
+      group_end.push_back(0);
+    }
+    groups[g].push_back(t);
+    group_end[g] = t.start + (uint64_t(1) << t.nbits);
+  }
+  for (auto& g : groups) fill_launch<ADD>(c, out, g, pairs);
 }
 
 void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, const std::vector<std::array<E8, 2>>& pairs) {

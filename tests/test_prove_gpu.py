@@ -62,6 +62,8 @@ def test_demo_proof_matches_reference_digest(oracle, recompute):
     (12, [(64, 64)], 6, 1),              # packed_log 1 (two lanes)
     (14, [(300, 200)], 7, 6),            # packed_log 6
     (10, [(150, 150)], 8, 4),            # 22 blocks: block sums folded every 16 blocks
+    (20, [(1000, 3000), (3, 5)], 9, 4),  # 16-lane terms of 2^10..2^17 rows on the MFMA projection, ragged end
+    (18, [(500, 1000)], 10, 4),          # column-padding terms sharing query rows (added in separate launches)
 ])
 def test_random_proofs_match_oracle(oracle, chunk_log, shapes, seed, packed_log):
     p = params(shapes, chunk_log, packed_log)
