@@ -654,9 +654,10 @@ TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L,
   } else {
     a.in = rebase(a0.in);
   }
+  // Every candidate keeps the XCD-paired walk where it applies: without it a
+  // pass over 64-byte tile rows fetches every 128-byte line twice, for no
+  // gain in time (the passes it applies to are bound by their VALU and LDS work).
   std::vector<TileShape> cands = {{false, 2, true}, {false, 1, true}, {true, 1, true}, {true, 2, true}};
-  const uint64_t wgs2 = std::min<uint64_t>(a.tiles, 2 * uint64_t(device_cus()));
-  if (xcd_applicable(a, wgs2)) cands.push_back({false, 2, false});
   hipEvent_t e0, e1;
   QF_HIP(hipEventCreate(&e0));
   QF_HIP(hipEventCreate(&e1));
