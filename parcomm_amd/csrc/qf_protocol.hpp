@@ -165,11 +165,6 @@ inline E8 weight_at(const std::vector<E8>& p, uint64_t pos) {
 }
 // Device tensor weights / equality table of an MSB-first point (tensor of two halves).
 
-inline void eq_tables(const std::vector<E8>& p, std::vector<E8>& hi, std::vector<E8>& lo) {
-  size_t m = p.size(), a = m / 2;
-  hi = eq_host(p.data(), m - a);
-  lo = eq_host(p.data() + (m - a), a);
-}
 inline void put_part(Bytes& w, const Bytes& part) { put_u64(w, part.size()); w.insert(w.end(), part.begin(), part.end()); }
 inline void put_tag(Bytes& w, const char* tag) { w.insert(w.end(), tag, tag + std::strlen(tag)); }
 inline void put_digest(Bytes& w, const Digest& d) { w.insert(w.end(), d.begin(), d.end()); }

@@ -61,6 +61,19 @@ void tw_device(DevCtx& c, E8* out, const std::vector<E8>& p) {
   size_t m = p.size(), a = m / 2;
   dv::tensor(c, out, tw_host(p.data(), m - a), tw_host(p.data() + (m - a), a));
 }
+// eq(p), MSB first (the first variables index the high bits), as two device
+// half tables: hi over the first m - m/2 variables and lo over the rest, each
+// the outer product of two host quarter tables of at most m/4 + 1 variables.
+void eq_tables_device(DevCtx& c, const std::vector<E8>& p, DVec& hi, DVec& lo) {
+  const size_t m = p.size(), a = m / 2;
+  auto half = [&](const E8* q, size_t n, DVec& out) {
+    const size_t b = n / 2;
+    out = DVec(uint64_t(1) << n);
+    dv::tensor(c, out.p, eq_host(q, n - b), eq_host(q + (n - b), b));
+  };
+  half(p.data(), m - a, hi);
+  half(p.data() + (m - a), a, lo);
+}
 Digest download_root(DevCtx& c, const uint32_t* tree, uint64_t R) {
   Digest d;
   QF_HIP(hipMemcpyAsync(d.data(), tree + 8 * (2 * R - 2), 32, hipMemcpyDeviceToHost, c.st));
@@ -986,8 +999,8 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
     for (int img = 0; img < LANES; img++) s += coefs[img] * e8_frob(e8_basis(lane), (LANES - img) % LANES);
     images[lane] = s;
   }
-  std::vector<E8> ehi, elo;
-  eq_tables(outer, ehi, elo);
+  DVec ehi, elo;
+  eq_tables_device(c, outer, ehi, elo);
   DVec weights(plen), values(plen);
   weight_images(c, weights.p, ehi, elo, images);
   tm.projection_ms = ms_since(t0);
@@ -1080,8 +1093,8 @@ Bytes open_single(DevCtx& c, const ParamsState& p, const CommittedState& cs, con
     for (int i = 0; i < LANES; i++) s += powers[i] * e8_frob(e8_basis(j), (LANES - i) % LANES);
     images[j] = s;
   }
-  std::vector<E8> ehi, elo;
-  eq_tables(outer, ehi, elo);
+  DVec ehi, elo;
+  eq_tables_device(c, outer, ehi, elo);
   DVec weights(plen), values(plen);
   weight_images(c, weights.p, ehi, elo, images);
   block_fold(c, cs.d_bva, p.layout.active, p.block_len, 1, {e8_one()}, values.p, plen);

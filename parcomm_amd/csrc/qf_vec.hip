@@ -1859,15 +1859,12 @@ void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_le
   QF_HIP(hipGetLastError());
 }
 
-void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo,
-                   const std::array<E8, LANES>& images) {
-  E8* t = scratch_tab();
-  put_table(c, t, hi.data(), hi.size() * 16);
-  put_table(c, t + hi.size(), lo.data(), lo.size() * 16);
+void weight_images(DevCtx& c, E8* out, const DVec& hi, const DVec& lo, const std::array<E8, LANES>& images) {
+  require(pow2(hi.n) && pow2(lo.n), QF_ERR_ARGUMENT, "weight tables");
   Images im;
   for (int j = 0; j < LANES; j++) im.m[j] = images[j];
-  uint64_t n = hi.size() * lo.size();
-  hipLaunchKernelGGL(k_weight_images, grid_for(n), dim3(WG), 0, c.st, out, n, t, t + hi.size(), ilog2u(lo.size()), im);
+  uint64_t n = hi.n * lo.n;
+  hipLaunchKernelGGL(k_weight_images, grid_for(n), dim3(WG), 0, c.st, out, n, hi.p, lo.p, ilog2u(lo.n), im);
   QF_HIP(hipGetLastError());
 }
 

@@ -133,6 +133,6 @@ std::array<E8, 3> eo_witness_first(DevCtx& c, const Val* bva, uint64_t available
 void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
                 const std::vector<E8>& fw, E8* values, uint64_t n);
 // weights[i] = sum_j eq(i).c[j] * images[j], eq = hi (x) lo tables
-void weight_images(DevCtx& c, E8* out, const std::vector<E8>& hi, const std::vector<E8>& lo, const std::array<E8, LANES>& images);
+void weight_images(DevCtx& c, E8* out, const DVec& hi, const DVec& lo, const std::array<E8, LANES>& images);
 
 }  // namespace qf
