@@ -145,6 +145,8 @@ inline bool e8_from_uniform(const uint8_t* b, E8& out) {
 #endif
 }
 inline void put_u64(Bytes& o, uint64_t v) { for (int i = 0; i < 8; i++) o.push_back(uint8_t(v >> (8 * i))); }
+// Overwrites the little-endian u64 at o[at, at + 8) (a length written as a placeholder).
+inline void patch_u64(Bytes& o, size_t at, uint64_t v) { for (int i = 0; i < 8; i++) o[at + i] = uint8_t(v >> (8 * i)); }
 inline void put_e8(Bytes& o, const E8& v) { const uint8_t* p = reinterpret_cast<const uint8_t*>(&v); o.insert(o.end(), p, p + 16); }
 inline int ilog2u(uint64_t x) { int r = -1; while (x) { x >>= 1; r++; } return r; }
 inline bool pow2(uint64_t x) { return x && !(x & (x - 1)); }

@@ -931,8 +931,10 @@ DCode block_code(DevCtx& c, const ParamsState& p, const CommittedState& cs, uint
   return d;
 }
 
-Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const std::vector<E8>& point,
-                 const Bytes& context, E8& claimed, ProveTimings& tm) {
+// Appends the batched opening's wire to w (the proof being built), so the
+// binding rows, most of a proof's bytes, are written once, in place.
+void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const std::vector<E8>& point,
+                const Bytes& context, E8& claimed, ProveTimings& tm, Bytes& w) {
   auto t0 = Clock::now();
   uint64_t blocks = cs.roots.size();
   size_t split = ilog2u(blocks), n_outer = point.size() - LOG_LANES - split;
@@ -1030,20 +1032,38 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
   uint64_t count = num_queries(top, p.pp.security_bits, R);
   bool all = count == R;
   std::vector<uint64_t> sel = sample_rows(count, R, seed);
-  Bytes binding;
-  put_tag(binding, "QAFOLDB1");
-  put_u64(binding, blocks);
+  // the wire up to the binding, then the binding in place (its length patched after)
+  const uint64_t rec = all ? 32 * R * sizeof(E8) : count * (16 + 32 * sizeof(E8) + 32 * uint64_t(ilog2u(R)));
+  w.reserve(w.size() + (blocks + 1) * (rec + 32) + (4u << 20));
+  put_tag(w, "QAPKBAT1");
+  put_e8(w, claimed);
+  for (auto& v : ev) put_e8(w, v);
+  for (auto& v : pos) put_e8(w, v);
+  for (auto& v : neg) put_e8(w, v);
+  put_digest(w, folded.root);
+  for (auto& r : rounds) { put_e8(w, r[0]); put_e8(w, r[1]); }
+  put_e8(w, fval);
+  const size_t bind_len_at = w.size();
+  put_u64(w, 0);
+  const size_t bind_at = w.size();
+  put_tag(w, "QAFOLDB1");
+  put_u64(w, blocks);
   DCode workspace;
-  Bytes payload;
+  size_t prev_at = 0, prev_len = 0;
   for (uint64_t b = 0; b < blocks; b++) {
     if (b == 0 || cs.roots[b] != cs.roots[b - 1]) {
       DCode bc = block_code(c, p, cs, b, workspace);
-      payload.clear();
-      write_rows(c, payload, bc, all, sel);
+      prev_at = w.size();
+      write_rows(c, w, bc, all, sel);
+      prev_len = w.size() - prev_at;
+    } else {  // the same codeword as the previous block: its rows again
+      const Bytes again(w.begin() + prev_at, w.begin() + prev_at + prev_len);
+      prev_at = w.size();
+      w.insert(w.end(), again.begin(), again.end());
     }
-    binding.insert(binding.end(), payload.begin(), payload.end());
   }
-  write_rows(c, binding, folded, all, sel);
+  write_rows(c, w, folded, all, sel);
+  patch_u64(w, bind_len_at, w.size() - bind_at);
   tm.binding_ms = ms_since(t0);
   t0 = Clock::now();
   Bytes opening;
@@ -1054,19 +1074,8 @@ Bytes open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, cons
   t0 = Clock::now();
   Bytes decide = prove_decide(c, p, pending);
   tm.decide_ms = ms_since(t0);
-  Bytes w;
-  put_tag(w, "QAPKBAT1");
-  put_e8(w, claimed);
-  for (auto& v : ev) put_e8(w, v);
-  for (auto& v : pos) put_e8(w, v);
-  for (auto& v : neg) put_e8(w, v);
-  put_digest(w, folded.root);
-  for (auto& r : rounds) { put_e8(w, r[0]); put_e8(w, r[1]); }
-  put_e8(w, fval);
-  put_part(w, binding);
   put_part(w, opening);
   put_part(w, decide);
-  return w;
 }
 
 // packed.rs:294-393 (one block)
@@ -1247,10 +1256,6 @@ Bytes prove(const ParamsState& p, const CommittedState& cs, const std::vector<Cl
   context.insert(context.end(), cx, cx + std::strlen(cx));
   context.insert(context.end(), tr.state.begin(), tr.state.end());
   put_u64(context, 0);
-  E8 actual;
-  Bytes wire = cs.roots.size() > 1 ? open_batch(c, p, cs, eo.point, context, actual, tm)
-                                   : open_single(c, p, cs, eo.point, context, actual, tm);
-  require(actual * eo.scale == eo.value, QF_ERR_INVALID_COMMITMENT, "QAFold/EO terminal mismatch");
   Bytes out;  // bincode fixint (protocol.rs:57-66, 89-95)
   put_u64(out, eo.pos.size());
   for (auto& v : eo.pos) put_e8(out, v);
@@ -1261,9 +1266,23 @@ Bytes prove(const ParamsState& p, const CommittedState& cs, const std::vector<Cl
   put_e8(out, eo.value);
   put_e8(out, eo.query);
   put_u64(out, 1);
-  put_e8(out, actual);
-  put_u64(out, wire.size());
-  out.insert(out.end(), wire.begin(), wire.end());
+  // the opening's value and the wire's length follow; both are patched once the
+  // opening, appended in place after them, is done
+  const size_t actual_at = out.size();
+  put_e8(out, E8{});
+  const size_t wire_len_at = out.size();
+  put_u64(out, 0);
+  const size_t wire_at = out.size();
+  E8 actual;
+  if (cs.roots.size() > 1) {
+    open_batch(c, p, cs, eo.point, context, actual, tm, out);
+  } else {
+    const Bytes wire = open_single(c, p, cs, eo.point, context, actual, tm);
+    out.insert(out.end(), wire.begin(), wire.end());
+  }
+  require(actual * eo.scale == eo.value, QF_ERR_INVALID_COMMITMENT, "QAFold/EO terminal mismatch");
+  std::memcpy(out.data() + actual_at, &actual, sizeof(E8));
+  patch_u64(out, wire_len_at, out.size() - wire_at);
   require(out.size() <= (256ull << 20), QF_ERR_ENCODING, "oversize ParComm proof");
   tm.total_ms = ms_since(start);
   return out;
