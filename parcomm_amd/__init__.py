@@ -356,13 +356,14 @@ class Parameters:
         return [((s.rows - 1).bit_length(), (s.cols - 1).bit_length()) for s in self.shapes]
 
     def claims_to_bytes(self, claims: Sequence[Claim]) -> bytes:
-        out = bytearray()
+        parts = []
         for c in claims:
-            for v in list(c.row) + list(c.col) + [c.value]:
-                if len(v) != 16:
-                    raise ValueError("scalars are 16 bytes")
-                out += v
-        return bytes(out)
+            parts.extend(c.row)
+            parts.extend(c.col)
+            parts.append(c.value)
+        if any(len(v) != 16 for v in parts):
+            raise ValueError("scalars are 16 bytes")
+        return b"".join(bytes(v) for v in parts)
 
     def claims_from_bytes(self, raw: bytes) -> List[Claim]:
         out, o = [], 0

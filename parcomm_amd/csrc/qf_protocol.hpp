@@ -13,9 +13,21 @@ namespace qf {
 
 // ================================================================ statement JSON (serde_json compact)
 inline void js_bytes(std::string& s, const uint8_t* b, size_t n) {
-  s += '[';
-  for (size_t i = 0; i < n; i++) { if (i) s += ','; s += std::to_string(unsigned(b[i])); }
-  s += ']';
+  char buf[4 * 64 + 2];  // "[", then up to 3 digits and a comma per byte, "]"
+  for (size_t at = 0; at < n; at += 64) {
+    size_t k = 0;
+    if (at == 0) buf[k++] = '[';
+    for (size_t i = at; i < n && i < at + 64; i++) {
+      if (i) buf[k++] = ',';
+      const unsigned v = b[i];
+      if (v >= 100) buf[k++] = char('0' + v / 100);
+      if (v >= 10) buf[k++] = char('0' + v / 10 % 10);
+      buf[k++] = char('0' + v % 10);
+    }
+    if (at + 64 >= n) buf[k++] = ']';
+    s.append(buf, k);
+  }
+  if (n == 0) s += "[]";
 }
 inline void js_e8(std::string& s, const E8& v) { js_bytes(s, reinterpret_cast<const uint8_t*>(&v), 16); }
 inline std::string js_scalars(const std::vector<E8>& v) {
