@@ -691,17 +691,21 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   o.pos.assign(s - 1, E8{});
   o.neg.assign(s - 1, E8{});
   int ls = ilog2u(s);
+  // the lane weights of every term of at least s entries, used by the projection and the query fold
+  std::vector<std::vector<E8>> term_low(q.terms.size());
+  for (size_t i = 0; i < q.terms.size(); i++)
+    if (q.terms[i].len() >= s) term_low[i] = tbv_w(q.terms[i].f.data(), ls);
   if (launch_mode() == LAUNCH_BATCHED) {
     // one projection launch over every term; the lane correlations on the device
     std::vector<ProjTask> tasks;
     std::vector<std::array<E8, 2>> pairs;
     std::vector<E8> lows;
-    for (const Term& t : q.terms) {
+    for (size_t i = 0; i < q.terms.size(); i++) {
+      const Term& t = q.terms[i];
       if (t.len() < s || t.offset >= l.active || e8_is_zero(t.scale)) continue;
       tasks.push_back({t.offset, t.len() / s, uint32_t(t.f.size() - ls), uint32_t(pairs.size()), t.scale});
       for (size_t b = ls; b < t.f.size(); b++) pairs.push_back(t.f[b]);
-      auto low = tbv_w(t.f.data(), ls);
-      lows.insert(lows.end(), low.begin(), low.end());
+      lows.insert(lows.end(), term_low[i].begin(), term_low[i].end());
     }
     auto pn = tensor_project_cross(c, bva, l.active, int(s), tasks, pairs, lows);
     o.pos = pn.first;
@@ -710,8 +714,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     std::vector<size_t> which;
     auto proj = project_terms(c, q, bva, l.active, s, which);
     for (size_t k = 0; k < which.size(); k++) {
-      const Term& t = q.terms[which[k]];
-      auto low = tbv_w(t.f.data(), ls);
+      const auto& low = term_low[which[k]];
       for (uint64_t d = 1; d < s; d++)
         for (uint64_t lane = 0; lane + d < s; lane++) {
           o.pos[d - 1] += proj[k][lane] * low[lane + d];
@@ -753,7 +756,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     for (size_t i = a; i < b; i++) {
       const Term& t = q.terms[i];
       if (t.len() >= s) {
-        auto low = tbv_w(t.f.data(), ls);
+        const auto& low = term_low[i];
         E8 acc{};
         for (uint64_t j = 0; j < s; j++) acc += low[j] * pows[j];
         FillTask ft{t.offset / s, uint32_t(t.f.size() - ls), uint32_t(pairs.size()), t.scale * acc};
