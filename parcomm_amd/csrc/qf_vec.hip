@@ -562,6 +562,18 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
   }
 }
 
+// MFMA projection: a wave stages MF_ROWS 16-byte BVA records per round in LDS;
+// ds_read_tr8_b64 on 8 consecutive 16-byte rows gives lane i of each 16-lane
+// group byte i of every row.
+constexpr int MF_ROWS = 256;            // records per wave and round: 8 blocks of 32, 4 KiB
+constexpr int MF_WAVES = WG / 64;
+typedef int32_t qf_i32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t qf_i32x2 __attribute__((ext_vector_type(2)));
+__device__ inline long tr8_b64(const uint8_t* p) {
+  const qf_i32x2 v = __builtin_amdgcn_ds_read_tr8_b64_v2i32(
+      (__attribute__((address_space(3))) qf_i32x2*)(const_cast<uint8_t*>(p)));
+  return (long(uint32_t(v.y)) << 32) | long(uint32_t(v.x));
+}
 #ifndef QF_KB
 // Projection chunks of 16- and 8-lane rows (the EO cross-term and the batch
 // lane projections) as an exact i8 MFMA contraction of the BVA against the
@@ -584,15 +596,6 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
 //   multiplied by the group's scale and accumulated by one thread per lane.
 // The chunk's rows start 16-byte aligned and a hi group holds at least
 // MF_ROWS records per wave (the host routes other chunks to k_tensor_project).
-constexpr int MF_ROWS = 256;            // records per wave and round: 8 blocks of 32, 4 KiB
-constexpr int MF_WAVES = WG / 64;
-typedef int32_t qf_i32x4 __attribute__((ext_vector_type(4)));
-typedef int32_t qf_i32x2 __attribute__((ext_vector_type(2)));
-__device__ inline long tr8_b64(const uint8_t* p) {
-  const qf_i32x2 v = __builtin_amdgcn_ds_read_tr8_b64_v2i32(
-      (__attribute__((address_space(3))) qf_i32x2*)(const_cast<uint8_t*>(p)));
-  return (long(uint32_t(v.y)) << 32) | long(uint32_t(v.x));
-}
 template <int L>
 __global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restrict__ bva, uint64_t available,
                                                             const ProjTaskDev* __restrict__ tasks,
@@ -691,6 +694,159 @@ __global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restric
       for (int p = 0; p <= RBITS; p++)
 #pragma unroll
         for (int c = 0; c < 8; c++) x[p].c[c] = pv[p * L + threadIdx.x][c];
+      V8 y = x[0];
+      if (RBITS) y = v8_add(v8_mul(x[0], v8_load(tp[0][0])), v8_mul(x[1], v8_load(tp[0][1])));
+      acc = v8_add(acc, v8_mul(y, v8_load(Chi[j / rounds])));
+    }
+    __syncthreads();  // the staging buffers are free again
+  }
+  if (threadIdx.x < L) out[uint64_t(blockIdx.x) * L + threadIdx.x] = v8_store(acc);
+}
+#else
+// KoalaBear form of the MFMA projection, over 16-byte records of 8 i16
+// entries: one half of a 16-lane row (the halves accumulate as two streams
+// against the same weight), one 8-lane row, or two consecutive 4-lane rows
+// joined with pairs[0] as in the F65521 form.
+// - Entries: an i16 entry v is 256 vh + (vl - 128) + 128 with its high byte vh
+//   and its offset low byte vl - 128 both in [-128, 127]; staging flips bit 7
+//   of each low byte, which gives that byte, and the 128 sum_r t_r this leaves
+//   out of a hi group is added back from ssum, the table's signed coefficient
+//   sums (the same for every hi group; zero entries past the active end cancel
+//   with it).
+// - Digits: a canonical coefficient t (< 2^31) is written as t' = t or t - p in
+//   (-2^30, 2^30) and split into four balanced digits d_i in [-128, 127] with
+//   t' = sum_i 256^i d_i; record r of the digit table holds digit i of
+//   coefficient c in byte 4 i + c.
+// - D[record byte][digit byte] sums products of at most 2^14 over at most 2^9
+//   records per wave and stream, 2^25 per workgroup: exact in i32; the
+//   recombination sum_i 256^i (256 D_high + D_low) + 128 ssum stays below 2^58.
+template <int L>
+__global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restrict__ bva, uint64_t available,
+                                                            const ProjTaskDev* __restrict__ tasks,
+                                                            const E8 (*__restrict__ pairs)[2], E8* __restrict__ out) {
+  static_assert(L == 16 || L == 8 || L == 4, "records of half a row, one row or two rows");
+  constexpr int RBITS = L == 4 ? 1 : 0;            // row bits inside a record
+  constexpr int NS = L == 16 ? 2 : 1;              // records (streams) per row
+  constexpr int RT = MF_ROWS / NS;                 // table entries per wave and round
+  __shared__ E8 T[1 << MF_TAB_LOG];                // table entry weights, then their digit bytes
+  __shared__ uint4 stage[MF_WAVES][MF_ROWS];       // a round of BVA records per wave; the combine reuses it
+  __shared__ uint32_t pv[16][4];                   // a group's record-entry sums mod p
+  __shared__ E8 Chi[32];                           // scale x high-bit weights of each hi group
+  __shared__ unsigned long long ssum[4];           // sum over the table of t', two's complement
+  const ProjTaskDev t = tasks[blockIdx.x];
+  const E8(*tp)[2] = pairs + t.pair_off;
+  if (threadIdx.x < 4) ssum[threadIdx.x] = 0;
+  build_table(T, tp + RBITS, int(t.k) - RBITS);   // its barriers order the clear above
+  const uint32_t lo_rows = 1u << t.k, te = lo_rows >> RBITS;
+  const uint32_t hi_count = 1u << (t.chunk_rows_log - t.k);  // <= 32
+  if (threadIdx.x < hi_count) {
+    const uint64_t row_hi = t.chunk_first + uint64_t(threadIdx.x) * lo_rows;
+    V8 c = v8_load(t.scale);
+    for (uint32_t b = t.k; b < t.nbits; b++) c = v8_mul(c, v8_load(tp[b][(row_hi >> b) & 1]));
+    Chi[threadIdx.x] = v8_store(c);
+  }
+  int64_t ls[4] = {0, 0, 0, 0};
+  for (uint32_t r = threadIdx.x; r < te; r += WG) {  // digits, in place
+    const E8 e = T[r];
+    uint32_t d[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      int32_t x = e.c[c] <= P / 2 ? int32_t(e.c[c]) : int32_t(e.c[c] - P);
+      ls[c] += x;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int32_t dd = ((x + 128) & 255) - 128;
+        d[i] |= uint32_t(uint8_t(dd)) << (8 * c);
+        x = (x - dd) >> 8;
+      }
+    }
+    reinterpret_cast<uint4*>(T)[r] = make_uint4(d[0], d[1], d[2], d[3]);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) atomicAdd(&ssum[c], (unsigned long long)ls[c]);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint8_t* tb = reinterpret_cast<const uint8_t*>(T);
+  uint8_t* st = reinterpret_cast<uint8_t*>(stage[wave]);
+  const uint32_t wte = te / MF_WAVES, rounds = wte / RT, total = hi_count * rounds;
+  // round j: hi group j / rounds, this wave's table entries [r0, r0 + RT) of it
+  auto fetch = [&](uint32_t j, uint4 (&w)[4]) {
+    const uint32_t r0 = wave * wte + (j % rounds) * RT;
+    const uint64_t e0 = t.base + (t.chunk_first + uint64_t(j / rounds) * lo_rows) * L + uint64_t(r0) * (8 * NS);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint64_t e = e0 + 8 * (64 * u + lane);
+      if (e + 8 <= available) {
+        w[u] = *reinterpret_cast<const uint4*>(bva + e);
+      } else {  // entries at or past the active end read as zero
+        uint32_t x[4] = {0, 0, 0, 0};
+        for (int q = 0; q < 8; q++)
+          if (e + q < available) x[q / 2] |= uint32_t(uint16_t(bva[e + q])) << (16 * (q % 2));
+        w[u] = make_uint4(x[0], x[1], x[2], x[3]);
+      }
+    }
+  };
+  V8 acc = v8_zero();  // threads 0..L-1: lane threadIdx.x of the chunk
+  qf_i32x4 dsum[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) dsum[s] = qf_i32x4{0, 0, 0, 0};
+  uint4 cur[4], nxt[4];
+  fetch(0, cur);
+  for (uint32_t j = 0; j < total; j++) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the previous round's reads of st are done
+#pragma unroll
+    for (int u = 0; u < 4; u++) {  // record q = 64 u + lane; with two streams the even ones go first
+      const uint32_t q = 64 * u + lane;
+      const uint32_t at = NS == 2 ? (q & 1) * (16 * RT) + 16 * (q >> 1) : 16 * q;
+      const uint32_t f = 0x00800080u;  // bit 7 of each low byte
+      *reinterpret_cast<uint4*>(st + at) = make_uint4(cur[u].x ^ f, cur[u].y ^ f, cur[u].z ^ f, cur[u].w ^ f);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (j + 1 < total) fetch(j + 1, nxt);
+    const uint32_t r0 = wave * wte + (j % rounds) * RT;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+      for (int b = 0; b < RT / 32; b++) {
+        const long a = tr8_b64(st + s * (16 * RT) + 512 * b + 8 * lane);
+        const long w = tr8_b64(tb + 16 * (r0 + 32 * b) + 8 * lane);
+        dsum[s] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a, w, dsum[s], 0, 0, 0);
+      }
+#pragma unroll
+    for (int u = 0; u < 4; u++) cur[u] = nxt[u];
+    if ((j + 1) % rounds) continue;
+    // end of hi group j / rounds: D[m][n] sits in lane n + 16 (m / 4), element m % 4
+    __syncthreads();  // every wave is done with its staging buffer
+    int32_t* red = reinterpret_cast<int32_t*>(&stage[0][0]);  // [stream][wave][16 m][16 n]
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        red[(s * MF_WAVES + wave) * 256 + (4 * (lane / 16) + i) * 16 + lane % 16] = dsum[s][i];
+      dsum[s] = qf_i32x4{0, 0, 0, 0};
+    }
+    __syncthreads();
+    if (threadIdx.x < 32 * NS) {  // stream s, record entry e, coefficient c
+      const int s = threadIdx.x / 32, e = (threadIdx.x / 4) % 8, c = threadIdx.x % 4;
+      const int32_t* rs = red + s * MF_WAVES * 256;
+      int64_t v = 0;
+      for (int i = 3; i >= 0; i--) {
+        int64_t lo = 0, hi = 0;
+        for (int w2 = 0; w2 < MF_WAVES; w2++) {
+          lo += rs[w2 * 256 + (2 * e) * 16 + 4 * i + c];
+          hi += rs[w2 * 256 + (2 * e + 1) * 16 + 4 * i + c];
+        }
+        v = 256 * v + 256 * hi + lo;
+      }
+      pv[s * 8 + e][c] = mod_signed(v + 128 * int64_t(ssum[c]));
+    }
+    __syncthreads();
+    if (threadIdx.x < L) {
+      V8 x[2];
+#pragma unroll
+      for (int p = 0; p <= RBITS; p++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) x[p].c[c] = pv[p * L + threadIdx.x][c];
       V8 y = x[0];
       if (RBITS) y = v8_add(v8_mul(x[0], v8_load(tp[0][0])), v8_mul(x[1], v8_load(tp[0][1])));
       acc = v8_add(acc, v8_mul(y, v8_load(Chi[j / rounds])));
@@ -1595,9 +1751,10 @@ int launch_mode() { return g_launch_mode; }
 // chunk's task in `owner`.
 static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
                            const std::vector<std::array<E8, 2>>& pairs, std::vector<uint32_t>& owner) {
-  // F65521 16- and 8-lane chunks with 16-byte aligned rows and at least 2^10
-  // records per hi group run on k_tensor_project_mfma (listed first), the rest
-  // on k_tensor_project.
+  // Chunks of 16-byte aligned rows that fill whole 16-byte records (F65521:
+  // 16 and 8 lanes; KoalaBear: 16, 8 and 4) and give every wave at least one
+  // round of MF_ROWS records per hi group run on k_tensor_project_mfma (listed
+  // first), the rest on k_tensor_project.
   // QF_PROJECT_MFMA=0 keeps every chunk on k_tensor_project (same results).
   static const bool use_mfma = [] {
     const char* e = std::getenv("QF_PROJECT_MFMA");
@@ -1608,9 +1765,13 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
   for (size_t i = 0; i < tasks.size(); i++) {
     const ProjTask& t = tasks[i];
     require(pow2(t.rows) && uint64_t(1) << t.nbits == t.rows, QF_ERR_ARGUMENT, "projection task rows");
-    // 16-byte records of 16 / L rows: 2^10 records per hi group at least, 2^11 at most
-    const uint32_t rbits = L == 8 ? 1 : 0;
-    const bool mfma = use_mfma && LANES == 8 && (L == 16 || L == 8) && t.base % 16 == 0 && t.nbits >= 10 + rbits;
+    // records of REC entries: 2^rbits rows, or 1 / ns of a row; a hi group
+    // holds 2^(k - rbits) <= 2^MF_TAB_LOG table entries, MF_WAVES MF_ROWS / ns
+    // of them at least
+    constexpr int REC = 16 / int(sizeof(Val));
+    const bool shape = LANES == 8 ? (L == 16 || L == 8) : (L == 16 || L == 8 || L == 4);
+    const uint32_t rbits = L < REC ? 1 : 0, nslog = L > REC ? 1 : 0;
+    const bool mfma = use_mfma && shape && t.base % REC == 0 && t.nbits + nslog >= 10 + rbits;
     uint32_t k = std::min<uint32_t>(t.nbits, mfma ? MF_TAB_LOG + rbits : PROJ_TAB_LOG);
     uint32_t clog = std::min<uint32_t>(t.nbits, 16);
     for (uint64_t first = 0; first < t.rows; first += uint64_t(1) << clog) {
@@ -1636,11 +1797,13 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
     auto* tk = d_tasks + first;
     E8* o = out.p + first * L;
     if (mf) {
-#ifndef QF_KB
       if (L == 16)
         hipLaunchKernelGGL(k_tensor_project_mfma<16>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
-      else
+      else if (L == 8)
         hipLaunchKernelGGL(k_tensor_project_mfma<8>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
+#ifdef QF_KB
+      else
+        hipLaunchKernelGGL(k_tensor_project_mfma<4>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
 #endif
     } else {
       switch (L) {

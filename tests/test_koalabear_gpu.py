@@ -87,6 +87,7 @@ def test_kb_demo_proof_matches_reference_digest(oracle_kb, recompute):
     (12, [(64, 64)], 6, 1, "bf16"),             # packed_log 1
     (14, [(300, 200)], 7, 6, "int16"),          # packed_log 6
     (18, [(500, 1000)], 8, 4, "bf16"),          # column-padding terms sharing query rows
+    (18, [(700, 1500), (3, 5)], 9, 4, "int16"),  # full-range i16 on the MFMA projection, ragged end
 ])
 def test_kb_random_proofs_match_oracle(oracle_kb, chunk_log, shapes, seed, packed_log, gen):
     p = params(shapes, chunk_log, packed_log)
