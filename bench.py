@@ -38,26 +38,24 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_mw_8b_traffic.md")
 
 
-def measured_traffic(field_tag):
-    """HBM bytes per launch of the dominant pass, from the committed PMC
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy kernel), or
-    None when that summary has no line for this field."""
+def measured_traffic(field_tag, kernel="mw"):
+    """HBM bytes per launch of a commit kernel ("mw": the Mobius + WHT pass of
+    the last row group; "leaf": the leaf hash), from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy kernel), or None when
+    that summary has no line for this field and kernel."""
     if not os.path.exists(TRAFFIC_PROFILE):
         return None
+    keys = [f"traffic_bytes_per_launch[{field_tag}:{kernel}]:"]
+    if kernel == "mw":
+        keys.append(f"traffic_bytes_per_launch[{field_tag}]:")
     for line in open(TRAFFIC_PROFILE):
-        if line.startswith(f"traffic_bytes_per_launch[{field_tag}]:"):
-            return int(line.split(":", 1)[1].strip())
+        for k in keys:
+            if line.startswith(k):
+                return int(line.split(":", 2)[-1].strip())
     return None
 
 
-def llama8b_shapes():
-    layer = [(4096, 4096), (1024, 4096), (1024, 4096), (4096, 4096), (14336, 4096), (14336, 4096), (4096, 14336)]
-    return [(128256, 4096), (128256, 4096)] + layer * 32
-
-
-def llama70b_shapes():
-    layer = [(8192, 8192), (1024, 8192), (1024, 8192), (8192, 8192), (28672, 8192), (28672, 8192), (8192, 28672)]
-    return [(128256, 8192), (128256, 8192)] + layer * 80
+from parcomm_amd.workloads import llama70b_shapes, llama8b_shapes  # noqa: E402
 
 
 MODELS = {  # shapes, packed_log, commit form (the 70B codewords, 512 GiB, do not fit HBM: roots only)
@@ -68,30 +66,43 @@ MODELS = {  # shapes, packed_log, commit form (the 70B codewords, 512 GiB, do no
 
 def cpu_baseline(seed=5, kb=False):
     """The CPU oracle (a parity-checked restatement, not the reference binary)
-    running the whole CLI flow (commit, benchmark claims, prove) on a bounded
-    sample of the workload: two 4096x4096 matrices (2^25 entries, two PCS
-    blocks of 2^24, so the batched opening path runs as in the 8B case) on the
-    host cores. Scaled to the 2^33-entry 8B BVA by entry count."""
+    on the host cores, timed over the prover's own phases as ProverProfile
+    counts them (protocol.rs:67-80): commit and prove; setup and the claims
+    are not timed. Bounded sample, scaled to the 8B BVA (2^33 logical entries,
+    8 PCS blocks of 2^30 of which 8 hold data):
+    - commit: one full 2^30-entry block (a 2^24-entry synthetic sample tiled
+      64 times), x 8 blocks;
+    - prove: the opening of 2 x 4096x4096 (2^25 entries, two blocks of 2^24,
+      the batched path), x 256 by entries."""
     from tests.conftest import ORACLE_KB_LIB, load_oracle
-    from tests.oracle_ref import oracle_prove, shapes_flat, synthetic_bf16, synthetic_int8
+    from tests.oracle_ref import oracle_commit, oracle_prove, shapes_flat, synthetic_bf16, synthetic_int8
     import numpy as np
     L = load_oracle(ORACLE_KB_LIB) if kb else load_oracle()
     make = synthetic_bf16 if kb else synthetic_int8
+    block = np.tile(make(1 << 24, seed), 64)
+    t0 = time.perf_counter()
+    oracle_commit(L, 30, block, 1)
+    commit_block_s = time.perf_counter() - t0
+    del block
     shapes = [(4096, 4096), (4096, 4096)]
     values = np.concatenate([make(4096 * 4096, seed + i) for i in range(2)])
-    t0 = time.perf_counter()
     oracle_prove(L, shapes_flat(shapes), len(shapes), 24, 4, values)
-    dt = time.perf_counter() - t0
+    ph = (ctypes.c_double * 4)()
+    L.orc_last_phase_seconds(ph)
+    prove_s = ph[3]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     scale = (1 << 33) // len(values)
     kind = "bf16-pattern int16" if kb else "int8"
     return {
-        "value": round(dt * scale, 2),
-        "unit": "s (commit+open, extrapolated to the 8B BVA)",
+        "value": round(8 * commit_block_s + scale * prove_s, 2),
+        "unit": "s (commit+open of the 8B BVA, extrapolated)",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle commit+claims+prove of 2 x 4096x4096 {kind} (2^25 entries, chunk_log 24, 2 blocks) "
-                  f"in {dt:.2f} s on {threads} threads, x{scale} by entries",
+        "commit_s": round(8 * commit_block_s, 2),
+        "open_s": round(scale * prove_s, 2),
+        "sample": f"oracle commit of one full 2^30-entry {kind} block in {commit_block_s:.2f} s (x8 blocks), and "
+                  f"Parameters::prove of 2 x 4096x4096 (2^25 entries, chunk_log 24, 2 blocks) in {prove_s:.2f} s "
+                  f"(x{scale} by entries), on {threads} threads; setup and claims not timed",
     }
 
 
@@ -176,26 +187,46 @@ def measure(pa, field, args, per_matrix=True, serve_requests=0, model="8b", step
     return res
 
 
+VALU_PEAK_OPS = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes x 2.4 GHz, 32-bit integer ops per second
+B2S_OPS = 1150                     # 32-bit operations of one BLAKE2s compression
+
+
 def roofline_of(pa, res, field_tag):
-    """Roofline object of the dominant kernel: the pass that finishes the
-    message (Mobius on the last row group of the commit schedule) and starts
-    the parity (WHT on the same bits). Per launch it reads the 32-column message half and writes
-    the message and parity halves: 3 x 32 n elements of 16 bytes. Its launch
-    time comes from HIP events on the commit stream around every launch."""
+    """Roofline object of the commit's dominant kernel: of the two kernels
+    timed per launch with HIP events on the commit stream, the one with the
+    longer launch. M_W (the Mobius + WHT pass of the last row group) reads the
+    32-column message half and writes the message and parity halves: 3 x 32 n
+    elements of 16 bytes. The leaf hash reads the 32 R-row codeword and writes
+    R digests; it is bound by VALU work (BLAKE2s), reported beside its bytes as
+    R x 10 compressions (587-byte leaves) of 1150 ops against the VALU peak."""
     params, timings = res["params"], res["timings"]
     n = params.block_len // (32 * pa.field_info(params.field)["lanes"])  # message rows per column
+    R = 2 * n
     launches = sum(t["commit"]["timed_blocks"] for t in timings)
     mw_ms = sum(t["commit"]["mw_ms"] for t in timings) / max(launches, 1)
     leaf_ms = sum(t["commit"]["leaf_ms"] for t in timings) / max(launches, 1)
-    if not mw_ms:
+    if not mw_ms or not leaf_ms:
         return None
     mw_bytes = 3 * 32 * n * 16
-    achieved = mw_bytes / (mw_ms * 1e-3) / 1e9
+    leaf_bytes = 32 * R * 16 + R * 32
+    leaf_ops = R * 10 * B2S_OPS
+    leaf = {"kernel": "k_leaf_hash<75> (BLAKE2s row leaves)", "algorithmic_bytes": leaf_bytes,
+            "avg_launch_ms": round(leaf_ms, 4),
+            "hbm_GB_per_s": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "compressions": R * 10, "valu_frac": round(leaf_ops / (leaf_ms * 1e-3) / VALU_PEAK_OPS, 4),
+            "traffic": measured_traffic(field_tag, "leaf")}
+    mw = {"kernel": "k_tile<M_W> (Mobius + WHT, last row group)", "algorithmic_bytes": mw_bytes,
+          "avg_launch_ms": round(mw_ms, 4), "traffic": measured_traffic(field_tag, "mw")}
+    if leaf_ms > mw_ms:
+        kernel, nbytes, ms, traffic = leaf["kernel"], leaf_bytes, leaf_ms, leaf["traffic"]
+    else:
+        kernel, nbytes, ms, traffic = mw["kernel"], mw_bytes, mw_ms, mw["traffic"]
+    achieved = nbytes / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(field_tag),
-            "kernel": "k_tile<M_W> (Mobius + WHT, last row group)",
-            "algorithmic_bytes": mw_bytes, "avg_launch_ms": round(mw_ms, 4), "launches": launches,
-            "leaf_hash_ms": round(leaf_ms, 4)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+            "algorithmic_bytes": nbytes, "avg_launch_ms": round(ms, 4), "launches": launches,
+            "mw": mw, "leaf_hash": leaf}
 
 
 def phase_hbm(pa, res):

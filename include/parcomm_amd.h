@@ -163,6 +163,25 @@ int qf_device_copy(void* device_dst, const void* device_src, size_t bytes, int p
  * holds idle, which its next allocations reuse. */
 int qf_device_memory(size_t* free_bytes, size_t* total_bytes);
 
+/* Layout::encode (layout.rs:115-135) on the device: scatters the n row-major
+ * matrices (matrix i has lens[i] == rows_i x cols_i entries, in the order of
+ * the shapes) into the BVA at device_bva (16-byte aligned, active entries),
+ * bucket by bucket, writing zero on every non-physical padding entry. The
+ * matrices are device pointers when on_device is nonzero (all buckets in one
+ * launch), else host memory staged through the device one matrix at a time.
+ * Count and length mismatches fail with QF_ERR_INVALID_PARAMETERS and the
+ * reference's messages ("matrix count mismatch", "matrix length mismatch").
+ * Synchronous. */
+int qf_encode(const qf_params* p, const void* const* matrices, const uint64_t* lens, size_t n, int on_device,
+              void* device_bva);
+/* One matrix of Layout::encode, for callers that stream a model into HBM one
+ * matrix at a time as the reference CLI streams its dense BVA input
+ * (main.rs:145-173): writes matrix `matrix`'s buckets (its physical rows and
+ * their zero padding) and leaves the rest of the BVA untouched. Encoding
+ * every matrix once gives the same BVA as qf_encode. Synchronous. */
+int qf_encode_matrix(const qf_params* p, uint64_t matrix, const void* values, uint64_t len, int on_device,
+                     void* device_bva);
+
 /* Synthetic workload generator (no reference counterpart; the reference CLI
  * streams real checkpoints): fills the active BVA in device memory, from a
  * counter-based generator keyed by `seed`, with clip(round(N(0, sigma)),
@@ -186,7 +205,9 @@ int qf_autotune_report(char* out, size_t cap, size_t* len);
  * reference runs on CPU threads): QF_LAUNCH_BATCHED (default) issues one launch
  * per phase over every bucket term of every matrix; QF_LAUNCH_PER_MATRIX issues
  * one launch and one host round trip per term, the plain per-matrix GPU form
- * the benchmark compares against. Proofs are identical in both. Process-wide. */
+ * the benchmark compares against. Proofs are identical in both. The form is
+ * per host thread: it applies to the calling thread's later calls only, so
+ * proofs running on other threads (e.g. serving workers) keep their own. */
 #define QF_LAUNCH_BATCHED 0
 #define QF_LAUNCH_PER_MATRIX 1
 int qf_set_launch_mode(int mode);

@@ -1,12 +1,19 @@
 // C entry points of the CPU oracle, loaded by tests/ and bench.py through
 // ctypes (test infrastructure only; see qf_core.hpp).
 #include "qf_commit.hpp"
+#include <chrono>
 #include <memory>
 
 using namespace orc;
 
 namespace {
 thread_local std::string g_err;
+// Wall seconds of the phases of the last orc_prove_* call on this thread:
+// setup, commit, claims, prove (Parameters::prove: EO, opening, Decide).
+thread_local double g_phase_s[4];
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
 template <class F>
 int guarded(F&& f) {
   try { f(); return 0; } catch (const std::exception& e) { g_err = e.what(); return 1; }
@@ -16,6 +23,8 @@ int guarded(F&& f) {
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+// Phase times (seconds) of this thread's last orc_prove_* call: setup, commit, claims, prove.
+void orc_last_phase_seconds(double* out4) { for (int i = 0; i < 4; i++) out4[i] = g_phase_s[i]; }
 // Extension degree of this build: 8 (F65521 / Ext8) or 4 (KoalaBear / Ext4).
 int orc_lanes() { return LANES; }
 
@@ -122,14 +131,27 @@ void orc_merkle_root_rows(const void* entries, size_t rows, size_t cols, uint8_t
 
 // ---------------------------------------------------------------- full prover (main.rs:175-220 flow)
 #include "qf_open.hpp"
+// Next coordinate of a packed point list (16 canonical bytes); advances the cursor.
+static E8 read_point_coord(const uint8_t*& q) {
+  E8 v;
+  check(e8_read(q, v), "non-canonical point coordinate");
+  q += 16;
+  return v;
+}
+
 extern "C" {
 // Commits the BVA, derives the benchmark points from the statement
 // (main.rs:190-204), evaluates the claims and proves. Writes the bincode proof
 // into `proof` (capacity `cap`), its length into `proof_len`, and the compact
 // JSON of the commitment and claims into the optional buffers.
-int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg, uint32_t packed_log,
-                        const void* values_in, uint64_t n_values, uint8_t* proof, size_t cap, size_t* proof_len,
-                        char* statement_json, size_t statement_cap, char* claims_json, size_t claims_cap) {
+// With `points` (per matrix its row point and then its column point, 16
+// canonical bytes per coordinate) the claims are evaluated at those points
+// instead of the benchmark points (tbv.rs:143-185), as the reference's own
+// tests do with hand-picked coordinates (end_to_end.rs:170-192).
+static int prove_flow(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg, uint32_t packed_log,
+                      const void* values_in, uint64_t n_values, const uint8_t* points_in, uint8_t* proof, size_t cap,
+                      size_t* proof_len, char* statement_json, size_t statement_cap, char* claims_json,
+                      size_t claims_cap) {
   return guarded([&] {
     const Val* values = static_cast<const Val*>(values_in);
     std::vector<Shape> s(n);
@@ -142,7 +164,10 @@ int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg
     check(blocks <= 128, "at most 128 PCS blocks are supported");
     u8 seed[32];
     std::memset(seed, 0x51, 32);
+    auto t0 = std::chrono::steady_clock::now();
     PcsParams pp = pcs_setup(size_t(1) << (chunk_log - LOG_LANES), 32, 2, 100, seed);
+    g_phase_s[0] = seconds_since(t0);
+    t0 = std::chrono::steady_clock::now();
     Statement st;
     st.config = {chunk_log_cfg, packed_log, 100};
     st.shapes = s;
@@ -153,6 +178,8 @@ int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg
       Committed c = packed_commit(pp, pack_block(values + (avail ? first : 0), avail, block_len));
       st.roots.push_back(c.root());
     }
+    g_phase_s[1] = seconds_since(t0);
+    t0 = std::chrono::steady_clock::now();
     OuterTranscript pts;
     pts.object("benchmark-points", json_statement(st));
     std::vector<std::pair<std::vector<E8>, std::vector<E8>>> points;
@@ -161,13 +188,16 @@ int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg
       while (rp < sh.rows) rp <<= 1;
       while (cp < sh.cols) cp <<= 1;
       std::vector<E8> row, col;
-      for (int i = 0; i < ilog2(rp); i++) row.push_back(pts.challenge("row"));
-      for (int i = 0; i < ilog2(cp); i++) col.push_back(pts.challenge("col"));
+      for (int i = 0; i < ilog2(rp); i++) row.push_back(points_in ? read_point_coord(points_in) : pts.challenge("row"));
+      for (int i = 0; i < ilog2(cp); i++) col.push_back(points_in ? read_point_coord(points_in) : pts.challenge("col"));
       points.push_back({row, col});
     }
     auto claims = evaluate_claims(l, values, n_values, points);
     Config cfg{chunk_log_cfg, packed_log, 100};
+    g_phase_s[2] = seconds_since(t0);
+    t0 = std::chrono::steady_clock::now();
     Bytes pf = parcomm_prove(l, cfg, pp, chunk_log, st, values, n_values, claims);
+    g_phase_s[3] = seconds_since(t0);
     *proof_len = pf.size();
     check(pf.size() <= cap, "proof buffer too small");
     std::memcpy(proof, pf.data(), pf.size());
@@ -179,6 +209,21 @@ int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg
     put(json_statement(st), statement_json, statement_cap);
     put(json_claims(claims), claims_json, claims_cap);
   });
+}
+
+int orc_prove_benchmark(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg, uint32_t packed_log,
+                        const void* values_in, uint64_t n_values, uint8_t* proof, size_t cap, size_t* proof_len,
+                        char* statement_json, size_t statement_cap, char* claims_json, size_t claims_cap) {
+  return prove_flow(shapes, n, chunk_log_cfg, packed_log, values_in, n_values, nullptr, proof, cap, proof_len,
+                    statement_json, statement_cap, claims_json, claims_cap);
+}
+
+int orc_prove_points(const uint64_t* shapes, size_t n, uint32_t chunk_log_cfg, uint32_t packed_log,
+                     const void* values_in, uint64_t n_values, const uint8_t* points, uint8_t* proof, size_t cap,
+                     size_t* proof_len, char* statement_json, size_t statement_cap, char* claims_json,
+                     size_t claims_cap) {
+  return prove_flow(shapes, n, chunk_log_cfg, packed_log, values_in, n_values, points, proof, cap, proof_len,
+                    statement_json, statement_cap, claims_json, claims_cap);
 }
 }
 

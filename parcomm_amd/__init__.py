@@ -91,6 +91,8 @@ def lib(field=QF_FIELD_F65521):
         L.qf_device_free.argtypes = [vp]
         L.qf_device_copy.argtypes = [vp, vp, sz, i32]
         L.qf_synthetic_bva.argtypes = [vp, u64, ctypes.c_float, vp]
+        L.qf_encode.argtypes = [vp, P(vp), P(u64), sz, i32, vp]
+        L.qf_encode_matrix.argtypes = [vp, u64, vp, u64, i32, vp]
         L.qf_claims_size.argtypes = [vp, P(sz)]
         L.qf_benchmark_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
         L.qf_prove.argtypes = [vp, vp, ctypes.c_char_p, sz, P(vp)]
@@ -417,6 +419,39 @@ class Parameters:
         roots = b"".join(statement.roots)
         self._check(self._L.qf_verify(self._h, roots, len(statement.roots), raw, len(raw), proof, len(proof)))
         return True
+
+    def encode(self, matrices, out=None):
+        """Layout::encode (layout.rs:115-135) on the device: the BVA of the
+        row-major matrices (numpy arrays in host memory, or DeviceBuffers),
+        as a device-resident DeviceBuffer (`out`, or a new one)."""
+        import numpy as np
+        item = np.dtype(self.entry_dtype).itemsize
+        if out is None:
+            out = DeviceBuffer(self.active * item, self.field)
+        on_device = bool(matrices) and all(isinstance(m, DeviceBuffer) for m in matrices)
+        keep = []
+        ptrs = (ctypes.c_void_p * max(len(matrices), 1))()
+        lens = (ctypes.c_uint64 * max(len(matrices), 1))()
+        for i, m in enumerate(matrices):
+            if on_device:
+                ptrs[i], lens[i] = m.ptr.value, m.nbytes // item
+            else:
+                arr = np.ascontiguousarray(m, dtype=self.entry_dtype)
+                keep.append(arr)
+                ptrs[i], lens[i] = arr.ctypes.data, arr.size
+        self._check(self._L.qf_encode(self._h, ptrs, lens, len(matrices), int(on_device), out.ptr))
+        return out
+
+    def encode_matrix(self, index, matrix, out):
+        """One matrix of Layout::encode into the device BVA `out` (streaming input)."""
+        import numpy as np
+        if isinstance(matrix, DeviceBuffer):
+            n = matrix.nbytes // np.dtype(self.entry_dtype).itemsize
+            self._check(self._L.qf_encode_matrix(self._h, index, matrix.ptr, n, 1, out.ptr))
+        else:
+            arr = np.ascontiguousarray(matrix, dtype=self.entry_dtype)
+            self._check(self._L.qf_encode_matrix(self._h, index, arr.ctypes.data, arr.size, 0, out.ptr))
+        return out
 
     def synthetic_bva(self, seed, sigma=None):
         """Device-resident synthetic BVA on physical entries: clip(round(N(0, sigma)))

@@ -39,6 +39,9 @@ int guarded(F&& f) {
   }
 }
 
+}  // namespace
+
+namespace qf {
 // Interval list of entries that must be zero (layout.rs:136-155), split into
 // chunks of at most 64 KiB so the device check spreads over many workgroups.
 std::vector<uint64_t> padding_ranges(const Layout& l) {
@@ -61,6 +64,16 @@ __global__ void k_check_zero(const Val* bva, const uint64_t* ranges, uint32_t* f
   if (bad) atomicOr(flag, 1u);
 }
 
+void launch_check_zero(const Val* bva, const uint64_t* d_ranges, size_t count, uint32_t* d_flag, hipStream_t st) {
+  for (size_t first = 0; first < count; first += 65535) {
+    size_t n = std::min<size_t>(65535, count - first);
+    hipLaunchKernelGGL(k_check_zero, dim3(uint32_t(n)), dim3(256), 0, st, bva, d_ranges + 2 * first, d_flag);
+  }
+}
+}  // namespace qf
+
+namespace {
+
 void validate_device(const ParamsState* p, const Val* d_bva, hipStream_t st) {
   {
     std::lock_guard<std::mutex> g(p->mu);
@@ -72,10 +85,7 @@ void validate_device(const ParamsState* p, const Val* d_bva, hipStream_t st) {
   uint32_t* d_flag = static_cast<uint32_t*>(dmalloc(4, st));
   QF_HIP(hipMemcpyAsync(d_ranges, p->zero_ranges.data(), p->zero_ranges.size() * 8, hipMemcpyHostToDevice, st));
   QF_HIP(hipMemsetAsync(d_flag, 0, 4, st));
-  for (size_t first = 0; first < count; first += 65535) {
-    size_t n = std::min<size_t>(65535, count - first);
-    hipLaunchKernelGGL(k_check_zero, dim3(uint32_t(n)), dim3(256), 0, st, d_bva, d_ranges + 2 * first, d_flag);
-  }
+  launch_check_zero(d_bva, d_ranges, count, d_flag, st);
   uint32_t flag = 0;
   QF_HIP(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, st));
   QF_HIP(hipStreamSynchronize(st));

@@ -69,6 +69,58 @@ __global__ void k_zero_rect(Val* base, uint64_t rows, uint64_t stride, uint64_t 
     base[(i / width) * stride + (i % width)] = 0;
 }
 
+// Layout::encode (layout.rs:115-135) on the device. A bucket's rows x cols
+// entries (both powers of two) are cut into chunks of 16 bytes (CH entries),
+// or of the whole bucket when it is smaller; entry (r, c) of a bucket is
+// matrix entry (row_offset + r, col_offset + c) for r < physical_rows and
+// c < physical_cols, and zero otherwise. Buckets are aligned to their length,
+// so a 16-byte chunk never straddles two buckets, and a work unit of
+// ENC_WG x ENC_PER chunks belongs to one bucket.
+constexpr int ENC_WG = 256, ENC_PER = 8;
+constexpr uint64_t ENC_UNIT = uint64_t(ENC_WG) * ENC_PER;
+constexpr int ENC_CH = 16 / int(sizeof(Val));  // entries per 16-byte chunk
+struct EncodeTask {
+  uint64_t src;         // first source entry: row_offset * mcols + col_offset
+  uint64_t mcols;       // source row stride (the matrix's columns)
+  uint64_t dst;         // bucket offset in the BVA (entries)
+  uint64_t chunks;      // chunks of the bucket
+  uint64_t first_unit;  // first work unit of the bucket
+  uint32_t log_cols, ch, prow, pcol, mat;
+};
+
+__global__ __launch_bounds__(ENC_WG) void k_encode(const Val* const* __restrict__ mats,
+                                                 const EncodeTask* __restrict__ tasks,
+                                                 const uint32_t* __restrict__ unit_task, Val* __restrict__ bva) {
+  const EncodeTask t = tasks[unit_task[blockIdx.x]];
+  const Val* __restrict__ m = mats[t.mat] + t.src;
+  const uint64_t cmask = (uint64_t(1) << t.log_cols) - 1;
+  const uint64_t c0 = (blockIdx.x - t.first_unit) * ENC_UNIT + threadIdx.x;
+  for (int k = 0; k < ENC_PER; k++) {
+    const uint64_t chunk = c0 + uint64_t(k) * ENC_WG;
+    if (chunk >= t.chunks) break;
+    const uint64_t e0 = chunk * t.ch;
+    Val* d = bva + t.dst + e0;
+    auto entry = [&](uint64_t e) -> Val {
+      const uint64_t r = e >> t.log_cols, c = e & cmask;
+      return (r < t.prow && c < t.pcol) ? m[r * t.mcols + c] : Val(0);
+    };
+    if (t.ch == ENC_CH) {  // a whole 16-byte store, inside one row when the bucket is at least CH wide
+      const uint64_t r = e0 >> t.log_cols, c = e0 & cmask;
+      const Val* s = m + r * t.mcols + c;
+      if (cmask + 1 >= uint64_t(ENC_CH) && r < t.prow && c + ENC_CH <= t.pcol &&
+          (reinterpret_cast<uintptr_t>(s) & 15) == 0) {
+        *reinterpret_cast<uint4*>(d) = *reinterpret_cast<const uint4*>(s);
+      } else {
+        union { Val v[ENC_CH]; uint4 w; } u;
+        for (int i = 0; i < ENC_CH; i++) u.v[i] = entry(e0 + i);
+        *reinterpret_cast<uint4*>(d) = u.w;
+      }
+    } else {  // a bucket shorter than 16 bytes: entry stores
+      for (uint32_t i = 0; i < t.ch; i++) d[i] = entry(e0 + i);
+    }
+  }
+}
+
 template <class F>
 int guard(F&& f) {
   set_last_error("");
@@ -119,6 +171,126 @@ int qf_device_copy(void* dst, const void* src, size_t bytes, int pattern) {
                          uint64_t(bytes / 16));
     QF_HIP(hipGetLastError());
     QF_HIP(hipDeviceSynchronize());
+  });
+}
+
+}  // extern "C"
+
+namespace {
+// Scatters the buckets of the matrices in `which` (device pointers in `mats`,
+// indexed by matrix) into the BVA on `st`.
+void encode_launch(const Layout& l, const std::vector<uint64_t>& which, const std::vector<const Val*>& mats, Val* bva,
+                   hipStream_t st) {
+  std::vector<EncodeTask> tasks;
+  std::vector<uint32_t> unit_task;
+  std::vector<char> want(l.shapes.size(), 0);
+  for (uint64_t m : which) want[m] = 1;
+  for (const Bucket& b : l.buckets) {
+    if (!want[b.matrix] || b.len() == 0) continue;
+    EncodeTask t;
+    t.src = b.row_offset * l.shapes[b.matrix].cols + b.col_offset;
+    t.mcols = l.shapes[b.matrix].cols;
+    t.dst = b.offset;
+    t.ch = uint32_t(std::min<uint64_t>(ENC_CH, b.len()));
+    t.chunks = b.len() / t.ch;
+    t.first_unit = unit_task.size();
+    t.log_cols = uint32_t(ilog2u(b.cols));
+    t.prow = uint32_t(b.physical_rows);
+    t.pcol = uint32_t(b.physical_cols);
+    t.mat = uint32_t(b.matrix);
+    require((uint64_t(1) << t.log_cols) == b.cols && b.len() % t.ch == 0 && b.offset % t.ch == 0, QF_ERR_ARGUMENT,
+            "bucket geometry");
+    const uint64_t units = (t.chunks + ENC_UNIT - 1) / ENC_UNIT;
+    unit_task.insert(unit_task.end(), units, uint32_t(tasks.size()));
+    tasks.push_back(t);
+  }
+  if (tasks.empty()) return;
+  require(unit_task.size() < (1ull << 31), QF_ERR_ARGUMENT, "encode grid too large");
+  auto* d_mats = static_cast<const Val**>(dmalloc(mats.size() * sizeof(Val*), st));
+  auto* d_tasks = static_cast<EncodeTask*>(dmalloc(tasks.size() * sizeof(EncodeTask), st));
+  auto* d_units = static_cast<uint32_t*>(dmalloc(unit_task.size() * 4, st));
+  QF_HIP(hipMemcpyAsync(d_mats, mats.data(), mats.size() * sizeof(Val*), hipMemcpyHostToDevice, st));
+  QF_HIP(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(EncodeTask), hipMemcpyHostToDevice, st));
+  QF_HIP(hipMemcpyAsync(d_units, unit_task.data(), unit_task.size() * 4, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_encode, dim3(uint32_t(unit_task.size())), dim3(ENC_WG), 0, st, d_mats, d_tasks, d_units, bva);
+  QF_HIP(hipGetLastError());
+  QF_HIP(hipStreamSynchronize(st));  // the host tables above are released when this returns
+  dfree(d_mats, st);
+  dfree(d_tasks, st);
+  dfree(d_units, st);
+}
+
+void check_matrix(const Layout& l, uint64_t m, uint64_t len) {
+  require(m < l.shapes.size(), QF_ERR_INVALID_PARAMETERS, "matrix count mismatch");
+  require(len == l.shapes[m].rows * l.shapes[m].cols, QF_ERR_INVALID_PARAMETERS, "matrix length mismatch");
+}
+}  // namespace
+
+extern "C" {
+
+int qf_encode(const qf_params* p, const void* const* matrices, const uint64_t* lens, size_t n, int on_device,
+              void* device_bva) {
+  return guard([&] {
+    require(p && device_bva && (n == 0 || (matrices && lens)), QF_ERR_ARGUMENT, "null argument");
+    require(reinterpret_cast<uintptr_t>(device_bva) % 16 == 0, QF_ERR_ARGUMENT, "device BVA must be 16-byte aligned");
+    const Layout& l = params_layout(p);
+    require(n == l.shapes.size(), QF_ERR_INVALID_PARAMETERS, "matrix count mismatch");  // layout.rs:116
+    for (size_t i = 0; i < n; i++) check_matrix(l, i, lens[i]);                        // layout.rs:117-122
+    device_setup();
+    hipStream_t st = thread_stream();
+    Val* bva = static_cast<Val*>(device_bva);
+    if (on_device) {  // every bucket of every matrix in one launch
+      std::vector<uint64_t> all(n);
+      std::vector<const Val*> mats(n);
+      for (size_t i = 0; i < n; i++) { all[i] = i; mats[i] = static_cast<const Val*>(matrices[i]); }
+      encode_launch(l, all, mats, bva, st);
+      return;
+    }
+    // host matrices stream through one device staging buffer, one matrix at a time
+    uint64_t most = 1;
+    for (size_t i = 0; i < n; i++) most = std::max<uint64_t>(most, lens[i]);
+    Val* stage = static_cast<Val*>(dmalloc(most * sizeof(Val), st));
+    std::vector<const Val*> mats(n, stage);
+    try {
+      for (size_t i = 0; i < n; i++) {
+        if (lens[i]) QF_HIP(hipMemcpyAsync(stage, matrices[i], lens[i] * sizeof(Val), hipMemcpyHostToDevice, st));
+        encode_launch(l, {uint64_t(i)}, mats, bva, st);
+      }
+    } catch (...) {
+      (void)hipStreamSynchronize(st);
+      dfree(stage, st);
+      throw;
+    }
+    dfree(stage, st);
+  });
+}
+
+int qf_encode_matrix(const qf_params* p, uint64_t matrix, const void* values, uint64_t len, int on_device,
+                     void* device_bva) {
+  return guard([&] {
+    require(p && device_bva && (values || len == 0), QF_ERR_ARGUMENT, "null argument");
+    require(reinterpret_cast<uintptr_t>(device_bva) % 16 == 0, QF_ERR_ARGUMENT, "device BVA must be 16-byte aligned");
+    const Layout& l = params_layout(p);
+    check_matrix(l, matrix, len);
+    device_setup();
+    hipStream_t st = thread_stream();
+    std::vector<const Val*> mats(l.shapes.size(), nullptr);
+    Val* stage = nullptr;
+    if (on_device) {
+      mats[matrix] = static_cast<const Val*>(values);
+    } else {
+      stage = static_cast<Val*>(dmalloc(std::max<uint64_t>(len, 1) * sizeof(Val), st));
+      if (len) QF_HIP(hipMemcpyAsync(stage, values, len * sizeof(Val), hipMemcpyHostToDevice, st));
+      mats[matrix] = stage;
+    }
+    try {
+      encode_launch(l, {matrix}, mats, static_cast<Val*>(device_bva), st);
+    } catch (...) {
+      (void)hipStreamSynchronize(st);
+      if (stage) dfree(stage, st);
+      throw;
+    }
+    if (stage) dfree(stage, st);
   });
 }
 

@@ -68,6 +68,10 @@ void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E
 
 // The tile-pass autotuner's choices so far, as a JSON array (qf_tile.hip).
 std::string autotune_report();
+// Whether tile passes issued from this thread may time new geometries (the
+// autotuner allocates scratch buffers); when not, an untuned geometry takes
+// the built-in launch shape.
+void set_tuning_allowed(bool allowed);
 
 // One register-radix tile pass (qf_tile.hip); false when the geometry does not
 // fit a 4096-element tile. Its launch shape is the autotuner's pick for the

@@ -79,4 +79,10 @@ void verify(const ParamsState& p, const std::vector<Digest>& roots, const std::v
             const uint8_t* proof, size_t len);
 std::string claims_json(const std::vector<ClaimIn>& claims);
 
+// Padding intervals of a layout as (start, len) pairs of at most 64 Ki entries
+// (layout.rs:136-155), and the device check that ORs a nonzero entry in any of
+// `count` such intervals (device table d_ranges) into *d_flag.
+std::vector<uint64_t> padding_ranges(const Layout& l);
+void launch_check_zero(const Val* bva, const uint64_t* d_ranges, size_t count, uint32_t* d_flag, hipStream_t st);
+
 }  // namespace qf

@@ -726,6 +726,10 @@ void save_tune_file(const std::map<TuneKey, TuneEntry>& tab) {
   std::fclose(fp);
 }
 
+// Callers that may not allocate device memory turn tuning off on their thread:
+// their passes take an earlier pick of the same geometry, else the built-in shape.
+thread_local bool t_tune_allowed = true;
+
 TileShape shape_for(int mode, bool pack, const TileArgs& a, uint64_t vectors, int L, bool xvec, hipStream_t st) {
   if (!autotune_on()) return default_shape(mode);
   const uint64_t n = uint64_t(1) << L;
@@ -739,6 +743,7 @@ TileShape shape_for(int mode, bool pack, const TileArgs& a, uint64_t vectors, in
     loaded = true;
   }
   auto it = tab.find(key);
+  if (it == tab.end() && !t_tune_allowed) return default_shape(mode);
   if (it == tab.end()) {
     it = tab.emplace(key, tune(mode, pack, a, vectors, L, key, st)).first;
     save_tune_file(tab);
@@ -796,6 +801,8 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
   QF_HIP(hipGetLastError());
   return true;
 }
+
+void set_tuning_allowed(bool allowed) { t_tune_allowed = allowed; }
 
 // JSON array of the autotuner's choices so far (this library build).
 std::string autotune_report() {

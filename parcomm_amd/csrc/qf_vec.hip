@@ -1741,10 +1741,12 @@ static void upload_pairs(DevCtx& c, const std::vector<std::array<E8, 2>>& pairs,
 }
 
 namespace {
-std::atomic<int> g_launch_mode{LAUNCH_BATCHED};
+// Each host thread has its own launch form, so a thread that times the
+// per-matrix form never changes the form of proofs running on other threads.
+thread_local int t_launch_mode = LAUNCH_BATCHED;
 }
-void set_launch_mode(int mode) { g_launch_mode = mode; }
-int launch_mode() { return g_launch_mode; }
+void set_launch_mode(int mode) { t_launch_mode = mode; }
+int launch_mode() { return t_launch_mode; }
 
 // Launches the batched projection: one workgroup per chunk of <= 2^16 rows of
 // a task; returns the chunk lane sums (chunks x L, on the device) and each
@@ -1755,8 +1757,9 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
   // 16 and 8 lanes; KoalaBear: 16, 8 and 4) and give every wave at least one
   // round of MF_ROWS records per hi group run on k_tensor_project_mfma (listed
   // first), the rest on k_tensor_project.
-  // QF_PROJECT_MFMA=0 keeps every chunk on k_tensor_project (same results).
-  static const bool use_mfma = [] {
+  // QF_PROJECT_MFMA=0 keeps every chunk on k_tensor_project (same results);
+  // read at every call, so a process can compare both forms.
+  const bool use_mfma = [] {
     const char* e = std::getenv("QF_PROJECT_MFMA");
     return !(e && e[0] == '0');
   }();
@@ -1847,23 +1850,10 @@ std::pair<std::vector<E8>, std::vector<E8>> tensor_project_cross(DevCtx& c, cons
   return {pos, neg};
 }
 
-std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
-                               const std::vector<std::array<E8, 2>>& pairs) {
-  require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
-  if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
-    // the plain per-matrix form: one launch (and one round trip) per bucket term
-    std::vector<E8> result;
-    for (const ProjTask& t : tasks) {
-      ProjTask one = t;
-      one.pair_off = 0;
-      std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
-      set_launch_mode(LAUNCH_BATCHED);
-      auto r = tensor_project(c, bva, available, L, {one}, own);
-      set_launch_mode(LAUNCH_PER_MATRIX);
-      result.insert(result.end(), r.begin(), r.end());
-    }
-    return result;
-  }
+// One launch over every task (the batched form).
+static std::vector<E8> tensor_project_launch(DevCtx& c, const Val* bva, uint64_t available, int L,
+                                             const std::vector<ProjTask>& tasks,
+                                             const std::vector<std::array<E8, 2>>& pairs) {
   std::vector<uint32_t> owner;
   DVec out = project_chunks(c, bva, available, L, tasks, pairs, owner);
   std::vector<E8> result(tasks.size() * L, E8{});
@@ -1871,6 +1861,22 @@ std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, in
   auto parts = dv::download(c, out);
   for (size_t ch = 0; ch < owner.size(); ch++)
     for (int l = 0; l < L; l++) result[owner[ch] * L + l] = e8_add(result[owner[ch] * L + l], parts[ch * L + l]);
+  return result;
+}
+
+std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
+                               const std::vector<std::array<E8, 2>>& pairs) {
+  require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
+  if (launch_mode() != LAUNCH_PER_MATRIX || tasks.size() <= 1) return tensor_project_launch(c, bva, available, L, tasks, pairs);
+  // the plain per-matrix form: one launch (and one round trip) per bucket term
+  std::vector<E8> result;
+  for (const ProjTask& t : tasks) {
+    ProjTask one = t;
+    one.pair_off = 0;
+    std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
+    auto r = tensor_project_launch(c, bva, available, L, {one}, own);
+    result.insert(result.end(), r.begin(), r.end());
+  }
   return result;
 }
 
@@ -1920,13 +1926,12 @@ template <bool ADD>
 static void fill_tasks(DevCtx& c, E8* out, const std::vector<FillTask>& tasks,
                        const std::vector<std::array<E8, 2>>& pairs) {
   if (launch_mode() == LAUNCH_PER_MATRIX && tasks.size() > 1) {
+    // one launch per term; a single task never shares rows with another in its launch
     for (const FillTask& t : tasks) {
       FillTask one = t;
       one.pair_off = 0;
       std::vector<std::array<E8, 2>> own(pairs.begin() + t.pair_off, pairs.begin() + t.pair_off + t.nbits);
-      set_launch_mode(LAUNCH_BATCHED);
-      fill_tasks<ADD>(c, out, {one}, own);
-      set_launch_mode(LAUNCH_PER_MATRIX);
+      fill_launch<ADD>(c, out, {one}, own);
     }
     return;
   }

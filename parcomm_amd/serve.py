@@ -1,19 +1,20 @@
 """Serving loop over a resident commitment: opening requests arrive as a
-Poisson stream, are admitted while the device-memory budget allows, and run
-concurrently, each on its worker thread's own HIP stream, so the kernels of
-requests in flight overlap on the GPU. Latency is measured from arrival to the
-finished proof.
+Poisson stream and wait in a queue; a request is admitted as soon as a slot is
+free and the device-memory budget has room for it, and runs on its worker
+thread's own HIP stream, so the kernels of the requests in flight overlap on
+the GPU. Latency is measured from arrival to the finished proof.
 
 A request picks a subset of the matrices and fresh public points for them; the
 others keep the statement's benchmark points (main.rs:188-209). Every proof
 still carries claims for all matrices, as Parameters::prove requires
 (tbv.rs:123-141), so each response verifies against the unchanged public
 commitment."""
+import collections
 import ctypes
 import threading
 import time
-from dataclasses import dataclass, field
-from typing import List
+from dataclasses import dataclass
+from typing import List, Optional
 
 import numpy as np
 
@@ -21,7 +22,8 @@ from . import Claim, _check, field_info, lib, QF_FIELD_F65521
 
 
 def device_memory(field=QF_FIELD_F65521):
-    """(free, total) bytes of the current device."""
+    """(free, total) bytes of the current device; free counts the blocks the
+    library's caching allocator holds idle."""
     free, total = ctypes.c_size_t(), ctypes.c_size_t()
     _check(lib(field).qf_device_memory(ctypes.byref(free), ctypes.byref(total)), field)
     return free.value, total.value
@@ -52,7 +54,9 @@ class Request:
     subset: List[int]
     claims: List[Claim] = None
     proof: bytes = b""
+    admitted: float = 0.0
     done: float = 0.0
+    error: Optional[str] = None
 
 
 def request_claims(params, committed, base_claims, subset, rng):
@@ -67,20 +71,57 @@ def request_claims(params, committed, base_claims, subset, rng):
     return params.evaluate_claims(committed, points)
 
 
-def serve(params, committed, base_claims, n_requests=32, rate=50.0, seed=1, workers=None, verify=False):
-    """Runs a Poisson stream of opening requests; returns (stats, requests)."""
+class _Admission:
+    """Slots and a device-memory budget. The budget is the free device memory
+    measured whenever no request is in flight (the commitment stays resident);
+    a request is admitted while the bytes of the admitted requests plus its
+    own fit in `fraction` of it, and at least one request always runs."""
+
+    def __init__(self, field, slots, per_request, fraction):
+        self.field, self.slots, self.per, self.fraction = field, slots, per_request, fraction
+        self.cv = threading.Condition()
+        self.in_flight = 0
+        self.max_in_flight = 0
+        self.budget = device_memory(field)[0]
+
+    def fits(self):
+        if self.in_flight >= self.slots:
+            return False
+        if self.in_flight == 0:
+            self.budget = device_memory(self.field)[0]
+            return True
+        return (self.in_flight + 1) * self.per <= self.fraction * self.budget
+
+    def acquire(self):
+        with self.cv:
+            while not self.fits():
+                self.cv.wait()
+            self.in_flight += 1
+            self.max_in_flight = max(self.max_in_flight, self.in_flight)
+
+    def release(self):
+        with self.cv:
+            self.in_flight -= 1
+            self.cv.notify_all()
+
+
+def serve(params, committed, base_claims, n_requests=32, rate=50.0, seed=1, workers=None, verify=False,
+          memory_fraction=0.9):
+    """Runs a Poisson stream of `n_requests` opening requests at `rate` per
+    second with at most `workers` (default 4) in flight; returns (stats,
+    requests). A request that fails keeps its error message and is counted
+    in stats["failed"]; latencies cover the completed requests only."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=n_requests)
     sizes = rng.integers(1, len(base_claims) + 1, size=n_requests)
     reqs = [Request(i, 0.0, sorted(rng.choice(len(base_claims), size=int(k), replace=False).tolist()))
             for i, k in enumerate(sizes)]
     per = request_bytes(params)
-    free, _ = device_memory(params.field)
-    budget_slots = max(1, int((free * 0.9) // per))
-    slots = max(1, min(workers or 4, budget_slots))
-    admit = threading.Semaphore(slots)
-    ready = []
-    cv = threading.Condition()
+    slots = max(1, workers or 4)
+    adm = _Admission(params.field, slots, per, memory_fraction)
+    queue = collections.deque()
+    qcv = threading.Condition()
+    stop = [False]
     t0 = time.perf_counter()
 
     def producer():
@@ -91,49 +132,69 @@ def serve(params, committed, base_claims, n_requests=32, rate=50.0, seed=1, work
             if delay > 0:
                 time.sleep(delay)
             r.arrival = time.perf_counter()
-            with cv:
-                ready.append(r)
-                cv.notify()
+            with qcv:
+                queue.append(r)
+                qcv.notify()
 
     def worker():
         while True:
-            with cv:
-                while not ready and not stop[0]:
-                    cv.wait()
-                if not ready:
+            with qcv:
+                while not queue and not stop[0]:
+                    qcv.wait()
+                if not queue:
                     return
-                r = ready.pop(0)
-            with admit:
+                r = queue.popleft()
+            adm.acquire()
+            r.admitted = time.perf_counter()
+            try:
                 rrng = np.random.default_rng(seed * 1_000_003 + r.rid)
                 r.claims = request_claims(params, committed, base_claims, r.subset, rrng)
                 r.proof, _ = params.prove(committed, r.claims)
                 r.done = time.perf_counter()
-            if verify:
-                params.verify(committed.statement, r.claims, r.proof)
+            except Exception as e:  # recorded on the request; the loop keeps serving
+                r.error = f"{type(e).__name__}: {e}"
+            finally:
+                adm.release()
+            if verify and r.error is None:
+                try:
+                    params.verify(committed.statement, r.claims, r.proof)
+                except Exception as e:
+                    r.error = f"verification failed: {e}"
 
-    stop = [False]
     threads = [threading.Thread(target=worker) for _ in range(slots)]
     for th in threads:
         th.start()
     prod = threading.Thread(target=producer)
     prod.start()
     prod.join()
-    with cv:
+    with qcv:
         stop[0] = True
-        cv.notify_all()
+        qcv.notify_all()
     for th in threads:
         th.join()
-    lat = np.array([r.done - r.arrival for r in reqs]) * 1e3
-    span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+    done = [r for r in reqs if r.error is None]
+    failed = [r for r in reqs if r.error is not None]
     stats = {
         "requests": n_requests,
+        "completed": len(done),
+        "failed": len(failed),
         "arrival_rate_per_s": rate,
         "concurrency": slots,
-        "p50_ms": round(float(np.percentile(lat, 50)), 2),
-        "p99_ms": round(float(np.percentile(lat, 99)), 2),
-        "mean_ms": round(float(lat.mean()), 2),
-        "openings_per_s": round(n_requests / span, 3),
+        "max_in_flight": adm.max_in_flight,
         "request_device_bytes": per,
         "seed": seed,
     }
+    if failed:
+        stats["first_error"] = failed[0].error
+    if done:
+        lat = np.array([r.done - r.arrival for r in done]) * 1e3
+        wait = np.array([r.admitted - r.arrival for r in done]) * 1e3
+        span = max(r.done for r in done) - min(r.arrival for r in reqs)
+        stats.update({
+            "p50_ms": round(float(np.percentile(lat, 50)), 2),
+            "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "mean_ms": round(float(lat.mean()), 2),
+            "mean_queue_ms": round(float(wait.mean()), 2),
+            "openings_per_s": round(len(done) / span, 3),
+        })
     return stats, reqs

@@ -103,3 +103,44 @@ def oracle_prove(oracle, flat, n_shapes, chunk_log, packed_log, values):
                                     ctypes.c_size_t(1 << 22))
     assert rc == 0, oracle.orc_last_error()
     return bytes(proof[:plen.value]), sj.value, cj.value
+
+
+def points_bytes(points):
+    """Packed point list: per matrix its row coordinates, then its column
+    coordinates, 16 canonical bytes each."""
+    return b"".join(bytes(v) for row, col in points for v in list(row) + list(col))
+
+
+def oracle_prove_points(oracle, flat, n_shapes, chunk_log, packed_log, values, points):
+    """As oracle_prove, with the claims evaluated at the given (row, col) points."""
+    cap = 1 << 26
+    proof = (ctypes.c_uint8 * cap)()
+    plen = ctypes.c_size_t()
+    sj = ctypes.create_string_buffer(1 << 22)
+    cj = ctypes.create_string_buffer(1 << 22)
+    pts = points_bytes(points)
+    rc = oracle.orc_prove_points(flat, ctypes.c_size_t(n_shapes), chunk_log, packed_log,
+                                 values.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(len(values)), pts, proof,
+                                 ctypes.c_size_t(cap), ctypes.byref(plen), sj, ctypes.c_size_t(1 << 22), cj,
+                                 ctypes.c_size_t(1 << 22))
+    assert rc == 0, oracle.orc_last_error()
+    return bytes(proof[:plen.value]), sj.value, cj.value
+
+
+# Magic tags that open the regions of a ParComm proof's wire
+# (packed_batch.rs:285-305, qafold_short_batch.rs:86-87,
+# qafold_short_pc.rs:240-261, qafold_short_decide_wire.rs:97-170).
+REGION_TAGS = {"batch_opening": b"QAPKBAT1", "binding": b"QAFOLDB1", "native_opening": b"QFSHORT2",
+               "decide": b"QFSDCID1"}
+
+
+def proof_regions(proof):
+    """Byte offset of one payload byte inside each region of a batched proof:
+    the packed EO proof at the start (its first cross term), then the first
+    payload byte past each region's tag and the 24 bytes that follow it."""
+    out = {"eo": 8 + 3}
+    for name, tag in REGION_TAGS.items():
+        at = proof.find(tag)
+        assert at >= 0, name
+        out[name] = at + len(tag) + 24
+    return out
