@@ -225,6 +225,24 @@ int qf_test_tile_pass(int mode, const void* device_in, void* device_out1, void* 
                       const void* device_alpha, uint64_t vectors, uint64_t in_vstride, uint64_t out_vstride, int L,
                       int s, int t, int xform_vectors);
 
+/* Commitment.roots of a device-resident BVA (the roots qf_commit_device
+ * gives, in either cache mode) computed without allocating device memory. No
+ * counterpart in the reference: this is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Every device buffer of the commit (one block's codeword and
+ * Merkle tree, alpha_1, the padding-check flag and ranges) lives in a fixed
+ * 64 MiB workspace that each device holds from the library's load until the
+ * process ends, so the effective chunk_log is limited to 23 (F65521) or 22
+ * (KoalaBear); larger blocks fail with QF_ERR_INVALID_PARAMETERS. The call
+ * runs on the null stream and returns when the roots are in `roots`
+ * (cap_blocks x 32 bytes, at least the parameters' block count). Calls on one
+ * device share the workspace: the library runs them one at a time, and the
+ * workspace keeps the last call's contents. Tile passes take an earlier
+ * autotuner pick of their geometry or the built-in launch shape (tuning
+ * allocates). */
+int qf_commit_roots_noalloc(const qf_params* p, const void* device_values, size_t len, uint8_t* roots,
+                            size_t cap_blocks);
+
 #ifdef __cplusplus
 }
 #endif
