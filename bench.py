@@ -188,7 +188,10 @@ def measure(pa, field, args, per_matrix=True, serve_requests=0, model="8b", step
 
 
 VALU_PEAK_OPS = 256 * 64 * 2.4e9  # 256 CUs x 64 lanes x 2.4 GHz, 32-bit integer ops per second
-B2S_OPS = 1150                     # 32-bit operations of one BLAKE2s compression
+# VALU instructions of one BLAKE2s compression in the leaf kernel's gfx950 code
+# (the loop over message blocks: 999 per iteration, window loads included)
+B2S_OPS = 999
+LEAF_DEVICE_COMPRESSIONS = 9       # per 587-byte leaf: the first 64 prefix bytes are compressed on the host
 
 
 def roofline_of(pa, res, field_tag):
@@ -197,8 +200,10 @@ def roofline_of(pa, res, field_tag):
     longer launch. M_W (the Mobius + WHT pass of the last row group) reads the
     32-column message half and writes the message and parity halves: 3 x 32 n
     elements of 16 bytes. The leaf hash reads the 32 R-row codeword and writes
-    R digests; it is bound by VALU work (BLAKE2s), reported beside its bytes as
-    R x 10 compressions (587-byte leaves) of 1150 ops against the VALU peak."""
+    R digests; it is bound by VALU work (BLAKE2s), reported beside its bytes:
+    R x 10 compressions per 587-byte leaf (algorithmic), of which 9 run on the
+    device after the host's prefix midstate, at 999 VALU instructions each,
+    against the VALU peak."""
     params, timings = res["params"], res["timings"]
     n = params.block_len // (32 * pa.field_info(params.field)["lanes"])  # message rows per column
     R = 2 * n
@@ -209,12 +214,13 @@ def roofline_of(pa, res, field_tag):
         return None
     mw_bytes = 3 * 32 * n * 16
     leaf_bytes = 32 * R * 16 + R * 32
-    leaf_ops = R * 10 * B2S_OPS
+    leaf_ops = R * LEAF_DEVICE_COMPRESSIONS * B2S_OPS
     leaf = {"kernel": "k_leaf_hash<75> (BLAKE2s row leaves)", "algorithmic_bytes": leaf_bytes,
             "avg_launch_ms": round(leaf_ms, 4),
             "hbm_GB_per_s": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9, 1),
             "hbm_frac": round(leaf_bytes / (leaf_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "compressions": R * 10, "valu_frac": round(leaf_ops / (leaf_ms * 1e-3) / VALU_PEAK_OPS, 4),
+            "compressions": R * 10, "device_compressions": R * LEAF_DEVICE_COMPRESSIONS,
+            "valu_frac": round(leaf_ops / (leaf_ms * 1e-3) / VALU_PEAK_OPS, 4),
             "traffic": measured_traffic(field_tag, "leaf")}
     mw = {"kernel": "k_tile<M_W> (Mobius + WHT, last row group)", "algorithmic_bytes": mw_bytes,
           "avg_launch_ms": round(mw_ms, 4), "traffic": measured_traffic(field_tag, "mw")}
@@ -280,7 +286,8 @@ def autotune_summary(pa, field):
         times = [t["ms"] for t in e["tried"]]  # empty when the pick came from QF_TUNE_FILE
         out.append({"pass": f"mode {e['mode']} bits {e['s']}+{e['t']} of L={e['L']} x{e['vectors']}"
                             + (" packed" if e["pack"] else ""),
-                    "kept": ("wide" if b["wide"] else "lean") + f" {b['per_cu']}/CU" + ("" if b["xcd"] else " no-xcd"),
+                    "kept": ("quad " if b.get("quad") else "radix ") + ("wide" if b["wide"] else "lean")
+                            + f" {b['per_cu']}/CU" + ("" if b["xcd"] else " no-xcd"),
                     "ms": round(min(times), 4) if times else None,
                     "slowest_ms": round(max(times), 4) if times else None})
     return out

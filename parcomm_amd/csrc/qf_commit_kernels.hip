@@ -369,11 +369,11 @@ static bool tiled_wht(E8* v, uint64_t vectors, uint64_t stride, int L, hipStream
 // Second half of the parity, in place: WHT on the low bits, alpha, WHT on the
 // low bits (one pass), then WHT on the high groups. Precondition: the parity
 // region holds the message with the WHT already applied on every high group.
-// F65521's one-pass WHT . alpha . WHT needs more registers than two workgroups
-// per CU allow; on full-size blocks it runs as two lean passes (WHT, then
-// alpha . WHT), which is faster despite the extra sweep.
+// On the register-radix kernels F65521's one-pass WHT . alpha . WHT needs more
+// registers than two workgroups per CU allow, so full-size blocks run it as two
+// lean passes (WHT, then alpha . WHT); the quad-split kernels run it as one.
 static void tiled_parity_tail(E8* par, uint64_t cols, uint64_t R, int L, const E8* alpha, hipStream_t st) {
-  if (LANES == 8 && L >= 16) {
+  if (LANES == 8 && L >= 16 && !quad_tiles_enabled()) {
     must(tile_pass(TP_W, false, par, par, nullptr, nullptr, cols, R, R, L, 0, 12, 0, 0, st));
     must(tile_pass(TP_AW, false, par, par, nullptr, alpha, cols, R, R, L, 0, 12, 0, 0, st));
   } else {

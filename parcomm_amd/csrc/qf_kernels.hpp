@@ -72,6 +72,10 @@ std::string autotune_report();
 // autotuner allocates scratch buffers); when not, an untuned geometry takes
 // the built-in launch shape.
 void set_tuning_allowed(bool allowed);
+// Whether tile passes may run on the quad-split kernels (QF_QTILE=0 keeps the
+// register-radix ones); the commit schedule fuses WHT . alpha . WHT into one
+// pass when they do.
+bool quad_tiles_enabled();
 
 // One register-radix tile pass (qf_tile.hip); false when the geometry does not
 // fit a 4096-element tile. Its launch shape is the autotuner's pick for the

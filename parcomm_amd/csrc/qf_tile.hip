@@ -27,6 +27,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 namespace qf {
@@ -58,6 +59,12 @@ struct TileArgs {
   uint32_t xcd_span;        // > 0: XCD-interleaved walk, tiles per XCD (even)
   int logC, t, s, logV, log_lo_chunks, log_hi_count;
   int xlo, xhi;             // transformed tile bits [xlo, xhi)
+  int quad_ok;              // the quad-split kernels' per-thread offsets fit 32 bits
+  // quad-split kernels: element offset of slot 1 of each mapping (slot q is q
+  // times that: the five slot bits lie in one field of the tile index), for the
+  // input (BVA bytes / 8 when packing), the outputs and alpha (positions)
+  uint64_t qs_in[2], qs_out[2], qs_alpha;
+  uint64_t pack_span;       // packing: bytes from a tile's first input byte to past its last
 };
 
 // Tile element e held by thread `tid` in register slot q: q supplies tile bits
@@ -501,6 +508,424 @@ __global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   tile_body<MODE, PACK, LO, true>(a);
 }
 
+// ---------------------------------------------------------------- quad-split tiles
+// A second kernel family for the same passes, with fewer LDS exchanges. A
+// thread holds one 32-bit word d of 32 tile elements (the four words of a
+// 16-byte element sit in the four lanes of a quad, lane bits 0..1 = d): two
+// u16 coefficients, as lazy signed i32 lanes (F65521), or one canonical u32
+// coefficient (KoalaBear). Five tile bits live in the register slots, so a
+// 12-bit tile takes two mappings:
+//   H: slots = tile bits 7..11; lane bits 2..5 = tile bits 0..3, wave bits = 4..6;
+//   L: slots = tile bits 2..6;  lane bits 2..5 = tile bits 0, 1, 7, 8, wave bits = 9..11.
+// Tile bits 0 and 1 are lane bits 2 and 3 in L and take DPP butterflies across
+// each row of 16 lanes. A transform over all 12 bits is one H phase, one LDS
+// exchange and one L phase (the register-radix form above takes four
+// mappings and four exchanges); a pass whose bits all lie in 7..11 needs no
+// exchange. Both mappings keep four consecutive elements (64 bytes) in 16
+// consecutive lanes, so loads and stores coalesce in either.
+constexpr int QB = 5;        // tile bits held in register slots
+constexpr int QN = 1 << QB;  // slots per thread
+constexpr int QV = LANES / 4;  // coefficient lanes per 32-bit word (F65521: two u16, KoalaBear: one u32)
+enum : int { QMAP_H = 0, QMAP_L = 1 };
+
+template <int MAP>
+constexpr int qmap_lo() { return MAP == QMAP_H ? 7 : 2; }
+// tile element of slot q held by thread tid: thread part | slot part (disjoint bits)
+template <int MAP>
+__device__ inline int qe_thread(int tid) {
+  const int u = (tid >> 2) & 15, w = tid >> 6;
+  return MAP == QMAP_H ? (u | (w << 4)) : ((u & 3) | ((u >> 2) << 7) | (w << 9));
+}
+template <int MAP>
+__host__ __device__ constexpr int qe_slot(int q) { return MAP == QMAP_H ? (q << 7) : (q << 2); }
+
+// Register butterflies on the slot bits of mapping MAP whose tile bit lies in
+// the transformed range ([LO, 12) at compile time, or [lo, hi) when LO < 0).
+template <bool WHT, int MAP, int LO>
+__device__ inline void qphase(int32_t (&x)[QN][QV], int lo, int hi) {
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int b = 0; b < QB; b++) {
+    const int tb = qmap_lo<MAP>() + b;
+    if (LO >= 0) {
+      if (tb < LO) continue;
+    } else if (tb < lo || tb >= hi) {
+      continue;
+    }
+#pragma unroll
+    for (int q = 0; q < QN; q++) {
+      if (q & (1 << b)) continue;
+      const int r = q | (1 << b);
+#pragma unroll
+      for (int k = 0; k < QV; k++) {
+        const int32_t u = x[q][k], w = x[r][k];
+        if (WHT) { x[q][k] = bf_add(u, w); x[r][k] = bf_sub(u, w); }
+        else x[r][k] = bf_sub(w, u);
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Partner values across lane bit 2 (tile bit 0 in L) and lane bit 3 (tile bit
+// 1): row_ror:N gives lane i the value of lane (i - N) mod 16 of its row, and
+// the bank mask picks the groups of four lanes that take it.
+__device__ inline int32_t dpp_partner4(int32_t v) {
+  const int32_t up = __builtin_amdgcn_update_dpp(0, v, 0x12C, 0xf, 0x5, false);  // banks 0, 2: lane i + 4
+  return __builtin_amdgcn_update_dpp(up, v, 0x124, 0xf, 0xa, false);            // banks 1, 3: lane i - 4
+}
+__device__ inline int32_t dpp_partner8(int32_t v) { return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xf, 0xf, false); }
+
+// One butterfly with the partner value p; m = -1 in the lanes that hold the
+// element with the bit set, else 0. WHT: (x + p, p - x); Mobius: (x, x - p).
+template <bool WHT>
+__device__ inline int32_t dpp_bfly(int32_t x, int32_t p, int32_t m) {
+#ifdef QF_KB
+  if (WHT) {
+    const uint32_t v = uint32_t(p) + ((uint32_t(x) ^ uint32_t(m)) - uint32_t(m));  // x + p or p - x (wrapping)
+    const uint32_t fix = m ? P : 0u - P;
+    const uint32_t w = v + fix;
+    return int32_t(v < w ? v : w);
+  }
+  const uint32_t v = uint32_t(x) - (uint32_t(p) & uint32_t(m));
+  const uint32_t w = v + P;
+  return int32_t(v < w ? v : w);
+#else
+  if (WHT) return p + ((x ^ m) - m);
+  return x - (p & m);
+#endif
+}
+
+// DPP butterflies on tile bits 0 and 1 (mapping L only) within the range.
+template <bool WHT, int LO>
+__device__ inline void qphase_dpp(int32_t (&x)[QN][QV], int lo, int hi) {
+  const int lane = threadIdx.x & 63;
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    if (LO >= 0) {
+      if (b < LO) continue;
+    } else if (b < lo || b >= hi) {
+      continue;
+    }
+    const int32_t m = -int32_t((lane >> (2 + b)) & 1);
+#pragma unroll
+    for (int q = 0; q < QN; q++) {
+#pragma unroll
+      for (int k = 0; k < QV; k++) {
+        const int32_t v = x[q][k];
+        const int32_t p = b ? dpp_partner8(v) : dpp_partner4(v);
+        x[q][k] = dpp_bfly<WHT>(v, p, m);
+      }
+      if (q % 2 == 1) __builtin_amdgcn_sched_barrier(0);  // bounds the partner values in flight
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// LDS word of (element e, word d): W = 4 e + d with W bits 9..10 (tile bits 7,
+// 8) folded into bits 4..5, A = W ^ (((W >> 9) & 3) << 4). The lanes of a wave
+// then reach distinct banks in both mappings (H varies W bits 0..5, L varies
+// W bits 0..3 and 9..10). Per thread: base[j] for slot bits 0..1 = j, and a
+// slot offset for the rest (an immediate of the LDS instruction).
+struct QLds {
+  uint32_t h[4], l[4];
+};
+__device__ inline QLds qlds_bases() {
+  const int tid = threadIdx.x, d = tid & 3;
+  QLds o;
+  const uint32_t wh = 4u * uint32_t(qe_thread<QMAP_H>(tid)) + d;  // below 512: slot bits 0..1 xor in
+  const uint32_t wl = 4u * uint32_t(qe_thread<QMAP_L>(tid)) + d;
+  const uint32_t kl = (wl >> 9) & 3;                                 // tile bits 7, 8 of the thread part
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    o.h[j] = 4u * (wh ^ (uint32_t(j) << 4));
+    o.l[j] = 4u * (wl + ((uint32_t(j) ^ kl) << 4));
+  }
+  return o;
+}
+template <int MAP>
+__device__ inline uint32_t qlds_addr(const QLds& b, int q) {  // byte address
+  return MAP == QMAP_H ? b.h[q & 3] + 4u * (uint32_t(q) << 9) : b.l[q & 3] + 4u * (uint32_t(q >> 2) << 6);
+}
+
+template <int FROM, int TO>
+__device__ inline void qexchange(int32_t (&x)[QN][QV], uint32_t* lds, const QLds& b) {
+  char* base = reinterpret_cast<char*>(lds);
+#pragma unroll
+  for (int k = 0; k < QV; k++) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < QN; q++) *reinterpret_cast<int32_t*>(base + qlds_addr<FROM>(b, q)) = x[q][k];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < QN; q++) x[q][k] = *reinterpret_cast<const int32_t*>(base + qlds_addr<TO>(b, q));
+  }
+}
+
+// A full transform over the range: the phases of the bits in each mapping,
+// from mapping FROM to the other one (TO), with one exchange between them.
+template <bool WHT, int MAP, int LO>
+__device__ inline void qphase_all(int32_t (&x)[QN][QV], int lo, int hi) {
+  qphase<WHT, MAP, LO>(x, lo, hi);
+  if (MAP == QMAP_L) qphase_dpp<WHT, LO>(x, lo, hi);
+}
+
+#ifndef QF_KB
+// LO >= 2: at most ten butterfly stages ran since the values were canonical.
+template <int LO>
+__device__ inline void qreduce(int32_t (&x)[QN][QV]) {
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < QN; q++)
+#pragma unroll
+    for (int k = 0; k < QV; k++) x[q][k] = int32_t(LO >= 2 ? reduce_lazy10(x[q][k]) : reduce_lazy(x[q][k]));
+  __builtin_amdgcn_sched_barrier(0);
+}
+__device__ inline uint32_t qpack(const int32_t (&v)[QV]) { return uint32_t(v[0]) | (uint32_t(v[1]) << 16); }
+__device__ inline void qunpack(uint32_t w, int32_t (&v)[QV]) { v[0] = int32_t(w & 0xffffu); v[1] = int32_t(w >> 16); }
+// two i8 BVA entries (bytes 2d, 2d + 1 of the element) -> canonical lanes
+__device__ inline void qunpack_bva(uint32_t w, int32_t (&v)[QV]) {
+  v[0] = int32_t(from_val(int8_t(w & 0xffu)));
+  v[1] = int32_t(from_val(int8_t((w >> 8) & 0xffu)));
+}
+#else
+template <int LO>
+__device__ inline void qreduce(int32_t (&)[QN][QV]) {}
+__device__ inline uint32_t qpack(const int32_t (&v)[QV]) { return uint32_t(v[0]); }
+__device__ inline void qunpack(uint32_t w, int32_t (&v)[QV]) { v[0] = int32_t(w); }
+// one i16 BVA entry (bytes 2d, 2d + 1 of the element) -> a canonical lane
+__device__ inline void qunpack_bva(uint32_t w, int32_t (&v)[QV]) { v[0] = int32_t(from_val(int16_t(w & 0xffffu))); }
+#endif
+
+// Offsets of a mapping's slots: a 32-bit per-thread part (bytes) and the
+// uniform per-slot part (64-bit, from the slot's tile bits).
+struct QOffs {
+  uint32_t in_t, out_t, alpha_t;
+};
+template <int MAP>
+__device__ inline QOffs qoffs(const TileArgs& a, bool pack) {
+  const int tid = threadIdx.x, d = tid & 3, e = qe_thread<MAP>(tid);
+  QOffs o;
+  o.in_t = pack ? 8u * tile_off(a, e, a.pack_n) + 2u * d : 16u * tile_off(a, e, a.in_vstride) + 4u * d;
+  o.out_t = 16u * tile_off(a, e, a.out_vstride) + 4u * d;
+  o.alpha_t = 16u * tile_off(a, e, 0);
+  return o;
+}
+
+// Slot q of a mapping is at byte (thread part + q x slot stride) from the
+// tile's first element, below 2^32 (checked on the host): a buffer resource
+// over the tile takes the thread part as the VGPR offset and the slot part as
+// the scalar offset, so the 32 slots share one address register.
+__device__ inline __amdgpu_buffer_rsrc_t qrsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7fffffff, 0x00020000);
+}
+
+template <int MAP, bool PACK>
+__device__ inline void qload(const TileArgs& a, uint64_t T, const QOffs& o, uint32_t (&raw)[QN]) {
+  uint64_t vec0, pos0;
+  tile_base(a, T, vec0, pos0);
+  if (PACK) {
+    const uint64_t b0 = a.block_first + 8 * (vec0 * a.pack_n + pos0);
+    const uint8_t* bva = static_cast<const uint8_t*>(a.in);
+    const uint32_t sb = uint32_t(8 * a.qs_in[MAP]);
+    if (b0 + a.pack_span <= a.available) {  // the whole tile lies before the active end
+      const __amdgpu_buffer_rsrc_t r = qrsrc(bva + b0);
+#pragma unroll
+      for (int q = 0; q < QN; q++) raw[q] = __builtin_amdgcn_raw_buffer_load_b16(r, o.in_t, uint32_t(q) * sb, 0);
+    } else {  // bytes at or past the active end read as zero
+#pragma unroll
+      for (int q = 0; q < QN; q++) {
+        const uint64_t byte = b0 + o.in_t + uint64_t(q) * sb;
+        if (byte + 2 <= a.available) raw[q] = *reinterpret_cast<const uint16_t*>(bva + byte);
+        else raw[q] = byte < a.available ? uint32_t(bva[byte]) : 0u;
+      }
+    }
+  } else {
+    const char* base = reinterpret_cast<const char*>(static_cast<const E8*>(a.in) + vec0 * a.in_vstride + pos0);
+    const __amdgpu_buffer_rsrc_t r = qrsrc(base);
+    const uint32_t sb = uint32_t(16 * a.qs_in[MAP]);
+#pragma unroll
+    for (int q = 0; q < QN; q++) raw[q] = __builtin_amdgcn_raw_buffer_load_b32(r, o.in_t, uint32_t(q) * sb, 0);
+  }
+}
+
+template <int MAP>
+__device__ inline void qstore(const TileArgs& a, uint64_t T, const QOffs& o, E8* out, const int32_t (&x)[QN][QV]) {
+  uint64_t vec0, pos0;
+  tile_base(a, T, vec0, pos0);
+  const __amdgpu_buffer_rsrc_t r = qrsrc(out + vec0 * a.out_vstride + pos0);
+  const uint32_t sb = uint32_t(16 * a.qs_out[MAP]);
+#pragma unroll
+  for (int q = 0; q < QN; q++) __builtin_amdgcn_raw_buffer_store_b32(qpack(x[q]), r, o.out_t, uint32_t(q) * sb, 0);
+}
+
+// x <- alpha . x in mapping L for canonical x. The four words of slot q's
+// element sit in the four lanes of a quad; two 2 x 2 transposes across the
+// quad (quad_perm) give lane d the whole element of slot 4 g + d for g = 0..7,
+// which is multiplied by its alpha as one extension product, and transposed back.
+__device__ inline void quad_transpose(uint32_t (&y)[QN]) {
+  const int lane = threadIdx.x & 63;
+  const bool b0 = lane & 1, b1 = lane & 2;
+#pragma unroll
+  for (int g = 0; g < QN; g += 4) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {  // slot bit 0 <-> lane bit 0
+      const uint32_t e = y[g + 2 * h], o = y[g + 2 * h + 1];
+      const uint32_t pe = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(e), 0xB1, 0xf, 0xf, false));  // quad_perm(1,0,3,2)
+      const uint32_t po = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(o), 0xB1, 0xf, 0xf, false));
+      y[g + 2 * h] = b0 ? po : e;
+      y[g + 2 * h + 1] = b0 ? o : pe;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {  // slot bit 1 <-> lane bit 1
+      const uint32_t e = y[g + h], o = y[g + h + 2];
+      const uint32_t pe = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(e), 0x4E, 0xf, 0xf, false));  // quad_perm(2,3,0,1)
+      const uint32_t po = uint32_t(__builtin_amdgcn_mov_dpp(int32_t(o), 0x4E, 0xf, 0xf, false));
+      y[g + h] = b1 ? po : e;
+      y[g + h + 2] = b1 ? o : pe;
+    }
+  }
+}
+__device__ inline void qmul_alpha(const TileArgs& a, uint64_t T, const QOffs& o, int32_t (&x)[QN][QV]) {
+  uint64_t vec0, pos0;
+  tile_base(a, T, vec0, pos0);
+  const int d = threadIdx.x & 3;
+  const __amdgpu_buffer_rsrc_t ar = qrsrc(a.alpha + pos0);
+  const uint32_t at = o.alpha_t + uint32_t(d) * uint32_t(16 * a.qs_alpha);
+  const uint32_t sa = uint32_t(16 * a.qs_alpha);
+  uint32_t y[QN];
+#pragma unroll
+  for (int q = 0; q < QN; q++) y[q] = qpack(x[q]);
+  quad_transpose(y);
+#pragma unroll
+  for (int g = 0; g < QN; g += 4) {  // element of slot g + d: words y[g .. g + 3]
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 bv = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(ar, at, uint32_t(g) * sa, 0));
+    const uint4 bw = make_uint4(bv.x, bv.y, bv.z, bv.w);
+    int32_t c[LANES];
+    unpack_e8(make_uint4(y[g], y[g + 1], y[g + 2], y[g + 3]), c);
+    mul_alpha(c, bw);
+    const uint4 r = pack_e8(c);
+    y[g] = r.x; y[g + 1] = r.y; y[g + 2] = r.z; y[g + 3] = r.w;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  quad_transpose(y);
+#pragma unroll
+  for (int q = 0; q < QN; q++) qunpack(y[q], x[q]);
+}
+
+// Transform over the range starting in mapping FROM: returns in the other
+// mapping when both hold transformed bits, else stays in FROM.
+template <int MAP>
+constexpr int qother() { return MAP == QMAP_H ? QMAP_L : QMAP_H; }
+
+template <int MODE, bool PACK, int LO, bool PF>
+__device__ inline void qtile_body(const TileArgs& a) {
+  __shared__ uint32_t lds[16384];  // 4096 elements x 4 words
+  uint64_t first, last, step;
+  if (a.xcd_span) {
+    const uint64_t g = blockIdx.x % 8, slots = gridDim.x / 8;
+    first = g * a.xcd_span + blockIdx.x / 8;
+    last = (g + 1) * a.xcd_span < a.tiles ? (g + 1) * a.xcd_span : a.tiles;
+    step = slots;
+  } else {
+    first = uint64_t(blockIdx.x) * a.per_wg;
+    last = first + a.per_wg < a.tiles ? first + a.per_wg : a.tiles;
+    step = 1;
+  }
+  if (first >= last) return;
+  const int lo = a.xlo, hi = a.xhi;
+  // which mappings hold transformed bits (uniform)
+  const bool in_h = (LO >= 0 ? true : hi > 7);
+  const bool in_l = (LO >= 0 ? LO < 7 : lo < 7);
+  const QLds lb = qlds_bases();
+  const QOffs oh = qoffs<QMAP_H>(a, PACK), ol = qoffs<QMAP_L>(a, PACK);
+  // The transform's bits may be taken in either order. alpha . WHT loads in
+  // L, where the alpha product runs; WHT . alpha . WHT starts in H so that its
+  // alpha product falls in L between the two halves; packing loads in H (128
+  // contiguous BVA bytes per wave); every other mode starts in L when L holds
+  // transformed bits (fewer registers live across its DPP stages).
+  constexpr bool AW = MODE == TM_AW;
+  constexpr bool WAW = MODE == TM_WAW || MODE == TM_COPY_WAW;
+  const bool start_h = AW ? false : (WAW || PACK) ? in_h : !in_l;
+  uint32_t raw[QN];
+  auto load = [&](uint64_t T) {
+    if (start_h) qload<QMAP_H, PACK>(a, T, oh, raw);
+    else qload<QMAP_L, PACK>(a, T, ol, raw);
+  };
+  if (PF) load(first);
+  for (uint64_t T = first; T < last; T += step) {
+    int32_t x[QN][QV];
+    if (!PF) load(T);
+#pragma unroll
+    for (int q = 0; q < QN; q++) {
+      if (PACK) qunpack_bva(raw[q], x[q]);
+      else qunpack(raw[q], x[q]);
+    }
+    if (PF && T + step < last) load(T + step);
+    // cur: the mapping the data is in (uniform)
+    bool cur_h = start_h;
+    auto store = [&](E8* out) {
+      if (cur_h) qstore<QMAP_H>(a, T, oh, out, x);
+      else qstore<QMAP_L>(a, T, ol, out, x);
+    };
+    // one transform over the range: the current mapping's bits, then (if the
+    // other mapping holds transformed bits) an exchange and its bits
+    auto xform = [&](auto wht) {
+      constexpr bool W = decltype(wht)::value;
+      if (cur_h) {
+        qphase_all<W, QMAP_H, LO>(x, lo, hi);
+        if (in_l) { qexchange<QMAP_H, QMAP_L>(x, lds, lb); cur_h = false; qphase_all<W, QMAP_L, LO>(x, lo, hi); }
+      } else {
+        qphase_all<W, QMAP_L, LO>(x, lo, hi);
+        if (in_h) { qexchange<QMAP_L, QMAP_H>(x, lds, lb); cur_h = true; qphase_all<W, QMAP_H, LO>(x, lo, hi); }
+      }
+    };
+    using Mob = std::integral_constant<bool, false>;
+    using Wht = std::integral_constant<bool, true>;
+    if (MODE == TM_COPY_W || MODE == TM_COPY_WAW) store(a.out1);
+    if (AW) {  // x canonical from memory, loaded in L
+      qmul_alpha(a, T, ol, x);
+      xform(Wht{});
+      qreduce<LO>(x);
+      store(a.out1);
+    }
+    if (MODE == TM_M || MODE == TM_M_W) {
+      xform(Mob{});
+      qreduce<LO>(x);
+      store(a.out1);
+    }
+    if (MODE == TM_M_W || MODE == TM_W || MODE == TM_COPY_W) {
+      xform(Wht{});
+      qreduce<LO>(x);
+      store(MODE == TM_W ? a.out1 : a.out2);
+    }
+    if (WAW) {
+      xform(Wht{});
+      qreduce<LO>(x);
+      if (cur_h) {  // the alpha product runs in L
+        qexchange<QMAP_H, QMAP_L>(x, lds, lb);
+        cur_h = false;
+      }
+      qmul_alpha(a, T, ol, x);
+      xform(Wht{});
+      qreduce<LO>(x);
+      store(MODE == TM_WAW ? a.out1 : a.out2);
+    }
+    __syncthreads();  // the next tile's first exchange reuses the LDS
+  }
+}
+
+template <int MODE, bool PACK, int LO>
+__global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(4))) void k_qtile(TileArgs a) {
+  qtile_body<MODE, PACK, LO, false>(a);
+}
+template <int MODE, bool PACK, int LO>
+__global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_qtile_wide(TileArgs a) {
+  qtile_body<MODE, PACK, LO, true>(a);
+}
+
 int device_cus() {
   static int cus = 0;
   if (!cus) {
@@ -522,6 +947,7 @@ struct TileShape {
   bool wide;
   int per_cu;
   bool xcd;
+  bool quad = false;  // the quad-split kernels (k_qtile) instead of the register-radix ones (k_tile)
 };
 
 bool xcd_applicable(const TileArgs& a, uint64_t wgs) {
@@ -545,7 +971,11 @@ void launch_shape(int mode, bool pack, TileArgs a, const TileShape& sh, hipStrea
   const int lo_ct = a.xhi == 12 && (a.xlo == 0 || a.xlo == 2 || a.xlo == 7) ? a.xlo : -1;
 #define QF_TILE_LAUNCH_LO(M, PK, LOV)                                   \
   do {                                                                  \
-    if (sh.wide)                                                        \
+    if (sh.quad && sh.wide)                                             \
+      hipLaunchKernelGGL((k_qtile_wide<M, PK, LOV>), g, b, 0, st, a);    \
+    else if (sh.quad)                                                   \
+      hipLaunchKernelGGL((k_qtile<M, PK, LOV>), g, b, 0, st, a);         \
+    else if (sh.wide)                                                   \
       hipLaunchKernelGGL((k_tile_wide<M, PK, LOV>), g, b, 0, st, a);     \
     else                                                                \
       hipLaunchKernelGGL((k_tile<M, PK, LOV>), g, b, 0, st, a);          \
@@ -576,9 +1006,17 @@ void launch_shape(int mode, bool pack, TileArgs a, const TileShape& sh, hipStrea
 #undef QF_TILE_LAUNCH_LO
 }
 
-TileShape default_shape(int mode) {
+// QF_QTILE=0 keeps every pass on the register-radix kernels.
+bool qtile_on() {
+  const char* e = std::getenv("QF_QTILE");
+  return !(e && e[0] == '0');
+}
+// The quad-split kernels when they apply (their per-thread offsets fit 32
+// bits), else the register-radix ones.
+TileShape default_shape(int mode, bool quad_ok) {
+  if (quad_ok && qtile_on()) return {false, 2, true, true};  // quad_ok: a compile-time bit range
   const bool w = wide_mode(mode);
-  return {w, w ? 1 : 2, true};
+  return {w, w ? 1 : 2, true, false};
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -619,7 +1057,7 @@ bool autotune_on() {
 }
 
 TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L, const TuneKey& key, hipStream_t st) {
-  TuneEntry out{default_shape(mode), {}};
+  TuneEntry out{default_shape(mode, a0.quad_ok), {}};
   const uint64_t n = uint64_t(1) << L;
   const uint64_t out_ext = ((vectors - 1) * a0.out_vstride + n) * 16;
   // one scratch span for the E8 buffers, at the callers' relative offsets
@@ -658,11 +1096,15 @@ TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L,
   // pass over 64-byte tile rows fetches every 128-byte line twice, for no
   // gain in time (the passes it applies to are bound by their VALU and LDS work).
   std::vector<TileShape> cands = {{false, 2, true}, {false, 1, true}, {true, 1, true}, {true, 2, true}};
+  if (a0.quad_ok && qtile_on()) {
+    cands.push_back({false, 2, true, true});
+    cands.push_back({true, 1, true, true});
+  }
   hipEvent_t e0, e1;
   QF_HIP(hipEventCreate(&e0));
   QF_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  const TileShape def = default_shape(mode);
+  const TileShape def = default_shape(mode, a0.quad_ok);
   float def_ms = -1;
   for (const TileShape& sh : cands) {
     launch_shape(mode, pack, a, sh, st);  // warm-up
@@ -679,7 +1121,7 @@ TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L,
     QF_HIP(hipGetLastError());
     out.tried.push_back({sh, t_best});
     if (t_best < best) { best = t_best; out.best = sh; }
-    if (sh.wide == def.wide && sh.per_cu == def.per_cu && sh.xcd == def.xcd) def_ms = t_best;
+    if (sh.wide == def.wide && sh.per_cu == def.per_cu && sh.xcd == def.xcd && sh.quad == def.quad) def_ms = t_best;
   }
   if (def_ms > 0 && best > def_ms / 1.05f) out.best = def;
   (void)hipEventDestroy(e0);
@@ -699,17 +1141,17 @@ void load_tune_file(std::map<TuneKey, TuneEntry>& tab) {
   FILE* fp = std::fopen(f, "r");
   if (!fp) return;
   TuneKey k{};
-  int pack = 0, xvec = 0, in_place = 0, wide = 0, per_cu = 0, xcd = 0;
+  int pack = 0, xvec = 0, in_place = 0, wide = 0, per_cu = 0, xcd = 0, quad = 0;
   unsigned long long v = 0, iv = 0, ov = 0;
-  while (std::fscanf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d", &k.mode, &k.L, &k.s, &k.t, &pack, &xvec,
-                     &v, &iv, &ov, &k.out2_rel, &in_place, &wide, &per_cu, &xcd) == 14) {
+  while (std::fscanf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d", &k.mode, &k.L, &k.s, &k.t, &pack,
+                     &xvec, &v, &iv, &ov, &k.out2_rel, &in_place, &wide, &per_cu, &xcd, &quad) == 15) {
     k.pack = pack;
     k.xvec = xvec;
     k.in_place = in_place;
     k.vectors = v;
     k.in_vs = iv;
     k.out_vs = ov;
-    if (per_cu == 1 || per_cu == 2) tab[k] = TuneEntry{{wide != 0, per_cu, xcd != 0}, {}};
+    if (per_cu == 1 || per_cu == 2) tab[k] = TuneEntry{{wide != 0, per_cu, xcd != 0, quad != 0}, {}};
   }
   std::fclose(fp);
 }
@@ -719,10 +1161,10 @@ void save_tune_file(const std::map<TuneKey, TuneEntry>& tab) {
   FILE* fp = std::fopen(f, "w");
   if (!fp) return;
   for (auto& [k, e] : tab)
-    std::fprintf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d\n", k.mode, k.L, k.s, k.t, int(k.pack),
+    std::fprintf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d\n", k.mode, k.L, k.s, k.t, int(k.pack),
                  int(k.xvec), (unsigned long long)k.vectors, (unsigned long long)k.in_vs,
                  (unsigned long long)k.out_vs, k.out2_rel, int(k.in_place), int(e.best.wide), e.best.per_cu,
-                 int(e.best.xcd));
+                 int(e.best.xcd), int(e.best.quad));
   std::fclose(fp);
 }
 
@@ -731,7 +1173,7 @@ void save_tune_file(const std::map<TuneKey, TuneEntry>& tab) {
 thread_local bool t_tune_allowed = true;
 
 TileShape shape_for(int mode, bool pack, const TileArgs& a, uint64_t vectors, int L, bool xvec, hipStream_t st) {
-  if (!autotune_on()) return default_shape(mode);
+  if (!autotune_on()) return default_shape(mode, a.quad_ok);
   const uint64_t n = uint64_t(1) << L;
   TuneKey key{mode, L, a.s, a.t, pack, xvec, vectors, a.in_vstride, a.out_vstride,
               !a.out2 ? 0 : (a.out2 == a.out1 + n ? 1 : 2), !pack && a.in == static_cast<const void*>(a.out1)};
@@ -743,7 +1185,7 @@ TileShape shape_for(int mode, bool pack, const TileArgs& a, uint64_t vectors, in
     loaded = true;
   }
   auto it = tab.find(key);
-  if (it == tab.end() && !t_tune_allowed) return default_shape(mode);
+  if (it == tab.end() && !t_tune_allowed) return default_shape(mode, a.quad_ok);
   if (it == tab.end()) {
     it = tab.emplace(key, tune(mode, pack, a, vectors, L, key, st)).first;
     save_tune_file(tab);
@@ -794,6 +1236,40 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
   if (thread_max + 16 >= (1ull << 32)) return false;
   a.xlo = a.logC;
   a.xhi = xform_vectors ? 12 : a.logC + a.t;
+  // the quad-split mappings' thread parts: tile bits 0..6 (H) and 0, 1, 7..11 (L)
+  const uint64_t qmax = std::max({tile_off64(a, 0x7f, in_vstride), tile_off64(a, 0xf83, in_vstride),
+                                  tile_off64(a, 0x7f, out_vstride), tile_off64(a, 0xf83, out_vstride),
+                                  tile_off64(a, 0xf83, 0)});
+  a.quad_ok = qmax * 16 + 16 < (1ull << 32) && (!pack || tile_off64(a, 0xf83, a.pack_n) * 8 + 16 < (1ull << 32));
+  // the quad kernels' runtime-range forms (partial ranges) run short of registers: the
+  // register-radix kernels take those geometries
+  a.quad_ok = a.quad_ok && a.xhi == 12 && (a.xlo == 0 || a.xlo == 2 || a.xlo == 7);
+  // per-slot strides; the quad kernels address slot q as q x stride
+  for (int m = 0; m < 2; m++) {
+    const int one = m == 0 ? qe_slot<QMAP_H>(1) : qe_slot<QMAP_L>(1);
+    a.qs_in[m] = tile_off64(a, one, pack ? a.pack_n : in_vstride);
+    a.qs_out[m] = tile_off64(a, one, out_vstride);
+    for (int q = 0; q < QN; q++) {
+      const int e = m == 0 ? qe_slot<QMAP_H>(q) : qe_slot<QMAP_L>(q);
+      if (tile_off64(a, e, pack ? a.pack_n : in_vstride) != uint64_t(q) * a.qs_in[m] ||
+          tile_off64(a, e, out_vstride) != uint64_t(q) * a.qs_out[m] ||
+          (m == 1 && tile_off64(a, e, 0) != uint64_t(q) * tile_off64(a, qe_slot<QMAP_L>(1), 0)))
+        a.quad_ok = 0;
+    }
+  }
+  a.qs_alpha = tile_off64(a, qe_slot<QMAP_L>(1), 0);
+  // ... as byte offsets below 2^31 from the tile's base (the buffer range)
+  {
+    const uint64_t lim = 1ull << 31;
+    const uint64_t in_w = pack ? 8 : 16, in_vs = pack ? a.pack_n : in_vstride;
+    const uint64_t th[2] = {0x7f, 0xf83};
+    for (int m = 0; m < 2; m++) {
+      if (tile_off64(a, int(th[m]), in_vs) * in_w + 31 * in_w * a.qs_in[m] + 16 >= lim) a.quad_ok = 0;
+      if (tile_off64(a, int(th[m]), out_vstride) * 16 + 31 * 16 * a.qs_out[m] + 16 >= lim) a.quad_ok = 0;
+    }
+    if (tile_off64(a, 0xf83, 0) * 16 + 31 * 16 * a.qs_alpha + 16 >= lim) a.quad_ok = 0;
+    a.pack_span = pack ? 8 * tile_off64(a, TN - 1, a.pack_n) + 8 : 0;
+  }
   const uint64_t tiles = (vectors << L) / TN;
   if (tiles == 0 || tiles >= (1ull << 31)) return false;
   a.tiles = uint32_t(tiles);
@@ -803,6 +1279,7 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
 }
 
 void set_tuning_allowed(bool allowed) { t_tune_allowed = allowed; }
+bool quad_tiles_enabled() { return qtile_on(); }
 
 // JSON array of the autotuner's choices so far (this library build).
 std::string autotune_report() {
@@ -810,7 +1287,7 @@ std::string autotune_report() {
   std::string o = "[";
   auto shape = [](const TileShape& s) {
     return std::string("{\"wide\":") + (s.wide ? "true" : "false") + ",\"per_cu\":" + std::to_string(s.per_cu) +
-           ",\"xcd\":" + (s.xcd ? "true" : "false") + "}";
+           ",\"xcd\":" + (s.xcd ? "true" : "false") + ",\"quad\":" + (s.quad ? "true" : "false") + "}";
   };
   bool first = true;
   for (auto& [k, e] : tune_table()) {

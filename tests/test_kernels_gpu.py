@@ -1,12 +1,29 @@
-"""Kernel-level parity: the register-radix transform passes (qf_tile.hip)
-against numpy restatements of the same 2x2 tensor maps, over F65521/Ext8
-(libparcomm_amd.so) and KoalaBear/Ext4 (libparcomm_amd_kb.so)."""
+"""Kernel-level parity: the transform passes (qf_tile.hip) of both kernel
+families, register-radix and quad-split, against numpy restatements of the
+same 2x2 tensor maps, over F65521/Ext8 (libparcomm_amd.so) and KoalaBear/Ext4
+(libparcomm_amd_kb.so). Each family is forced with the autotuner off
+(QF_AUTOTUNE=0; QF_QTILE=0 keeps the register-radix kernels, and the
+quad-split ones take every geometry they support)."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=["radix", "quad"])
+def family(request):
+    saved = {k: os.environ.get(k) for k in ("QF_AUTOTUNE", "QF_QTILE")}
+    os.environ["QF_AUTOTUNE"] = "0"
+    os.environ["QF_QTILE"] = "1" if request.param == "quad" else "0"
+    yield request.param
+    for k, v in saved.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
 # field -> (p, extension degree, X^degree wrap constant, coefficient dtype)
 FIELDS = {0: (65521, 8, 17, np.uint16), 1: (2130706433, 4, 3, np.uint32)}
 FIELD_IDS = {0: "f65521", 1: "koalabear"}
@@ -83,12 +100,13 @@ def field_values(rng, P, shape):
 @pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
 @pytest.mark.parametrize("vectors,L,s,t", GEOMS)
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6])
-def test_tile_pass_matches_numpy(vectors, L, s, t, mode, field):
+def test_tile_pass_matches_numpy(vectors, L, s, t, mode, field, family):
     P, lanes, _, _ = FIELDS[field]
     rng = np.random.default_rng(1000 * L + 10 * mode + s + 7 * field)
     x = field_values(rng, P, (vectors, 1 << L, lanes))
     alpha = field_values(rng, P, (1 << L, lanes))
-    print("tile pass", FIELD_IDS[field], "mode", mode, "vectors", vectors, "L", L, "bits", s, s + t, flush=True)
+    print("tile pass", family, FIELD_IDS[field], "mode", mode, "vectors", vectors, "L", L, "bits", s, s + t,
+          flush=True)
     o1, o2 = run(field, mode, x, alpha if mode in (3, 5, 6) else None, L, s, t,
                  vstride_in=(1 << L) + 64 if L < 20 else None)
     bits = list(range(s, s + t))
@@ -123,7 +141,7 @@ PLAN23 = [(1, 23, 7, 8), (1, 23, 15, 8), (1, 23, 12, 3), (1, 23, 12, 6), (1, 23,
 @pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
 @pytest.mark.parametrize("vectors,L,s,t", PLAN23)
 @pytest.mark.parametrize("mode", [0, 1, 2])
-def test_tile_pass_commit_plan_geometries(vectors, L, s, t, mode, field):
+def test_tile_pass_commit_plan_geometries(vectors, L, s, t, mode, field, family):
     P, lanes, _, _ = FIELDS[field]
     rng = np.random.default_rng(3000 + 10 * s + t + mode + 7 * field)
     x = field_values(rng, P, (vectors, 1 << L, lanes))
@@ -159,7 +177,7 @@ def transform_vectors(x, nbits, wht, P):
 @pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
 @pytest.mark.parametrize("L", [12, 15])
 @pytest.mark.parametrize("mode", [0, 2])
-def test_tile_pass_over_column_bits(L, mode, field):
+def test_tile_pass_over_column_bits(L, mode, field, family):
     # 32 columns x 128 rows per tile: the first commit pass (column bits + row bits 0..6)
     P, lanes, _, _ = FIELDS[field]
     rng = np.random.default_rng(77 + L + mode + field)
@@ -173,7 +191,7 @@ def test_tile_pass_over_column_bits(L, mode, field):
 
 @pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
 @pytest.mark.parametrize("mode", [0, 2])
-def test_column_bit_pass_with_wide_vector_stride(mode, field):
+def test_column_bit_pass_with_wide_vector_stride(mode, field, family):
     # The first commit pass of a 2^30-entry KoalaBear block: 32 codeword
     # columns 2^24 elements (256 MiB) apart, so a tile spans 8 GiB and the
     # per-slot offsets need 64 bits. Only the first 2^12 elements of every
