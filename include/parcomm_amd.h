@@ -92,9 +92,12 @@ int qf_committed_roots(const qf_committed* c, uint8_t* out, size_t cap_blocks);
 int qf_committed_codeword(const qf_committed* c, uint64_t block, void* out, size_t cap);
 
 /* Device time in milliseconds of the commit's kernels (HIP events on the
- * commit stream), summed over the encoded blocks: all work, the leaf-hash
- * kernel, the encoding before the leaves, and the Mobius+WHT pass of the last
- * high row group; timed_blocks counts those blocks. Any pointer may be null. */
+ * streams they run on), summed over the encoded blocks: all work, the
+ * leaf-hash kernel, the encoding before the leaves, and the Mobius+WHT pass
+ * of the last high row group; timed_blocks counts those blocks. A commit that
+ * replayed a captured HIP graph (the second and later commits of a parameter
+ * set, unless QF_COMMIT_GRAPH=0) reports total_ms only, with timed_blocks 0.
+ * Any pointer may be null. */
 int qf_committed_timing(const qf_committed* c, double* total_ms, double* leaf_ms, double* encode_ms, double* mw_ms,
                         uint64_t* timed_blocks);
 
@@ -129,6 +132,17 @@ int qf_prove(const qf_params* p, const qf_committed* c, const uint8_t* claims, s
  * rejected check in qf_last_error(). Host only; needs no device. */
 int qf_verify(const qf_params* p, const uint8_t* roots, size_t n_roots, const uint8_t* claims, size_t claims_len,
               const uint8_t* proof, size_t proof_len);
+
+/* Batch verification (no reference counterpart; the reference verifies one
+ * proof per call): `count` proofs against the same parameters and public
+ * roots, proof i with its own packed claims, each checked exactly as by
+ * qf_verify, spread over up to `threads` host threads (0: one per hardware
+ * thread). statuses[i] receives proof i's qf_verify code. Returns QF_OK when
+ * every proof is accepted, else the first rejected proof's code, with its
+ * index and check in qf_last_error(). Host only. */
+int qf_verify_batch(const qf_params* p, const uint8_t* roots, size_t n_roots, const uint8_t* const* claims,
+                    const size_t* claims_lens, const uint8_t* const* proofs, const size_t* proof_lens, size_t count,
+                    int threads, int* statuses);
 
 /* Proof bytes, valid while the handle lives. */
 int qf_proof_bytes(const qf_proof* pr, const uint8_t** data, size_t* len);
@@ -181,6 +195,16 @@ int qf_encode(const qf_params* p, const void* const* matrices, const uint64_t* l
  * every matrix once gives the same BVA as qf_encode. Synchronous. */
 int qf_encode_matrix(const qf_params* p, uint64_t matrix, const void* values, uint64_t len, int on_device,
                      void* device_bva);
+
+/* The reference CLI's dense BVA input (main.rs:145-173: the
+ * parcomm-dense-bva-stream produced by modelconvert.py, entries in BVA order,
+ * int8 or little-endian int16): reads exactly active entries from the file at
+ * `path` ("-" for standard input) straight into the device BVA, through two
+ * pinned staging buffers so reading and uploading overlap. Input that ends
+ * early ("BVA input ends early") or runs past the active length ("excess BVA
+ * input") fails with QF_ERR_ENCODING. *bytes_read (optional) receives the
+ * bytes consumed. Synchronous. */
+int qf_stream_bva(const qf_params* p, const char* path, void* device_bva, uint64_t* bytes_read);
 
 /* Synthetic workload generator (no reference counterpart; the reference CLI
  * streams real checkpoints): fills the active BVA in device memory, from a

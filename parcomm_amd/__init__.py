@@ -93,11 +93,14 @@ def lib(field=QF_FIELD_F65521):
         L.qf_synthetic_bva.argtypes = [vp, u64, ctypes.c_float, vp]
         L.qf_encode.argtypes = [vp, P(vp), P(u64), sz, i32, vp]
         L.qf_encode_matrix.argtypes = [vp, u64, vp, u64, i32, vp]
+        L.qf_stream_bva.argtypes = [vp, ctypes.c_char_p, vp, P(u64)]
         L.qf_claims_size.argtypes = [vp, P(sz)]
         L.qf_benchmark_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
         L.qf_prove.argtypes = [vp, vp, ctypes.c_char_p, sz, P(vp)]
         L.qf_evaluate_claims.argtypes = [vp, vp, ctypes.c_char_p, sz]
         L.qf_verify.argtypes = [vp, ctypes.c_char_p, sz, ctypes.c_char_p, sz, ctypes.c_char_p, sz]
+        L.qf_verify_batch.argtypes = [vp, ctypes.c_char_p, sz, P(ctypes.c_char_p), P(sz), P(ctypes.c_char_p), P(sz),
+                                      sz, i32, P(i32)]
         L.qf_proof_bytes.argtypes = [vp, P(P(ctypes.c_uint8)), P(sz)]
         L.qf_proof_timing.argtypes = [vp, P(ctypes.c_double)]
         L.qf_proof_free.argtypes = [vp]
@@ -452,6 +455,21 @@ class Parameters:
             arr = np.ascontiguousarray(matrix, dtype=self.entry_dtype)
             self._check(self._L.qf_encode_matrix(self._h, index, arr.ctypes.data, arr.size, 0, out.ptr))
         return out
+
+    def verify_batch(self, statement: "Commitment", items, threads=0):
+        """Batch verification of (claims, proof) pairs against one statement on
+        the host's threads: the list of qf_verify status codes (0 = accepted)."""
+        n = len(items)
+        raw_claims = [self.claims_to_bytes(c) for c, _ in items]
+        proofs = [bytes(pr) for _, pr in items]
+        cl = (ctypes.c_char_p * max(n, 1))(*raw_claims)
+        cll = (ctypes.c_size_t * max(n, 1))(*[len(x) for x in raw_claims])
+        pp = (ctypes.c_char_p * max(n, 1))(*proofs)
+        ppl = (ctypes.c_size_t * max(n, 1))(*[len(x) for x in proofs])
+        st = (ctypes.c_int * max(n, 1))()
+        roots = b"".join(statement.roots)
+        self._L.qf_verify_batch(self._h, roots, len(statement.roots), cl, cll, pp, ppl, n, threads, st)
+        return [st[i] for i in range(n)]
 
     def synthetic_bva(self, seed, sigma=None):
         """Device-resident synthetic BVA on physical entries: clip(round(N(0, sigma)))

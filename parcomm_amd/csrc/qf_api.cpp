@@ -4,7 +4,11 @@
 #include <algorithm>
 #include <cstring>
 #include "qf_vec.hpp"
+#include <map>
 #include <memory>
+#include <string>
+#include <thread>
+#include <atomic>
 
 using namespace qf;
 
@@ -94,11 +98,54 @@ void validate_device(const ParamsState* p, const Val* d_bva, hipStream_t st) {
   require(flag == 0, QF_ERR_INVALID_PARAMETERS, "nonzero boundary padding");
 }
 
+// This thread's timing events, pinned root buffer and captured commit graphs,
+// kept for the life of the thread.
+struct CommitScratch {
+  std::vector<hipEvent_t> ev;
+  uint8_t* roots = nullptr;  // pinned, 128 x 32 bytes
+  // captured block loops by everything their launches depend on: geometry,
+  // BVA, cache mode and the device buffers of every block
+  std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
+  std::vector<std::vector<uint64_t>> graph_order;  // oldest first; at most 8 kept
+  ~CommitScratch() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (roots) (void)hipHostFree(roots);
+  }
+  hipEvent_t* events(size_t n) {
+    while (ev.size() < n) {
+      hipEvent_t e;
+      QF_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+    return ev.data();
+  }
+};
+thread_local CommitScratch t_commit;
+
+// QF_COMMIT_GRAPH=0 issues every commit's launches one by one.
+bool commit_graph_on() {
+  const char* e = std::getenv("QF_COMMIT_GRAPH");
+  return !(e && e[0] == '0');
+}
+
+// The block loop of protocol.rs:162-185 as a two-stream pipeline: the work
+// stream encodes block b + 1 while the second stream hashes block b, so the
+// tile passes and the leaf hash share the CUs. Roots are copied to pinned
+// memory on the hashing stream and read once at the end. Recompute mode
+// alternates two workspaces. From the second commit of a parameter set on
+// (the first one runs the autotuner), the loop is captured once as a HIP
+// graph per buffer set and replayed with one launch; the graph holds no
+// memset nodes.
 void commit_impl(const qf_params* p, const Val* d_bva, size_t len, int cache_mode, qf_committed** out) {
   require(len == p->layout.active, QF_ERR_INVALID_PARAMETERS, "active BVA length mismatch");
   require(cache_mode == QF_CACHE_CACHED || cache_mode == QF_CACHE_RECOMPUTE, QF_ERR_ARGUMENT, "cache mode");
-  hipStream_t st = thread_stream();
+  hipStream_t st = thread_stream(), st2 = thread_stream2();
   std::unique_ptr<qf_committed> c(new qf_committed);
+  const bool cached = cache_mode == QF_CACHE_CACHED;
+  E8* ws_cw[2] = {nullptr, nullptr};
+  uint32_t* ws_tree[2] = {nullptr, nullptr};
+  bool capturing = false;
   try {
     c->p = p;
     c->cache_mode = cache_mode;
@@ -107,54 +154,135 @@ void commit_impl(const qf_params* p, const Val* d_bva, size_t len, int cache_mod
     const uint64_t n = uint64_t(1) << p->geom.log_n, R = 2 * n;
     const size_t cw_bytes = size_t(32 * R) * 16, tree_bytes = size_t(2 * R - 1) * 32;
     const E8* alpha = p->alpha_encode(p->pp.codes.size() - 1);
-    hipEvent_t e0, e1, ev[6];
-    QF_HIP(hipEventCreate(&e0)); QF_HIP(hipEventCreate(&e1));
-    for (auto& e : ev) QF_HIP(hipEventCreate(&e));
-    QF_HIP(hipEventRecord(e0, st));
+    // blocks to encode; the all-zero blocks past the data share the first one's commitment
+    std::vector<uint64_t> enc;
+    std::vector<size_t> src(p->blocks);
     int zero_block = -1;
     for (uint64_t b = 0; b < p->blocks; b++) {
-      uint64_t first = b * p->block_len;
-      if (first >= len && zero_block >= 0) {  // identical all-zero blocks share one commitment
-        c->roots.push_back(c->roots[zero_block]);
-        if (cache_mode == QF_CACHE_CACHED) { c->d_cw.push_back(c->d_cw[zero_block]); c->d_tree.push_back(c->d_tree[zero_block]); }
-        continue;
-      }
-      E8* cw = nullptr;
-      uint32_t* tree = nullptr;
-      if (cache_mode == QF_CACHE_CACHED || c->d_cw.empty()) {
-        cw = static_cast<E8*>(dmalloc(cw_bytes, st));
-        tree = static_cast<uint32_t*>(dmalloc(tree_bytes, st));
-        c->d_cw.push_back(cw);
-        c->d_tree.push_back(tree);
-      } else {
-        cw = c->d_cw[0];
-        tree = c->d_tree[0];
-      }
-      launch_commit_block(p->geom, d_bva, first, len, cw, alpha, tree, st, ev);
-      QF_HIP(hipGetLastError());
-      Digest root;
-      QF_HIP(hipMemcpyAsync(root.data(), tree + 8 * (2 * R - 2), 32, hipMemcpyDeviceToHost, st));
-      QF_HIP(hipStreamSynchronize(st));
-      float a = 0, b2 = 0, m = 0;
-      QF_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-      QF_HIP(hipEventElapsedTime(&b2, ev[1], ev[2]));
-      QF_HIP(hipEventElapsedTime(&m, ev[4], ev[5]));
-      c->encode_ms += a;
-      c->leaf_ms += b2;
-      c->mw_ms += m;
-      c->timed_blocks++;
-      c->roots.push_back(root);
+      const uint64_t first = b * p->block_len;
+      if (first >= len && zero_block >= 0) { src[b] = src[zero_block]; continue; }
+      src[b] = enc.size();
+      enc.push_back(b);
       if (first >= len) zero_block = int(b);
     }
-    QF_HIP(hipEventRecord(e1, st));
-    QF_HIP(hipEventSynchronize(e1));
+    if (!t_commit.roots) QF_HIP(hipHostMalloc(&t_commit.roots, 128 * 32, hipHostMallocDefault));
+    // per encoded block: 0 start, 1 encoded, 2 leaf start, 3 leaf end, 4/5 around M_W, 6 tree and root done
+    hipEvent_t* ev = t_commit.events(7 * enc.size() + 2);
+    hipEvent_t e_start = ev[7 * enc.size()], e_end = ev[7 * enc.size() + 1];
+    // every device buffer before any launch (allocations stay outside a captured graph)
+    std::vector<E8*> blk_cw(enc.size());
+    std::vector<uint32_t*> blk_tree(enc.size());
+    for (size_t k = 0; k < enc.size(); k++) {
+      if (cached) {  // owned by the handle from here on (freed with it on any failure)
+        blk_cw[k] = static_cast<E8*>(dmalloc(cw_bytes, st));
+        c->d_cw.push_back(blk_cw[k]);
+        blk_tree[k] = static_cast<uint32_t*>(dmalloc(tree_bytes, st));
+        c->d_tree.push_back(blk_tree[k]);
+      } else {
+        const int w = int(k % 2);
+        if (!ws_cw[w]) {
+          ws_cw[w] = static_cast<E8*>(dmalloc(cw_bytes, st));
+          ws_tree[w] = static_cast<uint32_t*>(dmalloc(tree_bytes, st));
+        }
+        blk_cw[k] = ws_cw[w];
+        blk_tree[k] = ws_tree[w];
+      }
+    }
+    auto issue = [&] {
+      for (size_t k = 0; k < enc.size(); k++) {
+        const uint64_t b = enc[k];
+        hipEvent_t* E = ev + 7 * k;
+        if (!cached && k >= 2) QF_HIP(hipStreamWaitEvent(st, ev[7 * (k - 2) + 6], 0));  // workspace hashed
+        QF_HIP(hipEventRecord(E[0], st));
+        launch_commit_encode(p->geom, d_bva, b * p->block_len, len, blk_cw[k], alpha, st, E + 4);
+        QF_HIP(hipEventRecord(E[1], st));
+        QF_HIP(hipStreamWaitEvent(st2, E[1], 0));
+        QF_HIP(hipEventRecord(E[2], st2));
+        launch_tree(blk_cw[k], R, p->geom.leaf, p->geom.inner, blk_tree[k], st2, E[3]);
+        QF_HIP(hipMemcpyAsync(t_commit.roots + 32 * k, blk_tree[k] + 8 * (2 * R - 2), 32, hipMemcpyDeviceToHost, st2));
+        QF_HIP(hipEventRecord(E[6], st2));
+        QF_HIP(hipGetLastError());
+      }
+      if (!enc.empty()) QF_HIP(hipStreamWaitEvent(st, ev[7 * (enc.size() - 1) + 6], 0));  // join
+    };
+    const bool graph = commit_graph_on() && p->commit_warm.load();
+    c->graphed = graph;
+    if (graph) {
+      std::vector<uint64_t> key = {reinterpret_cast<uintptr_t>(p), reinterpret_cast<uintptr_t>(d_bva), len,
+                                   uint64_t(cache_mode), reinterpret_cast<uintptr_t>(alpha), uint64_t(p->geom.log_n),
+                                   p->blocks, p->block_len};
+      for (size_t k = 0; k < enc.size(); k++) {
+        key.push_back(reinterpret_cast<uintptr_t>(blk_cw[k]));
+        key.push_back(reinterpret_cast<uintptr_t>(blk_tree[k]));
+      }
+      auto it = t_commit.graphs.find(key);
+      if (it == t_commit.graphs.end()) {
+        QF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        capturing = true;
+        set_tuning_allowed(false);  // nothing may allocate or synchronise under capture
+        struct Restore { ~Restore() { set_tuning_allowed(true); } } restore;
+        issue();
+        hipGraph_t g = nullptr;
+        capturing = false;
+        QF_HIP(hipStreamEndCapture(st, &g));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        QF_HIP(ie);
+        if (t_commit.graph_order.size() >= 8) {  // the oldest goes
+          auto old = t_commit.graphs.find(t_commit.graph_order.front());
+          if (old != t_commit.graphs.end()) { (void)hipGraphExecDestroy(old->second); t_commit.graphs.erase(old); }
+          t_commit.graph_order.erase(t_commit.graph_order.begin());
+        }
+        it = t_commit.graphs.emplace(key, ex).first;
+        t_commit.graph_order.push_back(key);
+      }
+      QF_HIP(hipEventRecord(e_start, st));
+      QF_HIP(hipGraphLaunch(it->second, st));
+      QF_HIP(hipEventRecord(e_end, st));
+    } else {
+      QF_HIP(hipEventRecord(e_start, st));
+      issue();
+      QF_HIP(hipEventRecord(e_end, st));
+    }
+    QF_HIP(hipEventSynchronize(e_end));
+    // A replayed graph does not restamp the events recorded inside it (they
+    // keep the times of the launch that captured them): such a commit reports
+    // its total time only.
+    for (size_t k = 0; k < enc.size() && !graph; k++) {
+      hipEvent_t* E = ev + 7 * k;
+      float a = 0, l = 0, m = 0;
+      QF_HIP(hipEventElapsedTime(&a, E[0], E[1]));
+      QF_HIP(hipEventElapsedTime(&l, E[2], E[3]));
+      QF_HIP(hipEventElapsedTime(&m, E[4], E[5]));
+      c->encode_ms += a;
+      c->leaf_ms += l;
+      c->mw_ms += m;
+      c->timed_blocks++;
+    }
     float ms = 0;
-    QF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    QF_HIP(hipEventElapsedTime(&ms, e_start, e_end));
     c->total_ms = ms;
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (cached) { c->d_cw.clear(); c->d_tree.clear(); }
+    for (uint64_t b = 0; b < p->blocks; b++) {
+      Digest root;
+      std::memcpy(root.data(), t_commit.roots + 32 * src[b], 32);
+      c->roots.push_back(root);
+      if (cached) { c->d_cw.push_back(blk_cw[src[b]]); c->d_tree.push_back(blk_tree[src[b]]); }
+    }
+    for (int w = 0; w < 2; w++)
+      if (ws_cw[w]) { dfree(ws_cw[w], st); dfree(ws_tree[w], st); ws_cw[w] = nullptr; ws_tree[w] = nullptr; }
+    p->commit_warm = true;
   } catch (...) {
+    if (capturing) {  // end the capture before the streams can be used again
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+    }
+    (void)hipStreamSynchronize(st2);
     (void)hipStreamSynchronize(st);
+    for (int w = 0; w < 2; w++)
+      if (ws_cw[w]) { dfree(ws_cw[w], st); dfree(ws_tree[w], st); }
     throw;
   }
   *out = c.release();
@@ -369,6 +497,45 @@ int qf_verify(const qf_params* p, const uint8_t* roots, size_t n_roots, const ui
     std::vector<Digest> rs(n_roots);
     for (size_t i = 0; i < n_roots; i++) std::memcpy(rs[i].data(), roots + 32 * i, 32);
     verify(*p, rs, parse_claims(*p, claims), proof, proof_len);
+  });
+}
+
+int qf_verify_batch(const qf_params* p, const uint8_t* roots, size_t n_roots, const uint8_t* const* claims,
+                    const size_t* claims_lens, const uint8_t* const* proofs, const size_t* proof_lens, size_t count,
+                    int threads, int* statuses) {
+  return guarded([&] {
+    require(p && roots && statuses && (count == 0 || (claims && claims_lens && proofs && proof_lens)), QF_ERR_ARGUMENT,
+            "null argument");
+    std::vector<Digest> rs(n_roots);
+    for (size_t i = 0; i < n_roots; i++) std::memcpy(rs[i].data(), roots + 32 * i, 32);
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t workers = std::max<size_t>(1, std::min<size_t>(count, threads > 0 ? size_t(threads) : (hw ? hw : 1)));
+    std::vector<std::string> msgs(count);
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+      set_verify_rows_parallel(workers == 1);  // one proof per thread: rows hash serially on each
+      for (size_t i; (i = next++) < count;) {
+        try {
+          require(proofs[i] && claims[i], QF_ERR_ARGUMENT, "null proof or claims");
+          require(claims_lens[i] == claims_bytes(*p), QF_ERR_INVALID_PARAMETERS, "claims length mismatch");
+          verify(*p, rs, parse_claims(*p, claims[i]), proofs[i], proof_lens[i]);
+          statuses[i] = QF_OK;
+        } catch (const QfError& e) {
+          statuses[i] = e.status;
+          msgs[i] = e.what();
+        } catch (const std::exception& e) {
+          statuses[i] = QF_ERR_ARGUMENT;
+          msgs[i] = e.what();
+        }
+      }
+      set_verify_rows_parallel(true);
+    };
+    std::vector<std::thread> ts;
+    for (size_t w = 1; w < workers; w++) ts.emplace_back(work);
+    work();
+    for (auto& t : ts) t.join();
+    for (size_t i = 0; i < count; i++)
+      if (statuses[i] != QF_OK) throw QfError(statuses[i], "proof " + std::to_string(i) + ": " + msgs[i]);
   });
 }
 

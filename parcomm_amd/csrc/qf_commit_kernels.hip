@@ -521,17 +521,16 @@ void launch_encode_form(const E8* in, E8* out, uint64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_encode_form, dim3(uint32_t(std::max<uint64_t>(g, 1))), dim3(WG), 0, st, in, out, n);
 }
 
-void launch_commit_block(const CommitGeom& g, const Val* values, uint64_t first_entry, uint64_t active, E8* cw,
-                         const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev) {
+void launch_commit_encode(const CommitGeom& g, const Val* values, uint64_t first_entry, uint64_t active, E8* cw,
+                          const E8* alpha, hipStream_t st, hipEvent_t* mw) {
   const int L = g.log_n;
   // the packing kernels address the BVA in bytes
   const uint8_t* bva = reinterpret_cast<const uint8_t*>(values);
   const uint64_t block_first = first_entry * sizeof(Val), available = active * sizeof(Val);
-  if (ev) QF_HIP(hipEventRecord(ev[0], st));
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  if (ev) QF_HIP(hipEventRecord(ev[4], st));  // both M_W events stay here if no tiled M_W pass runs
-  if (ev) QF_HIP(hipEventRecord(ev[5], st));
-  if (!tiled_commit_rows(true, bva, block_first, available, cw, L, alpha, st, ev ? ev + 4 : nullptr)) {
+  if (mw) QF_HIP(hipEventRecord(mw[0], st));  // both M_W events stay here if no tiled M_W pass runs
+  if (mw) QF_HIP(hipEventRecord(mw[1], st));
+  if (!tiled_commit_rows(true, bva, block_first, available, cw, L, alpha, st, mw)) {
     const int tp = L < 12 ? L : 12;
     // 1. pack + Mobius on the low bits within each column
     uint64_t packed = n * 32;
@@ -544,6 +543,13 @@ void launch_commit_block(const CommitGeom& g, const Val* values, uint64_t first_
     // 3. parity
     launch_encode_parity(cw, 32, L, alpha, st);
   }
+}
+
+void launch_commit_block(const CommitGeom& g, const Val* values, uint64_t first_entry, uint64_t active, E8* cw,
+                         const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev) {
+  const uint64_t R = 2 * (uint64_t(1) << g.log_n);
+  if (ev) QF_HIP(hipEventRecord(ev[0], st));
+  launch_commit_encode(g, values, first_entry, active, cw, alpha, st, ev ? ev + 4 : nullptr);
   // 4. leaves and inner levels
   if (ev) QF_HIP(hipEventRecord(ev[1], st));
   launch_tree(cw, R, g.leaf, g.inner, tree, st, ev ? ev[2] : nullptr);

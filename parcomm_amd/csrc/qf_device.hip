@@ -1,6 +1,8 @@
 // Device buffers and the synthetic BVA generator (plumbing for resident inputs).
 #include "../../include/parcomm_amd.h"
 #include "qf_kernels.hpp"
+#include <cstdio>
+#include <cstring>
 
 using namespace qf;
 
@@ -291,6 +293,53 @@ int qf_encode_matrix(const qf_params* p, uint64_t matrix, const void* values, ui
       throw;
     }
     if (stage) dfree(stage, st);
+  });
+}
+
+int qf_stream_bva(const qf_params* p, const char* path, void* device_bva, uint64_t* bytes_read) {
+  return guard([&] {
+    require(p && path && device_bva, QF_ERR_ARGUMENT, "null argument");
+    const Layout& l = params_layout(p);
+    const uint64_t total = l.active * sizeof(Val);
+    device_setup();
+    hipStream_t st = thread_stream();
+    FILE* f = std::strcmp(path, "-") == 0 ? stdin : std::fopen(path, "rb");
+    require(f != nullptr, QF_ERR_ARGUMENT, "cannot open the BVA input");
+    // two pinned staging buffers: the file is read into one while the other uploads
+    constexpr size_t CHUNK = size_t(64) << 20;
+    uint8_t* stage[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    auto cleanup = [&] {
+      (void)hipStreamSynchronize(st);
+      for (int i = 0; i < 2; i++) {
+        if (stage[i]) (void)hipHostFree(stage[i]);
+        if (done[i]) (void)hipEventDestroy(done[i]);
+      }
+      if (f != stdin) std::fclose(f);
+    };
+    try {
+      for (int i = 0; i < 2; i++) {
+        QF_HIP(hipHostMalloc(&stage[i], CHUNK, hipHostMallocDefault));
+        QF_HIP(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+      }
+      uint64_t off = 0;
+      for (int k = 0; off < total; k ^= 1) {
+        QF_HIP(hipEventSynchronize(done[k]));  // the upload from this buffer has finished
+        const size_t want = size_t(std::min<uint64_t>(CHUNK, total - off));
+        const size_t got = std::fread(stage[k], 1, want, f);
+        off += got;
+        if (bytes_read) *bytes_read = off;
+        require(got == want, QF_ERR_ENCODING, "BVA input ends early");  // main.rs:160-169 (read_exact)
+        QF_HIP(hipMemcpyAsync(static_cast<uint8_t*>(device_bva) + off - got, stage[k], got, hipMemcpyHostToDevice, st));
+        QF_HIP(hipEventRecord(done[k], st));
+      }
+      uint8_t extra = 0;
+      require(std::fread(&extra, 1, 1, f) == 0, QF_ERR_ENCODING, "excess BVA input");  // main.rs:170-173
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
   });
 }
 

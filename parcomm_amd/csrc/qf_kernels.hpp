@@ -22,6 +22,9 @@ hipStream_t current_stream();
 // kept for the life of the process): commits and proofs issued from one
 // thread run on it, so pooled blocks are reused in stream order.
 hipStream_t thread_stream();
+// A second persistent stream of this thread (default priority): the commit
+// hashes block b on it while the work stream encodes block b + 1.
+hipStream_t thread_stream2();
 // Once per process, before the library's first device call: host waits on
 // the device spin instead of sleeping (the opening's round trips are short).
 // Has no effect when the process already holds a device context.
@@ -47,6 +50,10 @@ struct CommitGeom {
 // [3] end, [4]/[5] around the Mobius+WHT pass of the last high row group.
 void launch_commit_block(const CommitGeom& g, const Val* bva, uint64_t first_entry, uint64_t active, E8* cw,
                          const E8* alpha, uint32_t* tree, hipStream_t st, hipEvent_t* ev = nullptr);
+// The encoding half of launch_commit_block: pack + Mobius into the message half
+// of `cw` and the parity half, on `st` (mw: optional 2 events around the M_W pass).
+void launch_commit_encode(const CommitGeom& g, const Val* bva, uint64_t first_entry, uint64_t active, E8* cw,
+                          const E8* alpha, hipStream_t st, hipEvent_t* mw = nullptr);
 // out[i] = encode form of in[i] (Montgomery form for KoalaBear, a copy for F65521).
 void launch_encode_form(const E8* in, E8* out, uint64_t n, hipStream_t st);
 // ev (optional, 4 events): recorded at start, before the leaf kernel, after it, and at the end.

@@ -4,6 +4,7 @@
 #pragma once
 #include "qf_kernels.hpp"
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 
 namespace qf {
@@ -22,6 +23,7 @@ struct ParamsState {
   mutable std::vector<E8*> d_setup_enc;       // setup level l's encoded matrix (column-major), on first Decide
   mutable std::vector<uint64_t> zero_ranges;  // padding intervals (start, len) chunked for the device check
   mutable std::mutex mu;                      // guards the lazily filled members above
+  mutable std::atomic<bool> commit_warm{false};  // a commit has run (and tuned its tile passes)
   ~ParamsState() {
     for (auto* a : d_setup_enc) if (a) (void)hipFree(a);
     for (auto* a : d_alpha_enc)
@@ -41,11 +43,12 @@ struct CommittedState {
   const ParamsState* p = nullptr;
   int cache_mode = 0;
   std::vector<Digest> roots;
-  std::vector<E8*> d_cw;        // per block (cached) or one workspace
+  std::vector<E8*> d_cw;        // per block (cached mode; blocks with equal roots share one)
   std::vector<uint32_t*> d_tree;
   const Val* d_bva = nullptr;  // the committed BVA (active entries) in HBM
   bool owns_bva = false;
   double total_ms = 0, leaf_ms = 0, encode_ms = 0, mw_ms = 0;
+  bool graphed = false;  // the block loop ran as a captured HIP graph
   uint64_t timed_blocks = 0;  // blocks actually encoded (zero blocks past the data reuse a root)
   ~CommittedState() {
     std::vector<void*> freed;
@@ -77,6 +80,9 @@ std::string statement_json(const ParamsState& p, const CommittedState& c);
 // Parameters::verify (protocol.rs:266-319): throws QfError on rejection.
 void verify(const ParamsState& p, const std::vector<Digest>& roots, const std::vector<ClaimIn>& claims,
             const uint8_t* proof, size_t len);
+// Whether verify() spreads its row hashing over the host's threads (the
+// default) on the calling thread.
+void set_verify_rows_parallel(bool on);
 std::string claims_json(const std::vector<ClaimIn>& claims);
 
 // Padding intervals of a layout as (start, len) pairs of at most 64 Ki entries

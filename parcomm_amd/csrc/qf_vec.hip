@@ -1501,6 +1501,12 @@ hipStream_t thread_stream() {
   if (!st) QF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   return st;
 }
+hipStream_t thread_stream2() {
+  device_setup();
+  thread_local hipStream_t st = nullptr;
+  if (!st) QF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  return st;
+}
 void* dmalloc(size_t bytes, hipStream_t st) {
   device_setup();
   BlockCache& bc = cache();

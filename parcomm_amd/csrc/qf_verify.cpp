@@ -6,6 +6,7 @@
 // direct-interleaved Decide (qafold_short_decide_driver.rs:320-518 with the
 // helpers of qafold_short_decide*.rs). It reads only the parameters, the
 // public roots, the claims and the proof bytes; all of it runs on the host.
+#include <thread>
 #include "qf_protocol.hpp"
 
 namespace qf {
@@ -107,16 +108,36 @@ bool verify_siblings(Digest d, uint64_t index, uint64_t leaves, const std::vecto
   }
   return d == root;
 }
+// Row hashing and Merkle paths, the bulk of a proof's verification, are
+// independent per row: they run on the host's threads. A batch verification,
+// which already runs one proof per thread, turns this off on its threads.
+thread_local bool t_rows_parallel = true;
+template <class F>
+void for_rows(uint64_t n, F&& body) {  // body(first, last)
+  const unsigned hw = std::thread::hardware_concurrency();
+  const uint64_t workers = std::min<uint64_t>(hw ? hw : 1, 16);
+  if (!t_rows_parallel || n < 32 || workers <= 1) { body(uint64_t(0), n); return; }
+  std::vector<std::thread> ts;
+  const uint64_t chunk = (n + workers - 1) / workers;
+  for (uint64_t w = 0; w < workers; w++) {
+    const uint64_t a = w * chunk, b = std::min(n, a + chunk);
+    if (a < b) ts.emplace_back([&body, a, b] { body(a, b); });
+  }
+  for (auto& t : ts) t.join();
+}
 Digest tree_root(const std::vector<E8>& entries, uint64_t R, uint64_t cols, const char* domain) {
   std::vector<Digest> leaves(R);
-  std::vector<E8> row(cols);
-  for (uint64_t r = 0; r < R; r++) {
-    for (uint64_t c = 0; c < cols; c++) row[c] = entries[c * R + r];
-    leaves[r] = hash_row(row.data(), cols, domain);
-  }
+  for_rows(R, [&](uint64_t a, uint64_t b) {
+    std::vector<E8> row(cols);
+    for (uint64_t r = a; r < b; r++) {
+      for (uint64_t c = 0; c < cols; c++) row[c] = entries[c * R + r];
+      leaves[r] = hash_row(row.data(), cols, domain);
+    }
+  });
   return merkle_levels(std::move(leaves)).back()[0];
 }
-// Authenticates a payload against `root` for the selected rows.
+// Authenticates a payload against `root` for the selected rows; the first
+// failing row (in row order) names the rejection.
 void authenticate(const RowsPayload& pl, const Sel& sel, const std::vector<uint64_t>& rows, uint64_t R, uint64_t cols,
                   const Digest& root, const char* domain) {
   if (sel.all) {
@@ -124,9 +145,16 @@ void authenticate(const RowsPayload& pl, const Sel& sel, const std::vector<uint6
     return;
   }
   check(!pl.all && pl.rows.size() == rows.size(), "partial row count");
-  for (size_t k = 0; k < rows.size(); k++) {
-    check(pl.rows[k].size() == cols, "row width");
-    check(verify_siblings(hash_row(pl.rows[k].data(), cols, domain), rows[k], R, pl.paths[k], root), "Merkle path");
+  std::vector<char> bad(rows.size(), 0);  // 1: row width, 2: Merkle path
+  for_rows(rows.size(), [&](uint64_t a, uint64_t b) {
+    for (uint64_t k = a; k < b; k++) {
+      if (pl.rows[k].size() != cols) { bad[k] = 1; continue; }
+      if (!verify_siblings(hash_row(pl.rows[k].data(), cols, domain), rows[k], R, pl.paths[k], root)) bad[k] = 2;
+    }
+  });
+  for (char v : bad) {
+    check(v != 1, "row width");
+    check(v != 2, "Merkle path");
   }
 }
 
@@ -754,5 +782,7 @@ void verify(const ParamsState& p, const std::vector<Digest>& roots, const std::v
   else verify_single(p, roots[0], point, actual, wire, context);
   require(actual * scale == value, QF_ERR_INVALID_COMMITMENT, "PCS terminal aggregate rejected");
 }
+
+void set_verify_rows_parallel(bool on) { t_rows_parallel = on; }
 
 }  // namespace qf

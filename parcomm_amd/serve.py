@@ -189,12 +189,14 @@ def serve(params, committed, base_claims, n_requests=32, rate=50.0, seed=1, work
     if done:
         lat = np.array([r.done - r.arrival for r in done]) * 1e3
         wait = np.array([r.admitted - r.arrival for r in done]) * 1e3
+        service = np.array([r.done - r.admitted for r in done]) * 1e3
         span = max(r.done for r in done) - min(r.arrival for r in reqs)
         stats.update({
             "p50_ms": round(float(np.percentile(lat, 50)), 2),
             "p99_ms": round(float(np.percentile(lat, 99)), 2),
             "mean_ms": round(float(lat.mean()), 2),
             "mean_queue_ms": round(float(wait.mean()), 2),
+            "mean_service_ms": round(float(service.mean()), 2),
             "openings_per_s": round(len(done) / span, 3),
         })
     return stats, reqs

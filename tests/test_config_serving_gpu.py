@@ -26,6 +26,14 @@ def test_8b_served_requests_match_their_proofs_alone():
             alone, _ = p.prove(committed, claims)
             assert alone == r.proof, r.rid
             print("request", r.rid, "matrices", len(r.subset), "equal alone", flush=True)
+        # the responses verify as a batch, and a tampered one is singled out
+        items = [(r.claims, r.proof) for r in reqs]
+        assert p.verify_batch(committed.statement, items) == [0] * len(items)
+        bad = bytearray(reqs[2].proof)
+        bad[len(bad) // 3] ^= 1
+        items[2] = (reqs[2].claims, bytes(bad))
+        statuses = p.verify_batch(committed.statement, items)
+        assert statuses[2] != 0 and statuses[:2] + statuses[3:] == [0] * (len(items) - 1)
     finally:
         committed.close()
         bva.close()

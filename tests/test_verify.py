@@ -98,3 +98,26 @@ def test_altered_statement_and_claims_are_rejected(oracle):
     bad = list(claims)
     bad[0] = pa.Claim([bytes(r)] + claims[0].row[1:], claims[0].col, claims[0].value)
     assert rejects(p, st, bad, proof)
+
+
+def test_batch_verification_matches_single_proof_verification(oracle):
+    # qf_verify_batch: each proof checked as by qf_verify, on the host's threads;
+    # one altered byte and one altered claim are rejected in the mix
+    import parcomm_amd as pa
+    p, st, claims, proof = random_case(oracle, [(300, 200), (7, 1)], 14, 2, 4)
+    bad_proof = bytearray(proof)
+    bad_proof[len(proof) // 2] ^= 1
+    v = bytearray(claims[0].value)
+    v[0] ^= 1
+    bad_claims = [pa.Claim(claims[0].row, claims[0].col, bytes(v))] + claims[1:]
+    items = [(claims, proof), (claims, bytes(bad_proof)), (claims, proof), (bad_claims, proof), (claims, proof)]
+    for threads in (1, 3, 0):
+        statuses = p.verify_batch(st, items, threads=threads)
+        assert statuses[0] == statuses[2] == statuses[4] == 0
+        assert statuses[1] != 0 and statuses[3] != 0
+    for (c, pr), s in zip(items, p.verify_batch(st, items)):
+        if s == 0:
+            assert p.verify(st, c, pr)
+        else:
+            with pytest.raises(pa.QfError):
+                p.verify(st, c, pr)
