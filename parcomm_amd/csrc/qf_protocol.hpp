@@ -7,7 +7,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <array>
 #include <map>
+#include <mutex>
+#include <unordered_map>
 
 namespace qf {
 
@@ -208,7 +211,7 @@ inline bool bound_at(uint64_t n, uint64_t index, double d, double ln_p, double& 
   logf = std::min(log_add_exp(fcl, ftl), log_add_exp(scl, stl));
   return true;
 }
-inline uint64_t num_queries(const QaCode& code, uint64_t bits, uint64_t rows) {
+inline uint64_t num_queries_uncached(const QaCode& code, uint64_t bits, uint64_t rows) {
   unsigned __int128 card = 1;
   for (int i = 0; i < LANES; i++) card *= P;
   double ln_p = std::log2(double(card)) * M_LN2;
@@ -227,10 +230,28 @@ inline uint64_t num_queries(const QaCode& code, uint64_t bits, uint64_t rows) {
   require(std::isfinite(q) && q >= 0.0, QF_ERR_ENCODING, "query count");
   return std::min<uint64_t>(uint64_t(std::ceil(q)), rows);
 }
+// The query count depends only on (message length, index, bits, rows); the
+// search above costs a few thousand transcendental calls, so every distinct
+// tuple is computed once per process.
+inline uint64_t num_queries(const QaCode& code, uint64_t bits, uint64_t rows) {
+  static std::mutex mu;
+  static std::map<std::array<uint64_t, 4>, uint64_t> memo;
+  const std::array<uint64_t, 4> key{code.message_len, code.index, bits, rows};
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+  }
+  const uint64_t q = num_queries_uncached(code, bits, rows);
+  std::lock_guard<std::mutex> g(mu);
+  memo.emplace(key, q);
+  return q;
+}
 inline std::vector<uint64_t> sample_rows(uint64_t count, uint64_t population, const Digest& seed) {  // qafold_short_pc.rs:3000-3047
   std::vector<uint64_t> out;
   if (count == population) { for (uint64_t i = 0; i < population; i++) out.push_back(i); return out; }
-  std::map<uint64_t, uint64_t> swaps;
+  std::unordered_map<uint64_t, uint64_t> swaps;  // the partial Fisher-Yates permutation
+  swaps.reserve(2 * count);
   uint64_t counter = 0;
   for (uint64_t off = 0; off < count; off++) {
     uint64_t remaining = population - off;
@@ -246,8 +267,9 @@ inline std::vector<uint64_t> sample_rows(uint64_t count, uint64_t population, co
       for (int i = 0; i < 8; i++) cand |= uint64_t(d[i]) << (8 * i);
       if ((unsigned __int128)cand < limit) { s = cand % remaining; break; }
     }
-    uint64_t last = remaining - 1;
-    uint64_t sv = swaps.count(s) ? swaps[s] : s, lv = swaps.count(last) ? swaps[last] : last;
+    const uint64_t last = remaining - 1;
+    const auto is = swaps.find(s), il = swaps.find(last);
+    const uint64_t sv = is == swaps.end() ? s : is->second, lv = il == swaps.end() ? last : il->second;
     swaps[s] = lv;
     out.push_back(sv);
   }

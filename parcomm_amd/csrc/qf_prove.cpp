@@ -3,6 +3,7 @@
 // host; every vector larger than a handful of elements lives in HBM and is
 // processed by the kernels of qf_vec.hip and qf_commit_kernels.hip.
 #include "qf_protocol.hpp"
+#include "qf_rounds.hpp"
 #include "qf_vec.hpp"
 #include <chrono>
 #include <cmath>
@@ -209,6 +210,16 @@ class TwoFold {
     if (host_) QF_HIP(hipMemcpyAsync(dst, a_.data(), n * sizeof(E8), hipMemcpyHostToDevice, c_.st));
     else dv::copy(c_, dst, da_.p, n);
   }
+  // The rounds from length n down to `stop` on the device, with the device
+  // copy of the transcript (device_fold_rounds): every round's (lc, rc, ch)
+  // in `out`, n = stop and (lc, rc) the next cross terms. False (nothing done)
+  // when the vectors are on the host or the range holds no round.
+  bool device_rounds(const Pcs& tr, const char* label, uint64_t& n, uint64_t stop, E8& lc, E8& rc,
+                     std::vector<DevRound>& out) {
+    if (host_ || !device_rounds_on() || stop < HOST_TAIL || n <= stop) return false;
+    out = device_fold_rounds(c_, da_.p, db_.p, n, stop, tr, label, lc, rc);
+    return true;
+  }
   E8 a0() { return host_ ? a_[0] : dv::download(c_, da_.p, 1)[0]; }
   E8 b0() { return host_ ? b_[0] : dv::download(c_, db_.p, 1)[0]; }
 
@@ -246,6 +257,20 @@ LB litebullet(DevCtx& c, E8 claimed, DVec& left, DVec& right, Pcs& tr, const cha
   TwoFold tf(c, left, right);
   if (n > 1) std::tie(lc, rc) = tf.cross(n);
   while (n > 1) {  // each fold also returns the next round's cross terms
+    std::vector<DevRound> dr;
+    if (tf.device_rounds(tr, fold_label, n, TwoFold::HOST_TAIL, lc, rc, dr)) {
+      for (const DevRound& r : dr) {  // the host transcript replays the device's rounds
+        tr.respond(r.lc);
+        tr.respond(r.rc);
+        E8 ch = tr.challenge(fold_label, true);
+        require(ch == r.ch, QF_ERR_TRANSCRIPT, "device transcript mismatch");
+        E8 ci = e8_inv(ch);
+        rhs += r.lc * ch + r.rc * ci;
+        lb.a_point.push_back(ci);
+        lb.b_point.push_back(ch);
+      }
+      continue;
+    }
     tr.respond(lc);
     tr.respond(rc);
     E8 ch = tr.challenge(fold_label, true), ci = e8_inv(ch);
@@ -318,14 +343,21 @@ RoundOut round_reductions(DevCtx& c, const ParamsState& p, size_t level, const s
 }
 
 // One opening round of the native PCS (qafold_short_pc.rs:1120-1451).
+// The top round takes the row weights and the projection its caller already
+// computed for the claimed value (`top_row_w`, `top_opening`).
 RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode& cache, const std::vector<E8>& point,
-                     const E8& value, bool top, Pcs& tr, Bytes& rows_wire, DCode& next, std::vector<E8>& terminal) {
+                     const E8& value, bool top, Pcs& tr, Bytes& rows_wire, DCode& next, std::vector<E8>& terminal,
+                     DVec* top_row_w = nullptr, const std::vector<E8>* top_opening = nullptr) {
   const QaCode& code = p.pp.codes[level];
   uint64_t n = code.message_len, R = cache.R;
   require(R == code.codeword_len() && point.size() == size_t(5 + ilog2u(n)), QF_ERR_INVALID_VARIABLES, "round dims");
-  std::vector<E8> rp(point.begin() + 5, point.end());
-  DVec row_w = tw_vec(c, rp);
-  auto opening = cache.projection(c, n, row_w.p);
+  DVec own_w;
+  if (!top_row_w) {
+    std::vector<E8> rp(point.begin() + 5, point.end());
+    own_w = tw_vec(c, rp);
+  }
+  const DVec& row_w = top_row_w ? *top_row_w : own_w;
+  const std::vector<E8> opening = top_opening ? *top_opening : cache.projection(c, n, row_w.p);
   if (!top) require(inner_host(opening, tw_host(point.data(), 5)) == value, QF_ERR_INVALID_COMMITMENT, "round opening");
   tr.digest("round-root", cache.root);
   for (auto& o : opening) tr.respond(o);
@@ -378,7 +410,7 @@ E8 open_with_reduction(DevCtx& c, const ParamsState& p, const DCode& top_code, c
   std::vector<std::pair<Digest, Bytes>> ext_rounds;
   DCode next;
   std::vector<E8> terminal;
-  RoundOut ro = prove_round(c, p, top, top_code, point, claimed, true, tr, top_rows, next, terminal);
+  RoundOut ro = prove_round(c, p, top, top_code, point, claimed, true, tr, top_rows, next, terminal, &row_w, &opening);
   pending.clear();
   auto settle = [&](size_t level, const RoundOut& o) {
     if (level > 0) pending.push_back({level, o.gen_value, o.gen_left, o.gen_right});
@@ -501,6 +533,21 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     TwoFold tf(c, left, right);
     if (n > 1) std::tie(lc, rc) = tf.cross(n);
     while (n > 1) {
+      std::vector<DevRound> dr;
+      const uint64_t stop = rows > TwoFold::HOST_TAIL && rows < n ? rows : TwoFold::HOST_TAIL;
+      if (tf.device_rounds(tr, "multi-eval-fold", n, stop, lc, rc, dr)) {
+        for (const DevRound& r : dr) {  // the host transcript replays the device's rounds
+          tr.respond(r.lc);
+          tr.respond(r.rc);
+          E8 ch = tr.challenge("multi-eval-fold", true);
+          require(ch == r.ch, QF_ERR_TRANSCRIPT, "device transcript mismatch");
+          E8 ci = e8_inv(ch);
+          rhs += r.lc * ch + r.rc * ci;
+          red_point.push_back(ci);
+        }
+        if (n == rows) tf.copy_a(folded_vector.p, rows);
+        continue;
+      }
       tr.respond(lc);
       tr.respond(rc);
       E8 ch = tr.challenge("multi-eval-fold", true), ci = e8_inv(ch);
