@@ -60,6 +60,7 @@ struct TileArgs {
   int logC, t, s, logV, log_lo_chunks, log_hi_count;
   int xlo, xhi;             // transformed tile bits [xlo, xhi)
   int quad_ok;              // the quad-split kernels' per-thread offsets fit 32 bits
+  int big_ok;               // the 8192-element tiles apply
   // quad-split kernels: element offset of slot 1 of each mapping (slot q is q
   // times that: the five slot bits lie in one field of the tile index), for the
   // input (BVA bytes / 8 when packing), the outputs and alpha (positions)
@@ -508,6 +509,134 @@ __global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   tile_body<MODE, PACK, LO, true>(a);
 }
 
+// ---------------------------------------------------------------- 8192-element tiles
+// A pass over 10 strided index bits leaves a 4096-element tile 4 contiguous
+// positions per tile row: 64-byte rows, which HBM serves at about 3.1 TB/s
+// (128-byte rows 4.6, 256 bytes and up 5.0 TB/s; measured with the
+// transforms switched off). The big tile holds 8192 elements, 8 contiguous
+// positions (128 bytes) x 2^10 rows, in one workgroup of 1024 threads per CU
+// (132 KiB of LDS). Each thread keeps 8 elements as in the 4096-element form;
+// the tile's 13 bits go through mappings with register bits 10..12 (A),
+// 7..9 (B), 4..6 (C) and 1..3 (D, of which only bit 3 is transformed). Tile
+// bits 0..2 are the contiguous positions and are never transformed.
+// Modes M, M_W and W.
+constexpr int BWG = 1024;
+constexpr int BN = BWG * TQ;            // 8192 elements
+constexpr int BPAD = BN + BN / 32;
+enum : int { BMAP_A = 10, BMAP_B = 7, BMAP_C = 4, BMAP_D = 1 };
+
+// Tile element of slot q in thread tid. A and B keep the thread bits in order
+// (a wave covers 8 contiguous positions x 8 rows in A, the load/store layout);
+// C and D permute them so that the 8-byte LDS accesses of every 32 lanes hit
+// distinct bank pairs under lds_slot (lane-bit weights 1, 2, 4, 8, 16 mod 32):
+// C: tid bits -> e bits 0 1 2 3 9 7 8 10 11 12 (e bit 9 weighs 528 = 16 mod 32);
+// D: tid bits -> e bits 0 6 7 8 4 5 9 10 11 12 (66, 132, 264 = 2, 4, 8 mod 32).
+template <int MAP>
+__device__ inline int btile_e(int tid, int q) {
+  if (MAP == BMAP_C)
+    return (tid & 15) | (q << 4) | (((tid >> 4) & 1) << 9) | (((tid >> 5) & 3) << 7) | ((tid >> 7) << 10);
+  if (MAP == BMAP_D)
+    return (tid & 1) | (q << 1) | (((tid >> 1) & 7) << 6) | (((tid >> 4) & 3) << 4) | (((tid >> 6) & 1) << 9) |
+           ((tid >> 7) << 10);
+  return (tid & ((1 << MAP) - 1)) | (q << MAP) | ((tid >> MAP) << (MAP + TQB));
+}
+
+// lds_slot is additive over disjoint bit fields (e + (e >> 5) with no carry
+// between them), so slot q of a mapping is the thread's base slot plus a
+// compile-time offset: one base register per mapping and LDS row.
+template <int MAP>
+__device__ inline int bslot_base(int tid) { return lds_slot(btile_e<MAP>(tid, 0)); }
+template <int MAP>
+constexpr int bslot_q(int q) {
+  return MAP == BMAP_D ? (q << 1) + ((q << 1) >> 5) : (q << MAP) + ((q << MAP) >> 5);
+}
+
+template <int FROM, int TO>
+__device__ inline void bexchange(int32_t (&x)[TQ][LANES], uint2 (*lds)[BPAD]) {
+  constexpr int XROUNDS = LANES / (2 * NR);
+  const int tid = threadIdx.x;
+  const int bf = bslot_base<FROM>(tid), bt = bslot_base<TO>(tid);
+#pragma unroll
+  for (int rnd = 0; rnd < XROUNDS; rnd++) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NR; p++) {
+      uint2* row = lds[p] + bf;
+#pragma unroll
+      for (int q = 0; q < TQ; q++) {
+        const int k = 2 * (rnd * NR + p);
+        row[bslot_q<FROM>(q)] = make_uint2(uint32_t(x[q][k]), uint32_t(x[q][k + 1]));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NR; p++) {
+      const uint2* row = lds[p] + bt;
+#pragma unroll
+      for (int q = 0; q < TQ; q++) {
+        const int k = 2 * (rnd * NR + p);
+        const uint2 v = row[bslot_q<TO>(q)];
+        x[q][k] = int32_t(v.x);
+        x[q][k + 1] = int32_t(v.y);
+      }
+    }
+  }
+}
+
+// Transform over tile bits 3..12 (the 10 strided bits), from and back to A.
+template <bool WHT>
+__device__ inline void btransform(int32_t (&x)[TQ][LANES], uint2 (*lds)[BPAD]) {
+  phase<WHT, BMAP_A, 3>(x, 3, 13);
+  bexchange<BMAP_A, BMAP_B>(x, lds);
+  phase<WHT, BMAP_B, 3>(x, 3, 13);
+  bexchange<BMAP_B, BMAP_C>(x, lds);
+  phase<WHT, BMAP_C, 3>(x, 3, 13);
+  bexchange<BMAP_C, BMAP_D>(x, lds);
+  phase<WHT, BMAP_D, 3>(x, 3, 13);
+  bexchange<BMAP_D, BMAP_A>(x, lds);
+}
+
+// Persistent workgroups (one per CU) take tiles blockIdx, blockIdx + grid, ...:
+// the tiles in flight at one time are neighbours along the contiguous bits.
+template <int MODE>
+__global__ __launch_bounds__(BWG) void k_btile(TileArgs a) {
+  static_assert(MODE == TM_M || MODE == TM_M_W || MODE == TM_W, "big tiles run M, M_W and W");
+  __shared__ uint2 lds[NR][BPAD];
+  const uint32_t in_t = 16u * tile_off(a, btile_e<BMAP_A>(threadIdx.x, 0), a.in_vstride);
+  const uint32_t out_t = 16u * tile_off(a, btile_e<BMAP_A>(threadIdx.x, 0), a.out_vstride);
+  for (uint32_t T = blockIdx.x; T < a.tiles; T += gridDim.x) {
+    uint64_t vec0, pos0;
+    tile_base(a, T, vec0, pos0);
+    int32_t x[TQ][LANES];
+    {
+      const char* base = reinterpret_cast<const char*>(static_cast<const E8*>(a.in) + vec0 * a.in_vstride + pos0);
+      uint4 raw[TQ];
+#pragma unroll
+      for (int q = 0; q < TQ; q++)
+        raw[q] = *reinterpret_cast<const uint4*>(base + 16 * tile_off64(a, BWG * q, a.in_vstride) + in_t);
+#pragma unroll
+      for (int q = 0; q < TQ; q++) unpack_e8(raw[q], x[q]);
+    }
+    auto store = [&](E8* out) {
+      char* base = reinterpret_cast<char*>(out + vec0 * a.out_vstride + pos0);
+#pragma unroll
+      for (int q = 0; q < TQ; q++)
+        *reinterpret_cast<uint4*>(base + 16 * tile_off64(a, BWG * q, a.out_vstride) + out_t) = pack_e8(x[q]);
+    };
+    if (MODE == TM_M || MODE == TM_M_W) {
+      btransform<false>(x, lds);
+      reduce_all<3>(x);
+      store(a.out1);
+    }
+    if (MODE == TM_M_W || MODE == TM_W) {
+      btransform<true>(x, lds);
+      reduce_all<3>(x);
+      store(MODE == TM_W ? a.out1 : a.out2);
+    }
+    __syncthreads();  // the next tile's first exchange reuses the LDS rows
+  }
+}
+
 // ---------------------------------------------------------------- quad-split tiles
 // A second kernel family for the same passes, with fewer LDS exchanges. A
 // thread holds one 32-bit word d of 32 tile elements (the four words of a
@@ -948,13 +1077,35 @@ struct TileShape {
   int per_cu;
   bool xcd;
   bool quad = false;  // the quad-split kernels (k_qtile) instead of the register-radix ones (k_tile)
+  bool big = false;   // the 8192-element tiles (k_btile), one workgroup per CU
 };
 
 bool xcd_applicable(const TileArgs& a, uint64_t wgs) {
   return a.logC == 2 && a.log_lo_chunks >= 1 && wgs % 16 == 0 && a.tiles >= 2 * wgs;
 }
 
+// The same pass in 8192-element tiles: 8 contiguous positions x the 10 bits.
+TileArgs big_args(TileArgs a) {
+  a.logC = 13 - a.t;
+  a.log_lo_chunks = a.s - a.logC;
+  a.tiles /= 2;
+  a.xlo = a.logC;
+  a.xhi = 13;
+  return a;
+}
+
 void launch_shape(int mode, bool pack, TileArgs a, const TileShape& sh, hipStream_t st) {
+  if (sh.big) {
+    a = big_args(a);
+    const uint32_t grid = uint32_t(std::min<uint64_t>(a.tiles, uint64_t(device_cus())));
+    dim3 g{grid, 1, 1}, b{uint32_t(BWG), 1, 1};
+    switch (mode) {
+      case TM_M: hipLaunchKernelGGL(k_btile<TM_M>, g, b, 0, st, a); break;
+      case TM_M_W: hipLaunchKernelGGL(k_btile<TM_M_W>, g, b, 0, st, a); break;
+      default: hipLaunchKernelGGL(k_btile<TM_W>, g, b, 0, st, a); break;
+    }
+    return;
+  }
   const uint64_t tiles = a.tiles;
   const uint64_t wgs = std::min<uint64_t>(tiles, uint64_t(sh.per_cu) * uint64_t(device_cus()));
   a.per_wg = uint32_t((tiles + wgs - 1) / wgs);
@@ -1011,9 +1162,16 @@ bool qtile_on() {
   const char* e = std::getenv("QF_QTILE");
   return !(e && e[0] == '0');
 }
-// The quad-split kernels when they apply (their per-thread offsets fit 32
-// bits), else the register-radix ones.
-TileShape default_shape(int mode, bool quad_ok) {
+// QF_BTILE=0 keeps 10-bit strided passes on 4096-element tiles.
+bool btile_on() {
+  const char* e = std::getenv("QF_BTILE");
+  return !(e && e[0] == '0');
+}
+// The 8192-element tiles for the 10-bit strided passes they serve, else the
+// quad-split kernels when they apply (their per-thread offsets fit 32 bits),
+// else the register-radix ones.
+TileShape default_shape(int mode, bool quad_ok, bool big_ok) {
+  if (big_ok && btile_on()) return {false, 1, false, false, true};
   if (quad_ok && qtile_on()) return {false, 2, true, true};  // quad_ok: a compile-time bit range
   const bool w = wide_mode(mode);
   return {w, w ? 1 : 2, true, false};
@@ -1057,7 +1215,7 @@ bool autotune_on() {
 }
 
 TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L, const TuneKey& key, hipStream_t st) {
-  TuneEntry out{default_shape(mode, a0.quad_ok), {}};
+  TuneEntry out{default_shape(mode, a0.quad_ok, a0.big_ok), {}};
   const uint64_t n = uint64_t(1) << L;
   const uint64_t out_ext = ((vectors - 1) * a0.out_vstride + n) * 16;
   // one scratch span for the E8 buffers, at the callers' relative offsets
@@ -1100,11 +1258,12 @@ TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L,
     cands.push_back({false, 2, true, true});
     cands.push_back({true, 1, true, true});
   }
+  if (a0.big_ok && btile_on()) cands.push_back({false, 1, false, false, true});
   hipEvent_t e0, e1;
   QF_HIP(hipEventCreate(&e0));
   QF_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  const TileShape def = default_shape(mode, a0.quad_ok);
+  const TileShape def = default_shape(mode, a0.quad_ok, a0.big_ok);
   float def_ms = -1;
   for (const TileShape& sh : cands) {
     launch_shape(mode, pack, a, sh, st);  // warm-up
@@ -1121,7 +1280,8 @@ TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L,
     QF_HIP(hipGetLastError());
     out.tried.push_back({sh, t_best});
     if (t_best < best) { best = t_best; out.best = sh; }
-    if (sh.wide == def.wide && sh.per_cu == def.per_cu && sh.xcd == def.xcd && sh.quad == def.quad) def_ms = t_best;
+    if (sh.wide == def.wide && sh.per_cu == def.per_cu && sh.xcd == def.xcd && sh.quad == def.quad && sh.big == def.big)
+      def_ms = t_best;
   }
   if (def_ms > 0 && best > def_ms / 1.05f) out.best = def;
   (void)hipEventDestroy(e0);
@@ -1141,17 +1301,17 @@ void load_tune_file(std::map<TuneKey, TuneEntry>& tab) {
   FILE* fp = std::fopen(f, "r");
   if (!fp) return;
   TuneKey k{};
-  int pack = 0, xvec = 0, in_place = 0, wide = 0, per_cu = 0, xcd = 0, quad = 0;
+  int pack = 0, xvec = 0, in_place = 0, wide = 0, per_cu = 0, xcd = 0, quad = 0, big = 0;
   unsigned long long v = 0, iv = 0, ov = 0;
-  while (std::fscanf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d", &k.mode, &k.L, &k.s, &k.t, &pack,
-                     &xvec, &v, &iv, &ov, &k.out2_rel, &in_place, &wide, &per_cu, &xcd, &quad) == 15) {
+  while (std::fscanf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d %d", &k.mode, &k.L, &k.s, &k.t, &pack,
+                     &xvec, &v, &iv, &ov, &k.out2_rel, &in_place, &wide, &per_cu, &xcd, &quad, &big) == 16) {
     k.pack = pack;
     k.xvec = xvec;
     k.in_place = in_place;
     k.vectors = v;
     k.in_vs = iv;
     k.out_vs = ov;
-    if (per_cu == 1 || per_cu == 2) tab[k] = TuneEntry{{wide != 0, per_cu, xcd != 0, quad != 0}, {}};
+    if (per_cu == 1 || per_cu == 2) tab[k] = TuneEntry{{wide != 0, per_cu, xcd != 0, quad != 0, big != 0}, {}};
   }
   std::fclose(fp);
 }
@@ -1161,10 +1321,10 @@ void save_tune_file(const std::map<TuneKey, TuneEntry>& tab) {
   FILE* fp = std::fopen(f, "w");
   if (!fp) return;
   for (auto& [k, e] : tab)
-    std::fprintf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d\n", k.mode, k.L, k.s, k.t, int(k.pack),
+    std::fprintf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d %d\n", k.mode, k.L, k.s, k.t, int(k.pack),
                  int(k.xvec), (unsigned long long)k.vectors, (unsigned long long)k.in_vs,
                  (unsigned long long)k.out_vs, k.out2_rel, int(k.in_place), int(e.best.wide), e.best.per_cu,
-                 int(e.best.xcd), int(e.best.quad));
+                 int(e.best.xcd), int(e.best.quad), int(e.best.big));
   std::fclose(fp);
 }
 
@@ -1173,7 +1333,7 @@ void save_tune_file(const std::map<TuneKey, TuneEntry>& tab) {
 thread_local bool t_tune_allowed = true;
 
 TileShape shape_for(int mode, bool pack, const TileArgs& a, uint64_t vectors, int L, bool xvec, hipStream_t st) {
-  if (!autotune_on()) return default_shape(mode, a.quad_ok);
+  if (!autotune_on()) return default_shape(mode, a.quad_ok, a.big_ok);
   const uint64_t n = uint64_t(1) << L;
   TuneKey key{mode, L, a.s, a.t, pack, xvec, vectors, a.in_vstride, a.out_vstride,
               !a.out2 ? 0 : (a.out2 == a.out1 + n ? 1 : 2), !pack && a.in == static_cast<const void*>(a.out1)};
@@ -1185,7 +1345,7 @@ TileShape shape_for(int mode, bool pack, const TileArgs& a, uint64_t vectors, in
     loaded = true;
   }
   auto it = tab.find(key);
-  if (it == tab.end() && !t_tune_allowed) return default_shape(mode, a.quad_ok);
+  if (it == tab.end() && !t_tune_allowed) return default_shape(mode, a.quad_ok, a.big_ok);
   if (it == tab.end()) {
     it = tab.emplace(key, tune(mode, pack, a, vectors, L, key, st)).first;
     save_tune_file(tab);
@@ -1273,6 +1433,15 @@ bool tile_pass(int mode, bool pack, const void* in, E8* out1, E8* out2, const E8
   const uint64_t tiles = (vectors << L) / TN;
   if (tiles == 0 || tiles >= (1ull << 31)) return false;
   a.tiles = uint32_t(tiles);
+  // 8192-element tiles for the M, M_W and W passes over 10 strided bits: an
+  // even number of 4096-element tiles, and per-thread offsets (tile bits 0..9
+  // in mapping A) below 2^32 bytes
+  {
+    const TileArgs b = big_args(a);
+    a.big_ok = !pack && s >= 3 && t == 10 && (mode == TM_M || mode == TM_M_W || mode == TM_W) && tiles % 2 == 0 &&
+               std::max(tile_off64(b, BWG - 1, in_vstride), tile_off64(b, BWG - 1, out_vstride)) * 16 + 16 <
+                   (1ull << 32);
+  }
   launch_shape(mode, pack, a, shape_for(mode, pack, a, vectors, L, xform_vectors, st), st);
   QF_HIP(hipGetLastError());
   return true;
@@ -1287,7 +1456,8 @@ std::string autotune_report() {
   std::string o = "[";
   auto shape = [](const TileShape& s) {
     return std::string("{\"wide\":") + (s.wide ? "true" : "false") + ",\"per_cu\":" + std::to_string(s.per_cu) +
-           ",\"xcd\":" + (s.xcd ? "true" : "false") + ",\"quad\":" + (s.quad ? "true" : "false") + "}";
+           ",\"xcd\":" + (s.xcd ? "true" : "false") + ",\"quad\":" + (s.quad ? "true" : "false") +
+           ",\"big\":" + (s.big ? "true" : "false") + "}";
   };
   bool first = true;
   for (auto& [k, e] : tune_table()) {

@@ -1,9 +1,11 @@
 """Kernel-level parity: the transform passes (qf_tile.hip) of both kernel
-families, register-radix and quad-split, against numpy restatements of the
-same 2x2 tensor maps, over F65521/Ext8 (libparcomm_amd.so) and KoalaBear/Ext4
+families, register-radix and quad-split, and of the 8192-element tiles of
+the 10-bit strided passes, against numpy restatements of the same 2x2 tensor
+maps, over F65521/Ext8 (libparcomm_amd.so) and KoalaBear/Ext4
 (libparcomm_amd_kb.so). Each family is forced with the autotuner off
-(QF_AUTOTUNE=0; QF_QTILE=0 keeps the register-radix kernels, and the
-quad-split ones take every geometry they support)."""
+(QF_AUTOTUNE=0; QF_QTILE=0 keeps the register-radix kernels, the quad-split
+ones take every geometry they support; QF_BTILE=0 keeps 10-bit strided passes
+off the 8192-element tiles, which otherwise take them)."""
 import ctypes
 import os
 
@@ -13,17 +15,34 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["radix", "quad"])
-def family(request):
-    saved = {k: os.environ.get(k) for k in ("QF_AUTOTUNE", "QF_QTILE")}
+def _forced(family_name):
+    saved = {k: os.environ.get(k) for k in ("QF_AUTOTUNE", "QF_QTILE", "QF_BTILE")}
     os.environ["QF_AUTOTUNE"] = "0"
-    os.environ["QF_QTILE"] = "1" if request.param == "quad" else "0"
-    yield request.param
+    os.environ["QF_QTILE"] = "1" if family_name == "quad" else "0"
+    os.environ["QF_BTILE"] = "1" if family_name == "big" else "0"
+    return saved
+
+
+def _restore(saved):
     for k, v in saved.items():
         if v is None:
             os.environ.pop(k, None)
         else:
             os.environ[k] = v
+
+
+@pytest.fixture(params=["radix", "quad"])
+def family(request):
+    saved = _forced(request.param)
+    yield request.param
+    _restore(saved)
+
+
+@pytest.fixture
+def big_tiles():
+    saved = _forced("big")
+    yield "big"
+    _restore(saved)
 # field -> (p, extension degree, X^degree wrap constant, coefficient dtype)
 FIELDS = {0: (65521, 8, 17, np.uint16), 1: (2130706433, 4, 3, np.uint32)}
 FIELD_IDS = {0: "f65521", 1: "koalabear"}
@@ -130,6 +149,33 @@ def test_tile_pass_matches_numpy(vectors, L, s, t, mode, field, family):
         w = transform(x, bits, True, P)
         assert np.array_equal(o1, x)
         assert np.array_equal(o2, transform(e8_mul(w, alpha[None], field), bits, True, P))
+
+
+# The 8192-element tiles: Mobius, Mobius then WHT, and WHT over 10 strided
+# bits (8 contiguous positions x 2^10 rows), as in the 2^22-row commit's high
+# row group, with several vectors, padded vector strides and a hi index.
+BIG_GEOMS = [(1, 22, 12, 10), (4, 14, 3, 10), (2, 16, 5, 10), (1, 18, 7, 10)]
+
+
+@pytest.mark.parametrize("field", [0, 1], ids=lambda f: FIELD_IDS[f])
+@pytest.mark.parametrize("vectors,L,s,t", BIG_GEOMS)
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_big_tile_pass_matches_numpy(vectors, L, s, t, mode, field, big_tiles):
+    P, lanes, _, _ = FIELDS[field]
+    rng = np.random.default_rng(5000 + 10 * s + L + mode + 7 * field)
+    x = field_values(rng, P, (vectors, 1 << L, lanes))
+    print("big tile pass", FIELD_IDS[field], "mode", mode, "vectors", vectors, "L", L, "bits", s, s + t, flush=True)
+    pad = (1 << L) + 4096 if L < 20 else None
+    o1, o2 = run(field, mode, x, None, L, s, t, vstride_in=pad, vstride_out=pad)
+    bits = list(range(s, s + t))
+    if mode == 0:
+        assert np.array_equal(o1, transform(x, bits, False, P))
+    elif mode == 1:
+        m = transform(x, bits, False, P)
+        assert np.array_equal(o1, m)
+        assert np.array_equal(o2, transform(m, bits, True, P))
+    else:
+        assert np.array_equal(o1, transform(x, bits, True, P))
 
 
 # Geometries of the 2^23-row commit schedule: Mobius on row bits 7..14 and
