@@ -732,6 +732,17 @@ std::vector<std::vector<E8>> project_terms(DevCtx& c, const Query& q, const Val*
 }
 
 // ================================================================ EO (eo.rs:97-181)
+// The packed reduction and its binary rounds. Every term of at least 2s
+// entries is projected once at stride 2s (its offset is a multiple of its
+// power-of-two length): the stride-s projections behind the packed cross
+// terms, their d = 0 sum and, once the packed challenge is known, the first
+// binary round's cross terms all follow from those chunk sums. Terms below 2s
+// entries lie in one row of stride s and are handled on the host from the BVA
+// entries around them. The reference's consistency check dot(w, qf) == claim
+// (eo.rs:134-138) equals, term for term, sum_i v[i] q[i] == query.claim, since
+// both sides add the same cross-term multiples of the challenge; the check is
+// made on that sum. The witness of stride s is never written: the first round
+// reads the BVA at stride 2s (see eo_first_fold).
 struct EoOut { std::vector<E8> pos, neg; std::vector<std::array<E8, 2>> rounds; E8 value, query; std::vector<E8> point; E8 scale; };
 EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Query& q, Outer& tr) {
   EoOut o;
@@ -742,44 +753,40 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   std::vector<std::vector<E8>> term_low(q.terms.size());
   for (size_t i = 0; i < q.terms.size(); i++)
     if (q.terms[i].len() >= s) term_low[i] = tbv_w(q.terms[i].f.data(), ls);
-  if (launch_mode() == LAUNCH_BATCHED) {
-    // one projection launch over every term; the lane correlations on the device
-    std::vector<ProjTask> tasks;
-    std::vector<std::array<E8, 2>> pairs;
-    std::vector<E8> lows;
-    for (size_t i = 0; i < q.terms.size(); i++) {
-      const Term& t = q.terms[i];
-      if (t.len() < s || t.offset >= l.active || e8_is_zero(t.scale)) continue;
-      tasks.push_back({t.offset, t.len() / s, uint32_t(t.f.size() - ls), uint32_t(pairs.size()), t.scale});
-      for (size_t b = ls; b < t.f.size(); b++) pairs.push_back(t.f[b]);
-      lows.insert(lows.end(), term_low[i].begin(), term_low[i].end());
-    }
-    auto pn = tensor_project_cross(c, bva, l.active, int(s), tasks, pairs, lows);
-    o.pos = pn.first;
-    o.neg = pn.second;
-  } else {
-    std::vector<size_t> which;
-    auto proj = project_terms(c, q, bva, l.active, s, which);
-    for (size_t k = 0; k < which.size(); k++) {
-      const auto& low = term_low[which[k]];
-      for (uint64_t d = 1; d < s; d++)
-        for (uint64_t lane = 0; lane + d < s; lane++) {
-          o.pos[d - 1] += proj[k][lane] * low[lane + d];
-          o.neg[d - 1] += proj[k][lane + d] * low[lane];
-        }
-    }
+  std::vector<ProjTask> tasks;
+  std::vector<std::array<E8, 2>> pairs, fbit;
+  std::vector<E8> lows;
+  std::vector<size_t> big;  // the term of each task
+  for (size_t i = 0; i < q.terms.size(); i++) {
+    const Term& t = q.terms[i];
+    if (t.len() < 2 * s || t.offset >= l.active || e8_is_zero(t.scale)) continue;
+    tasks.push_back({t.offset, t.len() / (2 * s), uint32_t(t.f.size() - ls - 1), uint32_t(pairs.size()), t.scale});
+    for (size_t b = ls + 1; b < t.f.size(); b++) pairs.push_back(t.f[b]);
+    lows.insert(lows.end(), term_low[i].begin(), term_low[i].end());
+    fbit.push_back(t.f[ls]);
+    big.push_back(i);
   }
+  EoProjection proj = eo_project(c, bva, l.active, int(s), tasks, pairs);
+  EoCorr corr = eo_correlations(c, proj, fbit, lows);
+  o.pos = corr.pos;
+  o.neg = corr.neg;
+  E8 actual = corr.zero;  // sum_i v[i] q[i]
+  // terms below 2s entries: one row k of stride s each
+  struct SmallTerm { uint64_t k; std::vector<E8> query; };
+  std::vector<SmallTerm> smalls;
   for (auto& t : q.terms) {
-    if (t.len() >= s) continue;
+    if (t.len() >= 2 * s) continue;
     uint64_t start = t.offset / s * s;
     auto vals = read_values(c, bva, l.active, start, s);
     std::vector<E8> query(s, E8{});
     for (uint64_t i = 0; i < t.len(); i++) query[t.offset - start + i] = t.weight(i);
+    for (uint64_t lane = 0; lane < s; lane++) actual += query[lane] * small_val(vals[lane]);
     for (uint64_t d = 1; d < s; d++)
       for (uint64_t lane = 0; lane + d < s; lane++) {
         o.pos[d - 1] += query[lane + d] * small_val(vals[lane]);
         o.neg[d - 1] += query[lane] * small_val(vals[lane + d]);
       }
+    smalls.push_back({t.offset / s, std::move(query)});
   }
   tr.object("packed-positive", js_scalars(o.pos));
   tr.object("packed-negative", js_scalars(o.neg));
@@ -791,30 +798,36 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   std::vector<E8> pinv(s), pows(s);
   pinv[0] = pows[0] = e8_one();
   for (uint64_t i = 1; i < s; i++) { pinv[i] = pinv[i - 1] * inv; pows[i] = pows[i - 1] * ch; }
+  require(actual == q.claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
   uint64_t rows = l.logical / s;
-  DVec w(rows), qv(rows);
+  // the query fold of each term of at least s entries: scale x sum_l low[l] c^l
+  std::vector<E8> fold_scale(q.terms.size(), E8{});
+  for (size_t i = 0; i < q.terms.size(); i++)
+    if (q.terms[i].len() >= s) {
+      E8 acc{};
+      for (uint64_t j = 0; j < s; j++) acc += term_low[i][j] * pows[j];
+      fold_scale[i] = acc;
+    }
+  DVec qv(rows);
   // query fold (tbv.rs:393-428): bucket terms (written, the rows outside them
   // zeroed), then padding/suffix terms added; each phase has disjoint supports.
   std::map<uint64_t, E8> small;
   for (int phase = 0; phase < 2; phase++) {
-    std::vector<FillTask> tasks;
-    std::vector<std::array<E8, 2>> pairs;
+    std::vector<FillTask> ftasks;
+    std::vector<std::array<E8, 2>> fpairs;
     size_t a = phase ? q.n_bucket_terms : 0, b = phase ? q.terms.size() : q.n_bucket_terms;
     for (size_t i = a; i < b; i++) {
       const Term& t = q.terms[i];
       if (t.len() >= s) {
-        const auto& low = term_low[i];
-        E8 acc{};
-        for (uint64_t j = 0; j < s; j++) acc += low[j] * pows[j];
-        FillTask ft{t.offset / s, uint32_t(t.f.size() - ls), uint32_t(pairs.size()), t.scale * acc};
-        for (size_t bb = ls; bb < t.f.size(); bb++) pairs.push_back(t.f[bb]);
-        tasks.push_back(ft);
+        FillTask ft{t.offset / s, uint32_t(t.f.size() - ls), uint32_t(fpairs.size()), t.scale * fold_scale[i]};
+        for (size_t bb = ls; bb < t.f.size(); bb++) fpairs.push_back(t.f[bb]);
+        ftasks.push_back(ft);
       } else {
         for (uint64_t j = 0; j < t.len(); j++) small[t.offset / s] += t.weight(j) * pows[(t.offset + j) % s];
       }
     }
-    if (phase == 0) tensor_fill(c, qv.p, rows, tasks, pairs);
-    else tensor_fill_add(c, qv.p, tasks, pairs);
+    if (phase == 0) tensor_fill(c, qv.p, rows, ftasks, fpairs);
+    else tensor_fill_add(c, qv.p, ftasks, fpairs);
   }
   {
     std::vector<uint64_t> idx;
@@ -824,24 +837,36 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   }
   E8 claim = q.claim, fwd = ch, bwd = inv;
   for (uint64_t d = 0; d + 1 < s; d++) { claim += o.pos[d] * fwd + o.neg[d] * bwd; fwd = fwd * ch; bwd = bwd * inv; }
-  // The witness is built in the same pass as the consistency check and the
-  // first round's cross terms, and every fold also returns the next round's
-  // cross terms (rows is a power of two).
+  // the first binary round's cross terms sum_j w[2j] qf[2j+1], sum_j w[2j+1] qf[2j]
   E8 c0{}, c1{};
   if (rows >= 2) {
-    auto first = eo_witness_first(c, bva, l.active, rows, int(s), pinv, qv.p, w.p);
-    require(first[0] == claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
-    c0 = first[1];
-    c1 = first[2];
-  } else {
-    eo_witness(c, bva, l.active, rows, int(s), pinv, w.p);
-    require(dv::inner(c, w.p, qv.p, rows) == claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
+    // the terms' scales are in the projections already: coef carries the lane fold and row bit 0
+    std::vector<std::array<E8, 2>> coef;
+    for (size_t k = 0; k < big.size(); k++) {
+      const E8 f = fold_scale[big[k]];
+      coef.push_back({f * fbit[k][0], f * fbit[k][1]});
+    }
+    std::tie(c0, c1) = eo_first_cross(c, proj, pinv, coef);
+    for (auto& st : smalls) {  // a term in row k pairs with the witness of row k ^ 1
+      E8 qf{};
+      for (uint64_t lane = 0; lane < s; lane++) qf += st.query[lane] * pows[lane];
+      auto vals = read_values(c, bva, l.active, (st.k ^ 1) * s, s);
+      E8 wv{};
+      for (uint64_t lane = 0; lane < s; lane++) wv += pinv[lane] * small_val(vals[lane]);
+      if (st.k & 1) c0 += wv * qf;
+      else c1 += wv * qf;
+    }
   }
-  DVec w2(rows / 2 ? rows / 2 : 1), q2(rows / 2 ? rows / 2 : 1);
+  // Long vectors: the first round reads the BVA (no witness of stride s); short
+  // ones materialize the witness and take the rounds below.
+  const bool from_bva = rows > TwoFold::HOST_TAIL;
+  DVec w(from_bva ? rows / 4 : rows), w2(rows / 2 ? rows / 2 : 1), q2(rows / 2 ? rows / 2 : 1);
+  if (!from_bva) eo_witness(c, bva, l.active, rows, int(s), pinv, w.p);
   DVec *cw = &w, *cq = &qv, *nw = &w2, *nq = &q2;
   uint64_t n = rows;
   // the last rounds run on host copies once the vectors are short (as TwoFold)
   std::vector<E8> hw, hq;
+  bool first = true;
   while (n > 1) {
     uint64_t h = n / 2;
     tr.object("binary-cross", js_scalars({c0, c1}));
@@ -849,6 +874,20 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     claim += c0 * r + c1 * ri;
     challenges.push_back(r);
     o.rounds.push_back({c0, c1});
+    if (first && from_bva) {  // w'[i] = sum_{L<2s} v[2is + L] pinv2[L], pinv2 = (c^-l, r^-1 c^-l)
+      std::vector<E8> pinv2(2 * s);
+      for (uint64_t lane = 0; lane < s; lane++) { pinv2[lane] = pinv[lane]; pinv2[s + lane] = ri * pinv[lane]; }
+      std::tie(c0, c1) = eo_first_fold(c, bva, l.active, rows, int(s), pinv2, qv.p, r, w2.p, q2.p);
+      // outputs in (w2, q2); the next round writes into w (rows / 4) and qv
+      cw = &w2;
+      cq = &q2;
+      nw = &w;
+      nq = &qv;
+      first = false;
+      n = h;
+      continue;
+    }
+    first = false;
     if (hw.empty() && n <= TwoFold::HOST_TAIL) {
       hw = dv::download(c, cw->p, n);
       hq = dv::download(c, cq->p, n);

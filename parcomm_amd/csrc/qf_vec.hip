@@ -922,11 +922,6 @@ __global__ __launch_bounds__(WG) void k_eo_witness(const Val* __restrict__ bva, 
   }
 }
 
-// The witness together with the packed reduction's consistency sum <w, q>
-// and the first binary round's cross terms (eo.rs:119-150), with the query q
-// already built: one read of the BVA and of q, w written once and not read
-// back. Pair i covers rows 2i and 2i + 1; sums {<w,q>, sum w[2i] q[2i+1],
-// sum w[2i+1] q[2i]} go to the block partials.
 #ifndef QF_KB
 // Lazy sums of F65521 extension products. A product a b splits a into 8-bit
 // limbs and b into the wrap-folded sequence D of v8_mul_dot2; each product
@@ -982,21 +977,34 @@ __device__ inline V8 lazy_value(const LazySum& s) {  // sl, sh < 2^32
   return r;
 }
 
-// F65521 form of the fused witness pass. Lanes are biased to u = v + 128 (one
-// xor per four lanes), paired as u16 and multiplied by u16 pairs of pinv
-// coefficients with v_dot2_u32_u16; negc[k] = -128 sum_lane pinv[lane].c[k]
-// mod P removes the bias. The three sums accumulate lazily (flushed every 8
-// pairs: sum 0 takes two products per pair).
-template <int S>
-__global__ __launch_bounds__(WG) void k_eo_witness_first(const Val* __restrict__ bva, uint64_t available, uint64_t h,
-                                                         const E8* __restrict__ pinv_g, const E8* __restrict__ q,
-                                                         E8* __restrict__ w, uint32_t* __restrict__ partials) {
-  constexpr int G = S < 4 ? 1 : S / 4;  // BVA words (4 lanes) per row
-  constexpr int NP = S / 2;             // lane pairs per row
-  __shared__ E8 pinv[S];
+#endif
+
+// ---------------------------------------------------------------- EO first round from the BVA
+// The first binary EO round folds witness rows 2i and 2i + 1 of stride s
+// (eo.rs:119-132, 140-163): w'[i] = w[2i] + w[2i+1] r^-1 is itself a witness
+// row of stride S2 = 2s over entries [2 i s, 2 i s + 2s), with the weights
+// pinv2[L] = c^-L (L < s) and r^-1 c^-(L-s) (L >= s). The fold therefore reads
+// the BVA instead of a materialized witness, and the witness of stride s is
+// never written. The query folds as q'[i] = q[2i] + q[2i+1] r; the pass
+// returns the next round's cross terms over its outputs,
+// sum_j w'[2j] q'[2j+1] and sum_j w'[2j+1] q'[2j].
+#ifndef QF_KB
+// F65521 form: the witness rows are biased-byte v_dot2_u32_u16 sums against u16
+// pairs of pinv2 coefficients (entries biased to u = v + 128, the bias removed
+// by a per-coefficient constant), four BVA words at a
+// time; each cross sum takes one lazy product per output pair (flushed every 24).
+template <int S2>
+__global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bva, uint64_t available, uint64_t hp,
+                                                      const E8* __restrict__ pinv_g, const E8* __restrict__ q, E8 r,
+                                                      E8* __restrict__ ow, E8* __restrict__ oq,
+                                                      uint32_t* __restrict__ partials) {
+  static_assert(S2 >= 4, "rows of at least one BVA word");
+  constexpr int G = S2 / 4;   // BVA words (4 lanes) per witness row
+  constexpr int NP = S2 / 2;  // lane pairs per row
+  __shared__ E8 pinv[S2];
   __shared__ uint32_t PU[8][NP];
   __shared__ uint32_t negc[8];
-  for (int i = threadIdx.x; i < S; i += WG) pinv[i] = pinv_g[i];
+  for (int i = threadIdx.x; i < S2; i += WG) pinv[i] = pinv_g[i];
   __syncthreads();
   for (int i = threadIdx.x; i < 8 * NP; i += WG) {
     const int k = i / NP, lp = i % NP;
@@ -1004,37 +1012,52 @@ __global__ __launch_bounds__(WG) void k_eo_witness_first(const Val* __restrict__
   }
   if (threadIdx.x < 8) {
     uint32_t t = 0;
-    for (int l = 0; l < S; l++) t = madd(t, pinv[l].c[threadIdx.x]);
+    for (int l = 0; l < S2; l++) t = madd(t, pinv[l].c[threadIdx.x]);
     negc[threadIdx.x] = msub(0, uint32_t((uint64_t(t) * 128) % P));
   }
+  __shared__ uint32_t sred[2 * 8][WG];
+  for (int j = 0; j < 16; j++) sred[j][threadIdx.x] = 0;
   __syncthreads();
-  auto witness = [&](const uint32_t (&raw)[G], V8& out) {
-    uint32_t up[NP];
+  const V8 vr = v8_encode_form(v8_load(r));
+  // one witness row of S2 entries, all below `available`: each dot2 adds less
+  // than 2^25, so the NP <= 64 of a row and the bias correction stay below 2^32
+  auto witness = [&](uint64_t row, V8& out) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(bva + row * S2);
+    uint32_t acc[8];
 #pragma unroll
-    for (int g = 0; g < G; g++) {
-      const uint32_t x = raw[g] ^ 0x80808080u;
-      up[2 * g] = __builtin_amdgcn_perm(0u, x, 0x0c010c00u);      // (u0, u1)
-      if (2 * g + 1 < NP) up[2 * g + 1] = __builtin_amdgcn_perm(0u, x, 0x0c030c02u);  // (u2, u3)
+    for (int k = 0; k < 8; k++) acc[k] = negc[k];
+#pragma unroll
+    for (int g0 = 0; g0 < G; g0 += 4) {
+      uint32_t raw[4];
+#pragma unroll
+      for (int g = 0; g < 4; g++)
+        if (g0 + g < G) raw[g] = src[g0 + g];
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        if (g0 + g >= G) continue;
+        const uint32_t x = raw[g] ^ 0x80808080u;
+        const uint32_t u01 = __builtin_amdgcn_perm(0u, x, 0x0c010c00u);  // (u0, u1)
+        const uint32_t u23 = __builtin_amdgcn_perm(0u, x, 0x0c030c02u);  // (u2, u3)
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          acc[k] = dot2_u16(u01, PU[k][2 * (g0 + g)], acc[k]);
+          acc[k] = dot2_u16(u23, PU[k][2 * (g0 + g) + 1], acc[k]);
+        }
+      }
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      uint32_t acc = negc[k];
-#pragma unroll
-      for (int lp = 0; lp < NP; lp++) acc = dot2_u16(up[lp], PU[k][lp], acc);  // < 2^28.1
-      uint32_t x = fold16(fold16(acc));
+      const uint32_t x = fold16(fold16(acc[k]));
       const uint32_t y = x - P;
       out.c[k] = x < y ? x : y;
     }
   };
-  __shared__ uint32_t sred[3 * 8][WG];  // the reduced sums, touched once per flush
-  for (int j = 0; j < 24; j++) sred[j][threadIdx.x] = 0;
-  LazySum ls[3];
+  LazySum ls[2];
   lazy_zero(ls[0]);
   lazy_zero(ls[1]);
-  lazy_zero(ls[2]);
   auto flush = [&]() {
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
+    for (int j = 0; j < 2; j++) {
       const V8 v = lazy_value(ls[j]);
 #pragma unroll
       for (int k = 0; k < 8; k++) sred[8 * j + k][threadIdx.x] = madd(sred[8 * j + k][threadIdx.x], v.c[k]);
@@ -1042,71 +1065,107 @@ __global__ __launch_bounds__(WG) void k_eo_witness_first(const Val* __restrict__
     }
   };
   uint32_t pending = 0;
-  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
-    // the coefficient pairs stay in LDS: reloading them costs less than 64 registers
-    asm volatile("" ::: "memory");
-    const V8 q0 = v8_load(q[2 * i]), q1 = v8_load(q[2 * i + 1]);
-    V8 wr[2];
-    const uint64_t first = 2 * i * S;
-    if (S >= 4 && first + 2 * S <= available) {
-      uint32_t raw[2][G];
-#pragma unroll
-      for (int r = 0; r < 2; r++)
-#pragma unroll
-        for (int g = 0; g < G; g++) raw[r][g] = *reinterpret_cast<const uint32_t*>(bva + first + r * S + 4 * g);
-      witness(raw[0], wr[0]);
-      witness(raw[1], wr[1]);
+  for (uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x; j < hp; j += uint64_t(gridDim.x) * WG) {
+    asm volatile("" ::: "memory");  // the coefficient pairs stay in LDS
+    const V8 qa = v8_load(q[4 * j]), qb = v8_load(q[4 * j + 1]), qc = v8_load(q[4 * j + 2]), qd = v8_load(q[4 * j + 3]);
+    V8 w[2];
+    if ((2 * j + 2) * S2 <= available) {
+      witness(2 * j, w[0]);
+      witness(2 * j + 1, w[1]);
     } else {
-      witness_rows<S, 2>(bva, 2 * i, available, pinv, wr);
+      witness_rows<S2, 2>(bva, 2 * j, available, pinv, w);
     }
-    w[2 * i] = v8_store(wr[0]);
-    w[2 * i + 1] = v8_store(wr[1]);
-    uint32_t lo0[4], hi0[4], lo1[4], hi1[4], D[15];
-    limb_split(wr[0], lo0, hi0);
-    limb_split(wr[1], lo1, hi1);
-    fold_seq(q0, D);
-    lazy_mac(ls[0], lo0, hi0, D);  // w0 q0
-    lazy_mac(ls[2], lo1, hi1, D);  // w1 q0
+    const V8 q0 = v8_add(qa, v8_mul_enc(qb, vr)), q1 = v8_add(qc, v8_mul_enc(qd, vr));
+    ow[2 * j] = v8_store(w[0]);
+    ow[2 * j + 1] = v8_store(w[1]);
+    oq[2 * j] = v8_store(q0);
+    oq[2 * j + 1] = v8_store(q1);
+    uint32_t lo[4], hi[4], D[15];
+    limb_split(w[0], lo, hi);
     fold_seq(q1, D);
-    lazy_mac(ls[0], lo1, hi1, D);  // w1 q1
-    lazy_mac(ls[1], lo0, hi0, D);  // w0 q1
-    if (++pending == 12) {  // sum 0 holds at most 24 products < 2^31.3
+    lazy_mac(ls[0], lo, hi, D);  // w'0 q'1
+    limb_split(w[1], lo, hi);
+    fold_seq(q0, D);
+    lazy_mac(ls[1], lo, hi, D);  // w'1 q'0
+    if (++pending == 24) {
       flush();
       pending = 0;
     }
   }
   flush();
-  // block sums in place (the layout of block_sums<3>: sum j coefficient k at 8 j + k)
   __syncthreads();
   for (int half = WG / 2; half > 0; half /= 2) {
     if (int(threadIdx.x) < half)
-      for (int j = 0; j < 24; j++) sred[j][threadIdx.x] = madd(sred[j][threadIdx.x], sred[j][threadIdx.x + half]);
+      for (int j = 0; j < 16; j++) sred[j][threadIdx.x] = madd(sred[j][threadIdx.x], sred[j][threadIdx.x + half]);
     __syncthreads();
   }
-  if (threadIdx.x < 24) partials[blockIdx.x * PSTRIDE + threadIdx.x] = sred[threadIdx.x][0];
+  if (threadIdx.x < 16) partials[blockIdx.x * PSTRIDE + threadIdx.x] = sred[threadIdx.x][0];
 }
 #else
-template <int S>
-__global__ __launch_bounds__(WG) void k_eo_witness_first(const Val* __restrict__ bva, uint64_t available, uint64_t h,
-                                                         const E8* __restrict__ pinv_g, const E8* __restrict__ q,
-                                                         E8* __restrict__ w, uint32_t* __restrict__ partials) {
-  __shared__ E8 pinv[S];
-  for (int i = threadIdx.x; i < S; i += WG) pinv[i] = pinv_g[i];
+template <int S2>
+__global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bva, uint64_t available, uint64_t hp,
+                                                      const E8* __restrict__ pinv_g, const E8* __restrict__ q, E8 r,
+                                                      E8* __restrict__ ow, E8* __restrict__ oq,
+                                                      uint32_t* __restrict__ partials) {
+  __shared__ E8 pinv[S2];
+  for (int i = threadIdx.x; i < S2; i += WG) pinv[i] = pinv_g[i];
   __syncthreads();
-  V8 s[3] = {v8_zero(), v8_zero(), v8_zero()};
-  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < h; i += uint64_t(gridDim.x) * WG) {
-    const V8 q0 = v8_load(q[2 * i]), q1 = v8_load(q[2 * i + 1]);
-    V8 wr[2];
-    witness_rows<S, 2>(bva, 2 * i, available, pinv, wr);
-    w[2 * i] = v8_store(wr[0]);
-    w[2 * i + 1] = v8_store(wr[1]);
-    s[0] = v8_add(s[0], v8_add(v8_mul(wr[0], q0), v8_mul(wr[1], q1)));
-    s[1] = v8_add(s[1], v8_mul(wr[0], q1));
-    s[2] = v8_add(s[2], v8_mul(wr[1], q0));
+  const V8 vr = v8_encode_form(v8_load(r));
+  V8 s[2] = {v8_zero(), v8_zero()};
+  for (uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x; j < hp; j += uint64_t(gridDim.x) * WG) {
+    const V8 qa = v8_load(q[4 * j]), qb = v8_load(q[4 * j + 1]), qc = v8_load(q[4 * j + 2]), qd = v8_load(q[4 * j + 3]);
+    V8 w[2];
+    witness_rows<S2, 2>(bva, 2 * j, available, pinv, w);
+    const V8 q0 = v8_add(qa, v8_mul_enc(qb, vr)), q1 = v8_add(qc, v8_mul_enc(qd, vr));
+    ow[2 * j] = v8_store(w[0]);
+    ow[2 * j + 1] = v8_store(w[1]);
+    oq[2 * j] = v8_store(q0);
+    oq[2 * j + 1] = v8_store(q1);
+    s[0] = v8_add(s[0], v8_mul(w[0], q1));
+    s[1] = v8_add(s[1], v8_mul(w[1], q0));
   }
-  block_sums<3>(s, partials);
+  block_sums<2>(s, partials);
 }
 #endif
+
+// Joined lanes of stride-2s projection chunks: out[ch][l] = f[o][0] parts[ch][l]
+// + f[o][1] parts[ch][s + l], o = owner[ch] (the stride-s projection, whose row
+// bit 0 carries the term's factor f[o]).
+__global__ __launch_bounds__(WG) void k_lane_join(const E8* __restrict__ parts, const uint32_t* __restrict__ owner,
+                                                  uint32_t nch, int s, const E8 (*__restrict__ f)[2],
+                                                  E8* __restrict__ out) {
+  const uint64_t total = uint64_t(nch) * s;
+  for (uint64_t i = uint64_t(blockIdx.x) * WG + threadIdx.x; i < total; i += uint64_t(gridDim.x) * WG) {
+    const uint64_t ch = i / s, l = i % s;
+    const uint32_t o = owner[ch];
+    const E8* pr = parts + ch * 2 * s;
+    out[i] = v8_store(v8_add(v8_mul(v8_load(pr[l]), v8_load(f[o][0])), v8_mul(v8_load(pr[s + l]), v8_load(f[o][1]))));
+  }
+}
+
+// The first binary round's cross terms from the stride-2s projection chunks,
+// once the packed fold challenge c is known: per chunk u0 = sum_{l<s} c^-l
+// parts[ch][l] (the even rows of its superrows), u1 the same over the odd rows,
+// and c0 = sum coef[o][1] u0, c1 = sum coef[o][0] u1, with coef[o][b] =
+// (sum_l low_o[l] c^l) f_o[b] the folded query's weight of row bit b.
+__global__ __launch_bounds__(WG) void k_first_cross(const E8* __restrict__ parts, const uint32_t* __restrict__ owner,
+                                                    uint32_t nch, int s, const E8* __restrict__ pinv,
+                                                    const E8 (*__restrict__ coef)[2], uint32_t* __restrict__ partials) {
+  V8 acc[2] = {v8_zero(), v8_zero()};
+  for (uint32_t ch = blockIdx.x * WG + threadIdx.x; ch < nch; ch += gridDim.x * WG) {
+    const E8* pr = parts + uint64_t(ch) * 2 * s;
+    V8 u0 = v8_zero(), u1 = v8_zero();
+    for (int l = 0; l < s; l++) {
+      const V8 pw = v8_load(pinv[l]);
+      u0 = v8_add(u0, v8_mul(v8_load(pr[l]), pw));
+      u1 = v8_add(u1, v8_mul(v8_load(pr[s + l]), pw));
+    }
+    const uint32_t o = owner[ch];
+    acc[0] = v8_add(acc[0], v8_mul(u0, v8_load(coef[o][1])));
+    acc[1] = v8_add(acc[1], v8_mul(u1, v8_load(coef[o][0])));
+  }
+  block_sums<2>(acc, partials);
+}
 
 struct FillTaskDev { uint64_t start, first; uint32_t nbits, pair_off, k, count_log; E8 scale; };
 // out[start + j] (+)= scale * W(j), j in [first, first + 2^count_log): ADD
@@ -1158,8 +1217,10 @@ __global__ __launch_bounds__(WG) void k_proj_cross(const E8* __restrict__ parts,
                                                    uint32_t nchunks, int L, const E8* __restrict__ lows,
                                                    E8* __restrict__ out) {
   __shared__ uint32_t red[WG][LANES];
-  const int d = int(blockIdx.x % uint32_t(L - 1)) + 1;
-  const bool neg = blockIdx.x >= uint32_t(L - 1);
+  // block 2 (L - 1): the d = 0 correlation sum parts[ch][lane] low[owner][lane]
+  const bool zero = blockIdx.x == uint32_t(2 * (L - 1));
+  const int d = zero ? 0 : int(blockIdx.x % uint32_t(L - 1)) + 1;
+  const bool neg = !zero && blockIdx.x >= uint32_t(L - 1);
   V8 acc = v8_zero();
   for (uint32_t ch = threadIdx.x; ch < nchunks; ch += WG) {
     const E8* pr = parts + uint64_t(ch) * L;
@@ -1821,7 +1882,8 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
         case 8: hipLaunchKernelGGL(k_tensor_project<8>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
         case 16: hipLaunchKernelGGL(k_tensor_project<16>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
         case 32: hipLaunchKernelGGL(k_tensor_project<32>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
-        default: hipLaunchKernelGGL(k_tensor_project<64>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+        case 64: hipLaunchKernelGGL(k_tensor_project<64>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
+        default: hipLaunchKernelGGL(k_tensor_project<128>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o); break;
       }
     }
     first = end;
@@ -1832,28 +1894,110 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
   return out;
 }
 
-std::pair<std::vector<E8>, std::vector<E8>> tensor_project_cross(DevCtx& c, const Val* bva, uint64_t available, int L,
-                                                                 const std::vector<ProjTask>& tasks,
-                                                                 const std::vector<std::array<E8, 2>>& pairs,
-                                                                 const std::vector<E8>& lows) {
-  require(L >= 2 && L <= 64 && pow2(uint64_t(L)), QF_ERR_ARGUMENT, "projection lane count");
-  require(lows.size() == tasks.size() * size_t(L), QF_ERR_ARGUMENT, "projection low weights");
-  std::vector<E8> pos(L - 1, E8{}), neg(L - 1, E8{});
-  std::vector<uint32_t> owner;
-  DVec parts = project_chunks(c, bva, available, L, tasks, pairs, owner);
-  if (owner.empty()) return {pos, neg};
+EoProjection eo_project(DevCtx& c, const Val* bva, uint64_t available, int s, const std::vector<ProjTask>& tasks,
+                        const std::vector<std::array<E8, 2>>& pairs) {
+  require(s >= 2 && s <= 64 && pow2(uint64_t(s)), QF_ERR_ARGUMENT, "packing stride");
+  EoProjection pr;
+  pr.s = s;
+  if (launch_mode() != LAUNCH_PER_MATRIX || tasks.size() <= 1) {
+    pr.parts = project_chunks(c, bva, available, 2 * s, tasks, pairs, pr.owner);
+    return pr;
+  }
+  // the plain per-matrix form: one launch and one host round trip per bucket term
+  std::vector<DVec> each;
+  for (size_t i = 0; i < tasks.size(); i++) {
+    ProjTask one = tasks[i];
+    one.pair_off = 0;
+    std::vector<std::array<E8, 2>> own(pairs.begin() + tasks[i].pair_off,
+                                       pairs.begin() + tasks[i].pair_off + tasks[i].nbits);
+    std::vector<uint32_t> ow;
+    each.push_back(project_chunks(c, bva, available, 2 * s, {one}, own, ow));
+    c.sync();
+    pr.owner.insert(pr.owner.end(), ow.size(), uint32_t(i));
+  }
+  if (pr.owner.empty()) return pr;
+  pr.parts = DVec(pr.owner.size() * 2 * uint64_t(s));
+  uint64_t at = 0;
+  for (auto& d : each)
+    if (d.n) {
+      dv::copy(c, pr.parts.p + at, d.p, d.n);
+      at += d.n;
+    }
+  return pr;
+}
+
+EoCorr eo_correlations(DevCtx& c, const EoProjection& pr, const std::vector<std::array<E8, 2>>& f,
+                       const std::vector<E8>& lows) {
+  const int s = pr.s;
+  EoCorr r;
+  r.pos.assign(s - 1, E8{});
+  r.neg.assign(s - 1, E8{});
+  r.zero = E8{};
+  if (pr.owner.empty()) return r;
+  require(lows.size() == f.size() * size_t(s), QF_ERR_ARGUMENT, "projection low weights");
+  const uint32_t nch = uint32_t(pr.owner.size());
+  E8(*d_f)[2] = nullptr;
+  upload_pairs(c, f, &d_f);
+  auto* d_owner = static_cast<uint32_t*>(dmalloc(pr.owner.size() * 4, c.st));
+  h2d(c, d_owner, pr.owner.data(), pr.owner.size() * 4);
+  DVec joined(uint64_t(nch) * s);
+  hipLaunchKernelGGL(k_lane_join, grid_for(uint64_t(nch) * s), dim3(WG), 0, c.st, pr.parts.p, d_owner, nch, s,
+                     (const E8(*)[2])d_f, joined.p);
   DVec d_lows = dv::upload(c, lows);
-  auto* d_owner = static_cast<uint32_t*>(dmalloc(owner.size() * 4, c.st));
-  h2d(c, d_owner, owner.data(), owner.size() * 4);
-  DVec out(2 * (L - 1));
-  hipLaunchKernelGGL(k_proj_cross, dim3(2 * (L - 1)), dim3(WG), 0, c.st, parts.p, d_owner, uint32_t(owner.size()), L,
-                     d_lows.p, out.p);
+  DVec out(2 * (s - 1) + 1);
+  hipLaunchKernelGGL(k_proj_cross, dim3(2 * (s - 1) + 1), dim3(WG), 0, c.st, joined.p, d_owner, nch, s, d_lows.p,
+                     out.p);
   QF_HIP(hipGetLastError());
-  auto r = dv::download(c, out);
+  auto v = dv::download(c, out);
   dfree(d_owner, c.st);
-  std::copy(r.begin(), r.begin() + (L - 1), pos.begin());
-  std::copy(r.begin() + (L - 1), r.end(), neg.begin());
-  return {pos, neg};
+  dfree(d_f, c.st);
+  std::copy(v.begin(), v.begin() + (s - 1), r.pos.begin());
+  std::copy(v.begin() + (s - 1), v.begin() + 2 * (s - 1), r.neg.begin());
+  r.zero = v[2 * (s - 1)];
+  return r;
+}
+
+std::pair<E8, E8> eo_first_cross(DevCtx& c, const EoProjection& pr, const std::vector<E8>& pinv,
+                                 const std::vector<std::array<E8, 2>>& coef) {
+  if (pr.owner.empty()) return {E8{}, E8{}};
+  require(int(pinv.size()) == pr.s, QF_ERR_ARGUMENT, "witness weights");
+  const uint32_t nch = uint32_t(pr.owner.size());
+  auto* d_owner = static_cast<uint32_t*>(dmalloc(pr.owner.size() * 4, c.st));
+  h2d(c, d_owner, pr.owner.data(), pr.owner.size() * 4);
+  E8(*d_coef)[2] = nullptr;
+  upload_pairs(c, coef, &d_coef);
+  E8* t = put_table(c, scratch_tab(), pinv.data(), pinv.size() * 16);
+  const uint32_t g = groups_for(nch);
+  hipLaunchKernelGGL(k_first_cross, dim3(g), dim3(WG), 0, c.st, pr.parts.p, d_owner, nch, pr.s, t,
+                     (const E8(*)[2])d_coef, c.partials);
+  QF_HIP(hipGetLastError());
+  auto sums = finish_sums<2>(c, g);
+  dfree(d_owner, c.st);
+  dfree(d_coef, c.st);
+  return {sums[0], sums[1]};
+}
+
+std::pair<E8, E8> eo_first_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t rows, int s,
+                                const std::vector<E8>& pinv2, const E8* q, const E8& r, E8* ow, E8* oq) {
+  require(s >= 2 && s <= 64 && pow2(uint64_t(s)) && int(pinv2.size()) == 2 * s, QF_ERR_ARGUMENT, "witness stride");
+  require(rows >= 4 && rows % 4 == 0, QF_ERR_ARGUMENT, "witness rows");
+  E8* t = put_table(c, scratch_tab(), pinv2.data(), pinv2.size() * 16);
+  const uint64_t hp = rows / 4;
+  const uint32_t g = groups_for(hp);
+#define QF_FIRST_FOLD(S2) \
+  hipLaunchKernelGGL(k_eo_first_fold<S2>, dim3(g), dim3(WG), 0, c.st, bva, available, hp, t, q, r, ow, oq, c.partials)
+  switch (2 * s) {
+    case 4: QF_FIRST_FOLD(4); break;
+    case 8: QF_FIRST_FOLD(8); break;
+    case 16: QF_FIRST_FOLD(16); break;
+    case 32: QF_FIRST_FOLD(32); break;
+    case 64: QF_FIRST_FOLD(64); break;
+    default: QF_FIRST_FOLD(128); break;
+  }
+#undef QF_FIRST_FOLD
+  QF_HIP(hipGetLastError());
+  auto sums = finish_sums<2>(c, g);
+  return {sums[0], sums[1]};
 }
 
 // One launch over every task (the batched form).
@@ -2003,25 +2147,6 @@ void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& ta
     dfree(d_gaps, c.st);
   }
   fill_tasks<false>(c, out, tasks, pairs);
-}
-
-std::array<E8, 3> eo_witness_first(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s,
-                                   const std::vector<E8>& pinv, const E8* q, E8* w) {
-  require(s >= 2 && s <= 64 && pow2(s) && int(pinv.size()) == s, QF_ERR_ARGUMENT, "witness stride");
-  require(n_rows >= 2 && n_rows % 2 == 0, QF_ERR_ARGUMENT, "witness rows");
-  E8* t = put_table(c, scratch_tab(), pinv.data(), pinv.size() * 16);
-  const uint64_t h = n_rows / 2;
-  const uint32_t g = groups_for(h);
-  switch (s) {
-    case 2: hipLaunchKernelGGL(k_eo_witness_first<2>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
-    case 4: hipLaunchKernelGGL(k_eo_witness_first<4>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
-    case 8: hipLaunchKernelGGL(k_eo_witness_first<8>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
-    case 16: hipLaunchKernelGGL(k_eo_witness_first<16>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
-    case 32: hipLaunchKernelGGL(k_eo_witness_first<32>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
-    default: hipLaunchKernelGGL(k_eo_witness_first<64>, dim3(g), dim3(WG), 0, c.st, bva, available, h, t, q, w, c.partials); break;
-  }
-  QF_HIP(hipGetLastError());
-  return finish_sums<3>(c, g);
 }
 
 void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,

@@ -107,13 +107,39 @@ struct ProjTask {
 // Returns tasks.size() x L lane sums.
 std::vector<E8> tensor_project(DevCtx& c, const Val* bva, uint64_t available, int L, const std::vector<ProjTask>& tasks,
                                const std::vector<std::array<E8, 2>>& pairs);
-// The shifted correlations of the projections with per-task low-bit weights
-// (lows: tasks x L), without bringing the projections to the host:
-// pos[d-1] = sum_task sum_lane proj[lane] low[lane+d], neg[d-1] = sum proj[lane+d] low[lane].
-std::pair<std::vector<E8>, std::vector<E8>> tensor_project_cross(DevCtx& c, const Val* bva, uint64_t available, int L,
-                                                                 const std::vector<ProjTask>& tasks,
-                                                                 const std::vector<std::array<E8, 2>>& pairs,
-                                                                 const std::vector<E8>& lows);
+// Packed EO (eo.rs:97-181) from projections at stride 2s. Each task is a
+// query term of at least 2s entries (its offset a multiple of 2s): rows =
+// len / 2s superrows, pairs = the term's factors above the row bit 0 of
+// stride s. The chunk lane sums (chunks x 2s) stay on the device.
+struct EoProjection {
+  DVec parts;
+  std::vector<uint32_t> owner;  // task of each chunk
+  int s = 0;
+};
+EoProjection eo_project(DevCtx& c, const Val* bva, uint64_t available, int s, const std::vector<ProjTask>& tasks,
+                        const std::vector<std::array<E8, 2>>& pairs);
+// The stride-s projections (each chunk's two halves joined with its task's
+// row-bit-0 factor f[task]) correlated with the tasks' lane weights (lows:
+// tasks x s): pos[d-1] = sum proj[lane] low[lane+d], neg[d-1] = sum
+// proj[lane+d] low[lane] (tbv.rs:361-392), and zero = sum proj[lane] low[lane],
+// the terms' part of sum_i v[i] q[i].
+struct EoCorr {
+  std::vector<E8> pos, neg;
+  E8 zero;
+};
+EoCorr eo_correlations(DevCtx& c, const EoProjection& pr, const std::vector<std::array<E8, 2>>& f,
+                       const std::vector<E8>& lows);
+// The terms' part of the first binary round's cross terms, sum_j w[2j] q[2j+1]
+// and sum_j w[2j+1] q[2j], once the packed challenge c is known: pinv[l] =
+// c^-l (l < s), coef[task] = (sum_l low[l] c^l) x the task's row-bit-0 factor.
+std::pair<E8, E8> eo_first_cross(DevCtx& c, const EoProjection& pr, const std::vector<E8>& pinv,
+                                 const std::vector<std::array<E8, 2>>& coef);
+// The first binary round read from the BVA (rows = logical / s, a multiple of
+// 4): ow[i] = sum_{L<2s} v[2is + L] pinv2[L] (the fold of witness rows 2i,
+// 2i+1 by r^-1), oq[i] = q[2i] + q[2i+1] r, for i < rows / 2; returns the next
+// round's cross terms over the outputs.
+std::pair<E8, E8> eo_first_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t rows, int s,
+                                const std::vector<E8>& pinv2, const E8* q, const E8& r, E8* ow, E8* oq);
 // Packed EO witness: w[row] = sum_lane v[row*s + lane] * pinv[lane] for rows < n_rows.
 void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
                 E8* w);
@@ -124,11 +150,6 @@ void tensor_fill_add(DevCtx& c, E8* out, const std::vector<FillTask>& tasks, con
 // being read and the rows outside every task are zeroed.
 void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& tasks,
                  const std::vector<std::array<E8, 2>>& pairs);
-// eo_witness for n_rows (even) rows fused with the packed reduction's
-// consistency sum and the first binary round's cross terms against the built
-// query q: {<w, q>, sum w[2i] q[2i+1], sum w[2i+1] q[2i]}.
-std::array<E8, 3> eo_witness_first(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s,
-                                   const std::vector<E8>& pinv, const E8* q, E8* w);
 // values[i] = sum_b fw[b] * packed(b, i), lane j of packed(b, i) = entry b*block_len + LANES*i + j
 void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
                 const std::vector<E8>& fw, E8* values, uint64_t n);
