@@ -808,33 +808,45 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
       for (uint64_t j = 0; j < s; j++) acc += term_low[i][j] * pows[j];
       fold_scale[i] = acc;
     }
-  DVec qv(rows);
-  // query fold (tbv.rs:393-428): bucket terms (written, the rows outside them
-  // zeroed), then padding/suffix terms added; each phase has disjoint supports.
-  std::map<uint64_t, E8> small;
-  for (int phase = 0; phase < 2; phase++) {
-    std::vector<FillTask> ftasks;
-    std::vector<std::array<E8, 2>> fpairs;
-    size_t a = phase ? q.n_bucket_terms : 0, b = phase ? q.terms.size() : q.n_bucket_terms;
-    for (size_t i = a; i < b; i++) {
-      const Term& t = q.terms[i];
-      if (t.len() >= s) {
-        FillTask ft{t.offset / s, uint32_t(t.f.size() - ls), uint32_t(fpairs.size()), t.scale * fold_scale[i]};
-        for (size_t bb = ls; bb < t.f.size(); bb++) fpairs.push_back(t.f[bb]);
-        ftasks.push_back(ft);
-      } else {
-        for (uint64_t j = 0; j < t.len(); j++) small[t.offset / s] += t.weight(j) * pows[(t.offset + j) % s];
+  // The query fold (tbv.rs:393-428) into out, at stride s (level 0: qf, rows
+  // entries) or folded once more by the first binary challenge r (level 1:
+  // q'[i] = qf[2i] + qf[2i+1] r, rows / 2 entries; a term's row-bit-0 factor
+  // joins its scale). Bucket terms are written (the rows outside them zeroed),
+  // then padding and suffix terms added; each phase has disjoint supports.
+  // Terms that do not span a whole row pair at the level go in as single values.
+  auto build_query = [&](E8* out, int level, const E8& r) {
+    const uint64_t span = s << level;  // entries per output row
+    std::map<uint64_t, E8> single;
+    for (int phase = 0; phase < 2; phase++) {
+      std::vector<FillTask> ftasks;
+      std::vector<std::array<E8, 2>> fpairs;
+      size_t a = phase ? q.n_bucket_terms : 0, b = phase ? q.terms.size() : q.n_bucket_terms;
+      for (size_t i = a; i < b; i++) {
+        const Term& t = q.terms[i];
+        if (t.len() >= span) {
+          E8 sc = t.scale * fold_scale[i];
+          if (level) sc = sc * (t.f[ls][0] + r * t.f[ls][1]);
+          FillTask ft{t.offset / span, uint32_t(t.f.size() - ls - level), uint32_t(fpairs.size()), sc};
+          for (size_t bb = ls + level; bb < t.f.size(); bb++) fpairs.push_back(t.f[bb]);
+          ftasks.push_back(ft);
+        } else {
+          E8 v{};  // the term's part of qf at its row k = offset / s
+          if (t.len() >= s) v = t.scale * fold_scale[i];
+          else
+            for (uint64_t j = 0; j < t.len(); j++) v += t.weight(j) * pows[(t.offset + j) % s];
+          const uint64_t k = t.offset / s;
+          if (level && (k & 1)) v = v * r;
+          single[k >> level] += v;
+        }
       }
+      if (phase == 0) tensor_fill(c, out, rows >> level, ftasks, fpairs);
+      else tensor_fill_add(c, out, ftasks, fpairs);
     }
-    if (phase == 0) tensor_fill(c, qv.p, rows, ftasks, fpairs);
-    else tensor_fill_add(c, qv.p, ftasks, fpairs);
-  }
-  {
     std::vector<uint64_t> idx;
     std::vector<E8> vals;
-    for (auto& [k, v] : small) { idx.push_back(k); vals.push_back(v); }
-    dv::scatter_add(c, qv.p, idx, vals);
-  }
+    for (auto& [k, v] : single) { idx.push_back(k); vals.push_back(v); }
+    dv::scatter_add(c, out, idx, vals);
+  };
   E8 claim = q.claim, fwd = ch, bwd = inv;
   for (uint64_t d = 0; d + 1 < s; d++) { claim += o.pos[d] * fwd + o.neg[d] * bwd; fwd = fwd * ch; bwd = bwd * inv; }
   // the first binary round's cross terms sum_j w[2j] qf[2j+1], sum_j w[2j+1] qf[2j]
@@ -857,11 +869,16 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
       else c1 += wv * qf;
     }
   }
-  // Long vectors: the first round reads the BVA (no witness of stride s); short
-  // ones materialize the witness and take the rounds below.
+  // Long vectors: the first round reads the BVA and the folded query (no
+  // witness or query of stride s is built); short ones materialize both and
+  // take the rounds below.
   const bool from_bva = rows > TwoFold::HOST_TAIL;
-  DVec w(from_bva ? rows / 4 : rows), w2(rows / 2 ? rows / 2 : 1), q2(rows / 2 ? rows / 2 : 1);
-  if (!from_bva) eo_witness(c, bva, l.active, rows, int(s), pinv, w.p);
+  DVec w(from_bva ? rows / 4 : rows), qv(from_bva ? rows / 4 : rows), w2(rows / 2 ? rows / 2 : 1),
+      q2(rows / 2 ? rows / 2 : 1);
+  if (!from_bva) {
+    build_query(qv.p, 0, E8{});
+    eo_witness(c, bva, l.active, rows, int(s), pinv, w.p);
+  }
   DVec *cw = &w, *cq = &qv, *nw = &w2, *nq = &q2;
   uint64_t n = rows;
   // the last rounds run on host copies once the vectors are short (as TwoFold)
@@ -877,8 +894,9 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     if (first && from_bva) {  // w'[i] = sum_{L<2s} v[2is + L] pinv2[L], pinv2 = (c^-l, r^-1 c^-l)
       std::vector<E8> pinv2(2 * s);
       for (uint64_t lane = 0; lane < s; lane++) { pinv2[lane] = pinv[lane]; pinv2[s + lane] = ri * pinv[lane]; }
-      std::tie(c0, c1) = eo_first_fold(c, bva, l.active, rows, int(s), pinv2, qv.p, r, w2.p, q2.p);
-      // outputs in (w2, q2); the next round writes into w (rows / 4) and qv
+      build_query(q2.p, 1, r);
+      std::tie(c0, c1) = eo_first_fold(c, bva, l.active, rows, int(s), pinv2, q2.p, w2.p);
+      // outputs in (w2, q2); the next round writes into w and qv (rows / 4 each)
       cw = &w2;
       cq = &q2;
       nw = &w;

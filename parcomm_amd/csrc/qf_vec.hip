@@ -575,11 +575,13 @@ __device__ inline long tr8_b64(const uint8_t* p) {
   return (long(uint32_t(v.y)) << 32) | long(uint32_t(v.x));
 }
 #ifndef QF_KB
-// Projection chunks of 16- and 8-lane rows (the EO cross-term and the batch
-// lane projections) as an exact i8 MFMA contraction of the BVA against the
-// weight table, over 16-byte records: one 16-lane row, or two consecutive
-// 8-lane rows 2r', 2r'+1, whose weights pairs[0][b] T'[r'] share the table T'
-// over the row bits above bit 0.
+// Projection chunks of 8-, 16-, 32- and 64-lane rows (the batch lane
+// projections, and the EO projections at stride 2s) as an exact i8 MFMA
+// contraction of the BVA against the weight table, over 16-byte records: two
+// consecutive 8-lane rows 2r', 2r'+1, whose weights pairs[0][b] T'[r'] share
+// the table T' over the row bits above bit 0; one 16-lane row; or one of the
+// NS = L / 16 records of a longer row, each accumulated as its own stream
+// against the row's weight (lanes 16 s .. 16 s + 15 of the row in stream s).
 // - Digits: a canonical table coefficient t is written as t' = t or t - p in
 //   [-32881, 32639]; h = (t' + 128) >> 8 and l = t' - 256 h then both lie in
 //   [-128, 127] and t' = 256 h + l. Record r of the digit table holds the low
@@ -600,16 +602,18 @@ template <int L>
 __global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restrict__ bva, uint64_t available,
                                                             const ProjTaskDev* __restrict__ tasks,
                                                             const E8 (*__restrict__ pairs)[2], E8* __restrict__ out) {
-  static_assert(L == 16 || L == 8, "16-byte records of one or two rows");
+  static_assert(L == 64 || L == 32 || L == 16 || L == 8, "16-byte records of two rows, one row, or part of a row");
   constexpr int RBITS = L == 8 ? 1 : 0;            // row bits inside a record
-  __shared__ E8 T[1 << MF_TAB_LOG];                // record weights, then their digit bytes
+  constexpr int NS = L <= 16 ? 1 : L / 16;         // records (streams) per row
+  constexpr int RT = MF_ROWS / NS;                 // table entries per wave and round
+  __shared__ E8 T[1 << MF_TAB_LOG];                // table entry weights, then their digit bytes
   __shared__ uint4 stage[MF_WAVES][MF_ROWS];       // a round of BVA records per wave; the combine reuses it
-  __shared__ uint32_t pv[16][8];                   // a group's record-byte sums mod p
+  __shared__ uint32_t pv[16 * NS][8];              // a group's record-byte sums mod p, per stream
   __shared__ E8 Chi[32];                           // scale x high-bit weights of each hi group
   const ProjTaskDev t = tasks[blockIdx.x];
   const E8(*tp)[2] = pairs + t.pair_off;
   build_table(T, tp + RBITS, int(t.k) - RBITS);
-  const uint32_t lo_rows = 1u << t.k, lo_recs = lo_rows >> RBITS;
+  const uint32_t lo_rows = 1u << t.k, lo_recs = lo_rows >> RBITS;  // table entries of a hi group
   const uint32_t hi_count = 1u << (t.chunk_rows_log - t.k);  // <= 32
   if (threadIdx.x < hi_count) {
     const uint64_t row_hi = t.chunk_first + uint64_t(threadIdx.x) * lo_rows;
@@ -633,11 +637,11 @@ __global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restric
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint8_t* tb = reinterpret_cast<const uint8_t*>(T);
   uint8_t* st = reinterpret_cast<uint8_t*>(stage[wave]);
-  const uint32_t wrecs = lo_recs / MF_WAVES, rounds = wrecs / MF_ROWS, total = hi_count * rounds;
-  // round j: hi group j / rounds, this wave's records [r0, r0 + MF_ROWS) of it
+  const uint32_t wrecs = lo_recs / MF_WAVES, rounds = wrecs / RT, total = hi_count * rounds;
+  // round j: hi group j / rounds, this wave's table entries [r0, r0 + RT) of it (MF_ROWS records)
   auto fetch = [&](uint32_t j, uint4 (&w)[4]) {
-    const uint32_t r0 = wave * wrecs + (j % rounds) * MF_ROWS;
-    const uint64_t b0 = t.base + (t.chunk_first + uint64_t(j / rounds) * lo_rows) * L + uint64_t(r0) * 16;
+    const uint32_t r0 = wave * wrecs + (j % rounds) * RT;
+    const uint64_t b0 = t.base + (t.chunk_first + uint64_t(j / rounds) * lo_rows) * L + uint64_t(r0) * 16 * NS;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const uint64_t b = b0 + 1024 * u + 16 * lane;
@@ -652,40 +656,52 @@ __global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restric
     }
   };
   V8 acc = v8_zero();  // threads 0..L-1: lane threadIdx.x of the chunk
-  qf_i32x4 dsum = {0, 0, 0, 0};
+  qf_i32x4 dsum[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) dsum[s] = qf_i32x4{0, 0, 0, 0};
   uint4 cur[4], nxt[4];
   fetch(0, cur);
   for (uint32_t j = 0; j < total; j++) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the previous round's reads of st are done
 #pragma unroll
-    for (int u = 0; u < 4; u++) *reinterpret_cast<uint4*>(st + 1024 * u + 16 * lane) = cur[u];
+    for (int u = 0; u < 4; u++) {  // record q = 64 u + lane goes to stream q % NS
+      const uint32_t q = 64 * u + lane;
+      *reinterpret_cast<uint4*>(st + (q % NS) * (16 * RT) + 16 * (q / NS)) = cur[u];
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     if (j + 1 < total) fetch(j + 1, nxt);
-    const uint32_t r0 = wave * wrecs + (j % rounds) * MF_ROWS;
+    const uint32_t r0 = wave * wrecs + (j % rounds) * RT;
 #pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const long a = tr8_b64(st + 512 * b + 8 * lane);
-      const long w = tr8_b64(tb + 16 * (r0 + 32 * b) + 8 * lane);
-      dsum = __builtin_amdgcn_mfma_i32_16x16x32_i8(a, w, dsum, 0, 0, 0);
-    }
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+      for (int b = 0; b < RT / 32; b++) {
+        const long a = tr8_b64(st + s * (16 * RT) + 512 * b + 8 * lane);
+        const long w = tr8_b64(tb + 16 * (r0 + 32 * b) + 8 * lane);
+        dsum[s] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a, w, dsum[s], 0, 0, 0);
+      }
 #pragma unroll
     for (int u = 0; u < 4; u++) cur[u] = nxt[u];
     if ((j + 1) % rounds) continue;
     // end of hi group j / rounds: D[m][n] sits in lane n + 16 (m / 4), element m % 4
     __syncthreads();  // every wave is done with its staging buffer
-    int32_t* red = reinterpret_cast<int32_t*>(&stage[0][0]);  // [wave][16 m][16 n]
+    int32_t* red = reinterpret_cast<int32_t*>(&stage[0][0]);  // [stream][wave][16 m][16 n]
 #pragma unroll
-    for (int i = 0; i < 4; i++) red[wave * 256 + (4 * (lane / 16) + i) * 16 + lane % 16] = dsum[i];
-    dsum = qf_i32x4{0, 0, 0, 0};
+    for (int s = 0; s < NS; s++) {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        red[(s * MF_WAVES + wave) * 256 + (4 * (lane / 16) + i) * 16 + lane % 16] = dsum[s][i];
+      dsum[s] = qf_i32x4{0, 0, 0, 0};
+    }
     __syncthreads();
-    if (threadIdx.x < 128) {
-      const int m = threadIdx.x / 8, c = threadIdx.x % 8;
+    for (int i = threadIdx.x; i < 128 * NS; i += WG) {  // stream s, record byte m, coefficient c
+      const int s = i / 128, m = (i / 8) % 16, c = i % 8;
+      const int32_t* rs = red + s * MF_WAVES * 256;
       int64_t lo = 0, hi = 0;
       for (int w2 = 0; w2 < MF_WAVES; w2++) {
-        lo += red[w2 * 256 + m * 16 + c];
-        hi += red[w2 * 256 + m * 16 + 8 + c];
+        lo += rs[w2 * 256 + m * 16 + c];
+        hi += rs[w2 * 256 + m * 16 + 8 + c];
       }
-      pv[m][c] = mod_signed(lo + 256 * hi);
+      pv[16 * s + m][c] = mod_signed(lo + 256 * hi);
     }
     __syncthreads();
     if (threadIdx.x < L) {
@@ -704,9 +720,9 @@ __global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restric
 }
 #else
 // KoalaBear form of the MFMA projection, over 16-byte records of 8 i16
-// entries: one half of a 16-lane row (the halves accumulate as two streams
-// against the same weight), one 8-lane row, or two consecutive 4-lane rows
-// joined with pairs[0] as in the F65521 form.
+// entries: one of the NS = L / 8 records of a 16- or 32-lane row (each
+// accumulates as its own stream against the row's weight), one 8-lane row, or
+// two consecutive 4-lane rows joined with pairs[0] as in the F65521 form.
 // - Entries: an i16 entry v is 256 vh + (vl - 128) + 128 with its high byte vh
 //   and its offset low byte vl - 128 both in [-128, 127]; staging flips bit 7
 //   of each low byte, which gives that byte, and the 128 sum_r t_r this leaves
@@ -724,13 +740,13 @@ template <int L>
 __global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restrict__ bva, uint64_t available,
                                                             const ProjTaskDev* __restrict__ tasks,
                                                             const E8 (*__restrict__ pairs)[2], E8* __restrict__ out) {
-  static_assert(L == 16 || L == 8 || L == 4, "records of half a row, one row or two rows");
+  static_assert(L == 32 || L == 16 || L == 8 || L == 4, "records of part of a row, one row or two rows");
   constexpr int RBITS = L == 4 ? 1 : 0;            // row bits inside a record
-  constexpr int NS = L == 16 ? 2 : 1;              // records (streams) per row
+  constexpr int NS = L >= 16 ? L / 8 : 1;          // records (streams) per row
   constexpr int RT = MF_ROWS / NS;                 // table entries per wave and round
   __shared__ E8 T[1 << MF_TAB_LOG];                // table entry weights, then their digit bytes
   __shared__ uint4 stage[MF_WAVES][MF_ROWS];       // a round of BVA records per wave; the combine reuses it
-  __shared__ uint32_t pv[16][4];                   // a group's record-entry sums mod p
+  __shared__ uint32_t pv[8 * NS < 16 ? 16 : 8 * NS][4];  // a group's record-entry sums mod p
   __shared__ E8 Chi[32];                           // scale x high-bit weights of each hi group
   __shared__ unsigned long long ssum[4];           // sum over the table of t', two's complement
   const ProjTaskDev t = tasks[blockIdx.x];
@@ -795,9 +811,9 @@ __global__ __launch_bounds__(WG) void k_tensor_project_mfma(const Val* __restric
   for (uint32_t j = 0; j < total; j++) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the previous round's reads of st are done
 #pragma unroll
-    for (int u = 0; u < 4; u++) {  // record q = 64 u + lane; with two streams the even ones go first
+    for (int u = 0; u < 4; u++) {  // record q = 64 u + lane goes to stream q % NS
       const uint32_t q = 64 * u + lane;
-      const uint32_t at = NS == 2 ? (q & 1) * (16 * RT) + 16 * (q >> 1) : 16 * q;
+      const uint32_t at = (q % NS) * (16 * RT) + 16 * (q / NS);
       const uint32_t f = 0x00800080u;  // bit 7 of each low byte
       *reinterpret_cast<uint4*>(st + at) = make_uint4(cur[u].x ^ f, cur[u].y ^ f, cur[u].z ^ f, cur[u].w ^ f);
     }
@@ -983,11 +999,12 @@ __device__ inline V8 lazy_value(const LazySum& s) {  // sl, sh < 2^32
 // The first binary EO round folds witness rows 2i and 2i + 1 of stride s
 // (eo.rs:119-132, 140-163): w'[i] = w[2i] + w[2i+1] r^-1 is itself a witness
 // row of stride S2 = 2s over entries [2 i s, 2 i s + 2s), with the weights
-// pinv2[L] = c^-L (L < s) and r^-1 c^-(L-s) (L >= s). The fold therefore reads
-// the BVA instead of a materialized witness, and the witness of stride s is
-// never written. The query folds as q'[i] = q[2i] + q[2i+1] r; the pass
-// returns the next round's cross terms over its outputs,
-// sum_j w'[2j] q'[2j+1] and sum_j w'[2j+1] q'[2j].
+// pinv2[L] = c^-L (L < s) and r^-1 c^-(L-s) (L >= s). The pass therefore
+// reads the BVA instead of a materialized witness, and the witness of stride s
+// is never written. The folded query q'[i] = q[2i] + q[2i+1] r is built
+// beforehand (its tensor terms fold like the query's own), so the pass only
+// reads it, writes w', and returns the next round's cross terms over its
+// outputs, sum_j w'[2j] q'[2j+1] and sum_j w'[2j+1] q'[2j].
 #ifndef QF_KB
 // F65521 form: the witness rows are biased-byte v_dot2_u32_u16 sums against u16
 // pairs of pinv2 coefficients (entries biased to u = v + 128, the bias removed
@@ -995,9 +1012,8 @@ __device__ inline V8 lazy_value(const LazySum& s) {  // sl, sh < 2^32
 // time; each cross sum takes one lazy product per output pair (flushed every 24).
 template <int S2>
 __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bva, uint64_t available, uint64_t hp,
-                                                      const E8* __restrict__ pinv_g, const E8* __restrict__ q, E8 r,
-                                                      E8* __restrict__ ow, E8* __restrict__ oq,
-                                                      uint32_t* __restrict__ partials) {
+                                                      const E8* __restrict__ pinv_g, const E8* __restrict__ qf,
+                                                      E8* __restrict__ ow, uint32_t* __restrict__ partials) {
   static_assert(S2 >= 4, "rows of at least one BVA word");
   constexpr int G = S2 / 4;   // BVA words (4 lanes) per witness row
   constexpr int NP = S2 / 2;  // lane pairs per row
@@ -1018,7 +1034,6 @@ __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bv
   __shared__ uint32_t sred[2 * 8][WG];
   for (int j = 0; j < 16; j++) sred[j][threadIdx.x] = 0;
   __syncthreads();
-  const V8 vr = v8_encode_form(v8_load(r));
   // one witness row of S2 entries, all below `available`: each dot2 adds less
   // than 2^25, so the NP <= 64 of a row and the bias correction stay below 2^32
   auto witness = [&](uint64_t row, V8& out) {
@@ -1067,7 +1082,7 @@ __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bv
   uint32_t pending = 0;
   for (uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x; j < hp; j += uint64_t(gridDim.x) * WG) {
     asm volatile("" ::: "memory");  // the coefficient pairs stay in LDS
-    const V8 qa = v8_load(q[4 * j]), qb = v8_load(q[4 * j + 1]), qc = v8_load(q[4 * j + 2]), qd = v8_load(q[4 * j + 3]);
+    const V8 q0 = v8_load(qf[2 * j]), q1 = v8_load(qf[2 * j + 1]);
     V8 w[2];
     if ((2 * j + 2) * S2 <= available) {
       witness(2 * j, w[0]);
@@ -1075,11 +1090,8 @@ __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bv
     } else {
       witness_rows<S2, 2>(bva, 2 * j, available, pinv, w);
     }
-    const V8 q0 = v8_add(qa, v8_mul_enc(qb, vr)), q1 = v8_add(qc, v8_mul_enc(qd, vr));
     ow[2 * j] = v8_store(w[0]);
     ow[2 * j + 1] = v8_store(w[1]);
-    oq[2 * j] = v8_store(q0);
-    oq[2 * j + 1] = v8_store(q1);
     uint32_t lo[4], hi[4], D[15];
     limb_split(w[0], lo, hi);
     fold_seq(q1, D);
@@ -1104,23 +1116,18 @@ __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bv
 #else
 template <int S2>
 __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bva, uint64_t available, uint64_t hp,
-                                                      const E8* __restrict__ pinv_g, const E8* __restrict__ q, E8 r,
-                                                      E8* __restrict__ ow, E8* __restrict__ oq,
-                                                      uint32_t* __restrict__ partials) {
+                                                      const E8* __restrict__ pinv_g, const E8* __restrict__ qf,
+                                                      E8* __restrict__ ow, uint32_t* __restrict__ partials) {
   __shared__ E8 pinv[S2];
   for (int i = threadIdx.x; i < S2; i += WG) pinv[i] = pinv_g[i];
   __syncthreads();
-  const V8 vr = v8_encode_form(v8_load(r));
   V8 s[2] = {v8_zero(), v8_zero()};
   for (uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x; j < hp; j += uint64_t(gridDim.x) * WG) {
-    const V8 qa = v8_load(q[4 * j]), qb = v8_load(q[4 * j + 1]), qc = v8_load(q[4 * j + 2]), qd = v8_load(q[4 * j + 3]);
+    const V8 q0 = v8_load(qf[2 * j]), q1 = v8_load(qf[2 * j + 1]);
     V8 w[2];
     witness_rows<S2, 2>(bva, 2 * j, available, pinv, w);
-    const V8 q0 = v8_add(qa, v8_mul_enc(qb, vr)), q1 = v8_add(qc, v8_mul_enc(qd, vr));
     ow[2 * j] = v8_store(w[0]);
     ow[2 * j + 1] = v8_store(w[1]);
-    oq[2 * j] = v8_store(q0);
-    oq[2 * j + 1] = v8_store(q1);
     s[0] = v8_add(s[0], v8_mul(w[0], q1));
     s[1] = v8_add(s[1], v8_mul(w[1], q0));
   }
@@ -1839,8 +1846,9 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
     // holds 2^(k - rbits) <= 2^MF_TAB_LOG table entries, MF_WAVES MF_ROWS / ns
     // of them at least
     constexpr int REC = 16 / int(sizeof(Val));
-    const bool shape = LANES == 8 ? (L == 16 || L == 8) : (L == 16 || L == 8 || L == 4);
-    const uint32_t rbits = L < REC ? 1 : 0, nslog = L > REC ? 1 : 0;
+    // F65521: 8 to 64 lanes (up to four records per row); KoalaBear: 4 to 32 (up to four)
+    const bool shape = LANES == 8 ? (L >= 8 && L <= 64) : (L >= 4 && L <= 32);
+    const uint32_t rbits = L < REC ? 1 : 0, nslog = L > REC ? uint32_t(ilog2u(uint64_t(L / REC))) : 0;
     const bool mfma = use_mfma && shape && t.base % REC == 0 && t.nbits + nslog >= 10 + rbits;
     uint32_t k = std::min<uint32_t>(t.nbits, mfma ? MF_TAB_LOG + rbits : PROJ_TAB_LOG);
     uint32_t clog = std::min<uint32_t>(t.nbits, 16);
@@ -1871,6 +1879,12 @@ static DVec project_chunks(DevCtx& c, const Val* bva, uint64_t available, int L,
         hipLaunchKernelGGL(k_tensor_project_mfma<16>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
       else if (L == 8)
         hipLaunchKernelGGL(k_tensor_project_mfma<8>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
+      else if (L == 32)
+        hipLaunchKernelGGL(k_tensor_project_mfma<32>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
+#ifndef QF_KB
+      else if (L == 64)
+        hipLaunchKernelGGL(k_tensor_project_mfma<64>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
+#endif
 #ifdef QF_KB
       else
         hipLaunchKernelGGL(k_tensor_project_mfma<4>, dim3(n), dim3(WG), 0, c.st, bva, available, tk, pr, o);
@@ -1978,14 +1992,14 @@ std::pair<E8, E8> eo_first_cross(DevCtx& c, const EoProjection& pr, const std::v
 }
 
 std::pair<E8, E8> eo_first_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t rows, int s,
-                                const std::vector<E8>& pinv2, const E8* q, const E8& r, E8* ow, E8* oq) {
+                                const std::vector<E8>& pinv2, const E8* qf, E8* ow) {
   require(s >= 2 && s <= 64 && pow2(uint64_t(s)) && int(pinv2.size()) == 2 * s, QF_ERR_ARGUMENT, "witness stride");
   require(rows >= 4 && rows % 4 == 0, QF_ERR_ARGUMENT, "witness rows");
   E8* t = put_table(c, scratch_tab(), pinv2.data(), pinv2.size() * 16);
   const uint64_t hp = rows / 4;
   const uint32_t g = groups_for(hp);
 #define QF_FIRST_FOLD(S2) \
-  hipLaunchKernelGGL(k_eo_first_fold<S2>, dim3(g), dim3(WG), 0, c.st, bva, available, hp, t, q, r, ow, oq, c.partials)
+  hipLaunchKernelGGL(k_eo_first_fold<S2>, dim3(g), dim3(WG), 0, c.st, bva, available, hp, t, qf, ow, c.partials)
   switch (2 * s) {
     case 4: QF_FIRST_FOLD(4); break;
     case 8: QF_FIRST_FOLD(8); break;

@@ -136,10 +136,10 @@ std::pair<E8, E8> eo_first_cross(DevCtx& c, const EoProjection& pr, const std::v
                                  const std::vector<std::array<E8, 2>>& coef);
 // The first binary round read from the BVA (rows = logical / s, a multiple of
 // 4): ow[i] = sum_{L<2s} v[2is + L] pinv2[L] (the fold of witness rows 2i,
-// 2i+1 by r^-1), oq[i] = q[2i] + q[2i+1] r, for i < rows / 2; returns the next
-// round's cross terms over the outputs.
+// 2i+1 by r^-1) for i < rows / 2, with the folded query qf (rows / 2, built
+// beforehand); returns the next round's cross terms over the outputs.
 std::pair<E8, E8> eo_first_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t rows, int s,
-                                const std::vector<E8>& pinv2, const E8* q, const E8& r, E8* ow, E8* oq);
+                                const std::vector<E8>& pinv2, const E8* qf, E8* ow);
 // Packed EO witness: w[row] = sum_lane v[row*s + lane] * pinv[lane] for rows < n_rows.
 void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
                 E8* w);

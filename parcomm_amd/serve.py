@@ -31,10 +31,12 @@ def device_memory(field=QF_FIELD_F65521):
 
 def request_bytes(params):
     """Device memory one opening of these parameters holds at its peak: the
-    packed even/odd witness and query (3 x logical/s elements of 16 bytes),
-    the batched folded codeword and tree, and the sumcheck vectors."""
+    even/odd witness and query vectors (from the first fold on: logical/2s
+    elements each, then a quarter of that each; 2 x logical/s elements of 16
+    bytes bound them with a margin), the batched folded codeword and tree, and
+    the sumcheck vectors."""
     s = 1 << params.config.packed_log
-    eo = 3 * (params.logical // s) * 16
+    eo = 2 * (params.logical // s) * 16
     n_values = params.block_len // field_info(params.field)["lanes"]
     folded = 2 * n_values * 16 + 2 * (n_values // 16) * 32
     return eo + folded + 2 * n_values * 16
