@@ -214,22 +214,22 @@ __global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uin
                                                   uint32_t* __restrict__ leaves) {
   constexpr int BLOCKS = (PLEN + 512 + 63) / 64;
   static_assert(BLOCKS == 10 && (16 * 1 - (PLEN + (4 - PLEN % 4) % 4) / 4) < 0, "prefix ends inside block 1");
-  const uint64_t r = uint64_t(blockIdx.x) * WG + threadIdx.x;
-  if (r >= rows) return;
-  const uint4* col = reinterpret_cast<const uint4*>(cw) + r;  // column k of this row at col[k * rows]
-  uint32_t h[8];
+  for (uint64_t r = uint64_t(blockIdx.x) * WG + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * WG) {
+    const uint4* col = reinterpret_cast<const uint4*>(cw) + r;  // column k of this row at col[k * rows]
+    uint32_t h[8];
 #pragma unroll
-  for (int i = 0; i < 8; i++) h[i] = lp.h[i];
-  uint32_t W[20];
+    for (int i = 0; i < 8; i++) h[i] = lp.h[i];
+    uint32_t W[20];
 #pragma unroll
-  for (int k = 16; k < 20; k++) W[k] = 0;  // column -1: before the row
-  leaf_window_block<PLEN>(W, h, col, rows, lp, 1, true, false);
+    for (int k = 16; k < 20; k++) W[k] = 0;  // column -1: before the row
+    leaf_window_block<PLEN>(W, h, col, rows, lp, 1, true, false);
 #pragma unroll 1
-  for (int b = 2; b < BLOCKS - 1; b++) leaf_window_block<PLEN>(W, h, col, rows, lp, b, false, false);
-  leaf_window_block<PLEN>(W, h, col, rows, lp, BLOCKS - 1, false, true);
-  uint4* o = reinterpret_cast<uint4*>(leaves + 8 * r);
-  o[0] = make_uint4(h[0], h[1], h[2], h[3]);
-  o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    for (int b = 2; b < BLOCKS - 1; b++) leaf_window_block<PLEN>(W, h, col, rows, lp, b, false, false);
+    leaf_window_block<PLEN>(W, h, col, rows, lp, BLOCKS - 1, false, true);
+    uint4* o = reinterpret_cast<uint4*>(leaves + 8 * r);
+    o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+  }
 }
 
 // One Merkle level: parent i = BLAKE2s(prefix50 || child 2i || child 2i+1), 114 bytes.

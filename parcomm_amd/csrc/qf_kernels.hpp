@@ -79,6 +79,8 @@ std::string autotune_report();
 // autotuner allocates scratch buffers); when not, an untuned geometry takes
 // the built-in launch shape.
 void set_tuning_allowed(bool allowed);
+// Compute units of the current device.
+int device_cus();
 // Whether tile passes may run on the quad-split kernels (QF_QTILE=0 keeps the
 // register-radix ones); the commit schedule fuses WHT . alpha . WHT into one
 // pass when they do.

@@ -1055,6 +1055,8 @@ __global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   qtile_body<MODE, PACK, LO, true>(a);
 }
 
+}  // namespace
+
 int device_cus() {
   static int cus = 0;
   if (!cus) {
@@ -1065,8 +1067,6 @@ int device_cus() {
   }
   return cus;
 }
-
-}  // namespace
 
 namespace {
 // Launch shape of a tile pass: kernel variant (lean: at most 128 registers, no
