@@ -245,6 +245,14 @@ def roofline_of(pa, res, field_tag):
     if not mw_ms or not leaf_ms:
         return None
     mw_bytes = 3 * 32 * n * 16
+    # the kernel the autotuner kept for the M_W geometry (the last row group of the 2^L-row columns)
+    L = n.bit_length() - 1
+    mw_family = "k_tile<M_W>"
+    for e in pa.autotune_report(params.field):
+        if e["mode"] == 1 and e["L"] == L and e["vectors"] == 32:
+            b = e["best"]
+            mw_family = "k_btile<M_W> (8192-element tiles)" if b.get("big") else (
+                "k_qtile<M_W>" if b.get("quad") else "k_tile<M_W>")
     leaf_bytes = 32 * R * 16 + R * 32
     leaf_ops = R * LEAF_DEVICE_COMPRESSIONS * B2S_OPS
     leaf = {"kernel": "k_leaf_hash<75> (BLAKE2s row leaves)", "algorithmic_bytes": leaf_bytes,
@@ -254,7 +262,7 @@ def roofline_of(pa, res, field_tag):
             "compressions": R * 10, "device_compressions": R * LEAF_DEVICE_COMPRESSIONS,
             "valu_frac": round(leaf_ops / (leaf_ms * 1e-3) / VALU_PEAK_OPS, 4),
             "traffic": measured_traffic(field_tag, "leaf")}
-    mw = {"kernel": "k_tile<M_W> (Mobius + WHT, last row group)", "algorithmic_bytes": mw_bytes,
+    mw = {"kernel": mw_family + ", Mobius + WHT of the last row group", "algorithmic_bytes": mw_bytes,
           "avg_launch_ms": round(mw_ms, 4), "traffic": measured_traffic(field_tag, "mw")}
     if leaf_ms > mw_ms:
         kernel, nbytes, ms, traffic = leaf["kernel"], leaf_bytes, leaf_ms, leaf["traffic"]

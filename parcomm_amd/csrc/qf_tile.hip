@@ -1162,16 +1162,19 @@ bool qtile_on() {
   const char* e = std::getenv("QF_QTILE");
   return !(e && e[0] == '0');
 }
-// QF_BTILE=0 keeps 10-bit strided passes on 4096-element tiles.
+// QF_BTILE=0 keeps 10-bit strided passes on 4096-element tiles; QF_BTILE=1
+// makes the 8192-element tiles the built-in shape of every pass they serve.
 bool btile_on() {
   const char* e = std::getenv("QF_BTILE");
   return !(e && e[0] == '0');
 }
-// The 8192-element tiles for the 10-bit strided passes they serve, else the
-// quad-split kernels when they apply (their per-thread offsets fit 32 bits),
-// else the register-radix ones.
+// The 8192-element tiles for the M_W pass over 10 strided bits (2.16 against
+// 2.33 ms on the 8B block; the W pass over the same bits is as fast in either
+// form and keeps the 4096-element default unless the autotuner finds the big
+// tile faster), else the quad-split kernels when they apply (their per-thread
+// offsets fit 32 bits), else the register-radix ones.
 TileShape default_shape(int mode, bool quad_ok, bool big_ok) {
-  if (big_ok && btile_on()) return {false, 1, false, false, true};
+  if (big_ok && btile_on() && (mode == TM_M_W || std::getenv("QF_BTILE"))) return {false, 1, false, false, true};
   if (quad_ok && qtile_on()) return {false, 2, true, true};  // quad_ok: a compile-time bit range
   const bool w = wide_mode(mode);
   return {w, w ? 1 : 2, true, false};
