@@ -6,6 +6,7 @@
 #include "qf_rounds.hpp"
 #include "qf_vec.hpp"
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <map>
 #include <tuple>
@@ -57,6 +58,16 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+// QF_TRACE_SYNC=1: host waits and their time per prover phase, and the marked
+// host intervals inside it, on stderr.
+void trace_phase(const char* name, double ms) {
+  if (!sync_trace_on()) return;
+  SyncTrace& t = sync_trace();
+  std::fprintf(stderr, "[qf] %-10s %8.3f ms, %4llu host waits, %8.3f ms waiting\n", name, ms,
+               (unsigned long long)t.syncs, t.wait_ms);
+  t = SyncTrace{};
+  trace_scopes_flush();
+}
 
 void tw_device(DevCtx& c, E8* out, const std::vector<E8>& p) {
   size_t m = p.size(), a = m / 2;
@@ -254,6 +265,7 @@ LB litebullet(DevCtx& c, E8 claimed, DVec& left, DVec& right, Pcs& tr, const cha
   LB lb;
   E8 rhs = claimed;
   E8 lc{}, rc{};
+  TraceLaps lap;
   TwoFold tf(c, left, right);
   if (n > 1) std::tie(lc, rc) = tf.cross(n);
   while (n > 1) {  // each fold also returns the next round's cross terms
@@ -285,6 +297,7 @@ LB litebullet(DevCtx& c, E8 claimed, DVec& left, DVec& right, Pcs& tr, const cha
   tr.respond(lb.a_value);
   tr.respond(lb.b_value);
   require(lb.a_value * lb.b_value == rhs, QF_ERR_INVALID_COMMITMENT, "litebullet terminal");
+  lap.mark(".litebullet (inside the above)");
   return lb;
 }
 
@@ -351,6 +364,7 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
   const QaCode& code = p.pp.codes[level];
   uint64_t n = code.message_len, R = cache.R;
   require(R == code.codeword_len() && point.size() == size_t(5 + ilog2u(n)), QF_ERR_INVALID_VARIABLES, "round dims");
+  TraceLaps lap;
   DVec own_w;
   if (!top_row_w) {
     std::vector<E8> rp(point.begin() + 5, point.end());
@@ -363,6 +377,7 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
   for (auto& o : opening) tr.respond(o);
   std::vector<E8> fr;
   for (int k = 0; k < 32; k++) fr.push_back(tr.challenge("freivalds"));
+  lap.mark("native.projection");
   E8 fv = inner_host(opening, fr);
   DVec folded(n);
   cache.fold(c, n, fr, folded.p);
@@ -375,6 +390,7 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
     tr.bytes("next-layer-kind", "terminal-vector");
     tr.digest("next-layer-binding", hash_terminal(terminal));
   }
+  lap.mark("native.fold+commit");
   uint64_t count = num_queries(code, p.pp.security_bits, R);
   Digest seed = tr.challenge_digest("row-selection");
   bool all = count == R;
@@ -389,7 +405,10 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
     sampled = dv::gather_weighted(c, cache.data(), R, cache.cols, nullptr, 0, 0, fr, sel);
   }
   tr.exts("sampled-values", sampled);
-  return round_reductions(c, p, level, point, fv, folded, row_w, sel, sampled, tr);
+  lap.mark("native.rows");
+  RoundOut ro = round_reductions(c, p, level, point, fv, folded, row_w, sel, sampled, tr);
+  lap.mark("native.reductions");
+  return ro;
 }
 
 // qafold_short_pc.rs:578-725
@@ -469,6 +488,7 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
   DCode carry;
   DVec carry_msg;
   bool have_carry = false;
+  TraceLaps lap;
   for (size_t level = levels - 1; level >= 1; level--) {
     const SetupLevel& sl = p.pp.setup[level];
     const QaCode& code = codes[level];
@@ -512,6 +532,7 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
       tr.exts("multi-eval-column-weight", cl.col_weight);
     }
     E8 rho = tr.challenge("multi-eval-batch");
+    lap.mark("decide.transcript");
     DVec left(padded * rows), right(padded * rows);
     dv::fill(c, left.p, padded * rows, E8{});
     dv::fill(c, right.p, padded * rows, E8{});
@@ -526,6 +547,7 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
         if (!e8_is_zero(cl.col_weight[k])) dv::axpy(c, right.p + k * rows, rw.p, rows, pw * cl.col_weight[k]);
       pw = pw * rho;
     }
+    lap.mark("decide.setup vectors");
     std::vector<E8> red_point;
     DVec folded_vector(rows);
     uint64_t n = padded * rows;
@@ -561,6 +583,7 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     tr.respond(red_value);
     tr.respond(bval);
     require(red_value * bval == rhs, QF_ERR_INVALID_COMMITMENT, "multi-eval terminal");
+    lap.mark("decide.multieval");
     std::vector<E8> col_point(red_point.begin(), red_point.begin() + log_cols);
     std::vector<E8> merged(red_point.begin() + log_cols, red_point.end());
     auto col_w = tw_host(col_point);
@@ -598,6 +621,7 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     std::vector<E8> iota = dv::gather_weighted(c, d_setup, n_enc, scols, have_carry ? carry.data() : nullptr,
                                                have_carry ? carry.R : 0, have_carry ? phys_carry : 0, iw, srows);
     for (auto& v : iota) tr.respond(v);
+    lap.mark("decide.iota");
     Bytes round;
     put_u64(round, level);
     uint64_t trows = target.message_len;
@@ -632,6 +656,7 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
       }
     }
     if (have_carry) { round.push_back(1); write_rows(c, round, carry, sel.all, srows); } else round.push_back(0);
+    lap.mark("decide.carry+rows");
     // direct code switch (qafold_short_decide_carry.rs:940-1027)
     std::vector<E8> base_point(merged.begin() + 5, merged.end());
     DVec tbp = tw_vec(c, base_point);
@@ -685,6 +710,7 @@ Bytes prove_decide(DevCtx& c, const ParamsState& p, const std::vector<Pending>& 
     }
     rounds.insert(rounds.end(), round.begin(), round.end());
     n_rounds++;
+    lap.mark("decide.switch");
   }
   Bytes w;
   put_tag(w, "QFSDCID1");
@@ -749,6 +775,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   o.pos.assign(s - 1, E8{});
   o.neg.assign(s - 1, E8{});
   int ls = ilog2u(s);
+  TraceLaps lap;
   // the lane weights of every term of at least s entries, used by the projection and the query fold
   std::vector<std::vector<E8>> term_low(q.terms.size());
   for (size_t i = 0; i < q.terms.size(); i++)
@@ -766,11 +793,13 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     fbit.push_back(t.f[ls]);
     big.push_back(i);
   }
+  lap.mark("eo.terms");
   EoProjection proj = eo_project(c, bva, l.active, int(s), tasks, pairs);
   EoCorr corr = eo_correlations(c, proj, fbit, lows);
   o.pos = corr.pos;
   o.neg = corr.neg;
   E8 actual = corr.zero;  // sum_i v[i] q[i]
+  lap.mark("eo.project+corr");
   // terms below 2s entries: one row k of stride s each
   struct SmallTerm { uint64_t k; std::vector<E8> query; };
   std::vector<SmallTerm> smalls;
@@ -799,6 +828,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   pinv[0] = pows[0] = e8_one();
   for (uint64_t i = 1; i < s; i++) { pinv[i] = pinv[i - 1] * inv; pows[i] = pows[i - 1] * ch; }
   require(actual == q.claim, QF_ERR_INVALID_COMMITMENT, "packed reduction claim mismatch");
+  lap.mark("eo.small+transcript");
   uint64_t rows = l.logical / s;
   // the query fold of each term of at least s entries: scale x sum_l low[l] c^l
   std::vector<E8> fold_scale(q.terms.size(), E8{});
@@ -847,6 +877,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     for (auto& [k, v] : single) { idx.push_back(k); vals.push_back(v); }
     dv::scatter_add(c, out, idx, vals);
   };
+  lap.mark("eo.fold_scale");
   E8 claim = q.claim, fwd = ch, bwd = inv;
   for (uint64_t d = 0; d + 1 < s; d++) { claim += o.pos[d] * fwd + o.neg[d] * bwd; fwd = fwd * ch; bwd = bwd * inv; }
   // the first binary round's cross terms sum_j w[2j] qf[2j+1], sum_j w[2j+1] qf[2j]
@@ -869,6 +900,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
       else c1 += wv * qf;
     }
   }
+  lap.mark("eo.first_cross");
   // Long vectors: the first round reads the BVA and the folded query (no
   // witness or query of stride s is built); short ones materialize both and
   // take the rounds below.
@@ -903,6 +935,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
       nq = &qv;
       first = false;
       n = h;
+      lap.mark("eo.query+first_fold");
       continue;
     }
     first = false;
@@ -933,6 +966,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     }
     n = h;
   }
+  lap.mark("eo.rounds");
   o.value = hw.empty() ? dv::download(c, cw->p, 1)[0] : hw[0];
   o.query = hq.empty() ? dv::download(c, cq->p, 1)[0] : hq[0];
   require(claim == o.value * o.query && query_eval(q, challenges) == o.query, QF_ERR_INVALID_COMMITMENT, "EO terminal mismatch");
@@ -1113,6 +1147,7 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   DVec weights(plen), values(plen);
   weight_images(c, weights.p, ehi, elo, images);
   tm.projection_ms = ms_since(t0);
+  trace_phase("projection", tm.projection_ms);
   t0 = Clock::now();
   block_fold(c, cs.d_bva, p.layout.active, p.block_len, blocks, fw, values.p, plen);
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
@@ -1122,6 +1157,7 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   folded.root = download_root(c, folded.tree, R);
   t.bytes(folded.root.data(), 32);
   tm.folding_ms = ms_since(t0);
+  trace_phase("folding", tm.folding_ms);
   t0 = Clock::now();
   std::vector<std::array<E8, 2>> rounds;
   std::vector<E8> reduced;
@@ -1130,6 +1166,7 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   require(fwt == batch_weight(outer.data(), n_outer, reduced, coefs) && claim == fval * fwt, QF_ERR_INVALID_COMMITMENT,
           "batch sumcheck terminal");
   tm.sumcheck_ms = ms_since(t0);
+  trace_phase("sumcheck", tm.sumcheck_ms);
   t0 = Clock::now();
   const char* bsd = "authenticated-codeword-fold-queries/v1";
   t.bytes(bsd, std::strlen(bsd));
@@ -1172,15 +1209,18 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   write_rows(c, w, folded, all, sel);
   patch_u64(w, bind_len_at, w.size() - bind_at);
   tm.binding_ms = ms_since(t0);
+  trace_phase("binding", tm.binding_ms);
   t0 = Clock::now();
   Bytes opening;
   std::vector<Pending> pending;
   E8 actual = open_with_reduction(c, p, folded, reduced, opening, pending);
   require(actual == fval, QF_ERR_INVALID_COMMITMENT, "native opening value");
   tm.native_ms = ms_since(t0);
+  trace_phase("native", tm.native_ms);
   t0 = Clock::now();
   Bytes decide = prove_decide(c, p, pending);
   tm.decide_ms = ms_since(t0);
+  trace_phase("decide", tm.decide_ms);
   put_part(w, opening);
   put_part(w, decide);
 }
@@ -1215,6 +1255,7 @@ Bytes open_single(DevCtx& c, const ParamsState& p, const CommittedState& cs, con
   weight_images(c, weights.p, ehi, elo, images);
   block_fold(c, cs.d_bva, p.layout.active, p.block_len, 1, {e8_one()}, values.p, plen);
   tm.projection_ms = ms_since(t0);
+  trace_phase("projection", tm.projection_ms);
   t0 = Clock::now();
   std::vector<std::array<E8, 2>> rounds;
   std::vector<E8> reduced;
@@ -1223,6 +1264,7 @@ Bytes open_single(DevCtx& c, const ParamsState& p, const CommittedState& cs, con
   require(fwt == batch_weight(point.data(), n_outer, reduced, powers) && claim == fval * fwt, QF_ERR_INVALID_COMMITMENT,
           "sumcheck terminal");
   tm.sumcheck_ms = ms_since(t0);
+  trace_phase("sumcheck", tm.sumcheck_ms);
   t0 = Clock::now();
   DCode workspace;
   DCode bc = block_code(c, p, cs, 0, workspace);
@@ -1231,9 +1273,11 @@ Bytes open_single(DevCtx& c, const ParamsState& p, const CommittedState& cs, con
   E8 actual = open_with_reduction(c, p, bc, reduced, opening, pending);
   require(actual == fval, QF_ERR_INVALID_COMMITMENT, "native opening value");
   tm.native_ms = ms_since(t0);
+  trace_phase("native", tm.native_ms);
   t0 = Clock::now();
   Bytes decide = prove_decide(c, p, pending);
   tm.decide_ms = ms_since(t0);
+  trace_phase("decide", tm.decide_ms);
   Bytes w;
   put_tag(w, PACKED_MAGIC);
   put_e8(w, claimed);
@@ -1355,9 +1399,11 @@ Bytes prove(const ParamsState& p, const CommittedState& cs, const std::vector<Cl
   tr.object("claims", claims_json(claims));
   Query q = compile(p.layout, claims, tr);
   tm.conversion_ms = ms_since(t0);
+  trace_phase("conversion", tm.conversion_ms);
   t0 = Clock::now();
   EoOut eo = eo_prove(c, p.layout, cs.d_bva, s, q, tr);
   tm.eo_ms = ms_since(t0);
+  trace_phase("eo", tm.eo_ms);
   Bytes context;
   const char* cx = TERMINAL_DOMAIN;
   context.insert(context.end(), cx, cx + std::strlen(cx));

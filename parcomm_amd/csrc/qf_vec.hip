@@ -1,11 +1,14 @@
 // Device vector operations and BVA-scale kernels of the opening (see qf_vec.hpp).
 #include "qf_vec.hpp"
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <type_traits>
 #include <map>
+#include <string>
+#include <cstdio>
 #include <mutex>
 #include <unordered_map>
 
@@ -1036,6 +1039,7 @@ __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bv
   __syncthreads();
   // one witness row of S2 entries, all below `available`: each dot2 adds less
   // than 2^25, so the NP <= 64 of a row and the bias correction stay below 2^32
+  // (the BVA is 16-byte aligned: rows of 16 entries or more load as 16-byte vectors)
   auto witness = [&](uint64_t row, V8& out) {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(bva + row * S2);
     uint32_t acc[8];
@@ -1044,9 +1048,14 @@ __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bv
 #pragma unroll
     for (int g0 = 0; g0 < G; g0 += 4) {
       uint32_t raw[4];
+      if constexpr (G % 4 == 0) {
+        const uint4 v = reinterpret_cast<const uint4*>(src)[g0 / 4];
+        raw[0] = v.x; raw[1] = v.y; raw[2] = v.z; raw[3] = v.w;
+      } else {
 #pragma unroll
-      for (int g = 0; g < 4; g++)
-        if (g0 + g < G) raw[g] = src[g0 + g];
+        for (int g = 0; g < 4; g++)
+          if (g0 + g < G) raw[g] = src[g0 + g];
+      }
 #pragma unroll
       for (int g = 0; g < 4; g++) {
         if (g0 + g >= G) continue;
@@ -1662,7 +1671,46 @@ DevCtx::~DevCtx() {
   if (st) (void)hipStreamSynchronize(st);
   g_current_stream = prev;
 }
-void DevCtx::sync() { QF_HIP(hipStreamSynchronize(st)); }
+bool sync_trace_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("QF_TRACE_SYNC");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+SyncTrace& sync_trace() {
+  thread_local SyncTrace t;
+  return t;
+}
+namespace {
+thread_local std::map<std::string, std::pair<double, uint64_t>> t_scopes;
+}
+TraceLaps::TraceLaps() {
+  if (sync_trace_on()) t = std::chrono::steady_clock::now();
+}
+void TraceLaps::mark(const char* name) {
+  if (!sync_trace_on()) return;
+  const auto now = std::chrono::steady_clock::now();
+  auto& e = t_scopes[name];
+  e.first += std::chrono::duration<double, std::milli>(now - t).count();
+  e.second++;
+  t = now;
+}
+void trace_scopes_flush() {
+  for (auto& [n, v] : t_scopes) std::fprintf(stderr, "[qf]     %-28s %8.3f ms  x%llu\n", n.c_str(), v.first, (unsigned long long)v.second);
+  t_scopes.clear();
+}
+void DevCtx::sync() {
+  if (!sync_trace_on()) {
+    QF_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  QF_HIP(hipStreamSynchronize(st));
+  SyncTrace& t = sync_trace();
+  t.syncs++;
+  t.wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
 
 namespace dv {
 DVec upload(DevCtx& c, const std::vector<E8>& v) {

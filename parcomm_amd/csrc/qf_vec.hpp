@@ -3,6 +3,7 @@
 // projections). Reductions return their result to the host: each workgroup
 // writes a partial sum and the host adds at most a few thousand partials.
 #pragma once
+#include <chrono>
 #include "qf_kernels.hpp"
 #include <memory>
 #include <utility>
@@ -24,6 +25,23 @@ struct DVec {
   ~DVec();
   void resize_view(uint64_t len) { n = len; }  // shrink the logical length (storage kept)
 };
+
+// Host waits of this thread (QF_TRACE_SYNC=1: counted and timed in DevCtx::sync).
+struct SyncTrace {
+  uint64_t syncs = 0;
+  double wait_ms = 0;
+};
+bool sync_trace_on();
+SyncTrace& sync_trace();
+// Wall time of named host intervals, accumulated per thread while
+// QF_TRACE_SYNC=1: mark(name) books the time since the previous mark (or the
+// construction) under name.
+struct TraceLaps {
+  std::chrono::steady_clock::time_point t;
+  TraceLaps();
+  void mark(const char* name);
+};
+void trace_scopes_flush();  // prints and clears this thread's interval totals
 
 // Stream and scratch for reduction partials, owned by the prover of one
 // proof; while it lives it is the thread's current stream. Its destructor

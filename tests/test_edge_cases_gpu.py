@@ -99,6 +99,7 @@ def test_rows_above_the_characteristic(oracle, oracle_kb, field):
 @pytest.mark.parametrize("shapes,chunk_log,packed_log", [
     ([(1024, 4096), (3, 5)], 22, 4),   # 16-lane terms of 2^10..2^18 rows: MFMA EO projection, ragged end
     ([(2048, 1024), (7, 3)], 20, 3),   # 8-lane terms of 2^13 rows and more: the packed_log 3 MFMA form
+    ([(1024, 4096), (5, 9)], 22, 5),   # EO projection at stride 64: F65521's four-record MFMA rows
 ])
 def test_full_range_entries_on_the_mfma_projections(oracle, oracle_kb, field, shapes, chunk_log, packed_log):
     pa, p = setup(field, shapes, chunk_log, packed_log)
