@@ -8,6 +8,9 @@
 #include <chrono>
 #include <cstdio>
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <memory>
 #include <map>
 #include <tuple>
 
@@ -757,6 +760,14 @@ std::vector<std::vector<E8>> project_terms(DevCtx& c, const Query& q, const Val*
   return out;
 }
 
+// Output pairs from which an EO round projects on the query's terms instead of
+// reading a materialized query (QF_EO_OTF_MIN; default 2^21: below that the
+// vectors are cache-resident and the extra launches cost more than the query).
+uint64_t eo_otf_min_pairs() {
+  const char* e = std::getenv("QF_EO_OTF_MIN");
+  return e ? uint64_t(std::strtoull(e, nullptr, 10)) : (uint64_t(1) << 21);
+}
+
 // ================================================================ EO (eo.rs:97-181)
 // The packed reduction and its binary rounds. Every term of at least 2s
 // entries is projected once at stride 2s (its offset is a multiple of its
@@ -768,7 +779,8 @@ std::vector<std::vector<E8>> project_terms(DevCtx& c, const Query& q, const Val*
 // (eo.rs:134-138) equals, term for term, sum_i v[i] q[i] == query.claim, since
 // both sides add the same cross-term multiples of the challenge; the check is
 // made on that sum. The witness of stride s is never written: the first round
-// reads the BVA at stride 2s (see eo_first_fold).
+// reads the BVA at stride 2s (OtfQuery::first_fold), and the long rounds
+// project their outputs on the query's terms rather than reading a query.
 struct EoOut { std::vector<E8> pos, neg; std::vector<std::array<E8, 2>> rounds; E8 value, query; std::vector<E8> point; E8 scale; };
 EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Query& q, Outer& tr) {
   EoOut o;
@@ -838,35 +850,47 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
       for (uint64_t j = 0; j < s; j++) acc += term_low[i][j] * pows[j];
       fold_scale[i] = acc;
     }
-  // The query fold (tbv.rs:393-428) into out, at stride s (level 0: qf, rows
-  // entries) or folded once more by the first binary challenge r (level 1:
-  // q'[i] = qf[2i] + qf[2i+1] r, rows / 2 entries; a term's row-bit-0 factor
-  // joins its scale). Bucket terms are written (the rows outside them zeroed),
-  // then padding and suffix terms added; each phase has disjoint supports.
-  // Terms that do not span a whole row pair at the level go in as single values.
-  auto build_query = [&](E8* out, int level, const E8& r) {
-    const uint64_t span = s << level;  // entries per output row
+  // a[i]: term i folded over the entry bits below an element of the current
+  // level (level L: the s lanes by c, then L binary rounds by their challenges;
+  // q_L[k] = q_{L-1}[2k] + q_{L-1}[2k+1] r_L, tbv.rs:393-428). A term of at
+  // least one element is a[i] times the tensor of its factors above those bits;
+  // a shorter one is the single value a[i] at its element.
+  std::vector<E8> a(q.terms.size());
+  for (size_t i = 0; i < q.terms.size(); i++) {
+    const Term& t = q.terms[i];
+    if (t.len() >= s) a[i] = t.scale * fold_scale[i];
+    else
+      for (uint64_t j = 0; j < t.len(); j++) a[i] += t.weight(j) * pows[(t.offset + j) % s];
+  }
+  // a from level L to L + 1 by the binary challenge r: the factor of entry bit
+  // ls + L folds to f[0] + r f[1]; a shorter term picks up r when that bit of its offset is set
+  auto advance = [&](int level, const E8& r) {
+    const size_t b = size_t(ls + level);
+    for (size_t i = 0; i < q.terms.size(); i++) {
+      const Term& t = q.terms[i];
+      if (b < t.f.size()) a[i] = a[i] * (t.f[b][0] + r * t.f[b][1]);
+      else if ((t.offset >> b) & 1) a[i] = a[i] * r;
+    }
+  };
+  // The query of level L into out (rows >> L entries): bucket terms are
+  // written (the entries outside them zeroed), then padding and suffix terms
+  // added; each phase has disjoint supports. Terms shorter than an element go
+  // in as single values.
+  auto build_query = [&](E8* out, int level) {
+    const uint64_t span = s << level;  // entries per element
     std::map<uint64_t, E8> single;
     for (int phase = 0; phase < 2; phase++) {
       std::vector<FillTask> ftasks;
       std::vector<std::array<E8, 2>> fpairs;
-      size_t a = phase ? q.n_bucket_terms : 0, b = phase ? q.terms.size() : q.n_bucket_terms;
-      for (size_t i = a; i < b; i++) {
+      size_t lo = phase ? q.n_bucket_terms : 0, hi = phase ? q.terms.size() : q.n_bucket_terms;
+      for (size_t i = lo; i < hi; i++) {
         const Term& t = q.terms[i];
         if (t.len() >= span) {
-          E8 sc = t.scale * fold_scale[i];
-          if (level) sc = sc * (t.f[ls][0] + r * t.f[ls][1]);
-          FillTask ft{t.offset / span, uint32_t(t.f.size() - ls - level), uint32_t(fpairs.size()), sc};
+          FillTask ft{t.offset / span, uint32_t(t.f.size() - ls - level), uint32_t(fpairs.size()), a[i]};
           for (size_t bb = ls + level; bb < t.f.size(); bb++) fpairs.push_back(t.f[bb]);
           ftasks.push_back(ft);
         } else {
-          E8 v{};  // the term's part of qf at its row k = offset / s
-          if (t.len() >= s) v = t.scale * fold_scale[i];
-          else
-            for (uint64_t j = 0; j < t.len(); j++) v += t.weight(j) * pows[(t.offset + j) % s];
-          const uint64_t k = t.offset / s;
-          if (level && (k & 1)) v = v * r;
-          single[k >> level] += v;
+          single[t.offset / span] += a[i];
         }
       }
       if (phase == 0) tensor_fill(c, out, rows >> level, ftasks, fpairs);
@@ -901,21 +925,77 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     }
   }
   lap.mark("eo.first_cross");
-  // Long vectors: the first round reads the BVA and the folded query (no
-  // witness or query of stride s is built); short ones materialize both and
-  // take the rounds below.
-  const bool from_bva = rows > TwoFold::HOST_TAIL;
-  DVec w(from_bva ? rows / 4 : rows), qv(from_bva ? rows / 4 : rows), w2(rows / 2 ? rows / 2 : 1),
-      q2(rows / 2 ? rows / 2 : 1);
-  if (!from_bva) {
-    build_query(qv.p, 0, E8{});
+  // Long vectors: the first round reads the BVA (no witness of stride s is
+  // written) and, while a round's outputs have at least eo_otf_min_pairs()
+  // pairs, it projects them on the query's terms instead of reading a query
+  // (OtfQuery); the query is materialized at the first level below that.
+  // Short vectors materialize the witness and the query at level 0.
+  const bool from_bva = rows / 4 >= (uint64_t(1) << OTF_LCP);
+  DVec w(from_bva ? rows / 4 : rows), w2(rows / 2 ? rows / 2 : 1), qa, qb;
+  DVec *cw = &w, *nw = &w2, *cq = nullptr, *nq = nullptr;
+  std::unique_ptr<OtfQuery> otf;
+  // every term's factors in one table (fofs: the first of term i), and the
+  // lowest bit at which a factor pair has a zero entry
+  std::vector<uint32_t> fofs(q.terms.size());
+  std::vector<size_t> zmin(q.terms.size(), SIZE_MAX);
+  if (from_bva) {
+    std::vector<std::array<E8, 2>> factors;
+    for (size_t i = 0; i < q.terms.size(); i++) {
+      fofs[i] = uint32_t(factors.size());
+      for (size_t b = 0; b < q.terms[i].f.size(); b++) {
+        const auto& fp = q.terms[i].f[b];
+        factors.push_back(fp);
+        if (zmin[i] == SIZE_MAX && (e8_is_zero(fp[0]) || e8_is_zero(fp[1]))) zmin[i] = b;
+      }
+    }
+    otf = std::make_unique<OtfQuery>(c, factors, (rows / 4) >> OTF_LCP, q.n_bucket_terms, 0, 0);
+  } else {
+    qa = DVec(rows);
+    qb = DVec(rows / 2 ? rows / 2 : 1);
+    build_query(qa.p, 0);
     eo_witness(c, bva, l.active, rows, int(s), pinv, w.p);
+    cq = &qa;
+    nq = &qb;
   }
-  DVec *cw = &w, *cq = &qv, *nw = &w2, *nq = &q2;
+  // The records of output level L: bucket terms of at least one chunk of pairs
+  // go whole, the other terms of at least one pair in pieces, and terms of at
+  // most one element as items. A term other than a bucket term covers padding
+  // only, whose entries are zero (validated at commit): while every factor
+  // with a zero entry lies above the pair bit, each pair in its range is two
+  // zero witness elements and it adds nothing; so does a term past the active entries.
+  auto otf_level = [&](int level) {
+    OtfLevel lv;
+    const size_t pb = size_t(ls + level);  // the entry bit of output bit 0
+    const uint64_t span = uint64_t(s) << level;
+    for (size_t i = 0; i < q.terms.size(); i++) {
+      const Term& t = q.terms[i];
+      if (e8_is_zero(a[i])) continue;
+      if (t.f.size() <= pb) {
+        lv.items.push_back({t.offset / span, 0, a[i]});
+        continue;
+      }
+      if (t.offset >= l.active) continue;
+      const bool bucket = i < q.n_bucket_terms;
+      if (!bucket && zmin[i] > pb) continue;
+      const uint32_t lp = uint32_t(t.f.size() - pb - 1);  // log2 of its pairs
+      const uint64_t pair0 = t.offset / (2 * span), np = uint64_t(1) << lp;
+      const E8 g0 = a[i] * t.f[pb][1], g1 = a[i] * t.f[pb][0];
+      const uint32_t fb = fofs[i] + uint32_t(pb) + 1;
+      if (bucket && lp >= uint32_t(OTF_LCP)) {
+        lv.fast.push_back({pair0, lp, fb, g0, g1});
+      } else {
+        for (uint64_t k = 0; k < np; k += OTF_PIECE)
+          lv.pieces.push_back({pair0 + k, k, uint32_t(std::min<uint64_t>(OTF_PIECE, np - k)), fb, lp, 0, g0, g1});
+      }
+    }
+    std::sort(lv.fast.begin(), lv.fast.end(), [](const OtfTermDev& x, const OtfTermDev& y) { return x.pair0 < y.pair0; });
+    return lv;
+  };
+  const uint64_t otf_min = std::max<uint64_t>(eo_otf_min_pairs(), uint64_t(1) << OTF_LCP);
   uint64_t n = rows;
+  int level = 0;
   // the last rounds run on host copies once the vectors are short (as TwoFold)
   std::vector<E8> hw, hq;
-  bool first = true;
   while (n > 1) {
     uint64_t h = n / 2;
     tr.object("binary-cross", js_scalars({c0, c1}));
@@ -923,22 +1003,33 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     claim += c0 * r + c1 * ri;
     challenges.push_back(r);
     o.rounds.push_back({c0, c1});
-    if (first && from_bva) {  // w'[i] = sum_{L<2s} v[2is + L] pinv2[L], pinv2 = (c^-l, r^-1 c^-l)
-      std::vector<E8> pinv2(2 * s);
-      for (uint64_t lane = 0; lane < s; lane++) { pinv2[lane] = pinv[lane]; pinv2[s + lane] = ri * pinv[lane]; }
-      build_query(q2.p, 1, r);
-      std::tie(c0, c1) = eo_first_fold(c, bva, l.active, rows, int(s), pinv2, q2.p, w2.p);
-      // outputs in (w2, q2); the next round writes into w and qv (rows / 4 each)
-      cw = &w2;
-      cq = &q2;
-      nw = &w;
-      nq = &qv;
-      first = false;
+    if (!cq) {  // w' and the cross terms over it from the query's terms
+      advance(level, r);
+      const OtfLevel lv = otf_level(level + 1);
+      lap.mark("eo.otf_records");
+      if (level == 0) {  // w'[i] = sum_{L<2s} v[2is + L] pinv2[L], pinv2 = (c^-l, r^-1 c^-l)
+        std::vector<E8> pinv2(2 * s);
+        for (uint64_t lane = 0; lane < s; lane++) { pinv2[lane] = pinv[lane]; pinv2[s + lane] = ri * pinv[lane]; }
+        std::tie(c0, c1) = otf->first_fold(c, bva, l.active, rows, int(s), pinv2, lv, w2.p);
+        cw = &w2;  // the next round writes into w (rows / 4)
+        nw = &w;
+      } else {
+        std::tie(c0, c1) = otf->fold(c, cw->p, nw->p, h, ri, lv);
+        std::swap(cw, nw);
+      }
+      level++;
       n = h;
-      lap.mark("eo.query+first_fold");
+      if (n / 4 < otf_min) {  // the next round's outputs are short: materialize the query of this level
+        qa = DVec(n);
+        qb = DVec(n / 2);
+        build_query(qa.p, level);
+        cq = &qa;
+        nq = &qb;
+      }
+      lap.mark(level == 1 ? "eo.first_fold" : "eo.otf_rounds");
       continue;
     }
-    first = false;
+    level++;
     if (hw.empty() && n <= TwoFold::HOST_TAIL) {
       hw = dv::download(c, cw->p, n);
       hq = dv::download(c, cq->p, n);

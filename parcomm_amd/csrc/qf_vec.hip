@@ -998,25 +998,161 @@ __device__ inline V8 lazy_value(const LazySum& s) {  // sl, sh < 2^32
 
 #endif
 
+// ---------------------------------------------------------------- EO rounds with the query on the fly
+// Chunk ch of a level covers the output pairs [ch 2^OTF_LCP, (ch + 1) 2^OTF_LCP);
+// pair i WG + t of a chunk belongs to thread t, and workgroup b takes chunks
+// b, b + grid, ... (all workgroups stream through one window of the vector). A chunk inside a chunked term
+// (OtfTermDev) carries the term and g0, g1 times the term's factors of the
+// chunk-index bits; the term's tables hold the factors of the thread bits
+// (tlo, WG entries) and of the i bits (thi, OTF_PPT entries), so the query
+// value of pair m is tlo[t] thi[i] times the chunk's g.
+struct OtfChunk {
+  int32_t term;  // -1: no chunked term covers the chunk
+  uint32_t pad[3];
+  E8 g0, g1;
+};
+static_assert(sizeof(OtfChunk) == 48 && sizeof(OtfTermDev) == 48 && sizeof(OtfPieceDev) == 64 &&
+                  sizeof(OtfItemDev) == 32, "device records are whole extension elements");
+static_assert(WG == 256 && OTF_PPT == 8 && OTF_LCP == 11, "pair index bits: 8 thread bits, 3 i bits");
+
+__device__ inline V8 v8_one() {
+  V8 r = v8_zero();
+  r.c[0] = 1;
+  return r;
+}
+// prod_b f[base + b][(x >> b) & 1], b < nbits
+__device__ inline V8 otf_tensor(const E8 (*__restrict__ f)[2], uint32_t base, uint32_t nbits, uint64_t x) {
+  V8 g = v8_one();
+  for (uint32_t b = 0; b < nbits; b++) g = v8_mul(g, v8_load(f[base + b][(x >> b) & 1]));
+  return g;
+}
+
+// The level's chunk records (threads below `chunks`, each locating its chunk's
+// term by bisection over the sorted terms) and the chunked terms' tables.
+__global__ __launch_bounds__(WG) void k_otf_desc(const OtfTermDev* __restrict__ terms, uint32_t nterms,
+                                                 const E8 (*__restrict__ f)[2], uint64_t chunks,
+                                                 OtfChunk* __restrict__ desc, E8* __restrict__ tlo,
+                                                 E8* __restrict__ thi) {
+  const uint64_t gid = uint64_t(blockIdx.x) * WG + threadIdx.x;
+  if (gid < chunks) {
+    const uint64_t p = gid << OTF_LCP;
+    uint32_t lo = 0, hi = nterms;  // terms [0, lo) start at or before p
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (terms[mid].pair0 <= p) lo = mid + 1;
+      else hi = mid;
+    }
+    OtfChunk d{};
+    d.term = -1;
+    if (lo > 0) {
+      const OtfTermDev& T = terms[lo - 1];
+      if (p - T.pair0 < (uint64_t(1) << T.lpairs)) {
+        const V8 g = otf_tensor(f, T.fbase + OTF_LCP, T.lpairs - OTF_LCP, (p - T.pair0) >> OTF_LCP);
+        d.term = int32_t(lo - 1);
+        d.g0 = v8_store(v8_mul(g, v8_load(T.g0)));
+        d.g1 = v8_store(v8_mul(g, v8_load(T.g1)));
+      }
+    }
+    desc[gid] = d;
+    return;
+  }
+  uint64_t e = gid - chunks;
+  if (e < uint64_t(nterms) * WG) {
+    tlo[e] = v8_store(otf_tensor(f, terms[e / WG].fbase, 8, e % WG));
+    return;
+  }
+  e -= uint64_t(nterms) * WG;
+  if (e < uint64_t(nterms) * OTF_PPT) thi[e] = v8_store(otf_tensor(f, terms[e / OTF_PPT].fbase + 8, 3, e % OTF_PPT));
+}
+
+// Per-chunk sums sum_i thi[i] y_i over one thread's OTF_PPT pairs: lazy dot2
+// products for F65521 (8 < 31 products between folds), plain ones for KoalaBear.
+#ifndef QF_KB
+struct OtfAcc {
+  LazySum l;
+};
+__device__ inline void otf_zero(OtfAcc& a) { lazy_zero(a.l); }
+__device__ inline void otf_mac2(OtfAcc& a0, OtfAcc& a1, const V8& y0, const V8& y1, const V8& th) {
+  uint32_t D[15], lo[4], hi[4];
+  fold_seq(th, D);
+  limb_split(y0, lo, hi);
+  lazy_mac(a0.l, lo, hi, D);
+  limb_split(y1, lo, hi);
+  lazy_mac(a1.l, lo, hi, D);
+}
+__device__ inline V8 otf_value(const OtfAcc& a) { return lazy_value(a.l); }
+#else
+struct OtfAcc {
+  V8 v;
+};
+__device__ inline void otf_zero(OtfAcc& a) { a.v = v8_zero(); }
+__device__ inline void otf_mac2(OtfAcc& a0, OtfAcc& a1, const V8& y0, const V8& y1, const V8& th) {
+  const V8 tm = v8_to_mont(th);
+  a0.v = v8_add(a0.v, v8_mul_mont(y0, tm));
+  a1.v = v8_add(a1.v, v8_mul_mont(y1, tm));
+}
+__device__ inline V8 otf_value(const OtfAcc& a) { return a.v; }
+#endif
+
+// A chunk's part of the cross terms: s0 += a0 tlo[t] g0, s1 += a1 tlo[t] g1.
+__device__ inline void otf_chunk_sum(const OtfChunk& d, const E8* __restrict__ tlo, int32_t& cur, V8& tl,
+                                     const OtfAcc& a0, const OtfAcc& a1, V8 (&s)[2]) {
+  if (d.term != cur) {
+    tl = v8_load(tlo[uint64_t(d.term) * WG + threadIdx.x]);
+    cur = d.term;
+  }
+  s[0] = v8_add(s[0], v8_mul(otf_value(a0), v8_mul(tl, v8_load(d.g0))));
+  s[1] = v8_add(s[1], v8_mul(otf_value(a1), v8_mul(tl, v8_load(d.g1))));
+}
+
+// One binary EO fold (eo.rs:57-79: w'[i] = w[2i] + w[2i+1] r^-1) of a length-2h
+// witness into ow (h), with the chunked terms' part of the next round's cross
+// terms over ow (the query is never read).
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_otf_fold(
+    const E8* __restrict__ w, E8* __restrict__ ow, uint64_t chunks, E8 ri, const OtfChunk* __restrict__ desc,
+    const E8* __restrict__ tlo, const E8* __restrict__ thi, uint32_t* __restrict__ partials) {
+  const V8 vri = v8_encode_form(v8_load(ri));
+  V8 s[2] = {v8_zero(), v8_zero()};
+  int32_t cur = -1;
+  V8 tl = v8_zero();
+  for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {  // the workgroups sweep the level together
+    const OtfChunk d = desc[ch];
+    OtfAcc a0, a1;
+    otf_zero(a0);
+    otf_zero(a1);
+#pragma unroll 2
+    for (int i = 0; i < OTF_PPT; i++) {
+      const uint64_t m = (ch << OTF_LCP) + uint64_t(i) * WG + threadIdx.x;
+      const E8 x0 = w[4 * m], x1 = w[4 * m + 1], x2 = w[4 * m + 2], x3 = w[4 * m + 3];
+      const V8 y0 = v8_add(v8_load(x0), v8_mul_enc(v8_load(x1), vri));
+      const V8 y1 = v8_add(v8_load(x2), v8_mul_enc(v8_load(x3), vri));
+      ow[2 * m] = v8_store(y0);
+      ow[2 * m + 1] = v8_store(y1);
+      if (d.term >= 0) otf_mac2(a0, a1, y0, y1, v8_load(thi[uint64_t(d.term) * OTF_PPT + i]));
+    }
+    if (d.term >= 0) otf_chunk_sum(d, tlo, cur, tl, a0, a1, s);
+  }
+  block_sums<2>(s, partials);
+}
+
 // ---------------------------------------------------------------- EO first round from the BVA
 // The first binary EO round folds witness rows 2i and 2i + 1 of stride s
 // (eo.rs:119-132, 140-163): w'[i] = w[2i] + w[2i+1] r^-1 is itself a witness
 // row of stride S2 = 2s over entries [2 i s, 2 i s + 2s), with the weights
 // pinv2[L] = c^-L (L < s) and r^-1 c^-(L-s) (L >= s). The pass therefore
 // reads the BVA instead of a materialized witness, and the witness of stride s
-// is never written. The folded query q'[i] = q[2i] + q[2i+1] r is built
-// beforehand (its tensor terms fold like the query's own), so the pass only
-// reads it, writes w', and returns the next round's cross terms over its
-// outputs, sum_j w'[2j] q'[2j+1] and sum_j w'[2j+1] q'[2j].
+// is never written; like k_otf_fold it writes w' and returns the chunked
+// terms' part of the next round's cross terms over it.
 #ifndef QF_KB
 // F65521 form: the witness rows are biased-byte v_dot2_u32_u16 sums against u16
 // pairs of pinv2 coefficients (entries biased to u = v + 128, the bias removed
-// by a per-coefficient constant), four BVA words at a
-// time; each cross sum takes one lazy product per output pair (flushed every 24).
+// by a per-coefficient constant), four BVA words at a time.
 template <int S2>
-__global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bva, uint64_t available, uint64_t hp,
-                                                      const E8* __restrict__ pinv_g, const E8* __restrict__ qf,
-                                                      E8* __restrict__ ow, uint32_t* __restrict__ partials) {
+__global__ __launch_bounds__(WG) void k_otf_first_fold(const Val* __restrict__ bva, uint64_t available,
+                                                       uint64_t chunks, const E8* __restrict__ pinv_g,
+                                                       E8* __restrict__ ow, const OtfChunk* __restrict__ desc,
+                                                       const E8* __restrict__ tlo, const E8* __restrict__ thi,
+                                                       uint32_t* __restrict__ partials) {
   static_assert(S2 >= 4, "rows of at least one BVA word");
   constexpr int G = S2 / 4;   // BVA words (4 lanes) per witness row
   constexpr int NP = S2 / 2;  // lane pairs per row
@@ -1034,8 +1170,6 @@ __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bv
     for (int l = 0; l < S2; l++) t = madd(t, pinv[l].c[threadIdx.x]);
     negc[threadIdx.x] = msub(0, uint32_t((uint64_t(t) * 128) % P));
   }
-  __shared__ uint32_t sred[2 * 8][WG];
-  for (int j = 0; j < 16; j++) sred[j][threadIdx.x] = 0;
   __syncthreads();
   // one witness row of S2 entries, all below `available`: each dot2 adds less
   // than 2^25, so the NP <= 64 of a row and the bias correction stay below 2^32
@@ -1076,73 +1210,92 @@ __global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bv
       out.c[k] = x < y ? x : y;
     }
   };
-  LazySum ls[2];
-  lazy_zero(ls[0]);
-  lazy_zero(ls[1]);
-  auto flush = [&]() {
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const V8 v = lazy_value(ls[j]);
-#pragma unroll
-      for (int k = 0; k < 8; k++) sred[8 * j + k][threadIdx.x] = madd(sred[8 * j + k][threadIdx.x], v.c[k]);
-      lazy_zero(ls[j]);
+  V8 s[2] = {v8_zero(), v8_zero()};
+  int32_t cur = -1;
+  V8 tl = v8_zero();
+  for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {  // the workgroups sweep the level together
+    const OtfChunk d = desc[ch];
+    OtfAcc a0, a1;
+    otf_zero(a0);
+    otf_zero(a1);
+#pragma unroll 1
+    for (int i = 0; i < OTF_PPT; i++) {
+      asm volatile("" ::: "memory");  // the coefficient pairs stay in LDS
+      const uint64_t m = (ch << OTF_LCP) + uint64_t(i) * WG + threadIdx.x;
+      V8 y[2];
+      if ((2 * m + 2) * S2 <= available) {
+        witness(2 * m, y[0]);
+        witness(2 * m + 1, y[1]);
+      } else {
+        witness_rows<S2, 2>(bva, 2 * m, available, pinv, y);
+      }
+      ow[2 * m] = v8_store(y[0]);
+      ow[2 * m + 1] = v8_store(y[1]);
+      if (d.term >= 0) otf_mac2(a0, a1, y[0], y[1], v8_load(thi[uint64_t(d.term) * OTF_PPT + i]));
     }
-  };
-  uint32_t pending = 0;
-  for (uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x; j < hp; j += uint64_t(gridDim.x) * WG) {
-    asm volatile("" ::: "memory");  // the coefficient pairs stay in LDS
-    const V8 q0 = v8_load(qf[2 * j]), q1 = v8_load(qf[2 * j + 1]);
-    V8 w[2];
-    if ((2 * j + 2) * S2 <= available) {
-      witness(2 * j, w[0]);
-      witness(2 * j + 1, w[1]);
-    } else {
-      witness_rows<S2, 2>(bva, 2 * j, available, pinv, w);
-    }
-    ow[2 * j] = v8_store(w[0]);
-    ow[2 * j + 1] = v8_store(w[1]);
-    uint32_t lo[4], hi[4], D[15];
-    limb_split(w[0], lo, hi);
-    fold_seq(q1, D);
-    lazy_mac(ls[0], lo, hi, D);  // w'0 q'1
-    limb_split(w[1], lo, hi);
-    fold_seq(q0, D);
-    lazy_mac(ls[1], lo, hi, D);  // w'1 q'0
-    if (++pending == 24) {
-      flush();
-      pending = 0;
-    }
+    if (d.term >= 0) otf_chunk_sum(d, tlo, cur, tl, a0, a1, s);
   }
-  flush();
-  __syncthreads();
-  for (int half = WG / 2; half > 0; half /= 2) {
-    if (int(threadIdx.x) < half)
-      for (int j = 0; j < 16; j++) sred[j][threadIdx.x] = madd(sred[j][threadIdx.x], sred[j][threadIdx.x + half]);
-    __syncthreads();
-  }
-  if (threadIdx.x < 16) partials[blockIdx.x * PSTRIDE + threadIdx.x] = sred[threadIdx.x][0];
+  block_sums<2>(s, partials);
 }
 #else
 template <int S2>
-__global__ __launch_bounds__(WG) void k_eo_first_fold(const Val* __restrict__ bva, uint64_t available, uint64_t hp,
-                                                      const E8* __restrict__ pinv_g, const E8* __restrict__ qf,
-                                                      E8* __restrict__ ow, uint32_t* __restrict__ partials) {
+__global__ __launch_bounds__(WG) void k_otf_first_fold(const Val* __restrict__ bva, uint64_t available,
+                                                       uint64_t chunks, const E8* __restrict__ pinv_g,
+                                                       E8* __restrict__ ow, const OtfChunk* __restrict__ desc,
+                                                       const E8* __restrict__ tlo, const E8* __restrict__ thi,
+                                                       uint32_t* __restrict__ partials) {
   __shared__ E8 pinv[S2];
   for (int i = threadIdx.x; i < S2; i += WG) pinv[i] = pinv_g[i];
   __syncthreads();
   V8 s[2] = {v8_zero(), v8_zero()};
-  for (uint64_t j = uint64_t(blockIdx.x) * WG + threadIdx.x; j < hp; j += uint64_t(gridDim.x) * WG) {
-    const V8 q0 = v8_load(qf[2 * j]), q1 = v8_load(qf[2 * j + 1]);
-    V8 w[2];
-    witness_rows<S2, 2>(bva, 2 * j, available, pinv, w);
-    ow[2 * j] = v8_store(w[0]);
-    ow[2 * j + 1] = v8_store(w[1]);
-    s[0] = v8_add(s[0], v8_mul(w[0], q1));
-    s[1] = v8_add(s[1], v8_mul(w[1], q0));
+  int32_t cur = -1;
+  V8 tl = v8_zero();
+  for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {  // the workgroups sweep the level together
+    const OtfChunk d = desc[ch];
+    OtfAcc a0, a1;
+    otf_zero(a0);
+    otf_zero(a1);
+    for (int i = 0; i < OTF_PPT; i++) {
+      const uint64_t m = (ch << OTF_LCP) + uint64_t(i) * WG + threadIdx.x;
+      V8 y[2];
+      witness_rows<S2, 2>(bva, 2 * m, available, pinv, y);
+      ow[2 * m] = v8_store(y[0]);
+      ow[2 * m + 1] = v8_store(y[1]);
+      if (d.term >= 0) otf_mac2(a0, a1, y[0], y[1], v8_load(thi[uint64_t(d.term) * OTF_PPT + i]));
+    }
+    if (d.term >= 0) otf_chunk_sum(d, tlo, cur, tl, a0, a1, s);
   }
   block_sums<2>(s, partials);
 }
 #endif
+
+// The rest of a level's cross terms over the folded w (length h): each piece
+// projects its pairs on its term's tensor (the factors of the pair-index bits
+// multiplied per pair), each item adds v times the output it pairs with.
+__global__ __launch_bounds__(WG) void k_otf_sparse(const E8* __restrict__ w, const OtfPieceDev* __restrict__ pieces,
+                                                   uint32_t npieces, const OtfItemDev* __restrict__ items,
+                                                   uint32_t nitems, const E8 (*__restrict__ f)[2],
+                                                   uint32_t* __restrict__ partials) {
+  V8 s[2] = {v8_zero(), v8_zero()};
+  for (uint32_t pc = blockIdx.x; pc < npieces; pc += gridDim.x) {
+    const OtfPieceDev pp = pieces[pc];
+    V8 a0 = v8_zero(), a1 = v8_zero();
+    for (uint32_t k = threadIdx.x; k < pp.count; k += WG) {
+      const V8 q = otf_tensor(f, pp.fbase, pp.nbits, pp.rel0 + k);
+      const uint64_t m = pp.pair0 + k;
+      a0 = v8_add(a0, v8_mul(v8_load(w[2 * m]), q));
+      a1 = v8_add(a1, v8_mul(v8_load(w[2 * m + 1]), q));
+    }
+    s[0] = v8_add(s[0], v8_mul(a0, v8_load(pp.g0)));
+    s[1] = v8_add(s[1], v8_mul(a1, v8_load(pp.g1)));
+  }
+  for (uint64_t it = uint64_t(blockIdx.x) * WG + threadIdx.x; it < nitems; it += uint64_t(gridDim.x) * WG) {
+    const OtfItemDev I = items[it];
+    if (I.j & 1) s[0] = v8_add(s[0], v8_mul(v8_load(w[I.j - 1]), v8_load(I.v)));
+    else s[1] = v8_add(s[1], v8_mul(v8_load(w[I.j + 1]), v8_load(I.v)));
+  }
+  block_sums<2>(s, partials);
+}
 
 // Joined lanes of stride-2s projection chunks: out[ch][l] = f[o][0] parts[ch][l]
 // + f[o][1] parts[ch][s + l], o = owner[ch] (the stride-s projection, whose row
@@ -2039,15 +2192,87 @@ std::pair<E8, E8> eo_first_cross(DevCtx& c, const EoProjection& pr, const std::v
   return {sums[0], sums[1]};
 }
 
-std::pair<E8, E8> eo_first_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t rows, int s,
-                                const std::vector<E8>& pinv2, const E8* qf, E8* ow) {
+namespace {
+// Workgroups of WG threads of `kernel` that are resident on the device at once:
+// a persistent launch of more would run its last workgroups after the others.
+template <typename K>
+uint32_t resident_groups(K kernel) {
+  int per = 0;
+  QF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, WG, 0));
+  return uint32_t(std::max(per, 1)) * uint32_t(device_cus());
+}
+}  // namespace
+
+OtfQuery::OtfQuery(DevCtx& c, const std::vector<std::array<E8, 2>>& factors, uint64_t max_chunks, size_t max_fast,
+                   size_t max_pieces, size_t max_items)
+    : f_(std::max<uint64_t>(2 * factors.size(), 2)),
+      terms_(3 * std::max<uint64_t>(max_fast, 1)),
+      desc_(3 * std::max<uint64_t>(max_chunks, 1)),
+      tlo_(std::max<uint64_t>(max_fast, 1) * WG),
+      thi_(std::max<uint64_t>(max_fast, 1) * OTF_PPT),
+      pieces_(4 * std::max<uint64_t>(max_pieces, 1)),
+      items_(2 * std::max<uint64_t>(max_items, 1)),
+      max_chunks_(max_chunks),
+      max_fast_(max_fast),
+      max_pieces_(std::max<size_t>(max_pieces, 1)),
+      max_items_(std::max<size_t>(max_items, 1)) {
+  h2d(c, f_.p, factors.data(), factors.size() * sizeof(factors[0]));
+}
+
+// Uploads the level's records, builds the chunk records and term tables, and
+// returns the number of partial rows the sparse pass will use.
+uint32_t OtfQuery::prepare(DevCtx& c, const OtfLevel& lv, uint64_t chunks) {
+  require(chunks <= max_chunks_ && lv.fast.size() <= max_fast_, QF_ERR_ARGUMENT, "on-the-fly query level size");
+  if (lv.pieces.size() > max_pieces_) {  // later levels may split more terms than the first
+    max_pieces_ = lv.pieces.size();
+    pieces_ = DVec(4 * max_pieces_);
+  }
+  if (lv.items.size() > max_items_) {
+    max_items_ = lv.items.size();
+    items_ = DVec(2 * max_items_);
+  }
+  h2d(c, terms_.p, lv.fast.data(), lv.fast.size() * sizeof(OtfTermDev));
+  h2d(c, pieces_.p, lv.pieces.data(), lv.pieces.size() * sizeof(OtfPieceDev));
+  h2d(c, items_.p, lv.items.data(), lv.items.size() * sizeof(OtfItemDev));
+  const uint32_t nt = uint32_t(lv.fast.size());
+  const uint64_t threads = chunks + uint64_t(nt) * (WG + OTF_PPT);
+  hipLaunchKernelGGL(k_otf_desc, dim3(uint32_t((threads + WG - 1) / WG)), dim3(WG), 0, c.st,
+                     reinterpret_cast<const OtfTermDev*>(terms_.p), nt, (const E8(*)[2])f_.p, chunks,
+                     reinterpret_cast<OtfChunk*>(desc_.p), tlo_.p, thi_.p);
+  QF_HIP(hipGetLastError());
+  if (lv.pieces.empty() && lv.items.empty()) return 0;
+  const uint64_t want = std::max<uint64_t>(lv.pieces.size(), (lv.items.size() + WG - 1) / WG);
+  return uint32_t(std::min<uint64_t>(want, 64));
+}
+
+// The sparse pass (gs workgroups, partial rows after the fold's `groups`) and the level's cross terms.
+std::pair<E8, E8> OtfQuery::finish(DevCtx& c, const OtfLevel& lv, const E8* ow, uint32_t groups, uint32_t gs) {
+  if (gs) {
+    hipLaunchKernelGGL(k_otf_sparse, dim3(gs), dim3(WG), 0, c.st, ow, reinterpret_cast<const OtfPieceDev*>(pieces_.p),
+                       uint32_t(lv.pieces.size()), reinterpret_cast<const OtfItemDev*>(items_.p),
+                       uint32_t(lv.items.size()), (const E8(*)[2])f_.p, c.partials + uint64_t(groups) * PSTRIDE);
+    QF_HIP(hipGetLastError());
+  }
+  auto sums = finish_sums<2>(c, groups + gs);
+  return {sums[0], sums[1]};
+}
+
+std::pair<E8, E8> OtfQuery::first_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t rows, int s,
+                                       const std::vector<E8>& pinv2, const OtfLevel& lv, E8* ow) {
   require(s >= 2 && s <= 64 && pow2(uint64_t(s)) && int(pinv2.size()) == 2 * s, QF_ERR_ARGUMENT, "witness stride");
-  require(rows >= 4 && rows % 4 == 0, QF_ERR_ARGUMENT, "witness rows");
+  require(rows % (uint64_t(4) << OTF_LCP) == 0, QF_ERR_ARGUMENT, "witness rows");
+  const uint64_t chunks = (rows / 4) >> OTF_LCP;
+  const uint32_t gs = prepare(c, lv, chunks);
   E8* t = put_table(c, scratch_tab(), pinv2.data(), pinv2.size() * 16);
-  const uint64_t hp = rows / 4;
-  const uint32_t g = groups_for(hp);
-#define QF_FIRST_FOLD(S2) \
-  hipLaunchKernelGGL(k_eo_first_fold<S2>, dim3(g), dim3(WG), 0, c.st, bva, available, hp, t, qf, ow, c.partials)
+  const OtfChunk* d = reinterpret_cast<const OtfChunk*>(desc_.p);
+  uint32_t g = 0;
+#define QF_FIRST_FOLD(S2)                                                                                         \
+  {                                                                                                                \
+    static const uint32_t resident = resident_groups(k_otf_first_fold<S2>);                                        \
+    g = uint32_t(std::min<uint64_t>({chunks, resident, uint64_t(DevCtx::MAX_GROUPS - gs)}));                        \
+    hipLaunchKernelGGL(k_otf_first_fold<S2>, dim3(g), dim3(WG), 0, c.st, bva, available, chunks, t, ow, d, tlo_.p, \
+                       thi_.p, c.partials);                                                                        \
+  }
   switch (2 * s) {
     case 4: QF_FIRST_FOLD(4); break;
     case 8: QF_FIRST_FOLD(8); break;
@@ -2058,8 +2283,19 @@ std::pair<E8, E8> eo_first_fold(DevCtx& c, const Val* bva, uint64_t available, u
   }
 #undef QF_FIRST_FOLD
   QF_HIP(hipGetLastError());
-  auto sums = finish_sums<2>(c, g);
-  return {sums[0], sums[1]};
+  return finish(c, lv, ow, g, gs);
+}
+
+std::pair<E8, E8> OtfQuery::fold(DevCtx& c, const E8* w, E8* ow, uint64_t h, const E8& ri, const OtfLevel& lv) {
+  require(h % (uint64_t(2) << OTF_LCP) == 0, QF_ERR_ARGUMENT, "on-the-fly fold length");
+  const uint64_t chunks = (h / 2) >> OTF_LCP;
+  const uint32_t gs = prepare(c, lv, chunks);
+  static const uint32_t resident = resident_groups(k_otf_fold);
+  const uint32_t g = uint32_t(std::min<uint64_t>({chunks, resident, uint64_t(DevCtx::MAX_GROUPS - gs)}));
+  hipLaunchKernelGGL(k_otf_fold, dim3(g), dim3(WG), 0, c.st, w, ow, chunks, ri,
+                     reinterpret_cast<const OtfChunk*>(desc_.p), tlo_.p, thi_.p, c.partials);
+  QF_HIP(hipGetLastError());
+  return finish(c, lv, ow, g, gs);
 }
 
 // One launch over every task (the batched form).

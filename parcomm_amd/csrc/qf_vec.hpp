@@ -152,12 +152,64 @@ EoCorr eo_correlations(DevCtx& c, const EoProjection& pr, const std::vector<std:
 // c^-l (l < s), coef[task] = (sum_l low[l] c^l) x the task's row-bit-0 factor.
 std::pair<E8, E8> eo_first_cross(DevCtx& c, const EoProjection& pr, const std::vector<E8>& pinv,
                                  const std::vector<std::array<E8, 2>>& coef);
-// The first binary round read from the BVA (rows = logical / s, a multiple of
-// 4): ow[i] = sum_{L<2s} v[2is + L] pinv2[L] (the fold of witness rows 2i,
-// 2i+1 by r^-1) for i < rows / 2, with the folded query qf (rows / 2, built
-// beforehand); returns the next round's cross terms over the outputs.
-std::pair<E8, E8> eo_first_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t rows, int s,
-                                const std::vector<E8>& pinv2, const E8* qf, E8* ow);
+// The EO query of a binary level L is a sum of tensor terms (tbv.rs:187-284
+// folded by eo.rs:57-79); the fold kernels below project each round's output
+// pairs on it instead of reading a materialized query. At output level L a
+// term of 2^nf entries (nf >= log2 s + L + 1) covers the pairs [pair0, pair0 +
+// 2^(nf - log2 s - L - 1)); its value at output 2m + x of its range is
+// a_L f[ls+L][x] prod_b f[ls+L+1+b][bit b of m - pair0] (a_L: the term's scale
+// folded over the lower bits), so the round's cross terms sum_m w'[2m] q'[2m+1]
+// and sum_m w'[2m+1] q'[2m] add g0 = a_L f[ls+L][1] and g1 = a_L f[ls+L][0]
+// times the projections of w' on the term's pair tensor.
+constexpr int OTF_PPT = 8;    // pairs per thread in one chunk
+constexpr int OTF_LCP = 11;   // log2 pairs per chunk (256 threads x OTF_PPT)
+constexpr uint32_t OTF_PIECE = 2048;  // pairs of one piece
+// A term projected in whole chunks (at least one chunk long), disjoint from the others.
+struct OtfTermDev {
+  uint64_t pair0;
+  uint32_t lpairs;  // log2 of the pair count, at least OTF_LCP
+  uint32_t fbase;   // factor-pair index of the pair-index bit 0 (the term's bit ls + L + 1)
+  E8 g0, g1;
+};
+// Up to OTF_PIECE pairs of a term projected pair by pair (terms shorter than a
+// chunk, and terms that overlap the chunked ones).
+struct OtfPieceDev {
+  uint64_t pair0;  // first pair at the level
+  uint64_t rel0;   // its pair index inside the term
+  uint32_t count, fbase, nbits, pad;
+  E8 g0, g1;
+};
+// A term of at most one output at the level: value v at output j, paired with output j ^ 1.
+struct OtfItemDev {
+  uint64_t j, pad;
+  E8 v;
+};
+struct OtfLevel {
+  std::vector<OtfTermDev> fast;  // sorted by pair0
+  std::vector<OtfPieceDev> pieces;
+  std::vector<OtfItemDev> items;
+};
+// Device state of one EO's on-the-fly rounds: every term's factor pairs and
+// the per-level tables (chunk coefficients, the chunked terms' low-bit tables).
+class OtfQuery {
+ public:
+  OtfQuery(DevCtx& c, const std::vector<std::array<E8, 2>>& factors, uint64_t max_chunks, size_t max_fast,
+           size_t max_pieces, size_t max_items);
+  // The first binary round read from the BVA (rows = logical / s, rows / 4 a
+  // multiple of the chunk): ow[i] = sum_{L<2s} v[2is + L] pinv2[L] (the fold of
+  // witness rows 2i, 2i+1 by r^-1) for i < rows / 2; returns the cross terms over ow.
+  std::pair<E8, E8> first_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t rows, int s,
+                               const std::vector<E8>& pinv2, const OtfLevel& lv, E8* ow);
+  // ow[i] = w[2i] + w[2i+1] ri, i < h (h / 2 a multiple of the chunk); returns the cross terms over ow.
+  std::pair<E8, E8> fold(DevCtx& c, const E8* w, E8* ow, uint64_t h, const E8& ri, const OtfLevel& lv);
+
+ private:
+  uint32_t prepare(DevCtx& c, const OtfLevel& lv, uint64_t chunks);
+  std::pair<E8, E8> finish(DevCtx& c, const OtfLevel& lv, const E8* ow, uint32_t groups, uint32_t gs);
+  DVec f_, terms_, desc_, tlo_, thi_, pieces_, items_;
+  uint64_t max_chunks_;
+  size_t max_fast_, max_pieces_, max_items_;
+};
 // Packed EO witness: w[row] = sum_lane v[row*s + lane] * pinv[lane] for rows < n_rows.
 void eo_witness(DevCtx& c, const Val* bva, uint64_t available, uint64_t n_rows, int s, const std::vector<E8>& pinv,
                 E8* w);

@@ -31,12 +31,13 @@ def device_memory(field=QF_FIELD_F65521):
 
 def request_bytes(params):
     """Device memory one opening of these parameters holds at its peak: the
-    even/odd witness and query vectors (from the first fold on: logical/2s
-    elements each, then a quarter of that each; 2 x logical/s elements of 16
-    bytes bound them with a margin), the batched folded codeword and tree, and
-    the sumcheck vectors."""
+    even/odd witness of the first two binary levels (rows/2 and rows/4
+    elements of 16 bytes, rows = logical/s) and the query materialized once
+    the rounds are short (n + n/2 elements, n at most rows/2 and 2^22), the
+    batched folded codeword and tree, and the sumcheck vectors."""
     s = 1 << params.config.packed_log
-    eo = 2 * (params.logical // s) * 16
+    rows = params.logical // s
+    eo = 12 * rows + 24 * min(rows // 2, 1 << 22)
     n_values = params.block_len // field_info(params.field)["lanes"]
     folded = 2 * n_values * 16 + 2 * (n_values // 16) * 32
     return eo + folded + 2 * n_values * 16
