@@ -281,8 +281,10 @@ def phase_hbm(pa, res):
     wall time, against the HBM peak. Per encoded block of 2^c entries of w
     bytes, L lanes, 16-byte elements and R = 2^(c+1) / (32 L) codeword rows:
     commit 2^c w + 4 (2^c / L) 16 + 2 R 32 (input, codeword write and
-    leaf-hash read, digests); packed EO 2 active w + 2 (logical/s) 16 + the
-    rounds' 3 n 16 for n = logical/s, /2, ...; projection B 2^c w + 3 (2^c/L)
+    leaf-hash read, digests); packed EO 2 active w (the projection and the
+    first fold read the BVA) + 2 (logical/s) 16 (every binary level's witness,
+    logical/2s + logical/4s + ... elements, written once and read once; the
+    query is a function of the claim terms, not data); projection B 2^c w + 3 (2^c/L)
     16 (lane sums, weights and weight map); folding B 2^c w + (2^c/L) 16 plus
     one block's commit; sumcheck 6 (2^c/L) 16."""
     params, timings = res["params"], res["timings"]
@@ -293,15 +295,11 @@ def phase_hbm(pa, res):
     rows = 2 * c2 // (32 * L)
     commit_block = c2 * w + 4 * (c2 // L) * 16 + 2 * rows * 32
     N = params.logical >> params.config.packed_log
-    eo_rounds, n = 0, N
-    while n > 1:
-        eo_rounds += 3 * n * 16
-        n //= 2
     k = len(timings)
     phases = {
         "commit": (sum(t["commit"]["timed_blocks"] for t in timings) / k * commit_block,
                    sum(t["commit_s"] for t in timings) / k),
-        "packed_eo": (2 * params.active * w + 2 * N * 16 + eo_rounds, "packed_eo_seconds"),
+        "packed_eo": (2 * params.active * w + 2 * N * 16, "packed_eo_seconds"),
         "projection": (B * c2 * w + 3 * (c2 // L) * 16, "pcs_projection_seconds"),
         "folding": (B * c2 * w + (c2 // L) * 16 + commit_block, "terminal_batch_folding_seconds"),
         "sumcheck": (6 * (c2 // L) * 16, "pcs_sumcheck_seconds"),
