@@ -1074,11 +1074,13 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
 }
 
 // ================================================================ packed PCS openings (packed.rs, packed_batch.rs)
+// `first`: the first message, when the caller has it already (computed beside other work).
 void sumcheck(DevCtx& c, DVec& v, DVec& w, E8& claim, Packed& t, std::vector<std::array<E8, 2>>& rounds,
-              std::vector<E8>& reduced, E8& fval, E8& fwt) {
+              std::vector<E8>& reduced, E8& fval, E8& fwt, const std::pair<E8, E8>* first = nullptr) {
   uint64_t n = v.n;
   E8 m0{}, m1{};
-  if (n > 1) std::tie(m0, m1) = dv::sc_msg(c, v.p, w.p, n / 2);
+  if (first) std::tie(m0, m1) = *first;
+  else if (n > 1) std::tie(m0, m1) = dv::sc_msg(c, v.p, w.p, n / 2);
   // the last rounds run on host copies once the vectors are short (as TwoFold)
   std::vector<E8> hv, hw;
   while (n > 1) {  // each fold also returns the next round's message
@@ -1236,15 +1238,46 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   DVec ehi, elo;
   eq_tables_device(c, outer, ehi, elo);
   DVec weights(plen), values(plen);
-  weight_images(c, weights.p, ehi, elo, images);
+  // The weight images and then the sumcheck's first message (it needs the
+  // folded values, not the folded root) run on the thread's second stream,
+  // beside the block fold and the folded block's commit on c.st: the weights
+  // are VALU work and the commit's leaf hash leaves HBM time free. c.st waits
+  // for both before the sumcheck rounds.
+  // (declared after the vectors the second stream reads: on an early exit it
+  // drains that stream before they are released)
+  struct SideWork {
+    hipStream_t s2 = nullptr;
+    hipEvent_t tables = nullptr, values = nullptr;
+    dv::SideMsg msg;
+    ~SideWork() {
+      if (s2) (void)hipStreamSynchronize(s2);
+      for (hipEvent_t e : {tables, values, msg.done})
+        if (e) (void)hipEventDestroy(e);
+    }
+  } sw;
+  const bool side = plen > 1;
+  if (side) {
+    sw.s2 = thread_stream2();
+    QF_HIP(hipEventCreateWithFlags(&sw.tables, hipEventDisableTiming));
+    QF_HIP(hipEventCreateWithFlags(&sw.values, hipEventDisableTiming));
+    QF_HIP(hipEventRecord(sw.tables, c.st));
+    QF_HIP(hipStreamWaitEvent(sw.s2, sw.tables, 0));
+  }
+  weight_images(c, weights.p, ehi, elo, images, sw.s2);
   tm.projection_ms = ms_since(t0);
   trace_phase("projection", tm.projection_ms);
   t0 = Clock::now();
   block_fold(c, cs.d_bva, p.layout.active, p.block_len, blocks, fw, values.p, plen);
+  if (side) {
+    QF_HIP(hipEventRecord(sw.values, c.st));
+    QF_HIP(hipStreamWaitEvent(sw.s2, sw.values, 0));
+    sw.msg = dv::sc_msg_launch(c, sw.s2, values.p, weights.p, plen / 2);
+  }
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode folded = alloc_code(R);
   launch_commit_values(p.geom, values.p, folded.cw.p, p.alpha_encode(p.pp.codes.size() - 1), folded.tree, c.st);
   QF_HIP(hipGetLastError());
+  if (side) QF_HIP(hipStreamWaitEvent(c.st, sw.msg.done, 0));
   folded.root = download_root(c, folded.tree, R);
   t.bytes(folded.root.data(), 32);
   tm.folding_ms = ms_since(t0);
@@ -1253,7 +1286,9 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   std::vector<std::array<E8, 2>> rounds;
   std::vector<E8> reduced;
   E8 fval, fwt;
-  sumcheck(c, values, weights, claim, t, rounds, reduced, fval, fwt);
+  std::pair<E8, E8> first_msg;
+  if (side) first_msg = dv::sc_msg_finish(c, sw.msg);
+  sumcheck(c, values, weights, claim, t, rounds, reduced, fval, fwt, side ? &first_msg : nullptr);
   require(fwt == batch_weight(outer.data(), n_outer, reduced, coefs) && claim == fval * fwt, QF_ERR_INVALID_COMMITMENT,
           "batch sumcheck terminal");
   tm.sumcheck_ms = ms_since(t0);

@@ -2456,13 +2456,44 @@ void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_le
   QF_HIP(hipGetLastError());
 }
 
-void weight_images(DevCtx& c, E8* out, const DVec& hi, const DVec& lo, const std::array<E8, LANES>& images) {
+void weight_images(DevCtx& c, E8* out, const DVec& hi, const DVec& lo, const std::array<E8, LANES>& images,
+                   hipStream_t st) {
   require(pow2(hi.n) && pow2(lo.n), QF_ERR_ARGUMENT, "weight tables");
   Images im;
   for (int j = 0; j < LANES; j++) im.m[j] = images[j];
   uint64_t n = hi.n * lo.n;
-  hipLaunchKernelGGL(k_weight_images, grid_for(n), dim3(WG), 0, c.st, out, n, hi.p, lo.p, ilog2u(lo.n), im);
+  hipLaunchKernelGGL(k_weight_images, grid_for(n), dim3(WG), 0, st ? st : c.st, out, n, hi.p, lo.p, ilog2u(lo.n),
+                     im);
   QF_HIP(hipGetLastError());
 }
+
+namespace dv {
+// The sumcheck message's block sums go past the rows every other reduction
+// uses in the thread's pinned partial buffer (MAX_GROUPS rows of PSTRIDE), so
+// kernels on the prover's own stream may write theirs meanwhile.
+constexpr uint64_t SIDE_PARTIALS = uint64_t(DevCtx::MAX_GROUPS) * PSTRIDE;
+SideMsg sc_msg_launch(DevCtx& c, hipStream_t st, const E8* v, const E8* w, uint64_t h) {
+  SideMsg m;
+  m.groups = groups_for(h);
+  hipLaunchKernelGGL(k_reduce<R_SC>, dim3(m.groups), dim3(WG), 0, st, v, w, h, c.partials + SIDE_PARTIALS);
+  QF_HIP(hipGetLastError());
+  QF_HIP(hipEventCreateWithFlags(&m.done, hipEventDisableTiming));
+  QF_HIP(hipEventRecord(m.done, st));
+  return m;
+}
+std::pair<E8, E8> sc_msg_finish(DevCtx& c, SideMsg& m) {
+  QF_HIP(hipEventSynchronize(m.done));
+  QF_HIP(hipEventDestroy(m.done));
+  m.done = nullptr;
+  V8 s0 = v8_zero(), s1 = v8_zero();
+  const uint32_t* hp = c.host_partials + SIDE_PARTIALS;
+  for (uint32_t g = 0; g < m.groups; g++)
+    for (int k = 0; k < LANES; k++) {
+      s0.c[k] = madd(s0.c[k], hp[g * 16 + k]);
+      s1.c[k] = madd(s1.c[k], hp[g * 16 + LANES + k]);
+    }
+  return {v8_store(s0), v8_store(s1)};
+}
+}  // namespace dv
 
 }  // namespace qf

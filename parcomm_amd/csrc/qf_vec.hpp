@@ -223,7 +223,15 @@ void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& ta
 // values[i] = sum_b fw[b] * packed(b, i), lane j of packed(b, i) = entry b*block_len + LANES*i + j
 void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
                 const std::vector<E8>& fw, E8* values, uint64_t n);
-// weights[i] = sum_j eq(i).c[j] * images[j], eq = hi (x) lo tables
-void weight_images(DevCtx& c, E8* out, const DVec& hi, const DVec& lo, const std::array<E8, LANES>& images);
+// weights[i] = sum_j eq(i).c[j] * images[j], eq = hi (x) lo tables (on st, or on c.st when st is null)
+void weight_images(DevCtx& c, E8* out, const DVec& hi, const DVec& lo, const std::array<E8, LANES>& images,
+                   hipStream_t st = nullptr);
+namespace dv {
+// sc_msg launched on another stream `st` (its block sums in a partial area of
+// their own); finish waits for it and returns the message.
+struct SideMsg { uint32_t groups = 0; hipEvent_t done = nullptr; };
+SideMsg sc_msg_launch(DevCtx& c, hipStream_t st, const E8* v, const E8* w, uint64_t h);
+std::pair<E8, E8> sc_msg_finish(DevCtx& c, SideMsg& m);
+}  // namespace dv
 
 }  // namespace qf
