@@ -1028,11 +1028,16 @@ __device__ inline V8 otf_tensor(const E8 (*__restrict__ f)[2], uint32_t base, ui
 }
 
 // The level's chunk records (threads below `chunks`, each locating its chunk's
-// term by bisection over the sorted terms) and the chunked terms' tables.
+// term by bisection over the sorted terms) and the chunked terms' tables: tlo
+// over the pair-index bits 0 .. LO-1 (2^LO entries per term), thi over bits
+// LO .. OTF_LCP-1 (LO = 8: the fold kernels' thread and i bits; LO = 7: the
+// MFMA first fold's wave/lane and iteration bits).
+template <int LO>
 __global__ __launch_bounds__(WG) void k_otf_desc(const OtfTermDev* __restrict__ terms, uint32_t nterms,
                                                  const E8 (*__restrict__ f)[2], uint64_t chunks,
                                                  OtfChunk* __restrict__ desc, E8* __restrict__ tlo,
                                                  E8* __restrict__ thi) {
+  constexpr uint32_t NLO = 1u << LO, NHI = 1u << (OTF_LCP - LO);
   const uint64_t gid = uint64_t(blockIdx.x) * WG + threadIdx.x;
   if (gid < chunks) {
     const uint64_t p = gid << OTF_LCP;
@@ -1057,12 +1062,12 @@ __global__ __launch_bounds__(WG) void k_otf_desc(const OtfTermDev* __restrict__ 
     return;
   }
   uint64_t e = gid - chunks;
-  if (e < uint64_t(nterms) * WG) {
-    tlo[e] = v8_store(otf_tensor(f, terms[e / WG].fbase, 8, e % WG));
+  if (e < uint64_t(nterms) * NLO) {
+    tlo[e] = v8_store(otf_tensor(f, terms[e / NLO].fbase, LO, e % NLO));
     return;
   }
-  e -= uint64_t(nterms) * WG;
-  if (e < uint64_t(nterms) * OTF_PPT) thi[e] = v8_store(otf_tensor(f, terms[e / OTF_PPT].fbase + 8, 3, e % OTF_PPT));
+  e -= uint64_t(nterms) * NLO;
+  if (e < uint64_t(nterms) * NHI) thi[e] = v8_store(otf_tensor(f, terms[e / NHI].fbase + LO, OTF_LCP - LO, e % NHI));
 }
 
 // Per-chunk sums sum_i thi[i] y_i over one thread's OTF_PPT pairs: lazy dot2
@@ -1235,6 +1240,132 @@ __global__ __launch_bounds__(WG) void k_otf_first_fold(const Val* __restrict__ b
     }
     if (d.term >= 0) otf_chunk_sum(d, tlo, cur, tl, a0, a1, s);
   }
+  block_sums<2>(s, partials);
+}
+
+// The same round on the matrix cores (S2 = 32, 64, 128): a witness row is S2
+// BVA entries times the pinv2 coefficients, an exact i8 contraction.
+// - Digits: coefficient t as t' = t or t - p in [-32881, 32639], t' = 256 h + l
+//   with h, l in [-128, 127] (as the MFMA projection). B column n < 8 holds the
+//   low digits of coefficient lane n, column 8 + n the high digits.
+// - v_mfma_i32_16x16x32_i8 with A[m][k] = entry k of row m (lane i of 16-lane
+//   group q: row i, entries 8q .. 8q + 7, one 8-byte load), B[k][n] constant
+//   per lane, D[m][n] in lane n + 16 (m / 4), element m % 4; |D| <= S2 2^14, so
+//   w[m] lane n = D[m][n] + 256 D[m][8 + n] (|.| < 2^29.1) is exact in i32.
+// - A wave takes 64 rows per iteration (four groups of 16), stores the values
+//   in LDS and reads back one row per lane: that row's 16-byte output.
+// - Cross sums: a workgroup iteration covers 256 rows (128 pairs), a chunk 16
+//   iterations. Pair-index bits 0..6 (wave, lane / 2) weigh tlo (128 entries),
+//   bits 7..10 (iteration) thi (16 entries); even lanes hold the element 2m of
+//   their pair (c0, with g0), odd lanes 2m + 1 (c1, with g1).
+template <int S2>
+__global__ __launch_bounds__(WG) void k_otf_first_fold_mfma(const Val* __restrict__ bva, uint64_t available,
+                                                            uint64_t chunks, const E8* __restrict__ pinv_g,
+                                                            E8* __restrict__ ow, const OtfChunk* __restrict__ desc,
+                                                            const E8* __restrict__ tlo, const E8* __restrict__ thi,
+                                                            uint32_t* __restrict__ partials) {
+  static_assert(S2 % 32 == 0 && S2 <= 128, "whole 32-entry k blocks");
+  constexpr int KB = S2 / 32;
+  __shared__ __align__(16) uint16_t tbuf[MF_WAVES][64][8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 15, q = lane >> 4, par = lane & 1;
+  long bop[KB];
+#pragma unroll
+  for (int kb = 0; kb < KB; kb++) {
+    uint64_t x = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int32_t t = int32_t(pinv_g[32 * kb + 8 * q + j].c[n & 7]);
+      const int32_t tv = t <= 32639 ? t : t - int32_t(P);
+      const int32_t hd = (tv + 128) >> 8, ld = tv - 256 * hd;
+      x |= uint64_t(uint8_t(int8_t(n < 8 ? ld : hd))) << (8 * j);
+    }
+    bop[kb] = long(x);
+  }
+  V8 sum = v8_zero();  // this lane's cross sum: c0 (even lanes) or c1 (odd lanes)
+  int32_t cur = -1;
+  V8 tl = v8_zero();
+  // A operands (8 entries of one row per lane, four groups of 16 rows) of
+  // iteration `it` of chunk `ch`; the next iteration's load while this one computes
+  auto load_a = [&](uint64_t ch, int it, long (&a)[4][KB]) {
+    const uint64_t row0 = (ch << (OTF_LCP + 1)) + uint64_t(it) * WG + uint64_t(wave) * 64;
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int kb = 0; kb < KB; kb++) {
+        const uint64_t at = (row0 + 16 * g + n) * S2 + 32 * kb + 8 * q;
+        if (at + 8 <= available) {
+          a[g][kb] = *reinterpret_cast<const long*>(bva + at);
+        } else {  // entries at or past the active end read as zero
+          uint64_t x = 0;
+          for (int j = 0; j < 8; j++)
+            if (at + j < available) x |= uint64_t(uint8_t(bva[at + j])) << (8 * j);
+          a[g][kb] = long(x);
+        }
+      }
+  };
+  // one iteration on operands `a`: the contraction, the digit join, the LDS
+  // transpose, the store and the row's cross-sum product
+  auto step = [&](const long (&a)[4][KB], uint64_t ch, int it, const OtfChunk& d, LazySum& acc) {
+    const uint64_t row0 = (ch << (OTF_LCP + 1)) + uint64_t(it) * WG + uint64_t(wave) * 64;  // this wave's rows
+    qf_i32x4 dsum[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      dsum[g] = qf_i32x4{0, 0, 0, 0};
+#pragma unroll
+      for (int kb = 0; kb < KB; kb++) dsum[g] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a[g][kb], bop[kb], dsum[g], 0, 0, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the previous iteration's reads of tbuf are done
+    // lanes n < 8 join rows 4q, 4q + 1 of each group, lanes n >= 8 rows 4q + 2, 4q + 3
+    // (the low digit sum from lane n & 7, the high one from lane 8 + (n & 7)); a
+    // multiple of p keeps the sum positive for two 16-bit folds
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int32_t own = dsum[g][j], other = __shfl_xor(own, 8, 64);
+        if ((j < 2) == (n < 8)) {
+          const int32_t lo = n < 8 ? own : other, hi = n < 8 ? other : own;
+          const uint32_t x = fold16(fold16(uint32_t(lo + 256 * hi + 16384 * int32_t(P))));
+          const uint32_t y = x - P;
+          tbuf[wave][16 * g + 4 * q + j][n & 7] = uint16_t(x < y ? x : y);
+        }
+      }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const uint4 raw = *reinterpret_cast<const uint4*>(&tbuf[wave][lane][0]);
+    E8 e;
+    *reinterpret_cast<uint4*>(&e) = raw;
+    ow[row0 + lane] = e;
+    if (d.term >= 0) {
+      uint32_t D[15], l4[4], h4[4];
+      fold_seq(v8_load(thi[uint64_t(d.term) * 16 + it]), D);
+      limb_split(v8_load(e), l4, h4);
+      lazy_mac(acc, l4, h4, D);
+    }
+  };
+  // two operand sets: the loads of the next iteration are in flight while one computes
+  long a0[4][KB], a1[4][KB];
+  if (blockIdx.x < chunks) load_a(blockIdx.x, 0, a0);
+  for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {  // the workgroups sweep the level together
+    const OtfChunk d = desc[ch];
+    LazySum acc;
+    lazy_zero(acc);
+    for (int it = 0; it < 16; it += 2) {
+      load_a(ch, it + 1, a1);
+      step(a0, ch, it, d, acc);
+      if (it + 2 < 16) load_a(ch, it + 2, a0);
+      else if (ch + gridDim.x < chunks) load_a(ch + gridDim.x, 0, a0);
+      step(a1, ch, it + 1, d, acc);
+    }
+    if (d.term >= 0) {
+      if (d.term != cur) {
+        tl = v8_load(tlo[uint64_t(d.term) * 128 + wave * 32 + (lane >> 1)]);
+        cur = d.term;
+      }
+      sum = v8_add(sum, v8_mul(lazy_value(acc), v8_mul(tl, v8_load(par ? d.g1 : d.g0))));
+    }
+  }
+  V8 s[2] = {par ? v8_zero() : sum, par ? sum : v8_zero()};
   block_sums<2>(s, partials);
 }
 #else
@@ -2209,7 +2340,7 @@ OtfQuery::OtfQuery(DevCtx& c, const std::vector<std::array<E8, 2>>& factors, uin
       terms_(3 * std::max<uint64_t>(max_fast, 1)),
       desc_(3 * std::max<uint64_t>(max_chunks, 1)),
       tlo_(std::max<uint64_t>(max_fast, 1) * WG),
-      thi_(std::max<uint64_t>(max_fast, 1) * OTF_PPT),
+      thi_(std::max<uint64_t>(max_fast, 1) * 16),
       pieces_(4 * std::max<uint64_t>(max_pieces, 1)),
       items_(2 * std::max<uint64_t>(max_items, 1)),
       max_chunks_(max_chunks),
@@ -2221,7 +2352,7 @@ OtfQuery::OtfQuery(DevCtx& c, const std::vector<std::array<E8, 2>>& factors, uin
 
 // Uploads the level's records, builds the chunk records and term tables, and
 // returns the number of partial rows the sparse pass will use.
-uint32_t OtfQuery::prepare(DevCtx& c, const OtfLevel& lv, uint64_t chunks) {
+uint32_t OtfQuery::prepare(DevCtx& c, const OtfLevel& lv, uint64_t chunks, bool mfma_tables) {
   require(chunks <= max_chunks_ && lv.fast.size() <= max_fast_, QF_ERR_ARGUMENT, "on-the-fly query level size");
   if (lv.pieces.size() > max_pieces_) {  // later levels may split more terms than the first
     max_pieces_ = lv.pieces.size();
@@ -2235,10 +2366,14 @@ uint32_t OtfQuery::prepare(DevCtx& c, const OtfLevel& lv, uint64_t chunks) {
   h2d(c, pieces_.p, lv.pieces.data(), lv.pieces.size() * sizeof(OtfPieceDev));
   h2d(c, items_.p, lv.items.data(), lv.items.size() * sizeof(OtfItemDev));
   const uint32_t nt = uint32_t(lv.fast.size());
-  const uint64_t threads = chunks + uint64_t(nt) * (WG + OTF_PPT);
-  hipLaunchKernelGGL(k_otf_desc, dim3(uint32_t((threads + WG - 1) / WG)), dim3(WG), 0, c.st,
-                     reinterpret_cast<const OtfTermDev*>(terms_.p), nt, (const E8(*)[2])f_.p, chunks,
-                     reinterpret_cast<OtfChunk*>(desc_.p), tlo_.p, thi_.p);
+  const uint64_t threads = chunks + uint64_t(nt) * (256 + 16);
+  const dim3 grid(uint32_t((threads + WG - 1) / WG));
+  const auto* td = reinterpret_cast<const OtfTermDev*>(terms_.p);
+  auto* dd = reinterpret_cast<OtfChunk*>(desc_.p);
+  if (mfma_tables)
+    hipLaunchKernelGGL(k_otf_desc<7>, grid, dim3(WG), 0, c.st, td, nt, (const E8(*)[2])f_.p, chunks, dd, tlo_.p, thi_.p);
+  else
+    hipLaunchKernelGGL(k_otf_desc<8>, grid, dim3(WG), 0, c.st, td, nt, (const E8(*)[2])f_.p, chunks, dd, tlo_.p, thi_.p);
   QF_HIP(hipGetLastError());
   if (lv.pieces.empty() && lv.items.empty()) return 0;
   const uint64_t want = std::max<uint64_t>(lv.pieces.size(), (lv.items.size() + WG - 1) / WG);
@@ -2262,10 +2397,36 @@ std::pair<E8, E8> OtfQuery::first_fold(DevCtx& c, const Val* bva, uint64_t avail
   require(s >= 2 && s <= 64 && pow2(uint64_t(s)) && int(pinv2.size()) == 2 * s, QF_ERR_ARGUMENT, "witness stride");
   require(rows % (uint64_t(4) << OTF_LCP) == 0, QF_ERR_ARGUMENT, "witness rows");
   const uint64_t chunks = (rows / 4) >> OTF_LCP;
-  const uint32_t gs = prepare(c, lv, chunks);
+#ifndef QF_KB
+  // rows of 32 to 128 entries run on the matrix cores (QF_PROJECT_MFMA=0: the vector-ALU form)
+  const char* pm = std::getenv("QF_PROJECT_MFMA");
+  const bool mfma = 2 * s >= 32 && !(pm && pm[0] == '0');
+#else
+  const bool mfma = false;
+#endif
+  const uint32_t gs = prepare(c, lv, chunks, mfma);
   E8* t = put_table(c, scratch_tab(), pinv2.data(), pinv2.size() * 16);
   const OtfChunk* d = reinterpret_cast<const OtfChunk*>(desc_.p);
   uint32_t g = 0;
+#ifndef QF_KB
+  if (mfma) {
+#define QF_FIRST_FOLD_MF(S2)                                                                                      \
+  {                                                                                                                \
+    static const uint32_t resident = resident_groups(k_otf_first_fold_mfma<S2>);                                   \
+    g = uint32_t(std::min<uint64_t>({chunks, resident, uint64_t(DevCtx::MAX_GROUPS - gs)}));                        \
+    hipLaunchKernelGGL(k_otf_first_fold_mfma<S2>, dim3(g), dim3(WG), 0, c.st, bva, available, chunks, t, ow, d,    \
+                       tlo_.p, thi_.p, c.partials);                                                                \
+  }
+    switch (2 * s) {
+      case 32: QF_FIRST_FOLD_MF(32); break;
+      case 64: QF_FIRST_FOLD_MF(64); break;
+      default: QF_FIRST_FOLD_MF(128); break;
+    }
+#undef QF_FIRST_FOLD_MF
+    QF_HIP(hipGetLastError());
+    return finish(c, lv, ow, g, gs);
+  }
+#endif
 #define QF_FIRST_FOLD(S2)                                                                                         \
   {                                                                                                                \
     static const uint32_t resident = resident_groups(k_otf_first_fold<S2>);                                        \
@@ -2289,7 +2450,7 @@ std::pair<E8, E8> OtfQuery::first_fold(DevCtx& c, const Val* bva, uint64_t avail
 std::pair<E8, E8> OtfQuery::fold(DevCtx& c, const E8* w, E8* ow, uint64_t h, const E8& ri, const OtfLevel& lv) {
   require(h % (uint64_t(2) << OTF_LCP) == 0, QF_ERR_ARGUMENT, "on-the-fly fold length");
   const uint64_t chunks = (h / 2) >> OTF_LCP;
-  const uint32_t gs = prepare(c, lv, chunks);
+  const uint32_t gs = prepare(c, lv, chunks, false);
   static const uint32_t resident = resident_groups(k_otf_fold);
   const uint32_t g = uint32_t(std::min<uint64_t>({chunks, resident, uint64_t(DevCtx::MAX_GROUPS - gs)}));
   hipLaunchKernelGGL(k_otf_fold, dim3(g), dim3(WG), 0, c.st, w, ow, chunks, ri,

@@ -204,7 +204,7 @@ class OtfQuery {
   std::pair<E8, E8> fold(DevCtx& c, const E8* w, E8* ow, uint64_t h, const E8& ri, const OtfLevel& lv);
 
  private:
-  uint32_t prepare(DevCtx& c, const OtfLevel& lv, uint64_t chunks);
+  uint32_t prepare(DevCtx& c, const OtfLevel& lv, uint64_t chunks, bool mfma_tables);
   std::pair<E8, E8> finish(DevCtx& c, const OtfLevel& lv, const E8* ow, uint32_t groups, uint32_t gs);
   DVec f_, terms_, desc_, tlo_, thi_, pieces_, items_;
   uint64_t max_chunks_;
