@@ -1120,15 +1120,24 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
   V8 s[2] = {v8_zero(), v8_zero()};
   int32_t cur = -1;
   V8 tl = v8_zero();
+  // the next pair's four inputs load while this one computes
+  E8 nx[4];
+  auto fetch = [&](uint64_t ch, int i) {
+    const uint64_t m = (ch << OTF_LCP) + uint64_t(i) * WG + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++) nx[k] = w[4 * m + k];
+  };
+  if (blockIdx.x < chunks) fetch(blockIdx.x, 0);
   for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {  // the workgroups sweep the level together
     const OtfChunk d = desc[ch];
     OtfAcc a0, a1;
     otf_zero(a0);
     otf_zero(a1);
-#pragma unroll 2
     for (int i = 0; i < OTF_PPT; i++) {
       const uint64_t m = (ch << OTF_LCP) + uint64_t(i) * WG + threadIdx.x;
-      const E8 x0 = w[4 * m], x1 = w[4 * m + 1], x2 = w[4 * m + 2], x3 = w[4 * m + 3];
+      const E8 x0 = nx[0], x1 = nx[1], x2 = nx[2], x3 = nx[3];
+      if (i + 1 < OTF_PPT) fetch(ch, i + 1);
+      else if (ch + gridDim.x < chunks) fetch(ch + gridDim.x, 0);
       const V8 y0 = v8_add(v8_load(x0), v8_mul_enc(v8_load(x1), vri));
       const V8 y1 = v8_add(v8_load(x2), v8_mul_enc(v8_load(x3), vri));
       ow[2 * m] = v8_store(y0);
