@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <memory>
 #include <map>
+#include <string>
 #include <tuple>
 
 namespace qf {
@@ -789,9 +790,19 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   int ls = ilog2u(s);
   TraceLaps lap;
   // the lane weights of every term of at least s entries, used by the projection and the query fold
+  // (the buckets of one matrix share their low factors, so equal factor lists share one table)
   std::vector<std::vector<E8>> term_low(q.terms.size());
-  for (size_t i = 0; i < q.terms.size(); i++)
-    if (q.terms[i].len() >= s) term_low[i] = tbv_w(q.terms[i].f.data(), ls);
+  std::vector<size_t> low_of(q.terms.size(), SIZE_MAX);  // the first term with the same low factors
+  {
+    std::map<std::string, size_t> seen;
+    for (size_t i = 0; i < q.terms.size(); i++) {
+      if (q.terms[i].len() < s) continue;
+      const std::string key(reinterpret_cast<const char*>(q.terms[i].f.data()), size_t(ls) * sizeof(q.terms[i].f[0]));
+      auto [it, fresh] = seen.emplace(key, i);
+      low_of[i] = it->second;
+      term_low[i] = fresh ? tbv_w(q.terms[i].f.data(), ls) : term_low[it->second];
+    }
+  }
   std::vector<ProjTask> tasks;
   std::vector<std::array<E8, 2>> pairs, fbit;
   std::vector<E8> lows;
@@ -846,6 +857,10 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
   std::vector<E8> fold_scale(q.terms.size(), E8{});
   for (size_t i = 0; i < q.terms.size(); i++)
     if (q.terms[i].len() >= s) {
+      if (low_of[i] != i) {
+        fold_scale[i] = fold_scale[low_of[i]];
+        continue;
+      }
       E8 acc{};
       for (uint64_t j = 0; j < s; j++) acc += term_low[i][j] * pows[j];
       fold_scale[i] = acc;
