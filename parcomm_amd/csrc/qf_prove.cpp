@@ -241,8 +241,7 @@ class TwoFold {
  private:
   void to_host(uint64_t n) {
     if (host_) return;
-    a_ = dv::download(c_, da_.p, n);
-    b_ = dv::download(c_, db_.p, n);
+    std::tie(a_, b_) = dv::download2(c_, da_.p, db_.p, n);
     host_ = true;
   }
   std::pair<E8, E8> host_cross(uint64_t g) const {
@@ -1051,8 +1050,7 @@ EoOut eo_prove(DevCtx& c, const Layout& l, const Val* bva, uint64_t s, const Que
     }
     level++;
     if (hw.empty() && n <= TwoFold::HOST_TAIL) {
-      hw = dv::download(c, cw->p, n);
-      hq = dv::download(c, cq->p, n);
+      std::tie(hw, hq) = dv::download2(c, cw->p, cq->p, n);
     }
     if (!hw.empty()) {  // w'[i] = w[2i] + w[2i+1] r^-1, q'[i] = q[2i] + q[2i+1] r, then the cross terms
       for (uint64_t i = 0; i < h; i++) {
@@ -1111,8 +1109,7 @@ void sumcheck(DevCtx& c, DVec& v, DVec& w, E8& claim, Packed& t, std::vector<std
     claim = m0 + r * ((claim - m0 - m0 - m1) + r * m1);
     rounds.push_back({m0, m1});
     if (hv.empty() && n <= TwoFold::HOST_TAIL) {
-      hv = dv::download(c, v.p, n);
-      hw = dv::download(c, w.p, n);
+      std::tie(hv, hw) = dv::download2(c, v.p, w.p, n);
     }
     if (!hv.empty()) {  // v' = v_lo + r (v_hi - v_lo); message of the halves of v', w'
       for (uint64_t i = 0; i < h; i++) {

@@ -2011,11 +2011,30 @@ DVec upload(DevCtx& c, const std::vector<E8>& v) {
   h2d(c, d.p, v.data(), v.size() * 16);
   return d;
 }
+// Small downloads land in the thread's pinned buffer past the reduction rows
+// (an asynchronous DMA and one synchronisation), larger ones in pageable memory.
+constexpr uint64_t PIN_DOWN = 65536;                                            // u32 offset
+constexpr uint64_t PIN_DOWN_BYTES = uint64_t(DevCtx::MAX_GROUPS) * 64 * 32 - 4 * PIN_DOWN;  // to the buffer's end
 std::vector<E8> download(DevCtx& c, const E8* p, uint64_t n) {
-  std::vector<E8> v(n);
-  if (n) QF_HIP(hipMemcpyAsync(v.data(), p, n * 16, hipMemcpyDeviceToHost, c.st));
+  return download2(c, p, nullptr, n).first;
+}
+std::pair<std::vector<E8>, std::vector<E8>> download2(DevCtx& c, const E8* a, const E8* b, uint64_t n) {
+  std::vector<E8> va(n), vb(b ? n : 0);
+  const uint64_t bytes = n * 16;
+  if (!n) return {va, vb};
+  if ((b ? 2 : 1) * bytes <= PIN_DOWN_BYTES) {
+    char* pin = reinterpret_cast<char*>(c.host_partials + PIN_DOWN);
+    QF_HIP(hipMemcpyAsync(pin, a, bytes, hipMemcpyDeviceToHost, c.st));
+    if (b) QF_HIP(hipMemcpyAsync(pin + bytes, b, bytes, hipMemcpyDeviceToHost, c.st));
+    c.sync();
+    std::memcpy(va.data(), pin, bytes);
+    if (b) std::memcpy(vb.data(), pin + bytes, bytes);
+    return {va, vb};
+  }
+  QF_HIP(hipMemcpyAsync(va.data(), a, bytes, hipMemcpyDeviceToHost, c.st));
+  if (b) QF_HIP(hipMemcpyAsync(vb.data(), b, bytes, hipMemcpyDeviceToHost, c.st));
   c.sync();
-  return v;
+  return {va, vb};
 }
 void fill(DevCtx& c, E8* p, uint64_t n, const E8& v) { hipLaunchKernelGGL(k_fill, grid_for(n), dim3(WG), 0, c.st, p, n, v); }
 void copy(DevCtx& c, E8* dst, const E8* src, uint64_t n) {

@@ -60,6 +60,8 @@ struct DevCtx {
 namespace dv {
 DVec upload(DevCtx& c, const std::vector<E8>& v);
 std::vector<E8> download(DevCtx& c, const E8* p, uint64_t n);
+// Two device vectors of n elements each (b may be null) with one synchronisation.
+std::pair<std::vector<E8>, std::vector<E8>> download2(DevCtx& c, const E8* a, const E8* b, uint64_t n);
 inline std::vector<E8> download(DevCtx& c, const DVec& v) { return download(c, v.p, v.n); }
 void fill(DevCtx& c, E8* p, uint64_t n, const E8& value);
 void copy(DevCtx& c, E8* dst, const E8* src, uint64_t n);
