@@ -199,11 +199,11 @@ def measure(pa, field, args, per_matrix=True, serve_requests=0, model="8b", step
         from parcomm_amd.serve import serve
         c = params.commit(bva)
         # the offered 50/s of the configuration, then a steady state at 3/4 of the rate it sustained
-        over, _ = serve(params, c, claims, n_requests=serve_requests, rate=50.0, seed=args.seed, workers=4)
+        over, _ = serve(params, c, claims, n_requests=serve_requests, rate=50.0, seed=args.seed, workers=6)
         print(f"# {tag} serving at 50/s offered {over}", file=sys.stderr, flush=True)
         steady_rate = round(0.75 * over.get("openings_per_s", 4.0), 1)
         steady, reqs = serve(params, c, claims, n_requests=serve_requests, rate=steady_rate, seed=args.seed + 1,
-                             workers=4)
+                             workers=6)
         print(f"# {tag} serving at {steady_rate}/s offered {steady}", file=sys.stderr, flush=True)
         # batch verification of the responses on the host's threads (qf_verify_batch)
         items = [(r.claims, r.proof) for r in reqs if r.error is None]

@@ -1276,6 +1276,7 @@ __global__ __launch_bounds__(WG) void k_otf_first_fold_mfma(const Val* __restric
   static_assert(S2 % 32 == 0 && S2 <= 128, "whole 32-entry k blocks");
   constexpr int KB = S2 / 32;
   __shared__ __align__(16) uint16_t tbuf[MF_WAVES][64][8];
+  __shared__ __align__(16) uint32_t tds[16][16];  // the chunk term's thi[it] as fold_seq sequences
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 15, q = lane >> 4, par = lane & 1;
   long bop[KB];
@@ -1345,18 +1346,35 @@ __global__ __launch_bounds__(WG) void k_otf_first_fold_mfma(const Val* __restric
     E8 e;
     *reinterpret_cast<uint4*>(&e) = raw;
     ow[row0 + lane] = e;
-    if (d.term >= 0) {
+    if (d.term >= 0) {  // the 8-bit limbs straight from the packed lane pairs, thi[it]'s sequence from LDS
       uint32_t D[15], l4[4], h4[4];
-      fold_seq(v8_load(thi[uint64_t(d.term) * 16 + it]), D);
-      limb_split(v8_load(e), l4, h4);
+      const uint32_t wv[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+      for (int jp = 0; jp < 4; jp++) {
+        l4[jp] = wv[jp] & 0x00ff00ffu;
+        h4[jp] = (wv[jp] >> 8) & 0x00ff00ffu;
+      }
+#pragma unroll
+      for (int m = 1; m < 15; m++) D[m] = tds[it][m];
       lazy_mac(acc, l4, h4, D);
     }
   };
   // two operand sets: the loads of the next iteration are in flight while one computes
   long a0[4][KB], a1[4][KB];
   if (blockIdx.x < chunks) load_a(blockIdx.x, 0, a0);
+  int32_t seq_term = -1;  // the term whose sequences tds holds
   for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {  // the workgroups sweep the level together
     const OtfChunk d = desc[ch];
+    if (d.term >= 0 && d.term != seq_term) {  // uniform: every wave takes this chunk
+      __syncthreads();
+      if (threadIdx.x < 16) {
+        uint32_t D[15];
+        fold_seq(v8_load(thi[uint64_t(d.term) * 16 + threadIdx.x]), D);
+        for (int m = 1; m < 15; m++) tds[threadIdx.x][m] = D[m];
+      }
+      __syncthreads();
+      seq_term = d.term;
+    }
     LazySum acc;
     lazy_zero(acc);
     for (int it = 0; it < 16; it += 2) {

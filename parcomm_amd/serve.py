@@ -111,7 +111,7 @@ class _Admission:
 def serve(params, committed, base_claims, n_requests=32, rate=50.0, seed=1, workers=None, verify=False,
           memory_fraction=0.9):
     """Runs a Poisson stream of `n_requests` opening requests at `rate` per
-    second with at most `workers` (default 4) in flight; returns (stats,
+    second with at most `workers` (default 6) in flight; returns (stats,
     requests). A request that fails keeps its error message and is counted
     in stats["failed"]; latencies cover the completed requests only."""
     rng = np.random.default_rng(seed)
@@ -120,7 +120,7 @@ def serve(params, committed, base_claims, n_requests=32, rate=50.0, seed=1, work
     reqs = [Request(i, 0.0, sorted(rng.choice(len(base_claims), size=int(k), replace=False).tolist()))
             for i, k in enumerate(sizes)]
     per = request_bytes(params)
-    slots = max(1, workers or 4)
+    slots = max(1, workers or 6)
     adm = _Admission(params.field, slots, per, memory_fraction)
     queue = collections.deque()
     qcv = threading.Condition()
