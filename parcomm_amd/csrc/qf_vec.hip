@@ -1267,8 +1267,9 @@ __global__ __launch_bounds__(WG) void k_otf_first_fold(const Val* __restrict__ b
 //   iterations. Pair-index bits 0..6 (wave, lane / 2) weigh tlo (128 entries),
 //   bits 7..10 (iteration) thi (16 entries); even lanes hold the element 2m of
 //   their pair (c0, with g0), odd lanes 2m + 1 (c1, with g1).
+// (32-entry rows keep five waves per SIMD at a 12-byte spill, which measures faster)
 template <int S2>
-__global__ __launch_bounds__(WG) void k_otf_first_fold_mfma(const Val* __restrict__ bva, uint64_t available,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(S2 == 32 ? 5 : 1))) void k_otf_first_fold_mfma(const Val* __restrict__ bva, uint64_t available,
                                                             uint64_t chunks, const E8* __restrict__ pinv_g,
                                                             E8* __restrict__ ow, const OtfChunk* __restrict__ desc,
                                                             const E8* __restrict__ tlo, const E8* __restrict__ thi,
