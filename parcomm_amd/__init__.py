@@ -391,11 +391,9 @@ class Parameters:
 
     def evaluate_claims(self, committed: "Committed", points) -> List[Claim]:
         """tbv.rs:143-185: claims at the given (row point, col point) pairs, one per matrix."""
-        raw = bytearray()
-        for row, col in points:
-            for v in list(row) + list(col) + [bytes(16)]:
-                raw += v
-        buf = ctypes.create_string_buffer(bytes(raw), len(raw))
+        zero = bytes(16)
+        raw = b"".join(b"".join(row) + b"".join(col) + zero for row, col in points)
+        buf = ctypes.create_string_buffer(raw, len(raw))
         self._check(self._L.qf_evaluate_claims(self._h, committed._h, buf, len(raw)))
         return self.claims_from_bytes(buf.raw)
 

@@ -1504,14 +1504,22 @@ void evaluate_claims(const ParamsState& p, const CommittedState& cs, std::vector
   }
   std::vector<size_t> which;
   auto proj = project_terms(c, q, cs.d_bva, p.layout.active, 16, which);
+  // a term's value is sum_i proj[i] low[i]; the buckets of one matrix share
+  // their low factors, so their lane sums are added first and weighted once
   std::vector<bool> done(q.terms.size(), false);
+  std::map<std::pair<size_t, std::string>, std::pair<size_t, std::array<E8, 16>>> groups;  // -> (a term, lane sums)
   for (size_t k = 0; k < which.size(); k++) {
     const Term& tm = q.terms[which[k]];
-    auto low = tbv_w(tm.f.data(), 4);
-    E8 v{};
-    for (int i = 0; i < 16; i++) v += proj[k][i] * low[i];
-    claims[p.layout.buckets[which[k]].matrix].value += v;
+    const std::string key(reinterpret_cast<const char*>(tm.f.data()), 4 * sizeof(tm.f[0]));
+    auto [it, fresh] = groups.try_emplace({p.layout.buckets[which[k]].matrix, key}, which[k], std::array<E8, 16>{});
+    for (int i = 0; i < 16; i++) it->second.second[i] += proj[k][i];
     done[which[k]] = true;
+  }
+  for (auto& [mk, g] : groups) {
+    const auto low = tbv_w(q.terms[g.first].f.data(), 4);
+    E8 v{};
+    for (int i = 0; i < 16; i++) v += g.second[i] * low[i];
+    claims[mk.first].value += v;
   }
   for (size_t i = 0; i < q.terms.size(); i++) {
     const Term& tm = q.terms[i];

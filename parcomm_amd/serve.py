@@ -43,13 +43,6 @@ def request_bytes(params):
     return eo + folded + 2 * n_values * 16
 
 
-def _element(rng, field):
-    """A uniform extension element in its 16-byte canonical form."""
-    info = field_info(field)
-    c = rng.integers(0, info["p"], size=info["lanes"], dtype=np.uint64)
-    return c.astype(info["coef"]).tobytes()
-
-
 @dataclass
 class Request:
     rid: int
@@ -63,12 +56,20 @@ class Request:
 
 
 def request_claims(params, committed, base_claims, subset, rng):
-    """Claims of one request: fresh points on `subset`, benchmark points elsewhere."""
-    points = []
+    """Claims of one request: fresh points on `subset`, benchmark points
+    elsewhere. The fresh coordinates come from one draw of uniform
+    coefficients (16-byte canonical elements)."""
+    info = field_info(params.field)
     chosen = set(subset)
+    total = sum(len(c.row) + len(c.col) for m, c in enumerate(base_claims) if m in chosen)
+    raw = rng.integers(0, info["p"], size=(total, info["lanes"]), dtype=np.uint64).astype(info["coef"]).tobytes()
+    elems = [raw[16 * i:16 * i + 16] for i in range(total)]
+    points, k = [], 0
     for m, c in enumerate(base_claims):
         if m in chosen:
-            points.append(([_element(rng, params.field) for _ in c.row], [_element(rng, params.field) for _ in c.col]))
+            nr, nc = len(c.row), len(c.col)
+            points.append((elems[k:k + nr], elems[k + nr:k + nr + nc]))
+            k += nr + nc
         else:
             points.append((list(c.row), list(c.col)))
     return params.evaluate_claims(committed, points)
