@@ -1574,6 +1574,91 @@ __global__ __launch_bounds__(WG) void k_proj_cross(const E8* __restrict__ parts,
 
 // values[i] = sum_b fw[b] * packed(b, i); `available` and `block_bytes` in bytes.
 #ifndef QF_KB
+// The F65521 block fold on the matrix cores: one exact i8 contraction over
+// k = 8 b + j (block b, byte j of the element).
+// - fw[b] * packed(b, i) = M_b v with v the element's 8 signed bytes and M_b
+//   the matrix of multiplication by fw[b] (column j = fw[b] X^j), so
+//   values[i][c] = sum_{b, j} M_b[c][j] v_{b,j}.
+// - Digits: M_b[c][j] as t' = t or t - p in [-32881, 32639], t' = 256 h + l,
+//   h and l in [-128, 127], built on the host. B column n < 8 holds the low
+//   digits of coefficient n, column 8 + n the high digits; step s covers
+//   blocks 4 s .. 4 s + 3 (lane n + 16 q: block 4 s + q), zero past the last.
+// - v_mfma_i32_16x16x32_i8 with A[m][k] = byte k of position m (lane i of
+//   16-lane group q: position i, block 4 s + q, one 8-byte load), D[m][n] in
+//   lane n + 16 (m / 4), element m % 4; a step adds at most 32 x 2^14 = 2^19
+//   and there are at most 32 steps, so |D| <= 2^24 and the high digit sum is
+//   reduced before it is scaled by 256.
+// - A wave takes 64 positions per iteration (four groups of 16), joins the
+//   digits across lanes n and n ^ 8, transposes through LDS and stores one
+//   16-byte element per lane.
+// Bytes at or past `available` read as zero.
+__global__ __launch_bounds__(WG) void k_block_fold_mfma(const uint8_t* __restrict__ bva, uint64_t available,
+                                                        uint64_t block_bytes, uint32_t steps,
+                                                        const long* __restrict__ bops, E8* __restrict__ values,
+                                                        uint64_t n) {
+  __shared__ long lbop[32][64];
+  __shared__ __align__(16) uint16_t tbuf[MF_WAVES][64][8];
+  for (uint32_t i = threadIdx.x; i < steps * 64; i += WG) lbop[i / 64][i % 64] = bops[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nn = lane & 15, q = lane >> 4;
+  for (uint64_t base = uint64_t(blockIdx.x) * WG; base < n; base += uint64_t(gridDim.x) * WG) {
+    const uint64_t pos0 = base + uint64_t(wave) * 64;  // this wave's 64 positions
+    qf_i32x4 dsum[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) dsum[g] = qf_i32x4{0, 0, 0, 0};
+    for (uint32_t s0 = 0; s0 < steps; s0 += 2) {  // two steps' loads in flight together
+      long a[2][4];
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const uint64_t pos = pos0 + 16 * g + nn;
+          const uint64_t at = uint64_t(4 * (s0 + h) + q) * block_bytes + 8 * pos;
+          uint64_t x = 0;
+          if (s0 + h < steps && pos < n) {
+            if (at + 8 <= available) {
+              x = *reinterpret_cast<const uint64_t*>(bva + at);
+            } else if (at < available) {
+              for (int j = 0; j < 8; j++)
+                if (at + j < available) x |= uint64_t(bva[at + j]) << (8 * j);
+            }
+          }
+          a[h][g] = long(x);
+        }
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (s0 + h >= steps) break;
+        const long bop = lbop[s0 + h][lane];
+#pragma unroll
+        for (int g = 0; g < 4; g++) dsum[g] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a[h][g], bop, dsum[g], 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the previous iteration's reads of tbuf are done
+    // lanes nn < 8 join rows 4q, 4q + 1 of each group, lanes nn >= 8 rows 4q + 2, 4q + 3
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int32_t own = dsum[g][j], other = __shfl_xor(own, 8, 64);
+        if ((j < 2) == (nn < 8)) {
+          const int32_t lo = nn < 8 ? own : other, hi = nn < 8 ? other : own;
+          const uint32_t h = fold16(fold16(uint32_t(hi + 16384 * int32_t(P))));  // < 2^16 + 135
+          const uint32_t x = fold16(fold16(uint32_t(lo + 16384 * int32_t(P)) + 256u * h));
+          const uint32_t y = x - P;
+          tbuf[wave][16 * g + 4 * q + j][nn & 7] = uint16_t(x < y ? x : y);
+        }
+      }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const uint4 raw = *reinterpret_cast<const uint4*>(&tbuf[wave][lane][0]);
+    if (pos0 + lane < n) {
+      E8 e;
+      *reinterpret_cast<uint4*>(&e) = raw;
+      values[pos0 + lane] = e;
+    }
+  }
+}
+
 // F65521 form. packed(b, i) has the 8 signed bytes v_j of the element as
 // coefficients; with u_j = v_j + 128 (one xor per four bytes) each is a single
 // 8-bit limb of the v8_mul_dot2 product, so fw[b] u costs 32 v_dot2_u32_u16
@@ -2655,9 +2740,46 @@ void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& ta
   fill_tasks<false>(c, out, tasks, pairs);
 }
 
+#ifndef QF_KB
+// QF_BLOCK_FOLD_MFMA=0 keeps the F65521 block fold on the vector ALUs.
+bool block_fold_mfma_on() {
+  const char* e = std::getenv("QF_BLOCK_FOLD_MFMA");
+  return !(e && e[0] == '0');
+}
+#endif
+
 void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
                 const std::vector<E8>& fw, E8* values, uint64_t n) {
   require(blocks <= 128 && fw.size() == blocks, QF_ERR_ARGUMENT, "block fold weights");
+#ifndef QF_KB
+  if (block_fold_mfma_on()) {
+    // B operands per step and lane (n + 16 q): digits of M_b[c][j], b = 4 s + q
+    const uint32_t steps = uint32_t((blocks + 3) / 4);
+    std::vector<uint64_t> bops(size_t(steps) * 64, 0);
+    for (uint64_t b = 0; b < blocks; b++) {
+      E8 col[8];
+      for (int j = 0; j < 8; j++) col[j] = fw[b] * e8_basis(j);
+      for (int nn = 0; nn < 16; nn++) {
+        uint64_t x = 0;
+        for (int j = 0; j < 8; j++) {
+          const int32_t t = int32_t(col[j].c[nn & 7]);
+          const int32_t tv = t <= 32639 ? t : t - int32_t(P);
+          const int32_t hd = (tv + 128) >> 8, ld = tv - 256 * hd;
+          x |= uint64_t(uint8_t(int8_t(nn < 8 ? ld : hd))) << (8 * j);
+        }
+        bops[(b / 4) * 64 + 16 * (b % 4) + nn] = x;
+      }
+    }
+    const long* t = reinterpret_cast<const long*>(put_table(c, scratch_tab(), bops.data(), bops.size() * 8));
+    static const uint32_t resident = resident_groups(k_block_fold_mfma);
+    const uint64_t g = std::min<uint64_t>(resident, (n + WG - 1) / WG);
+    hipLaunchKernelGGL(k_block_fold_mfma, dim3(uint32_t(std::max<uint64_t>(g, 1))), dim3(WG), 0, c.st,
+                       reinterpret_cast<const uint8_t*>(bva), available * sizeof(Val), block_len * sizeof(Val), steps,
+                       t, values, n);
+    QF_HIP(hipGetLastError());
+    return;
+  }
+#endif
   E8* t = put_table(c, scratch_tab(), fw.data(), fw.size() * 16);
   hipLaunchKernelGGL(k_block_fold, grid_for(n), dim3(WG), 0, c.st, reinterpret_cast<const uint8_t*>(bva),
                      available * sizeof(Val), block_len * sizeof(Val), uint32_t(blocks), t, values, n);
