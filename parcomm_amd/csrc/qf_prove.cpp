@@ -6,13 +6,18 @@
 #include "qf_rounds.hpp"
 #include "qf_vec.hpp"
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <exception>
+#include <functional>
 #include <memory>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 
 namespace qf {
@@ -92,8 +97,7 @@ void eq_tables_device(DevCtx& c, const std::vector<E8>& p, DVec& hi, DVec& lo) {
 }
 Digest download_root(DevCtx& c, const uint32_t* tree, uint64_t R) {
   Digest d;
-  QF_HIP(hipMemcpyAsync(d.data(), tree + 8 * (2 * R - 2), 32, hipMemcpyDeviceToHost, c.st));
-  c.sync();
+  dv::d2h(c, d.data(), tree + 8 * (2 * R - 2), 32);
   return d;
 }
 
@@ -319,8 +323,7 @@ void write_rows(DevCtx& c, Bytes& out, const DCode& d, bool all, const std::vect
     put_u64(out, d.R * d.cols);
     const size_t at = out.size();  // the column-major codeword, element bytes as stored
     out.resize(at + d.R * d.cols * sizeof(E8));
-    QF_HIP(hipMemcpyAsync(out.data() + at, d.data(), d.R * d.cols * sizeof(E8), hipMemcpyDeviceToHost, c.st));
-    c.sync();
+    dv::d2h(c, out.data() + at, d.data(), d.R * d.cols * sizeof(E8));
     return;
   }
   out.push_back(0);
@@ -735,8 +738,7 @@ std::vector<Val> read_values(DevCtx& c, const Val* bva, uint64_t active, uint64_
   std::vector<Val> out(n, 0);
   if (start < active) {
     uint64_t m = std::min(n, active - start);
-    QF_HIP(hipMemcpyAsync(out.data(), bva + start, m * sizeof(Val), hipMemcpyDeviceToHost, c.st));
-    c.sync();
+    dv::d2h(c, out.data(), bva + start, m * sizeof(Val));
   }
   return out;
 }
@@ -1161,6 +1163,82 @@ Lanes conjugates(const Lanes& lanes) {
 struct Timing { ProveTimings* t; };
 
 // Codeword of block b: the resident cache, or a fresh commit in recompute mode.
+// A persistent helper thread per prover thread, with its own stream and pinned
+// buffer (the DevCtx it makes for each task), running one task at a time.
+class SideThread {
+ public:
+  using Task = std::function<void(DevCtx&)>;
+  SideThread() : th_([this] { loop(); }) {}
+  ~SideThread() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    th_.join();
+  }
+  void start(int device, Task t) {
+    std::lock_guard<std::mutex> g(mu_);
+    device_ = device;
+    task_ = std::move(t);
+    err_ = nullptr;
+    busy_ = true;
+    cv_.notify_all();
+  }
+  // Waits for the task; rethrows its error.
+  void wait() {
+    std::unique_lock<std::mutex> g(mu_);
+    cv_.wait(g, [this] { return !busy_; });
+    if (err_) {
+      std::exception_ptr e = err_;
+      err_ = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+
+ private:
+  void loop() {
+    std::unique_lock<std::mutex> g(mu_);
+    for (;;) {
+      cv_.wait(g, [this] { return stop_ || (busy_ && task_); });
+      if (stop_) return;
+      Task t = std::move(task_);
+      task_ = nullptr;
+      const int dev = device_;
+      g.unlock();
+      std::exception_ptr e;
+      try {
+        QF_HIP(hipSetDevice(dev));
+        DevCtx c;
+        t(c);
+      } catch (...) {
+        e = std::current_exception();
+      }
+      g.lock();
+      err_ = e;
+      busy_ = false;
+      cv_.notify_all();
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_;
+  Task task_;
+  std::exception_ptr err_;
+  int device_ = 0;
+  bool busy_ = false, stop_ = false;
+  std::thread th_;  // last: it starts after the members it uses
+};
+SideThread& side_thread() {
+  thread_local SideThread s;
+  return s;
+}
+// QF_SIDE_BINDING=0 writes the binding rows on the prover's thread before the
+// native opening.
+bool side_binding_on() {
+  const char* e = std::getenv("QF_SIDE_BINDING");
+  return !(e && e[0] == '0');
+}
+
 DCode block_code(DevCtx& c, const ParamsState& p, const CommittedState& cs, uint64_t b, DCode& workspace) {
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode d;
@@ -1332,27 +1410,51 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   put_e8(w, fval);
   const size_t bind_len_at = w.size();
   put_u64(w, 0);
-  const size_t bind_at = w.size();
-  put_tag(w, "QAFOLDB1");
-  put_u64(w, blocks);
-  DCode workspace;
-  size_t prev_at = 0, prev_len = 0;
-  for (uint64_t b = 0; b < blocks; b++) {
-    if (b == 0 || cs.roots[b] != cs.roots[b - 1]) {
-      DCode bc = block_code(c, p, cs, b, workspace);
-      prev_at = w.size();
-      write_rows(c, w, bc, all, sel);
-      prev_len = w.size() - prev_at;
-    } else {  // the same codeword as the previous block: its rows again
-      const Bytes again(w.begin() + prev_at, w.begin() + prev_at + prev_len);
-      prev_at = w.size();
-      w.insert(w.end(), again.begin(), again.end());
+  // The binding rows (queried rows and Merkle paths of every block and of the
+  // folded block) depend on the transcript only through the row selection, and
+  // the native opening and Decide only on the sumcheck point and the folded
+  // code, so the binding runs on the thread's helper beside them: it appends
+  // to w while this thread writes `opening` and `decide`, and w is not touched
+  // here until the helper is done. Both only read the resident codewords.
+  auto binding = [&](DevCtx& bx) {
+    const auto tb = Clock::now();
+    const size_t bind_at = w.size();
+    put_tag(w, "QAFOLDB1");
+    put_u64(w, blocks);
+    DCode workspace;
+    size_t prev_at = 0, prev_len = 0;
+    for (uint64_t b = 0; b < blocks; b++) {
+      if (b == 0 || cs.roots[b] != cs.roots[b - 1]) {
+        DCode bc = block_code(bx, p, cs, b, workspace);
+        prev_at = w.size();
+        write_rows(bx, w, bc, all, sel);
+        prev_len = w.size() - prev_at;
+      } else {  // the same codeword as the previous block: its rows again
+        const Bytes again(w.begin() + prev_at, w.begin() + prev_at + prev_len);
+        prev_at = w.size();
+        w.insert(w.end(), again.begin(), again.end());
+      }
     }
+    write_rows(bx, w, folded, all, sel);
+    patch_u64(w, bind_len_at, w.size() - bind_at);
+    tm.binding_ms = ms_since(tb);
+    trace_phase("binding", tm.binding_ms);
+  };
+  SideThread* helper = side_binding_on() ? &side_thread() : nullptr;
+  struct Join {  // on any exit the helper has finished with w and the codes first
+    SideThread* s;
+    ~Join() {
+      if (s) try { s->wait(); } catch (...) {}
+    }
+  } join{nullptr};
+  if (helper) {
+    int dev = 0;
+    QF_HIP(hipGetDevice(&dev));
+    helper->start(dev, binding);
+    join.s = helper;
+  } else {
+    binding(c);
   }
-  write_rows(c, w, folded, all, sel);
-  patch_u64(w, bind_len_at, w.size() - bind_at);
-  tm.binding_ms = ms_since(t0);
-  trace_phase("binding", tm.binding_ms);
   t0 = Clock::now();
   Bytes opening;
   std::vector<Pending> pending;
@@ -1364,6 +1466,10 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   Bytes decide = prove_decide(c, p, pending);
   tm.decide_ms = ms_since(t0);
   trace_phase("decide", tm.decide_ms);
+  if (helper) {
+    join.s = nullptr;
+    helper->wait();  // rethrows an error of the binding
+  }
   put_part(w, opening);
   put_part(w, decide);
 }

@@ -59,6 +59,8 @@ struct DevCtx {
 
 namespace dv {
 DVec upload(DevCtx& c, const std::vector<E8>& v);
+// Device-to-host copy of `bytes` through the thread's pinned buffer (synchronous).
+void d2h(DevCtx& c, void* dst, const void* src, size_t bytes);
 std::vector<E8> download(DevCtx& c, const E8* p, uint64_t n);
 // Two device vectors of n elements each (b may be null) with one synchronisation.
 std::pair<std::vector<E8>, std::vector<E8>> download2(DevCtx& c, const E8* a, const E8* b, uint64_t n);
