@@ -263,7 +263,10 @@ def roofline_of(pa, res, field_tag):
             "valu_frac": round(leaf_ops / (leaf_ms * 1e-3) / VALU_PEAK_OPS, 4),
             "traffic": measured_traffic(field_tag, "leaf")}
     mw = {"kernel": mw_family + ", Mobius + WHT of the last row group", "algorithmic_bytes": mw_bytes,
-          "avg_launch_ms": round(mw_ms, 4), "traffic": measured_traffic(field_tag, "mw")}
+          "avg_launch_ms": round(mw_ms, 4),
+          "hbm_GB_per_s": round(mw_bytes / (mw_ms * 1e-3) / 1e9, 1),
+          "hbm_frac": round(mw_bytes / (mw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+          "traffic": measured_traffic(field_tag, "mw")}
     if leaf_ms > mw_ms:
         kernel, nbytes, ms, traffic = leaf["kernel"], leaf_bytes, leaf_ms, leaf["traffic"]
     else:

@@ -13,13 +13,13 @@
 // lanes are signed and unreduced (12 stages grow a canonical value by at most
 // 2^12, so |x| < 2^28) and are reduced to canonical u16 only before they are
 // stored or multiplied; KoalaBear lanes (p ~ 2^31) stay canonical through
-// modular butterflies. Workgroups are persistent. Each walks a contiguous
-// range of tiles, except in passes whose tile rows are 64 bytes (4 contiguous
-// positions): there the two tiles that share every 128-byte line are taken
-// at the same time by sibling workgroups on one XCD (workgroups are dealt to
-// the 8 XCDs round robin), so the second half of each line is still in that
-// XCD's L2 when its sibling asks for it; a line read by one tile long after
-// its partner has been evicted and is fetched from HBM twice.
+// modular butterflies. A workgroup takes one tile (or, persistent, walks a
+// contiguous range of tiles), except that in passes whose tile rows are 64
+// bytes (4 contiguous positions) the two tiles that share every 128-byte line
+// are taken at the same time by sibling workgroups on one XCD (workgroups are
+// dealt to the 8 XCDs round robin), so the second half of each line is still
+// in that XCD's L2 when its sibling asks for it; a line read by one tile long
+// after its partner has been evicted is fetched from HBM twice.
 #include "qf_kernels.hpp"
 #include <cstdio>
 #include <cstdlib>
@@ -596,8 +596,9 @@ __device__ inline void btransform(int32_t (&x)[TQ][LANES], uint2 (*lds)[BPAD]) {
   bexchange<BMAP_D, BMAP_A>(x, lds);
 }
 
-// Persistent workgroups (one per CU) take tiles blockIdx, blockIdx + grid, ...:
-// the tiles in flight at one time are neighbours along the contiguous bits.
+// Workgroups take tiles blockIdx, blockIdx + grid, ... (one tile each unless
+// persistent, one per CU): the tiles in flight at one time are neighbours
+// along the contiguous bits.
 template <int MODE>
 __global__ __launch_bounds__(BWG) void k_btile(TileArgs a) {
   static_assert(MODE == TM_M || MODE == TM_M_W || MODE == TM_W, "big tiles run M, M_W and W");
@@ -1070,8 +1071,9 @@ int device_cus() {
 
 namespace {
 // Launch shape of a tile pass: kernel variant (lean: at most 128 registers, no
-// prefetch; wide: prefetches the next tile), persistent workgroups per CU, and
-// the XCD-paired walk of 64-byte-row passes.
+// prefetch; wide: prefetches the next tile of a persistent workgroup),
+// workgroups per CU when persistent, and the XCD-paired walk of 64-byte-row
+// passes.
 struct TileShape {
   bool wide;
   int per_cu;
@@ -1094,10 +1096,29 @@ TileArgs big_args(TileArgs a) {
   return a;
 }
 
+// F65521 launches one tile per workgroup: the dispatcher then mixes
+// workgroups of the commit's other stream (the leaf hash of the previous
+// block) into the CUs between tiles (8B commit 79.3 -> 78.0 ms, folded block
+// 0.3 ms faster). KoalaBear keeps persistent workgroups that each walk a range
+// of tiles (sh.per_cu per CU; 8B commit 154 against 157 ms one tile each).
+// QF_TILE_PERSISTENT=1 or 0 picks either form in both fields.
+bool tile_persistent() {
+  static const bool on = [] {
+    const char* e = std::getenv("QF_TILE_PERSISTENT");
+#ifdef QF_KB
+    return !(e && e[0] == '0');
+#else
+    return e && e[0] == '1';
+#endif
+  }();
+  return on;
+}
+
 void launch_shape(int mode, bool pack, TileArgs a, const TileShape& sh, hipStream_t st) {
+  const bool persistent = tile_persistent();
   if (sh.big) {
     a = big_args(a);
-    const uint32_t grid = uint32_t(std::min<uint64_t>(a.tiles, uint64_t(device_cus())));
+    const uint32_t grid = persistent ? uint32_t(std::min<uint64_t>(a.tiles, uint64_t(device_cus()))) : a.tiles;
     dim3 g{grid, 1, 1}, b{uint32_t(BWG), 1, 1};
     switch (mode) {
       case TM_M: hipLaunchKernelGGL(k_btile<TM_M>, g, b, 0, st, a); break;
@@ -1116,6 +1137,15 @@ void launch_shape(int mode, bool pack, TileArgs a, const TileShape& sh, hipStrea
   if (sh.xcd && xcd_applicable(a, wgs)) {
     a.xcd_span = uint32_t(((tiles + 7) / 8 + 1) & ~uint64_t(1));
     grid = uint32_t(wgs);
+  }
+  if (!persistent) {  // one tile per workgroup (with the same XCD pairing)
+    a.per_wg = 1;
+    grid = uint32_t(tiles);
+    a.xcd_span = 0;
+    if (sh.xcd && a.logC == 2 && a.log_lo_chunks >= 1) {
+      a.xcd_span = uint32_t(((tiles + 7) / 8 + 1) & ~uint64_t(1));
+      grid = 8 * a.xcd_span;
+    }
   }
   dim3 g{grid, 1, 1}, b{uint32_t(TWG), 1, 1};
   // compile-time stage ranges for the full-width tiles ([logC, 12)), runtime otherwise
