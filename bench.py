@@ -327,9 +327,11 @@ def autotune_summary(pa, field):
         times = [t["ms"] for t in e["tried"]]  # empty when the pick came from QF_TUNE_FILE
         out.append({"pass": f"mode {e['mode']} bits {e['s']}+{e['t']} of L={e['L']} x{e['vectors']}"
                             + (" packed" if e["pack"] else ""),
-                    "kept": "8192-element tile 1/CU" if b.get("big") else
-                            ("quad " if b.get("quad") else "radix ") + ("wide" if b["wide"] else "lean")
-                            + f" {b['per_cu']}/CU" + ("" if b["xcd"] else " no-xcd"),
+                    "kept": ("8192-element tile" if b.get("big") else
+                             ("quad" if b.get("quad") else "radix")
+                             + ("" if b.get("one") else (" wide" if b["wide"] else " lean")))
+                            + (" 1 tile/workgroup" if b.get("one") else (" 1/CU" if b.get("big") else f" {b['per_cu']}/CU"))
+                            + ("" if b["xcd"] or b.get("big") else " no-xcd"),
                     "ms": round(min(times), 4) if times else None,
                     "slowest_ms": round(max(times), 4) if times else None})
     return out

@@ -1080,6 +1080,7 @@ struct TileShape {
   bool xcd;
   bool quad = false;  // the quad-split kernels (k_qtile) instead of the register-radix ones (k_tile)
   bool big = false;   // the 8192-element tiles (k_btile), one workgroup per CU
+  bool one = false;   // one tile per workgroup instead of persistent workgroups (wide, per_cu unused)
 };
 
 bool xcd_applicable(const TileArgs& a, uint64_t wgs) {
@@ -1096,12 +1097,13 @@ TileArgs big_args(TileArgs a) {
   return a;
 }
 
-// F65521 launches one tile per workgroup: the dispatcher then mixes
-// workgroups of the commit's other stream (the leaf hash of the previous
+// F65521's built-in shapes launch one tile per workgroup: the dispatcher then
+// mixes workgroups of the commit's other stream (the leaf hash of the previous
 // block) into the CUs between tiles (8B commit 79.3 -> 78.0 ms, folded block
-// 0.3 ms faster). KoalaBear keeps persistent workgroups that each walk a range
-// of tiles (sh.per_cu per CU; 8B commit 154 against 157 ms one tile each).
-// QF_TILE_PERSISTENT=1 or 0 picks either form in both fields.
+// 0.3 ms faster). KoalaBear's keep persistent workgroups that each walk a
+// range of tiles (sh.per_cu per CU; 8B commit 154 against 157 ms one tile
+// each). QF_TILE_PERSISTENT=1 or 0 picks either form in both fields; the
+// autotuner tries both.
 bool tile_persistent() {
   static const bool on = [] {
     const char* e = std::getenv("QF_TILE_PERSISTENT");
@@ -1115,7 +1117,7 @@ bool tile_persistent() {
 }
 
 void launch_shape(int mode, bool pack, TileArgs a, const TileShape& sh, hipStream_t st) {
-  const bool persistent = tile_persistent();
+  const bool persistent = !sh.one;
   if (sh.big) {
     a = big_args(a);
     const uint32_t grid = persistent ? uint32_t(std::min<uint64_t>(a.tiles, uint64_t(device_cus()))) : a.tiles;
@@ -1203,11 +1205,27 @@ bool btile_on() {
 // form and keeps the 4096-element default unless the autotuner finds the big
 // tile faster), else the quad-split kernels when they apply (their per-thread
 // offsets fit 32 bits), else the register-radix ones.
-TileShape default_shape(int mode, bool quad_ok, bool big_ok) {
+TileShape builtin_shape(int mode, bool quad_ok, bool big_ok) {
   if (big_ok && btile_on() && (mode == TM_M_W || std::getenv("QF_BTILE"))) return {false, 1, false, false, true};
   if (quad_ok && qtile_on()) return {false, 2, true, true};  // quad_ok: a compile-time bit range
   const bool w = wide_mode(mode);
   return {w, w ? 1 : 2, true, false};
+}
+TileShape default_shape(int mode, bool quad_ok, bool big_ok) {
+#ifndef QF_KB
+  // F65521's WHT . alpha . WHT pass, bound by its vector ALU work, keeps two
+  // persistent register-radix workgroups per CU beside the leaf hash (8B
+  // commit 75.5 against 76.3 ms with the one-tile quad-split form)
+  if ((mode == TM_WAW || mode == TM_COPY_WAW) && !std::getenv("QF_TILE_PERSISTENT"))
+    return {false, 2, true, false, false, false};
+#endif
+  TileShape s = builtin_shape(mode, quad_ok, big_ok);
+  if (!tile_persistent()) {  // one tile per workgroup: no prefetch, no per-CU count
+    s.one = true;
+    s.wide = false;
+    s.per_cu = 1;
+  }
+  return s;
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -1286,12 +1304,18 @@ TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L,
   // Every candidate keeps the XCD-paired walk where it applies: without it a
   // pass over 64-byte tile rows fetches every 128-byte line twice, for no
   // gain in time (the passes it applies to are bound by their VALU and LDS work).
-  std::vector<TileShape> cands = {{false, 2, true}, {false, 1, true}, {true, 1, true}, {true, 2, true}};
+  // Each kernel family persistent (lean or wide, 1 or 2 per CU) and one tile per workgroup.
+  std::vector<TileShape> cands = {{false, 2, true}, {false, 1, true}, {true, 1, true}, {true, 2, true},
+                                  {false, 1, true, false, false, true}};
   if (a0.quad_ok && qtile_on()) {
     cands.push_back({false, 2, true, true});
     cands.push_back({true, 1, true, true});
+    cands.push_back({false, 1, true, true, false, true});
   }
-  if (a0.big_ok && btile_on()) cands.push_back({false, 1, false, false, true});
+  if (a0.big_ok && btile_on()) {
+    cands.push_back({false, 1, false, false, true});
+    cands.push_back({false, 1, false, false, true, true});
+  }
   hipEvent_t e0, e1;
   QF_HIP(hipEventCreate(&e0));
   QF_HIP(hipEventCreate(&e1));
@@ -1313,7 +1337,8 @@ TuneEntry tune(int mode, bool pack, const TileArgs& a0, uint64_t vectors, int L,
     QF_HIP(hipGetLastError());
     out.tried.push_back({sh, t_best});
     if (t_best < best) { best = t_best; out.best = sh; }
-    if (sh.wide == def.wide && sh.per_cu == def.per_cu && sh.xcd == def.xcd && sh.quad == def.quad && sh.big == def.big)
+    if (sh.wide == def.wide && sh.per_cu == def.per_cu && sh.xcd == def.xcd && sh.quad == def.quad &&
+        sh.big == def.big && sh.one == def.one)
       def_ms = t_best;
   }
   if (def_ms > 0 && best > def_ms / 1.05f) out.best = def;
@@ -1334,17 +1359,19 @@ void load_tune_file(std::map<TuneKey, TuneEntry>& tab) {
   FILE* fp = std::fopen(f, "r");
   if (!fp) return;
   TuneKey k{};
-  int pack = 0, xvec = 0, in_place = 0, wide = 0, per_cu = 0, xcd = 0, quad = 0, big = 0;
+  int pack = 0, xvec = 0, in_place = 0, wide = 0, per_cu = 0, xcd = 0, quad = 0, big = 0, one = 0;
   unsigned long long v = 0, iv = 0, ov = 0;
-  while (std::fscanf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d %d", &k.mode, &k.L, &k.s, &k.t, &pack,
-                     &xvec, &v, &iv, &ov, &k.out2_rel, &in_place, &wide, &per_cu, &xcd, &quad, &big) == 16) {
+  while (std::fscanf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d %d %d", &k.mode, &k.L, &k.s, &k.t,
+                     &pack, &xvec, &v, &iv, &ov, &k.out2_rel, &in_place, &wide, &per_cu, &xcd, &quad, &big,
+                     &one) == 17) {
     k.pack = pack;
     k.xvec = xvec;
     k.in_place = in_place;
     k.vectors = v;
     k.in_vs = iv;
     k.out_vs = ov;
-    if (per_cu == 1 || per_cu == 2) tab[k] = TuneEntry{{wide != 0, per_cu, xcd != 0, quad != 0, big != 0}, {}};
+    if (per_cu == 1 || per_cu == 2)
+      tab[k] = TuneEntry{{wide != 0, per_cu, xcd != 0, quad != 0, big != 0, one != 0}, {}};
   }
   std::fclose(fp);
 }
@@ -1354,10 +1381,10 @@ void save_tune_file(const std::map<TuneKey, TuneEntry>& tab) {
   FILE* fp = std::fopen(f, "w");
   if (!fp) return;
   for (auto& [k, e] : tab)
-    std::fprintf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d %d\n", k.mode, k.L, k.s, k.t, int(k.pack),
-                 int(k.xvec), (unsigned long long)k.vectors, (unsigned long long)k.in_vs,
+    std::fprintf(fp, "%d %d %d %d %d %d %llu %llu %llu %d %d %d %d %d %d %d %d\n", k.mode, k.L, k.s, k.t,
+                 int(k.pack), int(k.xvec), (unsigned long long)k.vectors, (unsigned long long)k.in_vs,
                  (unsigned long long)k.out_vs, k.out2_rel, int(k.in_place), int(e.best.wide), e.best.per_cu,
-                 int(e.best.xcd), int(e.best.quad), int(e.best.big));
+                 int(e.best.xcd), int(e.best.quad), int(e.best.big), int(e.best.one));
   std::fclose(fp);
 }
 
@@ -1490,7 +1517,7 @@ std::string autotune_report() {
   auto shape = [](const TileShape& s) {
     return std::string("{\"wide\":") + (s.wide ? "true" : "false") + ",\"per_cu\":" + std::to_string(s.per_cu) +
            ",\"xcd\":" + (s.xcd ? "true" : "false") + ",\"quad\":" + (s.quad ? "true" : "false") +
-           ",\"big\":" + (s.big ? "true" : "false") + "}";
+           ",\"big\":" + (s.big ? "true" : "false") + ",\"one\":" + (s.one ? "true" : "false") + "}";
   };
   bool first = true;
   for (auto& [k, e] : tune_table()) {
