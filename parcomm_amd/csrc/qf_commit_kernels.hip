@@ -209,12 +209,14 @@ __device__ inline void leaf_window_block(uint32_t (&W)[20], uint32_t (&h)[8], co
   b2s_compress(h, m, last ? uint32_t(TOTAL) : uint32_t(64 * (b + 1)), last);
 }
 
+// Rows [first, first + count) of a codeword whose columns are `rows` apart.
 template <int PLEN>
-__global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uint64_t rows, LeafPrefix lp,
-                                                  uint32_t* __restrict__ leaves) {
+__global__ __launch_bounds__(WG) void k_leaf_hash(const E8* __restrict__ cw, uint64_t rows, uint64_t first,
+                                                  uint64_t count, LeafPrefix lp, uint32_t* __restrict__ leaves) {
   constexpr int BLOCKS = (PLEN + 512 + 63) / 64;
   static_assert(BLOCKS == 10 && (16 * 1 - (PLEN + (4 - PLEN % 4) % 4) / 4) < 0, "prefix ends inside block 1");
-  for (uint64_t r = uint64_t(blockIdx.x) * WG + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * WG) {
+  for (uint64_t r = first + uint64_t(blockIdx.x) * WG + threadIdx.x; r < first + count;
+       r += uint64_t(gridDim.x) * WG) {
     const uint4* col = reinterpret_cast<const uint4*>(cw) + r;  // column k of this row at col[k * rows]
     uint32_t h[8];
 #pragma unroll
@@ -401,14 +403,17 @@ static bool tiled_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, h
 // 0..6), then the Mobius groups of mobius_plan, the last of which also starts
 // the parity WHT.
 // mw (optional, 2 events): recorded around that last Mobius+WHT pass.
+// msg_ready (optional): recorded once the message halves are final, before the parity passes.
 static bool tiled_commit_rows(bool pack, const void* src, uint64_t block_first, uint64_t available, E8* cw, int L,
-                              const E8* alpha, hipStream_t st, hipEvent_t* mw = nullptr) {
+                              const E8* alpha, hipStream_t st, hipEvent_t* mw = nullptr,
+                              hipEvent_t msg_ready = nullptr) {
   if (L < 12) return false;
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
   if (!tile_pass(TP_M, pack, src, cw, nullptr, nullptr, 32, n, R, L, 0, 7, block_first, available, st, true))
     return false;
   if (L == 12) {  // message complete after row bits 7..11; one pass gives the parity
     must(tile_pass(TP_M, false, cw, cw, nullptr, nullptr, 32, R, R, L, 7, 5, 0, 0, st));
+    if (msg_ready) QF_HIP(hipEventRecord(msg_ready, st));
     must(tile_pass(TP_WAW, false, cw, cw + n, nullptr, alpha, 32, R, R, L, 0, 12, 0, 0, st));
     return true;
   }
@@ -420,6 +425,7 @@ static bool tiled_commit_rows(bool pack, const void* src, uint64_t block_first, 
   if (mw) QF_HIP(hipEventRecord(mw[0], st));
   must(tile_pass(TP_M_W, false, cw, cw, cw + n, nullptr, 32, R, R, L, mg.back().first, mg.back().second, 0, 0, st));
   if (mw) QF_HIP(hipEventRecord(mw[1], st));
+  if (msg_ready) QF_HIP(hipEventRecord(msg_ready, st));
   for (auto& g : plan.second)
     must(tile_pass(TP_W, false, cw + n, cw + n, nullptr, nullptr, 32, R, R, L, g.first, g.second, 0, 0, st));
   tiled_parity_tail(cw + n, 32, R, L, alpha, st);
@@ -455,16 +461,18 @@ void launch_encode_parity(E8* cw, uint64_t cols, int L, const E8* alpha, hipStre
   launch_pass_geom(cw + n, cw + n, cols, R, R, L, 0, groups[0].second, OP_WHT, nullptr, st);
 }
 
-void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
-                 hipStream_t st, hipEvent_t after_leaf) {
-  const dim3 grid{uint32_t((R + WG - 1) / WG), 1, 1}, block{uint32_t(WG), 1, 1};
+// Leaves of rows [first, first + count) of a codeword with R rows.
+static void launch_leaves(const E8* cw, uint64_t R, uint64_t first, uint64_t count, const LeafPrefix& leaf,
+                          uint32_t* tree, hipStream_t st) {
+  const dim3 grid{uint32_t((count + WG - 1) / WG), 1, 1}, block{uint32_t(WG), 1, 1};
   if (leaf.prefix_len == 75)
-    hipLaunchKernelGGL(k_leaf_hash<75>, grid, block, 0, st, cw, R, leaf, tree);
+    hipLaunchKernelGGL(k_leaf_hash<75>, grid, block, 0, st, cw, R, first, count, leaf, tree);
   else if (leaf.prefix_len == 80)
-    hipLaunchKernelGGL(k_leaf_hash<80>, grid, block, 0, st, cw, R, leaf, tree);
+    hipLaunchKernelGGL(k_leaf_hash<80>, grid, block, 0, st, cw, R, first, count, leaf, tree);
   else
     throw QfError(QF_ERR_ARGUMENT, "unsupported leaf prefix");
-  if (after_leaf) QF_HIP(hipEventRecord(after_leaf, st));
+}
+static void launch_inner_levels(uint64_t R, const InnerPrefix& inner, uint32_t* tree, hipStream_t st) {
   uint32_t* level = tree;
   for (uint64_t cnt = R; cnt > 1; cnt /= 2) {
     uint32_t* next = level + 8 * cnt;
@@ -472,6 +480,42 @@ void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPr
                        next);
     level = next;
   }
+}
+
+void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner, uint32_t* tree,
+                 hipStream_t st, hipEvent_t after_leaf) {
+  launch_leaves(cw, R, 0, R, leaf, tree, st);
+  if (after_leaf) QF_HIP(hipEventRecord(after_leaf, st));
+  launch_inner_levels(R, inner, tree, st);
+}
+
+// This thread's events for a tree split over two streams.
+struct SplitEvents {
+  hipEvent_t msg = nullptr, side = nullptr;
+  ~SplitEvents() {
+    if (msg) (void)hipEventDestroy(msg);
+    if (side) (void)hipEventDestroy(side);
+  }
+};
+static SplitEvents& split_events() {
+  thread_local SplitEvents e;
+  if (!e.msg) QF_HIP(hipEventCreateWithFlags(&e.msg, hipEventDisableTiming));
+  if (!e.side) QF_HIP(hipEventCreateWithFlags(&e.side, hipEventDisableTiming));
+  return e;
+}
+
+// The message rows [0, R/2) hash on `side` once msg_ready (recorded on st when
+// the message halves are final) has fired, beside the parity passes still
+// running on st; the parity rows and then the inner levels follow on st.
+static void launch_tree_split(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPrefix& inner,
+                              uint32_t* tree, hipStream_t st, hipStream_t side, hipEvent_t msg_ready,
+                              hipEvent_t side_done) {
+  QF_HIP(hipStreamWaitEvent(side, msg_ready, 0));
+  launch_leaves(cw, R, 0, R / 2, leaf, tree, side);
+  QF_HIP(hipEventRecord(side_done, side));
+  launch_leaves(cw, R, R / 2, R / 2, leaf, tree, st);
+  QF_HIP(hipStreamWaitEvent(st, side_done, 0));
+  launch_inner_levels(R, inner, tree, st);
 }
 
 // Mobius over the L row bits and the 5 column bits of the message halves (generic passes, in place).
@@ -483,10 +527,16 @@ static void launch_mobius_message(E8* cw, int L, hipStream_t st) {
 }
 
 void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E8* alpha, uint32_t* tree,
-                          hipStream_t st) {
+                          hipStream_t st, hipStream_t side) {
   const int L = g.log_n;
   const uint64_t n = uint64_t(1) << L, R = 2 * n;
-  if (!tiled_commit_rows(false, values, 0, 0, cw, L, alpha, st)) {
+  SplitEvents* ev = side ? &split_events() : nullptr;
+  if (tiled_commit_rows(false, values, 0, 0, cw, L, alpha, st, nullptr, ev ? ev->msg : nullptr)) {
+    if (ev) {
+      launch_tree_split(cw, R, g.leaf, g.inner, tree, st, side, ev->msg, ev->side);
+      return;
+    }
+  } else {
     hipLaunchKernelGGL(k_scatter_message, dim3(uint32_t((32 * n + WG - 1) / WG)), dim3(WG), 0, st, values, n, R, cw);
     launch_mobius_message(cw, L, st);
     launch_encode_parity(cw, 32, L, alpha, st);

@@ -70,8 +70,9 @@ void launch_tree(const E8* cw, uint64_t R, const LeafPrefix& leaf, const InnerPr
 void launch_commit_extension(const E8* values, E8* cw, int L, const E8* alpha, const LeafPrefix& leaf,
                              const InnerPrefix& inner, uint32_t* tree, hipStream_t st);
 // Commit of a flat packed vector of 32 * 2^L elements (the folded block of a batched opening).
+// With `side`, the message rows' leaves hash on that stream beside the parity passes.
 void launch_commit_values(const CommitGeom& g, const E8* values, E8* cw, const E8* alpha, uint32_t* tree,
-                          hipStream_t st);
+                          hipStream_t st, hipStream_t side = nullptr);
 
 // The tile-pass autotuner's choices so far, as a JSON array (qf_tile.hip).
 std::string autotune_report();

@@ -1232,6 +1232,12 @@ SideThread& side_thread() {
   thread_local SideThread s;
   return s;
 }
+// QF_SIDE_LEAVES=0 hashes all of the folded block's leaves on the prover's stream
+// after its parity passes.
+bool side_leaves_on() {
+  const char* e = std::getenv("QF_SIDE_LEAVES");
+  return !(e && e[0] == '0');
+}
 // QF_SIDE_BINDING=0 writes the binding rows on the prover's thread before the
 // native opening.
 bool side_binding_on() {
@@ -1370,7 +1376,8 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
   }
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode folded = alloc_code(R);
-  launch_commit_values(p.geom, values.p, folded.cw.p, p.alpha_encode(p.pp.codes.size() - 1), folded.tree, c.st);
+  launch_commit_values(p.geom, values.p, folded.cw.p, p.alpha_encode(p.pp.codes.size() - 1), folded.tree, c.st,
+                       side_leaves_on() ? sw.s2 : nullptr);
   QF_HIP(hipGetLastError());
   if (side) QF_HIP(hipStreamWaitEvent(c.st, sw.msg.done, 0));
   folded.root = download_root(c, folded.tree, R);
