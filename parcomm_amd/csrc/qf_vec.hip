@@ -571,6 +571,85 @@ __global__ __launch_bounds__(WG) void k_tensor_project(const Val* __restrict__ b
 constexpr int MF_ROWS = 256;            // records per wave and round: 8 blocks of 32, 4 KiB
 constexpr int MF_WAVES = WG / 64;
 typedef int32_t qf_i32x4 __attribute__((ext_vector_type(4)));
+
+#ifndef QF_KB
+// k_col_fold on the matrix cores (F65521, 4 | cols <= 128): out[r] = sum_col
+// f[col] cw[col][r] as an i8 contraction over k = 8 col + j.
+// - Digits of the data: coefficient x = xl + 256 xh with bytes xl, xh, fed as
+//   xl - 128 and xh - 128 (one xor per four bytes); the 128 sum_k B[k][n] this
+//   takes away is added back per output column (`bias`, from the host).
+// - B[k][n]: digit of M_col[c][j] (column j of M_col = f[col] X^j), low for
+//   n = c < 8, high for n = 8 + c, balanced as in the block fold; step s
+//   covers columns 4 s .. 4 s + 3 (lane n + 16 q: column 4 s + q).
+// - A[m][k]: row m of the 16-row group (lane i + 16 q: row i, column 4 s + q,
+//   one 16-byte load gives both digit operands). |D| < 2^23 per digit pair.
+// - value[c] = Ll + 256 (Lh + Hl) + 2^16 Hh with L/H the data digit and l/h
+//   the weight digit, each reduced first; a wave takes 64 rows per iteration
+//   and stores them through LDS, one 16-byte element per lane.
+__global__ __launch_bounds__(WG) void k_col_fold_mfma(const E8* __restrict__ cw, uint64_t R, uint64_t n,
+                                                      uint32_t steps, const long* __restrict__ bops,
+                                                      const int32_t* __restrict__ bias, E8* __restrict__ out) {
+  __shared__ long lbop[32][64];
+  __shared__ __align__(16) uint16_t tbuf[MF_WAVES][64][8];
+  for (uint32_t i = threadIdx.x; i < steps * 64; i += WG) lbop[i / 64][i % 64] = bops[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nn = lane & 15, q = lane >> 4;
+  const int32_t bl = bias[nn & 7], bh = bias[8 + (nn & 7)];  // this lane's two weight digits
+  for (uint64_t base = uint64_t(blockIdx.x) * WG; base < n; base += uint64_t(gridDim.x) * WG) {
+    const uint64_t row0 = base + uint64_t(wave) * 64;
+    qf_i32x4 dl[4], dh[4];  // data low / high digit sums
+#pragma unroll
+    for (int g = 0; g < 4; g++) dl[g] = dh[g] = qf_i32x4{0, 0, 0, 0};
+    for (uint32_t s = 0; s < steps; s++) {
+      uint4 v[4];
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const uint64_t r = row0 + 16 * g + nn;
+        v[g] = r < n ? *reinterpret_cast<const uint4*>(cw + uint64_t(4 * s + q) * R + r) : make_uint4(0, 0, 0, 0);
+      }
+      const long bop = lbop[s][lane];
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        // coefficient j's bytes: low byte of u16 j at byte 2 j (mod 4) of word j / 2
+        const uint32_t lo0 = __builtin_amdgcn_perm(v[g].y, v[g].x, 0x06040200u) ^ 0x80808080u;
+        const uint32_t lo1 = __builtin_amdgcn_perm(v[g].w, v[g].z, 0x06040200u) ^ 0x80808080u;
+        const uint32_t hi0 = __builtin_amdgcn_perm(v[g].y, v[g].x, 0x07050301u) ^ 0x80808080u;
+        const uint32_t hi1 = __builtin_amdgcn_perm(v[g].w, v[g].z, 0x07050301u) ^ 0x80808080u;
+        const long al = long(uint64_t(lo0) | (uint64_t(lo1) << 32)), ah = long(uint64_t(hi0) | (uint64_t(hi1) << 32));
+        dl[g] = __builtin_amdgcn_mfma_i32_16x16x32_i8(al, bop, dl[g], 0, 0, 0);
+        dh[g] = __builtin_amdgcn_mfma_i32_16x16x32_i8(ah, bop, dh[g], 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the previous iteration's reads of tbuf are done
+    const int32_t bown = nn < 8 ? bl : bh;  // bias of this lane's weight digit
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        // this lane's sums with 128 sum_k B added back, reduced mod p (nonnegative through the 16384 p bias)
+        const uint32_t L = fold16(fold16(uint32_t(dl[g][j] + bown + 16384 * int32_t(P))));
+        const uint32_t H = fold16(fold16(uint32_t(dh[g][j] + bown + 16384 * int32_t(P))));
+        const uint32_t Lo = __shfl_xor(L, 8, 64), Ho = __shfl_xor(H, 8, 64);
+        if ((j < 2) == (nn < 8)) {
+          // Ll, Lh, Hl, Hh: data digit L/H times weight digit l/h
+          const uint32_t Ll = nn < 8 ? L : Lo, Lh = nn < 8 ? Lo : L, Hl = nn < 8 ? H : Ho, Hh = nn < 8 ? Ho : H;
+          const uint32_t mid = fold16(Lh + Hl);                          // < 2^17, folded below 2^16 + 2^5
+          const uint32_t x = fold16(fold16(Ll + 256u * mid + 15u * Hh));  // 2^16 = 15 mod p
+          const uint32_t y = x - P;
+          tbuf[wave][16 * g + 4 * q + j][nn & 7] = uint16_t(x < y ? x : y);
+        }
+      }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const uint4 raw = *reinterpret_cast<const uint4*>(&tbuf[wave][lane][0]);
+    if (row0 + lane < n) {
+      E8 e;
+      *reinterpret_cast<uint4*>(&e) = raw;
+      out[row0 + lane] = e;
+    }
+  }
+}
+#endif
 typedef int32_t qf_i32x2 __attribute__((ext_vector_type(2)));
 __device__ inline long tr8_b64(const uint8_t* p) {
   const qf_i32x2 v = __builtin_amdgcn_ds_read_tr8_b64_v2i32(
@@ -2109,6 +2188,24 @@ void DevCtx::sync() {
   t.wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+namespace {
+// Workgroups of WG threads of `kernel` that are resident on the device at once:
+// a persistent launch of more would run its last workgroups after the others.
+template <typename K>
+uint32_t resident_groups(K kernel) {
+  int per = 0;
+  QF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, WG, 0));
+  return uint32_t(std::max(per, 1)) * uint32_t(device_cus());
+}
+#ifndef QF_KB
+// QF_BLOCK_FOLD_MFMA=0 keeps the F65521 block fold and column fold on the vector ALUs.
+bool block_fold_mfma_on() {
+  const char* e = std::getenv("QF_BLOCK_FOLD_MFMA");
+  return !(e && e[0] == '0');
+}
+#endif
+}  // namespace
+
 namespace dv {
 DVec upload(DevCtx& c, const std::vector<E8>& v) {
   DVec d(v.size());
@@ -2222,6 +2319,40 @@ std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, 
   return out;
 }
 void col_fold(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& f, E8* out) {
+#ifndef QF_KB
+  if (cols % 4 == 0 && cols <= 128 && f.size() >= cols && block_fold_mfma_on()) {
+    // B operands per step and lane (nn + 16 q): digits of M_col[c][j], col = 4 s + q; and
+    // bias[nn] = 128 x the sum of column nn's digits over every k
+    const uint32_t steps = uint32_t(cols / 4);
+    std::vector<uint64_t> bops(size_t(steps) * 64, 0);
+    int32_t bias[16] = {};
+    for (uint64_t col = 0; col < cols; col++) {
+      E8 m[8];
+      for (int j = 0; j < 8; j++) m[j] = f[col] * e8_basis(j);
+      for (int nn = 0; nn < 16; nn++) {
+        uint64_t x = 0;
+        for (int j = 0; j < 8; j++) {
+          const int32_t t = int32_t(m[j].c[nn & 7]);
+          const int32_t tv = t <= 32639 ? t : t - int32_t(P);
+          const int32_t hd = (tv + 128) >> 8, ld = tv - 256 * hd;
+          const int32_t d = nn < 8 ? ld : hd;
+          x |= uint64_t(uint8_t(int8_t(d))) << (8 * j);
+          bias[nn] += 128 * d;
+        }
+        bops[(col / 4) * 64 + 16 * (col % 4) + nn] = x;
+      }
+    }
+    E8* t = scratch_tab();
+    put_table(c, t, bops.data(), bops.size() * 8);
+    put_table(c, t + steps * 32, bias, sizeof(bias));
+    static const uint32_t resident = resident_groups(k_col_fold_mfma);
+    const uint64_t g = std::min<uint64_t>(resident, (n + WG - 1) / WG);
+    hipLaunchKernelGGL(k_col_fold_mfma, dim3(uint32_t(std::max<uint64_t>(g, 1))), dim3(WG), 0, c.st, cw, R, n, steps,
+                       reinterpret_cast<const long*>(t), reinterpret_cast<const int32_t*>(t + steps * 32), out);
+    QF_HIP(hipGetLastError());
+    return;
+  }
+#endif
   E8* t = put_table(c, scratch_tab(), f.data(), f.size() * 16);
   hipLaunchKernelGGL(k_col_fold, grid_for(n), dim3(WG), 0, c.st, cw, R, n, cols, t, out);
 }
@@ -2465,16 +2596,6 @@ std::pair<E8, E8> eo_first_cross(DevCtx& c, const EoProjection& pr, const std::v
   return {sums[0], sums[1]};
 }
 
-namespace {
-// Workgroups of WG threads of `kernel` that are resident on the device at once:
-// a persistent launch of more would run its last workgroups after the others.
-template <typename K>
-uint32_t resident_groups(K kernel) {
-  int per = 0;
-  QF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, WG, 0));
-  return uint32_t(std::max(per, 1)) * uint32_t(device_cus());
-}
-}  // namespace
 
 OtfQuery::OtfQuery(DevCtx& c, const std::vector<std::array<E8, 2>>& factors, uint64_t max_chunks, size_t max_fast,
                    size_t max_pieces, size_t max_items)
@@ -2749,14 +2870,6 @@ void tensor_fill(DevCtx& c, E8* out, uint64_t n, const std::vector<FillTask>& ta
   }
   fill_tasks<false>(c, out, tasks, pairs);
 }
-
-#ifndef QF_KB
-// QF_BLOCK_FOLD_MFMA=0 keeps the F65521 block fold on the vector ALUs.
-bool block_fold_mfma_on() {
-  const char* e = std::getenv("QF_BLOCK_FOLD_MFMA");
-  return !(e && e[0] == '0');
-}
-#endif
 
 void block_fold(DevCtx& c, const Val* bva, uint64_t available, uint64_t block_len, uint64_t blocks,
                 const std::vector<E8>& fw, E8* values, uint64_t n) {
