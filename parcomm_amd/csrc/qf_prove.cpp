@@ -132,6 +132,22 @@ struct DCode {
   std::vector<E8> projection(DevCtx& c, uint64_t n, const E8* w) const {
     return dv::col_projection(c, data(), R, n, cols, w);
   }
+  // The same projections with w = tw(point): w[r] = top[r >> 5] low[r & 31], so the
+  // low five coordinates first contract 32 consecutive rows of every column on the
+  // matrix cores (F65521) and the rest project the n / 32 results.
+  std::vector<E8> projection_tw(DevCtx& c, uint64_t n, const std::vector<E8>& point, const E8* w) const {
+    const size_t m = point.size();
+    if (m >= 10 && n == uint64_t(1) << m) {
+      const uint64_t h = n / 32;
+      DVec inner(cols * h);
+      if (dv::col_fold_rows32(c, data(), R, n, cols, tw_host(point.data() + (m - 5), 5), inner.p)) {
+        DVec top(h);
+        tw_device(c, top.p, std::vector<E8>(point.begin(), point.end() - 5));
+        return dv::col_projection(c, inner.p, h, h, cols, top.p);
+      }
+    }
+    return projection(c, n, w);
+  }
   // out[r] = sum_col f[col] * codeword[col][r], r < n
   void fold(DevCtx& c, uint64_t n, const std::vector<E8>& f, E8* out) const {
     dv::col_fold(c, data(), R, n, cols, f, out);
@@ -377,12 +393,10 @@ RoundOut prove_round(DevCtx& c, const ParamsState& p, size_t level, const DCode&
   require(R == code.codeword_len() && point.size() == size_t(5 + ilog2u(n)), QF_ERR_INVALID_VARIABLES, "round dims");
   TraceLaps lap;
   DVec own_w;
-  if (!top_row_w) {
-    std::vector<E8> rp(point.begin() + 5, point.end());
-    own_w = tw_vec(c, rp);
-  }
+  const std::vector<E8> rp(point.begin() + 5, point.end());
+  if (!top_row_w) own_w = tw_vec(c, rp);
   const DVec& row_w = top_row_w ? *top_row_w : own_w;
-  const std::vector<E8> opening = top_opening ? *top_opening : cache.projection(c, n, row_w.p);
+  const std::vector<E8> opening = top_opening ? *top_opening : cache.projection_tw(c, n, rp, row_w.p);
   if (!top) require(inner_host(opening, tw_host(point.data(), 5)) == value, QF_ERR_INVALID_COMMITMENT, "round opening");
   tr.digest("round-root", cache.root);
   for (auto& o : opening) tr.respond(o);
@@ -429,7 +443,7 @@ E8 open_with_reduction(DevCtx& c, const ParamsState& p, const DCode& top_code, c
   uint64_t n = p.pp.codes[top].message_len;
   std::vector<E8> rp(point.begin() + 5, point.end());
   DVec row_w = tw_vec(c, rp);
-  auto opening = top_code.projection(c, n, row_w.p);
+  auto opening = top_code.projection_tw(c, n, rp, row_w.p);
   E8 claimed = inner_host(opening, tw_host(point.data(), 5));
   Pcs tr;
   tr.digest("parameters", p.pp.parameter_digest);

@@ -586,9 +586,15 @@ typedef int32_t qf_i32x4 __attribute__((ext_vector_type(4)));
 // - value[c] = Ll + 256 (Lh + Hl) + 2^16 Hh with L/H the data digit and l/h
 //   the weight digit, each reduced first; a wave takes 64 rows per iteration
 //   and stores them through LDS, one 16-byte element per lane.
-__global__ __launch_bounds__(WG) void k_col_fold_mfma(const E8* __restrict__ cw, uint64_t R, uint64_t n,
-                                                      uint32_t steps, const long* __restrict__ bops,
-                                                      const int32_t* __restrict__ bias, E8* __restrict__ out) {
+// Element (col, r) is cw[col R + r rstride]; grid.y runs independent folds
+// `in_y` and `out_y` elements apart (the column projection's inner contraction
+// over 32 consecutive rows uses R = 1, rstride = 32, one fold per column).
+__global__ __launch_bounds__(WG) void k_col_fold_mfma(const E8* __restrict__ cw, uint64_t R, uint64_t rstride,
+                                                      uint64_t n, uint32_t steps, const long* __restrict__ bops,
+                                                      const int32_t* __restrict__ bias, E8* __restrict__ out,
+                                                      uint64_t in_y, uint64_t out_y) {
+  cw += uint64_t(blockIdx.y) * in_y;
+  out += uint64_t(blockIdx.y) * out_y;
   __shared__ long lbop[32][64];
   __shared__ __align__(16) uint16_t tbuf[MF_WAVES][64][8];
   for (uint32_t i = threadIdx.x; i < steps * 64; i += WG) lbop[i / 64][i % 64] = bops[i];
@@ -606,7 +612,8 @@ __global__ __launch_bounds__(WG) void k_col_fold_mfma(const E8* __restrict__ cw,
 #pragma unroll
       for (int g = 0; g < 4; g++) {
         const uint64_t r = row0 + 16 * g + nn;
-        v[g] = r < n ? *reinterpret_cast<const uint4*>(cw + uint64_t(4 * s + q) * R + r) : make_uint4(0, 0, 0, 0);
+        v[g] = r < n ? *reinterpret_cast<const uint4*>(cw + uint64_t(4 * s + q) * R + r * rstride)
+                     : make_uint4(0, 0, 0, 0);
       }
       const long bop = lbop[s][lane];
 #pragma unroll
@@ -2318,38 +2325,64 @@ std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, 
   }
   return out;
 }
+#ifndef QF_KB
+namespace {
+// B operands and bias of k_col_fold_mfma for the weights f[0 .. k) (4 | k <= 128), into
+// the scratch table: per step and lane (nn + 16 q), digits of M_f[4 s + q][c][j]; bias[nn]
+// = 128 x the sum of column nn's digits over every k.
+std::pair<const long*, const int32_t*> col_fold_tables(DevCtx& c, const E8* f, uint64_t k) {
+  const uint32_t steps = uint32_t(k / 4);
+  std::vector<uint64_t> bops(size_t(steps) * 64, 0);
+  int32_t bias[16] = {};
+  for (uint64_t col = 0; col < k; col++) {
+    E8 m[8];
+    for (int j = 0; j < 8; j++) m[j] = f[col] * e8_basis(j);
+    for (int nn = 0; nn < 16; nn++) {
+      uint64_t x = 0;
+      for (int j = 0; j < 8; j++) {
+        const int32_t t = int32_t(m[j].c[nn & 7]);
+        const int32_t tv = t <= 32639 ? t : t - int32_t(P);
+        const int32_t hd = (tv + 128) >> 8, ld = tv - 256 * hd;
+        const int32_t d = nn < 8 ? ld : hd;
+        x |= uint64_t(uint8_t(int8_t(d))) << (8 * j);
+        bias[nn] += 128 * d;
+      }
+      bops[(col / 4) * 64 + 16 * (col % 4) + nn] = x;
+    }
+  }
+  E8* t = scratch_tab();
+  put_table(c, t, bops.data(), bops.size() * 8);
+  put_table(c, t + steps * 32, bias, sizeof(bias));
+  return {reinterpret_cast<const long*>(t), reinterpret_cast<const int32_t*>(t + steps * 32)};
+}
+void launch_col_fold_mfma(DevCtx& c, const E8* cw, uint64_t R, uint64_t rstride, uint64_t n, uint64_t k,
+                          const std::pair<const long*, const int32_t*>& tab, E8* out, uint32_t ny, uint64_t in_y,
+                          uint64_t out_y) {
+  static const uint32_t resident = resident_groups(k_col_fold_mfma);
+  const uint64_t g = std::max<uint64_t>(1, std::min<uint64_t>(std::max<uint32_t>(resident / ny, 1), (n + WG - 1) / WG));
+  hipLaunchKernelGGL(k_col_fold_mfma, dim3(uint32_t(g), ny), dim3(WG), 0, c.st, cw, R, rstride, n, uint32_t(k / 4),
+                     tab.first, tab.second, out, in_y, out_y);
+  QF_HIP(hipGetLastError());
+}
+}  // namespace
+#endif
+
+bool col_fold_rows32(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& low,
+                     E8* inner) {
+#ifndef QF_KB
+  if (n % 32 == 0 && n >= 32 && low.size() == 32 && cols <= 64 && block_fold_mfma_on()) {
+    const uint64_t h = n / 32;
+    launch_col_fold_mfma(c, cw, 1, 32, h, 32, col_fold_tables(c, low.data(), 32), inner, uint32_t(cols), R, h);
+    return true;
+  }
+#endif
+  return false;
+}
+
 void col_fold(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& f, E8* out) {
 #ifndef QF_KB
   if (cols % 4 == 0 && cols <= 128 && f.size() >= cols && block_fold_mfma_on()) {
-    // B operands per step and lane (nn + 16 q): digits of M_col[c][j], col = 4 s + q; and
-    // bias[nn] = 128 x the sum of column nn's digits over every k
-    const uint32_t steps = uint32_t(cols / 4);
-    std::vector<uint64_t> bops(size_t(steps) * 64, 0);
-    int32_t bias[16] = {};
-    for (uint64_t col = 0; col < cols; col++) {
-      E8 m[8];
-      for (int j = 0; j < 8; j++) m[j] = f[col] * e8_basis(j);
-      for (int nn = 0; nn < 16; nn++) {
-        uint64_t x = 0;
-        for (int j = 0; j < 8; j++) {
-          const int32_t t = int32_t(m[j].c[nn & 7]);
-          const int32_t tv = t <= 32639 ? t : t - int32_t(P);
-          const int32_t hd = (tv + 128) >> 8, ld = tv - 256 * hd;
-          const int32_t d = nn < 8 ? ld : hd;
-          x |= uint64_t(uint8_t(int8_t(d))) << (8 * j);
-          bias[nn] += 128 * d;
-        }
-        bops[(col / 4) * 64 + 16 * (col % 4) + nn] = x;
-      }
-    }
-    E8* t = scratch_tab();
-    put_table(c, t, bops.data(), bops.size() * 8);
-    put_table(c, t + steps * 32, bias, sizeof(bias));
-    static const uint32_t resident = resident_groups(k_col_fold_mfma);
-    const uint64_t g = std::min<uint64_t>(resident, (n + WG - 1) / WG);
-    hipLaunchKernelGGL(k_col_fold_mfma, dim3(uint32_t(std::max<uint64_t>(g, 1))), dim3(WG), 0, c.st, cw, R, n, steps,
-                       reinterpret_cast<const long*>(t), reinterpret_cast<const int32_t*>(t + steps * 32), out);
-    QF_HIP(hipGetLastError());
+    launch_col_fold_mfma(c, cw, R, 1, n, cols, col_fold_tables(c, f.data(), cols), out, 1, 0, 0);
     return;
   }
 #endif

@@ -94,6 +94,10 @@ void scale(DevCtx& c, E8* y, uint64_t n, const E8& s);              // y *= s
 std::vector<E8> col_projection(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const E8* w);
 // out[r] = sum_col f[col] * cw[col*R + r], r < n
 void col_fold(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& f, E8* out);
+// inner[col n/32 + h] = sum_{l < 32} low[l] cw[col R + 32 h + l] (F65521 matrix cores; false
+// where it does not apply, leaving inner untouched).
+bool col_fold_rows32(DevCtx& c, const E8* cw, uint64_t R, uint64_t n, uint64_t cols, const std::vector<E8>& low,
+                     E8* inner);
 // Gather full rows (cols entries each) of a column-major matrix: out[k*cols + col].
 std::vector<E8> gather_rows(DevCtx& c, const E8* cw, uint64_t R, uint64_t cols, const std::vector<uint64_t>& rows);
 // out[k] = sum_{col < ca} w[col] a[col*Ra + rows[k]] + sum_{col < cb} w[ca + col] b[col*Rb + rows[k]]
