@@ -1176,7 +1176,6 @@ Lanes conjugates(const Lanes& lanes) {
 
 struct Timing { ProveTimings* t; };
 
-// Codeword of block b: the resident cache, or a fresh commit in recompute mode.
 // A persistent helper thread per prover thread, with its own stream and pinned
 // buffer (the DevCtx it makes for each task), running one task at a time.
 class SideThread {
@@ -1259,6 +1258,7 @@ bool side_binding_on() {
   return !(e && e[0] == '0');
 }
 
+// Codeword of block b: the resident cache, or a fresh commit in recompute mode.
 DCode block_code(DevCtx& c, const ParamsState& p, const CommittedState& cs, uint64_t b, DCode& workspace) {
   uint64_t R = 2 * (uint64_t(1) << p.geom.log_n);
   DCode d;
