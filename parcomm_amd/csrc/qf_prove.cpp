@@ -1285,13 +1285,16 @@ DCode block_code(DevCtx& c, const ParamsState& p, const CommittedState& cs, uint
 void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const std::vector<E8>& point,
                 const Bytes& context, E8& claimed, ProveTimings& tm, Bytes& w) {
   auto t0 = Clock::now();
+  TraceLaps lap;
   uint64_t blocks = cs.roots.size();
   size_t split = ilog2u(blocks), n_outer = point.size() - LOG_LANES - split;
   std::vector<E8> outer(point.begin() + split, point.begin() + split + n_outer);
   auto low = eq_host(point.data() + point.size() - LOG_LANES, LOG_LANES);
   auto high = eq_host(point.data(), split);
   uint64_t plen = uint64_t(1) << n_outer;
+  lap.mark("projection.setup");
   auto lanes = block_lanes(c, p, cs.d_bva, p.layout.active, blocks, outer);
+  lap.mark("projection.lanes");
   std::vector<Lanes> ebb(blocks);
   for (uint64_t b = 0; b < blocks; b++) ebb[b] = conjugates(lanes[b]);
   Lanes ev;
@@ -1350,9 +1353,11 @@ void open_batch(DevCtx& c, const ParamsState& p, const CommittedState& cs, const
     for (int img = 0; img < LANES; img++) s += coefs[img] * e8_frob(e8_basis(lane), (LANES - img) % LANES);
     images[lane] = s;
   }
+  lap.mark("projection.transcript");
   DVec ehi, elo;
   eq_tables_device(c, outer, ehi, elo);
   DVec weights(plen), values(plen);
+  lap.mark("projection.tables");
   // The weight images and then the sumcheck's first message (it needs the
   // folded values, not the folded root) run on the thread's second stream,
   // beside the block fold and the folded block's commit on c.st: the weights

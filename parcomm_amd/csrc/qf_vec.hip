@@ -2756,6 +2756,30 @@ std::pair<E8, E8> OtfQuery::fold(DevCtx& c, const E8* w, E8* ow, uint64_t h, con
 }
 
 // One launch over every task (the batched form).
+// out[t L + l] = sum of parts[ch L + l] over task t's chunks [r[2 t], r[2 t + 1]); one
+// workgroup per task, WG / L chunk slots per lane.
+__global__ __launch_bounds__(WG) void k_sum_chunks(const E8* __restrict__ parts, int L,
+                                                   const uint32_t* __restrict__ ranges, E8* __restrict__ out) {
+  __shared__ uint32_t red[WG][LANES];
+  const uint32_t t = blockIdx.x, l = threadIdx.x % uint32_t(L), slot = threadIdx.x / uint32_t(L),
+                 slots = WG / uint32_t(L);
+  V8 s = v8_zero();
+  for (uint32_t ch = ranges[2 * t] + slot; ch < ranges[2 * t + 1]; ch += slots)
+    s = v8_add(s, v8_load(parts[uint64_t(ch) * L + l]));
+  for (int k = 0; k < LANES; k++) red[threadIdx.x][k] = s.c[k];
+  __syncthreads();
+  for (uint32_t h = slots / 2; h > 0; h /= 2) {  // slots is a power of two
+    if (slot < h)
+      for (int k = 0; k < LANES; k++) red[threadIdx.x][k] = madd(red[threadIdx.x][k], red[threadIdx.x + h * L][k]);
+    __syncthreads();
+  }
+  if (slot == 0) {
+    V8 x;
+    for (int k = 0; k < LANES; k++) x.c[k] = red[threadIdx.x][k];
+    out[uint64_t(t) * L + l] = v8_store(x);
+  }
+}
+
 static std::vector<E8> tensor_project_launch(DevCtx& c, const Val* bva, uint64_t available, int L,
                                              const std::vector<ProjTask>& tasks,
                                              const std::vector<std::array<E8, 2>>& pairs) {
@@ -2763,6 +2787,29 @@ static std::vector<E8> tensor_project_launch(DevCtx& c, const Val* bva, uint64_t
   DVec out = project_chunks(c, bva, available, L, tasks, pairs, owner);
   std::vector<E8> result(tasks.size() * L, E8{});
   if (owner.empty()) return result;
+  if (owner.size() > 4 * tasks.size() && WG % L == 0) {
+    // many chunks per task: sum them on the device. A task's chunks are consecutive
+    // (all of them on the matrix cores or all on the vector ALUs, appended in task order).
+    std::vector<uint32_t> ranges(2 * tasks.size(), 0);
+    std::vector<uint8_t> seen(tasks.size(), 0);
+    for (size_t ch = 0; ch < owner.size(); ch++) {
+      const uint32_t t = owner[ch];
+      if (!seen[t]) {
+        ranges[2 * t] = uint32_t(ch);
+        seen[t] = 1;
+      }
+      require(ranges[2 * t + 1] == 0 || ranges[2 * t + 1] == ch, QF_ERR_ARGUMENT, "projection chunks out of order");
+      ranges[2 * t + 1] = uint32_t(ch + 1);
+    }
+    auto* d_r = static_cast<uint32_t*>(dmalloc(ranges.size() * 4, c.st));
+    h2d(c, d_r, ranges.data(), ranges.size() * 4);
+    DVec sums(tasks.size() * L);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(uint32_t(tasks.size())), dim3(WG), 0, c.st, out.p, L, d_r, sums.p);
+    QF_HIP(hipGetLastError());
+    result = dv::download(c, sums);
+    dfree(d_r, c.st);
+    return result;
+  }
   auto parts = dv::download(c, out);
   for (size_t ch = 0; ch < owner.size(); ch++)
     for (int l = 0; l < L; l++) result[owner[ch] * L + l] = e8_add(result[owner[ch] * L + l], parts[ch * L + l]);
