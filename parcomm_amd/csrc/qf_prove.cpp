@@ -136,8 +136,9 @@ struct DCode {
   // low five coordinates first contract 32 consecutive rows of every column on the
   // matrix cores (F65521) and the rest project the n / 32 results.
   std::vector<E8> projection_tw(DevCtx& c, uint64_t n, const std::vector<E8>& point, const E8* w) const {
+#ifndef QF_KB
     const size_t m = point.size();
-    if (m >= 10 && n == uint64_t(1) << m) {
+    if (m >= 10 && n == uint64_t(1) << m && cols <= 64) {
       const uint64_t h = n / 32;
       DVec inner(cols * h);
       if (dv::col_fold_rows32(c, data(), R, n, cols, tw_host(point.data() + (m - 5), 5), inner.p)) {
@@ -146,6 +147,9 @@ struct DCode {
         return dv::col_projection(c, inner.p, h, h, cols, top.p);
       }
     }
+#else
+    (void)point;
+#endif
     return projection(c, n, w);
   }
   // out[r] = sum_col f[col] * codeword[col][r], r < n
@@ -1192,6 +1196,7 @@ class SideThread {
   }
   void start(int device, Task t) {
     std::lock_guard<std::mutex> g(mu_);
+    require(!busy_, QF_ERR_ARGUMENT, "helper thread already has a task");
     device_ = device;
     task_ = std::move(t);
     err_ = nullptr;
