@@ -2,7 +2,10 @@
 PCS blocks into one by the powers of the batch challenge): whole proofs against
 the oracle's bytes, and against the vector-ALU form (QF_BLOCK_FOLD_MFMA=0), for
 1 to 128 blocks (1 to 32 contraction steps of four blocks, the last one
-partial), ragged ends and entries all at -128 (the largest digit sums)."""
+partial), ragged ends and entries all at -128 (the largest digit sums). The same
+switch covers the native opening's column folds (every case) and its top-level
+projection through 32-row groups (message halves of at least 2^10 rows: the
+chunk_log-20 case)."""
 import numpy as np
 import pytest
 
@@ -18,6 +21,7 @@ pytestmark = pytest.mark.gpu
     ([(40, 50)], 12, 1, "full"),                # one block
     ([(512, 1024)], 12, 3, "min"),              # 128 blocks of -128
     ([(1000, 999)], 13, 3, "full"),             # 128 blocks at packed_log 3
+    ([(1500, 1500)], 20, 4, "full"),            # 3 blocks; top-level projection in 32-row groups (2^12 rows)
 ])
 def test_block_fold_matches_the_oracle(oracle, shapes, chunk_log, packed_log, fill, monkeypatch):
     pa, p = setup("f65521", shapes, chunk_log, packed_log)
